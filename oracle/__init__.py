@@ -1,0 +1,314 @@
+"""ctypes front end of the oracle (test infrastructure only).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this module.
+It wraps two shared libraries built by oracle/Makefile:
+
+* ``liboracle.so`` -- the CPU restatement of CRAVA's path (see oracle.h for what each
+  function restates and what pins it);
+* ``_ref/libcrava_ref.so`` -- the reference's own FFTW 2.1.2 / lib_matr.c / random.cpp,
+  compiled from the CRAVA tree, driven by ref_harness.cpp (optional: absent where the
+  reference tree was not available at build time).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libcrava_ref.so")
+
+DATA, PARAMETER, COVARIANCE = 1, 2, 3
+
+_c_float_p = ctypes.POINTER(ctypes.c_float)
+_c_double_p = ctypes.POINTER(ctypes.c_double)
+_c_int_p = ctypes.POINTER(ctypes.c_int)
+_c_u8_p = ctypes.POINTER(ctypes.c_uint8)
+_c_u32_p = ctypes.POINTER(ctypes.c_uint32)
+
+
+def _ptr(a, ctype):
+    if a is None:
+        return None
+    return a.ctypes.data_as(ctypes.POINTER(ctype))
+
+
+class PostSpec(ctypes.Structure):
+    _fields_ = [
+        ("nxp", ctypes.c_int), ("nyp", ctypes.c_int), ("nzp", ctypes.c_int), ("ntheta", ctypes.c_int),
+        ("K", _c_float_p), ("e1", _c_float_p), ("e2", _c_float_p), ("e3", _c_float_p),
+        ("invert", _c_int_p), ("sigma0e", _c_double_p), ("wnc", ctypes.c_float),
+        ("prior_var0", _c_double_p), ("prior_mode", ctypes.c_int),
+        ("rxy", _c_float_p), ("rt", _c_float_p), ("lam_grid", _c_float_p), ("lam2d", _c_float_p),
+    ]
+
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            raise RuntimeError("oracle library not built: run `make -C oracle` (or __graft_entry__.build())")
+        L = ctypes.CDLL(ORACLE_SO)
+        L.orc_factorable.restype = ctypes.c_int
+        L.orc_factorable.argtypes = [ctypes.c_int]
+        L.orc_padding_size.restype = ctypes.c_int
+        L.orc_padding_size.argtypes = [ctypes.c_int, ctypes.c_double]
+        L.orc_cdft_naive.argtypes = [ctypes.c_int, ctypes.c_int, _c_double_p, _c_double_p]
+        L.orc_cfft.argtypes = [ctypes.c_int, ctypes.c_int, _c_double_p, ctypes.c_long, ctypes.c_long, ctypes.c_long]
+        for f in ("orc_rfft3d_r2c", "orc_rfft3d_c2r"):
+            getattr(L, f).argtypes = [ctypes.c_int] * 3 + [_c_double_p]
+        for f in ("orc_grid_fft", "orc_grid_ifft"):
+            getattr(L, f).argtypes = [ctypes.c_int] * 4 + [_c_double_p]
+        for f in ("orc_grid_fft_f32", "orc_grid_ifft_f32"):
+            getattr(L, f).argtypes = [ctypes.c_int] * 4 + [_c_float_p]
+        L.orc_posterior_f32.restype = ctypes.c_long
+        L.orc_posterior_f32.argtypes = [ctypes.POINTER(PostSpec), _c_float_p, _c_float_p, _c_float_p, _c_u8_p]
+        L.orc_posterior_f64.restype = ctypes.c_long
+        L.orc_posterior_f64.argtypes = [ctypes.POINTER(PostSpec), _c_double_p, _c_double_p, _c_double_p, _c_u8_p]
+        L.orc_unif01.restype = ctypes.c_double
+        L.orc_unif01.argtypes = [_c_u32_p]
+        L.orc_rnorm01.restype = ctypes.c_double
+        L.orc_rnorm01.argtypes = [_c_u32_p]
+        L.orc_rnorm01_n.argtypes = [_c_u32_p, _c_double_p, ctypes.c_long]
+        L.orc_fill_complex_noise.argtypes = [ctypes.c_int] * 3 + [_c_u32_p, _c_float_p]
+        L.orc_cb_normal2.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64, _c_float_p, _c_float_p]
+        L.orc_fill_complex_noise_cb.argtypes = [ctypes.c_int] * 3 + [ctypes.c_uint64, ctypes.c_uint32, _c_float_p]
+        L.orc_colour_f32.restype = ctypes.c_long
+        L.orc_colour_f32.argtypes = [ctypes.c_long, _c_float_p, _c_float_p]
+        L.orc_colour_f64.restype = ctypes.c_long
+        L.orc_colour_f64.argtypes = [ctypes.c_long, _c_double_p, _c_double_p]
+        L.orc_ricker.restype = ctypes.c_double
+        L.orc_ricker.argtypes = [ctypes.c_double, ctypes.c_double]
+        L.orc_aki_richards.argtypes = [ctypes.c_double, ctypes.c_double, _c_double_p]
+        L.orc_spherical_corr.restype = ctypes.c_float
+        L.orc_spherical_corr.argtypes = [ctypes.c_float] * 5
+        L.orc_corr_xy.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.c_float] * 5 + [_c_float_p]
+        L.orc_circ_corr_t.restype = ctypes.c_int
+        L.orc_circ_corr_t.argtypes = [_c_double_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_float_p]
+        L.orc_kz_tables.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, _c_float_p,
+                                    _c_float_p, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                    _c_float_p, _c_float_p, _c_float_p, _c_float_p, _c_int_p]
+        _lib = L
+    return _lib
+
+
+def ref_available():
+    return os.path.exists(REF_SO)
+
+
+def ref():
+    """The reference's own compiled code (FFTW 2.1.2, lib_matr, RandomGen)."""
+    global _ref
+    if _ref is None:
+        if not ref_available():
+            raise RuntimeError("oracle/_ref not built (needs the reference tree at build time)")
+        R = ctypes.CDLL(REF_SO)
+        for f in ("ref_grid_fft", "ref_grid_ifft"):
+            getattr(R, f).argtypes = [ctypes.c_int] * 4 + [_c_float_p]
+        R.ref_fft1d_r2c.argtypes = [ctypes.c_int, _c_float_p]
+        R.ref_posterior.restype = ctypes.c_long
+        R.ref_posterior.argtypes = ([ctypes.c_int] * 4 + [_c_float_p] * 4 + [_c_int_p, _c_double_p, ctypes.c_float,
+                                    _c_double_p, ctypes.c_int] + [_c_float_p] * 4 + [_c_float_p] * 3 + [_c_u8_p])
+        R.ref_colour.restype = ctypes.c_long
+        R.ref_colour.argtypes = [ctypes.c_long, _c_float_p, _c_float_p]
+        R.ref_rng_seed.argtypes = [ctypes.c_uint]
+        R.ref_unif01.restype = ctypes.c_double
+        R.ref_rnorm01.restype = ctypes.c_double
+        R.ref_rnorm01_n.argtypes = [_c_double_p, ctypes.c_long]
+        R.ref_fill_complex_noise.argtypes = [ctypes.c_int] * 3 + [_c_float_p]
+        _ref = R
+    return _ref
+
+
+# ----------------------------------------------------------------------------- helpers
+def rsize(nxp, nyp, nzp):
+    return 2 * (nxp // 2 + 1) * nyp * nzp
+
+
+def csize(nxp, nyp, nzp):
+    return (nxp // 2 + 1) * nyp * nzp
+
+
+def to_padded(real, nxp):
+    """(nzp, nyp, nxp) real array -> FFTW in-place layout (nzp, nyp, rnxp)."""
+    nzp, nyp, _ = real.shape
+    out = np.zeros((nzp, nyp, 2 * (nxp // 2 + 1)), dtype=real.dtype)
+    out[:, :, :nxp] = real
+    return out
+
+
+def grid_fft(real, cubetype, dtype=np.float64):
+    """FFTGrid::fftInPlace on a (nzp, nyp, nxp) real array; returns (nzp, nyp, cnxp) complex."""
+    nzp, nyp, nxp = real.shape
+    buf = np.ascontiguousarray(to_padded(np.asarray(real, dtype=dtype), nxp))
+    if dtype == np.float64:
+        lib().orc_grid_fft(nxp, nyp, nzp, cubetype, _ptr(buf, ctypes.c_double))
+        return buf.view(np.complex128)
+    lib().orc_grid_fft_f32(nxp, nyp, nzp, cubetype, _ptr(buf, ctypes.c_float))
+    return buf.view(np.complex64)
+
+
+def grid_ifft(spec, nxp, cubetype, dtype=np.float64):
+    """FFTGrid::invFFTInPlace on a (nzp, nyp, cnxp) complex array; returns (nzp, nyp, nxp) real."""
+    nzp, nyp, cnxp = spec.shape
+    cdt = np.complex128 if dtype == np.float64 else np.complex64
+    buf = np.ascontiguousarray(spec, dtype=cdt).view(dtype).copy()
+    if dtype == np.float64:
+        lib().orc_grid_ifft(nxp, nyp, nzp, cubetype, _ptr(buf, ctypes.c_double))
+    else:
+        lib().orc_grid_ifft_f32(nxp, nyp, nzp, cubetype, _ptr(buf, ctypes.c_float))
+    return buf.reshape(nzp, nyp, 2 * cnxp)[:, :, :nxp].copy()
+
+
+def ref_grid_fft(real, cubetype):
+    nzp, nyp, nxp = real.shape
+    buf = np.ascontiguousarray(to_padded(np.asarray(real, dtype=np.float32), nxp))
+    ref().ref_grid_fft(nxp, nyp, nzp, cubetype, _ptr(buf, ctypes.c_float))
+    return buf.view(np.complex64)
+
+
+def ref_grid_ifft(spec, nxp, cubetype):
+    nzp, nyp, cnxp = spec.shape
+    buf = np.ascontiguousarray(spec, dtype=np.complex64).view(np.float32).copy()
+    ref().ref_grid_ifft(nxp, nyp, nzp, cubetype, _ptr(buf, ctypes.c_float))
+    return buf.reshape(nzp, nyp, 2 * cnxp)[:, :, :nxp].copy()
+
+
+def make_spec(p, keep):
+    """Build a PostSpec from a problem dict (see tests/problem.py); `keep` holds references."""
+    s = PostSpec()
+    s.nxp, s.nyp, s.nzp, s.ntheta = p["nxp"], p["nyp"], p["nzp"], p["ntheta"]
+    arrs = {
+        "K": np.ascontiguousarray(p["K"], dtype=np.complex64).view(np.float32),
+        "e1": np.ascontiguousarray(p["e1"], dtype=np.complex64).view(np.float32),
+        "e2": np.ascontiguousarray(p["e2"], dtype=np.complex64).view(np.float32),
+        "e3": np.ascontiguousarray(p["e3"], dtype=np.complex64).view(np.float32),
+        "invert": np.ascontiguousarray(p["invert"], dtype=np.int32),
+        "sigma0e": np.ascontiguousarray(p["sigma0e"], dtype=np.float64),
+        "prior_var0": np.ascontiguousarray(p["prior_var0"], dtype=np.float64),
+    }
+    keep.update(arrs)
+    s.K = _ptr(arrs["K"], ctypes.c_float)
+    s.e1 = _ptr(arrs["e1"], ctypes.c_float)
+    s.e2 = _ptr(arrs["e2"], ctypes.c_float)
+    s.e3 = _ptr(arrs["e3"], ctypes.c_float)
+    s.invert = _ptr(arrs["invert"], ctypes.c_int)
+    s.sigma0e = _ptr(arrs["sigma0e"], ctypes.c_double)
+    s.wnc = p["wnc"]
+    s.prior_var0 = _ptr(arrs["prior_var0"], ctypes.c_double)
+    s.prior_mode = p.get("prior_mode", 1)
+    for name in ("rxy", "rt", "lam2d"):
+        if p.get(name) is not None:
+            keep[name] = np.ascontiguousarray(p[name], dtype=np.float32)
+            setattr(s, name, _ptr(keep[name], ctypes.c_float))
+    if p.get("lam_grid") is not None:
+        keep["lam_grid"] = np.ascontiguousarray(p["lam_grid"], dtype=np.complex64).view(np.float32)
+        s.lam_grid = _ptr(keep["lam_grid"], ctypes.c_float)
+    return s
+
+
+def posterior(p, mean, data, cov=None, precision="f32"):
+    """Per-wavenumber update.  mean (3, nzp, nyp, cnxp), data (nt, ...), cov (6, ...) complex.
+    Returns (mean_post, residual, cov_post, flags, nfail)."""
+    keep = {}
+    s = make_spec(p, keep)
+    dt = np.complex64 if precision == "f32" else np.complex128
+    rt = np.float32 if precision == "f32" else np.float64
+    m = np.ascontiguousarray(mean, dtype=dt).copy()
+    d = np.ascontiguousarray(data, dtype=dt).copy()
+    shape = m.shape[1:]
+    c = np.zeros((6,) + shape, dtype=dt) if cov is None else np.ascontiguousarray(cov, dtype=dt).copy()
+    flags = np.zeros(shape, dtype=np.uint8)
+    if precision == "f32":
+        nf = lib().orc_posterior_f32(ctypes.byref(s), _ptr(m.view(rt), ctypes.c_float),
+                                     _ptr(d.view(rt), ctypes.c_float), _ptr(c.view(rt), ctypes.c_float),
+                                     _ptr(flags, ctypes.c_uint8))
+    else:
+        nf = lib().orc_posterior_f64(ctypes.byref(s), _ptr(m.view(rt), ctypes.c_double),
+                                     _ptr(d.view(rt), ctypes.c_double), _ptr(c.view(rt), ctypes.c_double),
+                                     _ptr(flags, ctypes.c_uint8))
+    return m, d, c, flags, nf
+
+
+def ref_posterior(p, mean, data, cov=None):
+    keep = {}
+    s = make_spec(p, keep)
+    m = np.ascontiguousarray(mean, dtype=np.complex64).copy()
+    d = np.ascontiguousarray(data, dtype=np.complex64).copy()
+    shape = m.shape[1:]
+    c = np.zeros((6,) + shape, dtype=np.complex64) if cov is None else np.ascontiguousarray(cov, dtype=np.complex64).copy()
+    flags = np.zeros(shape, dtype=np.uint8)
+    f = ctypes.c_float
+    nf = ref().ref_posterior(s.nxp, s.nyp, s.nzp, s.ntheta, s.K, s.e1, s.e2, s.e3, s.invert, s.sigma0e, s.wnc,
+                             s.prior_var0, s.prior_mode, s.rxy, s.rt, s.lam_grid, s.lam2d,
+                             _ptr(m.view(np.float32), f), _ptr(d.view(np.float32), f),
+                             _ptr(c.view(np.float32), f), _ptr(flags, ctypes.c_uint8))
+    return m, d, c, flags, nf
+
+
+def colour(cov, seed, precision="f32"):
+    dt = np.complex64 if precision == "f32" else np.complex128
+    c = np.ascontiguousarray(cov, dtype=dt)
+    z = np.ascontiguousarray(seed, dtype=dt).copy()
+    n = c[0].size
+    if precision == "f32":
+        nf = lib().orc_colour_f32(n, _ptr(c.view(np.float32), ctypes.c_float), _ptr(z.view(np.float32), ctypes.c_float))
+    else:
+        nf = lib().orc_colour_f64(n, _ptr(c.view(np.float64), ctypes.c_double), _ptr(z.view(np.float64), ctypes.c_double))
+    return z, nf
+
+
+def ref_colour(cov, seed):
+    c = np.ascontiguousarray(cov, dtype=np.complex64)
+    z = np.ascontiguousarray(seed, dtype=np.complex64).copy()
+    nf = ref().ref_colour(c[0].size, _ptr(c.view(np.float32), ctypes.c_float), _ptr(z.view(np.float32), ctypes.c_float))
+    return z, nf
+
+
+def fill_complex_noise(nxp, nyp, nzp, state):
+    """Reference-stream noise; `state` is a 1-element uint32 array advanced in place."""
+    out = np.zeros((nzp, nyp, nxp // 2 + 1), dtype=np.complex64)
+    lib().orc_fill_complex_noise(nxp, nyp, nzp, _ptr(state, ctypes.c_uint32), _ptr(out.view(np.float32), ctypes.c_float))
+    return out
+
+
+def fill_complex_noise_cb(nxp, nyp, nzp, seed, stream):
+    out = np.zeros((nzp, nyp, nxp // 2 + 1), dtype=np.complex64)
+    lib().orc_fill_complex_noise_cb(nxp, nyp, nzp, seed, stream, _ptr(out.view(np.float32), ctypes.c_float))
+    return out
+
+
+def kz_tables(nz, nzp, dt_ms, wavelets, A, low_cut=5.0, high_cut=55.0, min_rel_thick=1.0):
+    nt = wavelets.shape[0]
+    w = np.ascontiguousarray(wavelets, dtype=np.float32)
+    a = np.ascontiguousarray(A, dtype=np.float32)
+    K = np.zeros((nzp, nt, 3), dtype=np.complex64)
+    e1 = np.zeros((nzp, nt), dtype=np.complex64)
+    e2 = np.zeros((nzp, nt), dtype=np.complex64)
+    e3 = np.zeros((nzp, nt), dtype=np.complex64)
+    inv = np.zeros(nzp, dtype=np.int32)
+    f = ctypes.c_float
+    lib().orc_kz_tables(nz, nzp, dt_ms, nt, _ptr(w, f), _ptr(a, f), low_cut, high_cut, min_rel_thick,
+                        _ptr(K.view(np.float32), f), _ptr(e1.view(np.float32), f), _ptr(e2.view(np.float32), f),
+                        _ptr(e3.view(np.float32), f), _ptr(inv, ctypes.c_int))
+    return K, e1, e2, e3, inv
+
+
+def corr_xy(nxp, nyp, dx, dy, range1, range2=None, angle=0.0):
+    out = np.zeros((nyp, nxp), dtype=np.float32)
+    lib().orc_corr_xy(nxp, nyp, dx, dy, range1, range1 if range2 is None else range2, angle,
+                      _ptr(out, ctypes.c_float))
+    return out
+
+
+def circ_corr_t(corr_t, min_int_fq, nzp):
+    c = np.ascontiguousarray(corr_t, dtype=np.float64)
+    out = np.zeros(nzp, dtype=np.float32)
+    err = lib().orc_circ_corr_t(_ptr(c, ctypes.c_double), len(c), min_int_fq, nzp, _ptr(out, ctypes.c_float))
+    if err:
+        raise ValueError("circular temporal correlation undefined (too few layers)")
+    return out

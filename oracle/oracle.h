@@ -1,0 +1,129 @@
+/*
+ * oracle.h -- CPU restatement of CRAVA's FFT-domain Bayesian AVO inversion path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP product path in
+ * crava_amd/: only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.
+ * The product never links, imports or falls back to it.
+ *
+ * Every function cites the reference file:line whose behaviour it restates.  Paths are
+ * relative to the CRAVA source tree.  Complex arrays are interleaved (re, im).
+ *
+ * Pins (what checks each function independently of its own formula):
+ *   orc_cfft / orc_rfft3d_*      naive DFT on all radices, FFTW 2.1.2 built from the
+ *                                reference sources (oracle/_ref), round trip, Parseval,
+ *                                Hermitian symmetry.
+ *   orc_posterior_f32/_f64       lib_matr.c compiled from the reference driven in
+ *                                avoinversion.cpp's call order (oracle/_ref), dense FP64
+ *                                Gaussian conditioning on tiny grids, prior pass-through.
+ *   orc_unif01/orc_rnorm01       random.cpp compiled from the reference (oracle/_ref), LCG
+ *                                closed form, moments.
+ *   orc_fill_complex_noise       reference RNG stream + Hermitian pattern checks (the fill
+ *                                itself lives in fftgrid.cpp, which cannot be built here:
+ *                                its stream parity is pinned through the RNG and the pattern).
+ *   orc_colour_f32               lib_matrCholCpx/lib_matrProdCholVec from oracle/_ref, sample
+ *                                covariance of realisations.
+ *   setup (padding, A, Ricker,   closed forms and hand-computed values.
+ *   correlations, tables)
+ */
+#ifndef CRAVA_ORACLE_H
+#define CRAVA_ORACLE_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* FFTGrid cube types (src/fftgrid.h:155). */
+enum { ORC_CTMISSING = 0, ORC_DATA = 1, ORC_PARAMETER = 2, ORC_COVARIANCE = 3 };
+
+/* ---------------------------------------------------------------- padding (a15) */
+int  orc_factorable(int leastint);                  /* fftgrid.cpp:2630-2658 */
+int  orc_padding_size(int n, double pad_fraction);  /* multiintervalgrid.cpp:922-928 */
+
+/* ---------------------------------------------------------------- FFT (a2, a3) */
+/* Naive O(n^2) DFT, sign -1 forward / +1 backward, unnormalised (tiny sizes). */
+void orc_cdft_naive(int n, int sign, const double *in, double *out);
+/* Mixed-radix FFT of `howmany` sequences: element e of sequence h at data[2*(h*dist+e*stride)]. */
+void orc_cfft(int n, int sign, double *data, long stride, long howmany, long dist);
+/* 3D real-to-complex in the FFTW in-place layout (row pitch 2*(nxp/2+1) reals); FFTW pass
+ * order x (r2c), y, z; sign -1; unnormalised.  rfftwnd.c:207-247. */
+void orc_rfft3d_r2c(int nxp, int nyp, int nzp, double *buf);
+/* 3D complex-to-real: z, y complex (sign +1), then x half-complex-to-real that ignores the
+ * imaginary part of the DC and Nyquist bins; unnormalised.  rfftwnd.c:249-288. */
+void orc_rfft3d_c2r(int nxp, int nyp, int nzp, double *buf);
+/* FFTGrid::fftInPlace / invFFTInPlace scaling by cube type (fftgrid.cpp:1587-1645). */
+void orc_grid_fft(int nxp, int nyp, int nzp, int cubetype, double *buf);
+void orc_grid_ifft(int nxp, int nyp, int nzp, int cubetype, double *buf);
+/* Same with float storage (transform evaluated in double, stored in float). */
+void orc_grid_fft_f32(int nxp, int nyp, int nzp, int cubetype, float *buf);
+void orc_grid_ifft_f32(int nxp, int nyp, int nzp, int cubetype, float *buf);
+
+/* ---------------------------------------------------------------- posterior (a5-a8) */
+typedef struct {
+  int nxp, nyp, nzp, ntheta;
+  const float  *K;          /* [nzp][ntheta][3] complex: K = diag(kD*conj(W_l)) A   (avoinversion.cpp:947-965) */
+  const float  *e1;         /* [nzp][ntheta] complex: errMult1                      (:972) */
+  const float  *e2;         /* [nzp][ntheta] complex: errMult2                      (:973) */
+  const float  *e3;         /* [nzp][ntheta] complex: errMult3                      (:974) */
+  const int    *invert;     /* [nzp] 1 where lowCut*minRelThick < f(k) < highCut    (:946,1029) */
+  const double *sigma0e;    /* [ntheta][ntheta] errThetaCov                          (modelavodynamic.cpp:623-651) */
+  float         wnc;        /* white-noise component                                 (modelsettings.cpp:98) */
+  const double *prior_var0; /* [3][3] priorVar0_                                      (seismicparametersholder.cpp:485-556) */
+  int           prior_mode; /* 0: six covariance spectra in cov[]; 1: separable rxy*rt */
+  const float  *rxy;        /* [nyp][cnxp] real, lateral correlation spectrum (mode 1) */
+  const float  *rt;         /* [nzp] real, temporal correlation spectrum      (mode 1) */
+  const float  *lam_grid;   /* errCorr spectrum, complex [nzp][nyp][cnxp], or NULL       */
+  const float  *lam2d;      /* errCorr spectrum, real [nyp][cnxp] (k-independent) if lam_grid NULL */
+} orc_post_spec;
+
+/* FP32 restatement in the reference's operation order (avoinversion.cpp:1016-1083 with
+ * getNextParameterCovariance, getNextErrorVariance and lib_matr*).  Buffers are packed:
+ * mean [3][csize] complex (in: prior mean spectra; out: posterior mean spectra), data
+ * [ntheta][csize] complex (in: seismic spectra; out: residual spectra), cov [6][csize] complex
+ * (in, prior_mode 0: prior covariance spectra; out: posterior covariance spectra, order
+ * Vp, Vs, Rho, VpVs, VpRho, VsRho).  flags[csize] (optional): 0 not inverted, 1 inverted,
+ * 2 Cholesky failed (prior kept).  Returns the number of Cholesky failures. */
+long orc_posterior_f32(const orc_post_spec *s, float *mean, float *data, float *cov, uint8_t *flags);
+/* The same algorithm evaluated in double precision (the accuracy reference). */
+long orc_posterior_f64(const orc_post_spec *s, double *mean, double *data, double *cov, uint8_t *flags);
+
+/* ---------------------------------------------------------------- RNG and noise (a10) */
+double orc_unif01(uint32_t *state);   /* random.cpp:113-118 */
+double orc_rnorm01(uint32_t *state);  /* random.cpp:38-94 (Marsaglia-Bray, Ripley p.84) */
+/* Draw n normals from the reference stream (test helper). */
+void   orc_rnorm01_n(uint32_t *state, double *out, long n);
+/* FFTGrid::fillInComplexNoise (fftgrid.cpp:997-1045), including the Nyquist-column parity rule. */
+void   orc_fill_complex_noise(int nxp, int nyp, int nzp, uint32_t *state, float *cvalue);
+/* Counter-based complex noise with the same Hermitian pattern (the serving/bench generator):
+ * normal pair of complex cell `idx` of stream `stream` = orc_cb_normal2(seed, stream, idx). */
+void   orc_cb_normal2(uint64_t seed, uint32_t stream, uint64_t idx, float *z0, float *z1);
+void   orc_fill_complex_noise_cb(int nxp, int nyp, int nzp, uint64_t seed, uint32_t stream, float *cvalue);
+
+/* ---------------------------------------------------------------- colouring (a9) */
+/* avoinversion.cpp:1421-1460: z <- chol(Sigma_post) z per wavenumber, zero on failure.
+ * cov [6][csize] complex posterior covariance spectra, seed [3][csize] complex in/out. */
+long orc_colour_f32(long csize, const float *cov, float *seed);
+long orc_colour_f64(long csize, const double *cov, double *seed);
+
+/* ---------------------------------------------------------------- setup (a12, a16, a17, f1) */
+double orc_ricker(double t_ms, double peak_hz);                              /* wavelet.cpp:1484-1488 */
+void   orc_aki_richards(double theta_rad, double vsvp, double *row3);        /* commondata.cpp:2915-2971 */
+float  orc_spherical_corr(float dx, float dy, float range1, float range2, float angle); /* vario.cpp:35-74 */
+/* Lateral correlation surface on the padded lattice with wrap-around (commondata.cpp:9180-9218). */
+void   orc_corr_xy(int nxp, int nyp, float dx, float dy, float range1, float range2, float angle, float *out);
+/* Circular temporal correlation made positive definite (seismicparametersholder.cpp:624-656,
+ * 805-837).  corr_t[0..n) input, out[nzp].  Returns 0, or 1 if undefined (circCorrT[0] tiny). */
+int    orc_circ_corr_t(const double *corr_t, int n, int min_int_fq, int nzp, float *out);
+/* Per-k_z tables of the constant-thickness branch (avoinversion.cpp:944-975 with fillkW,
+ * fillkWNorm, Wavelet1D difference operators wavelet1D.cpp:389-500, findNorm wavelet.cpp:882-900).
+ * wavelets [ntheta][nzp] real time-domain wavelets in FFT order; A [ntheta][3]. */
+void   orc_kz_tables(int nz, int nzp, double dt_ms, int ntheta, const float *wavelets,
+                     const float *A, float low_cut, float high_cut, float min_rel_thick,
+                     float *K, float *e1, float *e2, float *e3, int *invert);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
