@@ -1,0 +1,338 @@
+"""crava_amd -- MI355X-native FFT-domain Bayesian AVO inversion (CRAVA's hot path).
+
+Python mirror of the reference's interface for the path, over the C ABI of
+include/cravagpu.h (libcravagpu.so, hand-written HIP for gfx950):
+
+* ``FFTGrid``   -- src/fftgrid.h:22-285: padded cube in the FFTW in-place layout, fftInPlace /
+                   invFFTInPlace with cube-type scaling, complex accessors, add,
+                   multiplyByScalar, fillInComplexNoise (counter-based generator).
+* ``posterior`` -- AVOInversion::computePostMeanResidAndFFTCov, per-wavenumber part
+                   (src/avoinversion.cpp:944-1083).
+* ``simulate``  -- one realisation of AVOInversion::simulate (src/avoinversion.cpp:1401-1511).
+* ``build_kz_tables`` -- per-k_z tables of the constant-thickness branch (avoinversion.cpp:944-975).
+
+The library is required: importing the GPU entry points without it, or on a machine
+without a GPU, raises.  There is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcravagpu.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "cravagpu.h")
+
+CG_OK, CG_EINVAL, CG_ENOMEM, CG_EHIP, CG_ESTATE = 0, -1, -2, -3, -4
+DATA, PARAMETER, COVARIANCE = 1, 2, 3
+PRIOR_GRIDS, PRIOR_SEPARABLE = 0, 1
+PREC_REFERENCE, PREC_MIXED = 0, 1
+NOISE_COUNTER, NOISE_GRIDS = 0, 1
+K_FFT_X, K_FFT_Y, K_FFT_Z, K_POSTERIOR, K_NOISE, K_ELEMENTWISE = range(6)
+RMISSING = -99999.0
+
+_vp = ctypes.c_void_p
+_f_p = ctypes.POINTER(ctypes.c_float)
+_d_p = ctypes.POINTER(ctypes.c_double)
+_i_p = ctypes.POINTER(ctypes.c_int)
+_u64_p = ctypes.POINTER(ctypes.c_uint64)
+
+
+class PostArgs(ctypes.Structure):
+    _fields_ = [
+        ("ntheta", ctypes.c_int),
+        ("K", _f_p), ("e1", _f_p), ("e2", _f_p), ("e3", _f_p), ("invert", _i_p),
+        ("sigma0e", _d_p), ("wnc", ctypes.c_float), ("prior_var0", _d_p), ("prior_mode", ctypes.c_int),
+        ("cov", _vp * 6), ("rxy", _f_p), ("rt", _f_p), ("err_corr", _vp), ("lam2d", _f_p),
+        ("mean", _vp * 3), ("data", _vp * 8), ("want_residual", ctypes.c_int), ("precision", ctypes.c_int),
+        ("cond_limit", ctypes.c_double), ("stats", _u64_p),
+    ]
+
+
+class SimArgs(ctypes.Structure):
+    _fields_ = [
+        ("cov", _vp * 6), ("mean", _vp * 3), ("out", _vp * 3), ("noise_source", ctypes.c_int),
+        ("seed", ctypes.c_uint64), ("realisation", ctypes.c_uint32), ("stats", _u64_p),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    """Load libcravagpu.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("crava_amd: %s is missing; build it with `make -C crava_amd` "
+                               "(or __graft_entry__.build())" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        i, u64, f, d = ctypes.c_int, ctypes.c_uint64, ctypes.c_float, ctypes.c_double
+        sig = {
+            "cg_ctx_create": [ctypes.POINTER(_vp), i, u64],
+            "cg_ctx_destroy": [_vp], "cg_ctx_sync": [_vp],
+            "cg_ctx_memory": [_vp, _u64_p, _u64_p], "cg_ctx_stream": [_vp, ctypes.POINTER(_vp)],
+            "cg_ctx_set_profiling": [_vp, i], "cg_ctx_kernel_time": [_vp, i, _u64_p, _d_p],
+            "cg_ctx_reset_kernel_time": [_vp],
+            "cg_grid_create": [_vp, i, i, i, i, i, i, i, ctypes.POINTER(_vp)],
+            "cg_grid_destroy": [_vp], "cg_grid_set_type": [_vp, i], "cg_grid_is_transformed": [_vp, _i_p],
+            "cg_grid_dims": [_vp, _i_p], "cg_grid_device_ptr": [_vp, ctypes.POINTER(_vp)],
+            "cg_grid_upload_real": [_vp, _f_p], "cg_grid_upload_complex": [_vp, _f_p],
+            "cg_grid_download_real": [_vp, _f_p], "cg_grid_download_complex": [_vp, _f_p],
+            "cg_grid_get_complex": [_vp, i, i, i, i, _f_p], "cg_grid_set_complex": [_vp, i, i, i, i, _f_p],
+            "cg_fft_r2c_inplace": [_vp], "cg_fft_c2r_inplace": [_vp],
+            "cg_grid_add": [_vp, _vp], "cg_grid_multiply_by_scalar": [_vp, f],
+            "cg_posterior": [_vp, ctypes.POINTER(PostArgs)], "cg_simulate": [_vp, ctypes.POINTER(SimArgs)],
+            "cg_grid_fill_noise_counter": [_vp, u64, ctypes.c_uint32],
+            "cg_build_kz_tables": [i, i, d, i, _f_p, _f_p, f, f, f, _f_p, _f_p, _f_p, _f_p, _i_p],
+            "cg_factorable": [i],
+        }
+        for name, args in sig.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = ctypes.c_int
+        L.cg_last_error.argtypes = [_vp]
+        L.cg_last_error.restype = ctypes.c_char_p
+        _lib = L
+    return _lib
+
+
+class CravaError(RuntimeError):
+    pass
+
+
+def _ptr(a, ctype):
+    return None if a is None else a.ctypes.data_as(ctypes.POINTER(ctype))
+
+
+class Context:
+    """One device, one HIP stream, an HBM budget (0 = unlimited)."""
+
+    def __init__(self, device=0, hbm_budget=0):
+        self._h = _vp()
+        rc = lib().cg_ctx_create(ctypes.byref(self._h), device, hbm_budget)
+        if rc != CG_OK:
+            raise CravaError("cg_ctx_create failed (%d): no usable GPU %d" % (rc, device))
+
+    def check(self, rc, what=""):
+        if rc != CG_OK:
+            raise CravaError("%s failed (%d): %s" % (what, rc, lib().cg_last_error(self._h).decode()))
+        return rc
+
+    def sync(self):
+        self.check(lib().cg_ctx_sync(self._h), "cg_ctx_sync")
+
+    def memory(self):
+        a, b = ctypes.c_uint64(), ctypes.c_uint64()
+        lib().cg_ctx_memory(self._h, ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value
+
+    def set_profiling(self, on):
+        self.check(lib().cg_ctx_set_profiling(self._h, 1 if on else 0), "cg_ctx_set_profiling")
+
+    def kernel_time(self, kclass):
+        n, ms = ctypes.c_uint64(), ctypes.c_double()
+        self.check(lib().cg_ctx_kernel_time(self._h, kclass, ctypes.byref(n), ctypes.byref(ms)), "kernel_time")
+        return n.value, ms.value
+
+    def reset_kernel_time(self):
+        self.check(lib().cg_ctx_reset_kernel_time(self._h), "reset_kernel_time")
+
+    def close(self):
+        if self._h:
+            lib().cg_ctx_destroy(self._h)
+            self._h = _vp()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+class FFTGrid:
+    """Device-resident FFTGrid (src/fftgrid.h:22-285) in the reference's in-place layout."""
+    DATA, PARAMETER, COVARIANCE = DATA, PARAMETER, COVARIANCE
+
+    def __init__(self, ctx, nx, ny, nz, nxp, nyp, nzp, cubetype=PARAMETER):
+        self.ctx = ctx
+        self.nx, self.ny, self.nz, self.nxp, self.nyp, self.nzp = nx, ny, nz, nxp, nyp, nzp
+        self.cnxp = nxp // 2 + 1
+        self.rnxp = 2 * self.cnxp
+        self._h = _vp()
+        ctx.check(lib().cg_grid_create(ctx._h, nx, ny, nz, nxp, nyp, nzp, cubetype, ctypes.byref(self._h)),
+                  "cg_grid_create")
+
+    # -- state
+    def setType(self, cubetype):
+        self.ctx.check(lib().cg_grid_set_type(self._h, cubetype), "setType")
+
+    def getIsTransformed(self):
+        t = ctypes.c_int()
+        lib().cg_grid_is_transformed(self._h, ctypes.byref(t))
+        return bool(t.value)
+
+    def device_ptr(self):
+        p = _vp()
+        lib().cg_grid_device_ptr(self._h, ctypes.byref(p))
+        return p.value
+
+    # -- transforms and elementwise ops
+    def fftInPlace(self):
+        self.ctx.check(lib().cg_fft_r2c_inplace(self._h), "fftInPlace")
+
+    def invFFTInPlace(self):
+        self.ctx.check(lib().cg_fft_c2r_inplace(self._h), "invFFTInPlace")
+
+    def add(self, other):
+        self.ctx.check(lib().cg_grid_add(self._h, other._h), "add")
+
+    def multiplyByScalar(self, s):
+        self.ctx.check(lib().cg_grid_multiply_by_scalar(self._h, s), "multiplyByScalar")
+
+    def fillInComplexNoise(self, seed, stream=0):
+        self.ctx.check(lib().cg_grid_fill_noise_counter(self._h, seed, stream), "fillInComplexNoise")
+
+    # -- host mirror
+    def upload_real(self, real):
+        """real: (nzp, nyp, nxp) or padded (nzp, nyp, rnxp) float array."""
+        a = np.asarray(real, dtype=np.float32)
+        if a.shape[-1] == self.nxp and self.nxp != self.rnxp:
+            buf = np.zeros((self.nzp, self.nyp, self.rnxp), dtype=np.float32)
+            buf[:, :, :self.nxp] = a
+        else:
+            buf = np.ascontiguousarray(a)
+        assert buf.size == self.rnxp * self.nyp * self.nzp
+        self.ctx.check(lib().cg_grid_upload_real(self._h, _ptr(buf, ctypes.c_float)), "upload_real")
+
+    def upload_complex(self, spec):
+        a = np.ascontiguousarray(spec, dtype=np.complex64)
+        assert a.size == self.cnxp * self.nyp * self.nzp
+        self.ctx.check(lib().cg_grid_upload_complex(self._h, _ptr(a.view(np.float32), ctypes.c_float)),
+                       "upload_complex")
+
+    def download_real(self, padded=False):
+        buf = np.empty((self.nzp, self.nyp, self.rnxp), dtype=np.float32)
+        self.ctx.check(lib().cg_grid_download_real(self._h, _ptr(buf, ctypes.c_float)), "download_real")
+        return buf if padded else buf[:, :, :self.nxp].copy()
+
+    def download_complex(self):
+        buf = np.empty((self.nzp, self.nyp, self.cnxp), dtype=np.complex64)
+        self.ctx.check(lib().cg_grid_download_complex(self._h, _ptr(buf.view(np.float32), ctypes.c_float)),
+                       "download_complex")
+        return buf
+
+    def getComplexValue(self, i, j, k, extSimbox=False):
+        v = np.zeros(2, dtype=np.float32)
+        self.ctx.check(lib().cg_grid_get_complex(self._h, i, j, k, int(extSimbox), _ptr(v, ctypes.c_float)),
+                       "getComplexValue")
+        return complex(v[0], v[1])
+
+    def setComplexValue(self, i, j, k, value, extSimbox=False):
+        v = np.array([value.real, value.imag], dtype=np.float32)
+        rc = lib().cg_grid_set_complex(self._h, i, j, k, int(extSimbox), _ptr(v, ctypes.c_float))
+        if rc < 0:
+            self.ctx.check(rc, "setComplexValue")
+        return rc
+
+    def close(self):
+        if self._h:
+            lib().cg_grid_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def build_kz_tables(nz, nzp, dt_ms, wavelets, A, low_cut=5.0, high_cut=55.0, min_rel_thick=1.0):
+    """Host-side per-k_z tables (K, e1, e2, e3, invert) of avoinversion.cpp:944-975."""
+    w = np.ascontiguousarray(wavelets, dtype=np.float32)
+    nt = w.shape[0]
+    a = np.ascontiguousarray(A, dtype=np.float32)
+    K = np.zeros((nzp, nt, 3), dtype=np.complex64)
+    e1 = np.zeros((nzp, nt), dtype=np.complex64)
+    e2 = np.zeros((nzp, nt), dtype=np.complex64)
+    e3 = np.zeros((nzp, nt), dtype=np.complex64)
+    inv = np.zeros(nzp, dtype=np.int32)
+    f = ctypes.c_float
+    rc = lib().cg_build_kz_tables(nz, nzp, dt_ms, nt, _ptr(w, f), _ptr(a, f), low_cut, high_cut, min_rel_thick,
+                                  _ptr(K.view(np.float32), f), _ptr(e1.view(np.float32), f),
+                                  _ptr(e2.view(np.float32), f), _ptr(e3.view(np.float32), f),
+                                  _ptr(inv, ctypes.c_int))
+    if rc != CG_OK:
+        raise CravaError("cg_build_kz_tables failed (%d)" % rc)
+    return K, e1, e2, e3, inv
+
+
+def posterior(ctx, tables, mean, data, cov, prior_mode=PRIOR_SEPARABLE, rxy=None, rt=None, err_corr=None,
+              lam2d=None, want_residual=True, precision=PREC_REFERENCE, cond_limit=1e6):
+    """Per-wavenumber posterior on Fourier-domain FFTGrids (in place).
+
+    tables: dict with K, e1, e2, e3, invert, sigma0e, wnc, prior_var0.
+    Returns (inverted, cholesky_failures, fp64_resolves)."""
+    a = PostArgs()
+    keep = []
+
+    def arr(x, dt):
+        y = np.ascontiguousarray(x, dtype=dt)
+        keep.append(y)
+        return y
+
+    nt = len(data)
+    a.ntheta = nt
+    a.K = _ptr(arr(tables["K"], np.complex64).view(np.float32), ctypes.c_float)
+    a.e1 = _ptr(arr(tables["e1"], np.complex64).view(np.float32), ctypes.c_float)
+    a.e2 = _ptr(arr(tables["e2"], np.complex64).view(np.float32), ctypes.c_float)
+    a.e3 = _ptr(arr(tables["e3"], np.complex64).view(np.float32), ctypes.c_float)
+    a.invert = _ptr(arr(tables["invert"], np.int32), ctypes.c_int)
+    a.sigma0e = _ptr(arr(tables["sigma0e"], np.float64), ctypes.c_double)
+    a.wnc = float(tables["wnc"])
+    a.prior_var0 = _ptr(arr(tables["prior_var0"], np.float64), ctypes.c_double)
+    a.prior_mode = prior_mode
+    for g in range(6):
+        a.cov[g] = cov[g]._h
+    if rxy is not None:
+        a.rxy = _ptr(arr(rxy, np.float32), ctypes.c_float)
+    if rt is not None:
+        a.rt = _ptr(arr(rt, np.float32), ctypes.c_float)
+    a.err_corr = err_corr._h if err_corr is not None else None
+    if lam2d is not None:
+        a.lam2d = _ptr(arr(lam2d, np.float32), ctypes.c_float)
+    for g in range(3):
+        a.mean[g] = mean[g]._h
+    for l in range(nt):
+        a.data[l] = data[l]._h
+    a.want_residual = int(want_residual)
+    a.precision = precision
+    a.cond_limit = cond_limit
+    stats = np.zeros(3, dtype=np.uint64)
+    a.stats = _ptr(stats, ctypes.c_uint64)
+    ctx.check(lib().cg_posterior(ctx._h, ctypes.byref(a)), "cg_posterior")
+    return tuple(int(x) for x in stats)
+
+
+def simulate(ctx, cov, mean, out, seed=0, realisation=0, noise_source=NOISE_COUNTER, stats=False):
+    """One posterior realisation into out[3] (real domain).  Returns Cholesky failures."""
+    a = SimArgs()
+    for g in range(6):
+        a.cov[g] = cov[g]._h
+    for g in range(3):
+        a.mean[g] = mean[g]._h if mean is not None and mean[g] is not None else None
+        a.out[g] = out[g]._h
+    a.noise_source = noise_source
+    a.seed = seed
+    a.realisation = realisation
+    st = np.zeros(1, dtype=np.uint64)
+    a.stats = _ptr(st, ctypes.c_uint64) if stats else None
+    ctx.check(lib().cg_simulate(ctx._h, ctypes.byref(a)), "cg_simulate")
+    return int(st[0])
+
+
+def header_symbols():
+    """Function names declared in include/cravagpu.h."""
+    import re
+    txt = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(cg_\w+)\s*\(", txt, re.M)))

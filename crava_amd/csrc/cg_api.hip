@@ -1,0 +1,817 @@
+// cg_api.hip -- the C ABI of include/cravagpu.h on top of the HIP kernels.
+//
+// Contexts replace CRAVA's process-global FFTGrid statics (fftgrid.cpp:2661-2668): each owns
+// a device, a HIP stream, a device-memory budget, twiddle tables and staging buffers.  Grids
+// are device-resident in the reference's in-place FFTW layout.
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <new>
+
+#include "cg_internal.h"
+#include "cg_noise.h"
+#include "cg_post.h"
+
+namespace {
+
+constexpr float kRMissing = -99999.0f;  // RMISSING, definitions.h:35
+
+struct DeviceGuard {
+  explicit DeviceGuard(int dev) { (void)hipSetDevice(dev); }
+};
+
+size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+// Pinned staging of small host tables into one device buffer on the context stream.
+int stage_tables(cg_ctx *ctx, const std::vector<std::pair<const void *, size_t>> &parts, std::vector<void *> &dptrs)
+{
+  size_t total = 0;
+  for (auto &p : parts) total += align_up(p.second, 256);
+  if (total == 0) total = 256;
+  if (!ctx->stage_done) {
+    hipError_t e = hipEventCreateWithFlags(&ctx->stage_done, hipEventDisableTiming);
+    if (e != hipSuccess) return cg::hip_fail(ctx, e, "hipEventCreate");
+  } else {
+    hipError_t e = hipEventSynchronize(ctx->stage_done);  // previous copy has left the buffer
+    if (e != hipSuccess) return cg::hip_fail(ctx, e, "hipEventSynchronize");
+  }
+  if (ctx->stage_host_bytes < total) {
+    if (ctx->stage_host) (void)hipHostFree(ctx->stage_host);
+    ctx->stage_host = nullptr;
+    ctx->stage_host_bytes = 0;
+    hipError_t e = hipHostMalloc(&ctx->stage_host, total, hipHostMallocDefault);
+    if (e != hipSuccess) return cg::hip_fail(ctx, e, "hipHostMalloc(staging)");
+    ctx->stage_host_bytes = total;
+  }
+  char *dev = static_cast<char *>(cg::table_space(ctx, total));
+  if (!dev) return CG_ENOMEM;
+  char *h = static_cast<char *>(ctx->stage_host);
+  size_t off = 0;
+  dptrs.clear();
+  for (auto &p : parts) {
+    if (p.second) std::memcpy(h + off, p.first, p.second);
+    dptrs.push_back(p.second ? dev + off : nullptr);
+    off += align_up(p.second, 256);
+  }
+  hipError_t e = hipMemcpyAsync(dev, h, off ? off : 256, hipMemcpyHostToDevice, ctx->stream);
+  if (e != hipSuccess) return cg::hip_fail(ctx, e, "hipMemcpyAsync(tables)");
+  e = hipEventRecord(ctx->stage_done, ctx->stream);
+  return cg::hip_fail(ctx, e, "hipEventRecord");
+}
+
+bool valid_type(int t) { return t == CG_CT_DATA || t == CG_CT_PARAMETER || t == CG_CT_COVARIANCE; }
+
+bool same_shape(const cg_grid *a, const cg_grid *b)
+{
+  return a->nxp == b->nxp && a->nyp == b->nyp && a->nzp == b->nzp;
+}
+
+// Per-plane tables of the posterior (see cg_post.hip for the identities).
+template <typename R>
+void build_plane_tables(const cg_post_args *a, int nzp, std::vector<cg::PlaneTab<R>> &tab)
+{
+  typedef std::complex<double> C;
+  const int nt = a->ntheta;
+  const double *pv = a->prior_var0;
+  const double S0[3][3] = {{pv[0], pv[1], pv[2]}, {pv[3], pv[4], pv[5]}, {pv[6], pv[7], pv[8]}};
+  const double wnc = a->wnc;
+  tab.assign(nzp, cg::PlaneTab<R>());
+  for (int k = 0; k < nzp; ++k) {
+    cg::PlaneTab<R> &T = tab[k];
+    std::memset(&T, 0, sizeof(T));
+    C K[8][3], S[8][3], e1[8], e2[8];
+    for (int l = 0; l < nt; ++l) {
+      for (int j = 0; j < 3; ++j) {
+        const float *kp = a->K + 2 * ((k * nt + l) * 3 + j);
+        K[l][j] = C(kp[0], kp[1]);
+        T.K[l][j].re = (R)kp[0];
+        T.K[l][j].im = (R)kp[1];
+      }
+      e1[l] = C(a->e1[2 * (k * nt + l)], a->e1[2 * (k * nt + l) + 1]);
+      e2[l] = C(a->e2[2 * (k * nt + l)], a->e2[2 * (k * nt + l) + 1]);
+      const double e3r = a->e3[2 * (k * nt + l)];
+      T.D[l] = (R)(wnc * a->sigma0e[l * nt + l] * e3r * e3r);
+    }
+    for (int l = 0; l < nt; ++l)
+      for (int b = 0; b < 3; ++b) {
+        C x = 0;
+        for (int c = 0; c < 3; ++c) x += K[l][c] * S0[c][b];
+        S[l][b] = x;
+        T.S[l][b].re = (R)x.real();
+        T.S[l][b].im = (R)x.imag();
+      }
+    for (int l = 0; l < nt; ++l)
+      for (int m = 0; m <= l; ++m) {
+        C g = 0;
+        for (int c = 0; c < 3; ++c) g += S[l][c] * std::conj(K[m][c]);
+        const int t = l * (l + 1) / 2 + m;
+        T.G[t].re = (R)g.real();
+        T.G[t].im = (R)g.imag();
+        C q = 0.5 * (1.0 - wnc) * a->sigma0e[l * nt + m] * (e1[l] * std::conj(e1[m]) + e2[l] * std::conj(e2[m]));
+        T.Q[t].re = (R)q.real();
+        T.Q[t].im = (l == m) ? (R)0 : (R)q.imag();
+      }
+  }
+}
+
+// host DFT bin of a real sequence via a cosine/sine table (no trig in the loop)
+void dft_bin(const std::vector<double> &x, const std::vector<double> &ct, const std::vector<double> &st, int k,
+             double &re, double &im)
+{
+  const int n = (int)x.size();
+  double sr = 0, si = 0;
+  long e = 0;
+  for (int t = 0; t < n; ++t) {
+    sr += x[t] * ct[e];
+    si -= x[t] * st[e];
+    e += k;
+    if (e >= n) e -= n;
+  }
+  re = sr;
+  im = si;
+}
+
+float find_norm(const std::vector<float> &r)
+{
+  double n2 = 0;
+  for (float v : r) n2 += (double)(v * v);
+  return (float)std::sqrt(n2);
+}
+
+}  // namespace
+
+namespace cg {
+
+int fail(cg_ctx *ctx, int code, const std::string &msg)
+{
+  if (ctx) ctx->err = msg;
+  return code;
+}
+
+int hip_fail(cg_ctx *ctx, hipError_t e, const char *what)
+{
+  if (e == hipSuccess) return CG_OK;
+  if (ctx) ctx->err = std::string(what) + ": " + hipGetErrorString(e);
+  return CG_EHIP;
+}
+
+const float2 *twiddle_table(cg_ctx *ctx, int n)
+{
+  auto it = ctx->twiddles.find(n);
+  if (it != ctx->twiddles.end()) return it->second;
+  std::vector<float2> h(n);
+  for (int m = 0; m < n; ++m) {
+    const double a = -2.0 * M_PI * (double)m / (double)n;
+    h[m] = make_float2((float)std::cos(a), (float)std::sin(a));
+  }
+  float2 *d = nullptr;
+  hipError_t e = hipMalloc(&d, sizeof(float2) * n);
+  if (e != hipSuccess) {
+    hip_fail(ctx, e, "hipMalloc(twiddles)");
+    return nullptr;
+  }
+  e = hipMemcpyAsync(d, h.data(), sizeof(float2) * n, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) {
+    hip_fail(ctx, e, "hipMemcpy(twiddles)");
+    (void)hipFree(d);
+    return nullptr;
+  }
+  ctx->twiddles[n] = d;
+  ctx->in_use += sizeof(float2) * n;
+  ctx->peak = std::max(ctx->peak, ctx->in_use);
+  return d;
+}
+
+void *table_space(cg_ctx *ctx, size_t bytes)
+{
+  if (ctx->tables_bytes >= bytes) return ctx->tables;
+  if (ctx->tables) {
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(ctx->tables);
+    ctx->in_use -= ctx->tables_bytes;
+    ctx->tables = nullptr;
+    ctx->tables_bytes = 0;
+  }
+  const size_t want = align_up(bytes, 1 << 20);
+  hipError_t e = hipMalloc(&ctx->tables, want);
+  if (e != hipSuccess) {
+    hip_fail(ctx, e, "hipMalloc(tables)");
+    ctx->tables = nullptr;
+    return nullptr;
+  }
+  ctx->tables_bytes = want;
+  ctx->in_use += want;
+  ctx->peak = std::max(ctx->peak, ctx->in_use);
+  return ctx->tables;
+}
+
+void launch_begin(cg_ctx *ctx, int kclass, hipEvent_t *start)
+{
+  *start = nullptr;
+  if (!ctx->profiling) return;
+  if (hipEventCreate(start) != hipSuccess) {
+    *start = nullptr;
+    return;
+  }
+  (void)hipEventRecord(*start, ctx->stream);
+  (void)kclass;
+}
+
+void launch_end(cg_ctx *ctx, int kclass, hipEvent_t start)
+{
+  if (!ctx->profiling || !start) return;
+  hipEvent_t stop;
+  if (hipEventCreate(&stop) != hipSuccess) return;
+  (void)hipEventRecord(stop, ctx->stream);
+  ctx->timed.push_back(TimedLaunch{kclass, start, stop});
+}
+
+}  // namespace cg
+
+// ============================================================================ ABI
+extern "C" {
+
+int cg_ctx_create(cg_ctx **out, int device, uint64_t hbm_budget_bytes)
+{
+  if (!out) return CG_EINVAL;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return CG_EHIP;
+  if (device < 0 || device >= ndev) return CG_EINVAL;
+  cg_ctx *ctx = new (std::nothrow) cg_ctx();
+  if (!ctx) return CG_ENOMEM;
+  ctx->device = device;
+  ctx->budget = hbm_budget_bytes;
+  DeviceGuard g(device);
+  hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipMalloc(&ctx->counters, 8 * sizeof(uint64_t));
+  if (e != hipSuccess) {
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return CG_EHIP;
+  }
+  *out = ctx;
+  return CG_OK;
+}
+
+int cg_ctx_destroy(cg_ctx *ctx)
+{
+  if (!ctx) return CG_EINVAL;
+  DeviceGuard g(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  for (auto &t : ctx->timed) {
+    (void)hipEventDestroy(t.start);
+    (void)hipEventDestroy(t.stop);
+  }
+  for (cg_grid *gr : ctx->grids) {
+    (void)hipFree(gr->d);
+    delete gr;
+  }
+  ctx->grids.clear();
+  for (auto &kv : ctx->twiddles) (void)hipFree(kv.second);
+  if (ctx->tables) (void)hipFree(ctx->tables);
+  if (ctx->counters) (void)hipFree(ctx->counters);
+  if (ctx->stage_host) (void)hipHostFree(ctx->stage_host);
+  if (ctx->stage_done) (void)hipEventDestroy(ctx->stage_done);
+  if (ctx->list) (void)hipFree(ctx->list);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return CG_OK;
+}
+
+const char *cg_last_error(const cg_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int cg_ctx_sync(cg_ctx *ctx)
+{
+  if (!ctx) return CG_EINVAL;
+  DeviceGuard g(ctx->device);
+  return cg::hip_fail(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+}
+
+int cg_ctx_memory(const cg_ctx *ctx, uint64_t *in_use, uint64_t *peak)
+{
+  if (!ctx) return CG_EINVAL;
+  if (in_use) *in_use = ctx->in_use;
+  if (peak) *peak = ctx->peak;
+  return CG_OK;
+}
+
+int cg_ctx_stream(const cg_ctx *ctx, void **stream)
+{
+  if (!ctx || !stream) return CG_EINVAL;
+  *stream = (void *)ctx->stream;
+  return CG_OK;
+}
+
+int cg_ctx_set_profiling(cg_ctx *ctx, int enable)
+{
+  if (!ctx) return CG_EINVAL;
+  ctx->profiling = enable != 0;
+  return CG_OK;
+}
+
+static int collect_times(cg_ctx *ctx)
+{
+  DeviceGuard g(ctx->device);
+  hipError_t e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) return cg::hip_fail(ctx, e, "hipStreamSynchronize");
+  for (auto &t : ctx->timed) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, t.start, t.stop) == hipSuccess) {
+      ctx->launches[t.kclass] += 1;
+      ctx->kernel_ms[t.kclass] += ms;
+    }
+    (void)hipEventDestroy(t.start);
+    (void)hipEventDestroy(t.stop);
+  }
+  ctx->timed.clear();
+  return CG_OK;
+}
+
+int cg_ctx_kernel_time(cg_ctx *ctx, int kclass, uint64_t *launches, double *total_ms)
+{
+  if (!ctx || kclass < 0 || kclass >= CG_K_COUNT) return CG_EINVAL;
+  int rc = collect_times(ctx);
+  if (rc) return rc;
+  if (launches) *launches = ctx->launches[kclass];
+  if (total_ms) *total_ms = ctx->kernel_ms[kclass];
+  return CG_OK;
+}
+
+int cg_ctx_reset_kernel_time(cg_ctx *ctx)
+{
+  if (!ctx) return CG_EINVAL;
+  int rc = collect_times(ctx);
+  for (int i = 0; i < CG_K_COUNT; ++i) {
+    ctx->launches[i] = 0;
+    ctx->kernel_ms[i] = 0;
+  }
+  return rc;
+}
+
+// ---------------------------------------------------------------------------- grids
+int cg_grid_create(cg_ctx *ctx, int nx, int ny, int nz, int nxp, int nyp, int nzp, int cubetype, cg_grid **out)
+{
+  if (!ctx || !out) return CG_EINVAL;
+  *out = nullptr;
+  if (nxp < 1 || nyp < 1 || nzp < 1 || nx < 1 || ny < 1 || nz < 1 || nx > nxp || ny > nyp || nz > nzp)
+    return cg::fail(ctx, CG_EINVAL, "cg_grid_create: need 1 <= n <= n_padded on every axis");
+  if (!valid_type(cubetype)) return cg::fail(ctx, CG_EINVAL, "cg_grid_create: unknown cube type");
+  if ((long)nxp * nyp * nzp > (1L << 33)) return cg::fail(ctx, CG_EINVAL, "cg_grid_create: grid too large");
+  cg_grid *g = new (std::nothrow) cg_grid();
+  if (!g) return CG_ENOMEM;
+  g->ctx = ctx;
+  g->nx = nx; g->ny = ny; g->nz = nz;
+  g->nxp = nxp; g->nyp = nyp; g->nzp = nzp;
+  g->cnxp = nxp / 2 + 1;
+  g->rnxp = 2 * g->cnxp;
+  g->cubetype = cubetype;
+  g->csize = (size_t)g->cnxp * nyp * nzp;
+  g->rsize = 2 * g->csize;
+  const size_t bytes = g->rsize * sizeof(float);
+  if (ctx->budget && ctx->in_use + bytes > ctx->budget) {
+    delete g;
+    return cg::fail(ctx, CG_ENOMEM, "cg_grid_create: HBM budget of the context exceeded");
+  }
+  DeviceGuard dg(ctx->device);
+  hipError_t e = hipMalloc(&g->d, bytes);
+  if (e != hipSuccess) {
+    delete g;
+    return cg::fail(ctx, CG_ENOMEM, std::string("cg_grid_create: hipMalloc: ") + hipGetErrorString(e));
+  }
+  e = hipMemsetAsync(g->d, 0, bytes, ctx->stream);
+  if (e != hipSuccess) {
+    (void)hipFree(g->d);
+    delete g;
+    return cg::hip_fail(ctx, e, "hipMemsetAsync");
+  }
+  ctx->in_use += bytes;
+  ctx->peak = std::max(ctx->peak, ctx->in_use);
+  ctx->grids.insert(g);
+  *out = g;
+  return CG_OK;
+}
+
+int cg_grid_destroy(cg_grid *g)
+{
+  if (!g) return CG_EINVAL;
+  cg_ctx *ctx = g->ctx;
+  DeviceGuard dg(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  (void)hipFree(g->d);
+  ctx->in_use -= g->rsize * sizeof(float);
+  ctx->grids.erase(g);
+  delete g;
+  return CG_OK;
+}
+
+int cg_grid_set_type(cg_grid *g, int cubetype)
+{
+  if (!g) return CG_EINVAL;
+  if (!valid_type(cubetype)) return cg::fail(g->ctx, CG_EINVAL, "unknown cube type");
+  g->cubetype = cubetype;
+  return CG_OK;
+}
+
+int cg_grid_is_transformed(const cg_grid *g, int *t)
+{
+  if (!g || !t) return CG_EINVAL;
+  *t = g->transformed;
+  return CG_OK;
+}
+
+int cg_grid_dims(const cg_grid *g, int dims[6])
+{
+  if (!g || !dims) return CG_EINVAL;
+  dims[0] = g->nx; dims[1] = g->ny; dims[2] = g->nz;
+  dims[3] = g->nxp; dims[4] = g->nyp; dims[5] = g->nzp;
+  return CG_OK;
+}
+
+int cg_grid_device_ptr(cg_grid *g, void **p)
+{
+  if (!g || !p) return CG_EINVAL;
+  *p = g->d;
+  return CG_OK;
+}
+
+static int copy_sync(cg_grid *g, void *dst, const void *src, size_t bytes, hipMemcpyKind kind)
+{
+  DeviceGuard dg(g->ctx->device);
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, g->ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(g->ctx->stream);
+  return cg::hip_fail(g->ctx, e, "grid copy");
+}
+
+int cg_grid_upload_real(cg_grid *g, const float *h)
+{
+  if (!g || !h) return CG_EINVAL;
+  int rc = copy_sync(g, g->d, h, g->rsize * sizeof(float), hipMemcpyHostToDevice);
+  if (rc == CG_OK) g->transformed = 0;
+  return rc;
+}
+
+int cg_grid_upload_complex(cg_grid *g, const float *h)
+{
+  if (!g || !h) return CG_EINVAL;
+  int rc = copy_sync(g, g->d, h, g->rsize * sizeof(float), hipMemcpyHostToDevice);
+  if (rc == CG_OK) g->transformed = 1;
+  return rc;
+}
+
+int cg_grid_download_real(cg_grid *g, float *h)
+{
+  if (!g || !h) return CG_EINVAL;
+  if (g->transformed) return cg::fail(g->ctx, CG_ESTATE, "download_real: grid is in the Fourier domain");
+  return copy_sync(g, h, g->d, g->rsize * sizeof(float), hipMemcpyDeviceToHost);
+}
+
+int cg_grid_download_complex(cg_grid *g, float *h)
+{
+  if (!g || !h) return CG_EINVAL;
+  if (!g->transformed) return cg::fail(g->ctx, CG_ESTATE, "download_complex: grid is in the real domain");
+  return copy_sync(g, h, g->d, g->rsize * sizeof(float), hipMemcpyDeviceToHost);
+}
+
+static bool in_box(const cg_grid *g, int i, int j, int k, int ext)
+{
+  if (i < 0 || j < 0 || k < 0) return false;
+  return ext ? (i < g->nxp && j < g->nyp && k < g->nzp) : (i < g->nx && j < g->ny && k < g->nz);
+}
+
+int cg_grid_get_complex(cg_grid *g, int i, int j, int k, int ext, float v[2])
+{
+  if (!g || !v) return CG_EINVAL;
+  if (!g->transformed) return cg::fail(g->ctx, CG_ESTATE, "get_complex: grid is in the real domain");
+  // getComplexValue tests i < nx (or nxp), not i < cnxp, and then reads index
+  // i + cnxp*j + cnxp*nyp*k: for cnxp <= i it reads into the next row, as the reference
+  // does.  Indexes past the end of the buffer give RMISSING.
+  const size_t idx = (size_t)i + (size_t)g->cnxp * ((size_t)j + (size_t)g->nyp * k);
+  if (!in_box(g, i, j, k, ext) || idx >= g->csize) {
+    v[0] = v[1] = kRMissing;
+    return CG_OK;
+  }
+  return copy_sync(g, v, g->d + 2 * idx, 2 * sizeof(float), hipMemcpyDeviceToHost);
+}
+
+int cg_grid_set_complex(cg_grid *g, int i, int j, int k, int ext, const float v[2])
+{
+  if (!g || !v) return CG_EINVAL;
+  if (!g->transformed) return cg::fail(g->ctx, CG_ESTATE, "set_complex: grid is in the real domain");
+  const size_t idx = (size_t)i + (size_t)g->cnxp * ((size_t)j + (size_t)g->nyp * k);
+  if (!in_box(g, i, j, k, ext) || idx >= g->csize) return 1;
+  return copy_sync(g, g->d + 2 * idx, v, 2 * sizeof(float), hipMemcpyHostToDevice);
+}
+
+int cg_fft_r2c_inplace(cg_grid *g)
+{
+  if (!g) return CG_EINVAL;
+  if (g->transformed) return cg::fail(g->ctx, CG_ESTATE, "fftInPlace: grid already in the Fourier domain");
+  DeviceGuard dg(g->ctx->device);
+  int rc = cg::grid_fft_forward(g);
+  if (rc == CG_OK) g->transformed = 1;
+  return rc;
+}
+
+int cg_fft_c2r_inplace(cg_grid *g)
+{
+  if (!g) return CG_EINVAL;
+  if (!g->transformed) return cg::fail(g->ctx, CG_ESTATE, "invFFTInPlace: grid already in the real domain");
+  DeviceGuard dg(g->ctx->device);
+  int rc = cg::grid_fft_inverse(g, nullptr);
+  if (rc == CG_OK) g->transformed = 0;
+  return rc;
+}
+
+int cg_grid_add(cg_grid *dst, const cg_grid *src)
+{
+  if (!dst || !src) return CG_EINVAL;
+  if (!same_shape(dst, src)) return cg::fail(dst->ctx, CG_EINVAL, "add: grids differ in padded size");
+  if (dst->transformed != src->transformed) return cg::fail(dst->ctx, CG_ESTATE, "add: grids in different domains");
+  DeviceGuard dg(dst->ctx->device);
+  return cg::add_real(dst->ctx, dst->d, src->d, dst->rsize);
+}
+
+int cg_grid_multiply_by_scalar(cg_grid *g, float s)
+{
+  if (!g) return CG_EINVAL;
+  if (g->transformed) return cg::fail(g->ctx, CG_ESTATE, "multiplyByScalar: grid is in the Fourier domain");
+  DeviceGuard dg(g->ctx->device);
+  return cg::scale_real(g->ctx, g->d, g->rsize, s);
+}
+
+int cg_grid_fill_noise_counter(cg_grid *g, uint64_t seed, uint32_t stream)
+{
+  if (!g) return CG_EINVAL;
+  DeviceGuard dg(g->ctx->device);
+  int rc = cg::fill_noise(g->ctx, reinterpret_cast<float2 *>(g->d), (long)g->csize, g->cnxp, g->nyp, g->nzp, seed,
+                          stream);
+  if (rc == CG_OK) {
+    g->transformed = 1;
+    g->cubetype = CG_CT_PARAMETER;
+  }
+  return rc;
+}
+
+// ---------------------------------------------------------------------------- posterior
+int cg_posterior(cg_ctx *ctx, const cg_post_args *a)
+{
+  if (!ctx || !a) return CG_EINVAL;
+  const int nt = a->ntheta;
+  if (nt < 1 || nt > cg::kMaxTheta) return cg::fail(ctx, CG_EINVAL, "posterior: ntheta must be 1..8");
+  if (!a->K || !a->e1 || !a->e2 || !a->e3 || !a->invert || !a->sigma0e || !a->prior_var0)
+    return cg::fail(ctx, CG_EINVAL, "posterior: missing table");
+  if (a->prior_mode != CG_PRIOR_GRIDS && a->prior_mode != CG_PRIOR_SEPARABLE)
+    return cg::fail(ctx, CG_EINVAL, "posterior: unknown prior mode");
+  if (a->precision != CG_PREC_REFERENCE && a->precision != CG_PREC_MIXED)
+    return cg::fail(ctx, CG_EINVAL, "posterior: unknown precision");
+  const cg_grid *ref = a->mean[0];
+  if (!ref) return cg::fail(ctx, CG_EINVAL, "posterior: missing mean grid");
+  std::vector<const cg_grid *> all;
+  for (int g = 0; g < 3; ++g) all.push_back(a->mean[g]);
+  for (int l = 0; l < nt; ++l) all.push_back(a->data[l]);
+  for (int g = 0; g < 6; ++g) all.push_back(a->cov[g]);
+  if (a->err_corr) all.push_back(a->err_corr);
+  for (const cg_grid *g : all) {
+    if (!g) return cg::fail(ctx, CG_EINVAL, "posterior: missing grid");
+    if (g->ctx != ctx) return cg::fail(ctx, CG_EINVAL, "posterior: grid from another context");
+    if (!same_shape(g, ref)) return cg::fail(ctx, CG_EINVAL, "posterior: grids differ in padded size");
+  }
+  for (int g = 0; g < 3; ++g)
+    if (!a->mean[g]->transformed) return cg::fail(ctx, CG_ESTATE, "posterior: mean grids must be transformed");
+  for (int l = 0; l < nt; ++l)
+    if (!a->data[l]->transformed) return cg::fail(ctx, CG_ESTATE, "posterior: data grids must be transformed");
+  if (a->prior_mode == CG_PRIOR_GRIDS) {
+    for (int g = 0; g < 6; ++g)
+      if (!a->cov[g]->transformed) return cg::fail(ctx, CG_ESTATE, "posterior: covariance grids must be transformed");
+  } else if (!a->rxy || !a->rt) {
+    return cg::fail(ctx, CG_EINVAL, "posterior: separable prior needs rxy and rt");
+  }
+  if (a->err_corr) {
+    if (!a->err_corr->transformed) return cg::fail(ctx, CG_ESTATE, "posterior: errCorr grid must be transformed");
+  } else if (!a->lam2d) {
+    return cg::fail(ctx, CG_EINVAL, "posterior: need err_corr grid or lam2d table");
+  }
+  DeviceGuard dg(ctx->device);
+  const int nyp = ref->nyp, nzp = ref->nzp, cnxp = ref->cnxp;
+  const size_t plane = (size_t)cnxp * nyp;
+
+  std::vector<cg::PlaneTab<float>> t32;
+  std::vector<cg::PlaneTab<double>> t64;
+  build_plane_tables<float>(a, nzp, t32);
+  const bool mixed = a->precision == CG_PREC_MIXED;
+  if (mixed) build_plane_tables<double>(a, nzp, t64);
+  std::vector<int> inv(a->invert, a->invert + nzp);
+  size_t ninv = 0;
+  for (int v : inv) ninv += (v != 0);
+  std::vector<std::pair<const void *, size_t>> parts = {
+      {t32.data(), sizeof(cg::PlaneTab<float>) * nzp},
+      {mixed ? (const void *)t64.data() : nullptr, mixed ? sizeof(cg::PlaneTab<double>) * nzp : 0},
+      {inv.data(), sizeof(int) * nzp},
+      {a->prior_mode == CG_PRIOR_SEPARABLE ? a->rxy : nullptr, a->prior_mode == CG_PRIOR_SEPARABLE ? sizeof(float) * plane : 0},
+      {a->prior_mode == CG_PRIOR_SEPARABLE ? a->rt : nullptr, a->prior_mode == CG_PRIOR_SEPARABLE ? sizeof(float) * nzp : 0},
+      {a->err_corr ? nullptr : a->lam2d, a->err_corr ? 0 : sizeof(float) * plane},
+  };
+  // FP64 deferral list: capacity = inverted cells
+  const size_t list_bytes = mixed ? align_up(std::max<size_t>(ninv * plane, 1) * sizeof(unsigned), 256) : 0;
+  std::vector<void *> dp;
+  int rc = stage_tables(ctx, parts, dp);
+  if (rc) return rc;
+  unsigned *list = nullptr;
+  if (mixed) {
+    if (ctx->list_bytes < list_bytes) {
+      (void)hipStreamSynchronize(ctx->stream);
+      if (ctx->list) {
+        (void)hipFree(ctx->list);
+        ctx->in_use -= ctx->list_bytes;
+      }
+      ctx->list = nullptr;
+      ctx->list_bytes = 0;
+      hipError_t e = hipMalloc(&ctx->list, list_bytes);
+      if (e != hipSuccess) return cg::hip_fail(ctx, e, "hipMalloc(fp64 list)");
+      ctx->list_bytes = list_bytes;
+      ctx->in_use += list_bytes;
+      ctx->peak = std::max(ctx->peak, ctx->in_use);
+    }
+    list = ctx->list;
+  }
+  cg::PostParams p;
+  std::memset(&p, 0, sizeof(p));
+  p.nxp = ref->nxp; p.nyp = nyp; p.nzp = nzp; p.cnxp = cnxp; p.nt = nt;
+  p.csize = (long)ref->csize;
+  for (int g = 0; g < 3; ++g) p.mean[g] = reinterpret_cast<float2 *>(a->mean[g]->d);
+  for (int l = 0; l < nt; ++l) p.data[l] = reinterpret_cast<float2 *>(a->data[l]->d);
+  for (int g = 0; g < 6; ++g) p.cov[g] = reinterpret_cast<float2 *>(a->cov[g]->d);
+  p.tab32 = static_cast<const cg::PlaneTab<float> *>(dp[0]);
+  p.tab64 = static_cast<const cg::PlaneTab<double> *>(dp[1]);
+  p.invert = static_cast<const int *>(dp[2]);
+  p.rxy = static_cast<const float *>(dp[3]);
+  p.rt = static_cast<const float *>(dp[4]);
+  p.lam2d = static_cast<const float *>(dp[5]);
+  p.lam_grid = a->err_corr ? reinterpret_cast<const float2 *>(a->err_corr->d) : nullptr;
+  const double *pv = a->prior_var0;
+  const double s6[6] = {pv[0], pv[4], pv[8], pv[1], pv[2], pv[5]};
+  for (int g = 0; g < 6; ++g) {
+    p.s6f[g] = (float)s6[g];
+    p.s6d[g] = s6[g];
+  }
+  p.prior_mode = a->prior_mode;
+  p.want_residual = a->want_residual;
+  p.precision = a->precision;
+  p.cond_limit = (float)(a->cond_limit > 0 ? a->cond_limit : 1e6);
+  p.list = list;
+  const bool counting = mixed || a->stats;
+  p.counters = counting ? reinterpret_cast<unsigned long long *>(ctx->counters) : nullptr;
+  if (counting) {
+    rc = cg::zero_counters(ctx, p.counters, 4);
+    if (rc) return rc;
+  }
+  rc = cg::launch_posterior(ctx, p);
+  if (rc) return rc;
+  if (a->stats) {
+    uint64_t h[4];
+    hipError_t e = hipMemcpyAsync(h, ctx->counters, sizeof(h), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return cg::hip_fail(ctx, e, "posterior stats");
+    a->stats[0] = h[0];
+    a->stats[1] = h[1];
+    a->stats[2] = h[3];  // FP64 re-solves = deferred wavenumbers
+  }
+  return CG_OK;
+}
+
+// ---------------------------------------------------------------------------- simulation
+int cg_simulate(cg_ctx *ctx, const cg_sim_args *a)
+{
+  if (!ctx || !a) return CG_EINVAL;
+  const cg_grid *ref = a->cov[0];
+  if (!ref) return cg::fail(ctx, CG_EINVAL, "simulate: missing covariance grid");
+  for (int g = 0; g < 6; ++g) {
+    if (!a->cov[g] || !same_shape(a->cov[g], ref)) return cg::fail(ctx, CG_EINVAL, "simulate: bad covariance grid");
+    if (!a->cov[g]->transformed) return cg::fail(ctx, CG_ESTATE, "simulate: covariance grids must be transformed");
+  }
+  for (int g = 0; g < 3; ++g) {
+    if (!a->out[g] || !same_shape(a->out[g], ref)) return cg::fail(ctx, CG_EINVAL, "simulate: bad output grid");
+    if (a->mean[g]) {
+      if (!same_shape(a->mean[g], ref)) return cg::fail(ctx, CG_EINVAL, "simulate: bad mean grid");
+      if (a->mean[g]->transformed) return cg::fail(ctx, CG_ESTATE, "simulate: mean grids must be in the real domain");
+    }
+    if (a->noise_source == CG_NOISE_GRIDS && !a->out[g]->transformed)
+      return cg::fail(ctx, CG_ESTATE, "simulate: noise grids must be in the Fourier domain");
+  }
+  if (a->noise_source != CG_NOISE_COUNTER && a->noise_source != CG_NOISE_GRIDS)
+    return cg::fail(ctx, CG_EINVAL, "simulate: unknown noise source");
+  DeviceGuard dg(ctx->device);
+  cg::ColourParams p;
+  std::memset(&p, 0, sizeof(p));
+  p.csize = (long)ref->csize;
+  p.cnxp = ref->cnxp;
+  p.nyp = ref->nyp;
+  p.nzp = ref->nzp;
+  for (int g = 0; g < 6; ++g) p.cov[g] = reinterpret_cast<const float2 *>(a->cov[g]->d);
+  for (int g = 0; g < 3; ++g) p.out[g] = reinterpret_cast<float2 *>(a->out[g]->d);
+  p.from_grids = a->noise_source == CG_NOISE_GRIDS;
+  p.seed = a->seed;
+  p.stream0 = 3u * a->realisation;
+  p.fails = a->stats ? reinterpret_cast<unsigned long long *>(ctx->counters) : nullptr;
+  int rc = CG_OK;
+  if (p.fails) rc = cg::zero_counters(ctx, p.fails, 1);
+  if (rc) return rc;
+  rc = cg::colour(ctx, p);
+  if (rc) return rc;
+  for (int g = 0; g < 3; ++g) {
+    a->out[g]->cubetype = CG_CT_PARAMETER;
+    a->out[g]->transformed = 1;
+    rc = cg::grid_fft_inverse(a->out[g], a->mean[g] ? a->mean[g]->d : nullptr);
+    if (rc) return rc;
+    a->out[g]->transformed = 0;
+  }
+  if (a->stats) {
+    uint64_t h = 0;
+    hipError_t e = hipMemcpyAsync(&h, ctx->counters, sizeof(h), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return cg::hip_fail(ctx, e, "simulate stats");
+    a->stats[0] = h;
+  }
+  return CG_OK;
+}
+
+// ---------------------------------------------------------------------------- host setup
+int cg_factorable(int leastint)
+{
+  if (leastint <= 1) return 1;
+  long best = 1;
+  while (best < leastint) best *= 2;
+  for (long a = 1; a <= 2 * best; a *= 2)
+    for (long b = a; b <= 2 * best; b *= 3)
+      for (long c = b; c <= 2 * best; c *= 5)
+        for (long d = c; d <= 2 * best; d *= 7)
+          if (d >= leastint && d < best) best = d;
+  return (int)best;
+}
+
+int cg_build_kz_tables(int nz, int nzp, double dt_ms, int nt, const float *wav, const float *A, float low_cut,
+                       float high_cut, float min_rel_thick, float *K, float *e1, float *e2, float *e3, int *invert)
+{
+  if (nz < 1 || nzp < nz || nt < 1 || nt > cg::kMaxTheta || !wav || !A || !K || !e1 || !e2 || !e3 || !invert)
+    return CG_EINVAL;
+  std::vector<double> ct(nzp), st(nzp);
+  for (int m = 0; m < nzp; ++m) {
+    ct[m] = std::cos(2.0 * M_PI * m / nzp);
+    st[m] = std::sin(2.0 * M_PI * m / nzp);
+  }
+  // forward difference operator [-1, 0, ..., 0, +1] (wavelet1D.cpp:411-415)
+  std::vector<double> diff(nzp, 0.0);
+  diff[0] = -1.0;
+  diff[nzp - 1] += 1.0;
+  std::vector<std::vector<double>> W(nt, std::vector<double>(nzp)), S(nt, std::vector<double>(nzp));
+  std::vector<float> wn(nt), sn(nt);
+  for (int l = 0; l < nt; ++l) {
+    const float *wl = wav + (size_t)l * nzp;
+    std::vector<float> w(wl, wl + nzp), f(nzp), b(nzp), c(nzp);
+    wn[l] = find_norm(w);
+    // errorSmooth3 = central(backward(forward(w))), circular (wavelet1D.cpp:441-500)
+    for (int i = 0; i < nzp; ++i) f[i] = (i == nzp - 1) ? w[0] - w[i] : w[i + 1] - w[i];
+    for (int i = 0; i < nzp; ++i) b[i] = (i == 0) ? f[0] - f[nzp - 1] : f[i] - f[i - 1];
+    for (int i = 0; i < nzp; ++i) {
+      const float hi = (i == nzp - 1) ? b[0] : b[i + 1];
+      const float lo = (i == 0) ? b[nzp - 1] : b[i - 1];
+      c[i] = (float)(0.5 * (hi - lo));
+    }
+    sn[l] = find_norm(c);
+    for (int i = 0; i < nzp; ++i) {
+      W[l][i] = w[i];
+      S[l][i] = c[i];
+    }
+  }
+  const double lz = nz * dt_ms;
+  for (int k = 0; k < nzp; ++k) {
+    const float rf = (float)((nz * 1000.0f) / (lz * nzp) * std::min(k, nzp - k));
+    invert[k] = (rf > low_cut * (double)min_rel_thick) && (rf < high_cut);
+    double kdr, kdi;
+    dft_bin(diff, ct, st, k, kdr, kdi);
+    const float kDr = (float)kdr, kDi = (float)kdi;
+    for (int l = 0; l < nt; ++l) {
+      double wr, wi, sr, si;
+      dft_bin(W[l], ct, st, k, wr, wi);
+      dft_bin(S[l], ct, st, k, sr, si);
+      const float kwr = (float)wr, kwi = -(float)wi;  // fillkW: conj(W)
+      const float pr = kDr * kwr - kDi * kwi, pi = kDr * kwi + kDi * kwr;
+      for (int j = 0; j < 3; ++j) {
+        K[2 * ((k * nt + l) * 3 + j)] = pr * A[l * 3 + j];
+        K[2 * ((k * nt + l) * 3 + j) + 1] = pi * A[l * 3 + j];
+      }
+      e1[2 * (k * nt + l)] = (float)wr / wn[l];
+      e1[2 * (k * nt + l) + 1] = -((float)wi / wn[l]);
+      e2[2 * (k * nt + l)] = (float)sr / sn[l];
+      e2[2 * (k * nt + l) + 1] = -((float)si / sn[l]);
+      e3[2 * (k * nt + l)] = 1.0f;
+      e3[2 * (k * nt + l) + 1] = 0.0f;
+    }
+  }
+  return CG_OK;
+}
+
+}  // extern "C"

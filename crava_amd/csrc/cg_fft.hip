@@ -1,0 +1,532 @@
+// cg_fft.hip -- mixed-radix Stockham FFT passes staged in LDS, gfx950.
+//
+// Replaces the FFTW 2.1.2 transforms behind FFTGrid::fftInPlace / invFFTInPlace
+// (fftgrid.cpp:1587-1645; pass order rfftw/rfftwnd.c:207-288):
+//   forward : x real->half-complex (one row per LDS sequence), then y, then z complex passes
+//   inverse : z, y complex passes, then x half-complex->real ignoring Im of DC and Nyquist
+// Each pass reads and writes the grid once.  A workgroup loads a tile (rows for x, or TI
+// consecutive columns over the whole axis for y/z, 64-128 B row segments) into LDS, runs the
+// radix stages between two LDS buffers and writes the tile back.  Sign -1 forward.
+// Radices 2, 3, 4, 5, 7, 8 cover every padded length CRAVA produces (2^a 3^b 5^c 7^d).
+#include <cmath>
+
+#include "cg_internal.h"
+
+namespace {
+
+struct c2 {
+  float x, y;
+};
+
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b)
+{
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+__device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+// multiply by DIR * i
+template <int DIR>
+__device__ __forceinline__ float2 mul_di(float2 a)
+{
+  return DIR > 0 ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+
+// ---------------------------------------------------------------- small DFTs
+// X[q] = sum_r v[r] w^(r q), w = exp(DIR 2 pi i / p)
+template <int DIR>
+__device__ __forceinline__ void dft2(float2 *v)
+{
+  float2 a = v[0], b = v[1];
+  v[0] = cadd(a, b);
+  v[1] = csub(a, b);
+}
+
+template <int DIR>
+__device__ __forceinline__ void dft3(float2 *v)
+{
+  const float s = 0.86602540378443864676f;
+  float2 t1 = cadd(v[1], v[2]);
+  float2 t2 = make_float2(v[0].x - 0.5f * t1.x, v[0].y - 0.5f * t1.y);
+  float2 t3 = mul_di<DIR>(cscale(csub(v[1], v[2]), s));
+  v[0] = cadd(v[0], t1);
+  v[1] = cadd(t2, t3);
+  v[2] = csub(t2, t3);
+}
+
+template <int DIR>
+__device__ __forceinline__ void dft4(float2 *v)
+{
+  float2 a = cadd(v[0], v[2]), b = csub(v[0], v[2]);
+  float2 c = cadd(v[1], v[3]), d = mul_di<DIR>(csub(v[1], v[3]));
+  v[0] = cadd(a, c);
+  v[2] = csub(a, c);
+  v[1] = cadd(b, d);
+  v[3] = csub(b, d);
+}
+
+template <int DIR>
+__device__ __forceinline__ void dft5(float2 *v)
+{
+  const float c1 = 0.30901699437494742410f, c2 = -0.80901699437494742410f;
+  const float s1 = 0.95105651629515357212f, s2 = 0.58778525229247312917f;
+  float2 a1 = cadd(v[1], v[4]), b1 = csub(v[1], v[4]);
+  float2 a2 = cadd(v[2], v[3]), b2 = csub(v[2], v[3]);
+  float2 x0 = v[0];
+  float2 r1 = make_float2(x0.x + c1 * a1.x + c2 * a2.x, x0.y + c1 * a1.y + c2 * a2.y);
+  float2 r2 = make_float2(x0.x + c2 * a1.x + c1 * a2.x, x0.y + c2 * a1.y + c1 * a2.y);
+  float2 i1 = mul_di<DIR>(make_float2(s1 * b1.x + s2 * b2.x, s1 * b1.y + s2 * b2.y));
+  float2 i2 = mul_di<DIR>(make_float2(s2 * b1.x - s1 * b2.x, s2 * b1.y - s1 * b2.y));
+  v[0] = cadd(x0, cadd(a1, a2));
+  v[1] = cadd(r1, i1);
+  v[4] = csub(r1, i1);
+  v[2] = cadd(r2, i2);
+  v[3] = csub(r2, i2);
+}
+
+template <int DIR>
+__device__ __forceinline__ void dft7(float2 *v)
+{
+  // cos(2 pi m / 7), sin(2 pi m / 7), m = 1..3
+  const float C[3] = {0.62348980185873353053f, -0.22252093395631440429f, -0.90096886790241912624f};
+  const float S[3] = {0.78183148246802980871f, 0.97492791218182360702f, 0.43388373911755812048f};
+  float2 a[3], b[3];
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    a[m] = cadd(v[m + 1], v[6 - m]);
+    b[m] = csub(v[m + 1], v[6 - m]);
+  }
+  float2 x0 = v[0];
+  v[0] = cadd(x0, cadd(a[0], cadd(a[1], a[2])));
+#pragma unroll
+  for (int q = 1; q <= 3; ++q) {
+    float2 re = x0, im = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int m = 1; m <= 3; ++m) {
+      const int e = (m * q) % 7;  // exponent
+      // cos/sin(2 pi e / 7) from the m = 1..3 tables with symmetry
+      const float c = (e <= 3) ? C[e - 1] : C[7 - e - 1];
+      const float s = (e <= 3) ? S[e - 1] : -S[7 - e - 1];
+      re = make_float2(re.x + c * a[m - 1].x, re.y + c * a[m - 1].y);
+      im = make_float2(im.x + s * b[m - 1].x, im.y + s * b[m - 1].y);
+    }
+    float2 t = mul_di<DIR>(im);
+    v[q] = cadd(re, t);
+    v[7 - q] = csub(re, t);
+  }
+}
+
+template <int DIR>
+__device__ __forceinline__ void dft8(float2 *v)
+{
+  const float h = 0.70710678118654752440f;
+  float2 e[4] = {v[0], v[2], v[4], v[6]};
+  float2 o[4] = {v[1], v[3], v[5], v[7]};
+  dft4<DIR>(e);
+  dft4<DIR>(o);
+  // w^1 = (1 + DIR i)/sqrt2, w^2 = DIR i, w^3 = (-1 + DIR i)/sqrt2
+  float2 o1 = cscale(cadd(o[1], mul_di<DIR>(o[1])), h);
+  float2 o2 = mul_di<DIR>(o[2]);
+  float2 o3 = cscale(csub(mul_di<DIR>(o[3]), o[3]), h);
+  v[0] = cadd(e[0], o[0]);
+  v[4] = csub(e[0], o[0]);
+  v[1] = cadd(e[1], o1);
+  v[5] = csub(e[1], o1);
+  v[2] = cadd(e[2], o2);
+  v[6] = csub(e[2], o2);
+  v[3] = cadd(e[3], o3);
+  v[7] = csub(e[3], o3);
+}
+
+// One Stockham radix-P stage over `nbatch` sequences of length n held in LDS.
+// Element e of sequence b lives at buf[b*bstride + e*estride].  Butterfly j reads
+// elements j + q*(n/P), applies w_{Ns*P}^{q k} (k = j mod Ns) and writes elements
+// (j/Ns)*Ns*P + k + q*Ns.  `bfast` chooses which index runs fastest across threads.
+template <int DIR, int P>
+__device__ __forceinline__ void stage(const float2 *__restrict__ src, float2 *__restrict__ dst, int n, int Ns,
+                                      int estride, int bstride, int nbatch, bool bfast,
+                                      const float2 *__restrict__ tw, int tid, int nthr)
+{
+  const int nb = n / P;
+  const int total = nb * nbatch;
+  const int twstep = n / (Ns * P);
+  for (int w = tid; w < total; w += nthr) {
+    int b, j;
+    if (bfast) {
+      b = w % nbatch;
+      j = w / nbatch;
+    } else {
+      j = w % nb;
+      b = w / nb;
+    }
+    const int k = j % Ns;
+    float2 v[P];
+    const float2 *s = src + b * bstride;
+#pragma unroll
+    for (int q = 0; q < P; ++q) v[q] = s[(j + q * nb) * estride];
+    if (k != 0) {
+#pragma unroll
+      for (int q = 1; q < P; ++q) {
+        float2 t = tw[q * k * twstep];  // q k twstep < n
+        if (DIR > 0) t.y = -t.y;
+        v[q] = cmul(v[q], t);
+      }
+    }
+    if (P == 2) dft2<DIR>(v);
+    if (P == 3) dft3<DIR>(v);
+    if (P == 4) dft4<DIR>(v);
+    if (P == 5) dft5<DIR>(v);
+    if (P == 7) dft7<DIR>(v);
+    if (P == 8) dft8<DIR>(v);
+    const int d = (j / Ns) * Ns * P + k;
+    float2 *o = dst + b * bstride;
+#pragma unroll
+    for (int q = 0; q < P; ++q) o[(d + q * Ns) * estride] = v[q];
+  }
+}
+
+// Runs all stages; data starts in bufA and the final result is returned (bufA or bufB).
+template <int DIR>
+__device__ float2 *lds_fft(float2 *bufA, float2 *bufB, const cg::FftPlan &plan, int estride, int bstride,
+                           int nbatch, bool bfast, const float2 *tw, int tid, int nthr)
+{
+  float2 *src = bufA, *dst = bufB;
+  int Ns = 1;
+  for (int s = 0; s < plan.nstages; ++s) {
+    const int p = plan.radix[s];
+    switch (p) {
+      case 2: stage<DIR, 2>(src, dst, plan.n, Ns, estride, bstride, nbatch, bfast, tw, tid, nthr); break;
+      case 3: stage<DIR, 3>(src, dst, plan.n, Ns, estride, bstride, nbatch, bfast, tw, tid, nthr); break;
+      case 4: stage<DIR, 4>(src, dst, plan.n, Ns, estride, bstride, nbatch, bfast, tw, tid, nthr); break;
+      case 5: stage<DIR, 5>(src, dst, plan.n, Ns, estride, bstride, nbatch, bfast, tw, tid, nthr); break;
+      case 7: stage<DIR, 7>(src, dst, plan.n, Ns, estride, bstride, nbatch, bfast, tw, tid, nthr); break;
+      default: stage<DIR, 8>(src, dst, plan.n, Ns, estride, bstride, nbatch, bfast, tw, tid, nthr); break;
+    }
+    Ns *= p;
+    __syncthreads();
+    float2 *t = src;
+    src = dst;
+    dst = t;
+  }
+  return src;
+}
+
+__device__ __forceinline__ void load_twiddles(float2 *lds_tw, const float2 *gtw, int n, int tid, int nthr)
+{
+  for (int m = tid; m < n; m += nthr) lds_tw[m] = gtw[m];
+}
+
+// ---------------------------------------------------------------- x axis, forward
+// rows of nxp reals at pitch rnxp floats -> cnxp complex bins in place.  Even nxp: the row is
+// read as nxp/2 complex numbers z[m] = x[2m] + i x[2m+1]; after the half-length FFT,
+// X[k] = E[k] + w^k O[k] with E = (Z[k] + conj Z[n2-k]) / 2, O = -i (Z[k] - conj Z[n2-k]) / 2,
+// w = exp(-2 pi i / nxp).  Odd nxp: full-length complex FFT of (x, 0).
+__global__ void __launch_bounds__(cg::kThreads)
+k_fft_x_r2c(float *__restrict__ grid, int nxp, long rows, int rows_per_wg, cg::FftPlan plan,
+            const float2 *__restrict__ tw_fft, const float2 *__restrict__ tw_row, float scale)
+{
+  extern __shared__ float2 smem[];
+  const int n = plan.n;  // nxp/2 (even) or nxp (odd)
+  const int cnxp = nxp / 2 + 1, rnxp = 2 * cnxp;
+  const bool even = (nxp % 2) == 0;
+  float2 *tw = smem;
+  float2 *A = smem + n;
+  float2 *B = A + (size_t)rows_per_wg * n;
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const unsigned blk = cg_xcd_remap(blockIdx.x, gridDim.x);
+  const long row0 = (long)blk * rows_per_wg;
+  load_twiddles(tw, tw_fft, n, tid, nthr);
+  for (int idx = tid; idx < rows_per_wg * n; idx += nthr) {
+    const int r = idx / n, m = idx % n;
+    const long row = row0 + r;
+    float2 v = make_float2(0.f, 0.f);
+    if (row < rows) {
+      const float *p = grid + row * rnxp;
+      if (even) {
+        v = reinterpret_cast<const float2 *>(p)[m];
+        v = cscale(v, scale);
+      } else {
+        v = make_float2(p[m] * scale, 0.f);
+      }
+    }
+    A[idx] = v;
+  }
+  __syncthreads();
+  float2 *Z = lds_fft<-1>(A, B, plan, 1, n, rows_per_wg, false, tw, tid, nthr);
+  for (int idx = tid; idx < rows_per_wg * cnxp; idx += nthr) {
+    const int r = idx / cnxp, k = idx % cnxp;
+    const long row = row0 + r;
+    if (row >= rows) continue;
+    float2 X;
+    if (even) {
+      const float2 zk = Z[r * n + (k % n)];
+      const float2 zn = cconj(Z[r * n + ((n - k) % n)]);
+      const float2 E = cscale(cadd(zk, zn), 0.5f);
+      const float2 d = csub(zk, zn);
+      const float2 O = make_float2(0.5f * d.y, -0.5f * d.x);  // -i d / 2
+      X = cadd(E, cmul(tw_row[k], O));
+    } else {
+      X = Z[r * n + k];
+    }
+    reinterpret_cast<float2 *>(grid + row * rnxp)[k] = X;
+  }
+}
+
+// ---------------------------------------------------------------- x axis, inverse
+// cnxp half-complex bins -> nxp reals, unnormalised, then x scale (+ add[] if given).  The
+// imaginary parts of the DC and (even nxp) Nyquist bins are ignored, as FFTW's hc2r does.
+// Even nxp: Z[k] = (X[k] + conj X[n2-k]) + i w^-k (X[k] - conj X[n2-k]); z = IFFT_n2(Z);
+// x[2m] = Re z[m], x[2m+1] = Im z[m].  Odd nxp: Hermitian extension and full-length IFFT.
+__global__ void __launch_bounds__(cg::kThreads)
+k_fft_x_c2r(float *__restrict__ grid, int nxp, long rows, int rows_per_wg, cg::FftPlan plan,
+            const float2 *__restrict__ tw_fft, const float2 *__restrict__ tw_row, float scale,
+            const float *__restrict__ add)
+{
+  extern __shared__ float2 smem[];
+  const int n = plan.n;
+  const int cnxp = nxp / 2 + 1, rnxp = 2 * cnxp;
+  const bool even = (nxp % 2) == 0;
+  float2 *tw = smem;
+  float2 *A = smem + n;
+  float2 *B = A + (size_t)rows_per_wg * n;
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const unsigned blk = cg_xcd_remap(blockIdx.x, gridDim.x);
+  const long row0 = (long)blk * rows_per_wg;
+  load_twiddles(tw, tw_fft, n, tid, nthr);
+  for (int idx = tid; idx < rows_per_wg * n; idx += nthr) {
+    const int r = idx / n, k = idx % n;
+    const long row = row0 + r;
+    float2 v = make_float2(0.f, 0.f);
+    if (row < rows) {
+      const float2 *p = reinterpret_cast<const float2 *>(grid + row * rnxp);
+      if (even) {
+        float2 xk = p[k];
+        float2 xn = p[n - k];
+        if (k == 0) {
+          xk.y = 0.f;  // DC
+          xn.y = 0.f;  // Nyquist
+        }
+        xn = cconj(xn);
+        const float2 E = cadd(xk, xn);
+        const float2 O = cmul(csub(xk, xn), cconj(tw_row[k]));
+        v = make_float2(E.x - O.y, E.y + O.x);  // E + i O
+      } else {
+        if (k == 0) {
+          v = make_float2(p[0].x, 0.f);
+        } else if (k < cnxp) {
+          v = p[k];
+        } else {
+          v = cconj(p[n - k]);
+        }
+      }
+    }
+    A[idx] = v;
+  }
+  __syncthreads();
+  float2 *Z = lds_fft<+1>(A, B, plan, 1, n, rows_per_wg, false, tw, tid, nthr);
+  for (int idx = tid; idx < rows_per_wg * cnxp; idx += nthr) {
+    const int r = idx / cnxp, m = idx % cnxp;
+    const long row = row0 + r;
+    if (row >= rows) continue;
+    float *p = grid + row * rnxp;
+    const float *a = add ? add + row * rnxp : nullptr;
+    if (even) {
+      // complex slot m < n holds reals 2m, 2m+1; slot n is the pitch padding
+      float2 v = (m < n) ? cscale(Z[r * n + m], scale) : make_float2(0.f, 0.f);
+      if (a && m < n) {
+        v.x += a[2 * m];
+        v.y += a[2 * m + 1];
+      }
+      reinterpret_cast<float2 *>(p)[m] = v;
+    } else {
+      // reals 2m, 2m+1 from the real parts of z[2m], z[2m+1] (index nxp is padding)
+      const int i0 = 2 * m, i1 = 2 * m + 1;
+      float v0 = Z[r * n + i0].x * scale;
+      float v1 = (i1 < nxp) ? Z[r * n + i1].x * scale : 0.f;
+      if (a) {
+        v0 += a[i0];
+        if (i1 < nxp) v1 += a[i1];
+      }
+      reinterpret_cast<float2 *>(p)[m] = make_float2(v0, v1);
+    }
+  }
+}
+
+// ---------------------------------------------------------------- strided complex axis
+// Transforms along an axis with element stride S (complex units).  The tile is TI consecutive
+// inner columns over all n elements: data[o*OS + e*S + c0 + t].
+template <int DIR>
+__global__ void __launch_bounds__(cg::kThreads)
+k_fft_strided(float2 *__restrict__ data, cg::FftPlan plan, long S, long inner, long OS, int TI,
+              unsigned tiles_per_outer, const float2 *__restrict__ tw_fft, float scale)
+{
+  extern __shared__ float2 smem[];
+  const int n = plan.n;
+  float2 *tw = smem;
+  float2 *A = smem + n;
+  float2 *B = A + (size_t)n * TI;
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const unsigned blk = cg_xcd_remap(blockIdx.x, gridDim.x);
+  const unsigned o = blk / tiles_per_outer;
+  const long c0 = (long)(blk % tiles_per_outer) * TI;
+  float2 *base = data + (long)o * OS + c0;
+  const int ncol = (int)((inner - c0) < TI ? (inner - c0) : TI);
+  load_twiddles(tw, tw_fft, n, tid, nthr);
+  for (int idx = tid; idx < n * TI; idx += nthr) {
+    const int e = idx / TI, t = idx % TI;
+    A[idx] = (t < ncol) ? base[e * S + t] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  float2 *R = lds_fft<DIR>(A, B, plan, TI, 1, TI, true, tw, tid, nthr);
+  for (int idx = tid; idx < n * TI; idx += nthr) {
+    const int e = idx / TI, t = idx % TI;
+    if (t < ncol) base[e * S + t] = (scale == 1.0f) ? R[idx] : cscale(R[idx], scale);
+  }
+}
+
+constexpr size_t kLdsBudget = 80 * 1024;  // bytes per workgroup: two workgroups per CU (160 KB LDS)
+
+}  // namespace
+
+namespace cg {
+
+bool make_plan(int n, FftPlan *plan)
+{
+  plan->n = n;
+  plan->nstages = 0;
+  int m = n;
+  // largest radices first: fewer LDS round trips
+  const int order[] = {8, 7, 5, 4, 3, 2};
+  while (m > 1) {
+    bool found = false;
+    for (int p : order) {
+      if (m % p == 0) {
+        if (plan->nstages >= kMaxStages) return false;
+        plan->radix[plan->nstages++] = p;
+        m /= p;
+        found = true;
+        break;
+      }
+    }
+    if (!found) return false;
+  }
+  return true;
+}
+
+int fft_x_r2c(cg_ctx *ctx, float *grid, int nxp, long rows, float scale)
+{
+  const bool even = (nxp % 2) == 0;
+  FftPlan plan;
+  const int n = even ? nxp / 2 : nxp;
+  if (!make_plan(n, &plan)) return fail(ctx, CG_EINVAL, "x length has a prime factor above 7");
+  const float2 *twf = twiddle_table(ctx, n);
+  const float2 *twr = twiddle_table(ctx, nxp);
+  if (!twf || !twr) return CG_ENOMEM;
+  int rpw = (int)((kLdsBudget / sizeof(float2) - n) / (2 * (size_t)n));
+  if (rpw < 1) rpw = 1;
+  if (rpw > 16) rpw = 16;
+  const size_t lds = sizeof(float2) * ((size_t)n + 2 * (size_t)rpw * n);
+  const unsigned nblk = (unsigned)((rows + rpw - 1) / rpw);
+  hipEvent_t ev;
+  launch_begin(ctx, CG_K_FFT_X, &ev);
+  k_fft_x_r2c<<<nblk, kThreads, lds, ctx->stream>>>(grid, nxp, rows, rpw, plan, twf, twr, scale);
+  launch_end(ctx, CG_K_FFT_X, ev);
+  return hip_fail(ctx, hipGetLastError(), "k_fft_x_r2c");
+}
+
+int fft_x_c2r(cg_ctx *ctx, float *grid, int nxp, long rows, float scale, const float *add)
+{
+  const bool even = (nxp % 2) == 0;
+  FftPlan plan;
+  const int n = even ? nxp / 2 : nxp;
+  if (!make_plan(n, &plan)) return fail(ctx, CG_EINVAL, "x length has a prime factor above 7");
+  const float2 *twf = twiddle_table(ctx, n);
+  const float2 *twr = twiddle_table(ctx, nxp);
+  if (!twf || !twr) return CG_ENOMEM;
+  int rpw = (int)((kLdsBudget / sizeof(float2) - n) / (2 * (size_t)n));
+  if (rpw < 1) rpw = 1;
+  if (rpw > 16) rpw = 16;
+  const size_t lds = sizeof(float2) * ((size_t)n + 2 * (size_t)rpw * n);
+  const unsigned nblk = (unsigned)((rows + rpw - 1) / rpw);
+  hipEvent_t ev;
+  launch_begin(ctx, CG_K_FFT_X, &ev);
+  k_fft_x_c2r<<<nblk, kThreads, lds, ctx->stream>>>(grid, nxp, rows, rpw, plan, twf, twr, scale, add);
+  launch_end(ctx, CG_K_FFT_X, ev);
+  return hip_fail(ctx, hipGetLastError(), "k_fft_x_c2r");
+}
+
+int fft_strided(cg_ctx *ctx, float2 *data, int n, long S, long inner, long outer, long OS, int dir, float scale,
+                int kclass)
+{
+  FftPlan plan;
+  if (!make_plan(n, &plan)) return fail(ctx, CG_EINVAL, "axis length has a prime factor above 7");
+  const float2 *twf = twiddle_table(ctx, n);
+  if (!twf) return CG_ENOMEM;
+  int TI = 16;
+  while (TI > 1 && sizeof(float2) * ((size_t)n + 2 * (size_t)n * TI) > kLdsBudget) TI /= 2;
+  if (sizeof(float2) * ((size_t)n + 2 * (size_t)n * TI) > 160 * 1024)
+    return fail(ctx, CG_EINVAL, "axis length too long for the LDS engine");
+  const size_t lds = sizeof(float2) * ((size_t)n + 2 * (size_t)n * TI);
+  const unsigned tpo = (unsigned)((inner + TI - 1) / TI);
+  const unsigned long nblk = (unsigned long)tpo * (unsigned long)outer;
+  if (nblk > 0x7fffffffUL) return fail(ctx, CG_EINVAL, "grid too large");
+  hipEvent_t ev;
+  launch_begin(ctx, kclass, &ev);
+  if (dir < 0)
+    k_fft_strided<-1><<<(unsigned)nblk, kThreads, lds, ctx->stream>>>(data, plan, S, inner, OS, TI, tpo, twf,
+                                                                        scale);
+  else
+    k_fft_strided<+1><<<(unsigned)nblk, kThreads, lds, ctx->stream>>>(data, plan, S, inner, OS, TI, tpo, twf,
+                                                                        scale);
+  launch_end(ctx, kclass, ev);
+  return hip_fail(ctx, hipGetLastError(), "k_fft_strided");
+}
+
+int fft_strided(cg_ctx *ctx, float2 *data, int n, long S, long inner, long outer, long OS, int dir, float scale)
+{
+  return fft_strided(ctx, data, n, S, inner, outer, OS, dir, scale, CG_K_FFT_Y);
+}
+
+// FFTGrid::fftInPlace: pre-scale (fused into the x pass), x r2c, y, z.
+int grid_fft_forward(cg_grid *g)
+{
+  cg_ctx *ctx = g->ctx;
+  const long N = (long)g->nxp * g->nyp * g->nzp;
+  const float s = (g->cubetype == CG_CT_COVARIANCE) ? 1.0f : 1.0f / sqrtf((float)N);
+  int rc = fft_x_r2c(ctx, g->d, g->nxp, (long)g->nyp * g->nzp, s);
+  if (rc) return rc;
+  float2 *c = reinterpret_cast<float2 *>(g->d);
+  const long cnxp = g->cnxp, plane = cnxp * g->nyp;
+  if (g->nyp > 1) {
+    rc = fft_strided(ctx, c, g->nyp, cnxp, cnxp, g->nzp, plane, -1, 1.0f, CG_K_FFT_Y);
+    if (rc) return rc;
+  }
+  if (g->nzp > 1) {
+    rc = fft_strided(ctx, c, g->nzp, plane, plane, 1, 0, -1, 1.0f, CG_K_FFT_Z);
+    if (rc) return rc;
+  }
+  return CG_OK;
+}
+
+// FFTGrid::invFFTInPlace: z, y complex, x c2r with the scale (and an optional real add) fused.
+int grid_fft_inverse(cg_grid *g, const float *add_real)
+{
+  cg_ctx *ctx = g->ctx;
+  const size_t N = (size_t)g->nxp * g->nyp * g->nzp;
+  const float s = (g->cubetype == CG_CT_COVARIANCE) ? float(1.0 / (double)N) : float(1.0 / sqrtf(float(N)));
+  float2 *c = reinterpret_cast<float2 *>(g->d);
+  const long cnxp = g->cnxp, plane = cnxp * g->nyp;
+  int rc;
+  if (g->nzp > 1) {
+    rc = fft_strided(ctx, c, g->nzp, plane, plane, 1, 0, +1, 1.0f, CG_K_FFT_Z);
+    if (rc) return rc;
+  }
+  if (g->nyp > 1) {
+    rc = fft_strided(ctx, c, g->nyp, cnxp, cnxp, g->nzp, plane, +1, 1.0f, CG_K_FFT_Y);
+    if (rc) return rc;
+  }
+  return fft_x_c2r(ctx, g->d, g->nxp, (long)g->nyp * g->nzp, s, add_real);
+}
+
+}  // namespace cg

@@ -1,0 +1,103 @@
+// cg_internal.h -- shared declarations of the HIP implementation behind include/cravagpu.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <set>
+#include <string>
+#include <vector>
+
+#include "cravagpu.h"
+
+namespace cg {
+
+constexpr int kMaxStages = 24;
+constexpr int kMaxTheta = 8;
+constexpr int kThreads = 256;
+
+// A 1D transform length factored into the radices the LDS engine implements.
+struct FftPlan {
+  int n;
+  int nstages;
+  int radix[kMaxStages];
+};
+
+bool make_plan(int n, FftPlan *plan);  // false if n has a prime factor above 7
+
+struct TimedLaunch {
+  int kclass;
+  hipEvent_t start, stop;
+};
+
+}  // namespace cg
+
+struct cg_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  uint64_t budget = 0, in_use = 0, peak = 0;
+  // twiddle tables exp(-2 pi i m / n), m in [0, n), per length, device memory
+  std::map<int, float2 *> twiddles;
+  // per-call device staging for host tables, grown on demand
+  void *tables = nullptr;
+  size_t tables_bytes = 0;
+  uint64_t *counters = nullptr;  // device counters for posterior/simulation statistics
+  // pinned host staging of small tables; `stage_done` is recorded after the copy out of it
+  void *stage_host = nullptr;
+  size_t stage_host_bytes = 0;
+  hipEvent_t stage_done = nullptr;
+  // wavenumbers deferred to the FP64 posterior pass
+  unsigned *list = nullptr;
+  size_t list_bytes = 0;
+  std::set<cg_grid *> grids;  // live grids; cg_ctx_destroy releases any left over
+  bool profiling = false;
+  std::vector<cg::TimedLaunch> timed;
+  std::vector<hipEvent_t> event_pool;
+  uint64_t launches[CG_K_COUNT] = {0};
+  double kernel_ms[CG_K_COUNT] = {0};
+};
+
+struct cg_grid {
+  cg_ctx *ctx = nullptr;
+  int nx = 0, ny = 0, nz = 0, nxp = 0, nyp = 0, nzp = 0;
+  int cnxp = 0, rnxp = 0;
+  int cubetype = 0;
+  int transformed = 0;
+  size_t rsize = 0, csize = 0;  // floats, complex cells
+  float *d = nullptr;
+};
+
+namespace cg {
+
+int fail(cg_ctx *ctx, int code, const std::string &msg);
+int hip_fail(cg_ctx *ctx, hipError_t e, const char *what);
+const float2 *twiddle_table(cg_ctx *ctx, int n);
+void *table_space(cg_ctx *ctx, size_t bytes);
+void launch_begin(cg_ctx *ctx, int kclass, hipEvent_t *start);
+void launch_end(cg_ctx *ctx, int kclass, hipEvent_t start);
+
+// FFT passes (cg_fft.hip).  All enqueue on ctx->stream and return a CG_* status.
+int fft_x_r2c(cg_ctx *ctx, float *grid, int nxp, long rows, float scale);
+int fft_x_c2r(cg_ctx *ctx, float *grid, int nxp, long rows, float scale, const float *add);
+int fft_strided(cg_ctx *ctx, float2 *data, int n, long elem_stride, long inner, long outer, long outer_stride,
+                int dir, float scale);
+int grid_fft_forward(cg_grid *g);
+int grid_fft_inverse(cg_grid *g, const float *add_real);
+
+// Elementwise kernels (cg_noise.hip).
+int scale_real(cg_ctx *ctx, float *d, size_t n, float s);
+int add_real(cg_ctx *ctx, float *d, const float *s, size_t n);
+
+}  // namespace cg
+
+// XCD-aware block order: MI355X dispatches consecutive workgroups round-robin over its 8
+// XCDs, each with its own L2.  This bijection gives every XCD a contiguous range of logical
+// tiles, so tiles that share cache lines stay in one L2.
+__device__ inline unsigned cg_xcd_remap(unsigned b, unsigned nb)
+{
+  const unsigned q = nb / 8u, r = nb % 8u;
+  const unsigned xcd = b % 8u, local = b / 8u;
+  return xcd * q + (xcd < r ? xcd : r) + local;
+}

@@ -1,0 +1,23 @@
+// cg_noise.h -- parameters of the realisation kernels (cg_noise.hip).
+#pragma once
+
+#include "cg_internal.h"
+
+namespace cg {
+
+struct ColourParams {
+  long csize;
+  int cnxp, nyp, nzp;
+  const float2 *cov[6];   // posterior covariance spectra 00 11 22 01 02 12
+  float2 *out[3];         // coloured spectra (noise in, if from_grids)
+  int from_grids;
+  uint64_t seed;
+  uint32_t stream0;       // streams stream0, stream0+1, stream0+2
+  unsigned long long *fails;
+};
+
+int fill_noise(cg_ctx *ctx, float2 *out, long csize, int cnxp, int nyp, int nzp, uint64_t seed, uint32_t stream);
+int colour(cg_ctx *ctx, const ColourParams &p);
+int zero_counters(cg_ctx *ctx, unsigned long long *d, int n);
+
+}  // namespace cg

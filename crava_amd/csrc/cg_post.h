@@ -1,0 +1,50 @@
+// cg_post.h -- parameters of the posterior kernels (cg_post.hip), filled by cg_api.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "cg_internal.h"
+
+#include <cstdint>
+
+namespace cg {
+
+// Per-k_z plane tables, lower triangles at t = i (i + 1) / 2 + j.
+template <typename R>
+struct PlaneTab {
+  struct C {
+    R re, im;
+  };
+  C K[8][3];  // K = diag(kD conj W_l) A                (avoinversion.cpp:947-965)
+  C S[8][3];  // K Sigma0
+  C G[36];    // K Sigma0 K^H
+  C Q[36];    // 0.5 (1 - wnc) Sigma0e_lm (e1_l e1_m* + e2_l e2_m*)   (avoinversion.cpp:1254-1266)
+  R D[8];     // wnc Sigma0e_ll |e3_l|^2
+};
+
+struct PostParams {
+  int nxp, nyp, nzp, cnxp, nt;
+  long csize;
+  float2 *mean[3];
+  float2 *data[8];
+  float2 *cov[6];
+  const float2 *lam_grid;  // errCorr spectrum grid, or null
+  const float *lam2d;      // [nyp][cnxp] when lam_grid is null
+  const float *rxy;        // [nyp][cnxp] separable prior
+  const float *rt;         // [nzp]
+  const int *invert;       // [nzp]
+  const PlaneTab<float> *tab32;
+  const PlaneTab<double> *tab64;
+  float s6f[6];            // priorVar0 entries 00 11 22 01 02 12 as float
+  double s6d[6];
+  int prior_mode;
+  int want_residual;
+  int precision;
+  float cond_limit;
+  unsigned long long *counters;  // [0] inverted [1] failures [2] FP64 re-solves [3] list length
+  unsigned *list;                // wavenumbers deferred to FP64 (capacity: inverted cells)
+};
+
+int launch_posterior(cg_ctx *ctx, const PostParams &p);
+
+}  // namespace cg

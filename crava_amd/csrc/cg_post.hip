@@ -1,0 +1,461 @@
+// cg_post.hip -- per-wavenumber Gaussian conditioning, one wavenumber per lane.
+//
+// Replaces the loop of AVOInversion::computePostMeanResidAndFFTCov (avoinversion.cpp:944-1083)
+// with getNextParameterCovariance (seismicparametersholder.cpp:485-556), getNextErrorVariance
+// (avoinversion.cpp:1236-1272) and the lib_matr kernels (lib_matr.c:78-167, 330-365, 524-736).
+// For every wavenumber (i, j, k) with lowCut < f(k) < highCut:
+//   M      = K Sigma_m K^H + Sigma_e                      (ntheta x ntheta Hermitian)
+//   L      = chol(M)  [normalised by M00, TOL 1e-20, prior kept on failure]
+//   X      = M^-1 K Sigma_m
+//   Sigma' = Sigma_m - (K Sigma_m)^H X
+//   m'     = m + X^H (d - K m),   residual = d - K m'
+// Outside the band the prior passes through.  The Cholesky factor, right-hand sides and
+// products stay in registers; no MFMA (nothing here is a dense contraction).
+//
+// Structure used for the separable prior Sigma_m(k) = r(k) Sigma0 (r = |rxy(kx,ky) rt(kz)|):
+//   K Sigma_m K^H = r G(kz),   G = K Sigma0 K^H,   K Sigma_m = r S(kz),   S = K Sigma0,
+//   Sigma_e = lambda Q(kz) + diag(D(kz)),  Q_lm = 0.5 (1-wnc) Sigma0e_lm (e1_l e1_m* + e2_l e2_m*),
+//   D_l = wnc Sigma0e_ll |e3_l|^2,
+// so each lane assembles M from three per-plane tables (computed once on the host in double).
+#include <cmath>
+
+#include "cg_internal.h"
+#include "cg_post.h"
+
+namespace {
+
+template <typename R>
+struct cpx {
+  R re, im;
+};
+
+template <typename R>
+__device__ __forceinline__ cpx<R> mk(R a, R b)
+{
+  return cpx<R>{a, b};
+}
+
+constexpr double kTol = 1e-20;  // lib_matr.c:19
+
+__device__ __forceinline__ int tri(int i, int j) { return i * (i + 1) / 2 + j; }  // j <= i
+
+// lib_matrCholCpx on the lower triangle (the reference never reads the upper one after the
+// normalisation): M /= M00 (fail if M00 <= 0), row-wise Cholesky, fail if a diagonal entry or
+// a pivot is <= TOL, then L *= sqrt(M00).  Returns 0 on success.  dmin/dmax: extreme squared
+// pivots of the normalised factor (a lower bound of cond(M) is dmax/dmin).
+template <int NT, typename R>
+__device__ __forceinline__ int chol(cpx<R> (&m)[NT * (NT + 1) / 2], R &dmin, R &dmax)
+{
+  const R factor = m[0].re;
+  if (!(factor > R(0))) return 1;
+  const R inv = R(1) / factor;
+#pragma unroll
+  for (int t = 0; t < NT * (NT + 1) / 2; ++t) {
+    m[t].re = m[t].re * inv;
+    m[t].im = m[t].im * inv;
+  }
+  dmin = R(1e30);
+  dmax = R(0);
+#pragma unroll
+  for (int i = 0; i < NT; ++i) {
+    if (!((double)m[tri(i, i)].re > kTol)) return 1;
+#pragma unroll
+    for (int j = 0; j < i; ++j) {
+      R lr = 0, li = 0;
+#pragma unroll
+      for (int k = 0; k < j; ++k) {
+        lr += m[tri(i, k)].re * m[tri(j, k)].re + m[tri(i, k)].im * m[tri(j, k)].im;
+        li += -(m[tri(i, k)].re * m[tri(j, k)].im) + m[tri(i, k)].im * m[tri(j, k)].re;
+      }
+      // L_jj is real and positive here: (a - l) / L_jj
+      const R dj = m[tri(j, j)].re;
+      m[tri(i, j)].re = (m[tri(i, j)].re - lr) / dj;
+      m[tri(i, j)].im = (m[tri(i, j)].im - li) / dj;
+    }
+    R s = 0;
+#pragma unroll
+    for (int k = 0; k < i; ++k) s += m[tri(i, k)].re * m[tri(i, k)].re + m[tri(i, k)].im * m[tri(i, k)].im;
+    const R piv = m[tri(i, i)].re - s;
+    if (!((double)piv > kTol)) return 1;
+    dmin = piv < dmin ? piv : dmin;
+    dmax = piv > dmax ? piv : dmax;
+    m[tri(i, i)].re = sqrt(piv);
+    m[tri(i, i)].im = 0;
+  }
+  const R sf = sqrt(factor);
+#pragma unroll
+  for (int t = 0; t < NT * (NT + 1) / 2; ++t) {
+    m[t].re *= sf;
+    m[t].im *= sf;
+  }
+  return 0;
+}
+
+// lib_matrAXeqBMatCpx: x <- L^-H L^-1 x for one column.
+template <int NT, typename R>
+__device__ __forceinline__ void solve_col(const cpx<R> (&L)[NT * (NT + 1) / 2], cpx<R> (&x)[NT])
+{
+#pragma unroll
+  for (int i = 0; i < NT; ++i) {
+    R xr = x[i].re, xi = x[i].im;
+#pragma unroll
+    for (int j = 0; j < i; ++j) {
+      xr -= x[j].re * L[tri(i, j)].re - x[j].im * L[tri(i, j)].im;
+      xi -= x[j].im * L[tri(i, j)].re + x[j].re * L[tri(i, j)].im;
+    }
+    const R d = L[tri(i, i)].re;
+    x[i].re = xr / d;
+    x[i].im = xi / d;
+  }
+#pragma unroll
+  for (int i = NT - 1; i >= 0; --i) {
+    R xr = x[i].re, xi = x[i].im;
+#pragma unroll
+    for (int j = NT - 1; j > i; --j) {
+      xr -= x[j].re * L[tri(j, i)].re + x[j].im * L[tri(j, i)].im;
+      xi -= -x[j].re * L[tri(j, i)].im + x[j].im * L[tri(j, i)].re;
+    }
+    const R d = L[tri(i, i)].re;
+    x[i].re = xr / d;
+    x[i].im = xi / d;
+  }
+}
+
+// The update for one wavenumber in precision R.  Inputs: prior mean m3, data d, prior
+// covariance P (3x3 real symmetric, 6 entries 00 11 22 01 02 12), K (ntheta x 3), and either
+// the separable plane tables (r, S, G) or nothing (general prior), plus lambda, Q, D.
+// Returns 0 on success (outputs written), 1 on Cholesky failure (outputs untouched).
+template <int NT, typename R, bool SEP>
+__device__ __forceinline__ int update(const cg::PlaneTab<R> &T, R r, R lam, const R (&P6)[6], cpx<R> (&m3)[3],
+                                      cpx<R> (&d)[NT], cpx<R> (&cov6)[6], cpx<R> (&res)[NT], bool want_res,
+                                      R &cond_lb)
+{
+  cpx<R> KS[NT][3];
+  cpx<R> M[NT * (NT + 1) / 2];
+  if (SEP) {
+#pragma unroll
+    for (int l = 0; l < NT; ++l)
+#pragma unroll
+      for (int a = 0; a < 3; ++a) KS[l][a] = mk<R>(r * T.S[l][a].re, r * T.S[l][a].im);
+#pragma unroll
+    for (int l = 0; l < NT; ++l)
+#pragma unroll
+      for (int m = 0; m <= l; ++m) {
+        const int t = tri(l, m);
+        M[t].re = r * T.G[t].re + lam * T.Q[t].re;
+        M[t].im = r * T.G[t].im + lam * T.Q[t].im;
+        if (l == m) M[t].re += T.D[l];
+      }
+  } else {
+    const R Pm[3][3] = {{P6[0], P6[3], P6[4]}, {P6[3], P6[1], P6[5]}, {P6[4], P6[5], P6[2]}};
+#pragma unroll
+    for (int l = 0; l < NT; ++l)
+#pragma unroll
+      for (int b = 0; b < 3; ++b) {
+        R xr = 0, xi = 0;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          xr += T.K[l][a].re * Pm[a][b];
+          xi += T.K[l][a].im * Pm[a][b];
+        }
+        KS[l][b] = mk<R>(xr, xi);
+      }
+#pragma unroll
+    for (int l = 0; l < NT; ++l)
+#pragma unroll
+      for (int m = 0; m <= l; ++m) {
+        R xr = 0, xi = 0;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          xr += KS[l][a].re * T.K[m][a].re + KS[l][a].im * T.K[m][a].im;
+          xi += KS[l][a].im * T.K[m][a].re - KS[l][a].re * T.K[m][a].im;
+        }
+        const int t = tri(l, m);
+        M[t].re = xr + lam * T.Q[t].re;
+        M[t].im = xi + lam * T.Q[t].im;
+        if (l == m) M[t].re += T.D[l];
+      }
+  }
+  R dmin, dmax;
+  if (chol<NT, R>(M, dmin, dmax)) return 1;
+  cond_lb = dmax / dmin;
+  // X = M^-1 KS, column by column; reduce = KS^H X; m' = m + X^H (d - K m)
+  cpx<R> X[3][NT];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    cpx<R> col[NT];
+#pragma unroll
+    for (int l = 0; l < NT; ++l) col[l] = KS[l][a];
+    solve_col<NT, R>(M, col);
+#pragma unroll
+    for (int l = 0; l < NT; ++l) X[a][l] = col[l];
+  }
+  const int ia[6] = {0, 1, 2, 0, 0, 1}, ib[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+  for (int g = 0; g < 6; ++g) {
+    R xr = 0, xi = 0;
+#pragma unroll
+    for (int l = 0; l < NT; ++l) {  // (KS^H)[a][l] * X[b][l]
+      const cpx<R> u = KS[l][ia[g]], v = X[ib[g]][l];
+      xr += u.re * v.re + u.im * v.im;
+      xi += u.re * v.im - u.im * v.re;
+    }
+    cov6[g] = mk<R>(P6[g] - xr, -xi);
+  }
+  cpx<R> rd[NT];
+#pragma unroll
+  for (int l = 0; l < NT; ++l) {
+    R xr = d[l].re, xi = d[l].im;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      xr -= T.K[l][a].re * m3[a].re - T.K[l][a].im * m3[a].im;
+      xi -= T.K[l][a].im * m3[a].re + T.K[l][a].re * m3[a].im;
+    }
+    rd[l] = mk<R>(xr, xi);
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    R xr = 0, xi = 0;
+#pragma unroll
+    for (int l = 0; l < NT; ++l) {  // X^H rd
+      xr += X[a][l].re * rd[l].re + X[a][l].im * rd[l].im;
+      xi += -X[a][l].im * rd[l].re + X[a][l].re * rd[l].im;
+    }
+    m3[a].re += xr;
+    m3[a].im += xi;
+  }
+  if (want_res) {
+#pragma unroll
+    for (int l = 0; l < NT; ++l) {
+      R xr = d[l].re, xi = d[l].im;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        xr -= T.K[l][a].re * m3[a].re - T.K[l][a].im * m3[a].im;
+        xi -= T.K[l][a].im * m3[a].re + T.K[l][a].re * m3[a].im;
+      }
+      res[l] = mk<R>(xr, xi);
+    }
+  }
+  return 0;
+}
+
+// Wave-aggregated counter increment: one atomic per wavefront.  Every lane of the wave must
+// reach this point (the kernels below have no early returns).
+__device__ __forceinline__ void wave_count(unsigned long long *ctr, bool pred)
+{
+  const unsigned long long m = __ballot(pred);
+  if (__lane_id() == 0 && m) atomicAdd(ctr, (unsigned long long)__popcll(m));
+}
+
+// Loads the prior covariance entries of one wavenumber (getNextParameterCovariance:
+// |Re(c_ab)/s_ab| * s_ab with zero imaginary part).
+template <bool SEP>
+__device__ __forceinline__ float prior6(const cg::PostParams &p, long idx, int i, int j, int k, float (&P6)[6])
+{
+  float r = 0.f;
+  if (SEP) {
+    const float t = p.rxy[(long)j * p.cnxp + i] * p.rt[k];
+    r = sqrtf(t * t);
+#pragma unroll
+    for (int g = 0; g < 6; ++g) P6[g] = r * p.s6f[g];
+  } else {
+#pragma unroll
+    for (int g = 0; g < 6; ++g) {
+      float t = p.cov[g][idx].x;
+      if (p.s6f[g] != 0.f) t = t / p.s6f[g];
+      P6[g] = sqrtf(t * t) * p.s6f[g];
+    }
+  }
+  return r;
+}
+
+// FP32 pass.  grid: x over the cells of one k plane, y = k, so the per-plane tables are
+// wavefront-uniform loads.  With MIXED, wavenumbers whose FP32 factor fails or whose pivot
+// ratio exceeds cond_limit are appended to p.list untouched, for the FP64 pass.
+template <int NT, bool SEP, bool MIXED>
+__global__ void __launch_bounds__(cg::kThreads) k_posterior(cg::PostParams p)
+{
+  const int k = blockIdx.y;
+  const long plane = (long)p.cnxp * p.nyp;
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = c < plane;
+  const int i = active ? (int)(c % p.cnxp) : 0, j = active ? (int)(c / p.cnxp) : 0;
+  const long idx = (long)k * plane + (active ? c : 0);
+  const bool inv = p.invert[k] != 0;
+  bool did_inv = false, did_fail = false, defer = false;
+  if (active) {
+    float P6[6];
+    const float r = prior6<SEP>(p, idx, i, j, k, P6);
+    bool write_prior = !inv;
+    if (inv) {
+      did_inv = true;
+      const float lt = p.lam_grid ? p.lam_grid[idx].x : p.lam2d[(long)j * p.cnxp + i];
+      const float lam = sqrtf(lt * lt);
+      cpx<float> m3[3], d[NT], cov6[6], res[NT];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const float2 v = p.mean[a][idx];
+        m3[a] = mk<float>(v.x, v.y);
+      }
+#pragma unroll
+      for (int l = 0; l < NT; ++l) {
+        const float2 v = p.data[l][idx];
+        d[l] = mk<float>(v.x, v.y);
+      }
+      float cond = 0.f;
+      const int fail = update<NT, float, SEP>(p.tab32[k], r, lam, P6, m3, d, cov6, res, p.want_residual != 0, cond);
+      if (MIXED && (fail || cond > p.cond_limit)) {
+        defer = true;
+      } else if (fail) {  // posterior = prior (avoinversion.cpp:1040-1042)
+        did_fail = true;
+        write_prior = true;
+      } else {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) p.mean[a][idx] = make_float2(m3[a].re, m3[a].im);
+#pragma unroll
+        for (int g = 0; g < 6; ++g) p.cov[g][idx] = make_float2(cov6[g].re, cov6[g].im);
+        if (p.want_residual) {
+#pragma unroll
+          for (int l = 0; l < NT; ++l) p.data[l][idx] = make_float2(res[l].re, res[l].im);
+        }
+      }
+    }
+    if (write_prior) {
+#pragma unroll
+      for (int g = 0; g < 6; ++g) p.cov[g][idx] = make_float2(P6[g], 0.f);
+    }
+  }
+  if (MIXED) {  // wave-aggregated append to the FP64 list
+    const unsigned long long m = __ballot(defer);
+    if (m) {
+      unsigned long long base = 0;
+      if (__lane_id() == 0) base = atomicAdd(&p.counters[3], (unsigned long long)__popcll(m));
+      base = __shfl(base, 0);
+      const unsigned lane = __lane_id();
+      const unsigned long long below = lane ? (m & ((~0ull) >> (64 - lane))) : 0ull;
+      if (defer) p.list[base + __popcll(below)] = (unsigned)idx;
+    }
+  }
+  if (p.counters) {
+    wave_count(&p.counters[0], did_inv);
+    wave_count(&p.counters[1], did_fail);
+  }
+}
+
+// FP64 pass over the deferred wavenumbers.  Fixed grid, every thread strides over the list
+// and stops at its length, so every wave exits.
+template <int NT, bool SEP>
+__global__ void __launch_bounds__(cg::kThreads) k_posterior64(cg::PostParams p)
+{
+  const unsigned long long n = p.counters[3];
+  const long plane = (long)p.cnxp * p.nyp;
+  for (unsigned long long e = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (unsigned long long)gridDim.x * blockDim.x) {
+    const long idx = p.list[e];
+    const int k = (int)(idx / plane);
+    const long c = idx % plane;
+    const int i = (int)(c % p.cnxp), j = (int)(c / p.cnxp);
+    float P6[6];
+    prior6<SEP>(p, idx, i, j, k, P6);
+    double P6d[6], rd = 0.0;
+    if (SEP) {
+      rd = fabs((double)p.rxy[(long)j * p.cnxp + i] * (double)p.rt[k]);
+#pragma unroll
+      for (int g = 0; g < 6; ++g) P6d[g] = rd * p.s6d[g];
+    } else {
+#pragma unroll
+      for (int g = 0; g < 6; ++g) P6d[g] = (double)P6[g];
+    }
+    const float lt = p.lam_grid ? p.lam_grid[idx].x : p.lam2d[(long)j * p.cnxp + i];
+    const double lam = fabs((double)lt);
+    cpx<double> m3[3], d[NT], cov6[6], res[NT];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const float2 v = p.mean[a][idx];
+      m3[a] = mk<double>(v.x, v.y);
+    }
+#pragma unroll
+    for (int l = 0; l < NT; ++l) {
+      const float2 v = p.data[l][idx];
+      d[l] = mk<double>(v.x, v.y);
+    }
+    double cond = 0.0;
+    const int fail = update<NT, double, SEP>(p.tab64[k], rd, lam, P6d, m3, d, cov6, res, p.want_residual != 0, cond);
+    if (fail) {
+      if (p.counters) atomicAdd(&p.counters[1], 1ull);
+#pragma unroll
+      for (int g = 0; g < 6; ++g) p.cov[g][idx] = make_float2(P6[g], 0.f);
+    } else {
+#pragma unroll
+      for (int a = 0; a < 3; ++a) p.mean[a][idx] = make_float2((float)m3[a].re, (float)m3[a].im);
+#pragma unroll
+      for (int g = 0; g < 6; ++g) p.cov[g][idx] = make_float2((float)cov6[g].re, (float)cov6[g].im);
+      if (p.want_residual) {
+#pragma unroll
+        for (int l = 0; l < NT; ++l) p.data[l][idx] = make_float2((float)res[l].re, (float)res[l].im);
+      }
+    }
+  }
+}
+
+template <int NT>
+void launch_nt(const cg::PostParams &p, dim3 grid, hipStream_t s, bool sep, bool mixed)
+{
+  if (sep) {
+    if (mixed) k_posterior<NT, true, true><<<grid, cg::kThreads, 0, s>>>(p);
+    else k_posterior<NT, true, false><<<grid, cg::kThreads, 0, s>>>(p);
+  } else {
+    if (mixed) k_posterior<NT, false, true><<<grid, cg::kThreads, 0, s>>>(p);
+    else k_posterior<NT, false, false><<<grid, cg::kThreads, 0, s>>>(p);
+  }
+}
+
+template <int NT>
+void launch_nt64(const cg::PostParams &p, unsigned blocks, hipStream_t s, bool sep)
+{
+  if (sep) k_posterior64<NT, true><<<blocks, cg::kThreads, 0, s>>>(p);
+  else k_posterior64<NT, false><<<blocks, cg::kThreads, 0, s>>>(p);
+}
+
+}  // namespace
+
+namespace cg {
+
+int launch_posterior(cg_ctx *ctx, const PostParams &p)
+{
+  const long plane = (long)p.cnxp * p.nyp;
+  dim3 grid((unsigned)((plane + kThreads - 1) / kThreads), (unsigned)p.nzp);
+  const bool sep = p.prior_mode == CG_PRIOR_SEPARABLE;
+  const bool mixed = p.precision == CG_PREC_MIXED;
+  hipEvent_t ev;
+  launch_begin(ctx, CG_K_POSTERIOR, &ev);
+  switch (p.nt) {
+    case 1: launch_nt<1>(p, grid, ctx->stream, sep, mixed); break;
+    case 2: launch_nt<2>(p, grid, ctx->stream, sep, mixed); break;
+    case 3: launch_nt<3>(p, grid, ctx->stream, sep, mixed); break;
+    case 4: launch_nt<4>(p, grid, ctx->stream, sep, mixed); break;
+    case 5: launch_nt<5>(p, grid, ctx->stream, sep, mixed); break;
+    case 6: launch_nt<6>(p, grid, ctx->stream, sep, mixed); break;
+    case 7: launch_nt<7>(p, grid, ctx->stream, sep, mixed); break;
+    default: launch_nt<8>(p, grid, ctx->stream, sep, mixed); break;
+  }
+  int rc = hip_fail(ctx, hipGetLastError(), "k_posterior");
+  if (rc == CG_OK && mixed) {
+    const unsigned blocks = 1024;  // grid-stride over the deferred list
+    switch (p.nt) {
+      case 1: launch_nt64<1>(p, blocks, ctx->stream, sep); break;
+      case 2: launch_nt64<2>(p, blocks, ctx->stream, sep); break;
+      case 3: launch_nt64<3>(p, blocks, ctx->stream, sep); break;
+      case 4: launch_nt64<4>(p, blocks, ctx->stream, sep); break;
+      case 5: launch_nt64<5>(p, blocks, ctx->stream, sep); break;
+      case 6: launch_nt64<6>(p, blocks, ctx->stream, sep); break;
+      case 7: launch_nt64<7>(p, blocks, ctx->stream, sep); break;
+      default: launch_nt64<8>(p, blocks, ctx->stream, sep); break;
+    }
+    rc = hip_fail(ctx, hipGetLastError(), "k_posterior64");
+  }
+  launch_end(ctx, CG_K_POSTERIOR, ev);
+  return rc;
+}
+
+}  // namespace cg

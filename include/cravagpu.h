@@ -1,0 +1,191 @@
+/*
+ * cravagpu.h -- C ABI of the MI355X-native FFT-domain Bayesian AVO inversion.
+ *
+ * A drop-in for the hot path of CRAVA: FFTGrid's 3D FFTs and complex accessors, and
+ * AVOInversion's posterior (computePostMeanResidAndFFTCov) and simulation (simulate) calls.
+ * Host C++ calls hand-written HIP kernels built for gfx950 through this ABI; nothing here
+ * exposes torch or HIP types.  Reference paths are relative to the CRAVA source tree.
+ *
+ * Conventions
+ *  - Every function returns 0 on success or a negative CG_E* code; cg_last_error(ctx) holds
+ *    the message of the last failure on that context.  No exception crosses the ABI.
+ *  - One context owns one device and one HIP stream.  A context is not re-entrant: use one
+ *    per host thread.  All calls enqueue on the context stream; cg_ctx_sync waits for them.
+ *  - Grids keep CRAVA's in-place FFTW layout on the device (fftgrid.cpp:40-70): real cells at
+ *    i + rnxp*j + rnxp*nyp*k with rnxp = 2*(nxp/2+1) floats; complex cells at
+ *    i + cnxp*j + cnxp*nyp*k with cnxp = nxp/2+1, i = x being the half-spectrum axis.
+ *  - Host buffers are borrowed for the duration of a call only.
+ */
+#ifndef CRAVAGPU_H
+#define CRAVAGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CG_ABI_VERSION 1
+
+/* status codes */
+#define CG_OK 0
+#define CG_EINVAL (-1)   /* bad argument (sizes, null pointers, wrong grid state) */
+#define CG_ENOMEM (-2)   /* device allocation failed or HBM budget exceeded */
+#define CG_EHIP (-3)     /* HIP runtime error */
+#define CG_ESTATE (-4)   /* operation not valid in the grid's current domain */
+
+/* FFTGrid cube types (fftgrid.h:155); the value selects FFT scaling. */
+#define CG_CT_DATA 1
+#define CG_CT_PARAMETER 2
+#define CG_CT_COVARIANCE 3
+
+typedef struct cg_ctx cg_ctx;
+typedef struct cg_grid cg_grid;
+
+/* ------------------------------------------------------------------ context */
+/* Replaces the process-global FFTGrid statics (fftgrid.cpp:2661-2668): the budget is the
+ * analogue of setMaxAllowedGrids/terminateOnMaxGrid, enforced per context (0 = no limit). */
+int cg_ctx_create(cg_ctx **ctx, int device, uint64_t hbm_budget_bytes);
+/* Waits for the stream, then releases the context and any grids still attached to it (their
+ * handles become invalid). */
+int cg_ctx_destroy(cg_ctx *ctx);
+const char *cg_last_error(const cg_ctx *ctx);
+int cg_ctx_sync(cg_ctx *ctx);
+/* Bytes of device memory held by grids and work buffers of this context. */
+int cg_ctx_memory(const cg_ctx *ctx, uint64_t *in_use, uint64_t *peak);
+/* The context's HIP stream (hipStream_t) for callers that interoperate; opaque here. */
+int cg_ctx_stream(const cg_ctx *ctx, void **stream);
+/* Per-kernel timing with HIP events on the context stream: when enabled, every launch of the
+ * named kernel class is bracketed by events; cg_ctx_kernel_time returns launch count and the
+ * summed duration in ms (synchronises).  Classes: see CG_K_* below. */
+int cg_ctx_set_profiling(cg_ctx *ctx, int enable);
+int cg_ctx_kernel_time(cg_ctx *ctx, int kernel_class, uint64_t *launches, double *total_ms);
+int cg_ctx_reset_kernel_time(cg_ctx *ctx);
+#define CG_K_FFT_X 0     /* x-axis r2c / c2r passes */
+#define CG_K_FFT_Y 1     /* y-axis complex passes */
+#define CG_K_FFT_Z 2     /* z-axis complex passes */
+#define CG_K_POSTERIOR 3 /* per-wavenumber posterior update */
+#define CG_K_NOISE 4     /* noise generation and colouring */
+#define CG_K_ELEMENTWISE 5
+#define CG_K_COUNT 6
+
+/* ------------------------------------------------------------------ grids (FFTGrid) */
+/* FFTGrid(nx,ny,nz,nxp,nyp,nzp) + createRealGrid/createComplexGrid (fftgrid.cpp:40, 1048-1063),
+ * device-resident.  The new grid is in the real (time) domain and zero-filled. */
+int cg_grid_create(cg_ctx *ctx, int nx, int ny, int nz, int nxp, int nyp, int nzp, int cubetype,
+                   cg_grid **grid);
+int cg_grid_destroy(cg_grid *grid);
+/* setType (fftgrid.h) */
+int cg_grid_set_type(cg_grid *grid, int cubetype);
+/* getIsTransformed (fftgrid.h): 1 in the Fourier domain, 0 in the real domain. */
+int cg_grid_is_transformed(const cg_grid *grid, int *transformed);
+int cg_grid_dims(const cg_grid *grid, int dims[6]);
+/* Raw device pointer to the rnxp*nyp*nzp floats (aliasing the complex view). */
+int cg_grid_device_ptr(cg_grid *grid, void **ptr);
+/* Host mirror for cold callers: whole-grid copies in the reference layout.  Upload of a real
+ * grid sets the real domain; upload of a complex grid sets the Fourier domain
+ * (createComplexGrid + setNextComplex loop). */
+int cg_grid_upload_real(cg_grid *grid, const float *host_rvalue);
+int cg_grid_upload_complex(cg_grid *grid, const float *host_cvalue);
+int cg_grid_download_real(cg_grid *grid, float *host_rvalue);
+int cg_grid_download_complex(cg_grid *grid, float *host_cvalue);
+/* getComplexValue / setComplexValue (fftgrid.cpp:1335-1363, 1467-1487): random access; out of
+ * range gives RMISSING (-99999) in both parts, resp. returns 1 and writes nothing.  With
+ * ext_simbox = 0 the range is the unpadded nx*ny*nz box, else the padded box. */
+int cg_grid_get_complex(cg_grid *grid, int i, int j, int k, int ext_simbox, float value[2]);
+int cg_grid_set_complex(cg_grid *grid, int i, int j, int k, int ext_simbox, const float value[2]);
+
+/* FFTGrid::fftInPlace (fftgrid.cpp:1587-1612): DATA/PARAMETER are scaled by 1/sqrt(N) before
+ * the forward transform, COVARIANCE is not.  Sign -1, FFTW pass order x (r2c), y, z. */
+int cg_fft_r2c_inplace(cg_grid *grid);
+/* FFTGrid::invFFTInPlace (fftgrid.cpp:1615-1645): z, y complex, then x half-complex-to-real
+ * that ignores Im of the DC and Nyquist bins; x 1/N (COVARIANCE) or 1/sqrt(N) afterwards. */
+int cg_fft_c2r_inplace(cg_grid *grid);
+/* FFTGrid::add / multiplyByScalar (fftgrid.cpp:1659-1678, 1766-1773). */
+int cg_grid_add(cg_grid *dst, const cg_grid *src);
+int cg_grid_multiply_by_scalar(cg_grid *grid, float scalar);
+
+/* ------------------------------------------------------------------ posterior (AVOInversion) */
+#define CG_PRIOR_GRIDS 0     /* six prior covariance spectra, as SeismicParametersHolder holds */
+#define CG_PRIOR_SEPARABLE 1 /* Sigma_m(k) = |rxy(kx,ky) * rt(kz)| * Sigma0 (no grids read) */
+
+#define CG_PREC_REFERENCE 0 /* FP32 throughout, the reference's decisions (prior on failure) */
+#define CG_PREC_MIXED 1     /* FP32, FP64 re-solve where cond(M) > cond_limit or FP32 fails */
+
+typedef struct {
+  int ntheta;               /* number of angle gathers, 1..8 */
+  /* per-k_z tables (avoinversion.cpp:944-975), host memory, complex interleaved:
+   * K [nzp][ntheta][3], e1/e2/e3 [nzp][ntheta]; invert [nzp] (0/1) */
+  const float *K;
+  const float *e1;
+  const float *e2;
+  const float *e3;
+  const int *invert;
+  const double *sigma0e;    /* [ntheta][ntheta] errThetaCov (modelavodynamic.cpp:623-651) */
+  float wnc;                /* white-noise component (modelsettings.cpp:98) */
+  const double *prior_var0; /* [3][3] priorVar0_ */
+  int prior_mode;           /* CG_PRIOR_* */
+  /* CG_PRIOR_GRIDS: six COVARIANCE grids in the Fourier domain, order Vp, Vs, Rho, VpVs,
+   * VpRho, VsRho; they receive the posterior covariance spectra.
+   * CG_PRIOR_SEPARABLE: host tables rxy [nyp][cnxp] and rt [nzp]; the posterior covariance
+   * spectra are written to the six grids in cov[] (which must exist, any content). */
+  cg_grid *cov[6];
+  const float *rxy;
+  const float *rt;
+  /* error-correlation spectrum lambda: either the errCorr COVARIANCE grid in the Fourier
+   * domain (avoinversion.cpp:917, 1248) or a host table lam2d [nyp][cnxp] (fillInErrCorr puts
+   * the lateral correlation in the k = 0 plane only, so its spectrum does not depend on kz). */
+  cg_grid *err_corr;
+  const float *lam2d;
+  cg_grid *mean[3];         /* Fourier-domain prior means in, posterior means out */
+  cg_grid *data[8];         /* Fourier-domain seismic in, residual out (if want_residual) */
+  int want_residual;
+  int precision;            /* CG_PREC_* */
+  double cond_limit;        /* CG_PREC_MIXED threshold, e.g. 1e6 */
+  uint64_t *stats;          /* optional host [3]: inverted, cholesky failures, FP64 re-solves */
+} cg_post_args;
+
+/* AVOInversion::computePostMeanResidAndFFTCov, per-wavenumber part (avoinversion.cpp:944-1083).
+ * All grids in the Fourier domain; the caller transforms before and after, as the reference
+ * does at :229-244, :908-917 and :1112-1114. */
+int cg_posterior(cg_ctx *ctx, const cg_post_args *args);
+
+/* ------------------------------------------------------------------ simulation */
+#define CG_NOISE_COUNTER 0   /* counter-based Philox-4x32-10 + Box-Muller, keyed by seed */
+#define CG_NOISE_GRIDS 1     /* caller-supplied noise spectra in seed[] (e.g. the reference stream) */
+
+typedef struct {
+  cg_grid *cov[6];          /* posterior covariance spectra (Fourier domain, order as above) */
+  cg_grid *mean[3];         /* posterior means, real domain (after cg_fft_c2r_inplace) */
+  cg_grid *out[3];          /* realisation grids: receive Vp, Vs, Rho real fields */
+  int noise_source;         /* CG_NOISE_* */
+  uint64_t seed;            /* CG_NOISE_COUNTER key */
+  uint32_t realisation;     /* CG_NOISE_COUNTER realisation number (streams 3r, 3r+1, 3r+2) */
+  /* CG_NOISE_GRIDS: out[] already hold the noise spectra (fillInComplexNoise result) */
+  uint64_t *stats;          /* optional host [1]: Cholesky failures (noise zeroed) */
+} cg_sim_args;
+
+/* One realisation of AVOInversion::simulate (avoinversion.cpp:1401-1511): noise,
+ * z <- chol(Sigma_post) z per wavenumber (zero where the 3x3 Cholesky fails), unitary C2R,
+ * + posterior mean. */
+int cg_simulate(cg_ctx *ctx, const cg_sim_args *args);
+
+/* FFTGrid::fillInComplexNoise layout (fftgrid.cpp:997-1045) with the counter-based generator:
+ * the grid becomes a PARAMETER grid in the Fourier domain. */
+int cg_grid_fill_noise_counter(cg_grid *grid, uint64_t seed, uint32_t stream);
+
+/* ------------------------------------------------------------------ host-side setup */
+/* Per-k_z tables of the constant-thickness branch (avoinversion.cpp:944-975; Wavelet1D
+ * difference operators wavelet1D.cpp:389-500; findNorm wavelet.cpp:882-900).  wavelets
+ * [ntheta][nzp] real in FFT order, A [ntheta][3].  Outputs as in cg_post_args. */
+int cg_build_kz_tables(int nz, int nzp, double dt_ms, int ntheta, const float *wavelets, const float *A,
+                       float low_cut, float high_cut, float min_rel_thick, float *K, float *e1, float *e2,
+                       float *e3, int *invert);
+/* FFTGrid::findClosestFactorableNumber (fftgrid.cpp:2630-2658). */
+int cg_factorable(int leastint);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -77,6 +77,7 @@ def lib():
         L.orc_fill_complex_noise.argtypes = [ctypes.c_int] * 3 + [_c_u32_p, _c_float_p]
         L.orc_cb_normal2.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64, _c_float_p, _c_float_p]
         L.orc_fill_complex_noise_cb.argtypes = [ctypes.c_int] * 3 + [ctypes.c_uint64, ctypes.c_uint32, _c_float_p]
+        L.orc_philox4x32_10.argtypes = [_c_u32_p, _c_u32_p, _c_u32_p]
         L.orc_colour_f32.restype = ctypes.c_long
         L.orc_colour_f32.argtypes = [ctypes.c_long, _c_float_p, _c_float_p]
         L.orc_colour_f64.restype = ctypes.c_long

@@ -100,6 +100,7 @@ void   orc_fill_complex_noise(int nxp, int nyp, int nzp, uint32_t *state, float 
  * normal pair of complex cell `idx` of stream `stream` = orc_cb_normal2(seed, stream, idx). */
 void   orc_cb_normal2(uint64_t seed, uint32_t stream, uint64_t idx, float *z0, float *z1);
 void   orc_fill_complex_noise_cb(int nxp, int nyp, int nzp, uint64_t seed, uint32_t stream, float *cvalue);
+void   orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 
 /* ---------------------------------------------------------------- colouring (a9) */
 /* avoinversion.cpp:1421-1460: z <- chol(Sigma_post) z per wavenumber, zero on failure.

@@ -132,21 +132,28 @@ static void philox(uint32_t ctr[4], const uint32_t key0[2])
   }
 }
 
-/* Two standard normals for (seed, stream, idx): Box-Muller on two 64-bit-resolution uniforms
- * in (0, 1], evaluated in double and rounded to float. */
+/* Two standard normals for (seed, stream, idx): Box-Muller on the 24-bit uniforms
+ * u1 = ((c0 >> 8) + 1) 2^-24 in (0, 1] and u2 = (c1 >> 8) 2^-24, evaluated in double here
+ * (the device evaluates the same formula in float). */
 void orc_cb_normal2(uint64_t seed, uint32_t stream, uint64_t idx, float *z0, float *z1)
 {
   uint32_t c[4] = {(uint32_t)idx, (uint32_t)(idx >> 32), stream, 0u};
   uint32_t k[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
   philox(c, k);
-  uint64_t a = ((uint64_t)c[0] << 21) ^ (uint64_t)c[1];   /* 53 bits used below */
-  uint64_t b = ((uint64_t)c[2] << 21) ^ (uint64_t)c[3];
-  double u1 = ((double)(a & ((1ull << 53) - 1)) + 1.0) * (1.0 / 9007199254740992.0);
-  double u2 = (double)(b & ((1ull << 53) - 1)) * (1.0 / 9007199254740992.0);
+  double u1 = (double)((c[0] >> 8) + 1u) * (1.0 / 16777216.0);
+  double u2 = (double)(c[1] >> 8) * (1.0 / 16777216.0);
   double r = sqrt(-2.0 * log(u1));
   double th = 2.0 * M_PI * u2;
   *z0 = (float)(r * cos(th));
   *z1 = (float)(r * sin(th));
+}
+
+/* Raw Philox-4x32-10 block (known-answer tests). */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4])
+{
+  uint32_t c[4] = {ctr[0], ctr[1], ctr[2], ctr[3]};
+  philox(c, key);
+  for (int i = 0; i < 4; i++) out[i] = c[i];
 }
 
 /* Same Hermitian pattern as the reference fill; value of cell i from its own counter, the

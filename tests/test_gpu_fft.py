@@ -1,0 +1,90 @@
+"""GPU FFT passes vs the oracle (FFTGrid::fftInPlace / invFFTInPlace semantics)."""
+import numpy as np
+import pytest
+
+import crava_amd as C
+import oracle as O
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+# even/odd x, every radix (2,3,4,5,7,8), y/z lengths of 1 and primes
+SIZES = [(8, 6, 10), (12, 10, 14), (9, 7, 5), (16, 1, 8), (84, 84, 30), (30, 42, 49), (64, 64, 128),
+         (15, 21, 35), (2, 2, 2), (40, 1, 1)]
+
+
+def _grid(ctx, shape, ct):
+    nzp, nyp, nxp = shape
+    return C.FFTGrid(ctx, nxp, nyp, nzp, nxp, nyp, nzp, ct)
+
+
+@pytest.mark.parametrize("nxp,nyp,nzp", SIZES)
+@pytest.mark.parametrize("ct", [C.DATA, C.COVARIANCE])
+def test_forward_matches_oracle(ctx, nxp, nyp, nzp, ct):
+    print("forward", nxp, nyp, nzp, ct, flush=True)
+    rng = np.random.default_rng(nxp * 1000 + nyp * 10 + nzp)
+    x = rng.standard_normal((nzp, nyp, nxp)).astype(np.float32)
+    g = _grid(ctx, x.shape, ct)
+    g.upload_real(x)
+    g.fftInPlace()
+    assert g.getIsTransformed()
+    got = g.download_complex()
+    want = O.grid_fft(x, ct)
+    assert rel_l2(got, want) < 2e-6
+    g.close()
+
+
+@pytest.mark.parametrize("nxp,nyp,nzp", SIZES)
+def test_inverse_matches_oracle_including_non_hermitian_input(ctx, nxp, nyp, nzp):
+    print("inverse", nxp, nyp, nzp, flush=True)
+    rng = np.random.default_rng(7 + nxp)
+    cnxp = nxp // 2 + 1
+    z = (rng.standard_normal((nzp, nyp, cnxp)) + 1j * rng.standard_normal((nzp, nyp, cnxp))).astype(np.complex64)
+    for ct in (C.PARAMETER, C.COVARIANCE):
+        g = _grid(ctx, (nzp, nyp, nxp), ct)
+        g.upload_complex(z)
+        g.invFFTInPlace()
+        got = g.download_real()
+        want = O.grid_ifft(z, nxp, ct)
+        assert rel_l2(got, want) < 2e-6
+        g.close()
+
+
+def test_round_trip_and_parseval(ctx):
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal((40, 36, 50)).astype(np.float32)
+    g = _grid(ctx, x.shape, C.DATA)
+    g.upload_real(x)
+    g.fftInPlace()
+    s = g.download_complex()
+    # unitary: sum |x|^2 = |X_0|^2 + 2 sum_{0<kx<n/2} |X|^2 + |X_{n/2}|^2
+    e = np.sum(np.abs(s[:, :, 0]) ** 2) + 2 * np.sum(np.abs(s[:, :, 1:-1]) ** 2) + np.sum(np.abs(s[:, :, -1]) ** 2)
+    assert abs(e / np.sum(x.astype(np.float64) ** 2) - 1) < 1e-5
+    g.invFFTInPlace()
+    assert rel_l2(g.download_real(), x) < 1e-6
+    g.close()
+
+
+def test_accessors_and_elementwise(ctx):
+    rng = np.random.default_rng(3)
+    g = C.FFTGrid(ctx, 10, 6, 8, 12, 8, 10, C.PARAMETER)
+    x = rng.standard_normal((10, 8, 12)).astype(np.float32)
+    g.upload_real(x)
+    g.multiplyByScalar(2.0)
+    h = C.FFTGrid(ctx, 10, 6, 8, 12, 8, 10, C.PARAMETER)
+    h.upload_real(x)
+    g.add(h)
+    assert rel_l2(g.download_real(), 3 * x) < 1e-7
+    g.fftInPlace()
+    s = g.download_complex()
+    assert g.getComplexValue(3, 2, 5) == pytest.approx(complex(s[5, 2, 3]), rel=1e-6)
+    # outside the unpadded box -> RMISSING; inside the padded box with extSimbox
+    assert g.getComplexValue(3, 7, 5) == complex(C.RMISSING, C.RMISSING)
+    assert g.getComplexValue(3, 7, 9, extSimbox=True) == pytest.approx(complex(s[9, 7, 3]), rel=1e-6)
+    assert g.setComplexValue(1, 1, 1, 2 + 3j) == 0
+    assert g.getComplexValue(1, 1, 1) == 2 + 3j
+    assert g.setComplexValue(11, 1, 1, 1j) == 1
+    with pytest.raises(C.CravaError):
+        g.fftInPlace()  # already transformed
+    g.close()
+    h.close()
