@@ -16,6 +16,7 @@ without a GPU, raises.  There is no CPU fallback.
 """
 import ctypes
 import os
+import weakref
 
 import numpy as np
 
@@ -110,6 +111,7 @@ class Context:
 
     def __init__(self, device=0, hbm_budget=0):
         self._h = _vp()
+        self._grids = weakref.WeakSet()
         rc = lib().cg_ctx_create(ctypes.byref(self._h), device, hbm_budget)
         if rc != CG_OK:
             raise CravaError("cg_ctx_create failed (%d): no usable GPU %d" % (rc, device))
@@ -140,8 +142,16 @@ class Context:
 
     def close(self):
         if self._h:
+            for g in list(self._grids):
+                g.close()
             lib().cg_ctx_destroy(self._h)
             self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
     def __enter__(self):
         return self
@@ -162,6 +172,7 @@ class FFTGrid:
         self._h = _vp()
         ctx.check(lib().cg_grid_create(ctx._h, nx, ny, nz, nxp, nyp, nzp, cubetype, ctypes.byref(self._h)),
                   "cg_grid_create")
+        ctx._grids.add(self)
 
     # -- state
     def setType(self, cubetype):
@@ -236,9 +247,10 @@ class FFTGrid:
         return rc
 
     def close(self):
-        if self._h:
+        # a grid outlived by its context was released with it (cg_ctx_destroy)
+        if self._h and self.ctx._h:
             lib().cg_grid_destroy(self._h)
-            self._h = _vp()
+        self._h = _vp()
 
     def __del__(self):
         try:
