@@ -87,6 +87,10 @@ def lib():
             "cg_grid_fill_noise_counter": [_vp, u64, ctypes.c_uint32],
             "cg_build_kz_tables": [i, i, d, i, _f_p, _f_p, f, f, f, _f_p, _f_p, _f_p, _f_p, _i_p],
             "cg_factorable": [i],
+            "cg_fft_r2c": [_vp, _vp],
+            "cg_forward_model": [_vp, i, _f_p, _vp * 3, _vp * 8],
+            "cg_grid_add_white_noise": [_vp, f, u64, ctypes.c_uint32],
+            "cg_grid_sum_squares": [_vp, i, _d_p],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -195,6 +199,18 @@ class FFTGrid:
     def invFFTInPlace(self):
         self.ctx.check(lib().cg_fft_c2r_inplace(self._h), "invFFTInPlace")
 
+    def fft_from(self, src):
+        """Out-of-place forward transform: this grid receives src's spectrum, src is kept."""
+        self.ctx.check(lib().cg_fft_r2c(src._h, self._h), "cg_fft_r2c")
+
+    def add_white_noise(self, sigma, seed, stream=0):
+        self.ctx.check(lib().cg_grid_add_white_noise(self._h, sigma, seed, stream), "add_white_noise")
+
+    def sum_squares(self, extSimbox=False):
+        v = ctypes.c_double()
+        self.ctx.check(lib().cg_grid_sum_squares(self._h, int(extSimbox), ctypes.byref(v)), "sum_squares")
+        return v.value
+
     def add(self, other):
         self.ctx.check(lib().cg_grid_add(self._h, other._h), "add")
 
@@ -279,8 +295,17 @@ def build_kz_tables(nz, nzp, dt_ms, wavelets, A, low_cut=5.0, high_cut=55.0, min
     return K, e1, e2, e3, inv
 
 
+def forward_model(ctx, K, m, d):
+    """d_l(k) = sum_j K_lj(kz) m_j(k) on Fourier-domain grids (synthetic seismic spectra)."""
+    Kc = np.ascontiguousarray(K, dtype=np.complex64)
+    mm = (_vp * 3)(*[g._h for g in m])
+    dd = (_vp * 8)(*([g._h for g in d] + [None] * (8 - len(d))))
+    ctx.check(lib().cg_forward_model(ctx._h, len(d), _ptr(Kc.view(np.float32), ctypes.c_float), mm, dd),
+              "cg_forward_model")
+
+
 def posterior(ctx, tables, mean, data, cov, prior_mode=PRIOR_SEPARABLE, rxy=None, rt=None, err_corr=None,
-              lam2d=None, want_residual=True, precision=PREC_REFERENCE, cond_limit=1e6):
+              lam2d=None, want_residual=True, precision=PREC_REFERENCE, cond_limit=1e6, stats=True):
     """Per-wavenumber posterior on Fourier-domain FFTGrids (in place).
 
     tables: dict with K, e1, e2, e3, invert, sigma0e, wnc, prior_var0.
@@ -320,8 +345,10 @@ def posterior(ctx, tables, mean, data, cov, prior_mode=PRIOR_SEPARABLE, rxy=None
     a.want_residual = int(want_residual)
     a.precision = precision
     a.cond_limit = cond_limit
+    want_stats = stats
     stats = np.zeros(3, dtype=np.uint64)
-    a.stats = _ptr(stats, ctypes.c_uint64)
+    # statistics cost a device-to-host copy and a stream synchronisation
+    a.stats = _ptr(stats, ctypes.c_uint64) if want_stats else None
     ctx.check(lib().cg_posterior(ctx._h, ctypes.byref(a)), "cg_posterior")
     return tuple(int(x) for x in stats)
 

@@ -510,8 +510,20 @@ int cg_fft_r2c_inplace(cg_grid *g)
   if (!g) return CG_EINVAL;
   if (g->transformed) return cg::fail(g->ctx, CG_ESTATE, "fftInPlace: grid already in the Fourier domain");
   DeviceGuard dg(g->ctx->device);
-  int rc = cg::grid_fft_forward(g);
+  int rc = cg::grid_fft_forward(g, nullptr);
   if (rc == CG_OK) g->transformed = 1;
+  return rc;
+}
+
+int cg_fft_r2c(const cg_grid *src, cg_grid *dst)
+{
+  if (!src || !dst) return CG_EINVAL;
+  if (src->ctx != dst->ctx || !same_shape(src, dst)) return cg::fail(dst->ctx, CG_EINVAL, "fft_r2c: grids differ");
+  if (src->transformed) return cg::fail(dst->ctx, CG_ESTATE, "fft_r2c: source is in the Fourier domain");
+  DeviceGuard dg(dst->ctx->device);
+  dst->cubetype = src->cubetype;
+  int rc = cg::grid_fft_forward(dst, src == dst ? nullptr : src->d);
+  if (rc == CG_OK) dst->transformed = 1;
   return rc;
 }
 
@@ -736,6 +748,60 @@ int cg_simulate(cg_ctx *ctx, const cg_sim_args *a)
     a->stats[0] = h;
   }
   return CG_OK;
+}
+
+// ---------------------------------------------------------------------------- forward modelling
+int cg_forward_model(cg_ctx *ctx, int nt, const float *K, cg_grid *m[3], cg_grid *d[])
+{
+  if (!ctx || !K || !m || !d) return CG_EINVAL;
+  if (nt < 1 || nt > cg::kMaxTheta) return cg::fail(ctx, CG_EINVAL, "forward_model: ntheta must be 1..8");
+  const cg_grid *ref = m[0];
+  if (!ref) return cg::fail(ctx, CG_EINVAL, "forward_model: missing grid");
+  for (int g = 0; g < 3 + nt; ++g) {
+    const cg_grid *x = g < 3 ? m[g] : d[g - 3];
+    if (!x || x->ctx != ctx || !same_shape(x, ref)) return cg::fail(ctx, CG_EINVAL, "forward_model: bad grid");
+    if (g < 3 && !x->transformed) return cg::fail(ctx, CG_ESTATE, "forward_model: elastic grids must be transformed");
+  }
+  DeviceGuard dg(ctx->device);
+  std::vector<void *> dp;
+  const size_t kb = sizeof(float) * 2 * 3 * (size_t)nt * ref->nzp;
+  int rc = stage_tables(ctx, {{K, kb}}, dp);
+  if (rc) return rc;
+  cg::ForwardParams p;
+  std::memset(&p, 0, sizeof(p));
+  p.plane = (long)ref->cnxp * ref->nyp;
+  p.nzp = ref->nzp;
+  p.nt = nt;
+  p.K = static_cast<const float2 *>(dp[0]);
+  for (int g = 0; g < 3; ++g) p.m[g] = reinterpret_cast<const float2 *>(m[g]->d);
+  for (int l = 0; l < nt; ++l) {
+    p.d[l] = reinterpret_cast<float2 *>(d[l]->d);
+    d[l]->transformed = 1;
+  }
+  return cg::forward_model(ctx, p);
+}
+
+int cg_grid_add_white_noise(cg_grid *g, float sigma, uint64_t seed, uint32_t stream)
+{
+  if (!g) return CG_EINVAL;
+  if (g->transformed) return cg::fail(g->ctx, CG_ESTATE, "add_white_noise: grid is in the Fourier domain");
+  DeviceGuard dg(g->ctx->device);
+  return cg::add_white_noise(g->ctx, g->d, g->nxp, g->rnxp, (long)g->nyp * g->nzp, sigma, seed, stream);
+}
+
+int cg_grid_sum_squares(cg_grid *g, int ext, double *sum)
+{
+  if (!g || !sum) return CG_EINVAL;
+  if (g->transformed) return cg::fail(g->ctx, CG_ESTATE, "sum_squares: grid is in the Fourier domain");
+  cg_ctx *ctx = g->ctx;
+  DeviceGuard dg(ctx->device);
+  double *d = reinterpret_cast<double *>(ctx->counters + 4);  // counters[4] as a double slot
+  int rc = cg::sum_squares(ctx, g->d, ext ? g->nxp : g->nx, ext ? g->nyp : g->ny, ext ? g->nzp : g->nz, g->rnxp,
+                           g->nyp, d);
+  if (rc) return rc;
+  hipError_t e = hipMemcpyAsync(sum, d, sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  return cg::hip_fail(ctx, e, "sum_squares");
 }
 
 // ---------------------------------------------------------------------------- host setup

@@ -222,8 +222,10 @@ __device__ __forceinline__ void load_twiddles(float2 *lds_tw, const float2 *gtw,
 // read as nxp/2 complex numbers z[m] = x[2m] + i x[2m+1]; after the half-length FFT,
 // X[k] = E[k] + w^k O[k] with E = (Z[k] + conj Z[n2-k]) / 2, O = -i (Z[k] - conj Z[n2-k]) / 2,
 // w = exp(-2 pi i / nxp).  Odd nxp: full-length complex FFT of (x, 0).
+// `src` and `grid` may alias (in-place): every load of a workgroup's rows precedes the
+// barrier and every store follows the last stage.
 __global__ void __launch_bounds__(cg::kThreads)
-k_fft_x_r2c(float *__restrict__ grid, int nxp, long rows, int rows_per_wg, cg::FftPlan plan,
+k_fft_x_r2c(const float *src, float *grid, int nxp, long rows, int rows_per_wg, cg::FftPlan plan,
             const float2 *__restrict__ tw_fft, const float2 *__restrict__ tw_row, float scale)
 {
   extern __shared__ float2 smem[];
@@ -242,7 +244,7 @@ k_fft_x_r2c(float *__restrict__ grid, int nxp, long rows, int rows_per_wg, cg::F
     const long row = row0 + r;
     float2 v = make_float2(0.f, 0.f);
     if (row < rows) {
-      const float *p = grid + row * rnxp;
+      const float *p = src + row * rnxp;
       if (even) {
         v = reinterpret_cast<const float2 *>(p)[m];
         v = cscale(v, scale);
@@ -414,7 +416,7 @@ bool make_plan(int n, FftPlan *plan)
   return true;
 }
 
-int fft_x_r2c(cg_ctx *ctx, float *grid, int nxp, long rows, float scale)
+int fft_x_r2c(cg_ctx *ctx, const float *src, float *grid, int nxp, long rows, float scale)
 {
   const bool even = (nxp % 2) == 0;
   FftPlan plan;
@@ -430,7 +432,7 @@ int fft_x_r2c(cg_ctx *ctx, float *grid, int nxp, long rows, float scale)
   const unsigned nblk = (unsigned)((rows + rpw - 1) / rpw);
   hipEvent_t ev;
   launch_begin(ctx, CG_K_FFT_X, &ev);
-  k_fft_x_r2c<<<nblk, kThreads, lds, ctx->stream>>>(grid, nxp, rows, rpw, plan, twf, twr, scale);
+  k_fft_x_r2c<<<nblk, kThreads, lds, ctx->stream>>>(src, grid, nxp, rows, rpw, plan, twf, twr, scale);
   launch_end(ctx, CG_K_FFT_X, ev);
   return hip_fail(ctx, hipGetLastError(), "k_fft_x_r2c");
 }
@@ -488,13 +490,14 @@ int fft_strided(cg_ctx *ctx, float2 *data, int n, long S, long inner, long outer
   return fft_strided(ctx, data, n, S, inner, outer, OS, dir, scale, CG_K_FFT_Y);
 }
 
-// FFTGrid::fftInPlace: pre-scale (fused into the x pass), x r2c, y, z.
-int grid_fft_forward(cg_grid *g)
+// FFTGrid::fftInPlace: pre-scale (fused into the x pass), x r2c, y, z.  With src != null the
+// x pass reads the real cube from src (same layout) and leaves it unchanged.
+int grid_fft_forward(cg_grid *g, const float *src)
 {
   cg_ctx *ctx = g->ctx;
   const long N = (long)g->nxp * g->nyp * g->nzp;
   const float s = (g->cubetype == CG_CT_COVARIANCE) ? 1.0f : 1.0f / sqrtf((float)N);
-  int rc = fft_x_r2c(ctx, g->d, g->nxp, (long)g->nyp * g->nzp, s);
+  int rc = fft_x_r2c(ctx, src ? src : g->d, g->d, g->nxp, (long)g->nyp * g->nzp, s);
   if (rc) return rc;
   float2 *c = reinterpret_cast<float2 *>(g->d);
   const long cnxp = g->cnxp, plane = cnxp * g->nyp;

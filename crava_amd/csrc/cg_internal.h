@@ -79,11 +79,11 @@ void launch_begin(cg_ctx *ctx, int kclass, hipEvent_t *start);
 void launch_end(cg_ctx *ctx, int kclass, hipEvent_t start);
 
 // FFT passes (cg_fft.hip).  All enqueue on ctx->stream and return a CG_* status.
-int fft_x_r2c(cg_ctx *ctx, float *grid, int nxp, long rows, float scale);
+int fft_x_r2c(cg_ctx *ctx, const float *src, float *grid, int nxp, long rows, float scale);
 int fft_x_c2r(cg_ctx *ctx, float *grid, int nxp, long rows, float scale, const float *add);
 int fft_strided(cg_ctx *ctx, float2 *data, int n, long elem_stride, long inner, long outer, long outer_stride,
                 int dir, float scale);
-int grid_fft_forward(cg_grid *g);
+int grid_fft_forward(cg_grid *g, const float *src);
 int grid_fft_inverse(cg_grid *g, const float *add_real);
 
 // Elementwise kernels (cg_noise.hip).

@@ -16,6 +16,17 @@ struct ColourParams {
   unsigned long long *fails;
 };
 
+struct ForwardParams {
+  long plane;
+  int nzp, nt;
+  const float2 *K;  // [nzp][nt][3], device
+  const float2 *m[3];
+  float2 *d[8];
+};
+
+int forward_model(cg_ctx *ctx, const ForwardParams &p);
+int add_white_noise(cg_ctx *ctx, float *x, int nxp, int rnxp, long rows, float sigma, uint64_t seed, uint32_t stream);
+int sum_squares(cg_ctx *ctx, const float *x, int nx, int ny, int nz, int rnxp, int nyp, double *dout);
 int fill_noise(cg_ctx *ctx, float2 *out, long csize, int cnxp, int nyp, int nzp, uint64_t seed, uint32_t stream);
 int colour(cg_ctx *ctx, const ColourParams &p);
 int zero_counters(cg_ctx *ctx, unsigned long long *d, int n);

@@ -102,6 +102,10 @@ int cg_fft_r2c_inplace(cg_grid *grid);
 /* FFTGrid::invFFTInPlace (fftgrid.cpp:1615-1645): z, y complex, then x half-complex-to-real
  * that ignores Im of the DC and Nyquist bins; x 1/N (COVARIANCE) or 1/sqrt(N) afterwards. */
 int cg_fft_c2r_inplace(cg_grid *grid);
+/* Out-of-place form of fftInPlace for callers that keep the real cube: dst (same padded size)
+ * receives src's spectrum with src's cube type; src is unchanged.  No reference counterpart
+ * (FFTGrid only transforms in place); it lets resident inputs be transformed repeatedly. */
+int cg_fft_r2c(const cg_grid *src, cg_grid *dst);
 /* FFTGrid::add / multiplyByScalar (fftgrid.cpp:1659-1678, 1766-1773). */
 int cg_grid_add(cg_grid *dst, const cg_grid *src);
 int cg_grid_multiply_by_scalar(cg_grid *grid, float scalar);
@@ -174,6 +178,19 @@ int cg_simulate(cg_ctx *ctx, const cg_sim_args *args);
 /* FFTGrid::fillInComplexNoise layout (fftgrid.cpp:997-1045) with the counter-based generator:
  * the grid becomes a PARAMETER grid in the Fourier domain. */
 int cg_grid_fill_noise_counter(cg_grid *grid, uint64_t seed, uint32_t stream);
+
+/* ------------------------------------------------------------------ forward modelling */
+/* Synthetic seismic spectra d_l(k) = sum_j K_lj(kz) m_j(k) from Fourier-domain elastic grids
+ * m[3], with the per-k_z operator K of cg_post_args (host table [nzp][ntheta][3] complex).
+ * The spectral counterpart of CravaResult::ComputeSyntSeismic (cravaresult.cpp:2312-2377),
+ * which applies the same wavelet-times-reflectivity operator trace by trace. */
+int cg_forward_model(cg_ctx *ctx, int ntheta, const float *K, cg_grid *m[3], cg_grid *d[]);
+/* Real-domain white noise x += sigma * N(0,1) on every padded cell (counter-based generator). */
+int cg_grid_add_white_noise(cg_grid *grid, float sigma, uint64_t seed, uint32_t stream);
+/* Sum of squares of a real-domain grid over the unpadded box (ext_simbox = 0) or the padded
+ * box; the basis of the data variance of ModelAVODynamic::ComputeDataVariance
+ * (modelavodynamic.cpp:575-620).  Synchronises. */
+int cg_grid_sum_squares(cg_grid *grid, int ext_simbox, double *sum);
 
 /* ------------------------------------------------------------------ host-side setup */
 /* Per-k_z tables of the constant-thickness branch (avoinversion.cpp:944-975; Wavelet1D
