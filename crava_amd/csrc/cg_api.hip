@@ -682,6 +682,10 @@ int cg_posterior(cg_ctx *ctx, const cg_post_args *a)
   }
   rc = cg::launch_posterior(ctx, p);
   if (rc) return rc;
+  for (int g = 0; g < 6; ++g) {  // the covariance grids now hold posterior covariance spectra
+    a->cov[g]->transformed = 1;
+    a->cov[g]->cubetype = CG_CT_COVARIANCE;
+  }
   if (a->stats) {
     uint64_t h[4];
     hipError_t e = hipMemcpyAsync(h, ctx->counters, sizeof(h), hipMemcpyDeviceToHost, ctx->stream);
