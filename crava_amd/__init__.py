@@ -30,6 +30,7 @@ PRIOR_GRIDS, PRIOR_SEPARABLE = 0, 1
 PREC_REFERENCE, PREC_MIXED = 0, 1
 NOISE_COUNTER, NOISE_GRIDS = 0, 1
 K_FFT_X, K_FFT_Y, K_FFT_Z, K_POSTERIOR, K_NOISE, K_ELEMENTWISE = range(6)
+FFT_AUTO, FFT_GENERIC = 0, 1
 RMISSING = -99999.0
 
 _vp = ctypes.c_void_p
@@ -91,6 +92,7 @@ def lib():
             "cg_forward_model": [_vp, i, _f_p, _vp * 3, _vp * 8],
             "cg_grid_add_white_noise": [_vp, f, u64, ctypes.c_uint32],
             "cg_grid_sum_squares": [_vp, i, _d_p],
+            "cg_ctx_set_fft_engine": [_vp, i],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -132,6 +134,10 @@ class Context:
         a, b = ctypes.c_uint64(), ctypes.c_uint64()
         lib().cg_ctx_memory(self._h, ctypes.byref(a), ctypes.byref(b))
         return a.value, b.value
+
+    def set_fft_engine(self, engine):
+        """FFT_AUTO (compile-time plans where available) or FFT_GENERIC (runtime plans only)."""
+        self.check(lib().cg_ctx_set_fft_engine(self._h, engine), "set_fft_engine")
 
     def set_profiling(self, on):
         self.check(lib().cg_ctx_set_profiling(self._h, 1 if on else 0), "cg_ctx_set_profiling")

@@ -305,6 +305,13 @@ int cg_ctx_stream(const cg_ctx *ctx, void **stream)
   return CG_OK;
 }
 
+int cg_ctx_set_fft_engine(cg_ctx *ctx, int engine)
+{
+  if (!ctx || (engine != CG_FFT_AUTO && engine != CG_FFT_GENERIC)) return CG_EINVAL;
+  ctx->fft_engine = engine;
+  return CG_OK;
+}
+
 int cg_ctx_set_profiling(cg_ctx *ctx, int enable)
 {
   if (!ctx) return CG_EINVAL;

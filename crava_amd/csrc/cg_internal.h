@@ -52,6 +52,7 @@ struct cg_ctx {
   unsigned *list = nullptr;
   size_t list_bytes = 0;
   std::set<cg_grid *> grids;  // live grids; cg_ctx_destroy releases any left over
+  int fft_engine = CG_FFT_AUTO;
   bool profiling = false;
   std::vector<cg::TimedLaunch> timed;
   std::vector<hipEvent_t> event_pool;
@@ -84,6 +85,11 @@ int fft_x_c2r(cg_ctx *ctx, float *grid, int nxp, long rows, float scale, const f
 int fft_strided(cg_ctx *ctx, float2 *data, int n, long elem_stride, long inner, long outer, long outer_stride,
                 int dir, float scale);
 int grid_fft_forward(cg_grid *g, const float *src);
+// Compile-time-plan passes (cg_fft_fixed.hip): return 1 and set *rc if the length has a plan.
+int fixed_strided(cg_ctx *ctx, float2 *data, int n, long S, long inner, long outer, long OS, int dir, float scale,
+                  int *rc);
+int fixed_x(cg_ctx *ctx, int forward, const float *src, float *grid, int nxp, long rows, float scale, const float *add,
+            int *rc);
 int grid_fft_inverse(cg_grid *g, const float *add_real);
 
 // Elementwise kernels (cg_noise.hip).

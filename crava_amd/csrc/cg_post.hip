@@ -39,61 +39,56 @@ constexpr double kTol = 1e-20;  // lib_matr.c:19
 
 __device__ __forceinline__ int tri(int i, int j) { return i * (i + 1) / 2 + j; }  // j <= i
 
-// lib_matrCholCpx on the lower triangle (the reference never reads the upper one after the
-// normalisation): M /= M00 (fail if M00 <= 0), row-wise Cholesky, fail if a diagonal entry or
-// a pivot is <= TOL, then L *= sqrt(M00).  Returns 0 on success.  dmin/dmax: extreme squared
-// pivots of the normalised factor (a lower bound of cond(M) is dmax/dmin).
+// Reciprocal: one v_rcp (about 1 ulp) in FP32, an exact division in FP64.
+__device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
+
+// Cholesky of the Hermitian M (lower triangle) with lib_matrCholCpx's accept/reject rules
+// (lib_matr.c:78-167): fail if M00 <= 0, or if a diagonal entry or a pivot of M / M00 is
+// <= TOL.  The reference divides M by M00, factors, and rescales L by sqrt(M00); factoring M
+// directly and testing against TOL * M00 is the same decision without the two rescaling
+// passes.  On success m holds L and rd[i] = 1 / L_ii.  cond_lb = max pivot / min pivot, a
+// lower bound of cond(M).
 template <int NT, typename R>
-__device__ __forceinline__ int chol(cpx<R> (&m)[NT * (NT + 1) / 2], R &dmin, R &dmax)
+__device__ __forceinline__ int chol(cpx<R> (&m)[NT * (NT + 1) / 2], R (&rd)[NT], R &cond_lb)
 {
   const R factor = m[0].re;
   if (!(factor > R(0))) return 1;
-  const R inv = R(1) / factor;
-#pragma unroll
-  for (int t = 0; t < NT * (NT + 1) / 2; ++t) {
-    m[t].re = m[t].re * inv;
-    m[t].im = m[t].im * inv;
-  }
-  dmin = R(1e30);
-  dmax = R(0);
+  const double tol = kTol * (double)factor;
+  R pmin = R(1e30), pmax = R(0);
 #pragma unroll
   for (int i = 0; i < NT; ++i) {
-    if (!((double)m[tri(i, i)].re > kTol)) return 1;
+    if (!((double)m[tri(i, i)].re > tol)) return 1;
 #pragma unroll
     for (int j = 0; j < i; ++j) {
-      R lr = 0, li = 0;
+      R lr = m[tri(i, j)].re, li = m[tri(i, j)].im;
 #pragma unroll
-      for (int k = 0; k < j; ++k) {
-        lr += m[tri(i, k)].re * m[tri(j, k)].re + m[tri(i, k)].im * m[tri(j, k)].im;
-        li += -(m[tri(i, k)].re * m[tri(j, k)].im) + m[tri(i, k)].im * m[tri(j, k)].re;
+      for (int k = 0; k < j; ++k) {  // a_ij - sum_k L_ik conj(L_jk)
+        lr -= m[tri(i, k)].re * m[tri(j, k)].re + m[tri(i, k)].im * m[tri(j, k)].im;
+        li -= m[tri(i, k)].im * m[tri(j, k)].re - m[tri(i, k)].re * m[tri(j, k)].im;
       }
-      // L_jj is real and positive here: (a - l) / L_jj
-      const R dj = m[tri(j, j)].re;
-      m[tri(i, j)].re = (m[tri(i, j)].re - lr) / dj;
-      m[tri(i, j)].im = (m[tri(i, j)].im - li) / dj;
+      m[tri(i, j)].re = lr * rd[j];
+      m[tri(i, j)].im = li * rd[j];
     }
     R s = 0;
 #pragma unroll
     for (int k = 0; k < i; ++k) s += m[tri(i, k)].re * m[tri(i, k)].re + m[tri(i, k)].im * m[tri(i, k)].im;
     const R piv = m[tri(i, i)].re - s;
-    if (!((double)piv > kTol)) return 1;
-    dmin = piv < dmin ? piv : dmin;
-    dmax = piv > dmax ? piv : dmax;
-    m[tri(i, i)].re = sqrt(piv);
+    if (!((double)piv > tol)) return 1;
+    pmin = piv < pmin ? piv : pmin;
+    pmax = piv > pmax ? piv : pmax;
+    const R l = sqrt(piv);
+    m[tri(i, i)].re = l;
     m[tri(i, i)].im = 0;
+    rd[i] = rcp(l);
   }
-  const R sf = sqrt(factor);
-#pragma unroll
-  for (int t = 0; t < NT * (NT + 1) / 2; ++t) {
-    m[t].re *= sf;
-    m[t].im *= sf;
-  }
+  cond_lb = pmax / pmin;
   return 0;
 }
 
-// lib_matrAXeqBMatCpx: x <- L^-H L^-1 x for one column.
+// x <- L^-1 x (forward substitution, as lib_matrLXeqBMatCpx, lib_matr.c:390-410).
 template <int NT, typename R>
-__device__ __forceinline__ void solve_col(const cpx<R> (&L)[NT * (NT + 1) / 2], cpx<R> (&x)[NT])
+__device__ __forceinline__ void lsolve(const cpx<R> (&L)[NT * (NT + 1) / 2], const R (&rd)[NT], cpx<R> (&x)[NT])
 {
 #pragma unroll
   for (int i = 0; i < NT; ++i) {
@@ -103,40 +98,28 @@ __device__ __forceinline__ void solve_col(const cpx<R> (&L)[NT * (NT + 1) / 2], 
       xr -= x[j].re * L[tri(i, j)].re - x[j].im * L[tri(i, j)].im;
       xi -= x[j].im * L[tri(i, j)].re + x[j].re * L[tri(i, j)].im;
     }
-    const R d = L[tri(i, i)].re;
-    x[i].re = xr / d;
-    x[i].im = xi / d;
-  }
-#pragma unroll
-  for (int i = NT - 1; i >= 0; --i) {
-    R xr = x[i].re, xi = x[i].im;
-#pragma unroll
-    for (int j = NT - 1; j > i; --j) {
-      xr -= x[j].re * L[tri(j, i)].re + x[j].im * L[tri(j, i)].im;
-      xi -= -x[j].re * L[tri(j, i)].im + x[j].im * L[tri(j, i)].re;
-    }
-    const R d = L[tri(i, i)].re;
-    x[i].re = xr / d;
-    x[i].im = xi / d;
+    x[i].re = xr * rd[i];
+    x[i].im = xi * rd[i];
   }
 }
 
-// The update for one wavenumber in precision R.  Inputs: prior mean m3, data d, prior
-// covariance P (3x3 real symmetric, 6 entries 00 11 22 01 02 12), K (ntheta x 3), and either
-// the separable plane tables (r, S, G) or nothing (general prior), plus lambda, Q, D.
+// The update for one wavenumber in precision R.  With M = L L^H and Y = L^-1 (K Sigma_m):
+//   (K Sigma_m)^H M^-1 (K Sigma_m) = Y^H Y,   X^H r = Y^H (L^-1 r)   (X = M^-1 K Sigma_m),
+// which is the reference's M^-1 solve (lib_matrAXeqBMatCpx, forward then backward
+// substitution) rearranged so that only forward substitutions remain.
 // Returns 0 on success (outputs written), 1 on Cholesky failure (outputs untouched).
 template <int NT, typename R, bool SEP>
 __device__ __forceinline__ int update(const cg::PlaneTab<R> &T, R r, R lam, const R (&P6)[6], cpx<R> (&m3)[3],
                                       cpx<R> (&d)[NT], cpx<R> (&cov6)[6], cpx<R> (&res)[NT], bool want_res,
                                       R &cond_lb)
 {
-  cpx<R> KS[NT][3];
+  cpx<R> Y[3][NT];  // columns of K Sigma_m, then L^-1 of them
   cpx<R> M[NT * (NT + 1) / 2];
   if (SEP) {
 #pragma unroll
     for (int l = 0; l < NT; ++l)
 #pragma unroll
-      for (int a = 0; a < 3; ++a) KS[l][a] = mk<R>(r * T.S[l][a].re, r * T.S[l][a].im);
+      for (int a = 0; a < 3; ++a) Y[a][l] = mk<R>(r * T.S[l][a].re, r * T.S[l][a].im);
 #pragma unroll
     for (int l = 0; l < NT; ++l)
 #pragma unroll
@@ -158,7 +141,7 @@ __device__ __forceinline__ int update(const cg::PlaneTab<R> &T, R r, R lam, cons
           xr += T.K[l][a].re * Pm[a][b];
           xi += T.K[l][a].im * Pm[a][b];
         }
-        KS[l][b] = mk<R>(xr, xi);
+        Y[b][l] = mk<R>(xr, xi);
       }
 #pragma unroll
     for (int l = 0; l < NT; ++l)
@@ -166,9 +149,9 @@ __device__ __forceinline__ int update(const cg::PlaneTab<R> &T, R r, R lam, cons
       for (int m = 0; m <= l; ++m) {
         R xr = 0, xi = 0;
 #pragma unroll
-        for (int a = 0; a < 3; ++a) {
-          xr += KS[l][a].re * T.K[m][a].re + KS[l][a].im * T.K[m][a].im;
-          xi += KS[l][a].im * T.K[m][a].re - KS[l][a].re * T.K[m][a].im;
+        for (int a = 0; a < 3; ++a) {  // (K Sigma_m K^H)_lm = sum_a KS_la conj(K_ma)
+          xr += Y[a][l].re * T.K[m][a].re + Y[a][l].im * T.K[m][a].im;
+          xi += Y[a][l].im * T.K[m][a].re - Y[a][l].re * T.K[m][a].im;
         }
         const int t = tri(l, m);
         M[t].re = xr + lam * T.Q[t].re;
@@ -176,33 +159,10 @@ __device__ __forceinline__ int update(const cg::PlaneTab<R> &T, R r, R lam, cons
         if (l == m) M[t].re += T.D[l];
       }
   }
-  R dmin, dmax;
-  if (chol<NT, R>(M, dmin, dmax)) return 1;
-  cond_lb = dmax / dmin;
-  // X = M^-1 KS, column by column; reduce = KS^H X; m' = m + X^H (d - K m)
-  cpx<R> X[3][NT];
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    cpx<R> col[NT];
-#pragma unroll
-    for (int l = 0; l < NT; ++l) col[l] = KS[l][a];
-    solve_col<NT, R>(M, col);
-#pragma unroll
-    for (int l = 0; l < NT; ++l) X[a][l] = col[l];
-  }
-  const int ia[6] = {0, 1, 2, 0, 0, 1}, ib[6] = {0, 1, 2, 1, 2, 2};
-#pragma unroll
-  for (int g = 0; g < 6; ++g) {
-    R xr = 0, xi = 0;
-#pragma unroll
-    for (int l = 0; l < NT; ++l) {  // (KS^H)[a][l] * X[b][l]
-      const cpx<R> u = KS[l][ia[g]], v = X[ib[g]][l];
-      xr += u.re * v.re + u.im * v.im;
-      xi += u.re * v.im - u.im * v.re;
-    }
-    cov6[g] = mk<R>(P6[g] - xr, -xi);
-  }
-  cpx<R> rd[NT];
+  R rd[NT];
+  if (chol<NT, R>(M, rd, cond_lb)) return 1;
+  // z = L^-1 (d - K m)
+  cpx<R> z[NT];
 #pragma unroll
   for (int l = 0; l < NT; ++l) {
     R xr = d[l].re, xi = d[l].im;
@@ -211,15 +171,31 @@ __device__ __forceinline__ int update(const cg::PlaneTab<R> &T, R r, R lam, cons
       xr -= T.K[l][a].re * m3[a].re - T.K[l][a].im * m3[a].im;
       xi -= T.K[l][a].im * m3[a].re + T.K[l][a].re * m3[a].im;
     }
-    rd[l] = mk<R>(xr, xi);
+    z[l] = mk<R>(xr, xi);
+  }
+  lsolve<NT, R>(M, rd, z);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) lsolve<NT, R>(M, rd, Y[a]);
+  // Sigma' = Sigma_m - Y^H Y (entries 00 11 22 01 02 12), m' = m + Y^H z
+  const int ia[6] = {0, 1, 2, 0, 0, 1}, ib[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+  for (int g = 0; g < 6; ++g) {
+    R xr = 0, xi = 0;
+#pragma unroll
+    for (int l = 0; l < NT; ++l) {
+      const cpx<R> u = Y[ia[g]][l], v = Y[ib[g]][l];
+      xr += u.re * v.re + u.im * v.im;
+      if (g >= 3) xi += u.re * v.im - u.im * v.re;
+    }
+    cov6[g] = mk<R>(P6[g] - xr, -xi);
   }
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     R xr = 0, xi = 0;
 #pragma unroll
-    for (int l = 0; l < NT; ++l) {  // X^H rd
-      xr += X[a][l].re * rd[l].re + X[a][l].im * rd[l].im;
-      xi += -X[a][l].im * rd[l].re + X[a][l].re * rd[l].im;
+    for (int l = 0; l < NT; ++l) {
+      xr += Y[a][l].re * z[l].re + Y[a][l].im * z[l].im;
+      xi += Y[a][l].re * z[l].im - Y[a][l].im * z[l].re;
     }
     m3[a].re += xr;
     m3[a].im += xi;

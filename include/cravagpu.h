@@ -69,6 +69,11 @@ int cg_ctx_reset_kernel_time(cg_ctx *ctx);
 #define CG_K_NOISE 4     /* noise generation and colouring */
 #define CG_K_ELEMENTWISE 5
 #define CG_K_COUNT 6
+/* FFT engine selection: CG_FFT_AUTO uses the compile-time radix plans where a length has one
+ * and the runtime-plan engine otherwise; CG_FFT_GENERIC forces the runtime-plan engine. */
+#define CG_FFT_AUTO 0
+#define CG_FFT_GENERIC 1
+int cg_ctx_set_fft_engine(cg_ctx *ctx, int engine);
 
 /* ------------------------------------------------------------------ grids (FFTGrid) */
 /* FFTGrid(nx,ny,nz,nxp,nyp,nzp) + createRealGrid/createComplexGrid (fftgrid.cpp:40, 1048-1063),

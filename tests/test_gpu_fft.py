@@ -88,3 +88,49 @@ def test_accessors_and_elementwise(ctx):
         g.fftInPlace()  # already transformed
     g.close()
     h.close()
+
+
+# Every compile-time plan length (160 192 256 288 320 384 512 576 640 768 1152) on every axis:
+# the x pass transforms nxp/2 points, y and z their full lengths.
+FIXED = [(320, 192, 8), (576, 288, 4), (8, 4, 320), (1152, 4, 4), (8, 576, 4), (8, 4, 384), (1024, 2, 4),
+         (8, 512, 2), (8, 2, 640), (8, 768, 2), (8, 2, 1152), (1536, 2, 2), (2304, 2, 2), (512, 4, 160),
+         (640, 256, 2), (768, 2, 192), (8, 160, 256)]
+
+
+@pytest.mark.parametrize("nxp,nyp,nzp", FIXED)
+def test_fixed_plan_lengths_match_oracle_and_generic_engine(ctx, nxp, nyp, nzp):
+    print("fixed plans", nxp, nyp, nzp, flush=True)
+    rng = np.random.default_rng(nxp + 7 * nyp + 13 * nzp)
+    x = rng.standard_normal((nzp, nyp, nxp)).astype(np.float32)
+    cnxp = nxp // 2 + 1
+    z = (rng.standard_normal((nzp, nyp, cnxp)) + 1j * rng.standard_normal((nzp, nyp, cnxp))).astype(np.complex64)
+    res = {}
+    for eng in (C.FFT_AUTO, C.FFT_GENERIC):
+        ctx.set_fft_engine(eng)
+        g = _grid(ctx, x.shape, C.DATA)
+        g.upload_real(x)
+        g.fftInPlace()
+        fwd = g.download_complex()
+        g.upload_complex(z)
+        g.invFFTInPlace()
+        res[eng] = (fwd, g.download_real())
+        g.close()
+    ctx.set_fft_engine(C.FFT_AUTO)
+    assert rel_l2(res[C.FFT_AUTO][0], O.grid_fft(x, C.DATA)) < 2e-6
+    assert rel_l2(res[C.FFT_AUTO][1], O.grid_ifft(z, nxp, C.DATA)) < 2e-6
+    assert rel_l2(res[C.FFT_AUTO][0], res[C.FFT_GENERIC][0]) < 2e-6
+    assert rel_l2(res[C.FFT_AUTO][1], res[C.FFT_GENERIC][1]) < 2e-6
+
+
+def test_out_of_place_forward_keeps_source(ctx):
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal((16, 12, 576)).astype(np.float32)
+    src = _grid(ctx, x.shape, C.DATA)
+    dst = _grid(ctx, x.shape, C.PARAMETER)
+    src.upload_real(x)
+    dst.fft_from(src)
+    assert not src.getIsTransformed() and dst.getIsTransformed()
+    assert np.array_equal(src.download_real(), x)
+    assert rel_l2(dst.download_complex(), O.grid_fft(x, C.DATA)) < 2e-6
+    src.close()
+    dst.close()
