@@ -118,6 +118,7 @@ class Context:
     def __init__(self, device=0, hbm_budget=0):
         self._h = _vp()
         self._grids = weakref.WeakSet()
+        self._batches = weakref.WeakSet()
         rc = lib().cg_ctx_create(ctypes.byref(self._h), device, hbm_budget)
         if rc != CG_OK:
             raise CravaError("cg_ctx_create failed (%d): no usable GPU %d" % (rc, device))
@@ -152,6 +153,8 @@ class Context:
 
     def close(self):
         if self._h:
+            for b in list(self._batches):
+                b.close()
             for g in list(self._grids):
                 g.close()
             lib().cg_ctx_destroy(self._h)
@@ -381,3 +384,105 @@ def header_symbols():
     import re
     txt = open(HEADER).read()
     return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(cg_\w+)\s*\(", txt, re.M)))
+
+
+class InversionDesc(ctypes.Structure):
+    _fields_ = [("post", PostArgs), ("mean_in", _vp * 3), ("seis_in", _vp * 8), ("nreal", ctypes.c_int),
+                ("out", ctypes.POINTER(_vp)), ("seed", ctypes.c_uint64)]
+
+
+def _post_args(tables, mean, data, cov, rxy, rt, lam2d, precision, cond_limit, want_residual, keep):
+    a = PostArgs()
+
+    def arr(x, dt):
+        y = np.ascontiguousarray(x, dtype=dt)
+        keep.append(y)
+        return y
+
+    a.ntheta = len(data)
+    a.K = _ptr(arr(tables["K"], np.complex64).view(np.float32), ctypes.c_float)
+    a.e1 = _ptr(arr(tables["e1"], np.complex64).view(np.float32), ctypes.c_float)
+    a.e2 = _ptr(arr(tables["e2"], np.complex64).view(np.float32), ctypes.c_float)
+    a.e3 = _ptr(arr(tables["e3"], np.complex64).view(np.float32), ctypes.c_float)
+    a.invert = _ptr(arr(tables["invert"], np.int32), ctypes.c_int)
+    a.sigma0e = _ptr(arr(tables["sigma0e"], np.float64), ctypes.c_double)
+    a.wnc = float(tables["wnc"])
+    a.prior_var0 = _ptr(arr(tables["prior_var0"], np.float64), ctypes.c_double)
+    a.prior_mode = PRIOR_SEPARABLE
+    for g in range(6):
+        a.cov[g] = cov[g]._h
+    a.rxy = _ptr(arr(rxy, np.float32), ctypes.c_float)
+    a.rt = _ptr(arr(rt, np.float32), ctypes.c_float)
+    a.lam2d = _ptr(arr(lam2d, np.float32), ctypes.c_float)
+    for g in range(3):
+        a.mean[g] = mean[g]._h
+    for l in range(len(data)):
+        a.data[l] = data[l]._h
+    a.want_residual = int(want_residual)
+    a.precision = precision
+    a.cond_limit = cond_limit
+    return a
+
+
+class Batch:
+    """Many inversions of any sizes planned together (cg_batch): every FFT pass, the posterior
+    (per angle count) and the colouring of all of them share launches.  Each item is a dict
+    with tables, rxy, rt, lam2d, mean_in[3], seis_in[nt] (real-domain inputs), mean[3],
+    data[nt], cov[6] (work grids), out (list of [3] grids per realisation), seed."""
+
+    def __init__(self, ctx, items, precision=PREC_MIXED, cond_limit=1e6, want_residual=False):
+        self.ctx = ctx
+        self._keep = []
+        descs = (InversionDesc * len(items))()
+        for d, it in zip(descs, items):
+            d.post = _post_args(it["tables"], it["mean"], it["data"], it["cov"], it["rxy"], it["rt"], it["lam2d"],
+                                precision, cond_limit, want_residual, self._keep)
+            for g in range(3):
+                d.mean_in[g] = it["mean_in"][g]._h
+            for l, s in enumerate(it["seis_in"]):
+                d.seis_in[l] = s._h
+            outs = it.get("out", [])
+            d.nreal = len(outs)
+            arr = (_vp * max(1, 3 * len(outs)))(*[g._h for r in outs for g in r])
+            self._keep.append(arr)
+            d.out = ctypes.cast(arr, ctypes.POINTER(_vp))
+            d.seed = int(it.get("seed", 0))
+        self._h = _vp()
+        L = lib()
+        L.cg_batch_create.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(InversionDesc), ctypes.POINTER(_vp)]
+        L.cg_batch_create.restype = ctypes.c_int
+        for name in ("cg_batch_run", "cg_batch_wait", "cg_batch_destroy"):
+            getattr(L, name).argtypes = [_vp]
+            getattr(L, name).restype = ctypes.c_int
+        L.cg_batch_use_graph.argtypes = [_vp, ctypes.c_int]
+        L.cg_batch_use_graph.restype = ctypes.c_int
+        L.cg_batch_query.argtypes = [_vp, _i_p]
+        L.cg_batch_query.restype = ctypes.c_int
+        ctx.check(L.cg_batch_create(ctx._h, len(items), descs, ctypes.byref(self._h)), "cg_batch_create")
+        ctx._batches.add(self)
+        self._keep = None  # host tables were copied to the device
+
+    def use_graph(self, on=True):
+        self.ctx.check(lib().cg_batch_use_graph(self._h, 1 if on else 0), "cg_batch_use_graph")
+
+    def run(self):
+        self.ctx.check(lib().cg_batch_run(self._h), "cg_batch_run")
+
+    def done(self):
+        d = ctypes.c_int()
+        self.ctx.check(lib().cg_batch_query(self._h, ctypes.byref(d)), "cg_batch_query")
+        return bool(d.value)
+
+    def wait(self):
+        self.ctx.check(lib().cg_batch_wait(self._h), "cg_batch_wait")
+
+    def close(self):
+        if self._h and self.ctx._h:
+            lib().cg_batch_destroy(self._h)
+        self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -892,3 +892,472 @@ int cg_build_kz_tables(int nz, int nzp, double dt_ms, int nt, const float *wav, 
 }
 
 }  // extern "C"
+
+// ============================================================================ batches
+// A cg_batch holds the device descriptors of every pass of its inversions, built once; a run
+// is a fixed sequence of launches (capturable into a hipGraph).
+struct cg_batch {
+  cg_ctx *ctx = nullptr;
+  int n = 0;
+  std::vector<cg_inversion> inv;
+  std::vector<std::vector<cg_grid *>> outs;
+  void *dev = nullptr;
+  size_t dev_bytes = 0;
+  struct Pass {
+    int kind = 0, dir = -1, kclass = 0, nitems = 0;
+    const cg::FftItem *items = nullptr;
+    unsigned blocks = 0;
+    size_t lds = 0;
+  };
+  std::vector<Pass> fwd, inv_mean;
+  std::vector<std::vector<Pass>> sim;  // per realisation: z, y, x passes
+  struct PostGroup {
+    int nt = 0;
+    bool sep = false, mixed = false;
+    const cg::PostItem *items = nullptr;
+    int nitems = 0;
+    unsigned blocks = 0;
+  };
+  std::vector<PostGroup> post;
+  struct ColourPass {
+    const cg::ColourItem *items = nullptr;
+    int nitems = 0;
+    unsigned blocks = 0;
+  };
+  std::vector<ColourPass> colour;
+  unsigned long long *counters = nullptr;
+  int ncounters = 0;
+  bool use_graph = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipEvent_t done = nullptr;
+};
+
+namespace {
+
+struct Layout {
+  size_t size = 0;
+  size_t add(size_t bytes)
+  {
+    const size_t off = size;
+    size += align_up(bytes ? bytes : 1, 256);
+    return off;
+  }
+};
+
+int enqueue_batch(cg_batch *b)
+{
+  cg_ctx *ctx = b->ctx;
+  int rc = CG_OK;
+  if (b->ncounters) rc = cg::zero_counters(ctx, b->counters, b->ncounters);
+  for (auto &p : b->fwd)
+    if (!rc) rc = cg::launch_fft_batch(ctx, p.kind, p.dir, p.items, p.nitems, p.blocks, p.lds, p.kclass);
+  for (auto &g : b->post)
+    if (!rc) rc = cg::launch_posterior_batch(ctx, g.nt, g.sep, g.mixed, g.items, g.nitems, g.blocks);
+  for (auto &p : b->inv_mean)
+    if (!rc) rc = cg::launch_fft_batch(ctx, p.kind, p.dir, p.items, p.nitems, p.blocks, p.lds, p.kclass);
+  for (size_t r = 0; r < b->sim.size(); ++r) {
+    if (!rc) rc = cg::colour_batch(ctx, b->colour[r].items, b->colour[r].nitems, b->colour[r].blocks);
+    for (auto &p : b->sim[r])
+      if (!rc) rc = cg::launch_fft_batch(ctx, p.kind, p.dir, p.items, p.nitems, p.blocks, p.lds, p.kclass);
+  }
+  return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cg_batch_create(cg_ctx *ctx, int n, const cg_inversion *inv, cg_batch **out)
+{
+  if (!ctx || !inv || !out || n < 1) return CG_EINVAL;
+  *out = nullptr;
+  DeviceGuard dg(ctx->device);
+  // ---- validation
+  for (int i = 0; i < n; ++i) {
+    const cg_inversion &v = inv[i];
+    const cg_post_args &a = v.post;
+    const int nt = a.ntheta;
+    if (nt < 1 || nt > cg::kMaxTheta) return cg::fail(ctx, CG_EINVAL, "batch: ntheta must be 1..8");
+    if (!a.K || !a.e1 || !a.e2 || !a.e3 || !a.invert || !a.sigma0e || !a.prior_var0)
+      return cg::fail(ctx, CG_EINVAL, "batch: missing table");
+    if (a.prior_mode != CG_PRIOR_SEPARABLE || !a.rxy || !a.rt || !a.lam2d || a.err_corr)
+      return cg::fail(ctx, CG_EINVAL, "batch: inversions use the separable prior with host rxy, rt, lam2d");
+    if (a.precision != CG_PREC_REFERENCE && a.precision != CG_PREC_MIXED)
+      return cg::fail(ctx, CG_EINVAL, "batch: unknown precision");
+    const cg_grid *ref = v.mean_in[0];
+    if (!ref) return cg::fail(ctx, CG_EINVAL, "batch: missing input grid");
+    std::vector<const cg_grid *> all;
+    for (int g = 0; g < 3; ++g) all.insert(all.end(), {v.mean_in[g], a.mean[g]});
+    for (int l = 0; l < nt; ++l) all.insert(all.end(), {v.seis_in[l], a.data[l]});
+    for (int g = 0; g < 6; ++g) all.push_back(a.cov[g]);
+    if (v.nreal < 0 || (v.nreal > 0 && !v.out)) return cg::fail(ctx, CG_EINVAL, "batch: bad realisations");
+    for (int r = 0; r < 3 * v.nreal; ++r) all.push_back(v.out[r]);
+    for (const cg_grid *g : all) {
+      if (!g || g->ctx != ctx || !same_shape(g, ref)) return cg::fail(ctx, CG_EINVAL, "batch: bad grid");
+    }
+    for (int g = 0; g < 3; ++g)
+      if (v.mean_in[g]->transformed) return cg::fail(ctx, CG_ESTATE, "batch: inputs must be in the real domain");
+    for (int l = 0; l < nt; ++l)
+      if (v.seis_in[l]->transformed) return cg::fail(ctx, CG_ESTATE, "batch: inputs must be in the real domain");
+  }
+  cg_batch *b = new (std::nothrow) cg_batch();
+  if (!b) return CG_ENOMEM;
+  b->ctx = ctx;
+  b->n = n;
+  b->inv.assign(inv, inv + n);
+  b->outs.resize(n);
+  int maxR = 0;
+  for (int i = 0; i < n; ++i) {
+    b->outs[i].assign(inv[i].out, inv[i].out + 3 * inv[i].nreal);
+    b->inv[i].out = nullptr;
+    maxR = std::max(maxR, inv[i].nreal);
+  }
+  // ---- host-side items (device pointers filled once the layout is allocated)
+  std::vector<cg::FftItem> fx, fy, fz, iz, iy, ix;
+  std::vector<std::vector<cg::FftItem>> sz(maxR), sy(maxR), sx(maxR);
+  std::vector<cg::ColourItem> col_items;
+  std::vector<int> col_count(maxR, 0);
+  size_t lds_fx = 0, lds_fy = 0, lds_fz = 0, lds_ix = 0, lds_iy = 0, lds_iz = 0;
+  std::vector<size_t> lds_sx(maxR, 0), lds_sy(maxR, 0), lds_sz(maxR, 0);
+  int rc = CG_OK;
+  auto add_x = [&](std::vector<cg::FftItem> &v, size_t &lds, int kind, const float *src, float *grid,
+                   const cg_grid *g, float scale, const float *add) {
+    cg::FftItem it;
+    std::memset(&it, 0, sizeof(it));
+    it.src = src;
+    it.grid = grid;
+    it.add = add;
+    it.nxp = g->nxp;
+    it.rows = (long)g->nyp * g->nzp;
+    it.scale = scale;
+    size_t l = 0;
+    if (!rc) rc = cg::plan_fft_item(ctx, kind, &it, &l);
+    lds = std::max(lds, l);
+    v.push_back(it);
+  };
+  auto add_s = [&](std::vector<cg::FftItem> &v, size_t &lds, cg_grid *g, int axis) {
+    cg::FftItem it;
+    std::memset(&it, 0, sizeof(it));
+    const long plane = (long)g->cnxp * g->nyp;
+    it.data = reinterpret_cast<float2 *>(g->d);
+    if (axis == 1) {
+      it.n = g->nyp; it.S = g->cnxp; it.inner = g->cnxp; it.outer = g->nzp; it.OS = plane;
+    } else {
+      it.n = g->nzp; it.S = plane; it.inner = plane; it.outer = 1; it.OS = 0;
+    }
+    if (it.n < 2) return;
+    it.scale = 1.0f;
+    size_t l = 0;
+    if (!rc) rc = cg::plan_fft_item(ctx, cg::FFT_ITEM_STRIDED, &it, &l);
+    lds = std::max(lds, l);
+    v.push_back(it);
+  };
+  for (int i = 0; i < n && !rc; ++i) {
+    const cg_inversion &v = b->inv[i];
+    const cg_post_args &a = v.post;
+    const cg_grid *ref = v.mean_in[0];
+    const size_t N = (size_t)ref->nxp * ref->nyp * ref->nzp;
+    const float sf = 1.0f / sqrtf((float)N);
+    const float si = float(1.0 / sqrtf(float(N)));
+    for (int g = 0; g < 3; ++g) add_x(fx, lds_fx, cg::FFT_ITEM_X_R2C, v.mean_in[g]->d, a.mean[g]->d, ref, sf, nullptr);
+    for (int l = 0; l < a.ntheta; ++l)
+      add_x(fx, lds_fx, cg::FFT_ITEM_X_R2C, v.seis_in[l]->d, a.data[l]->d, ref, sf, nullptr);
+    for (int g = 0; g < 3; ++g) add_s(fy, lds_fy, a.mean[g], 1), add_s(fz, lds_fz, a.mean[g], 2);
+    for (int l = 0; l < a.ntheta; ++l) add_s(fy, lds_fy, a.data[l], 1), add_s(fz, lds_fz, a.data[l], 2);
+    for (int g = 0; g < 3; ++g) {
+      add_s(iz, lds_iz, a.mean[g], 2);
+      add_s(iy, lds_iy, a.mean[g], 1);
+      add_x(ix, lds_ix, cg::FFT_ITEM_X_C2R, nullptr, a.mean[g]->d, ref, si, nullptr);
+    }
+    for (int r = 0; r < v.nreal; ++r)
+      for (int g = 0; g < 3; ++g) {
+        cg_grid *o = b->outs[i][3 * r + g];
+        add_s(sz[r], lds_sz[r], o, 2);
+        add_s(sy[r], lds_sy[r], o, 1);
+        add_x(sx[r], lds_sx[r], cg::FFT_ITEM_X_C2R, nullptr, o->d, ref, si, a.mean[g]->d);
+      }
+  }
+  if (rc) {
+    delete b;
+    return rc;
+  }
+  // ---- layout of the device area
+  Layout L;
+  auto off_items = [&](std::vector<cg::FftItem> &v) { return L.add(sizeof(cg::FftItem) * v.size()); };
+  const size_t o_fx = off_items(fx), o_fy = off_items(fy), o_fz = off_items(fz);
+  const size_t o_iz = off_items(iz), o_iy = off_items(iy), o_ix = off_items(ix);
+  std::vector<size_t> o_sz(maxR), o_sy(maxR), o_sx(maxR), o_col(maxR);
+  for (int r = 0; r < maxR; ++r) {
+    o_sz[r] = off_items(sz[r]);
+    o_sy[r] = off_items(sy[r]);
+    o_sx[r] = off_items(sx[r]);
+    int cnt = 0;
+    for (int i = 0; i < n; ++i) cnt += b->inv[i].nreal > r;
+    col_count[r] = cnt;
+    o_col[r] = L.add(sizeof(cg::ColourItem) * cnt);
+  }
+  // per-inversion posterior tables
+  struct PT {
+    std::vector<cg::PlaneTab<float>> t32;
+    std::vector<cg::PlaneTab<double>> t64;
+    size_t o32, o64, oinv, orxy, ort, olam, olist, ninv;
+  };
+  std::vector<PT> pt(n);
+  const size_t o_counters = L.add(sizeof(unsigned long long) * 4 * n);
+  for (int i = 0; i < n; ++i) {
+    const cg_post_args &a = b->inv[i].post;
+    const cg_grid *ref = b->inv[i].mean_in[0];
+    const size_t plane = (size_t)ref->cnxp * ref->nyp;
+    build_plane_tables<float>(&a, ref->nzp, pt[i].t32);
+    const bool mixed = a.precision == CG_PREC_MIXED;
+    if (mixed) build_plane_tables<double>(&a, ref->nzp, pt[i].t64);
+    pt[i].ninv = 0;
+    for (int k = 0; k < ref->nzp; ++k) pt[i].ninv += a.invert[k] != 0;
+    pt[i].o32 = L.add(sizeof(cg::PlaneTab<float>) * ref->nzp);
+    pt[i].o64 = mixed ? L.add(sizeof(cg::PlaneTab<double>) * ref->nzp) : 0;
+    pt[i].oinv = L.add(sizeof(int) * ref->nzp);
+    pt[i].orxy = L.add(sizeof(float) * plane);
+    pt[i].ort = L.add(sizeof(float) * ref->nzp);
+    pt[i].olam = L.add(sizeof(float) * plane);
+    pt[i].olist = mixed ? L.add(sizeof(unsigned) * std::max<size_t>(pt[i].ninv * plane, 1)) : 0;
+  }
+  // posterior groups by (nt, precision)
+  std::map<std::pair<int, int>, std::vector<int>> groups;
+  for (int i = 0; i < n; ++i) groups[{b->inv[i].post.ntheta, b->inv[i].post.precision}].push_back(i);
+  std::vector<size_t> o_groups;
+  for (auto &kv : groups) o_groups.push_back(L.add(sizeof(cg::PostItem) * kv.second.size()));
+  // ---- allocate
+  hipError_t e = hipMalloc(&b->dev, L.size);
+  if (e != hipSuccess) {
+    delete b;
+    return cg::fail(ctx, CG_ENOMEM, std::string("batch: hipMalloc: ") + hipGetErrorString(e));
+  }
+  b->dev_bytes = L.size;
+  ctx->in_use += L.size;
+  ctx->peak = std::max(ctx->peak, ctx->in_use);
+  char *dev = static_cast<char *>(b->dev);
+  std::vector<char> host(L.size, 0);
+  auto put = [&](size_t off, const void *src, size_t bytes) {
+    if (bytes) std::memcpy(host.data() + off, src, bytes);
+  };
+  // ---- passes: block offsets, copy items
+  auto finish = [&](std::vector<cg::FftItem> &v, size_t off, int kind, int dir, int kclass, size_t lds) {
+    cg_batch::Pass p;
+    unsigned blk = 0;
+    for (auto &it : v) {
+      it.block0 = blk;
+      blk += it.nblocks;
+    }
+    put(off, v.data(), sizeof(cg::FftItem) * v.size());
+    p.kind = kind; p.dir = dir; p.kclass = kclass; p.nitems = (int)v.size();
+    p.items = reinterpret_cast<const cg::FftItem *>(dev + off);
+    p.blocks = blk;
+    p.lds = lds;
+    return p;
+  };
+  b->fwd.push_back(finish(fx, o_fx, cg::FFT_ITEM_X_R2C, -1, CG_K_FFT_X, lds_fx));
+  b->fwd.push_back(finish(fy, o_fy, cg::FFT_ITEM_STRIDED, -1, CG_K_FFT_Y, lds_fy));
+  b->fwd.push_back(finish(fz, o_fz, cg::FFT_ITEM_STRIDED, -1, CG_K_FFT_Z, lds_fz));
+  b->inv_mean.push_back(finish(iz, o_iz, cg::FFT_ITEM_STRIDED, +1, CG_K_FFT_Z, lds_iz));
+  b->inv_mean.push_back(finish(iy, o_iy, cg::FFT_ITEM_STRIDED, +1, CG_K_FFT_Y, lds_iy));
+  b->inv_mean.push_back(finish(ix, o_ix, cg::FFT_ITEM_X_C2R, +1, CG_K_FFT_X, lds_ix));
+  b->sim.resize(maxR);
+  b->colour.resize(maxR);
+  for (int r = 0; r < maxR; ++r) {
+    b->sim[r].push_back(finish(sz[r], o_sz[r], cg::FFT_ITEM_STRIDED, +1, CG_K_FFT_Z, lds_sz[r]));
+    b->sim[r].push_back(finish(sy[r], o_sy[r], cg::FFT_ITEM_STRIDED, +1, CG_K_FFT_Y, lds_sy[r]));
+    b->sim[r].push_back(finish(sx[r], o_sx[r], cg::FFT_ITEM_X_C2R, +1, CG_K_FFT_X, lds_sx[r]));
+    std::vector<cg::ColourItem> ci;
+    unsigned blk = 0;
+    for (int i = 0; i < n; ++i) {
+      if (b->inv[i].nreal <= r) continue;
+      cg::ColourItem it;
+      std::memset(&it, 0, sizeof(it));
+      const cg_grid *ref = b->inv[i].mean_in[0];
+      it.p.csize = (long)ref->csize;
+      it.p.cnxp = ref->cnxp;
+      it.p.nyp = ref->nyp;
+      it.p.nzp = ref->nzp;
+      for (int g = 0; g < 6; ++g) it.p.cov[g] = reinterpret_cast<const float2 *>(b->inv[i].post.cov[g]->d);
+      for (int g = 0; g < 3; ++g) it.p.out[g] = reinterpret_cast<float2 *>(b->outs[i][3 * r + g]->d);
+      it.p.seed = b->inv[i].seed;
+      it.p.stream0 = 3u * (unsigned)r;
+      it.block0 = blk;
+      blk += (unsigned)((ref->csize + cg::kThreads - 1) / cg::kThreads);
+      ci.push_back(it);
+    }
+    put(o_col[r], ci.data(), sizeof(cg::ColourItem) * ci.size());
+    b->colour[r].items = reinterpret_cast<const cg::ColourItem *>(dev + o_col[r]);
+    b->colour[r].nitems = (int)ci.size();
+    b->colour[r].blocks = blk;
+  }
+  // ---- posterior items
+  b->counters = reinterpret_cast<unsigned long long *>(dev + o_counters);
+  b->ncounters = 4 * n;
+  size_t gi = 0;
+  for (auto &kv : groups) {
+    std::vector<cg::PostItem> items;
+    unsigned blk = 0;
+    for (int i : kv.second) {
+      const cg_post_args &a = b->inv[i].post;
+      const cg_grid *ref = b->inv[i].mean_in[0];
+      const size_t plane = (size_t)ref->cnxp * ref->nyp;
+      put(pt[i].o32, pt[i].t32.data(), sizeof(cg::PlaneTab<float>) * ref->nzp);
+      if (!pt[i].t64.empty()) put(pt[i].o64, pt[i].t64.data(), sizeof(cg::PlaneTab<double>) * ref->nzp);
+      put(pt[i].oinv, a.invert, sizeof(int) * ref->nzp);
+      put(pt[i].orxy, a.rxy, sizeof(float) * plane);
+      put(pt[i].ort, a.rt, sizeof(float) * ref->nzp);
+      put(pt[i].olam, a.lam2d, sizeof(float) * plane);
+      cg::PostItem it;
+      std::memset(&it, 0, sizeof(it));
+      cg::PostParams &p = it.p;
+      p.nxp = ref->nxp; p.nyp = ref->nyp; p.nzp = ref->nzp; p.cnxp = ref->cnxp; p.nt = a.ntheta;
+      p.csize = (long)ref->csize;
+      for (int g = 0; g < 3; ++g) p.mean[g] = reinterpret_cast<float2 *>(a.mean[g]->d);
+      for (int l = 0; l < a.ntheta; ++l) p.data[l] = reinterpret_cast<float2 *>(a.data[l]->d);
+      for (int g = 0; g < 6; ++g) p.cov[g] = reinterpret_cast<float2 *>(a.cov[g]->d);
+      p.tab32 = reinterpret_cast<const cg::PlaneTab<float> *>(dev + pt[i].o32);
+      p.tab64 = pt[i].t64.empty() ? nullptr : reinterpret_cast<const cg::PlaneTab<double> *>(dev + pt[i].o64);
+      p.invert = reinterpret_cast<const int *>(dev + pt[i].oinv);
+      p.rxy = reinterpret_cast<const float *>(dev + pt[i].orxy);
+      p.rt = reinterpret_cast<const float *>(dev + pt[i].ort);
+      p.lam2d = reinterpret_cast<const float *>(dev + pt[i].olam);
+      const double *pv = a.prior_var0;
+      const double s6[6] = {pv[0], pv[4], pv[8], pv[1], pv[2], pv[5]};
+      for (int g = 0; g < 6; ++g) {
+        p.s6f[g] = (float)s6[g];
+        p.s6d[g] = s6[g];
+      }
+      p.prior_mode = CG_PRIOR_SEPARABLE;
+      p.want_residual = a.want_residual;
+      p.precision = a.precision;
+      p.cond_limit = (float)(a.cond_limit > 0 ? a.cond_limit : 1e6);
+      p.counters = b->counters + 4 * i;
+      p.list = pt[i].t64.empty() ? nullptr : reinterpret_cast<unsigned *>(dev + pt[i].olist);
+      it.bpp = (unsigned)((plane + cg::kThreads - 1) / cg::kThreads);
+      it.block0 = blk;
+      blk += it.bpp * (unsigned)ref->nzp;
+      items.push_back(it);
+    }
+    put(o_groups[gi], items.data(), sizeof(cg::PostItem) * items.size());
+    cg_batch::PostGroup pg;
+    pg.nt = kv.first.first;
+    pg.sep = true;
+    pg.mixed = kv.first.second == CG_PREC_MIXED;
+    pg.items = reinterpret_cast<const cg::PostItem *>(dev + o_groups[gi]);
+    pg.nitems = (int)items.size();
+    pg.blocks = blk;
+    b->post.push_back(pg);
+    ++gi;
+  }
+  e = hipMemcpyAsync(b->dev, host.data(), L.size, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&b->done, hipEventDisableTiming);
+  if (e != hipSuccess) {
+    (void)hipFree(b->dev);
+    ctx->in_use -= b->dev_bytes;
+    delete b;
+    return cg::hip_fail(ctx, e, "batch: upload of descriptors");
+  }
+  *out = b;
+  return CG_OK;
+}
+
+int cg_batch_use_graph(cg_batch *b, int enable)
+{
+  if (!b) return CG_EINVAL;
+  b->use_graph = enable != 0;
+  return CG_OK;
+}
+
+int cg_batch_run(cg_batch *b)
+{
+  if (!b) return CG_EINVAL;
+  cg_ctx *ctx = b->ctx;
+  DeviceGuard dg(ctx->device);
+  int rc = CG_OK;
+  if (b->use_graph) {
+    if (!b->exec) {
+      const bool prof = ctx->profiling;
+      ctx->profiling = false;  // no event records inside the captured sequence
+      hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+      if (e != hipSuccess) {
+        ctx->profiling = prof;
+        return cg::hip_fail(ctx, e, "hipStreamBeginCapture");
+      }
+      rc = enqueue_batch(b);
+      e = hipStreamEndCapture(ctx->stream, &b->graph);
+      ctx->profiling = prof;
+      if (rc) return rc;
+      if (e == hipSuccess) e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+      if (e != hipSuccess) return cg::hip_fail(ctx, e, "batch graph capture");
+    }
+    rc = cg::hip_fail(ctx, hipGraphLaunch(b->exec, ctx->stream), "hipGraphLaunch");
+  } else {
+    rc = enqueue_batch(b);
+  }
+  if (rc) return rc;
+  for (int i = 0; i < b->n; ++i) {  // domains after a run
+    cg_inversion &v = b->inv[i];
+    for (int g = 0; g < 3; ++g) {
+      v.post.mean[g]->transformed = 0;
+      v.post.mean[g]->cubetype = CG_CT_PARAMETER;
+    }
+    for (int l = 0; l < v.post.ntheta; ++l) {
+      v.post.data[l]->transformed = 1;
+      v.post.data[l]->cubetype = CG_CT_DATA;
+    }
+    for (int g = 0; g < 6; ++g) {
+      v.post.cov[g]->transformed = 1;
+      v.post.cov[g]->cubetype = CG_CT_COVARIANCE;
+    }
+    for (cg_grid *o : b->outs[i]) {
+      o->transformed = 0;
+      o->cubetype = CG_CT_PARAMETER;
+    }
+  }
+  return cg::hip_fail(ctx, hipEventRecord(b->done, ctx->stream), "hipEventRecord");
+}
+
+int cg_batch_query(cg_batch *b, int *done)
+{
+  if (!b || !done) return CG_EINVAL;
+  DeviceGuard dg(b->ctx->device);
+  const hipError_t e = hipEventQuery(b->done);
+  if (e == hipSuccess) {
+    *done = 1;
+    return CG_OK;
+  }
+  if (e == hipErrorNotReady) {
+    *done = 0;
+    return CG_OK;
+  }
+  return cg::hip_fail(b->ctx, e, "hipEventQuery");
+}
+
+int cg_batch_wait(cg_batch *b)
+{
+  if (!b) return CG_EINVAL;
+  DeviceGuard dg(b->ctx->device);
+  return cg::hip_fail(b->ctx, hipEventSynchronize(b->done), "hipEventSynchronize");
+}
+
+int cg_batch_destroy(cg_batch *b)
+{
+  if (!b) return CG_EINVAL;
+  cg_ctx *ctx = b->ctx;
+  DeviceGuard dg(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->done) (void)hipEventDestroy(b->done);
+  if (b->dev) {
+    (void)hipFree(b->dev);
+    ctx->in_use -= b->dev_bytes;
+  }
+  delete b;
+  return CG_OK;
+}
+
+}  // extern "C"

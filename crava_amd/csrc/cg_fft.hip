@@ -102,9 +102,9 @@ __device__ __forceinline__ void load_twiddles(float2 *lds_tw, const float2 *gtw,
 // w = exp(-2 pi i / nxp).  Odd nxp: full-length complex FFT of (x, 0).
 // `src` and `grid` may alias (in-place): every load of a workgroup's rows precedes the
 // barrier and every store follows the last stage.
-__global__ void __launch_bounds__(cg::kThreads)
-k_fft_x_r2c(const float *src, float *grid, int nxp, long rows, int rows_per_wg, cg::FftPlan plan,
-            const float2 *__restrict__ tw_fft, const float2 *__restrict__ tw_row, float scale)
+__device__ __forceinline__ void x_r2c_body(const float *src, float *grid, int nxp, long rows, int rows_per_wg,
+                                           const cg::FftPlan &plan, const float2 *__restrict__ tw_fft,
+                                           const float2 *__restrict__ tw_row, float scale, unsigned blk)
 {
   extern __shared__ float2 smem[];
   const int n = plan.n;  // nxp/2 (even) or nxp (odd)
@@ -114,7 +114,6 @@ k_fft_x_r2c(const float *src, float *grid, int nxp, long rows, int rows_per_wg, 
   float2 *A = smem + n;
   float2 *B = A + (size_t)rows_per_wg * n;
   const int tid = threadIdx.x, nthr = blockDim.x;
-  const unsigned blk = cg_xcd_remap(blockIdx.x, gridDim.x);
   const long row0 = (long)blk * rows_per_wg;
   load_twiddles(tw, tw_fft, n, tid, nthr);
   for (int idx = tid; idx < rows_per_wg * n; idx += nthr) {
@@ -153,15 +152,22 @@ k_fft_x_r2c(const float *src, float *grid, int nxp, long rows, int rows_per_wg, 
   }
 }
 
+__global__ void __launch_bounds__(cg::kThreads)
+k_fft_x_r2c(const float *src, float *grid, int nxp, long rows, int rows_per_wg, cg::FftPlan plan,
+            const float2 *__restrict__ tw_fft, const float2 *__restrict__ tw_row, float scale)
+{
+  x_r2c_body(src, grid, nxp, rows, rows_per_wg, plan, tw_fft, tw_row, scale, cg_xcd_remap(blockIdx.x, gridDim.x));
+}
+
 // ---------------------------------------------------------------- x axis, inverse
 // cnxp half-complex bins -> nxp reals, unnormalised, then x scale (+ add[] if given).  The
 // imaginary parts of the DC and (even nxp) Nyquist bins are ignored, as FFTW's hc2r does.
 // Even nxp: Z[k] = (X[k] + conj X[n2-k]) + i w^-k (X[k] - conj X[n2-k]); z = IFFT_n2(Z);
 // x[2m] = Re z[m], x[2m+1] = Im z[m].  Odd nxp: Hermitian extension and full-length IFFT.
-__global__ void __launch_bounds__(cg::kThreads)
-k_fft_x_c2r(float *__restrict__ grid, int nxp, long rows, int rows_per_wg, cg::FftPlan plan,
-            const float2 *__restrict__ tw_fft, const float2 *__restrict__ tw_row, float scale,
-            const float *__restrict__ add)
+__device__ __forceinline__ void x_c2r_body(float *__restrict__ grid, int nxp, long rows, int rows_per_wg,
+                                           const cg::FftPlan &plan, const float2 *__restrict__ tw_fft,
+                                           const float2 *__restrict__ tw_row, float scale,
+                                           const float *__restrict__ add, unsigned blk)
 {
   extern __shared__ float2 smem[];
   const int n = plan.n;
@@ -171,7 +177,6 @@ k_fft_x_c2r(float *__restrict__ grid, int nxp, long rows, int rows_per_wg, cg::F
   float2 *A = smem + n;
   float2 *B = A + (size_t)rows_per_wg * n;
   const int tid = threadIdx.x, nthr = blockDim.x;
-  const unsigned blk = cg_xcd_remap(blockIdx.x, gridDim.x);
   const long row0 = (long)blk * rows_per_wg;
   load_twiddles(tw, tw_fft, n, tid, nthr);
   for (int idx = tid; idx < rows_per_wg * n; idx += nthr) {
@@ -233,13 +238,21 @@ k_fft_x_c2r(float *__restrict__ grid, int nxp, long rows, int rows_per_wg, cg::F
   }
 }
 
+__global__ void __launch_bounds__(cg::kThreads)
+k_fft_x_c2r(float *__restrict__ grid, int nxp, long rows, int rows_per_wg, cg::FftPlan plan,
+            const float2 *__restrict__ tw_fft, const float2 *__restrict__ tw_row, float scale,
+            const float *__restrict__ add)
+{
+  x_c2r_body(grid, nxp, rows, rows_per_wg, plan, tw_fft, tw_row, scale, add, cg_xcd_remap(blockIdx.x, gridDim.x));
+}
+
 // ---------------------------------------------------------------- strided complex axis
 // Transforms along an axis with element stride S (complex units).  The tile is TI consecutive
 // inner columns over all n elements: data[o*OS + e*S + c0 + t].
 template <int DIR>
-__global__ void __launch_bounds__(cg::kThreads)
-k_fft_strided(float2 *__restrict__ data, cg::FftPlan plan, long S, long inner, long OS, int TI,
-              unsigned tiles_per_outer, const float2 *__restrict__ tw_fft, float scale)
+__device__ __forceinline__ void strided_body(float2 *__restrict__ data, const cg::FftPlan &plan, long S, long inner,
+                                             long OS, int TI, unsigned tiles_per_outer,
+                                             const float2 *__restrict__ tw_fft, float scale, unsigned blk)
 {
   extern __shared__ float2 smem[];
   const int n = plan.n;
@@ -247,7 +260,6 @@ k_fft_strided(float2 *__restrict__ data, cg::FftPlan plan, long S, long inner, l
   float2 *A = smem + n;
   float2 *B = A + (size_t)n * TI;
   const int tid = threadIdx.x, nthr = blockDim.x;
-  const unsigned blk = cg_xcd_remap(blockIdx.x, gridDim.x);
   const unsigned o = blk / tiles_per_outer;
   const long c0 = (long)(blk % tiles_per_outer) * TI;
   float2 *base = data + (long)o * OS + c0;
@@ -263,6 +275,41 @@ k_fft_strided(float2 *__restrict__ data, cg::FftPlan plan, long S, long inner, l
     const int e = idx / TI, t = idx % TI;
     if (t < ncol) base[e * S + t] = (scale == 1.0f) ? R[idx] : cscale(R[idx], scale);
   }
+}
+
+template <int DIR>
+__global__ void __launch_bounds__(cg::kThreads)
+k_fft_strided(float2 *__restrict__ data, cg::FftPlan plan, long S, long inner, long OS, int TI,
+              unsigned tiles_per_outer, const float2 *__restrict__ tw_fft, float scale)
+{
+  strided_body<DIR>(data, plan, S, inner, OS, TI, tiles_per_outer, tw_fft, scale, cg_xcd_remap(blockIdx.x, gridDim.x));
+}
+
+// ---------------------------------------------------------------- batched passes
+// One launch transforms many grids of any sizes: item i owns blocks [block0_i, block0_{i+1}).
+__device__ __forceinline__ int find_item(const cg::FftItem *items, int nitems, unsigned g)
+{
+  int lo = 0, hi = nitems - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (items[mid].block0 <= g) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <int KIND, int DIR>
+__global__ void __launch_bounds__(cg::kThreads) k_fft_batch(const cg::FftItem *__restrict__ items, int nitems)
+{
+  const unsigned g = cg_xcd_remap(blockIdx.x, gridDim.x);
+  const cg::FftItem &it = items[find_item(items, nitems, g)];
+  const unsigned blk = g - it.block0;
+  if (KIND == cg::FFT_ITEM_X_R2C)
+    x_r2c_body(it.src, it.grid, it.nxp, it.rows, it.tile, it.plan, it.tw, it.twr, it.scale, blk);
+  else if (KIND == cg::FFT_ITEM_X_C2R)
+    x_c2r_body(it.grid, it.nxp, it.rows, it.tile, it.plan, it.tw, it.twr, it.scale, it.add, blk);
+  else
+    strided_body<DIR>(it.data, it.plan, it.S, it.inner, it.OS, it.tile, it.tiles_per_outer, it.tw, it.scale, blk);
 }
 
 constexpr size_t kLdsBudget = 80 * 1024;  // bytes per workgroup: two workgroups per CU (160 KB LDS)
@@ -387,6 +434,56 @@ int fft_strided(cg_ctx *ctx, float2 *data, int n, long S, long inner, long outer
 int fft_strided(cg_ctx *ctx, float2 *data, int n, long S, long inner, long outer, long OS, int dir, float scale)
 {
   return fft_strided(ctx, data, n, S, inner, outer, OS, dir, scale, CG_K_FFT_Y);
+}
+
+int plan_fft_item(cg_ctx *ctx, int kind, FftItem *it, size_t *lds)
+{
+  if (kind == FFT_ITEM_X_R2C || kind == FFT_ITEM_X_C2R) {
+    const bool even = (it->nxp % 2) == 0;
+    const int n = even ? it->nxp / 2 : it->nxp;
+    if (!make_plan(n, &it->plan)) return fail(ctx, CG_EINVAL, "x length has a prime factor above 7");
+    it->n = n;
+    it->tw = twiddle_table(ctx, n);
+    it->twr = twiddle_table(ctx, it->nxp);
+    if (!it->tw || !it->twr) return CG_ENOMEM;
+    int rpw = (int)((kLdsBudget / sizeof(float2) - n) / (2 * (size_t)n));
+    rpw = rpw < 1 ? 1 : (rpw > 16 ? 16 : rpw);
+    it->tile = rpw;
+    it->nblocks = (unsigned)((it->rows + rpw - 1) / rpw);
+    *lds = sizeof(float2) * ((size_t)n + 2 * (size_t)rpw * n);
+    return CG_OK;
+  }
+  const int n = it->n;
+  if (!make_plan(n, &it->plan)) return fail(ctx, CG_EINVAL, "axis length has a prime factor above 7");
+  it->tw = twiddle_table(ctx, n);
+  if (!it->tw) return CG_ENOMEM;
+  int TI = 16;
+  while (TI > 1 && sizeof(float2) * ((size_t)n + 2 * (size_t)n * TI) > kLdsBudget) TI /= 2;
+  if (sizeof(float2) * ((size_t)n + 2 * (size_t)n * TI) > 160 * 1024)
+    return fail(ctx, CG_EINVAL, "axis length too long for the LDS engine");
+  it->tile = TI;
+  it->tiles_per_outer = (unsigned)((it->inner + TI - 1) / TI);
+  it->nblocks = it->tiles_per_outer * (unsigned)it->outer;
+  *lds = sizeof(float2) * ((size_t)n + 2 * (size_t)n * TI);
+  return CG_OK;
+}
+
+int launch_fft_batch(cg_ctx *ctx, int kind, int dir, const FftItem *dev_items, int nitems, unsigned total_blocks,
+                     size_t lds, int kclass)
+{
+  if (nitems <= 0 || total_blocks == 0) return CG_OK;
+  hipEvent_t ev;
+  launch_begin(ctx, kclass, &ev);
+  if (kind == FFT_ITEM_X_R2C)
+    k_fft_batch<FFT_ITEM_X_R2C, -1><<<total_blocks, kThreads, lds, ctx->stream>>>(dev_items, nitems);
+  else if (kind == FFT_ITEM_X_C2R)
+    k_fft_batch<FFT_ITEM_X_C2R, +1><<<total_blocks, kThreads, lds, ctx->stream>>>(dev_items, nitems);
+  else if (dir < 0)
+    k_fft_batch<FFT_ITEM_STRIDED, -1><<<total_blocks, kThreads, lds, ctx->stream>>>(dev_items, nitems);
+  else
+    k_fft_batch<FFT_ITEM_STRIDED, +1><<<total_blocks, kThreads, lds, ctx->stream>>>(dev_items, nitems);
+  launch_end(ctx, kclass, ev);
+  return hip_fail(ctx, hipGetLastError(), "k_fft_batch");
 }
 
 // FFTGrid::fftInPlace: pre-scale (fused into the x pass), x r2c, y, z.  With src != null the

@@ -26,6 +26,22 @@ struct FftPlan {
 
 bool make_plan(int n, FftPlan *plan);  // false if n has a prime factor above 7
 
+// One grid pass in a batched launch (cg_fft.hip: k_fft_batch).  Item i owns the blocks
+// [block0, block0 + nblocks) of the launch.
+enum { FFT_ITEM_X_R2C = 0, FFT_ITEM_X_C2R = 1, FFT_ITEM_STRIDED = 2 };
+struct FftItem {
+  const float *src;  // x forward: real source (may equal grid)
+  float *grid;       // x passes: the grid
+  float2 *data;      // strided passes: complex view
+  const float *add;  // x inverse: optional real grid added after scaling
+  long S, inner, OS, rows, outer;
+  unsigned tiles_per_outer, block0, nblocks;
+  int nxp, n, tile;
+  float scale;
+  FftPlan plan;
+  const float2 *tw, *twr;
+};
+
 struct TimedLaunch {
   int kclass;
   hipEvent_t start, stop;
@@ -85,6 +101,12 @@ int fft_x_c2r(cg_ctx *ctx, float *grid, int nxp, long rows, float scale, const f
 int fft_strided(cg_ctx *ctx, float2 *data, int n, long elem_stride, long inner, long outer, long outer_stride,
                 int dir, float scale);
 int grid_fft_forward(cg_grid *g, const float *src);
+// Batched passes: plan_fft_item completes an item whose geometry fields are set (kind x:
+// src/grid/add/nxp/rows/scale; strided: data/n/S/inner/outer/OS/scale) and returns the LDS
+// bytes it needs; launch_fft_batch runs items already copied to device memory.
+int plan_fft_item(cg_ctx *ctx, int kind, FftItem *it, size_t *lds);
+int launch_fft_batch(cg_ctx *ctx, int kind, int dir, const FftItem *dev_items, int nitems, unsigned total_blocks,
+                     size_t lds, int kclass);
 // Compile-time-plan passes (cg_fft_fixed.hip): return 1 and set *rc if the length has a plan.
 int fixed_strided(cg_ctx *ctx, float2 *data, int n, long S, long inner, long outer, long OS, int dir, float scale,
                   int *rc);

@@ -29,6 +29,12 @@ int add_white_noise(cg_ctx *ctx, float *x, int nxp, int rnxp, long rows, float s
 int sum_squares(cg_ctx *ctx, const float *x, int nx, int ny, int nz, int rnxp, int nyp, double *dout);
 int fill_noise(cg_ctx *ctx, float2 *out, long csize, int cnxp, int nyp, int nzp, uint64_t seed, uint32_t stream);
 int colour(cg_ctx *ctx, const ColourParams &p);
+
+struct ColourItem {
+  ColourParams p;
+  unsigned block0;
+};
+int colour_batch(cg_ctx *ctx, const ColourItem *dev_items, int n, unsigned blocks);
 int zero_counters(cg_ctx *ctx, unsigned long long *d, int n);
 
 }  // namespace cg

@@ -45,6 +45,14 @@ struct PostParams {
   unsigned *list;                // wavenumbers deferred to FP64 (capacity: inverted cells)
 };
 
+struct PostItem {
+  PostParams p;
+  unsigned block0;  // first block of this inversion in a batched launch
+  unsigned bpp;     // blocks per k plane
+};
+
 int launch_posterior(cg_ctx *ctx, const PostParams &p);
+int launch_posterior_batch(cg_ctx *ctx, int nt, bool sep, bool mixed, const PostItem *dev_items, int n,
+                           unsigned blocks);
 
 }  // namespace cg

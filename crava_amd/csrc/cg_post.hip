@@ -249,11 +249,9 @@ __device__ __forceinline__ float prior6(const cg::PostParams &p, long idx, int i
 // wavefront-uniform loads.  With MIXED, wavenumbers whose FP32 factor fails or whose pivot
 // ratio exceeds cond_limit are appended to p.list untouched, for the FP64 pass.
 template <int NT, bool SEP, bool MIXED>
-__global__ void __launch_bounds__(cg::kThreads) k_posterior(cg::PostParams p)
+__device__ __forceinline__ void post32_body(const cg::PostParams &p, int k, long c)
 {
-  const int k = blockIdx.y;
   const long plane = (long)p.cnxp * p.nyp;
-  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const bool active = c < plane;
   const int i = active ? (int)(c % p.cnxp) : 0, j = active ? (int)(c / p.cnxp) : 0;
   const long idx = (long)k * plane + (active ? c : 0);
@@ -318,15 +316,20 @@ __global__ void __launch_bounds__(cg::kThreads) k_posterior(cg::PostParams p)
   }
 }
 
+template <int NT, bool SEP, bool MIXED>
+__global__ void __launch_bounds__(cg::kThreads) k_posterior(cg::PostParams p)
+{
+  post32_body<NT, SEP, MIXED>(p, blockIdx.y, (long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 // FP64 pass over the deferred wavenumbers.  Fixed grid, every thread strides over the list
 // and stops at its length, so every wave exits.
 template <int NT, bool SEP>
-__global__ void __launch_bounds__(cg::kThreads) k_posterior64(cg::PostParams p)
+__device__ __forceinline__ void post64_body(const cg::PostParams &p, unsigned long long e0, unsigned long long stride)
 {
   const unsigned long long n = p.counters[3];
   const long plane = (long)p.cnxp * p.nyp;
-  for (unsigned long long e = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; e < n;
-       e += (unsigned long long)gridDim.x * blockDim.x) {
+  for (unsigned long long e = e0; e < n; e += stride) {
     const long idx = p.list[e];
     const int k = (int)(idx / plane);
     const long c = idx % plane;
@@ -374,6 +377,58 @@ __global__ void __launch_bounds__(cg::kThreads) k_posterior64(cg::PostParams p)
   }
 }
 
+template <int NT, bool SEP>
+__global__ void __launch_bounds__(cg::kThreads) k_posterior64(cg::PostParams p)
+{
+  post64_body<NT, SEP>(p, (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x,
+                       (unsigned long long)gridDim.x * blockDim.x);
+}
+
+// Batched forms: many inversions of any size (same nt and prior form) in one launch.  Item i
+// owns blocks [block0_i, block0_i + nzp_i * bpp_i); block b of an item covers plane b / bpp.
+__device__ __forceinline__ int find_post_item(const cg::PostItem *items, int nitems, unsigned g)
+{
+  int lo = 0, hi = nitems - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (items[mid].block0 <= g) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <int NT, bool SEP, bool MIXED>
+__global__ void __launch_bounds__(cg::kThreads) k_posterior_batch(const cg::PostItem *__restrict__ items, int nitems)
+{
+  const cg::PostItem &it = items[find_post_item(items, nitems, blockIdx.x)];
+  const unsigned local = blockIdx.x - it.block0;
+  post32_body<NT, SEP, MIXED>(it.p, (int)(local / it.bpp), (long)(local % it.bpp) * blockDim.x + threadIdx.x);
+}
+
+template <int NT, bool SEP>
+__global__ void __launch_bounds__(cg::kThreads) k_posterior64_batch(const cg::PostItem *__restrict__ items)
+{
+  post64_body<NT, SEP>(items[blockIdx.y].p, (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x,
+                       (unsigned long long)gridDim.x * blockDim.x);
+}
+
+template <int NT>
+void launch_batch_nt(const cg::PostItem *items, int n, unsigned blocks, hipStream_t s, bool sep, bool mixed)
+{
+  if (sep) {
+    if (mixed) k_posterior_batch<NT, true, true><<<blocks, cg::kThreads, 0, s>>>(items, n);
+    else k_posterior_batch<NT, true, false><<<blocks, cg::kThreads, 0, s>>>(items, n);
+  } else {
+    if (mixed) k_posterior_batch<NT, false, true><<<blocks, cg::kThreads, 0, s>>>(items, n);
+    else k_posterior_batch<NT, false, false><<<blocks, cg::kThreads, 0, s>>>(items, n);
+  }
+  if (mixed) {
+    dim3 g64(128, (unsigned)n);
+    if (sep) k_posterior64_batch<NT, true><<<g64, cg::kThreads, 0, s>>>(items);
+    else k_posterior64_batch<NT, false><<<g64, cg::kThreads, 0, s>>>(items);
+  }
+}
+
 template <int NT>
 void launch_nt(const cg::PostParams &p, dim3 grid, hipStream_t s, bool sep, bool mixed)
 {
@@ -396,6 +451,26 @@ void launch_nt64(const cg::PostParams &p, unsigned blocks, hipStream_t s, bool s
 }  // namespace
 
 namespace cg {
+
+int launch_posterior_batch(cg_ctx *ctx, int nt, bool sep, bool mixed, const PostItem *dev_items, int n,
+                           unsigned blocks)
+{
+  if (n <= 0 || blocks == 0) return CG_OK;
+  hipEvent_t ev;
+  launch_begin(ctx, CG_K_POSTERIOR, &ev);
+  switch (nt) {
+    case 1: launch_batch_nt<1>(dev_items, n, blocks, ctx->stream, sep, mixed); break;
+    case 2: launch_batch_nt<2>(dev_items, n, blocks, ctx->stream, sep, mixed); break;
+    case 3: launch_batch_nt<3>(dev_items, n, blocks, ctx->stream, sep, mixed); break;
+    case 4: launch_batch_nt<4>(dev_items, n, blocks, ctx->stream, sep, mixed); break;
+    case 5: launch_batch_nt<5>(dev_items, n, blocks, ctx->stream, sep, mixed); break;
+    case 6: launch_batch_nt<6>(dev_items, n, blocks, ctx->stream, sep, mixed); break;
+    case 7: launch_batch_nt<7>(dev_items, n, blocks, ctx->stream, sep, mixed); break;
+    default: launch_batch_nt<8>(dev_items, n, blocks, ctx->stream, sep, mixed); break;
+  }
+  launch_end(ctx, CG_K_POSTERIOR, ev);
+  return hip_fail(ctx, hipGetLastError(), "k_posterior_batch");
+}
 
 int launch_posterior(cg_ctx *ctx, const PostParams &p)
 {

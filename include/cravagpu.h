@@ -184,6 +184,36 @@ int cg_simulate(cg_ctx *ctx, const cg_sim_args *args);
  * the grid becomes a PARAMETER grid in the Fourier domain. */
 int cg_grid_fill_noise_counter(cg_grid *grid, uint64_t seed, uint32_t stream);
 
+/* ------------------------------------------------------------------ batched inversions */
+/* Many independent inversions of any sizes (the reference runs one per process) planned
+ * together: every pass of all of them shares one launch (x, y, z FFT passes over all grids;
+ * one posterior launch per angle count; colouring), so small cubes no longer pay one launch per
+ * grid and pass.  No reference counterpart; it serves batched and continuously batched
+ * requests.  Results are identical to running each inversion alone in a batch of one. */
+typedef struct cg_batch cg_batch;
+typedef struct {
+  /* tables, prior, precision and work grids as for cg_posterior: post.mean[3] and post.data[]
+   * receive the spectra of mean_in / seis_in; post.cov[6] the posterior covariance spectra;
+   * after a run post.mean[] hold the real-domain posterior means. */
+  cg_post_args post;
+  cg_grid *mean_in[3]; /* real-domain prior means (PARAMETER), left unchanged */
+  cg_grid *seis_in[8]; /* real-domain seismic cubes (DATA), left unchanged */
+  int nreal;           /* posterior realisations */
+  cg_grid **out;       /* [nreal][3] realisation grids (real domain after a run) */
+  uint64_t seed;       /* counter-noise key; realisation r uses streams 3r, 3r+1, 3r+2 */
+} cg_inversion;
+
+int cg_batch_create(cg_ctx *ctx, int n, const cg_inversion *inv, cg_batch **batch);
+/* Enqueue one run of the whole batch on the context stream (forward FFTs from the inputs,
+ * posterior, inverse FFTs of the means, realisations). */
+int cg_batch_run(cg_batch *batch);
+/* With enable = 1 the next run captures its launches into a hipGraph, later runs replay it. */
+int cg_batch_use_graph(cg_batch *batch, int enable);
+/* done = 1 once the last enqueued run has completed (non-blocking). */
+int cg_batch_query(cg_batch *batch, int *done);
+int cg_batch_wait(cg_batch *batch);
+int cg_batch_destroy(cg_batch *batch);
+
 /* ------------------------------------------------------------------ forward modelling */
 /* Synthetic seismic spectra d_l(k) = sum_j K_lj(kz) m_j(k) from Fourier-domain elastic grids
  * m[3], with the per-k_z operator K of cg_post_args (host table [nzp][ntheta][3] complex).
