@@ -246,7 +246,7 @@ int cg_ctx_create(cg_ctx **out, int device, uint64_t hbm_budget_bytes)
   ctx->budget = hbm_budget_bytes;
   DeviceGuard g(device);
   hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
-  if (e == hipSuccess) e = hipMalloc(&ctx->counters, 8 * sizeof(uint64_t));
+  if (e == hipSuccess) e = hipMalloc(&ctx->counters, 256 * sizeof(uint64_t));
   if (e != hipSuccess) {
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -681,10 +681,12 @@ int cg_posterior(cg_ctx *ctx, const cg_post_args *a)
   p.precision = a->precision;
   p.cond_limit = (float)(a->cond_limit > 0 ? a->cond_limit : 1e6);
   p.list = list;
-  const bool counting = mixed || a->stats;
-  p.counters = counting ? reinterpret_cast<unsigned long long *>(ctx->counters) : nullptr;
-  if (counting) {
-    rc = cg::zero_counters(ctx, p.counters, 4);
+  // counters[0]: FP64 list length; counters[8, 136): statistics slots
+  unsigned long long *cnt = reinterpret_cast<unsigned long long *>(ctx->counters);
+  p.list_len = mixed ? cnt : nullptr;
+  p.stats = a->stats ? cnt + 8 : nullptr;
+  if (mixed || a->stats) {
+    rc = cg::zero_counters(ctx, cnt, 136);
     if (rc) return rc;
   }
   rc = cg::launch_posterior(ctx, p);
@@ -694,13 +696,18 @@ int cg_posterior(cg_ctx *ctx, const cg_post_args *a)
     a->cov[g]->cubetype = CG_CT_COVARIANCE;
   }
   if (a->stats) {
-    uint64_t h[4];
+    uint64_t h[136];
     hipError_t e = hipMemcpyAsync(h, ctx->counters, sizeof(h), hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return cg::hip_fail(ctx, e, "posterior stats");
-    a->stats[0] = h[0];
-    a->stats[1] = h[1];
-    a->stats[2] = h[3];  // FP64 re-solves = deferred wavenumbers
+    uint64_t inv_n = 0, fail_n = 0;
+    for (int s = 0; s < 64; ++s) {
+      inv_n += h[8 + s];
+      fail_n += h[8 + 64 + s];
+    }
+    a->stats[0] = inv_n;
+    a->stats[1] = fail_n;
+    a->stats[2] = mixed ? h[0] : 0;  // FP64 re-solves = deferred wavenumbers
   }
   return CG_OK;
 }
@@ -806,7 +813,7 @@ int cg_grid_sum_squares(cg_grid *g, int ext, double *sum)
   if (g->transformed) return cg::fail(g->ctx, CG_ESTATE, "sum_squares: grid is in the Fourier domain");
   cg_ctx *ctx = g->ctx;
   DeviceGuard dg(ctx->device);
-  double *d = reinterpret_cast<double *>(ctx->counters + 4);  // counters[4] as a double slot
+  double *d = reinterpret_cast<double *>(ctx->counters + 200);  // a double slot past the statistics
   int rc = cg::sum_squares(ctx, g->d, ext ? g->nxp : g->nx, ext ? g->nyp : g->ny, ext ? g->nzp : g->nz, g->rnxp,
                            g->nyp, d);
   if (rc) return rc;
@@ -1104,7 +1111,7 @@ int cg_batch_create(cg_ctx *ctx, int n, const cg_inversion *inv, cg_batch **out)
     size_t o32, o64, oinv, orxy, ort, olam, olist, ninv;
   };
   std::vector<PT> pt(n);
-  const size_t o_counters = L.add(sizeof(unsigned long long) * 4 * n);
+  const size_t o_counters = L.add(sizeof(unsigned long long) * n);
   for (int i = 0; i < n; ++i) {
     const cg_post_args &a = b->inv[i].post;
     const cg_grid *ref = b->inv[i].mean_in[0];
@@ -1194,7 +1201,7 @@ int cg_batch_create(cg_ctx *ctx, int n, const cg_inversion *inv, cg_batch **out)
   }
   // ---- posterior items
   b->counters = reinterpret_cast<unsigned long long *>(dev + o_counters);
-  b->ncounters = 4 * n;
+  b->ncounters = n;
   size_t gi = 0;
   for (auto &kv : groups) {
     std::vector<cg::PostItem> items;
@@ -1233,7 +1240,8 @@ int cg_batch_create(cg_ctx *ctx, int n, const cg_inversion *inv, cg_batch **out)
       p.want_residual = a.want_residual;
       p.precision = a.precision;
       p.cond_limit = (float)(a.cond_limit > 0 ? a.cond_limit : 1e6);
-      p.counters = b->counters + 4 * i;
+      p.list_len = b->counters + i;
+      p.stats = nullptr;
       p.list = pt[i].t64.empty() ? nullptr : reinterpret_cast<unsigned *>(dev + pt[i].olist);
       it.bpp = (unsigned)((plane + cg::kThreads - 1) / cg::kThreads);
       it.block0 = blk;

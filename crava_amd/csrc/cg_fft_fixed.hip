@@ -114,8 +114,16 @@ k_ffts(float2 *__restrict__ data, long S, long inner, long OS, unsigned tiles_pe
   const int ncol = (int)((inner - c0) < TI ? (inner - c0) : TI);
   load_tw(tw, twg, N, tid, NT);
   if (ncol == TI) {
-#pragma unroll 4
-    for (int idx = tid; idx < N * TI; idx += NT) buf[idx] = base[(long)(idx / TI) * S + (idx % TI)];
+    // all loads issued before the first LDS store: N / TPS requests in flight per thread
+    constexpr int PER = N * TI / NT;
+    float2 v[PER];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int idx = tid + i * NT;
+      v[i] = base[(long)(idx / TI) * S + (idx % TI)];
+    }
+#pragma unroll
+    for (int i = 0; i < PER; ++i) buf[tid + i * NT] = v[i];
   } else {
     for (int idx = tid; idx < N * TI; idx += NT) {
       const int c = idx % TI;
@@ -125,8 +133,11 @@ k_ffts(float2 *__restrict__ data, long S, long inner, long OS, unsigned tiles_pe
   __syncthreads();
   Plan<N>::template run<DIR, TI>(buf + (tid % TI), tid / TI, tw);
   if (ncol == TI) {
-#pragma unroll 4
-    for (int idx = tid; idx < N * TI; idx += NT) base[(long)(idx / TI) * S + (idx % TI)] = cscale(buf[idx], scale);
+#pragma unroll
+    for (int i = 0; i < N * TI / NT; ++i) {
+      const int idx = tid + i * NT;
+      base[(long)(idx / TI) * S + (idx % TI)] = cscale(buf[idx], scale);
+    }
   } else {
     for (int idx = tid; idx < N * TI; idx += NT) {
       const int c = idx % TI;
@@ -136,8 +147,16 @@ k_ffts(float2 *__restrict__ data, long S, long inner, long OS, unsigned tiles_pe
 }
 
 // ---------------------------------------------------------------- x pass, even nxp = 2 N
+// A block takes RW consecutive rows.  Each row holds N + 1 complex slots (the in-place layout:
+// 2(N + 1) floats), so the block's rows are one contiguous span of global memory, copied in and
+// out flat (coalesced).  In LDS the rows are interleaved, slot m of row r at m * ES + r with odd
+// ES = RW + 1: the FFT stages then address consecutive rows from consecutive lanes (no bank
+// conflicts), and the flat copies stride by ES (odd, so also conflict-free).
+//
 // Forward: z[m] = x[2m] + i x[2m+1] (scaled), Z = FFT_N(z),
-//   X[k] = (Z[k] + conj Z[N-k]) / 2 - i w^k (Z[k] - conj Z[N-k]) / 2,  w = exp(-2 pi i / 2N).
+//   X[k] = (Z[k] + conj Z[N-k]) / 2 - i w^k (Z[k] - conj Z[N-k]) / 2,  w = exp(-2 pi i / 2N);
+// bins k and N - k are formed from the same pair of slots, so each work item rewrites its own
+// pair in place.
 template <int N, int RW>
 __global__ void __launch_bounds__(RW *Plan<N>::TPS)
 k_fftx_r2c(const float *src, float *dst, long rows, const float2 *__restrict__ twg,
@@ -145,39 +164,56 @@ k_fftx_r2c(const float *src, float *dst, long rows, const float2 *__restrict__ t
 {
   constexpr int TPS = Plan<N>::TPS;
   constexpr int NT = RW * TPS;
-  constexpr int RS = N + 1;  // LDS row pitch
-  constexpr int CN = N + 1;  // complex bins per row
-  constexpr int RN = 2 * CN; // row pitch in floats
+  constexpr int ES = RW + 1;
+  constexpr int CN = N + 1;  // complex slots per row
   extern __shared__ float2 smem[];
   float2 *tw = smem;
   float2 *buf = smem + N;
   const int tid = threadIdx.x;
   const long row0 = (long)cg_xcd_remap(blockIdx.x, gridDim.x) * RW;
+  const int nr = (int)((rows - row0) < RW ? (rows - row0) : RW);
+  const float2 *in = reinterpret_cast<const float2 *>(src) + row0 * CN;
+  float2 *out = reinterpret_cast<float2 *>(dst) + row0 * CN;
   load_tw(tw, twg, N, tid, NT);
-  for (int idx = tid; idx < RW * N; idx += NT) {
-    const int r = idx / N, m = idx % N;
-    const long row = row0 + r;
-    float2 v = make_float2(0.f, 0.f);
-    if (row < rows) v = cscale(reinterpret_cast<const float2 *>(src + row * RN)[m], scale);
-    buf[r * RS + m] = v;
+  constexpr int PER = (RW * CN + NT - 1) / NT;
+  float2 v[PER];
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const int f = tid + i * NT;
+    v[i] = f < nr * CN ? in[f] : make_float2(0.f, 0.f);
+  }
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const int f = tid + i * NT;
+    const int r = f / CN, m = f - r * CN;
+    if (f < RW * CN) buf[m * ES + r] = cscale(v[i], scale);
   }
   __syncthreads();
-  Plan<N>::template run<-1, 1>(buf + (tid / TPS) * RS, tid % TPS, tw);
-  for (int idx = tid; idx < RW * CN; idx += NT) {
-    const int r = idx / CN, k = idx % CN;
-    const long row = row0 + r;
-    if (row >= rows) continue;
-    const float2 zk = buf[r * RS + (k == N ? 0 : k)];
-    const float2 zn = cconj(buf[r * RS + (k == 0 ? 0 : N - k)]);
-    const float2 E = cscale(cadd(zk, zn), 0.5f);
-    const float2 d = csub(zk, zn);
-    const float2 O = make_float2(0.5f * d.y, -0.5f * d.x);
-    reinterpret_cast<float2 *>(dst + row * RN)[k] = cadd(E, cmul(tw_row[k], O));
+  Plan<N>::template run<-1, ES>(buf + tid % RW, tid / RW, tw);
+  for (int idx = tid; idx < RW * (N / 2 + 1); idx += NT) {
+    const int r = idx % RW, k = idx / RW;
+    const int kn = k == 0 ? 0 : N - k;
+    const float2 a = buf[k * ES + r], b = buf[kn * ES + r];
+    // X[k] from (Z[k], conj Z[N-k]); X[N-k] from (Z[N-k], conj Z[k])
+    const float2 ea = cscale(cadd(a, cconj(b)), 0.5f), da = csub(a, cconj(b));
+    const float2 eb = cscale(cadd(b, cconj(a)), 0.5f), db = csub(b, cconj(a));
+    const float2 xa = cadd(ea, cmul(tw_row[k], make_float2(0.5f * da.y, -0.5f * da.x)));
+    const float2 xb = cadd(eb, cmul(tw_row[N - k], make_float2(0.5f * db.y, -0.5f * db.x)));
+    buf[k * ES + r] = xa;
+    buf[(N - k) * ES + r] = xb;  // k = 0 writes bin N into the spare slot
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const int f = tid + i * NT;
+    const int r = f / CN, m = f - r * CN;
+    if (f < nr * CN) out[f] = buf[m * ES + r];
   }
 }
 
 // Inverse: Z[k] = (X[k] + conj X[N-k]) + i w^-k (X[k] - conj X[N-k]) with Im X[0] = Im X[N] = 0,
-// z = IFFT_N(Z), x[2m] = Re z[m], x[2m+1] = Im z[m]; then x scale (+ add).
+// z = IFFT_N(z), x[2m] = Re z[m], x[2m+1] = Im z[m]; then x scale (+ add); the row's spare slot
+// is written as zero.
 template <int N, int RW>
 __global__ void __launch_bounds__(RW *Plan<N>::TPS)
 k_fftx_c2r(float *grid, long rows, const float2 *__restrict__ twg, const float2 *__restrict__ tw_row, float scale,
@@ -185,49 +221,65 @@ k_fftx_c2r(float *grid, long rows, const float2 *__restrict__ twg, const float2 
 {
   constexpr int TPS = Plan<N>::TPS;
   constexpr int NT = RW * TPS;
-  constexpr int RS = N + 1;
+  constexpr int ES = RW + 1;
   constexpr int CN = N + 1;
-  constexpr int RN = 2 * CN;
   extern __shared__ float2 smem[];
   float2 *tw = smem;
   float2 *buf = smem + N;
   const int tid = threadIdx.x;
   const long row0 = (long)cg_xcd_remap(blockIdx.x, gridDim.x) * RW;
+  const int nr = (int)((rows - row0) < RW ? (rows - row0) : RW);
+  float2 *io = reinterpret_cast<float2 *>(grid) + row0 * CN;
+  const float2 *ad = add ? reinterpret_cast<const float2 *>(add) + row0 * CN : nullptr;
   load_tw(tw, twg, N, tid, NT);
-  for (int idx = tid; idx < RW * N; idx += NT) {
-    const int r = idx / N, k = idx % N;
-    const long row = row0 + r;
-    float2 v = make_float2(0.f, 0.f);
-    if (row < rows) {
-      const float2 *p = reinterpret_cast<const float2 *>(grid + row * RN);
-      float2 xk = p[k], xn = p[N - k];
-      if (k == 0) {
-        xk.y = 0.f;
-        xn.y = 0.f;
-      }
-      xn = cconj(xn);
-      const float2 E = cadd(xk, xn);
-      const float2 O = cmul(csub(xk, xn), cconj(tw_row[k]));
-      v = make_float2(E.x - O.y, E.y + O.x);
+  constexpr int PER = (RW * CN + NT - 1) / NT;
+  {
+    float2 v[PER];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int f = tid + i * NT;
+      v[i] = f < nr * CN ? io[f] : make_float2(0.f, 0.f);
     }
-    buf[r * RS + k] = v;
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int f = tid + i * NT;
+      const int r = f / CN, m = f - r * CN;
+      if (f < RW * CN) buf[m * ES + r] = v[i];
+    }
   }
   __syncthreads();
-  Plan<N>::template run<+1, 1>(buf + (tid / TPS) * RS, tid % TPS, tw);
-  for (int idx = tid; idx < RW * CN; idx += NT) {
-    const int r = idx / CN, m = idx % CN;
-    const long row = row0 + r;
-    if (row >= rows) continue;
-    float2 v = make_float2(0.f, 0.f);
-    if (m < N) {
-      v = cscale(buf[r * RS + m], scale);
-      if (add) {
-        const float2 a = reinterpret_cast<const float2 *>(add + row * RN)[m];
-        v.x += a.x;
-        v.y += a.y;
-      }
+  for (int idx = tid; idx < RW * (N / 2 + 1); idx += NT) {
+    const int r = idx % RW, k = idx / RW;
+    float2 a = buf[k * ES + r], b = buf[(N - k) * ES + r];  // X[k], X[N-k]
+    if (k == 0) {
+      a.y = 0.f;
+      b.y = 0.f;
     }
-    reinterpret_cast<float2 *>(grid + row * RN)[m] = v;
+    const float2 ea = cadd(a, cconj(b)), oa = cmul(csub(a, cconj(b)), cconj(tw_row[k]));
+    buf[k * ES + r] = make_float2(ea.x - oa.y, ea.y + oa.x);
+    if (k > 0 && k < N - k) {
+      const float2 eb = cadd(b, cconj(a)), ob = cmul(csub(b, cconj(a)), cconj(tw_row[N - k]));
+      buf[(N - k) * ES + r] = make_float2(eb.x - ob.y, eb.y + ob.x);
+    }
+  }
+  __syncthreads();
+  Plan<N>::template run<+1, ES>(buf + tid % RW, tid / RW, tw);
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const int f = tid + i * NT;
+    const int r = f / CN, m = f - r * CN;
+    if (f < nr * CN) {
+      float2 v = make_float2(0.f, 0.f);
+      if (m < N) {
+        v = cscale(buf[m * ES + r], scale);
+        if (ad) {
+          const float2 q = ad[f];
+          v.x += q.x;
+          v.y += q.y;
+        }
+      }
+      io[f] = v;
+    }
   }
 }
 
@@ -250,17 +302,23 @@ int strided_n(cg_ctx *ctx, float2 *data, long S, long inner, long outer, long OS
   return launch_strided<N, DIR, 8>(ctx, data, S, inner, outer, OS, scale, tw);
 }
 
+// Rows per block: the most (32, 16, 8) whose LDS (twiddles + (N + 1) (RW + 1) slots) fits 80 KB.
+template <int N>
+constexpr bool x_fits(int rw)
+{
+  return (N + (N + 1) * (rw + 1)) * 8 <= 80 * 1024 && rw * Plan<N>::TPS <= 512;
+}
 template <int N>
 constexpr int rows_per_wg()
 {
-  return (Plan<N>::TPS * 32 <= 512 && (N + 1) * 32 * 8 <= 80 * 1024) ? 32 : 16;
+  return x_fits<N>(32) ? 32 : x_fits<N>(16) ? 16 : 8;
 }
 
 template <int N>
 int x_r2c_n(cg_ctx *ctx, const float *src, float *dst, long rows, float scale, const float2 *tw, const float2 *twr)
 {
   constexpr int RW = rows_per_wg<N>();
-  const size_t lds = sizeof(float2) * (N + (size_t)(N + 1) * RW);
+  const size_t lds = sizeof(float2) * (N + (size_t)(N + 1) * (RW + 1));
   const unsigned nblk = (unsigned)((rows + RW - 1) / RW);
   k_fftx_r2c<N, RW><<<nblk, RW * Plan<N>::TPS, lds, ctx->stream>>>(src, dst, rows, tw, twr, scale);
   return cg::hip_fail(ctx, hipGetLastError(), "k_fftx_r2c");
@@ -270,7 +328,7 @@ template <int N>
 int x_c2r_n(cg_ctx *ctx, float *grid, long rows, float scale, const float *add, const float2 *tw, const float2 *twr)
 {
   constexpr int RW = rows_per_wg<N>();
-  const size_t lds = sizeof(float2) * (N + (size_t)(N + 1) * RW);
+  const size_t lds = sizeof(float2) * (N + (size_t)(N + 1) * (RW + 1));
   const unsigned nblk = (unsigned)((rows + RW - 1) / RW);
   k_fftx_c2r<N, RW><<<nblk, RW * Plan<N>::TPS, lds, ctx->stream>>>(grid, rows, tw, twr, scale, add);
   return cg::hip_fail(ctx, hipGetLastError(), "k_fftx_c2r");

@@ -41,7 +41,8 @@ struct PostParams {
   int want_residual;
   int precision;
   float cond_limit;
-  unsigned long long *counters;  // [0] inverted [1] failures [2] FP64 re-solves [3] list length
+  unsigned long long *list_len;   // MIXED: number of wavenumbers deferred to FP64
+  unsigned long long *stats;      // optional: [0,64) inverted, [64,128) Cholesky failures (slots summed on the host)
   unsigned *list;                // wavenumbers deferred to FP64 (capacity: inverted cells)
 };
 

@@ -215,12 +215,13 @@ __device__ __forceinline__ int update(const cg::PlaneTab<R> &T, R r, R lam, cons
   return 0;
 }
 
-// Wave-aggregated counter increment: one atomic per wavefront.  Every lane of the wave must
-// reach this point (the kernels below have no early returns).
-__device__ __forceinline__ void wave_count(unsigned long long *ctr, bool pred)
+// Wave-aggregated counter increment: one atomic per wavefront, spread over 64 slots by block
+// so that waves do not serialise on one address.  Every lane of the wave must reach this
+// point (the kernels below have no early returns).
+__device__ __forceinline__ void wave_count(unsigned long long *slots, bool pred)
 {
   const unsigned long long m = __ballot(pred);
-  if (__lane_id() == 0 && m) atomicAdd(ctr, (unsigned long long)__popcll(m));
+  if (__lane_id() == 0 && m) atomicAdd(slots + (blockIdx.x & 63u), (unsigned long long)__popcll(m));
 }
 
 // Loads the prior covariance entries of one wavenumber (getNextParameterCovariance:
@@ -303,16 +304,16 @@ __device__ __forceinline__ void post32_body(const cg::PostParams &p, int k, long
     const unsigned long long m = __ballot(defer);
     if (m) {
       unsigned long long base = 0;
-      if (__lane_id() == 0) base = atomicAdd(&p.counters[3], (unsigned long long)__popcll(m));
+      if (__lane_id() == 0) base = atomicAdd(p.list_len, (unsigned long long)__popcll(m));
       base = __shfl(base, 0);
       const unsigned lane = __lane_id();
       const unsigned long long below = lane ? (m & ((~0ull) >> (64 - lane))) : 0ull;
       if (defer) p.list[base + __popcll(below)] = (unsigned)idx;
     }
   }
-  if (p.counters) {
-    wave_count(&p.counters[0], did_inv);
-    wave_count(&p.counters[1], did_fail);
+  if (p.stats) {
+    wave_count(p.stats, did_inv);
+    wave_count(p.stats + 64, did_fail);
   }
 }
 
@@ -327,7 +328,7 @@ __global__ void __launch_bounds__(cg::kThreads) k_posterior(cg::PostParams p)
 template <int NT, bool SEP>
 __device__ __forceinline__ void post64_body(const cg::PostParams &p, unsigned long long e0, unsigned long long stride)
 {
-  const unsigned long long n = p.counters[3];
+  const unsigned long long n = *p.list_len;
   const long plane = (long)p.cnxp * p.nyp;
   for (unsigned long long e = e0; e < n; e += stride) {
     const long idx = p.list[e];
@@ -361,7 +362,7 @@ __device__ __forceinline__ void post64_body(const cg::PostParams &p, unsigned lo
     double cond = 0.0;
     const int fail = update<NT, double, SEP>(p.tab64[k], rd, lam, P6d, m3, d, cov6, res, p.want_residual != 0, cond);
     if (fail) {
-      if (p.counters) atomicAdd(&p.counters[1], 1ull);
+      if (p.stats) atomicAdd(p.stats + 64, 1ull);  // rare: the FP64 factor failed too
 #pragma unroll
       for (int g = 0; g < 6; ++g) p.cov[g][idx] = make_float2(P6[g], 0.f);
     } else {
