@@ -28,7 +28,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 import crava_amd as C  # noqa: E402
-from crava_amd import model as M  # noqa: E402
+from crava_amd import synthetic as S  # noqa: E402
 
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
 
@@ -40,75 +40,6 @@ WORKLOAD = dict(nx=512, ny=512, nz=256, nxp=576, nyp=576, nzp=320, dx=25.0, dy=2
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
-
-
-class Inversion:
-    """Device-resident inputs, work grids and the per-step call sequence."""
-
-    def __init__(self, ctx, w, realisations):
-        self.ctx, self.w, self.R = ctx, w, realisations
-        rng = np.random.default_rng(w["seed"])
-        nx, ny, nz, nxp, nyp, nzp = (w[k] for k in ("nx", "ny", "nz", "nxp", "nyp", "nzp"))
-        self.nt = nt = len(w["angles"])
-        self.cnxp = nxp // 2 + 1
-        scales = rng.uniform(0.8, 1.2, nt)
-        sn = rng.uniform(3.0, 8.0, nt)
-        tab = M.inversion_tables(nz, nzp, w["dt"], w["angles"], w["peak_hz"], scales, w["vsvp"], w["low_cut"],
-                                 w["high_cut"])
-        cxy = M.spherical_corr_xy(nxp, nyp, w["dx"], w["dy"], w["lateral_range"])
-        self.rxy = M.lateral_spectrum(cxy)
-        self.rt = M.temporal_spectrum(M.circ_corr_t(w["dt"], w["temporal_range"], nzp, w["low_cut"]))
-        S0 = M.sigma0(w["std"], w["corr"])
-        mk = lambda ct: C.FFTGrid(ctx, nx, ny, nz, nxp, nyp, nzp, ct)
-        self.mean_in = [mk(C.PARAMETER) for _ in range(3)]
-        self.seis_in = [mk(C.DATA) for _ in range(nt)]
-        self.mean_w = [mk(C.PARAMETER) for _ in range(3)]
-        self.data_w = [mk(C.DATA) for _ in range(nt)]
-        self.cov = [mk(C.COVARIANCE) for _ in range(6)]
-        self.out = [[mk(C.PARAMETER) for _ in range(3)] for _ in range(max(realisations, 1))]
-        rnxp = 2 * self.cnxp
-        for g, v in zip(self.mean_in, (math.log(3000.0), math.log(1500.0), math.log(2.3))):
-            g.upload_real(np.full((nzp, nyp, rnxp), v, dtype=np.float32))
-        # prior covariance spectra: a posterior with no inverted plane writes the prior
-        for g, s in zip(self.mean_w, self.mean_in):
-            g.fft_from(s)
-        for g, s in zip(self.data_w, self.seis_in):
-            g.fft_from(s)
-        base = dict(K=tab["K"], e1=tab["e1"], e2=tab["e2"], e3=tab["e3"], sigma0e=np.eye(nt), wnc=w["wnc"],
-                    prior_var0=S0)
-        C.posterior(ctx, dict(base, invert=np.zeros(nzp, np.int32)), self.mean_w, self.data_w, self.cov,
-                    rxy=self.rxy, rt=self.rt, lam2d=self.rxy, want_residual=False)
-        # elastic field from the prior, forward-modelled seismic, white noise at S/N
-        truth = self.out[0]
-        C.simulate(ctx, self.cov, self.mean_in, truth, seed=w["seed"], realisation=1000)
-        for g, s in zip(self.mean_w, truth):
-            g.fft_from(s)
-        C.forward_model(ctx, tab["K"], self.mean_w, self.data_w)
-        sig_e = np.zeros(nt)
-        for l in range(nt):
-            self.data_w[l].setType(C.DATA)
-            self.data_w[l].invFFTInPlace()
-            var_s = self.data_w[l].sum_squares() / (nx * ny * nz)
-            sig_e[l] = math.sqrt(var_s / (sn[l] - 1.0))  # S/N = (|s|^2 + |e|^2) / |e|^2
-            self.seis_in[l].add(self.data_w[l])
-            self.seis_in[l].add_white_noise(float(sig_e[l]), w["seed"], 500 + l)
-        self.tables = dict(base, invert=tab["invert"], sigma0e=M.angular_error_covariance(sig_e, w["angles"]))
-        self.n_inverted_planes = int(np.sum(tab["invert"] != 0))
-        ctx.sync()
-
-    def step(self, realisations=None, stats=False):
-        R = self.R if realisations is None else realisations
-        for g, s in zip(self.mean_w, self.mean_in):
-            g.fft_from(s)
-        for g, s in zip(self.data_w, self.seis_in):
-            g.fft_from(s)
-        st = C.posterior(self.ctx, self.tables, self.mean_w, self.data_w, self.cov, rxy=self.rxy, rt=self.rt,
-                         lam2d=self.rxy, want_residual=False, precision=C.PREC_MIXED, cond_limit=1e6, stats=stats)
-        for g in self.mean_w:
-            g.invFFTInPlace()
-        for r in range(R):
-            C.simulate(self.ctx, self.cov, self.mean_w, self.out[r], seed=self.w["seed"], realisation=r)
-        return st
 
 
 def kernel_bytes(w, nt, R, n_inv_planes):
@@ -153,20 +84,20 @@ def cpu_baseline(w, inv, R, seconds_hint=20.0):
     R_.ref_grid_ifft(nxp, nyp, nzp, O.PARAMETER, O._ptr(x, O.ctypes.c_float))
     t_inv = time.perf_counter() - t
     del x
-    # posterior loop on S planes of inverted and S planes of pass-through wavenumbers, with the
+    # posterior loop on NPL planes of inverted and NPL planes of pass-through wavenumbers, with the
     # workload's tables, in the reference's covariance-grid form
-    S = 4
+    NPL = 4  # planes sampled per kind
     tb = inv.tables
     inv_k = np.where(tb["invert"] != 0)[0]
     pass_k = np.where(tb["invert"] == 0)[0]
     times = {}
     cov_post = None
-    for label, ks in (("inv", inv_k[np.linspace(0, len(inv_k) - 1, S).astype(int)]),
-                      ("pass", pass_k[np.linspace(0, len(pass_k) - 1, S).astype(int)])):
-        p = dict(nxp=nxp, nyp=nyp, nzp=S, ntheta=inv.nt, K=tb["K"][ks], e1=tb["e1"][ks], e2=tb["e2"][ks],
+    for label, ks in (("inv", inv_k[np.linspace(0, len(inv_k) - 1, NPL).astype(int)]),
+                      ("pass", pass_k[np.linspace(0, len(pass_k) - 1, NPL).astype(int)])):
+        p = dict(nxp=nxp, nyp=nyp, nzp=NPL, ntheta=inv.nt, K=tb["K"][ks], e1=tb["e1"][ks], e2=tb["e2"][ks],
                  e3=tb["e3"][ks], invert=tb["invert"][ks], sigma0e=tb["sigma0e"], wnc=tb["wnc"],
                  prior_var0=tb["prior_var0"], prior_mode=0, rxy=None, rt=None, lam2d=None)
-        shape = (S, nyp, cnxp)
+        shape = (NPL, nyp, cnxp)
         r = np.abs(inv.rxy[None] * inv.rt[ks][:, None, None])
         S0 = tb["prior_var0"]
         covs = np.stack([(r * S0[a, b]).astype(np.complex64) for a, b in
@@ -176,18 +107,18 @@ def cpu_baseline(w, inv, R, seconds_hint=20.0):
         data = (rng.standard_normal((inv.nt,) + shape) + 1j * rng.standard_normal((inv.nt,) + shape)).astype(np.complex64)
         t = time.perf_counter()
         _, _, c, _, _ = O.ref_posterior(p, mean, data, covs)
-        times[label] = (time.perf_counter() - t) / (S * plane)
+        times[label] = (time.perf_counter() - t) / (NPL * plane)
         if label == "inv":
             cov_post = c
     seed = (rng.standard_normal((3,) + cov_post.shape[1:]) + 0j).astype(np.complex64)
     t = time.perf_counter()
     O.ref_colour(cov_post, seed)
-    t_col = (time.perf_counter() - t) / (S * plane)
+    t_col = (time.perf_counter() - t) / (NPL * plane)
     R_.ref_rng_seed(23665)
-    noise = np.zeros((S, nyp, cnxp), dtype=np.complex64)
+    noise = np.zeros((NPL, nyp, cnxp), dtype=np.complex64)
     t = time.perf_counter()
-    R_.ref_fill_complex_noise(nxp, nyp, S, O._ptr(noise.view(np.float32), O.ctypes.c_float))
-    t_noise = (time.perf_counter() - t) / (S * plane)
+    R_.ref_fill_complex_noise(nxp, nyp, NPL, O._ptr(noise.view(np.float32), O.ctypes.c_float))
+    t_noise = (time.perf_counter() - t) / (NPL * plane)
     n_inv = inv.n_inverted_planes * plane
     n_all = nzp * plane
     total = (15 * t_fwd + 9 * t_inv + n_inv * times["inv"] + (n_all - n_inv) * times["pass"]
@@ -200,7 +131,97 @@ def cpu_baseline(w, inv, R, seconds_hint=20.0):
                    "reference lib_matr posterior loop on %d inverted + %d pass-through k planes (%.2f / %.3f us "
                    "per wavenumber); lib_matr colouring and RandomGen noise fill on %d planes; extrapolated to "
                    "15 forward + 9 inverse FFTs, the full loop and %d realisations" %
-                   (t_fwd, t_inv, S, S, 1e6 * times["inv"], 1e6 * times["pass"], S, R)),
+                   (t_fwd, t_inv, NPL, NPL, 1e6 * times["inv"], 1e6 * times["pass"], NPL, R)),
+    }
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-30))
+
+
+def bench_mix(ctx, n=64, reps=3):
+    """64 independent inversions of uneven sizes (posterior mean + covariance each): the plain
+    per-request path (each request's launches, then a synchronisation) against one batch of all
+    of them (every pass of every cube in shared launches, replayed as a captured graph)."""
+    specs = S.uneven_mix(n)
+    t0 = time.perf_counter()
+    probs = [S.Problem(ctx, s) for s in specs]
+    log("mix setup %.1f s (%d cubes, %.1f GB)" % (time.perf_counter() - t0, n,
+                                                 sum(S.footprint(s) for s in specs) / 1e9))
+    cells = sum(S.cells(s) for s in specs)
+    for p in probs:  # warm: twiddle tables and plans of every size
+        p.step(realisations=0)
+    ctx.sync()
+    plain = []
+    for _ in range(reps):
+        t = time.perf_counter()
+        for p in probs:
+            p.step(realisations=0)
+            ctx.sync()
+        plain.append(time.perf_counter() - t)
+    plain_means = [np.stack([g.download_real() for g in p.mean_w]) for p in probs]
+    b = C.Batch(ctx, [p.item(realisations=0) for p in probs])
+    b.use_graph(True)
+    b.run()  # captures the graph
+    b.wait()
+    batched = []
+    for _ in range(reps):
+        t = time.perf_counter()
+        b.run()
+        b.wait()
+        batched.append(time.perf_counter() - t)
+    err = 0.0
+    for p, pm in zip(probs, plain_means):
+        bm = np.stack([g.download_real() for g in p.mean_w])
+        bg = np.asarray(S.BACKGROUND, dtype=np.float32)[:, None, None, None]
+        err = max(err, rel_l2(bm - bg, pm - bg))
+    b.close()
+    for p in probs:
+        p.close()
+    tp, tb = min(plain), min(batched)
+    sizes = sorted((s["nx"] * s["ny"] * s["nz"], s["nxp"], s["nyp"], s["nzp"]) for s in specs)
+    return {
+        "workload": "%d inversions, sizes log-uniform 32x32x64..256x256x192 (padded %dx%dx%d..%dx%dx%d by the "
+                    "reference rule), 3-6 angles, peak U(20,40) Hz, std U(0.05,0.2), lateral range "
+                    "U(300,2000) m; posterior mean + covariance, FP32 with FP64 re-solve where cond > 1e6"
+                    % ((n,) + sizes[0][1:] + sizes[-1][1:]),
+        "cells": cells,
+        "plain_ms": round(1000 * tp, 3), "batched_ms": round(1000 * tb, 3),
+        "plain_cells_per_s": cells / tp, "batched_cells_per_s": cells / tb, "speedup": tp / tb,
+        "max_rel_l2_mean_batched_vs_plain": err,
+        "plain": "each request: its FFT/posterior launches on one stream, then a synchronisation",
+        "batched": "one cg_batch of all requests, replayed as a captured HIP graph",
+    }
+
+
+def bench_serving(n=48, lanes=3, load=0.7):
+    """Continuous batching under Poisson arrivals (crava_amd.serve): requests of log-uniform sizes
+    64^2 x 64 .. 384^2 x 256, 3-8 angles, 0-10 realisations, admitted against a 200 GB HBM
+    budget.  A first pass with every request arriving at once measures the server's capacity
+    (and builds every FFT table); the reported pass offers Poisson arrivals at `load` times that
+    capacity.  Latency = completion - arrival (queueing, admission, input generation on the
+    device, batched compute)."""
+    from crava_amd import serve
+    specs = S.serving_requests(n)
+    ex = serve.GpuExecutor(lanes=lanes)
+    try:
+        srv = serve.Server(ex, hbm_budget=200e9, lanes=lanes, max_batch=16)
+        cap = serve.run_open_loop(srv, specs, np.zeros(n))
+        rate = load * cap["requests"] / cap["seconds"]
+        srv = serve.Server(ex, hbm_budget=200e9, lanes=lanes, max_batch=16)
+        rep = serve.run_open_loop(srv, specs, serve.poisson_arrivals(n, rate, seed=7))
+        ingest = np.mean([r.ready - r.admitted for r in srv.done]) * 1e3
+    finally:
+        ex.close()
+    return {
+        "workload": "%d requests, sizes log-uniform 64x64x64..384x384x256 (padded by the reference rule), 3-8 "
+                    "angles, 0-10 realisations, Poisson arrivals at %.1f req/s (%.0f%% of capacity), 200 GB HBM "
+                    "budget, %d lanes" % (n, rate, 100 * load, lanes),
+        "cells_per_s": rep["cells_per_s"], "latency_ms": rep["latency_ms"], "requests": rep["requests"],
+        "batches": rep["batches"], "mean_batch": rep["mean_batch"], "peak_admitted_gb": rep["peak_admitted_gb"],
+        "mean_ingest_ms": float(ingest),
+        "capacity": {"cells_per_s": cap["cells_per_s"], "requests_per_s": cap["requests"] / cap["seconds"],
+                     "latency_ms": cap["latency_ms"]},
     }
 
 
@@ -211,6 +232,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--realisations", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-mix", action="store_true", help="skip the batched-vs-plain uneven mix")
+    ap.add_argument("--no-serve", action="store_true", help="skip the continuous-batching serving run")
     ap.add_argument("--profile-steps", type=int, default=2)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -221,7 +244,7 @@ def main():
     R = args.realisations
     ctx = C.Context(0)
     t0 = time.perf_counter()
-    inv = Inversion(ctx, w, R)
+    inv = S.Problem(ctx, S.spec(**w, realisations=R))
     log("setup %.1f s" % (time.perf_counter() - t0))
     for _ in range(args.warmup):
         inv.step()
@@ -278,6 +301,7 @@ def main():
     achieved = per_launch_bytes / avg_s
     G = (w["nxp"] // 2 + 1) * w["nyp"] * w["nzp"] * 8
     model_bytes = G * (5 * (3 + inv.nt) + 21 + 24 * R)
+    model_bytes_r0 = G * (5 * (3 + inv.nt) + 21)
     model_min = G * ((3 + inv.nt) + 9 + 3 * R)
     result = {
         "metric": "posterior cells/s, 512²×256×5 angles, batched vs plain; p50/p99 latency",
@@ -302,12 +326,21 @@ def main():
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                        "samples": len(lat), "kind": "plain single request, synchronised"},
         "mean_cov_only_ms": ms_meancov,
+        "mean_cov_model_v1": {"bytes": model_bytes_r0,
+                              "frac_of_roofline": model_bytes_r0 / (ms_meancov * 1e-3) / HBM_PEAK},
         "model_v1": {"bytes": model_bytes, "frac_of_roofline": model_bytes / (ms_per_step * 1e-3) / HBM_PEAK,
                      "bytes_min": model_min},
         "kernels": kernels,
         "posterior_stats": {"inverted": stats[0], "cholesky_failures": stats[1], "fp64_resolves": stats[2]},
         "cpu_baseline": None,
     }
+    inv.close()
+    if not args.no_mix:
+        log("uneven mix: plain vs batched ...")
+        result["batched"] = bench_mix(ctx)
+    if not args.no_serve:
+        log("serving ...")
+        result["serving"] = bench_serving()
     if not args.no_cpu:
         log("cpu baseline ...")
         try:

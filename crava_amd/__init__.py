@@ -83,7 +83,7 @@ def lib():
             "cg_grid_download_real": [_vp, _f_p], "cg_grid_download_complex": [_vp, _f_p],
             "cg_grid_get_complex": [_vp, i, i, i, i, _f_p], "cg_grid_set_complex": [_vp, i, i, i, i, _f_p],
             "cg_fft_r2c_inplace": [_vp], "cg_fft_c2r_inplace": [_vp],
-            "cg_grid_add": [_vp, _vp], "cg_grid_multiply_by_scalar": [_vp, f],
+            "cg_grid_add": [_vp, _vp], "cg_grid_multiply_by_scalar": [_vp, f], "cg_grid_fill_constant": [_vp, f],
             "cg_posterior": [_vp, ctypes.POINTER(PostArgs)], "cg_simulate": [_vp, ctypes.POINTER(SimArgs)],
             "cg_grid_fill_noise_counter": [_vp, u64, ctypes.c_uint32],
             "cg_build_kz_tables": [i, i, d, i, _f_p, _f_p, f, f, f, _f_p, _f_p, _f_p, _f_p, _i_p],
@@ -93,11 +93,15 @@ def lib():
             "cg_grid_add_white_noise": [_vp, f, u64, ctypes.c_uint32],
             "cg_grid_sum_squares": [_vp, i, _d_p],
             "cg_ctx_set_fft_engine": [_vp, i],
+            "cg_arena_create": [_vp, u64, ctypes.POINTER(_vp)], "cg_arena_destroy": [_vp],
+            "cg_grid_create_in": [_vp, i, i, i, i, i, i, i, ctypes.POINTER(_vp)],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
             fn.argtypes = args
             fn.restype = ctypes.c_int
+        L.cg_arena_grid_bytes.argtypes = [i, i, i]
+        L.cg_arena_grid_bytes.restype = ctypes.c_uint64
         L.cg_last_error.argtypes = [_vp]
         L.cg_last_error.restype = ctypes.c_char_p
         _lib = L
@@ -119,6 +123,7 @@ class Context:
         self._h = _vp()
         self._grids = weakref.WeakSet()
         self._batches = weakref.WeakSet()
+        self._arenas = weakref.WeakSet()
         rc = lib().cg_ctx_create(ctypes.byref(self._h), device, hbm_budget)
         if rc != CG_OK:
             raise CravaError("cg_ctx_create failed (%d): no usable GPU %d" % (rc, device))
@@ -157,6 +162,8 @@ class Context:
                 b.close()
             for g in list(self._grids):
                 g.close()
+            for a in list(self._arenas):
+                a.close()
             lib().cg_ctx_destroy(self._h)
             self._h = _vp()
 
@@ -173,18 +180,48 @@ class Context:
         self.close()
 
 
+class Arena:
+    """One device allocation that grids are carved from (cg_arena): a request's grids cost one
+    allocation and one release.  Close every grid of the arena before the arena."""
+
+    def __init__(self, ctx, nbytes):
+        self.ctx = ctx
+        self._h = _vp()
+        ctx.check(lib().cg_arena_create(ctx._h, int(nbytes), ctypes.byref(self._h)), "cg_arena_create")
+        ctx._arenas.add(self)
+
+    @staticmethod
+    def grid_bytes(nxp, nyp, nzp):
+        return int(lib().cg_arena_grid_bytes(nxp, nyp, nzp))
+
+    def close(self):
+        if self._h and self.ctx._h:
+            self.ctx.check(lib().cg_arena_destroy(self._h), "cg_arena_destroy")
+        self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class FFTGrid:
     """Device-resident FFTGrid (src/fftgrid.h:22-285) in the reference's in-place layout."""
     DATA, PARAMETER, COVARIANCE = DATA, PARAMETER, COVARIANCE
 
-    def __init__(self, ctx, nx, ny, nz, nxp, nyp, nzp, cubetype=PARAMETER):
+    def __init__(self, ctx, nx, ny, nz, nxp, nyp, nzp, cubetype=PARAMETER, arena=None):
         self.ctx = ctx
         self.nx, self.ny, self.nz, self.nxp, self.nyp, self.nzp = nx, ny, nz, nxp, nyp, nzp
         self.cnxp = nxp // 2 + 1
         self.rnxp = 2 * self.cnxp
         self._h = _vp()
-        ctx.check(lib().cg_grid_create(ctx._h, nx, ny, nz, nxp, nyp, nzp, cubetype, ctypes.byref(self._h)),
-                  "cg_grid_create")
+        if arena is None:
+            ctx.check(lib().cg_grid_create(ctx._h, nx, ny, nz, nxp, nyp, nzp, cubetype, ctypes.byref(self._h)),
+                      "cg_grid_create")
+        else:
+            ctx.check(lib().cg_grid_create_in(arena._h, nx, ny, nz, nxp, nyp, nzp, cubetype, ctypes.byref(self._h)),
+                      "cg_grid_create_in")
         ctx._grids.add(self)
 
     # -- state
@@ -225,6 +262,10 @@ class FFTGrid:
 
     def multiplyByScalar(self, s):
         self.ctx.check(lib().cg_grid_multiply_by_scalar(self._h, s), "multiplyByScalar")
+
+    def fillInConstant(self, value):
+        """FFTGrid::fillInConstant: value on the cube, RMISSING on the row padding (real domain)."""
+        self.ctx.check(lib().cg_grid_fill_constant(self._h, value), "fillInConstant")
 
     def fillInComplexNoise(self, seed, stream=0):
         self.ctx.check(lib().cg_grid_fill_noise_counter(self._h, seed, stream), "fillInComplexNoise")

@@ -4,6 +4,7 @@
 // a device, a HIP stream, a device-memory budget, twiddle tables and staging buffers.  Grids
 // are device-resident in the reference's in-place FFTW layout.
 #include <algorithm>
+#include <mutex>
 #include <cmath>
 #include <complex>
 #include <cstring>
@@ -156,10 +157,28 @@ int hip_fail(cg_ctx *ctx, hipError_t e, const char *what)
   return CG_EHIP;
 }
 
+hipError_t dev_alloc(cg_ctx *ctx, void **p, size_t bytes)
+{
+  return ctx->async_alloc ? hipMallocAsync(p, bytes, ctx->stream) : hipMalloc(p, bytes);
+}
+
+void dev_free(cg_ctx *ctx, void *p)
+{
+  if (!p) return;
+  if (ctx->async_alloc) (void)hipFreeAsync(p, ctx->stream);
+  else (void)hipFree(p);
+}
+
+// Twiddle tables exp(-2 pi i m / n), m < n, shared by every context of a device (immutable,
+// built once per length and kept until the process ends).
 const float2 *twiddle_table(cg_ctx *ctx, int n)
 {
-  auto it = ctx->twiddles.find(n);
-  if (it != ctx->twiddles.end()) return it->second;
+  static std::mutex mu;
+  static std::map<std::pair<int, int>, float2 *> tables;
+  std::lock_guard<std::mutex> lock(mu);
+  const auto key = std::make_pair(ctx->device, n);
+  auto it = tables.find(key);
+  if (it != tables.end()) return it->second;
   std::vector<float2> h(n);
   for (int m = 0; m < n; ++m) {
     const double a = -2.0 * M_PI * (double)m / (double)n;
@@ -171,16 +190,13 @@ const float2 *twiddle_table(cg_ctx *ctx, int n)
     hip_fail(ctx, e, "hipMalloc(twiddles)");
     return nullptr;
   }
-  e = hipMemcpyAsync(d, h.data(), sizeof(float2) * n, hipMemcpyHostToDevice, ctx->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  e = hipMemcpy(d, h.data(), sizeof(float2) * n, hipMemcpyHostToDevice);
   if (e != hipSuccess) {
     hip_fail(ctx, e, "hipMemcpy(twiddles)");
     (void)hipFree(d);
     return nullptr;
   }
-  ctx->twiddles[n] = d;
-  ctx->in_use += sizeof(float2) * n;
-  ctx->peak = std::max(ctx->peak, ctx->in_use);
+  tables[key] = d;
   return d;
 }
 
@@ -247,6 +263,17 @@ int cg_ctx_create(cg_ctx **out, int device, uint64_t hbm_budget_bytes)
   DeviceGuard g(device);
   hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipMalloc(&ctx->counters, 256 * sizeof(uint64_t));
+  if (e == hipSuccess) {
+    int pools = 0;
+    if (hipDeviceGetAttribute(&pools, hipDeviceAttributeMemoryPoolsSupported, device) == hipSuccess && pools) {
+      hipMemPool_t pool;
+      if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess) {
+        // keep freed blocks in the pool instead of unmapping them at every synchronisation
+        uint64_t keep = UINT64_MAX;
+        ctx->async_alloc = hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep) == hipSuccess;
+      }
+    }
+  }
   if (e != hipSuccess) {
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -266,11 +293,24 @@ int cg_ctx_destroy(cg_ctx *ctx)
     (void)hipEventDestroy(t.stop);
   }
   for (cg_grid *gr : ctx->grids) {
-    (void)hipFree(gr->d);
+    if (!gr->arena) cg::dev_free(ctx, gr->d);
     delete gr;
   }
   ctx->grids.clear();
-  for (auto &kv : ctx->twiddles) (void)hipFree(kv.second);
+  for (cg_arena *a : ctx->arenas) {
+    cg::dev_free(ctx, a->base);
+    delete a;
+  }
+  ctx->arenas.clear();
+  (void)hipStreamSynchronize(ctx->stream);
+  for (int k = 0; k < cg_ctx::kSide; ++k) {
+    if (ctx->side[k]) {
+      (void)hipStreamSynchronize(ctx->side[k]);
+      (void)hipStreamDestroy(ctx->side[k]);
+    }
+    if (ctx->join[k]) (void)hipEventDestroy(ctx->join[k]);
+  }
+  if (ctx->fork) (void)hipEventDestroy(ctx->fork);
   if (ctx->tables) (void)hipFree(ctx->tables);
   if (ctx->counters) (void)hipFree(ctx->counters);
   if (ctx->stage_host) (void)hipHostFree(ctx->stage_host);
@@ -359,7 +399,15 @@ int cg_ctx_reset_kernel_time(cg_ctx *ctx)
 }
 
 // ---------------------------------------------------------------------------- grids
-int cg_grid_create(cg_ctx *ctx, int nx, int ny, int nz, int nxp, int nyp, int nzp, int cubetype, cg_grid **out)
+namespace {
+
+size_t grid_alloc_bytes(int nxp, int nyp, int nzp)
+{
+  return align_up((size_t)2 * (size_t)(nxp / 2 + 1) * (size_t)nyp * (size_t)nzp * sizeof(float), 256);
+}
+
+int grid_create(cg_ctx *ctx, cg_arena *arena, int nx, int ny, int nz, int nxp, int nyp, int nzp, int cubetype,
+                cg_grid **out)
 {
   if (!ctx || !out) return CG_EINVAL;
   *out = nullptr;
@@ -378,26 +426,100 @@ int cg_grid_create(cg_ctx *ctx, int nx, int ny, int nz, int nxp, int nyp, int nz
   g->csize = (size_t)g->cnxp * nyp * nzp;
   g->rsize = 2 * g->csize;
   const size_t bytes = g->rsize * sizeof(float);
-  if (ctx->budget && ctx->in_use + bytes > ctx->budget) {
-    delete g;
-    return cg::fail(ctx, CG_ENOMEM, "cg_grid_create: HBM budget of the context exceeded");
-  }
   DeviceGuard dg(ctx->device);
-  hipError_t e = hipMalloc(&g->d, bytes);
-  if (e != hipSuccess) {
-    delete g;
-    return cg::fail(ctx, CG_ENOMEM, std::string("cg_grid_create: hipMalloc: ") + hipGetErrorString(e));
+  hipError_t e;
+  if (arena) {
+    const size_t need = grid_alloc_bytes(nxp, nyp, nzp);
+    if (arena->used + need > arena->bytes) {
+      delete g;
+      return cg::fail(ctx, CG_ENOMEM, "cg_grid_create_in: arena full");
+    }
+    g->d = reinterpret_cast<float *>(arena->base + arena->used);
+    g->arena = arena;
+    arena->used += need;
+    arena->live += 1;
+  } else {
+    if (ctx->budget && ctx->in_use + bytes > ctx->budget) {
+      delete g;
+      return cg::fail(ctx, CG_ENOMEM, "cg_grid_create: HBM budget of the context exceeded");
+    }
+    e = cg::dev_alloc(ctx, reinterpret_cast<void **>(&g->d), bytes);
+    if (e != hipSuccess) {
+      delete g;
+      return cg::fail(ctx, CG_ENOMEM, std::string("cg_grid_create: allocation: ") + hipGetErrorString(e));
+    }
+    ctx->in_use += bytes;
+    ctx->peak = std::max(ctx->peak, ctx->in_use);
   }
   e = hipMemsetAsync(g->d, 0, bytes, ctx->stream);
   if (e != hipSuccess) {
-    (void)hipFree(g->d);
+    if (arena) {
+      arena->live -= 1;
+    } else {
+      cg::dev_free(ctx, g->d);
+      ctx->in_use -= bytes;
+    }
     delete g;
     return cg::hip_fail(ctx, e, "hipMemsetAsync");
   }
-  ctx->in_use += bytes;
-  ctx->peak = std::max(ctx->peak, ctx->in_use);
   ctx->grids.insert(g);
   *out = g;
+  return CG_OK;
+}
+
+}  // namespace
+
+int cg_grid_create(cg_ctx *ctx, int nx, int ny, int nz, int nxp, int nyp, int nzp, int cubetype, cg_grid **out)
+{
+  return grid_create(ctx, nullptr, nx, ny, nz, nxp, nyp, nzp, cubetype, out);
+}
+
+int cg_grid_create_in(cg_arena *arena, int nx, int ny, int nz, int nxp, int nyp, int nzp, int cubetype,
+                      cg_grid **out)
+{
+  if (!arena) return CG_EINVAL;
+  return grid_create(arena->ctx, arena, nx, ny, nz, nxp, nyp, nzp, cubetype, out);
+}
+
+uint64_t cg_arena_grid_bytes(int nxp, int nyp, int nzp)
+{
+  if (nxp < 1 || nyp < 1 || nzp < 1) return 0;
+  return grid_alloc_bytes(nxp, nyp, nzp);
+}
+
+int cg_arena_create(cg_ctx *ctx, uint64_t bytes, cg_arena **out)
+{
+  if (!ctx || !out || bytes == 0) return CG_EINVAL;
+  *out = nullptr;
+  if (ctx->budget && ctx->in_use + bytes > ctx->budget)
+    return cg::fail(ctx, CG_ENOMEM, "cg_arena_create: HBM budget of the context exceeded");
+  DeviceGuard dg(ctx->device);
+  cg_arena *a = new (std::nothrow) cg_arena();
+  if (!a) return CG_ENOMEM;
+  a->ctx = ctx;
+  a->bytes = bytes;
+  hipError_t e = cg::dev_alloc(ctx, reinterpret_cast<void **>(&a->base), bytes);
+  if (e != hipSuccess) {
+    delete a;
+    return cg::fail(ctx, CG_ENOMEM, std::string("cg_arena_create: allocation: ") + hipGetErrorString(e));
+  }
+  ctx->in_use += bytes;
+  ctx->peak = std::max(ctx->peak, ctx->in_use);
+  ctx->arenas.insert(a);
+  *out = a;
+  return CG_OK;
+}
+
+int cg_arena_destroy(cg_arena *a)
+{
+  if (!a) return CG_EINVAL;
+  cg_ctx *ctx = a->ctx;
+  if (a->live) return cg::fail(ctx, CG_ESTATE, "cg_arena_destroy: grids of the arena are still alive");
+  DeviceGuard dg(ctx->device);
+  cg::dev_free(ctx, a->base);
+  ctx->in_use -= a->bytes;
+  ctx->arenas.erase(a);
+  delete a;
   return CG_OK;
 }
 
@@ -406,9 +528,13 @@ int cg_grid_destroy(cg_grid *g)
   if (!g) return CG_EINVAL;
   cg_ctx *ctx = g->ctx;
   DeviceGuard dg(ctx->device);
-  (void)hipStreamSynchronize(ctx->stream);
-  (void)hipFree(g->d);
-  ctx->in_use -= g->rsize * sizeof(float);
+  if (g->arena) {
+    g->arena->live -= 1;  // the arena owns the memory
+  } else {
+    if (!ctx->async_alloc) (void)hipStreamSynchronize(ctx->stream);
+    cg::dev_free(ctx, g->d);  // stream-ordered: after the work already enqueued on ctx->stream
+    ctx->in_use -= g->rsize * sizeof(float);
+  }
   ctx->grids.erase(g);
   delete g;
   return CG_OK;
@@ -559,6 +685,15 @@ int cg_grid_multiply_by_scalar(cg_grid *g, float s)
   if (g->transformed) return cg::fail(g->ctx, CG_ESTATE, "multiplyByScalar: grid is in the Fourier domain");
   DeviceGuard dg(g->ctx->device);
   return cg::scale_real(g->ctx, g->d, g->rsize, s);
+}
+
+int cg_grid_fill_constant(cg_grid *g, float value)
+{
+  if (!g) return CG_EINVAL;
+  DeviceGuard dg(g->ctx->device);
+  const int rc = cg::fill_constant(g->ctx, g->d, g->rsize, g->nxp, g->rnxp, value);
+  if (rc == CG_OK) g->transformed = 0;
+  return rc;
 }
 
 int cg_grid_fill_noise_counter(cg_grid *g, uint64_t seed, uint32_t stream)
@@ -910,11 +1045,21 @@ struct cg_batch {
   std::vector<std::vector<cg_grid *>> outs;
   void *dev = nullptr;
   size_t dev_bytes = 0;
+  // One launch of a compile-time-plan kernel over every grid of the pass that has that length.
+  struct FixedGroup {
+    const cg::FixedEntry *e = nullptr;
+    const void *items = nullptr;  // cg::StridedArgs or cg::XArgs, device memory
+    int nitems = 0, nxp = 0;
+    unsigned blocks = 0;
+  };
+  // A pass over many grids: one launch per compiled length, and one runtime-plan launch for the
+  // grids whose length has no compiled plan.
   struct Pass {
     int kind = 0, dir = -1, kclass = 0, nitems = 0;
     const cg::FftItem *items = nullptr;
     unsigned blocks = 0;
     size_t lds = 0;
+    std::vector<FixedGroup> fixed;
   };
   std::vector<Pass> fwd, inv_mean;
   std::vector<std::vector<Pass>> sim;  // per realisation: z, y, x passes
@@ -952,21 +1097,106 @@ struct Layout {
   }
 };
 
+// Side streams (default priority) of a context, created once: the launches of one batched pass
+// (one per compiled length) and the posterior groups are independent and run concurrently;
+// each pass forks from and joins back into the context stream, so the order of passes is kept
+// (also inside a captured graph).
+int ensure_side_streams(cg_ctx *ctx)
+{
+  if (ctx->fork) return CG_OK;
+  hipError_t e = hipSuccess;
+  for (int k = 0; k < cg_ctx::kSide && e == hipSuccess; ++k) {
+    if (!ctx->side[k]) e = hipStreamCreateWithFlags(&ctx->side[k], hipStreamNonBlocking);
+    if (e == hipSuccess && !ctx->join[k]) e = hipEventCreateWithFlags(&ctx->join[k], hipEventDisableTiming);
+  }
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming);
+  return cg::hip_fail(ctx, e, "side streams");
+}
+
+// Runs launch(i) for i < n: on the context stream when n < 2, else spread over the side streams
+// between a fork from and a join into the context stream.
+template <typename F>
+int fork_join(cg_batch *b, int n, F launch)
+{
+  cg_ctx *ctx = b->ctx;
+  if (n < 2 || !ctx->fork) {
+    for (int i = 0; i < n; ++i)
+      if (int rc = launch(i)) return rc;
+    return CG_OK;
+  }
+  hipStream_t main = ctx->stream;
+  const int ns = std::min(n, cg_ctx::kSide);
+  hipError_t e = hipEventRecord(ctx->fork, main);
+  for (int k = 0; k < ns && e == hipSuccess; ++k) e = hipStreamWaitEvent(ctx->side[k], ctx->fork, 0);
+  if (e != hipSuccess) return cg::hip_fail(ctx, e, "batch fork");
+  int rc = CG_OK;
+  for (int i = 0; i < n && !rc; ++i) {
+    ctx->stream = ctx->side[i % ns];
+    rc = launch(i);
+  }
+  ctx->stream = main;
+  for (int k = 0; k < ns && e == hipSuccess; ++k) {
+    e = hipEventRecord(ctx->join[k], ctx->side[k]);
+    if (e == hipSuccess) e = hipStreamWaitEvent(main, ctx->join[k], 0);
+  }
+  if (rc) return rc;
+  return cg::hip_fail(ctx, e, "batch join");
+}
+
+int run_pass(cg_batch *b, const cg_batch::Pass &p)
+{
+  cg_ctx *ctx = b->ctx;
+  int rc = fork_join(b, (int)p.fixed.size(), [&](int i) {
+    const auto &g = p.fixed[i];
+    hipEvent_t ev;
+    cg::launch_begin(ctx, p.kclass, &ev);
+    int r;
+    if (p.kind == cg::FFT_ITEM_STRIDED)
+      r = cg::fixed_strided_list(ctx, g.e, p.dir, static_cast<const cg::StridedArgs *>(g.items), g.nitems,
+                                 g.blocks);
+    else
+      r = cg::fixed_x_list(ctx, g.e, p.kind == cg::FFT_ITEM_X_R2C, static_cast<const cg::XArgs *>(g.items),
+                           g.nitems, g.blocks, g.nxp);
+    cg::launch_end(ctx, p.kclass, ev);
+    return r;
+  });
+  if (rc) return rc;
+  return cg::launch_fft_batch(ctx, p.kind, p.dir, p.items, p.nitems, p.blocks, p.lds, p.kclass);
+}
+
+// Waits for the batch's work and frees everything it holds.
+void release_batch(cg_batch *b)
+{
+  cg_ctx *ctx = b->ctx;
+  (void)hipStreamSynchronize(ctx->stream);  // the side streams join into it
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->done) (void)hipEventDestroy(b->done);
+  if (b->dev) {
+    cg::dev_free(ctx, b->dev);
+    ctx->in_use -= b->dev_bytes;
+  }
+  delete b;
+}
+
 int enqueue_batch(cg_batch *b)
 {
   cg_ctx *ctx = b->ctx;
   int rc = CG_OK;
   if (b->ncounters) rc = cg::zero_counters(ctx, b->counters, b->ncounters);
   for (auto &p : b->fwd)
-    if (!rc) rc = cg::launch_fft_batch(ctx, p.kind, p.dir, p.items, p.nitems, p.blocks, p.lds, p.kclass);
-  for (auto &g : b->post)
-    if (!rc) rc = cg::launch_posterior_batch(ctx, g.nt, g.sep, g.mixed, g.items, g.nitems, g.blocks);
+    if (!rc) rc = run_pass(b, p);
+  if (!rc)
+    rc = fork_join(b, (int)b->post.size(), [&](int i) {
+      const auto &g = b->post[i];
+      return cg::launch_posterior_batch(ctx, g.nt, g.sep, g.mixed, g.items, g.nitems, g.blocks);
+    });
   for (auto &p : b->inv_mean)
-    if (!rc) rc = cg::launch_fft_batch(ctx, p.kind, p.dir, p.items, p.nitems, p.blocks, p.lds, p.kclass);
+    if (!rc) rc = run_pass(b, p);
   for (size_t r = 0; r < b->sim.size(); ++r) {
     if (!rc) rc = cg::colour_batch(ctx, b->colour[r].items, b->colour[r].nitems, b->colour[r].blocks);
     for (auto &p : b->sim[r])
-      if (!rc) rc = cg::launch_fft_batch(ctx, p.kind, p.dir, p.items, p.nitems, p.blocks, p.lds, p.kclass);
+      if (!rc) rc = run_pass(b, p);
   }
   return rc;
 }
@@ -1001,7 +1231,8 @@ int cg_batch_create(cg_ctx *ctx, int n, const cg_inversion *inv, cg_batch **out)
     if (v.nreal < 0 || (v.nreal > 0 && !v.out)) return cg::fail(ctx, CG_EINVAL, "batch: bad realisations");
     for (int r = 0; r < 3 * v.nreal; ++r) all.push_back(v.out[r]);
     for (const cg_grid *g : all) {
-      if (!g || g->ctx != ctx || !same_shape(g, ref)) return cg::fail(ctx, CG_EINVAL, "batch: bad grid");
+      if (!g || g->ctx->device != ctx->device || !same_shape(g, ref))
+        return cg::fail(ctx, CG_EINVAL, "batch: bad grid");
     }
     for (int g = 0; g < 3; ++g)
       if (v.mean_in[g]->transformed) return cg::fail(ctx, CG_ESTATE, "batch: inputs must be in the real domain");
@@ -1091,14 +1322,56 @@ int cg_batch_create(cg_ctx *ctx, int n, const cg_inversion *inv, cg_batch **out)
   }
   // ---- layout of the device area
   Layout L;
-  auto off_items = [&](std::vector<cg::FftItem> &v) { return L.add(sizeof(cg::FftItem) * v.size()); };
-  const size_t o_fx = off_items(fx), o_fy = off_items(fy), o_fz = off_items(fz);
-  const size_t o_iz = off_items(iz), o_iy = off_items(iy), o_ix = off_items(ix);
-  std::vector<size_t> o_sz(maxR), o_sy(maxR), o_sx(maxR), o_col(maxR);
+  // Each pass: grids grouped by compiled length (one launch per group), the rest left to the
+  // runtime-plan batch kernel.
+  struct PassBuild {
+    int kind = 0;
+    std::vector<cg::FftItem> generic;
+    size_t off_generic = 0;
+    struct G {
+      const cg::FixedEntry *e;
+      std::vector<cg::FftItem> items;
+      size_t off;
+    };
+    std::vector<G> groups;
+  };
+  auto plan_pass = [&](const std::vector<cg::FftItem> &v, int kind) {
+    PassBuild pb;
+    pb.kind = kind;
+    std::map<const cg::FixedEntry *, size_t> index;
+    for (const auto &it : v) {
+      const cg::FixedEntry *e = nullptr;
+      if (ctx->fft_engine == CG_FFT_AUTO) {
+        if (kind == cg::FFT_ITEM_STRIDED) e = cg::fixed_find(cg::FIX_STRIDED, it.n);
+        else if (it.nxp % 2 == 0) e = cg::fixed_find(cg::FIX_XE, it.nxp / 2);
+        else e = cg::fixed_find(cg::FIX_XO, it.nxp);
+      }
+      if (!e) {
+        pb.generic.push_back(it);
+        continue;
+      }
+      auto f = index.find(e);
+      if (f == index.end()) {
+        f = index.emplace(e, pb.groups.size()).first;
+        pb.groups.push_back({e, {}, 0});
+      }
+      pb.groups[f->second].items.push_back(it);
+    }
+    pb.off_generic = L.add(sizeof(cg::FftItem) * pb.generic.size());
+    const size_t esz = kind == cg::FFT_ITEM_STRIDED ? sizeof(cg::StridedArgs) : sizeof(cg::XArgs);
+    for (auto &g : pb.groups) g.off = L.add(esz * g.items.size());
+    return pb;
+  };
+  PassBuild o_fx = plan_pass(fx, cg::FFT_ITEM_X_R2C), o_fy = plan_pass(fy, cg::FFT_ITEM_STRIDED),
+            o_fz = plan_pass(fz, cg::FFT_ITEM_STRIDED);
+  PassBuild o_iz = plan_pass(iz, cg::FFT_ITEM_STRIDED), o_iy = plan_pass(iy, cg::FFT_ITEM_STRIDED),
+            o_ix = plan_pass(ix, cg::FFT_ITEM_X_C2R);
+  std::vector<PassBuild> o_sz(maxR), o_sy(maxR), o_sx(maxR);
+  std::vector<size_t> o_col(maxR);
   for (int r = 0; r < maxR; ++r) {
-    o_sz[r] = off_items(sz[r]);
-    o_sy[r] = off_items(sy[r]);
-    o_sx[r] = off_items(sx[r]);
+    o_sz[r] = plan_pass(sz[r], cg::FFT_ITEM_STRIDED);
+    o_sy[r] = plan_pass(sy[r], cg::FFT_ITEM_STRIDED);
+    o_sx[r] = plan_pass(sx[r], cg::FFT_ITEM_X_C2R);
     int cnt = 0;
     for (int i = 0; i < n; ++i) cnt += b->inv[i].nreal > r;
     col_count[r] = cnt;
@@ -1135,7 +1408,7 @@ int cg_batch_create(cg_ctx *ctx, int n, const cg_inversion *inv, cg_batch **out)
   std::vector<size_t> o_groups;
   for (auto &kv : groups) o_groups.push_back(L.add(sizeof(cg::PostItem) * kv.second.size()));
   // ---- allocate
-  hipError_t e = hipMalloc(&b->dev, L.size);
+  hipError_t e = cg::dev_alloc(ctx, &b->dev, L.size);
   if (e != hipSuccess) {
     delete b;
     return cg::fail(ctx, CG_ENOMEM, std::string("batch: hipMalloc: ") + hipGetErrorString(e));
@@ -1149,32 +1422,62 @@ int cg_batch_create(cg_ctx *ctx, int n, const cg_inversion *inv, cg_batch **out)
     if (bytes) std::memcpy(host.data() + off, src, bytes);
   };
   // ---- passes: block offsets, copy items
-  auto finish = [&](std::vector<cg::FftItem> &v, size_t off, int kind, int dir, int kclass, size_t lds) {
+  auto finish = [&](PassBuild &pb, int dir, int kclass, size_t lds) {
     cg_batch::Pass p;
     unsigned blk = 0;
-    for (auto &it : v) {
+    for (auto &it : pb.generic) {
       it.block0 = blk;
       blk += it.nblocks;
     }
-    put(off, v.data(), sizeof(cg::FftItem) * v.size());
-    p.kind = kind; p.dir = dir; p.kclass = kclass; p.nitems = (int)v.size();
-    p.items = reinterpret_cast<const cg::FftItem *>(dev + off);
+    put(pb.off_generic, pb.generic.data(), sizeof(cg::FftItem) * pb.generic.size());
+    p.kind = pb.kind; p.dir = dir; p.kclass = kclass; p.nitems = (int)pb.generic.size();
+    p.items = reinterpret_cast<const cg::FftItem *>(dev + pb.off_generic);
     p.blocks = blk;
     p.lds = lds;
+    for (auto &g : pb.groups) {
+      cg_batch::FixedGroup fg;
+      fg.e = g.e;
+      fg.nitems = (int)g.items.size();
+      fg.items = dev + g.off;
+      unsigned gb = 0;
+      if (pb.kind == cg::FFT_ITEM_STRIDED) {
+        std::vector<cg::StridedArgs> a;
+        for (const auto &it : g.items) {
+          const unsigned long ncols = (unsigned long)it.inner * (unsigned long)it.outer;
+          a.push_back({it.data, it.S, it.OS, (unsigned)it.inner, (unsigned)ncols, gb, it.scale});
+          gb += (unsigned)((ncols + g.e->tile - 1) / g.e->tile);
+        }
+        put(g.off, a.data(), sizeof(cg::StridedArgs) * a.size());
+      } else {
+        std::vector<cg::XArgs> a;
+        for (const auto &it : g.items) {
+          const bool fwd = pb.kind == cg::FFT_ITEM_X_R2C;
+          a.push_back({fwd ? it.src : nullptr, it.grid, fwd ? nullptr : it.add, it.rows, gb, it.scale});
+          gb += (unsigned)((it.rows + g.e->tile - 1) / g.e->tile);
+          fg.nxp = it.nxp;
+        }
+        put(g.off, a.data(), sizeof(cg::XArgs) * a.size());
+      }
+      fg.blocks = gb;
+      p.fixed.push_back(fg);
+    }
+    // largest launches first, so that the side streams finish together
+    std::stable_sort(p.fixed.begin(), p.fixed.end(),
+                     [](const cg_batch::FixedGroup &x, const cg_batch::FixedGroup &y) { return x.blocks > y.blocks; });
     return p;
   };
-  b->fwd.push_back(finish(fx, o_fx, cg::FFT_ITEM_X_R2C, -1, CG_K_FFT_X, lds_fx));
-  b->fwd.push_back(finish(fy, o_fy, cg::FFT_ITEM_STRIDED, -1, CG_K_FFT_Y, lds_fy));
-  b->fwd.push_back(finish(fz, o_fz, cg::FFT_ITEM_STRIDED, -1, CG_K_FFT_Z, lds_fz));
-  b->inv_mean.push_back(finish(iz, o_iz, cg::FFT_ITEM_STRIDED, +1, CG_K_FFT_Z, lds_iz));
-  b->inv_mean.push_back(finish(iy, o_iy, cg::FFT_ITEM_STRIDED, +1, CG_K_FFT_Y, lds_iy));
-  b->inv_mean.push_back(finish(ix, o_ix, cg::FFT_ITEM_X_C2R, +1, CG_K_FFT_X, lds_ix));
+  b->fwd.push_back(finish(o_fx, -1, CG_K_FFT_X, lds_fx));
+  b->fwd.push_back(finish(o_fy, -1, CG_K_FFT_Y, lds_fy));
+  b->fwd.push_back(finish(o_fz, -1, CG_K_FFT_Z, lds_fz));
+  b->inv_mean.push_back(finish(o_iz, +1, CG_K_FFT_Z, lds_iz));
+  b->inv_mean.push_back(finish(o_iy, +1, CG_K_FFT_Y, lds_iy));
+  b->inv_mean.push_back(finish(o_ix, +1, CG_K_FFT_X, lds_ix));
   b->sim.resize(maxR);
   b->colour.resize(maxR);
   for (int r = 0; r < maxR; ++r) {
-    b->sim[r].push_back(finish(sz[r], o_sz[r], cg::FFT_ITEM_STRIDED, +1, CG_K_FFT_Z, lds_sz[r]));
-    b->sim[r].push_back(finish(sy[r], o_sy[r], cg::FFT_ITEM_STRIDED, +1, CG_K_FFT_Y, lds_sy[r]));
-    b->sim[r].push_back(finish(sx[r], o_sx[r], cg::FFT_ITEM_X_C2R, +1, CG_K_FFT_X, lds_sx[r]));
+    b->sim[r].push_back(finish(o_sz[r], +1, CG_K_FFT_Z, lds_sz[r]));
+    b->sim[r].push_back(finish(o_sy[r], +1, CG_K_FFT_Y, lds_sy[r]));
+    b->sim[r].push_back(finish(o_sx[r], +1, CG_K_FFT_X, lds_sx[r]));
     std::vector<cg::ColourItem> ci;
     unsigned blk = 0;
     for (int i = 0; i < n; ++i) {
@@ -1262,10 +1565,9 @@ int cg_batch_create(cg_ctx *ctx, int n, const cg_inversion *inv, cg_batch **out)
   e = hipMemcpyAsync(b->dev, host.data(), L.size, hipMemcpyHostToDevice, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&b->done, hipEventDisableTiming);
+  if (e == hipSuccess && ensure_side_streams(ctx) != CG_OK) e = hipErrorOutOfMemory;
   if (e != hipSuccess) {
-    (void)hipFree(b->dev);
-    ctx->in_use -= b->dev_bytes;
-    delete b;
+    release_batch(b);
     return cg::hip_fail(ctx, e, "batch: upload of descriptors");
   }
   *out = b;
@@ -1354,17 +1656,8 @@ int cg_batch_wait(cg_batch *b)
 int cg_batch_destroy(cg_batch *b)
 {
   if (!b) return CG_EINVAL;
-  cg_ctx *ctx = b->ctx;
-  DeviceGuard dg(ctx->device);
-  (void)hipStreamSynchronize(ctx->stream);
-  if (b->exec) (void)hipGraphExecDestroy(b->exec);
-  if (b->graph) (void)hipGraphDestroy(b->graph);
-  if (b->done) (void)hipEventDestroy(b->done);
-  if (b->dev) {
-    (void)hipFree(b->dev);
-    ctx->in_use -= b->dev_bytes;
-  }
-  delete b;
+  DeviceGuard dg(b->ctx->device);
+  release_batch(b);
   return CG_OK;
 }
 

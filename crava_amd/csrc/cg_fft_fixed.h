@@ -1,0 +1,491 @@
+// cg_fft_fixed.h -- FFT passes with compile-time radix plans (templates; instantiated per length
+// by cg_fft_fixed_part.hip, dispatched by cg_fft_fixed.hip).
+//
+// Same passes and conventions as cg_fft.hip (FFTGrid::fftInPlace / invFFTInPlace,
+// fftgrid.cpp:1587-1645; FFTW pass order rfftwnd.c:207-288), but the transform length, the
+// radix sequence and the thread split are template parameters:
+//  * every sequence of length N is worked on by TPS threads holding the butterfly inputs of a
+//    stage in registers; a stage reads all its inputs from LDS, waits at a barrier, and writes
+//    its outputs back in place (Stockham autosort), so one LDS buffer suffices;
+//  * index arithmetic (j mod Ns, j / Ns, strides) folds to shifts and multiplies;
+//  * every load of a tile is issued before the first LDS store (N / TPS requests in flight per
+//    thread), which is what keeps HBM busy;
+//  * one kernel serves one grid or a list of grids of the same length (descriptor per grid,
+//    blocks [block0_i, block0_{i+1}) belong to grid i), so a batch of cubes of mixed sizes
+//    needs one launch per (pass, length).
+// The plans (n, TPS, radices) come from cg_fft_plans.h.
+#pragma once
+
+#include "cg_dft.h"
+#include "cg_fft_plans.h"
+#include "cg_internal.h"
+
+namespace cgfix {
+
+using namespace cgdft;
+
+// ---------------------------------------------------------------- radix stages
+template <int N, int DIR, int TPS, int ES, int NS, int... RS>
+struct Stages;
+
+template <int N, int DIR, int TPS, int ES, int NS>
+struct Stages<N, DIR, TPS, ES, NS> {
+  static __device__ __forceinline__ void run(float2 *, int, const float2 *) {}
+};
+
+// Element e of the sequence lives at buf[e * ES].  Butterfly j (j < N / P) reads elements
+// j + q N/P, applies w_{Ns P}^{q k} (k = j mod Ns) and writes (j / Ns) Ns P + k + q Ns.
+template <int N, int DIR, int TPS, int ES, int NS, int P, int... RS>
+struct Stages<N, DIR, TPS, ES, NS, P, RS...> {
+  static __device__ __forceinline__ void run(float2 *buf, int t, const float2 *tw)
+  {
+    constexpr int NB = N / P;
+    constexpr int BPT = (NB + TPS - 1) / TPS;
+    constexpr bool EVEN = NB % TPS == 0;  // otherwise the last butterfly of some threads is idle
+    constexpr int TWS = N / (NS * P);
+    float2 v[BPT][P];
+#pragma unroll
+    for (int b = 0; b < BPT; ++b) {
+      const int j = t + b * TPS;
+      if (EVEN || j < NB) {
+#pragma unroll
+        for (int q = 0; q < P; ++q) v[b][q] = buf[(j + q * NB) * ES];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < BPT; ++b) {
+      const int j = t + b * TPS;
+      if (EVEN || j < NB) {
+        const int k = j % NS;
+        if (NS > 1) {
+#pragma unroll
+          for (int q = 1; q < P; ++q) {
+            float2 w = tw[q * k * TWS];
+            if (DIR > 0) w.y = -w.y;
+            v[b][q] = cmul(v[b][q], w);
+          }
+        }
+        dft<DIR, P>(v[b]);
+        const int d = (j / NS) * NS * P + k;
+#pragma unroll
+        for (int q = 0; q < P; ++q) buf[(d + q * NS) * ES] = v[b][q];
+      }
+    }
+    __syncthreads();
+    Stages<N, DIR, TPS, ES, NS * P, RS...>::run(buf, t, tw);
+  }
+};
+
+template <int N>
+struct Plan;
+#define CG_PLAN(n, tps, ...)                                                      \
+  template <>                                                                     \
+  struct Plan<n> {                                                                \
+    static constexpr int TPS = tps;                                               \
+    template <int DIR, int ES>                                                    \
+    static __device__ __forceinline__ void run(float2 *b, int t, const float2 *w) \
+    {                                                                             \
+      Stages<n, DIR, tps, ES, 1, __VA_ARGS__>::run(b, t, w);                      \
+    }                                                                             \
+  };
+CG_FFT_PLANS(CG_PLAN)
+#undef CG_PLAN
+
+constexpr size_t kLds = 80 * 1024;  // bytes per workgroup aimed for: two workgroups per CU
+
+// Columns per y/z tile: 16 (128 B row segments) while the tile fits kLds, else 8; more for
+// short lengths so that a workgroup has at least two wavefronts.
+template <int N>
+constexpr int strided_tile()
+{
+  constexpr int T = Plan<N>::TPS;
+  int ti = ((size_t)N * 17 * 8 <= kLds) ? 16 : 8;
+  while (ti < 64 && ti * T < 128 && (size_t)N * (2 * ti + 1) * 8 <= kLds) ti *= 2;
+  return ti;
+}
+
+// Rows per x tile: the most (up to 128) whose LDS (N twiddles + (N + 1)(RW + 1) slots) fits kLds
+// with at most 512 threads.
+template <int N>
+constexpr int x_rows()
+{
+  constexpr int T = Plan<N>::TPS;
+  int rw = 128;
+  while (rw > 1 && ((size_t)(N + (N + 1) * (rw + 1)) * 8 > kLds || rw * T > 512)) rw /= 2;
+  return rw;
+}
+
+__device__ __forceinline__ void load_tw(float2 *tw, const float2 *g, int n, int tid, int nthr)
+{
+  for (int m = tid; m < n; m += nthr) tw[m] = g[m];
+}
+
+template <typename A>
+__device__ __forceinline__ int find_item(const A *items, int nitems, unsigned g)
+{
+  int lo = 0, hi = nitems - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (items[mid].block0 <= g) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Resolves the grid of this workgroup: `one` for a single-grid launch, else the item that owns
+// the (XCD-remapped) block index.
+template <typename A>
+__device__ __forceinline__ unsigned resolve(const A &one, const A *items, int nitems, A &a)
+{
+  const unsigned g = cg_xcd_remap(blockIdx.x, gridDim.x);
+  if (!items) {
+    a = one;
+    return g;
+  }
+  a = items[find_item(items, nitems, g)];
+  return g - a.block0;
+}
+
+// ---------------------------------------------------------------- y / z passes
+// Columns are numbered q = c + inner * o (c < inner the contiguous index, o < outer the slab) and
+// a block takes TI consecutive q, so every tile but the grid's last is full even where inner is
+// not a multiple of TI (y pass: inner = nxp/2 + 1); a tile that crosses a slab boundary reads
+// two shorter segments.  Element e of column q lives at (q % inner) + (q / inner) OS + e S.
+template <int N, int DIR, int TI>
+__global__ void __launch_bounds__(TI *Plan<N>::TPS)
+k_ffts(cg::StridedArgs one, const cg::StridedArgs *__restrict__ items, int nitems, const float2 *__restrict__ twg)
+{
+  constexpr int NT = TI * Plan<N>::TPS;
+  constexpr int PER = (N * TI + NT - 1) / NT;  // elements per thread, all of one column
+  extern __shared__ float2 smem[];
+  float2 *tw = smem;
+  float2 *buf = smem + N;
+  cg::StridedArgs a;
+  const unsigned blk = resolve(one, items, nitems, a);
+  const int tid = threadIdx.x;
+  const int c = tid % TI;  // (tid + i NT) % TI == tid % TI: each thread stays on one column
+  const unsigned q = blk * TI + c;
+  const bool live = q < a.ncols;
+  float2 *col = a.data + (live ? (long)(q % a.inner) + (long)(q / a.inner) * a.OS : 0);
+  load_tw(tw, twg, N, tid, NT);
+  {
+    float2 v[PER];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int idx = tid + i * NT;
+      v[i] = (live && idx < N * TI) ? col[(long)(idx / TI) * a.S] : make_float2(0.f, 0.f);
+    }
+#pragma unroll
+    for (int i = 0; i < PER; ++i)
+      if (tid + i * NT < N * TI) buf[tid + i * NT] = v[i];
+  }
+  __syncthreads();
+  Plan<N>::template run<DIR, TI>(buf + c, tid / TI, tw);
+  if (live) {
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int idx = tid + i * NT;
+      if (idx < N * TI) col[(long)(idx / TI) * a.S] = cscale(buf[idx], a.scale);
+    }
+  }
+}
+
+// ---------------------------------------------------------------- x pass
+// A block takes RW consecutive rows of the in-place layout (CN complex slots = 2 CN floats per
+// row), one contiguous span of global memory copied in and out flat (coalesced).  In LDS the
+// rows are interleaved, slot m of row r at m * ES + r with odd ES = RW + 1: the FFT stages
+// address consecutive rows from consecutive lanes, and the flat copies stride by ES (odd, so
+// free of bank conflicts too).
+
+// Even nxp = 2N.  Forward: z[m] = x[2m] + i x[2m+1] (scaled), Z = FFT_N(z),
+//   X[k] = (Z[k] + conj Z[N-k]) / 2 - i w^k (Z[k] - conj Z[N-k]) / 2,  w = exp(-2 pi i / 2N);
+// bins k and N - k come from the same pair of slots, so each work item rewrites its own pair.
+template <int N, int RW>
+__global__ void __launch_bounds__(RW *Plan<N>::TPS)
+k_fftx_r2c(cg::XArgs one, const cg::XArgs *__restrict__ items, int nitems, const float2 *__restrict__ twg,
+           const float2 *__restrict__ tw_row)
+{
+  constexpr int TPS = Plan<N>::TPS;
+  constexpr int NT = RW * TPS;
+  constexpr int ES = RW + 1;
+  constexpr int CN = N + 1;  // complex slots per row
+  constexpr int PER = (RW * CN + NT - 1) / NT;
+  extern __shared__ float2 smem[];
+  float2 *tw = smem;
+  float2 *buf = smem + N;
+  cg::XArgs a;
+  const unsigned blk = resolve(one, items, nitems, a);
+  const int tid = threadIdx.x;
+  const long row0 = (long)blk * RW;
+  const int nr = (int)((a.rows - row0) < RW ? (a.rows - row0) : RW);
+  const float2 *in = reinterpret_cast<const float2 *>(a.src) + row0 * CN;
+  float2 *out = reinterpret_cast<float2 *>(a.dst) + row0 * CN;
+  load_tw(tw, twg, N, tid, NT);
+  {
+    float2 v[PER];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int f = tid + i * NT;
+      v[i] = f < nr * CN ? in[f] : make_float2(0.f, 0.f);
+    }
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int f = tid + i * NT;
+      const int r = f / CN, m = f - r * CN;
+      if (f < RW * CN) buf[m * ES + r] = cscale(v[i], a.scale);
+    }
+  }
+  __syncthreads();
+  Plan<N>::template run<-1, ES>(buf + tid % RW, tid / RW, tw);
+  for (int idx = tid; idx < RW * (N / 2 + 1); idx += NT) {
+    const int r = idx % RW, k = idx / RW;
+    const int kn = k == 0 ? 0 : N - k;
+    const float2 za = buf[k * ES + r], zb = buf[kn * ES + r];
+    // X[k] from (Z[k], conj Z[N-k]); X[N-k] from (Z[N-k], conj Z[k])
+    const float2 ea = cscale(cadd(za, cconj(zb)), 0.5f), da = csub(za, cconj(zb));
+    const float2 eb = cscale(cadd(zb, cconj(za)), 0.5f), db = csub(zb, cconj(za));
+    const float2 xa = cadd(ea, cmul(tw_row[k], make_float2(0.5f * da.y, -0.5f * da.x)));
+    const float2 xb = cadd(eb, cmul(tw_row[N - k], make_float2(0.5f * db.y, -0.5f * db.x)));
+    buf[k * ES + r] = xa;
+    buf[(N - k) * ES + r] = xb;  // k = 0 writes bin N into the spare slot
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const int f = tid + i * NT;
+    const int r = f / CN, m = f - r * CN;
+    if (f < nr * CN) out[f] = buf[m * ES + r];
+  }
+}
+
+// Even nxp, inverse: Z[k] = (X[k] + conj X[N-k]) + i w^-k (X[k] - conj X[N-k]) with
+// Im X[0] = Im X[N] = 0, z = IFFT_N(Z), x[2m] = Re z[m], x[2m+1] = Im z[m]; then x scale (+ add);
+// the row's spare slot is written as zero.
+template <int N, int RW>
+__global__ void __launch_bounds__(RW *Plan<N>::TPS)
+k_fftx_c2r(cg::XArgs one, const cg::XArgs *__restrict__ items, int nitems, const float2 *__restrict__ twg,
+           const float2 *__restrict__ tw_row)
+{
+  constexpr int TPS = Plan<N>::TPS;
+  constexpr int NT = RW * TPS;
+  constexpr int ES = RW + 1;
+  constexpr int CN = N + 1;
+  constexpr int PER = (RW * CN + NT - 1) / NT;
+  extern __shared__ float2 smem[];
+  float2 *tw = smem;
+  float2 *buf = smem + N;
+  cg::XArgs a;
+  const unsigned blk = resolve(one, items, nitems, a);
+  const int tid = threadIdx.x;
+  const long row0 = (long)blk * RW;
+  const int nr = (int)((a.rows - row0) < RW ? (a.rows - row0) : RW);
+  float2 *io = reinterpret_cast<float2 *>(a.dst) + row0 * CN;
+  const float2 *ad = a.add ? reinterpret_cast<const float2 *>(a.add) + row0 * CN : nullptr;
+  load_tw(tw, twg, N, tid, NT);
+  {
+    float2 v[PER];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int f = tid + i * NT;
+      v[i] = f < nr * CN ? io[f] : make_float2(0.f, 0.f);
+    }
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int f = tid + i * NT;
+      const int r = f / CN, m = f - r * CN;
+      if (f < RW * CN) buf[m * ES + r] = v[i];
+    }
+  }
+  __syncthreads();
+  for (int idx = tid; idx < RW * (N / 2 + 1); idx += NT) {
+    const int r = idx % RW, k = idx / RW;
+    float2 xa = buf[k * ES + r], xb = buf[(N - k) * ES + r];  // X[k], X[N-k]
+    if (k == 0) {
+      xa.y = 0.f;
+      xb.y = 0.f;
+    }
+    const float2 ea = cadd(xa, cconj(xb)), oa = cmul(csub(xa, cconj(xb)), cconj(tw_row[k]));
+    buf[k * ES + r] = make_float2(ea.x - oa.y, ea.y + oa.x);
+    if (k > 0 && k < N - k) {
+      const float2 eb = cadd(xb, cconj(xa)), ob = cmul(csub(xb, cconj(xa)), cconj(tw_row[N - k]));
+      buf[(N - k) * ES + r] = make_float2(eb.x - ob.y, eb.y + ob.x);
+    }
+  }
+  __syncthreads();
+  Plan<N>::template run<+1, ES>(buf + tid % RW, tid / RW, tw);
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const int f = tid + i * NT;
+    const int r = f / CN, m = f - r * CN;
+    if (f < nr * CN) {
+      float2 v = make_float2(0.f, 0.f);
+      if (m < N) {
+        v = cscale(buf[m * ES + r], a.scale);
+        if (ad) {
+          const float2 q = ad[f];
+          v.x += q.x;
+          v.y += q.y;
+        }
+      }
+      io[f] = v;
+    }
+  }
+}
+
+// Odd nxp = N: each row holds N reals and one padding float, CN = (N + 1) / 2 float pairs.
+// Forward: full-length complex FFT of (x, 0); bins 0..CN-1 are kept.
+template <int N, int RW>
+__global__ void __launch_bounds__(RW *Plan<N>::TPS)
+k_fftxo_r2c(cg::XArgs one, const cg::XArgs *__restrict__ items, int nitems, const float2 *__restrict__ twg)
+{
+  constexpr int TPS = Plan<N>::TPS;
+  constexpr int NT = RW * TPS;
+  constexpr int ES = RW + 1;
+  constexpr int CN = (N + 1) / 2;
+  constexpr int PER = (RW * CN + NT - 1) / NT;
+  extern __shared__ float2 smem[];
+  float2 *tw = smem;
+  float2 *buf = smem + N;
+  cg::XArgs a;
+  const unsigned blk = resolve(one, items, nitems, a);
+  const int tid = threadIdx.x;
+  const long row0 = (long)blk * RW;
+  const int nr = (int)((a.rows - row0) < RW ? (a.rows - row0) : RW);
+  const float2 *in = reinterpret_cast<const float2 *>(a.src) + row0 * CN;
+  float2 *out = reinterpret_cast<float2 *>(a.dst) + row0 * CN;
+  load_tw(tw, twg, N, tid, NT);
+  {
+    float2 v[PER];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int f = tid + i * NT;
+      v[i] = f < nr * CN ? in[f] : make_float2(0.f, 0.f);
+    }
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int f = tid + i * NT;
+      const int r = f / CN, m = f - r * CN;
+      if (f < RW * CN) {
+        buf[(2 * m) * ES + r] = make_float2(v[i].x * a.scale, 0.f);
+        if (2 * m + 1 < N) buf[(2 * m + 1) * ES + r] = make_float2(v[i].y * a.scale, 0.f);
+      }
+    }
+  }
+  __syncthreads();
+  Plan<N>::template run<-1, ES>(buf + tid % RW, tid / RW, tw);
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const int f = tid + i * NT;
+    const int r = f / CN, m = f - r * CN;
+    if (f < nr * CN) out[f] = buf[m * ES + r];
+  }
+}
+
+// Odd nxp, inverse: Hermitian extension Z[0] = Re X[0], Z[k] = X[k], Z[N-k] = conj X[k]
+// (1 <= k < CN), z = IFFT_N(Z), x[m] = Re z[m] scale (+ add); the padding float is zero.
+template <int N, int RW>
+__global__ void __launch_bounds__(RW *Plan<N>::TPS)
+k_fftxo_c2r(cg::XArgs one, const cg::XArgs *__restrict__ items, int nitems, const float2 *__restrict__ twg)
+{
+  constexpr int TPS = Plan<N>::TPS;
+  constexpr int NT = RW * TPS;
+  constexpr int ES = RW + 1;
+  constexpr int CN = (N + 1) / 2;
+  constexpr int PER = (RW * CN + NT - 1) / NT;
+  extern __shared__ float2 smem[];
+  float2 *tw = smem;
+  float2 *buf = smem + N;
+  cg::XArgs a;
+  const unsigned blk = resolve(one, items, nitems, a);
+  const int tid = threadIdx.x;
+  const long row0 = (long)blk * RW;
+  const int nr = (int)((a.rows - row0) < RW ? (a.rows - row0) : RW);
+  float2 *io = reinterpret_cast<float2 *>(a.dst) + row0 * CN;
+  const float2 *ad = a.add ? reinterpret_cast<const float2 *>(a.add) + row0 * CN : nullptr;
+  load_tw(tw, twg, N, tid, NT);
+  {
+    float2 v[PER];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int f = tid + i * NT;
+      v[i] = f < nr * CN ? io[f] : make_float2(0.f, 0.f);
+    }
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int f = tid + i * NT;
+      const int r = f / CN, k = f - r * CN;
+      if (f < RW * CN) {
+        if (k == 0) {
+          buf[r] = make_float2(v[i].x, 0.f);
+        } else {
+          buf[k * ES + r] = v[i];
+          buf[(N - k) * ES + r] = cconj(v[i]);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  Plan<N>::template run<+1, ES>(buf + tid % RW, tid / RW, tw);
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const int f = tid + i * NT;
+    const int r = f / CN, m = f - r * CN;
+    if (f < nr * CN) {
+      float2 v;
+      v.x = buf[(2 * m) * ES + r].x * a.scale;
+      v.y = (2 * m + 1 < N) ? buf[(2 * m + 1) * ES + r].x * a.scale : 0.f;
+      if (ad) {
+        const float2 q = ad[f];
+        v.x += q.x;
+        if (2 * m + 1 < N) v.y += q.y;
+      }
+      io[f] = v;
+    }
+  }
+}
+
+// ---------------------------------------------------------------- host launchers
+template <int N>
+int strided_launch(cg_ctx *ctx, int dir, const cg::StridedArgs *one, const cg::StridedArgs *items, int nitems,
+                   unsigned nblocks, const float2 *tw)
+{
+  constexpr int TI = strided_tile<N>();
+  const size_t lds = sizeof(float2) * (N + (size_t)N * TI);
+  const cg::StridedArgs o = one ? *one : cg::StridedArgs{};
+  if (dir < 0)
+    k_ffts<N, -1, TI><<<nblocks, TI * Plan<N>::TPS, lds, ctx->stream>>>(o, items, nitems, tw);
+  else
+    k_ffts<N, +1, TI><<<nblocks, TI * Plan<N>::TPS, lds, ctx->stream>>>(o, items, nitems, tw);
+  return cg::hip_fail(ctx, hipGetLastError(), "k_ffts");
+}
+
+template <int N>
+int xe_launch(cg_ctx *ctx, int forward, const cg::XArgs *one, const cg::XArgs *items, int nitems, unsigned nblocks,
+              const float2 *tw, const float2 *twr)
+{
+  constexpr int RW = x_rows<N>();
+  const size_t lds = sizeof(float2) * (N + (size_t)(N + 1) * (RW + 1));
+  const cg::XArgs o = one ? *one : cg::XArgs{};
+  if (forward)
+    k_fftx_r2c<N, RW><<<nblocks, RW * Plan<N>::TPS, lds, ctx->stream>>>(o, items, nitems, tw, twr);
+  else
+    k_fftx_c2r<N, RW><<<nblocks, RW * Plan<N>::TPS, lds, ctx->stream>>>(o, items, nitems, tw, twr);
+  return cg::hip_fail(ctx, hipGetLastError(), "k_fftx");
+}
+
+template <int N>
+int xo_launch(cg_ctx *ctx, int forward, const cg::XArgs *one, const cg::XArgs *items, int nitems, unsigned nblocks,
+              const float2 *tw, const float2 *)
+{
+  constexpr int RW = x_rows<N>();
+  const size_t lds = sizeof(float2) * (N + (size_t)(N + 1) * (RW + 1));
+  const cg::XArgs o = one ? *one : cg::XArgs{};
+  if (forward)
+    k_fftxo_r2c<N, RW><<<nblocks, RW * Plan<N>::TPS, lds, ctx->stream>>>(o, items, nitems, tw);
+  else
+    k_fftxo_c2r<N, RW><<<nblocks, RW * Plan<N>::TPS, lds, ctx->stream>>>(o, items, nitems, tw);
+  return cg::hip_fail(ctx, hipGetLastError(), "k_fftxo");
+}
+
+}  // namespace cgfix

@@ -42,6 +42,37 @@ struct FftItem {
   const float2 *tw, *twr;
 };
 
+// Descriptors of the compile-time-plan passes (cg_fft_fixed.h): one grid of a y/z pass, or one
+// grid of an x pass.  In a list, item i owns the blocks [block0_i, block0_{i+1}) of the launch.
+struct StridedArgs {
+  float2 *data;
+  long S, OS;             // element stride, slab stride (complex units)
+  unsigned inner, ncols;  // contiguous columns per slab, columns in all
+  unsigned block0;
+  float scale;
+};
+struct XArgs {
+  const float *src;  // forward: real source (may equal dst)
+  float *dst;        // the grid
+  const float *add;  // inverse: optional real grid added after scaling
+  long rows;
+  unsigned block0;
+  float scale;
+};
+
+// Registry of compiled lengths: kind STRIDED (y/z, n = axis length), XE (even nxp, n = nxp/2),
+// XO (odd nxp, n = nxp); `tile` is TI columns (STRIDED) or RW rows (XE/XO) per workgroup.
+enum { FIX_STRIDED = 0, FIX_XE = 1, FIX_XO = 2, FIX_KINDS = 3 };
+typedef int (*StridedFn)(cg_ctx *, int dir, const StridedArgs *one, const StridedArgs *items, int nitems,
+                         unsigned nblocks, const float2 *tw);
+typedef int (*XFn)(cg_ctx *, int forward, const XArgs *one, const XArgs *items, int nitems, unsigned nblocks,
+                   const float2 *tw, const float2 *twr);
+struct FixedEntry {
+  int kind, n, tile;
+  StridedFn s;
+  XFn x;
+};
+
 struct TimedLaunch {
   int kclass;
   hipEvent_t start, stop;
@@ -54,8 +85,6 @@ struct cg_ctx {
   hipStream_t stream = nullptr;
   std::string err;
   uint64_t budget = 0, in_use = 0, peak = 0;
-  // twiddle tables exp(-2 pi i m / n), m in [0, n), per length, device memory
-  std::map<int, float2 *> twiddles;
   // per-call device staging for host tables, grown on demand
   void *tables = nullptr;
   size_t tables_bytes = 0;
@@ -68,6 +97,17 @@ struct cg_ctx {
   unsigned *list = nullptr;
   size_t list_bytes = 0;
   std::set<cg_grid *> grids;  // live grids; cg_ctx_destroy releases any left over
+  // Grid and batch memory comes from the device's stream-ordered pool when it has one: frees
+  // then wait for nothing but the context stream (a plain hipFree waits for the whole device,
+  // which would serialise concurrently running batches of a server).
+  bool async_alloc = false;
+  std::set<cg_arena *> arenas;
+  // side streams (default priority) and fork/join events of batched launches, created on the
+  // first batch and reused
+  static constexpr int kSide = 4;
+  hipStream_t side[kSide] = {};
+  hipEvent_t fork = nullptr;
+  hipEvent_t join[kSide] = {};
   int fft_engine = CG_FFT_AUTO;
   bool profiling = false;
   std::vector<cg::TimedLaunch> timed;
@@ -76,8 +116,16 @@ struct cg_ctx {
   double kernel_ms[CG_K_COUNT] = {0};
 };
 
+struct cg_arena {
+  cg_ctx *ctx = nullptr;
+  char *base = nullptr;
+  size_t bytes = 0, used = 0;
+  int live = 0;  // grids carved and not yet destroyed
+};
+
 struct cg_grid {
   cg_ctx *ctx = nullptr;
+  cg_arena *arena = nullptr;  // memory owner when carved from an arena
   int nx = 0, ny = 0, nz = 0, nxp = 0, nyp = 0, nzp = 0;
   int cnxp = 0, rnxp = 0;
   int cubetype = 0;
@@ -91,6 +139,8 @@ namespace cg {
 int fail(cg_ctx *ctx, int code, const std::string &msg);
 int hip_fail(cg_ctx *ctx, hipError_t e, const char *what);
 const float2 *twiddle_table(cg_ctx *ctx, int n);
+hipError_t dev_alloc(cg_ctx *ctx, void **p, size_t bytes);  // stream-ordered on ctx->stream when possible
+void dev_free(cg_ctx *ctx, void *p);
 void *table_space(cg_ctx *ctx, size_t bytes);
 void launch_begin(cg_ctx *ctx, int kclass, hipEvent_t *start);
 void launch_end(cg_ctx *ctx, int kclass, hipEvent_t start);
@@ -112,10 +162,17 @@ int fixed_strided(cg_ctx *ctx, float2 *data, int n, long S, long inner, long out
                   int *rc);
 int fixed_x(cg_ctx *ctx, int forward, const float *src, float *grid, int nxp, long rows, float scale, const float *add,
             int *rc);
+const FixedEntry *fixed_find(int kind, int n);  // null if the length was not compiled
+// Lists of one kind and length (device copies of the descriptors, block0 filled in).
+int fixed_strided_list(cg_ctx *ctx, const FixedEntry *e, int dir, const StridedArgs *dev_items, int nitems,
+                       unsigned nblocks);
+int fixed_x_list(cg_ctx *ctx, const FixedEntry *e, int forward, const XArgs *dev_items, int nitems, unsigned nblocks,
+                 int nxp);
 int grid_fft_inverse(cg_grid *g, const float *add_real);
 
 // Elementwise kernels (cg_noise.hip).
 int scale_real(cg_ctx *ctx, float *d, size_t n, float s);
+int fill_constant(cg_ctx *ctx, float *d, size_t n, int nxp, int rnxp, float v);
 int add_real(cg_ctx *ctx, float *d, const float *s, size_t n);
 
 }  // namespace cg

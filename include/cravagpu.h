@@ -42,6 +42,7 @@ extern "C" {
 
 typedef struct cg_ctx cg_ctx;
 typedef struct cg_grid cg_grid;
+typedef struct cg_arena cg_arena;
 
 /* ------------------------------------------------------------------ context */
 /* Replaces the process-global FFTGrid statics (fftgrid.cpp:2661-2668): the budget is the
@@ -81,6 +82,17 @@ int cg_ctx_set_fft_engine(cg_ctx *ctx, int engine);
 int cg_grid_create(cg_ctx *ctx, int nx, int ny, int nz, int nxp, int nyp, int nzp, int cubetype,
                    cg_grid **grid);
 int cg_grid_destroy(cg_grid *grid);
+/* An arena: one device allocation from which grids are carved (256-byte aligned), so that a
+ * request holding dozens of grids costs one allocation and one release.  No reference
+ * counterpart (FFTGrid allocates per grid).  Grids of an arena are destroyed as usual (which
+ * releases no memory); cg_arena_destroy needs every grid of the arena destroyed first and
+ * frees the allocation after the work already enqueued on the context stream.
+ * cg_arena_grid_bytes gives the bytes a grid of that padded size takes in an arena. */
+int cg_arena_create(cg_ctx *ctx, uint64_t bytes, cg_arena **arena);
+int cg_arena_destroy(cg_arena *arena);
+uint64_t cg_arena_grid_bytes(int nxp, int nyp, int nzp);
+int cg_grid_create_in(cg_arena *arena, int nx, int ny, int nz, int nxp, int nyp, int nzp, int cubetype,
+                      cg_grid **grid);
 /* setType (fftgrid.h) */
 int cg_grid_set_type(cg_grid *grid, int cubetype);
 /* getIsTransformed (fftgrid.h): 1 in the Fourier domain, 0 in the real domain. */
@@ -114,6 +126,9 @@ int cg_fft_r2c(const cg_grid *src, cg_grid *dst);
 /* FFTGrid::add / multiplyByScalar (fftgrid.cpp:1659-1678, 1766-1773). */
 int cg_grid_add(cg_grid *dst, const cg_grid *src);
 int cg_grid_multiply_by_scalar(cg_grid *grid, float scalar);
+/* FFTGrid::fillInConstant (fftgrid.cpp:634-651): real domain, value on every padded cell with
+ * x < nxp, RMISSING (-99999) on the row padding. */
+int cg_grid_fill_constant(cg_grid *grid, float value);
 
 /* ------------------------------------------------------------------ posterior (AVOInversion) */
 #define CG_PRIOR_GRIDS 0     /* six prior covariance spectra, as SeismicParametersHolder holds */
@@ -186,10 +201,14 @@ int cg_grid_fill_noise_counter(cg_grid *grid, uint64_t seed, uint32_t stream);
 
 /* ------------------------------------------------------------------ batched inversions */
 /* Many independent inversions of any sizes (the reference runs one per process) planned
- * together: every pass of all of them shares one launch (x, y, z FFT passes over all grids;
- * one posterior launch per angle count; colouring), so small cubes no longer pay one launch per
- * grid and pass.  No reference counterpart; it serves batched and continuously batched
- * requests.  Results are identical to running each inversion alone in a batch of one. */
+ * together: each pass of all of them shares a few launches (one per compiled FFT length, run
+ * concurrently on side streams; one posterior launch per angle count; colouring), so small
+ * cubes no longer pay one launch per grid and pass.  No reference counterpart; it serves
+ * batched and continuously batched requests.  Results are identical to running each inversion
+ * alone in a batch of one.  Grids may belong to any context on the same device (a server
+ * generates inputs on one stream and runs batches on others); the batch runs on the stream of
+ * `ctx`, so work that other contexts enqueued on its grids must have completed (cg_ctx_sync)
+ * before cg_batch_run. */
 typedef struct cg_batch cg_batch;
 typedef struct {
   /* tables, prior, precision and work grids as for cg_posterior: post.mean[3] and post.data[]

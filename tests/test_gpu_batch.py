@@ -110,3 +110,23 @@ def test_batch_matches_single_request_api(ctx):
     mean_s = np.stack([g.download_real() for g in mean])
     assert rel_l2(mean_b - p["mean_real"], mean_s - p["mean_real"]) < 1e-5
     b.close()
+
+
+def test_batch_compiled_plans_match_runtime_plans(ctx):
+    """The batch groups grids by compiled FFT length; the runtime-plan kernels (every grid in one
+    launch per pass) must give the same inversion."""
+    probs = [P.make_problem(**kw) for kw in MIX]
+    res = {}
+    for eng in (C.FFT_AUTO, C.FFT_GENERIC):
+        ctx.set_fft_engine(eng)
+        items = [_item(ctx, p, 1, 40 + i) for i, p in enumerate(probs)]
+        b = C.Batch(ctx, items)
+        b.run()
+        b.wait()
+        res[eng] = [_results(it) for it in items]
+        b.close()
+    ctx.set_fft_engine(C.FFT_AUTO)
+    for (ma, ca, oa), (mg, cg_, og), p in zip(res[C.FFT_AUTO], res[C.FFT_GENERIC], probs):
+        assert rel_l2(ma - p["mean_real"], mg - p["mean_real"]) < 1e-5
+        assert rel_l2(ca, cg_) < 1e-5
+        assert rel_l2(oa[0] - ma, og[0] - mg) < 1e-5

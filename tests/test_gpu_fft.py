@@ -134,3 +134,68 @@ def test_out_of_place_forward_keeps_source(ctx):
     assert rel_l2(dst.download_complex(), O.grid_fft(x, C.DATA)) < 2e-6
     src.close()
     dst.close()
+
+
+def _factorable(n):
+    for p in (2, 3, 5, 7):
+        while n % p == 0:
+            n //= p
+    return n == 1
+
+
+# every length the compile-time engine instantiates (2^a 3^b 5^c 7^d up to 1152)
+PLAN_LENGTHS = [n for n in range(2, 1153) if _factorable(n)]
+
+
+def test_every_compiled_length_matches_oracle(ctx):
+    """y and z passes of every length, x passes of every even nxp = 2n (n <= 576) and every odd
+    nxp = n: forward and inverse against the FP64 oracle."""
+    rng = np.random.default_rng(11)
+    shapes = []
+    for n in PLAN_LENGTHS:
+        if n >= 4:
+            shapes += [(2, n, 4), (n, 2, 4)]  # (nzp, nyp, nxp): y length n; z length n
+        if n <= 576:
+            shapes.append((2, 3, 2 * n))
+        if n % 2:
+            shapes.append((3, 2, n))
+    worst = 0.0
+    for shape in shapes:
+        nzp, nyp, nxp = shape
+        x = rng.standard_normal(shape).astype(np.float32)
+        cnxp = nxp // 2 + 1
+        z = (rng.standard_normal((nzp, nyp, cnxp)) + 1j * rng.standard_normal((nzp, nyp, cnxp))).astype(np.complex64)
+        g = _grid(ctx, shape, C.DATA)
+        g.upload_real(x)
+        g.fftInPlace()
+        e1 = rel_l2(g.download_complex(), O.grid_fft(x, C.DATA))
+        g.upload_complex(z)
+        g.invFFTInPlace()
+        e2 = rel_l2(g.download_real(), O.grid_ifft(z, nxp, C.DATA))
+        g.close()
+        worst = max(worst, e1, e2)
+        assert e1 < 2e-6 and e2 < 2e-6, (shape, e1, e2)
+    print("compiled lengths: %d grids, worst rel L2 %.2e" % (len(shapes), worst), flush=True)
+
+
+def test_grids_carved_from_an_arena(ctx):
+    """Grids in one arena are independent (distinct memory, same results as stand-alone grids);
+    the arena refuses to go while a grid of it is alive."""
+    rng = np.random.default_rng(9)
+    shape = (12, 10, 18)
+    xs = [rng.standard_normal(shape).astype(np.float32) for _ in range(3)]
+    arena = C.Arena(ctx, 3 * C.Arena.grid_bytes(18, 10, 12))
+    gs = [C.FFTGrid(ctx, 18, 10, 12, 18, 10, 12, C.DATA, arena=arena) for _ in range(3)]
+    with pytest.raises(C.CravaError):
+        C.FFTGrid(ctx, 18, 10, 12, 18, 10, 12, C.DATA, arena=arena)  # full
+    for g, x in zip(gs, xs):
+        g.upload_real(x)
+    for g in gs:
+        g.fftInPlace()
+    for g, x in zip(gs, xs):
+        assert rel_l2(g.download_complex(), O.grid_fft(x, C.DATA)) < 2e-6
+    with pytest.raises(C.CravaError):
+        arena.close()  # grids still alive
+    for g in gs:
+        g.close()
+    arena.close()
