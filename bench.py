@@ -146,6 +146,8 @@ def bench_mix(ctx, n=64, reps=3):
     specs = S.uneven_mix(n)
     t0 = time.perf_counter()
     probs = [S.Problem(ctx, s) for s in specs]
+    for shape in sorted({(s["nxp"], s["nyp"], s["nzp"]) for s in specs}):
+        ctx.autotune(*shape)  # both paths use the tuned tiles
     log("mix setup %.1f s (%d cubes, %.1f GB)" % (time.perf_counter() - t0, n,
                                                  sum(S.footprint(s) for s in specs) / 1e9))
     cells = sum(S.cells(s) for s in specs)
@@ -234,6 +236,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-mix", action="store_true", help="skip the batched-vs-plain uneven mix")
     ap.add_argument("--no-serve", action="store_true", help="skip the continuous-batching serving run")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a copy-like kernel of known traffic (for scaling hardware byte counters)")
     ap.add_argument("--profile-steps", type=int, default=2)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -243,9 +247,21 @@ def main():
     w = dict(WORKLOAD)
     R = args.realisations
     ctx = C.Context(0)
+    if args.copy_calibration:
+        # a copy-like kernel of known traffic for scaling hardware byte counters: FFTGrid::add
+        # on two grids of the workload's size reads 2 and writes 1 grid (k_add), 5 launches
+        a = C.FFTGrid(ctx, w["nx"], w["ny"], w["nz"], w["nxp"], w["nyp"], w["nzp"], C.PARAMETER)
+        b = C.FFTGrid(ctx, w["nx"], w["ny"], w["nz"], w["nxp"], w["nyp"], w["nzp"], C.PARAMETER)
+        for _ in range(5):
+            a.add(b)
+        ctx.sync()
+        log("copy calibration: 5 x k_add over %d floats" % (2 * (w["nxp"] // 2 + 1) * w["nyp"] * w["nzp"]))
+        ctx.close()
+        return
     t0 = time.perf_counter()
     inv = S.Problem(ctx, S.spec(**w, realisations=R))
-    log("setup %.1f s" % (time.perf_counter() - t0))
+    ctx.autotune(w["nxp"], w["nyp"], w["nzp"])
+    log("setup %.1f s (incl. autotuning the FFT tiles of the grid)" % (time.perf_counter() - t0))
     for _ in range(args.warmup):
         inv.step()
     ctx.sync()

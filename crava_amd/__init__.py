@@ -92,7 +92,8 @@ def lib():
             "cg_forward_model": [_vp, i, _f_p, _vp * 3, _vp * 8],
             "cg_grid_add_white_noise": [_vp, f, u64, ctypes.c_uint32],
             "cg_grid_sum_squares": [_vp, i, _d_p],
-            "cg_ctx_set_fft_engine": [_vp, i],
+            "cg_ctx_set_fft_engine": [_vp, i], "cg_ctx_set_fft_variant": [_vp, i],
+            "cg_autotune": [_vp, i, i, i],
             "cg_arena_create": [_vp, u64, ctypes.POINTER(_vp)], "cg_arena_destroy": [_vp],
             "cg_grid_create_in": [_vp, i, i, i, i, i, i, i, ctypes.POINTER(_vp)],
         }
@@ -140,6 +141,15 @@ class Context:
         a, b = ctypes.c_uint64(), ctypes.c_uint64()
         lib().cg_ctx_memory(self._h, ctypes.byref(a), ctypes.byref(b))
         return a.value, b.value
+
+    def autotune(self, nxp, nyp, nzp):
+        """Time both tile variants of every FFT pass of an nxp x nyp x nzp grid and keep the
+        faster for that geometry (every context of the device)."""
+        self.check(lib().cg_autotune(self._h, nxp, nyp, nzp), "cg_autotune")
+
+    def set_fft_variant(self, variant):
+        """Force tile variant 0 or 1 of the compile-time-plan passes, or -1 for the tuned choice."""
+        self.check(lib().cg_ctx_set_fft_variant(self._h, variant), "cg_ctx_set_fft_variant")
 
     def set_fft_engine(self, engine):
         """FFT_AUTO (compile-time plans where available) or FFT_GENERIC (runtime plans only)."""

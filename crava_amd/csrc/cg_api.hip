@@ -345,6 +345,20 @@ int cg_ctx_stream(const cg_ctx *ctx, void **stream)
   return CG_OK;
 }
 
+int cg_ctx_set_fft_variant(cg_ctx *ctx, int variant)
+{
+  if (!ctx || variant < -1 || variant > 1) return CG_EINVAL;
+  ctx->fft_variant = variant;
+  return CG_OK;
+}
+
+int cg_autotune(cg_ctx *ctx, int nxp, int nyp, int nzp)
+{
+  if (!ctx || nxp < 1 || nyp < 1 || nzp < 1) return CG_EINVAL;
+  DeviceGuard dg(ctx->device);
+  return cg::autotune_geometry(ctx, nxp, nyp, nzp);
+}
+
 int cg_ctx_set_fft_engine(cg_ctx *ctx, int engine)
 {
   if (!ctx || (engine != CG_FFT_AUTO && engine != CG_FFT_GENERIC)) return CG_EINVAL;
@@ -1048,6 +1062,8 @@ struct cg_batch {
   // One launch of a compile-time-plan kernel over every grid of the pass that has that length.
   struct FixedGroup {
     const cg::FixedEntry *e = nullptr;
+    int variant = 0;              // tile variant (the autotuned one for the first grid's geometry)
+    int add = 0;                  // x inverse: the grids add a real grid
     const void *items = nullptr;  // cg::StridedArgs or cg::XArgs, device memory
     int nitems = 0, nxp = 0;
     unsigned blocks = 0;
@@ -1152,11 +1168,11 @@ int run_pass(cg_batch *b, const cg_batch::Pass &p)
     cg::launch_begin(ctx, p.kclass, &ev);
     int r;
     if (p.kind == cg::FFT_ITEM_STRIDED)
-      r = cg::fixed_strided_list(ctx, g.e, p.dir, static_cast<const cg::StridedArgs *>(g.items), g.nitems,
-                                 g.blocks);
+      r = cg::fixed_strided_list(ctx, g.e, g.variant, p.dir, static_cast<const cg::StridedArgs *>(g.items),
+                                 g.nitems, g.blocks);
     else
-      r = cg::fixed_x_list(ctx, g.e, p.kind == cg::FFT_ITEM_X_R2C, static_cast<const cg::XArgs *>(g.items),
-                           g.nitems, g.blocks, g.nxp);
+      r = cg::fixed_x_list(ctx, g.e, g.variant, p.kind == cg::FFT_ITEM_X_R2C, g.add,
+                           static_cast<const cg::XArgs *>(g.items), g.nitems, g.blocks, g.nxp);
     cg::launch_end(ctx, p.kclass, ev);
     return r;
   });
@@ -1440,20 +1456,26 @@ int cg_batch_create(cg_ctx *ctx, int n, const cg_inversion *inv, cg_batch **out)
       fg.nitems = (int)g.items.size();
       fg.items = dev + g.off;
       unsigned gb = 0;
+      const auto &f0 = g.items.front();
       if (pb.kind == cg::FFT_ITEM_STRIDED) {
+        fg.variant = cg::fixed_variant(ctx, g.e, f0.S, f0.inner, f0.inner * f0.outer);
+        const int tile = g.e->tile[fg.variant];
         std::vector<cg::StridedArgs> a;
         for (const auto &it : g.items) {
           const unsigned long ncols = (unsigned long)it.inner * (unsigned long)it.outer;
           a.push_back({it.data, it.S, it.OS, (unsigned)it.inner, (unsigned)ncols, gb, it.scale});
-          gb += (unsigned)((ncols + g.e->tile - 1) / g.e->tile);
+          gb += (unsigned)((ncols + tile - 1) / tile);
         }
         put(g.off, a.data(), sizeof(cg::StridedArgs) * a.size());
       } else {
+        const bool fwd = pb.kind == cg::FFT_ITEM_X_R2C;
+        fg.add = !fwd && f0.add != nullptr;
+        fg.variant = cg::fixed_variant(ctx, g.e, f0.rows, fwd, fg.add);
+        const int tile = g.e->tile[fg.variant];
         std::vector<cg::XArgs> a;
         for (const auto &it : g.items) {
-          const bool fwd = pb.kind == cg::FFT_ITEM_X_R2C;
           a.push_back({fwd ? it.src : nullptr, it.grid, fwd ? nullptr : it.add, it.rows, gb, it.scale});
-          gb += (unsigned)((it.rows + g.e->tile - 1) / g.e->tile);
+          gb += (unsigned)((it.rows + tile - 1) / tile);
           fg.nxp = it.nxp;
         }
         put(g.off, a.data(), sizeof(cg::XArgs) * a.size());

@@ -94,26 +94,41 @@ CG_FFT_PLANS(CG_PLAN)
 
 constexpr size_t kLds = 80 * 1024;  // bytes per workgroup aimed for: two workgroups per CU
 
-// Columns per y/z tile: 16 (128 B row segments) while the tile fits kLds, else 8; more for
-// short lengths so that a workgroup has at least two wavefronts.
+constexpr size_t kLdsMax = 160 * 1024;  // LDS of one CU
+
+// Tile variants.  Every length is compiled with two tile sizes per pass (where two exist); the
+// autotuner (cg_autotune) times both on a grid's geometry and keeps the faster, otherwise
+// variant 0 runs.
+// y/z passes, variant 0: 16 columns (128 B row segments) while the tile fits kLds, else 8; more
+// for short lengths so that a workgroup has at least two wavefronts.  Variant 1: twice as wide
+// (segments that straddle fewer cache lines when the row pitch is not a multiple of 128 B) if
+// that fits one CU's LDS, else half as wide.
 template <int N>
-constexpr int strided_tile()
+constexpr int strided_tile0()
 {
   constexpr int T = Plan<N>::TPS;
   int ti = ((size_t)N * 17 * 8 <= kLds) ? 16 : 8;
   while (ti < 64 && ti * T < 128 && (size_t)N * (2 * ti + 1) * 8 <= kLds) ti *= 2;
   return ti;
 }
+template <int N, int V>
+constexpr int strided_tile()
+{
+  constexpr int t0 = strided_tile0<N>();
+  if (V == 0) return t0;
+  if ((size_t)N * (2 * t0 + 1) * 8 <= kLdsMax && 2 * t0 * Plan<N>::TPS <= 1024) return 2 * t0;
+  return t0 >= 8 ? t0 / 2 : t0;
+}
 
-// Rows per x tile: the most (up to 128) whose LDS (N twiddles + (N + 1)(RW + 1) slots) fits kLds
-// with at most 512 threads.
-template <int N>
+// x pass, variant 0: the most rows (up to 128) whose LDS (N twiddles + (N + 1)(RW + 1) slots)
+// fits kLds with at most 512 threads; variant 1: half as many rows (more workgroups per CU).
+template <int N, int V>
 constexpr int x_rows()
 {
   constexpr int T = Plan<N>::TPS;
   int rw = 128;
   while (rw > 1 && ((size_t)(N + (N + 1) * (rw + 1)) * 8 > kLds || rw * T > 512)) rw /= 2;
-  return rw;
+  return (V == 1 && rw > 1) ? rw / 2 : rw;
 }
 
 __device__ __forceinline__ void load_tw(float2 *tw, const float2 *g, int n, int tid, int nthr)
@@ -245,8 +260,9 @@ k_fftx_r2c(cg::XArgs one, const cg::XArgs *__restrict__ items, int nitems, const
     // X[k] from (Z[k], conj Z[N-k]); X[N-k] from (Z[N-k], conj Z[k])
     const float2 ea = cscale(cadd(za, cconj(zb)), 0.5f), da = csub(za, cconj(zb));
     const float2 eb = cscale(cadd(zb, cconj(za)), 0.5f), db = csub(zb, cconj(za));
-    const float2 xa = cadd(ea, cmul(tw_row[k], make_float2(0.5f * da.y, -0.5f * da.x)));
-    const float2 xb = cadd(eb, cmul(tw_row[N - k], make_float2(0.5f * db.y, -0.5f * db.x)));
+    const float2 w = tw_row[k];  // w^(N-k) = -conj(w^k) for w = exp(-i pi / N)
+    const float2 xa = cadd(ea, cmul(w, make_float2(0.5f * da.y, -0.5f * da.x)));
+    const float2 xb = cadd(eb, cmul(make_float2(-w.x, w.y), make_float2(0.5f * db.y, -0.5f * db.x)));
     buf[k * ES + r] = xa;
     buf[(N - k) * ES + r] = xb;  // k = 0 writes bin N into the spare slot
   }
@@ -262,7 +278,7 @@ k_fftx_r2c(cg::XArgs one, const cg::XArgs *__restrict__ items, int nitems, const
 // Even nxp, inverse: Z[k] = (X[k] + conj X[N-k]) + i w^-k (X[k] - conj X[N-k]) with
 // Im X[0] = Im X[N] = 0, z = IFFT_N(Z), x[2m] = Re z[m], x[2m+1] = Im z[m]; then x scale (+ add);
 // the row's spare slot is written as zero.
-template <int N, int RW>
+template <int N, int RW, bool ADD>
 __global__ void __launch_bounds__(RW *Plan<N>::TPS)
 k_fftx_c2r(cg::XArgs one, const cg::XArgs *__restrict__ items, int nitems, const float2 *__restrict__ twg,
            const float2 *__restrict__ tw_row)
@@ -305,30 +321,38 @@ k_fftx_c2r(cg::XArgs one, const cg::XArgs *__restrict__ items, int nitems, const
       xa.y = 0.f;
       xb.y = 0.f;
     }
-    const float2 ea = cadd(xa, cconj(xb)), oa = cmul(csub(xa, cconj(xb)), cconj(tw_row[k]));
+    const float2 w = tw_row[k];  // w^(N-k) = -conj(w^k) for w = exp(-i pi / N)
+    const float2 ea = cadd(xa, cconj(xb)), oa = cmul(csub(xa, cconj(xb)), cconj(w));
     buf[k * ES + r] = make_float2(ea.x - oa.y, ea.y + oa.x);
     if (k > 0 && k < N - k) {
-      const float2 eb = cadd(xb, cconj(xa)), ob = cmul(csub(xb, cconj(xa)), cconj(tw_row[N - k]));
+      const float2 eb = cadd(xb, cconj(xa)), ob = cmul(csub(xb, cconj(xa)), make_float2(-w.x, -w.y));
       buf[(N - k) * ES + r] = make_float2(eb.x - ob.y, eb.y + ob.x);
     }
   }
   __syncthreads();
   Plan<N>::template run<+1, ES>(buf + tid % RW, tid / RW, tw);
+  float2 v[ADD ? PER : 1];
+  if (ADD) {  // every load of the added grid in flight before the first store
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int f = tid + i * NT;
+      v[ADD ? i : 0] = f < nr * CN ? ad[f] : make_float2(0.f, 0.f);
+    }
+  }
 #pragma unroll
   for (int i = 0; i < PER; ++i) {
     const int f = tid + i * NT;
     const int r = f / CN, m = f - r * CN;
     if (f < nr * CN) {
-      float2 v = make_float2(0.f, 0.f);
+      float2 o = make_float2(0.f, 0.f);
       if (m < N) {
-        v = cscale(buf[m * ES + r], a.scale);
-        if (ad) {
-          const float2 q = ad[f];
-          v.x += q.x;
-          v.y += q.y;
+        o = cscale(buf[m * ES + r], a.scale);
+        if (ADD) {
+          o.x += v[ADD ? i : 0].x;
+          o.y += v[ADD ? i : 0].y;
         }
       }
-      io[f] = v;
+      io[f] = o;
     }
   }
 }
@@ -384,7 +408,7 @@ k_fftxo_r2c(cg::XArgs one, const cg::XArgs *__restrict__ items, int nitems, cons
 
 // Odd nxp, inverse: Hermitian extension Z[0] = Re X[0], Z[k] = X[k], Z[N-k] = conj X[k]
 // (1 <= k < CN), z = IFFT_N(Z), x[m] = Re z[m] scale (+ add); the padding float is zero.
-template <int N, int RW>
+template <int N, int RW, bool ADD>
 __global__ void __launch_bounds__(RW *Plan<N>::TPS)
 k_fftxo_c2r(cg::XArgs one, const cg::XArgs *__restrict__ items, int nitems, const float2 *__restrict__ twg)
 {
@@ -427,30 +451,37 @@ k_fftxo_c2r(cg::XArgs one, const cg::XArgs *__restrict__ items, int nitems, cons
   }
   __syncthreads();
   Plan<N>::template run<+1, ES>(buf + tid % RW, tid / RW, tw);
+  float2 v[ADD ? PER : 1];
+  if (ADD) {  // every load of the added grid in flight before the first store
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int f = tid + i * NT;
+      v[ADD ? i : 0] = f < nr * CN ? ad[f] : make_float2(0.f, 0.f);
+    }
+  }
 #pragma unroll
   for (int i = 0; i < PER; ++i) {
     const int f = tid + i * NT;
     const int r = f / CN, m = f - r * CN;
     if (f < nr * CN) {
-      float2 v;
-      v.x = buf[(2 * m) * ES + r].x * a.scale;
-      v.y = (2 * m + 1 < N) ? buf[(2 * m + 1) * ES + r].x * a.scale : 0.f;
-      if (ad) {
-        const float2 q = ad[f];
-        v.x += q.x;
-        if (2 * m + 1 < N) v.y += q.y;
+      float2 o;
+      o.x = buf[(2 * m) * ES + r].x * a.scale;
+      o.y = (2 * m + 1 < N) ? buf[(2 * m + 1) * ES + r].x * a.scale : 0.f;
+      if (ADD) {
+        o.x += v[ADD ? i : 0].x;
+        if (2 * m + 1 < N) o.y += v[ADD ? i : 0].y;
       }
-      io[f] = v;
+      io[f] = o;
     }
   }
 }
 
 // ---------------------------------------------------------------- host launchers
-template <int N>
+template <int N, int V>
 int strided_launch(cg_ctx *ctx, int dir, const cg::StridedArgs *one, const cg::StridedArgs *items, int nitems,
                    unsigned nblocks, const float2 *tw)
 {
-  constexpr int TI = strided_tile<N>();
+  constexpr int TI = strided_tile<N, V>();
   const size_t lds = sizeof(float2) * (N + (size_t)N * TI);
   const cg::StridedArgs o = one ? *one : cg::StridedArgs{};
   if (dir < 0)
@@ -460,31 +491,36 @@ int strided_launch(cg_ctx *ctx, int dir, const cg::StridedArgs *one, const cg::S
   return cg::hip_fail(ctx, hipGetLastError(), "k_ffts");
 }
 
-template <int N>
-int xe_launch(cg_ctx *ctx, int forward, const cg::XArgs *one, const cg::XArgs *items, int nitems, unsigned nblocks,
-              const float2 *tw, const float2 *twr)
+// `add`: the inverse adds a real grid (every descriptor of a list has one, or none has).
+template <int N, int V>
+int xe_launch(cg_ctx *ctx, int forward, int add, const cg::XArgs *one, const cg::XArgs *items, int nitems,
+              unsigned nblocks, const float2 *tw, const float2 *twr)
 {
-  constexpr int RW = x_rows<N>();
+  constexpr int RW = x_rows<N, V>();
   const size_t lds = sizeof(float2) * (N + (size_t)(N + 1) * (RW + 1));
   const cg::XArgs o = one ? *one : cg::XArgs{};
   if (forward)
     k_fftx_r2c<N, RW><<<nblocks, RW * Plan<N>::TPS, lds, ctx->stream>>>(o, items, nitems, tw, twr);
+  else if (add)
+    k_fftx_c2r<N, RW, true><<<nblocks, RW * Plan<N>::TPS, lds, ctx->stream>>>(o, items, nitems, tw, twr);
   else
-    k_fftx_c2r<N, RW><<<nblocks, RW * Plan<N>::TPS, lds, ctx->stream>>>(o, items, nitems, tw, twr);
+    k_fftx_c2r<N, RW, false><<<nblocks, RW * Plan<N>::TPS, lds, ctx->stream>>>(o, items, nitems, tw, twr);
   return cg::hip_fail(ctx, hipGetLastError(), "k_fftx");
 }
 
-template <int N>
-int xo_launch(cg_ctx *ctx, int forward, const cg::XArgs *one, const cg::XArgs *items, int nitems, unsigned nblocks,
-              const float2 *tw, const float2 *)
+template <int N, int V>
+int xo_launch(cg_ctx *ctx, int forward, int add, const cg::XArgs *one, const cg::XArgs *items, int nitems,
+              unsigned nblocks, const float2 *tw, const float2 *)
 {
-  constexpr int RW = x_rows<N>();
+  constexpr int RW = x_rows<N, V>();
   const size_t lds = sizeof(float2) * (N + (size_t)(N + 1) * (RW + 1));
   const cg::XArgs o = one ? *one : cg::XArgs{};
   if (forward)
     k_fftxo_r2c<N, RW><<<nblocks, RW * Plan<N>::TPS, lds, ctx->stream>>>(o, items, nitems, tw);
+  else if (add)
+    k_fftxo_c2r<N, RW, true><<<nblocks, RW * Plan<N>::TPS, lds, ctx->stream>>>(o, items, nitems, tw);
   else
-    k_fftxo_c2r<N, RW><<<nblocks, RW * Plan<N>::TPS, lds, ctx->stream>>>(o, items, nitems, tw);
+    k_fftxo_c2r<N, RW, false><<<nblocks, RW * Plan<N>::TPS, lds, ctx->stream>>>(o, items, nitems, tw);
   return cg::hip_fail(ctx, hipGetLastError(), "k_fftxo");
 }
 

@@ -1,7 +1,9 @@
 // cg_fft_fixed.hip -- dispatch to the compile-time-plan FFT passes (cg_fft_fixed.h), which
 // cg_fft_fixed_part.hip instantiates for every length of cg_fft_plans.h.  Lengths without a
 // compiled plan fall back to the runtime-plan kernels of cg_fft.hip.
+#include <array>
 #include <map>
+#include <mutex>
 
 #include "cg_fft_plans.h"
 #include "cg_internal.h"
@@ -40,7 +42,26 @@ const Registry &registry()
   return r;
 }
 
+// Autotuned tile variants per (device, kind, length, pass geometry), shared by the contexts of a
+// device.
+using TuneKey = std::array<long, 6>;
+std::mutex tune_mu;
+std::map<TuneKey, int> tune_table;
+
+TuneKey tune_key(const cg_ctx *ctx, const FixedEntry *e, long g0, long g1, long g2)
+{
+  return TuneKey{(long)ctx->device, (long)e->kind, (long)e->n, g0, g1, g2};
+}
+
 }  // namespace
+
+int fixed_variant(cg_ctx *ctx, const FixedEntry *e, long g0, long g1, long g2)
+{
+  if (ctx->fft_variant == 0 || ctx->fft_variant == 1) return ctx->fft_variant;
+  std::lock_guard<std::mutex> lock(tune_mu);
+  const auto it = tune_table.find(tune_key(ctx, e, g0, g1, g2));
+  return it == tune_table.end() ? 0 : it->second;
+}
 
 const FixedEntry *fixed_find(int kind, int n)
 {
@@ -66,8 +87,9 @@ int fixed_strided(cg_ctx *ctx, float2 *data, int n, long S, long inner, long out
     return 1;
   }
   StridedArgs a{data, S, OS, (unsigned)inner, (unsigned)ncols, 0u, scale};
-  const unsigned nblk = (unsigned)((ncols + e->tile - 1) / e->tile);
-  *rc = e->s(ctx, dir, &a, nullptr, 0, nblk, tw);
+  const int v = fixed_variant(ctx, e, S, inner, (long)ncols);
+  const unsigned nblk = (unsigned)((ncols + e->tile[v] - 1) / e->tile[v]);
+  *rc = e->s[v](ctx, dir, &a, nullptr, 0, nblk, tw);
   return 1;
 }
 
@@ -83,34 +105,112 @@ int fixed_x(cg_ctx *ctx, int forward, const float *src, float *grid, int nxp, lo
     *rc = CG_ENOMEM;
     return 1;
   }
-  const unsigned long nblk = ((unsigned long)rows + e->tile - 1) / e->tile;
+  const int v = fixed_variant(ctx, e, rows, forward, add != nullptr);
+  const unsigned long nblk = ((unsigned long)rows + e->tile[v] - 1) / e->tile[v];
   if (nblk >= (1ul << 31)) {
     *rc = fail(ctx, CG_EINVAL, "x FFT: too many rows");
     return 1;
   }
   XArgs a{forward ? src : nullptr, grid, forward ? nullptr : add, rows, 0u, scale};
-  *rc = e->x(ctx, forward, &a, nullptr, 0, (unsigned)nblk, tw, twr);
+  *rc = e->x[v](ctx, forward, !forward && add, &a, nullptr, 0, (unsigned)nblk, tw, twr);
   return 1;
 }
 
-int fixed_strided_list(cg_ctx *ctx, const FixedEntry *e, int dir, const StridedArgs *dev_items, int nitems,
+int fixed_strided_list(cg_ctx *ctx, const FixedEntry *e, int v, int dir, const StridedArgs *dev_items, int nitems,
                        unsigned nblocks)
 {
   if (nitems <= 0 || nblocks == 0) return CG_OK;
   const float2 *tw = twiddle_table(ctx, e->n);
   if (!tw) return CG_ENOMEM;
-  return e->s(ctx, dir, nullptr, dev_items, nitems, nblocks, tw);
+  return e->s[v](ctx, dir, nullptr, dev_items, nitems, nblocks, tw);
 }
 
-int fixed_x_list(cg_ctx *ctx, const FixedEntry *e, int forward, const XArgs *dev_items, int nitems, unsigned nblocks,
-                 int nxp)
+int fixed_x_list(cg_ctx *ctx, const FixedEntry *e, int v, int forward, int add, const XArgs *dev_items, int nitems,
+                 unsigned nblocks, int nxp)
 {
   if (nitems <= 0 || nblocks == 0) return CG_OK;
   const bool even = e->kind == FIX_XE;
   const float2 *tw = twiddle_table(ctx, e->n);
   const float2 *twr = even ? twiddle_table(ctx, nxp) : nullptr;
   if (!tw || (even && !twr)) return CG_ENOMEM;
-  return e->x(ctx, forward, nullptr, dev_items, nitems, nblocks, tw, twr);
+  return e->x[v](ctx, forward, !forward && add, nullptr, dev_items, nitems, nblocks, tw, twr);
+}
+
+// Times both tile variants of every pass of a grid of padded size nxp x nyp x nzp (x forward
+// out of place, x inverse with and without the added grid, y and z forward + inverse) on scratch
+// grids and records the faster for that geometry.
+int autotune_geometry(cg_ctx *ctx, int nxp, int nyp, int nzp)
+{
+  const long cnxp = nxp / 2 + 1, plane = cnxp * nyp, csize = plane * nzp;
+  float2 *A = nullptr, *B = nullptr;
+  hipError_t e = hipMalloc(&A, sizeof(float2) * csize);
+  if (e == hipSuccess) e = hipMalloc(&B, sizeof(float2) * csize);
+  if (e == hipSuccess) e = hipMemsetAsync(A, 0, sizeof(float2) * csize, ctx->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(B, 0, sizeof(float2) * csize, ctx->stream);
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  if (e == hipSuccess) e = hipEventCreate(&t0);
+  if (e == hipSuccess) e = hipEventCreate(&t1);
+  int rc = e == hipSuccess ? CG_OK : hip_fail(ctx, e, "autotune scratch");
+  // mean time of `reps` runs of f after one untimed run; negative on failure
+  auto time_it = [&](auto f) -> float {
+    if (f()) return -1.f;
+    const int reps = 3;
+    if (hipEventRecord(t0, ctx->stream) != hipSuccess) return -1.f;
+    for (int i = 0; i < reps; ++i)
+      if (f()) return -1.f;
+    if (hipEventRecord(t1, ctx->stream) != hipSuccess || hipEventSynchronize(t1) != hipSuccess) return -1.f;
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, t0, t1);
+    return ms / reps;
+  };
+  auto record = [&](const FixedEntry *en, long g0, long g1, long g2, const float (&ms)[2]) {
+    if (ms[0] < 0.f || ms[1] < 0.f) return;
+    std::lock_guard<std::mutex> lock(tune_mu);
+    tune_table[tune_key(ctx, en, g0, g1, g2)] = ms[1] < ms[0] ? 1 : 0;
+  };
+  const bool even = nxp % 2 == 0;
+  const FixedEntry *ex = fixed_find(even ? FIX_XE : FIX_XO, even ? nxp / 2 : nxp);
+  if (!rc && ex && ex->tile[0] != ex->tile[1]) {
+    const float2 *tw = twiddle_table(ctx, ex->n), *twr = even ? twiddle_table(ctx, nxp) : nullptr;
+    const long rows = (long)nyp * nzp;
+    for (int mode = 0; mode < 3 && tw; ++mode) {  // forward; inverse; inverse + add
+      const int fwd = mode == 0;
+      const float *add = mode == 2 ? reinterpret_cast<const float *>(B) : nullptr;
+      float ms[2];
+      for (int v = 0; v < 2; ++v) {
+        XArgs a{fwd ? reinterpret_cast<const float *>(B) : nullptr, reinterpret_cast<float *>(A), add, rows, 0u,
+                1.0f};
+        const unsigned nblk = (unsigned)((rows + ex->tile[v] - 1) / ex->tile[v]);
+        ms[v] = time_it([&] { return ex->x[v](ctx, fwd, add != nullptr, &a, nullptr, 0, nblk, tw, twr); });
+      }
+      record(ex, rows, fwd, add != nullptr, ms);
+    }
+  }
+  for (int axis = 1; axis <= 2 && !rc; ++axis) {
+    const int n = axis == 1 ? nyp : nzp;
+    const FixedEntry *es = n > 1 ? fixed_find(FIX_STRIDED, n) : nullptr;
+    if (!es || es->tile[0] == es->tile[1]) continue;
+    const float2 *tw = twiddle_table(ctx, n);
+    if (!tw) continue;
+    const long S = axis == 1 ? cnxp : plane, inner = S, OS = axis == 1 ? plane : 0;
+    const long ncols = axis == 1 ? cnxp * nzp : plane;
+    float ms[2];
+    for (int v = 0; v < 2; ++v) {
+      StridedArgs a{A, S, OS, (unsigned)inner, (unsigned)ncols, 0u, 1.0f};
+      const unsigned nblk = (unsigned)((ncols + es->tile[v] - 1) / es->tile[v]);
+      ms[v] = time_it([&] {
+        int r = es->s[v](ctx, -1, &a, nullptr, 0, nblk, tw);
+        return r ? r : es->s[v](ctx, +1, &a, nullptr, 0, nblk, tw);
+      });
+    }
+    record(es, S, inner, ncols, ms);
+  }
+  if (t0) (void)hipEventDestroy(t0);
+  if (t1) (void)hipEventDestroy(t1);
+  (void)hipStreamSynchronize(ctx->stream);
+  if (A) (void)hipFree(A);
+  if (B) (void)hipFree(B);
+  return rc;
 }
 
 }  // namespace cg

@@ -11,9 +11,16 @@
 
 namespace {
 
-#define S(n) {cg::FIX_STRIDED, n, cgfix::strided_tile<n>(), &cgfix::strided_launch<n>, nullptr},
-#define XE(n) {cg::FIX_XE, n, cgfix::x_rows<n>(), nullptr, &cgfix::xe_launch<n>},
-#define XO(n) {cg::FIX_XO, n, cgfix::x_rows<n>(), nullptr, &cgfix::xo_launch<n>},
+// two tile variants per length and pass (the second may repeat the first where no other fits)
+#define S(n)                                                                                  \
+  {cg::FIX_STRIDED, n, {cgfix::strided_tile<n, 0>(), cgfix::strided_tile<n, 1>()},           \
+   {&cgfix::strided_launch<n, 0>, &cgfix::strided_launch<n, 1>}, {nullptr, nullptr}},
+#define XE(n)                                                                        \
+  {cg::FIX_XE, n, {cgfix::x_rows<n, 0>(), cgfix::x_rows<n, 1>()}, {nullptr, nullptr}, \
+   {&cgfix::xe_launch<n, 0>, &cgfix::xe_launch<n, 1>}},
+#define XO(n)                                                                        \
+  {cg::FIX_XO, n, {cgfix::x_rows<n, 0>(), cgfix::x_rows<n, 1>()}, {nullptr, nullptr}, \
+   {&cgfix::xo_launch<n, 0>, &cgfix::xo_launch<n, 1>}},
 const cg::FixedEntry entries[] = {CG_CAT(CG_FFT_PART, CG_PART)(S, XE, XO)};
 #undef S
 #undef XE

@@ -65,13 +65,18 @@ struct XArgs {
 enum { FIX_STRIDED = 0, FIX_XE = 1, FIX_XO = 2, FIX_KINDS = 3 };
 typedef int (*StridedFn)(cg_ctx *, int dir, const StridedArgs *one, const StridedArgs *items, int nitems,
                          unsigned nblocks, const float2 *tw);
-typedef int (*XFn)(cg_ctx *, int forward, const XArgs *one, const XArgs *items, int nitems, unsigned nblocks,
-                   const float2 *tw, const float2 *twr);
+typedef int (*XFn)(cg_ctx *, int forward, int add, const XArgs *one, const XArgs *items, int nitems,
+                   unsigned nblocks, const float2 *tw, const float2 *twr);
 struct FixedEntry {
-  int kind, n, tile;
-  StridedFn s;
-  XFn x;
+  int kind, n;
+  int tile[2];  // the two tile variants (equal when the length has only one)
+  StridedFn s[2];
+  XFn x[2];
 };
+// Tile variant for a pass geometry: the autotuner's choice (cg_autotune) for this exact
+// geometry, or the context's forced variant, else 0.  `geom` identifies the pass: strided
+// (S, inner, ncols), x (rows, forward, add).
+int fixed_variant(cg_ctx *ctx, const FixedEntry *e, long g0, long g1, long g2);
 
 struct TimedLaunch {
   int kclass;
@@ -109,6 +114,7 @@ struct cg_ctx {
   hipEvent_t fork = nullptr;
   hipEvent_t join[kSide] = {};
   int fft_engine = CG_FFT_AUTO;
+  int fft_variant = -1;  // -1: autotuned choice or 0; 0/1: forced tile variant
   bool profiling = false;
   std::vector<cg::TimedLaunch> timed;
   std::vector<hipEvent_t> event_pool;
@@ -164,10 +170,11 @@ int fixed_x(cg_ctx *ctx, int forward, const float *src, float *grid, int nxp, lo
             int *rc);
 const FixedEntry *fixed_find(int kind, int n);  // null if the length was not compiled
 // Lists of one kind and length (device copies of the descriptors, block0 filled in).
-int fixed_strided_list(cg_ctx *ctx, const FixedEntry *e, int dir, const StridedArgs *dev_items, int nitems,
+int fixed_strided_list(cg_ctx *ctx, const FixedEntry *e, int v, int dir, const StridedArgs *dev_items, int nitems,
                        unsigned nblocks);
-int fixed_x_list(cg_ctx *ctx, const FixedEntry *e, int forward, const XArgs *dev_items, int nitems, unsigned nblocks,
-                 int nxp);
+int fixed_x_list(cg_ctx *ctx, const FixedEntry *e, int v, int forward, int add, const XArgs *dev_items, int nitems,
+                 unsigned nblocks, int nxp);
+int autotune_geometry(cg_ctx *ctx, int nxp, int nyp, int nzp);
 int grid_fft_inverse(cg_grid *g, const float *add_real);
 
 // Elementwise kernels (cg_noise.hip).

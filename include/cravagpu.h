@@ -75,6 +75,15 @@ int cg_ctx_reset_kernel_time(cg_ctx *ctx);
 #define CG_FFT_AUTO 0
 #define CG_FFT_GENERIC 1
 int cg_ctx_set_fft_engine(cg_ctx *ctx, int engine);
+/* Autotuner (no reference counterpart; FFTW 2's planner plays this part in the reference): every
+ * compiled length has two tile variants per pass (y/z: columns per workgroup; x: rows per
+ * workgroup).  cg_autotune times both for every pass of a grid of this padded size on scratch
+ * memory and keeps the faster for later transforms of that geometry, by any context of the
+ * device (batched launches use the choice for their groups' first grid).  Without tuning,
+ * variant 0 runs.  cg_ctx_set_fft_variant forces a variant (0 or 1) or restores the tuned
+ * choice (-1). */
+int cg_autotune(cg_ctx *ctx, int nxp, int nyp, int nzp);
+int cg_ctx_set_fft_variant(cg_ctx *ctx, int variant);
 
 /* ------------------------------------------------------------------ grids (FFTGrid) */
 /* FFTGrid(nx,ny,nz,nxp,nyp,nzp) + createRealGrid/createComplexGrid (fftgrid.cpp:40, 1048-1063),

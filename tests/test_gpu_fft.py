@@ -147,10 +147,12 @@ def _factorable(n):
 PLAN_LENGTHS = [n for n in range(2, 1153) if _factorable(n)]
 
 
-def test_every_compiled_length_matches_oracle(ctx):
+@pytest.mark.parametrize("variant", [0, 1])
+def test_every_compiled_length_matches_oracle(ctx, variant):
     """y and z passes of every length, x passes of every even nxp = 2n (n <= 576) and every odd
-    nxp = n: forward and inverse against the FP64 oracle."""
-    rng = np.random.default_rng(11)
+    nxp = n, in both tile variants: forward and inverse against the FP64 oracle."""
+    ctx.set_fft_variant(variant)
+    rng = np.random.default_rng(11 + variant)
     shapes = []
     for n in PLAN_LENGTHS:
         if n >= 4:
@@ -174,8 +176,25 @@ def test_every_compiled_length_matches_oracle(ctx):
         e2 = rel_l2(g.download_real(), O.grid_ifft(z, nxp, C.DATA))
         g.close()
         worst = max(worst, e1, e2)
-        assert e1 < 2e-6 and e2 < 2e-6, (shape, e1, e2)
-    print("compiled lengths: %d grids, worst rel L2 %.2e" % (len(shapes), worst), flush=True)
+        assert e1 < 2e-6 and e2 < 2e-6, (shape, variant, e1, e2)
+    ctx.set_fft_variant(-1)
+    print("compiled lengths, variant %d: %d grids, worst rel L2 %.2e" % (variant, len(shapes), worst), flush=True)
+
+
+def test_autotuned_transforms_match_oracle(ctx):
+    """After the autotuner has picked tile variants for a geometry, transforms of that geometry
+    (plain and with the inverse's added grid) still match the oracle."""
+    rng = np.random.default_rng(12)
+    for nxp, nyp, nzp in ((576, 96, 40), (63, 70, 48), (160, 576, 12)):
+        ctx.autotune(nxp, nyp, nzp)
+        x = rng.standard_normal((nzp, nyp, nxp)).astype(np.float32)
+        g = _grid(ctx, x.shape, C.DATA)
+        g.upload_real(x)
+        g.fftInPlace()
+        assert rel_l2(g.download_complex(), O.grid_fft(x, C.DATA)) < 2e-6
+        g.invFFTInPlace()
+        assert rel_l2(g.download_real(), x) < 2e-6
+        g.close()
 
 
 def test_grids_carved_from_an_arena(ctx):
