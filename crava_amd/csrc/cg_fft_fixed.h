@@ -77,17 +77,86 @@ struct Stages<N, DIR, TPS, ES, NS, P, RS...> {
   }
 };
 
+// The same stages with the first stage's inputs taken from registers and the last stage's
+// outputs handed to a sink instead of LDS (the y/z passes load the first stage's butterfly
+// inputs straight from global memory and store the last stage's outputs straight back, which
+// saves two LDS round trips and two barriers per tile).
+//   First<P1>::Regs<BPT1>: v[b][q] = element (t + b TPS) + q N/P1 of the sequence.
+//   sink(e, value) receives element e of the transformed sequence.
+template <int N, int DIR, int TPS, int ES, int NS, int... RS>
+struct StagesIO;
+
+template <int N, int DIR, int TPS, int ES, int NS, int P, int... RS>
+struct StagesIO<N, DIR, TPS, ES, NS, P, RS...> {
+  static constexpr int P_ = P;
+  static constexpr int NB = N / P;
+  static constexpr int BPT = (NB + TPS - 1) / TPS;
+  static constexpr bool EVEN = NB % TPS == 0;
+  static constexpr bool LAST = sizeof...(RS) == 0;
+
+  // one stage from values already in registers (v), outputs to LDS or, last, to the sink
+  template <class Sink>
+  static __device__ __forceinline__ void step(float2 (&v)[BPT][P], float2 *buf, int t, const float2 *tw, Sink &sink)
+  {
+    constexpr int TWS = N / (NS * P);
+#pragma unroll
+    for (int b = 0; b < BPT; ++b) {
+      const int j = t + b * TPS;
+      if (EVEN || j < NB) {
+        const int k = j % NS;
+        if (NS > 1) {
+#pragma unroll
+          for (int q = 1; q < P; ++q) {
+            float2 w = tw[q * k * TWS];
+            if (DIR > 0) w.y = -w.y;
+            v[b][q] = cmul(v[b][q], w);
+          }
+        }
+        dft<DIR, P>(v[b]);
+        const int d = (j / NS) * NS * P + k;
+#pragma unroll
+        for (int q = 0; q < P; ++q) {
+          if constexpr (LAST) sink(d + q * NS, v[b][q]);
+          else buf[(d + q * NS) * ES] = v[b][q];
+        }
+      }
+    }
+    if constexpr (!LAST) {
+      __syncthreads();
+      StagesIO<N, DIR, TPS, ES, NS * P, RS...>::from_lds(buf, t, tw, sink);
+    }
+  }
+
+  template <class Sink>
+  static __device__ __forceinline__ void from_lds(float2 *buf, int t, const float2 *tw, Sink &sink)
+  {
+    float2 v[BPT][P];
+#pragma unroll
+    for (int b = 0; b < BPT; ++b) {
+      const int j = t + b * TPS;
+      if (EVEN || j < NB) {
+#pragma unroll
+        for (int q = 0; q < P; ++q) v[b][q] = buf[(j + q * NB) * ES];
+      }
+    }
+    __syncthreads();
+    step(v, buf, t, tw, sink);
+  }
+};
+
 template <int N>
 struct Plan;
-#define CG_PLAN(n, tps, ...)                                                      \
-  template <>                                                                     \
-  struct Plan<n> {                                                                \
-    static constexpr int TPS = tps;                                               \
-    template <int DIR, int ES>                                                    \
-    static __device__ __forceinline__ void run(float2 *b, int t, const float2 *w) \
-    {                                                                             \
-      Stages<n, DIR, tps, ES, 1, __VA_ARGS__>::run(b, t, w);                      \
-    }                                                                             \
+#define CG_PLAN(n, tps, ...)                                                                    \
+  template <>                                                                                   \
+  struct Plan<n> {                                                                              \
+    static constexpr int TPS = tps;                                                             \
+    template <int DIR, int ES>                                                                  \
+    static __device__ __forceinline__ void run(float2 *b, int t, const float2 *w)               \
+    {                                                                                           \
+      Stages<n, DIR, tps, ES, 1, __VA_ARGS__>::run(b, t, w);                                    \
+    }                                                                                           \
+    template <int DIR, int ES>                                                                  \
+    using IO = StagesIO<n, DIR, tps, ES, 1, __VA_ARGS__>;                                       \
   };
 CG_FFT_PLANS(CG_PLAN)
 #undef CG_PLAN
@@ -171,39 +240,35 @@ template <int N, int DIR, int TI>
 __global__ void __launch_bounds__(TI *Plan<N>::TPS)
 k_ffts(cg::StridedArgs one, const cg::StridedArgs *__restrict__ items, int nitems, const float2 *__restrict__ twg)
 {
+  using IO = typename Plan<N>::template IO<DIR, TI>;
   constexpr int NT = TI * Plan<N>::TPS;
-  constexpr int PER = (N * TI + NT - 1) / NT;  // elements per thread, all of one column
   extern __shared__ float2 smem[];
   float2 *tw = smem;
   float2 *buf = smem + N;
   cg::StridedArgs a;
   const unsigned blk = resolve(one, items, nitems, a);
   const int tid = threadIdx.x;
-  const int c = tid % TI;  // (tid + i NT) % TI == tid % TI: each thread stays on one column
+  const int c = tid % TI, t = tid / TI;
   const unsigned q = blk * TI + c;
   const bool live = q < a.ncols;
   float2 *col = a.data + (live ? (long)(q % a.inner) + (long)(q / a.inner) * a.OS : 0);
   load_tw(tw, twg, N, tid, NT);
-  {
-    float2 v[PER];
+  // the first stage's butterfly inputs, straight from global memory: lanes of a wavefront
+  // cover TI consecutive columns of consecutive rows (TI * 8-byte row segments)
+  float2 v[IO::BPT][IO::P_];
 #pragma unroll
-    for (int i = 0; i < PER; ++i) {
-      const int idx = tid + i * NT;
-      v[i] = (live && idx < N * TI) ? col[(long)(idx / TI) * a.S] : make_float2(0.f, 0.f);
-    }
+  for (int b = 0; b < IO::BPT; ++b) {
+    const int j = t + b * Plan<N>::TPS;
 #pragma unroll
-    for (int i = 0; i < PER; ++i)
-      if (tid + i * NT < N * TI) buf[tid + i * NT] = v[i];
+    for (int r = 0; r < IO::P_; ++r)
+      v[b][r] = (live && (IO::EVEN || j < IO::NB)) ? col[(long)(j + r * IO::NB) * a.S] : make_float2(0.f, 0.f);
   }
-  __syncthreads();
-  Plan<N>::template run<DIR, TI>(buf + c, tid / TI, tw);
-  if (live) {
-#pragma unroll
-    for (int i = 0; i < PER; ++i) {
-      const int idx = tid + i * NT;
-      if (idx < N * TI) col[(long)(idx / TI) * a.S] = cscale(buf[idx], a.scale);
-    }
-  }
+  __syncthreads();  // twiddles in LDS
+  const float scale = a.scale;
+  auto sink = [&](int e, float2 x) {
+    if (live) col[(long)e * a.S] = cscale(x, scale);
+  };
+  IO::step(v, buf + c, t, tw, sink);
 }
 
 // ---------------------------------------------------------------- x pass
