@@ -135,6 +135,19 @@ def cpu_baseline(w, inv, R, seconds_hint=20.0):
     }
 
 
+TRAFFIC = os.path.join(ROOT, "profiles", "single_cube_576x576x320_traffic.json")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of a kernel class on this workload, from the rocprofv3 FETCH_SIZE /
+    WRITE_SIZE runs summarised in profiles/ (scaled by a copy kernel of known traffic)."""
+    try:
+        with open(TRAFFIC) as f:
+            return json.load(f)["bytes_per_launch"].get(kernel)
+    except (OSError, ValueError, KeyError):
+        return None
+
+
 def rel_l2(a, b):
     return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-30))
 
@@ -298,7 +311,16 @@ def main():
     for _ in range(args.profile_steps):
         inv.step()
     kt = {k: ctx.kernel_time(k) for k in NAMES}
+    # streaming ceiling of this GPU in the same run: FFTGrid::add (k_add) on two grids of the
+    # workload's size reads 2 and writes 1 grid, nothing else
+    ctx.reset_kernel_time()
+    for _ in range(5):
+        inv.mean_in[0].add(inv.mean_in[1])
+    n_add, ms_add = ctx.kernel_time(C.K_ELEMENTWISE)
     ctx.set_profiling(False)
+    for g, v in zip(inv.mean_in, S.BACKGROUND):
+        g.fillInConstant(v)  # restore the prior means
+    stream_gbps = 3 * inv.mean_in[0].rnxp * w["nyp"] * w["nzp"] * 4 / (ms_add * 1e-3 / n_add) / 1e9
     kb = kernel_bytes(w, inv.nt, R, inv.n_inverted_planes)
     kernels = {}
     for k, (n, ms) in kt.items():
@@ -337,7 +359,7 @@ def main():
                    "grid": [w["nx"], w["ny"], w["nz"]], "padded": [w["nxp"], w["nyp"], w["nzp"]],
                    "angles": list(w["angles"]), "realisations": R, "inputs": "resident in HBM"},
         "roofline": {"bound": "hbm", "kernel": NAMES[dom], "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": measured_traffic(NAMES[dom]),
                      "bytes_per_launch": per_launch_bytes, "avg_launch_us": avg_s * 1e6},
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                        "samples": len(lat), "kind": "plain single request, synchronised"},
@@ -347,6 +369,8 @@ def main():
         "model_v1": {"bytes": model_bytes, "frac_of_roofline": model_bytes / (ms_per_step * 1e-3) / HBM_PEAK,
                      "bytes_min": model_min},
         "kernels": kernels,
+        "stream_ceiling": {"kernel": "k_add (read 2 grids, write 1)", "gbps": round(stream_gbps, 1),
+                           "frac_of_peak": stream_gbps * 1e9 / HBM_PEAK},
         "posterior_stats": {"inverted": stats[0], "cholesky_failures": stats[1], "fp64_resolves": stats[2]},
         "cpu_baseline": None,
     }
