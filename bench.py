@@ -321,6 +321,13 @@ def main():
     for g, v in zip(inv.mean_in, S.BACKGROUND):
         g.fillInConstant(v)  # restore the prior means
     stream_gbps = 3 * inv.mean_in[0].rnxp * w["nyp"] * w["nzp"] * 4 / (ms_add * 1e-3 / n_add) / 1e9
+    # the reference's own noise stream (RandomGen + fillInComplexNoise), bit for bit, on one grid
+    seed = 23665
+    seed = inv.out[0][0].fillInComplexNoiseReference(seed)  # warm
+    t = time.perf_counter()
+    for g in inv.out[0]:
+        seed = g.fillInComplexNoiseReference(seed)
+    ms_refnoise = 1000.0 * (time.perf_counter() - t) / 3
     kb = kernel_bytes(w, inv.nt, R, inv.n_inverted_planes)
     kernels = {}
     for k, (n, ms) in kt.items():
@@ -369,6 +376,9 @@ def main():
         "model_v1": {"bytes": model_bytes, "frac_of_roofline": model_bytes / (ms_per_step * 1e-3) / HBM_PEAK,
                      "bytes_min": model_min},
         "kernels": kernels,
+        "reference_noise_fill": {"ms_per_grid": ms_refnoise, "normals_per_grid": 2 * (w["nxp"] // 2 + 1) * w["nyp"] * w["nzp"],
+                                 "what": "RandomGen + fillInComplexNoise stream, bit-identical, three consecutive "
+                                         "grids (one realisation's seeds), synchronised"},
         "stream_ceiling": {"kernel": "k_add (read 2 grids, write 1)", "gbps": round(stream_gbps, 1),
                            "frac_of_peak": stream_gbps * 1e9 / HBM_PEAK},
         "posterior_stats": {"inverted": stats[0], "cholesky_failures": stats[1], "fp64_resolves": stats[2]},

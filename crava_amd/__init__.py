@@ -86,6 +86,7 @@ def lib():
             "cg_grid_add": [_vp, _vp], "cg_grid_multiply_by_scalar": [_vp, f], "cg_grid_fill_constant": [_vp, f],
             "cg_posterior": [_vp, ctypes.POINTER(PostArgs)], "cg_simulate": [_vp, ctypes.POINTER(SimArgs)],
             "cg_grid_fill_noise_counter": [_vp, u64, ctypes.c_uint32],
+            "cg_grid_fill_noise_reference": [_vp, ctypes.POINTER(ctypes.c_uint32)],
             "cg_build_kz_tables": [i, i, d, i, _f_p, _f_p, f, f, f, _f_p, _f_p, _f_p, _f_p, _i_p],
             "cg_factorable": [i],
             "cg_fft_r2c": [_vp, _vp],
@@ -276,6 +277,13 @@ class FFTGrid:
     def fillInConstant(self, value):
         """FFTGrid::fillInConstant: value on the cube, RMISSING on the row padding (real domain)."""
         self.ctx.check(lib().cg_grid_fill_constant(self._h, value), "fillInConstant")
+
+    def fillInComplexNoiseReference(self, seed):
+        """FFTGrid::fillInComplexNoise with the reference's RandomGen stream from state `seed`;
+        returns the state after the fill (the reference's seed_ after the call)."""
+        st = ctypes.c_uint32(seed)
+        self.ctx.check(lib().cg_grid_fill_noise_reference(self._h, ctypes.byref(st)), "fillInComplexNoiseReference")
+        return st.value
 
     def fillInComplexNoise(self, seed, stream=0):
         self.ctx.check(lib().cg_grid_fill_noise_counter(self._h, seed, stream), "fillInComplexNoise")

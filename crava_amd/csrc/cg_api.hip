@@ -710,6 +710,18 @@ int cg_grid_fill_constant(cg_grid *g, float value)
   return rc;
 }
 
+int cg_grid_fill_noise_reference(cg_grid *g, uint32_t *seed)
+{
+  if (!g || !seed) return CG_EINVAL;
+  DeviceGuard dg(g->ctx->device);
+  const int rc = cg::fill_noise_reference(g->ctx, reinterpret_cast<float2 *>(g->d), g->nxp, g->nyp, g->nzp, seed);
+  if (rc == CG_OK) {
+    g->transformed = 1;
+    g->cubetype = CG_CT_PARAMETER;
+  }
+  return rc;
+}
+
 int cg_grid_fill_noise_counter(cg_grid *g, uint64_t seed, uint32_t stream)
 {
   if (!g) return CG_EINVAL;

@@ -179,6 +179,8 @@ int grid_fft_inverse(cg_grid *g, const float *add_real);
 
 // Elementwise kernels (cg_noise.hip).
 int scale_real(cg_ctx *ctx, float *d, size_t n, float s);
+// The reference's noise stream (cg_refrng.hip).
+int fill_noise_reference(cg_ctx *ctx, float2 *out, int nxp, int nyp, int nzp, uint32_t *seed);
 int fill_constant(cg_ctx *ctx, float *d, size_t n, int nxp, int rnxp, float v);
 int add_real(cg_ctx *ctx, float *d, const float *s, size_t n);
 

@@ -207,6 +207,13 @@ int cg_simulate(cg_ctx *ctx, const cg_sim_args *args);
 /* FFTGrid::fillInComplexNoise layout (fftgrid.cpp:997-1045) with the counter-based generator:
  * the grid becomes a PARAMETER grid in the Fourier domain. */
 int cg_grid_fill_noise_counter(cg_grid *grid, uint64_t seed, uint32_t stream);
+/* FFTGrid::fillInComplexNoise (fftgrid.cpp:997-1045) with RandomGen (random.cpp:38-118,
+ * random.h:11-14): the reference's own stream, bit for bit, generated in parallel.  *seed is the
+ * reference's process-wide RandomGen seed before the fill and is advanced exactly as the
+ * reference advances it, so consecutive calls continue one stream (three calls = the three seed
+ * grids of one realisation, as AVOInversion::simulate draws them).  The grid becomes a
+ * PARAMETER grid in the Fourier domain.  Synchronises the context stream. */
+int cg_grid_fill_noise_reference(cg_grid *grid, uint32_t *seed);
 
 /* ------------------------------------------------------------------ batched inversions */
 /* Many independent inversions of any sizes (the reference runs one per process) planned

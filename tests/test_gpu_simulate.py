@@ -31,13 +31,36 @@ def test_counter_noise_matches_oracle(ctx, shape):
     g.close()
 
 
-@pytest.mark.parametrize("source", ["grids", "counter"])
+# odd and even nxp (the parity rule of the x = cnxp-1 column), nyp/nzp of 1, the plain case's
+# reference padding (84 x 84 x 180, 1.3 M normals)
+@pytest.mark.parametrize("shape", [(12, 10, 16), (9, 7, 10), (16, 1, 1), (15, 6, 1), (84, 84, 180)])
+def test_reference_noise_stream_is_bit_identical(ctx, shape):
+    """The device's parallel generation of the reference's RandomGen + fillInComplexNoise stream
+    equals the oracle (itself bit-identical to the reference-compiled RandomGen) bit for bit,
+    including the advanced seed, over three consecutive fills (one realisation's seed grids)."""
+    nxp, nyp, nzp = shape
+    print("reference noise", shape, flush=True)
+    g = C.FFTGrid(ctx, nxp, nyp, nzp, nxp, nyp, nzp, C.PARAMETER)
+    seed = 23665  # RandomGen's default seed (random.cpp:136)
+    state = np.array([seed], dtype=np.uint32)
+    for _ in range(3):
+        seed = g.fillInComplexNoiseReference(seed)
+        want = O.fill_complex_noise(nxp, nyp, nzp, state)
+        got = g.download_complex()
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+        assert seed == int(state[0])
+    g.close()
+
+
+@pytest.mark.parametrize("source", ["grids", "device_reference", "counter"])
 def test_realisation_matches_oracle(ctx, source):
+    """grids: the reference stream drawn on the host and uploaded; device_reference: the same
+    stream drawn on the device (cg_grid_fill_noise_reference); counter: the counter generator."""
     p = P.make_problem(16, 12, 20, seed=11)
     om, oc = _posterior_cov(p)
     nxp, nyp, nzp = p["nxp"], p["nyp"], p["nzp"]
     mean_real = np.stack([O.grid_ifft(m, nxp, O.PARAMETER, np.float32) for m in om])
-    if source == "grids":
+    if source in ("grids", "device_reference"):
         st = np.array([23665], dtype=np.uint32)  # the reference stream (RandomGen default seed)
         noise = np.stack([O.fill_complex_noise(nxp, nyp, nzp, st) for _ in range(3)])
     else:
@@ -47,12 +70,15 @@ def test_realisation_matches_oracle(ctx, source):
     gc, gmean, gout = _grids(ctx, p, 6, C.COVARIANCE), _grids(ctx, p, 3, C.PARAMETER), _grids(ctx, p, 3, C.PARAMETER)
     for g in range(6):
         gc[g].upload_complex(oc[g])
+    seed = 23665
     for g in range(3):
         gmean[g].upload_real(mean_real[g])
         if source == "grids":
             gout[g].upload_complex(noise[g])
+        elif source == "device_reference":
+            seed = gout[g].fillInComplexNoiseReference(seed)
     fails = C.simulate(ctx, gc, gmean, gout, seed=99, realisation=4,
-                       noise_source=C.NOISE_GRIDS if source == "grids" else C.NOISE_COUNTER, stats=True)
+                       noise_source=C.NOISE_COUNTER if source == "counter" else C.NOISE_GRIDS, stats=True)
     got = np.stack([g.download_real() for g in gout])
     assert fails == nf
     assert rel_l2(got - mean_real, want - mean_real) < 1e-5
