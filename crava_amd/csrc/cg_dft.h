@@ -1,4 +1,5 @@
-// cg_dft.h -- complex helpers and the radix-2/3/4/5/7/8 DFT butterflies of the LDS FFT engines.
+// cg_dft.h -- complex helpers and the DFT butterflies of the LDS FFT engines: radix 2/3/4/5/7/8,
+// and composite sizes up to 32 computed as two smaller DFTs in registers.
 // X[q] = sum_r v[r] w^(r q), w = exp(DIR 2 pi i / p); DIR = -1 forward (FFTW sign), +1 inverse.
 #pragma once
 
@@ -127,15 +128,128 @@ __device__ __forceinline__ void dft8(float2 *v)
   v[7] = csub(e[3], o3);
 }
 
+// ---------------------------------------------------------------- composite sizes
+// P = A B (A the largest base radix dividing P with B = P / A > 1), in registers:
+//   X[k1 + A k2] = sum_n2 w_B^(n2 k2) [ w_P^(n2 k1) sum_n1 v[B n1 + n2] w_A^(n1 k1) ]
+// with compile-time twiddles w_P^m (constexpr tables in double, rounded to float).
+constexpr double kTwoPi = 6.283185307179586476925286766559;
+
+constexpr double cx_sin_taylor(double x)  // |x| <= pi / 2: 14 terms reach double precision
+{
+  double term = x, sum = x;
+  for (int k = 1; k < 14; ++k) {
+    term *= -x * x / ((2.0 * k) * (2.0 * k + 1.0));
+    sum += term;
+  }
+  return sum;
+}
+constexpr double cx_cos_taylor(double x)
+{
+  double term = 1.0, sum = 1.0;
+  for (int k = 1; k < 14; ++k) {
+    term *= -x * x / ((2.0 * k - 1.0) * (2.0 * k));
+    sum += term;
+  }
+  return sum;
+}
+// cos and sin of 2 pi m / p: quadrant q = floor(4 m / p) and residual angle
+// (pi / 2) (4 m - q p) / p in [0, pi / 2), both exact in integers.
+constexpr void cx_sincos(long m, long p, double &c, double &s)
+{
+  m %= p;
+  if (m < 0) m += p;
+  const long q = (4 * m) / p;
+  const double phi = (kTwoPi / 4.0) * (double)(4 * m - q * p) / (double)p;
+  const double cp = cx_cos_taylor(phi), sp = cx_sin_taylor(phi);
+  switch (q) {
+    case 0: c = cp; s = sp; break;
+    case 1: c = -sp; s = cp; break;
+    case 2: c = -cp; s = -sp; break;
+    default: c = sp; s = -cp; break;
+  }
+}
+
+template <int P>
+struct TwTable {
+  float c[P];
+  float s[P];
+};
+template <int P>
+constexpr TwTable<P> make_tw()
+{
+  TwTable<P> t{};
+  for (int m = 0; m < P; ++m) {
+    double c = 0, s = 0;
+    cx_sincos(m, P, c, s);
+    t.c[m] = (float)c;
+    t.s[m] = (float)s;
+  }
+  return t;
+}
+template <int P>
+struct Tw {
+  static constexpr TwTable<P> t = make_tw<P>();
+};
+
+constexpr int base_radix(int p)
+{
+  return p % 8 == 0 && p > 8 ? 8 : p % 7 == 0 && p > 7 ? 7 : p % 5 == 0 && p > 5 ? 5 : p % 4 == 0 && p > 4 ? 4
+         : p % 3 == 0 && p > 3 ? 3 : p % 2 == 0 && p > 2 ? 2 : p;
+}
+
+template <int DIR, int P>
+__device__ __forceinline__ void dft(float2 *v);
+
+template <int DIR, int P>
+__device__ __forceinline__ void dft_composite(float2 *v)
+{
+  constexpr int A = base_radix(P);
+  constexpr int B = P / A;
+  float2 y[P];
+#pragma unroll
+  for (int n2 = 0; n2 < B; ++n2) {
+    float2 t[A];
+#pragma unroll
+    for (int n1 = 0; n1 < A; ++n1) t[n1] = v[B * n1 + n2];
+    dft<DIR, A>(t);
+#pragma unroll
+    for (int k1 = 0; k1 < A; ++k1) {
+      const int m = (n2 * k1) % P;
+      float2 w = make_float2(Tw<P>::t.c[m], Tw<P>::t.s[m]);  // exp(+i 2 pi m / P)
+      if (DIR < 0) w.y = -w.y;
+      y[n2 * A + k1] = (n2 == 0 || k1 == 0) ? t[k1] : cmul(t[k1], w);
+    }
+  }
+#pragma unroll
+  for (int k1 = 0; k1 < A; ++k1) {
+    float2 t[B];
+#pragma unroll
+    for (int n2 = 0; n2 < B; ++n2) t[n2] = y[n2 * A + k1];
+    dft<DIR, B>(t);
+#pragma unroll
+    for (int k2 = 0; k2 < B; ++k2) v[k1 + A * k2] = t[k2];
+  }
+}
+
 template <int DIR, int P>
 __device__ __forceinline__ void dft(float2 *v)
 {
-  if (P == 2) dft2<DIR>(v);
-  if (P == 3) dft3<DIR>(v);
-  if (P == 4) dft4<DIR>(v);
-  if (P == 5) dft5<DIR>(v);
-  if (P == 7) dft7<DIR>(v);
-  if (P == 8) dft8<DIR>(v);
+  if constexpr (P == 1) {
+  } else if constexpr (P == 2) {
+    dft2<DIR>(v);
+  } else if constexpr (P == 3) {
+    dft3<DIR>(v);
+  } else if constexpr (P == 4) {
+    dft4<DIR>(v);
+  } else if constexpr (P == 5) {
+    dft5<DIR>(v);
+  } else if constexpr (P == 7) {
+    dft7<DIR>(v);
+  } else if constexpr (P == 8) {
+    dft8<DIR>(v);
+  } else {
+    dft_composite<DIR, P>(v);
+  }
 }
 
 }  // namespace cgdft
