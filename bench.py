@@ -209,6 +209,42 @@ def bench_mix(ctx, n=64, reps=3):
     }
 
 
+def bench_large(ctx, steps=2):
+    """The large single cube with simulation: 1024 x 1024 x 320 padded to 1152 x 1152 x 384, 6
+    angles 0-50 deg, Ormsby 5-10-60-80 Hz wavelet, 1500 m lateral range, 20 posterior realisations
+    (streamed through one set of three output grids), covariance kept as the 6 Hermitian spectra."""
+    sp = S.spec(1024, 1024, 320, tuple(float(a) for a in range(0, 60, 10)), 25.0, seed=20261205, nxp=1152,
+                nyp=1152, nzp=384, lateral_range=1500.0, ormsby_hz=(5.0, 10.0, 60.0, 80.0), realisations=20,
+                out_sets=1)
+    t0 = time.perf_counter()
+    prob = S.Problem(ctx, sp)
+    ctx.autotune(1152, 1152, 384)
+    log("large cube setup %.1f s (%.1f GB)" % (time.perf_counter() - t0, S.footprint(sp) / 1e9))
+    prob.step()
+    ctx.sync()
+    t = time.perf_counter()
+    for _ in range(steps):
+        prob.step()
+    ctx.sync()
+    ms = 1000.0 * (time.perf_counter() - t) / steps
+    t = time.perf_counter()
+    prob.step(realisations=0)
+    ctx.sync()
+    ms0 = 1000.0 * (time.perf_counter() - t)
+    G = (sp["nxp"] // 2 + 1) * sp["nyp"] * sp["nzp"] * 8
+    nt = len(sp["angles"])
+    cells = S.cells(sp)
+    prob.close()
+    return {
+        "workload": "single cube 1024x1024x320 padded 1152x1152x384, 6 angles 0-50 deg, Ormsby 5-10-60-80 Hz, "
+                    "1500 m lateral range, posterior mean + covariance + 20 realisations",
+        "ms_per_step": ms, "cells_per_s": cells / (ms * 1e-3), "mean_cov_only_ms": ms0,
+        "model_v1_frac": G * (5 * (3 + nt) + 21 + 24 * 20) / (ms * 1e-3) / HBM_PEAK,
+        "mean_cov_model_v1_frac": G * (5 * (3 + nt) + 21) / (ms0 * 1e-3) / HBM_PEAK,
+        "steps": steps,
+    }
+
+
 def bench_serving(n=48, lanes=3, load=0.7):
     """Continuous batching under Poisson arrivals (crava_amd.serve): requests of log-uniform sizes
     64^2 x 64 .. 384^2 x 256, 3-8 angles, 0-10 realisations, admitted against a 200 GB HBM
@@ -249,6 +285,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-mix", action="store_true", help="skip the batched-vs-plain uneven mix")
     ap.add_argument("--no-serve", action="store_true", help="skip the continuous-batching serving run")
+    ap.add_argument("--no-large", action="store_true", help="skip the 1152x1152x384 cube with 20 realisations")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a copy-like kernel of known traffic (for scaling hardware byte counters)")
     ap.add_argument("--profile-steps", type=int, default=2)
@@ -391,6 +428,9 @@ def main():
     if not args.no_serve:
         log("serving ...")
         result["serving"] = bench_serving()
+    if not args.no_large:
+        log("large cube ...")
+        result["large_cube"] = bench_large(ctx)
     if not args.no_cpu:
         log("cpu baseline ...")
         try:

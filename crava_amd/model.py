@@ -34,12 +34,27 @@ def ricker(t_ms, peak_hz):
     return (1 - 2 * c) * np.exp(-c)
 
 
-def wavelet_fft_order(nzp, dt_ms, peak_hz, scale=1.0):
+def ormsby(t_ms, f1, f2, f3, f4):
+    """Zero-phase Ormsby band-pass wavelet with corner frequencies f1 < f2 < f3 < f4 (Hz), peak 1
+    (the configs' band-limited wavelet; the reference reads such wavelets from file)."""
+    t = np.asarray(t_ms, dtype=np.float64) * 1e-3
+
+    def term(f):
+        return np.pi * f * f * np.sinc(f * t) ** 2  # numpy sinc(x) = sin(pi x) / (pi x)
+
+    w = (term(f4) - term(f3)) / (f4 - f3) - (term(f2) - term(f1)) / (f2 - f1)
+    w0 = (np.pi * (f4 * f4 - f3 * f3) / (f4 - f3)) - (np.pi * (f2 * f2 - f1 * f1) / (f2 - f1))
+    return w / w0
+
+
+def wavelet_fft_order(nzp, dt_ms, peak_hz, scale=1.0, ormsby_hz=None):
     """A zero-phase wavelet sampled at z = dt k (k <= nzp/2) and dt (k - nzp) otherwise: the FFT-order
-    layout Wavelet::resample produces (wavelet.cpp:424-475), sampled analytically."""
+    layout Wavelet::resample produces (wavelet.cpp:424-475), sampled analytically.  Ricker of the
+    peak frequency, or Ormsby with the corner frequencies `ormsby_hz`."""
     k = np.arange(nzp)
     t = np.where(k < nzp // 2 + 1, k, k - nzp) * dt_ms
-    return (scale * ricker(t, peak_hz)).astype(np.float32)
+    w = ricker(t, peak_hz) if ormsby_hz is None else ormsby(t, *ormsby_hz)
+    return (scale * w).astype(np.float32)
 
 
 def sigma0(std, corr):
@@ -105,11 +120,11 @@ def angular_error_covariance(sigma_e, angles_deg, range_deg=10.0):
 
 
 def inversion_tables(nz, nzp, dt_ms, angles_deg, peak_hz, wavelet_scales=None, vsvp=0.5, low_cut=5.0,
-                     high_cut=55.0):
+                     high_cut=55.0, ormsby_hz=None):
     """Reflection matrix, wavelets and the per-k_z tables (K, e1, e2, e3, invert)."""
     nt = len(angles_deg)
     scales = wavelet_scales if wavelet_scales is not None else [1.0] * nt
     A = aki_richards(angles_deg, vsvp)
-    wav = np.stack([wavelet_fft_order(nzp, dt_ms, peak_hz, s) for s in scales])
+    wav = np.stack([wavelet_fft_order(nzp, dt_ms, peak_hz, s, ormsby_hz) for s in scales])
     K, e1, e2, e3, inv = build_kz_tables(nz, nzp, dt_ms, wav, A, low_cut, high_cut, 1.0)
     return dict(A=A, wavelets=wav, K=K, e1=e1, e2=e2, e3=e3, invert=inv)

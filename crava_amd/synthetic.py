@@ -23,11 +23,14 @@ BACKGROUND = (math.log(3000.0), math.log(1500.0), math.log(2.3))
 
 def spec(nx, ny, nz, angles, peak_hz, seed, nxp=None, nyp=None, nzp=None, dx=25.0, dy=25.0, dt=4.0,
          std=(0.10, 0.15, 0.05), corr=(0.7, 0.4, 0.3), lateral_range=1000.0, temporal_range=40.0, wnc=0.1,
-         vsvp=0.5, low_cut=5.0, high_cut=55.0, realisations=0, wavelet_scale=(0.8, 1.2), sn=(3.0, 8.0)):
+         vsvp=0.5, low_cut=5.0, high_cut=55.0, realisations=0, wavelet_scale=(0.8, 1.2), sn=(3.0, 8.0),
+         ormsby_hz=None, out_sets=None):
     """One problem.  Unpadded sizes are padded by the reference's rule when nxp/nyp/nzp are not
     given: x/y by half the lateral range, z by a 200 ms wavelet length
     (CommonData::EstimateXYPaddingSizes, commondata.cpp:9621-9682; EstimateZPaddingSize,
-    multiintervalgrid.cpp:885-912), each rounded up to a 2^a 3^b 5^c 7^d length."""
+    multiintervalgrid.cpp:885-912), each rounded up to a 2^a 3^b 5^c 7^d length.  ormsby_hz: corner
+    frequencies of an Ormsby wavelet instead of the Ricker; out_sets: realisation grid sets kept
+    (realisations beyond are streamed through them; default one set per realisation)."""
     if nxp is None:
         nxp = M.padding_size(nx, min(1.0, max(0.5 * lateral_range, dx) / (nx * dx)))
     if nyp is None:
@@ -37,7 +40,8 @@ def spec(nx, ny, nz, angles, peak_hz, seed, nxp=None, nyp=None, nzp=None, dx=25.
     return dict(nx=nx, ny=ny, nz=nz, nxp=nxp, nyp=nyp, nzp=nzp, dx=dx, dy=dy, dt=dt, angles=tuple(angles),
                 peak_hz=peak_hz, std=tuple(std), corr=tuple(corr), lateral_range=lateral_range,
                 temporal_range=temporal_range, wnc=wnc, vsvp=vsvp, low_cut=low_cut, high_cut=high_cut, seed=seed,
-                realisations=realisations, wavelet_scale=wavelet_scale, sn=sn)
+                realisations=realisations, wavelet_scale=wavelet_scale, sn=sn, ormsby_hz=ormsby_hz,
+                out_sets=out_sets)
 
 
 def cells(s):
@@ -50,10 +54,16 @@ def grid_bytes(s):
     return 2 * (s["nxp"] // 2 + 1) * s["nyp"] * s["nzp"] * 4
 
 
+def out_sets(s):
+    """Realisation grid sets a problem holds (at least one, used to draw the synthetic truth)."""
+    k = s.get("out_sets")
+    return max(1, s["realisations"] if k is None else k)
+
+
 def ngrids(s):
     """Grids a problem holds: 3 + n_theta inputs, 3 + n_theta work spectra, 6 covariance grids and
-    3 grids per realisation (at least one set, used to draw the synthetic truth)."""
-    return 2 * (3 + len(s["angles"])) + 6 + 3 * max(1, s["realisations"])
+    3 grids per realisation set."""
+    return 2 * (3 + len(s["angles"])) + 6 + 3 * out_sets(s)
 
 
 def footprint(s):
@@ -114,7 +124,7 @@ class Problem:
         scales = rng.uniform(*s["wavelet_scale"], nt)
         sn = rng.uniform(*s["sn"], nt)
         tab = M.inversion_tables(nz, nzp, s["dt"], s["angles"], s["peak_hz"], scales, s["vsvp"], s["low_cut"],
-                                 s["high_cut"])
+                                 s["high_cut"], s.get("ormsby_hz"))
         cxy = M.spherical_corr_xy(nxp, nyp, s["dx"], s["dy"], s["lateral_range"])
         self.rxy = M.lateral_spectrum(cxy)
         self.rt = M.temporal_spectrum(M.circ_corr_t(s["dt"], s["temporal_range"], nzp, s["low_cut"]))
@@ -125,7 +135,7 @@ class Problem:
         self.mean_w = [mk(PARAMETER) for _ in range(3)]
         self.data_w = [mk(DATA) for _ in range(nt)]
         self.cov = [mk(COVARIANCE) for _ in range(6)]
-        self.out = [[mk(PARAMETER) for _ in range(3)] for _ in range(max(self.R, 1))]
+        self.out = [[mk(PARAMETER) for _ in range(3)] for _ in range(out_sets(s))]
         for g, v in zip(self.mean_in, BACKGROUND):
             g.fillInConstant(v)
         # prior covariance spectra: a posterior with no inverted plane writes the prior
@@ -167,13 +177,15 @@ class Problem:
                        lam2d=self.rxy, want_residual=False, precision=precision, cond_limit=1e6, stats=stats)
         for g in self.mean_w:
             g.invFFTInPlace()
-        for r in range(R):
-            simulate(self.ctx, self.cov, self.mean_w, self.out[r], seed=self.s["seed"], realisation=r)
+        for r in range(R):  # realisations beyond the kept sets are streamed through them
+            simulate(self.ctx, self.cov, self.mean_w, self.out[r % len(self.out)], seed=self.s["seed"], realisation=r)
         return st
 
     def item(self, realisations=None):
         """Batch descriptor: the same inputs, work grids and outputs as step()."""
         R = self.R if realisations is None else realisations
+        if R > len(self.out):
+            raise ValueError("a batch needs one realisation grid set per realisation")
         return dict(tables=self.tables, rxy=self.rxy, rt=self.rt, lam2d=self.rxy, seed=self.s["seed"],
                     mean_in=self.mean_in, seis_in=self.seis_in, mean=self.mean_w, data=self.data_w, cov=self.cov,
                     out=self.out[:R])
