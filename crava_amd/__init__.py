@@ -94,6 +94,7 @@ def lib():
             "cg_grid_add_white_noise": [_vp, f, u64, ctypes.c_uint32],
             "cg_grid_sum_squares": [_vp, i, _d_p],
             "cg_ctx_set_fft_engine": [_vp, i], "cg_ctx_set_fft_variant": [_vp, i],
+            "cg_ctx_set_fused_colour": [_vp, i],
             "cg_autotune": [_vp, i, i, i],
             "cg_arena_create": [_vp, u64, ctypes.POINTER(_vp)], "cg_arena_destroy": [_vp],
             "cg_grid_create_in": [_vp, i, i, i, i, i, i, i, ctypes.POINTER(_vp)],
@@ -151,6 +152,11 @@ class Context:
     def set_fft_variant(self, variant):
         """Force tile variant 0 or 1 of the compile-time-plan passes, or -1 for the tuned choice."""
         self.check(lib().cg_ctx_set_fft_variant(self._h, variant), "cg_ctx_set_fft_variant")
+
+    def set_fused_colour(self, mode):
+        """Realisations with counter noise: -1 automatic (autotuned, else by occupancy), 0 the
+        colouring kernel then three z passes, 1 colouring fused into the inverse z pass."""
+        self.check(lib().cg_ctx_set_fused_colour(self._h, mode), "cg_ctx_set_fused_colour")
 
     def set_fft_engine(self, engine):
         """FFT_AUTO (compile-time plans where available) or FFT_GENERIC (runtime plans only)."""

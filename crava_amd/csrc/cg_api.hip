@@ -352,6 +352,13 @@ int cg_ctx_set_fft_variant(cg_ctx *ctx, int variant)
   return CG_OK;
 }
 
+int cg_ctx_set_fused_colour(cg_ctx *ctx, int mode)
+{
+  if (!ctx || mode < -1 || mode > 1) return CG_EINVAL;
+  ctx->fused_colour = mode;
+  return CG_OK;
+}
+
 int cg_autotune(cg_ctx *ctx, int nxp, int nyp, int nzp)
 {
   if (!ctx || nxp < 1 || nyp < 1 || nzp < 1) return CG_EINVAL;
@@ -910,12 +917,18 @@ int cg_simulate(cg_ctx *ctx, const cg_sim_args *a)
   int rc = CG_OK;
   if (p.fails) rc = cg::zero_counters(ctx, p.fails, 1);
   if (rc) return rc;
-  rc = cg::colour(ctx, p);
+  // Counter noise with a compiled z plan: the colouring runs inside the inverse z pass, so the
+  // coloured spectra never make a round trip through HBM.
+  bool z_done = false;
+  if (!p.from_grids && ctx->fft_engine == CG_FFT_AUTO && p.nzp > 1)
+    z_done = cg::fixed_colour_zinv(ctx, p, p.nzp, (long)p.cnxp * p.nyp, &rc) != 0;
+  if (!z_done) rc = cg::colour(ctx, p);
   if (rc) return rc;
   for (int g = 0; g < 3; ++g) {
     a->out[g]->cubetype = CG_CT_PARAMETER;
     a->out[g]->transformed = 1;
-    rc = cg::grid_fft_inverse(a->out[g], a->mean[g] ? a->mean[g]->d : nullptr);
+    const float *add = a->mean[g] ? a->mean[g]->d : nullptr;
+    rc = z_done ? cg::grid_fft_inverse_xy(a->out[g], add) : cg::grid_fft_inverse(a->out[g], add);
     if (rc) return rc;
     a->out[g]->transformed = 0;
   }

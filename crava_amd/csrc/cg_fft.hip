@@ -509,7 +509,7 @@ int grid_fft_forward(cg_grid *g, const float *src)
 }
 
 // FFTGrid::invFFTInPlace: z, y complex, x c2r with the scale (and an optional real add) fused.
-int grid_fft_inverse(cg_grid *g, const float *add_real)
+static int grid_fft_inverse_impl(cg_grid *g, const float *add_real, bool with_z)
 {
   cg_ctx *ctx = g->ctx;
   const size_t N = (size_t)g->nxp * g->nyp * g->nzp;
@@ -517,7 +517,7 @@ int grid_fft_inverse(cg_grid *g, const float *add_real)
   float2 *c = reinterpret_cast<float2 *>(g->d);
   const long cnxp = g->cnxp, plane = cnxp * g->nyp;
   int rc;
-  if (g->nzp > 1) {
+  if (with_z && g->nzp > 1) {
     rc = fft_strided(ctx, c, g->nzp, plane, plane, 1, 0, +1, 1.0f, CG_K_FFT_Z);
     if (rc) return rc;
   }
@@ -527,5 +527,9 @@ int grid_fft_inverse(cg_grid *g, const float *add_real)
   }
   return fft_x_c2r(ctx, g->d, g->nxp, (long)g->nyp * g->nzp, s, add_real);
 }
+
+int grid_fft_inverse(cg_grid *g, const float *add_real) { return grid_fft_inverse_impl(g, add_real, true); }
+
+int grid_fft_inverse_xy(cg_grid *g, const float *add_real) { return grid_fft_inverse_impl(g, add_real, false); }
 
 }  // namespace cg

@@ -16,6 +16,7 @@
 // The plans (n, TPS, radices) come from cg_fft_plans.h.
 #pragma once
 
+#include "cg_colour.h"
 #include "cg_dft.h"
 #include "cg_fft_plans.h"
 #include "cg_internal.h"
@@ -541,7 +542,85 @@ k_fftxo_c2r(cg::XArgs one, const cg::XArgs *__restrict__ items, int nitems, cons
   }
 }
 
+// ---------------------------------------------------------------- realisation: colour + inverse z
+// AVOInversion::simulate colours the noise (avoinversion.cpp:1421-1460) and then transforms the
+// three grids back (z, y, x passes).  Here one workgroup colours TI columns over all N planes
+// of the three output grids straight into LDS (counter noise, chol(Sigma_post), L z) and runs
+// their inverse z transforms before anything reaches HBM: the coloured spectra are never
+// written and read back.  The y and x passes follow as usual.
+// TI columns: at most 16, within kLds (two workgroups per CU) and 1024 threads.
+template <int N>
+constexpr int colour_tile()
+{
+  constexpr int T = Plan<N>::TPS;
+  int ti = 16;
+  while (ti > 1 && ((size_t)(N + 3 * (size_t)N * ti) * 8 > kLds || 3 * ti * T > 1024)) ti /= 2;
+  return ti;
+}
+
+template <int N, int TI>
+__global__ void __launch_bounds__(3 * TI *Plan<N>::TPS)
+k_colour_zinv(cg::ColourParams p, long plane, const float2 *__restrict__ twg)
+{
+  using IO = typename Plan<N>::template IO<+1, TI>;
+  constexpr int TPS = Plan<N>::TPS;
+  constexpr int NT = 3 * TI * TPS;
+  constexpr int CELLS = N * TI;
+  constexpr int PER = (CELLS + NT - 1) / NT;
+  extern __shared__ float2 smem[];
+  float2 *tw = smem;
+  float2 *buf = smem + N;  // grid g, plane z, column c at (g N + z) TI + c
+  const int tid = threadIdx.x;
+  const long q0 = (long)cg_xcd_remap(blockIdx.x, gridDim.x) * TI;
+  load_tw(tw, twg, N, tid, NT);
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const int e = tid + i * NT;
+    const int c = e % TI, z = e / TI;
+    const bool live = e < CELLS && q0 + c < plane;
+    float2 zn[3], o[3] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f), make_float2(0.f, 0.f)};
+    bool failed = false;
+    if (live) failed = cgcol::colour_cell(p, q0 + c + (long)z * plane, zn, o);
+    if (p.fails) {  // one atomic per wavefront, issued by the lowest failing lane
+      const unsigned long long msk = __ballot(failed);
+      if (failed && (int)__lane_id() == __ffsll((long long)msk) - 1)
+        atomicAdd(p.fails, (unsigned long long)__popcll(msk));
+    }
+    if (e < CELLS) {
+#pragma unroll
+      for (int g = 0; g < 3; ++g) buf[(g * N + z) * TI + c] = o[g];
+    }
+  }
+  __syncthreads();
+  const int g = tid / (TI * TPS), r = tid % (TI * TPS), c = r % TI, t = r / TI;
+  const bool live = q0 + c < plane;
+  float2 *col = p.out[g] + (live ? q0 + c : 0);
+  auto sink = [&](int e, float2 x) {
+    if (live) col[(long)e * plane] = x;
+  };
+  IO::from_lds(buf + g * N * TI + c, t, tw, sink);
+}
+
+// Wavefronts of the fused kernel one CU holds (LDS-limited).
+template <int N>
+constexpr int colour_waves()
+{
+  constexpr int TI = colour_tile<N>();
+  constexpr size_t lds = sizeof(float2) * (N + 3 * (size_t)N * TI);
+  return (int)(kLdsMax / lds) * ((3 * TI * Plan<N>::TPS + 63) / 64);
+}
+
 // ---------------------------------------------------------------- host launchers
+template <int N>
+int colour_zinv_launch(cg_ctx *ctx, const cg::ColourParams &p, long plane, const float2 *tw)
+{
+  constexpr int TI = colour_tile<N>();
+  const size_t lds = sizeof(float2) * (N + 3 * (size_t)N * TI);
+  const unsigned nblk = (unsigned)((plane + TI - 1) / TI);
+  k_colour_zinv<N, TI><<<nblk, 3 * TI * Plan<N>::TPS, lds, ctx->stream>>>(p, plane, tw);
+  return cg::hip_fail(ctx, hipGetLastError(), "k_colour_zinv");
+}
+
 template <int N, int V>
 int strided_launch(cg_ctx *ctx, int dir, const cg::StridedArgs *one, const cg::StridedArgs *items, int nitems,
                    unsigned nblocks, const float2 *tw)

@@ -7,6 +7,7 @@
 
 #include "cg_fft_plans.h"
 #include "cg_internal.h"
+#include "cg_noise.h"
 
 namespace cg {
 
@@ -136,6 +137,33 @@ int fixed_x_list(cg_ctx *ctx, const FixedEntry *e, int v, int forward, int add, 
   return e->x[v](ctx, forward, !forward && add, nullptr, dev_items, nitems, nblocks, tw, twr);
 }
 
+// Tune-table key of the fused-or-not choice for a realisation of nzp planes of `plane` cells.
+static TuneKey colour_key(const cg_ctx *ctx, int nzp, long plane)
+{
+  return TuneKey{(long)ctx->device, (long)FIX_KINDS, (long)nzp, plane, 0, 0};
+}
+
+int fixed_colour_zinv(cg_ctx *ctx, const ColourParams &p, int nzp, long plane, int *rc)
+{
+  const FixedEntry *e = fixed_find(FIX_STRIDED, nzp);
+  if (!e || !e->cz || p.from_grids || ctx->fused_colour == 0) return 0;
+  if (ctx->fused_colour != 1) {
+    std::lock_guard<std::mutex> lock(tune_mu);
+    const auto it = tune_table.find(colour_key(ctx, nzp, plane));
+    if (it != tune_table.end() ? it->second == 0 : e->cz_waves < 12) return 0;
+  }
+  const float2 *tw = twiddle_table(ctx, nzp);
+  if (!tw) {
+    *rc = CG_ENOMEM;
+    return 1;
+  }
+  hipEvent_t ev;
+  launch_begin(ctx, CG_K_NOISE, &ev);
+  *rc = e->cz(ctx, p, plane, tw);
+  launch_end(ctx, CG_K_NOISE, ev);
+  return 1;
+}
+
 // Times both tile variants of every pass of a grid of padded size nxp x nyp x nzp (x forward
 // out of place, x inverse with and without the added grid, y and z forward + inverse) on scratch
 // grids and records the faster for that geometry.
@@ -160,7 +188,7 @@ int autotune_geometry(cg_ctx *ctx, int nxp, int nyp, int nzp)
       if (f()) return -1.f;
     if (hipEventRecord(t1, ctx->stream) != hipSuccess || hipEventSynchronize(t1) != hipSuccess) return -1.f;
     float ms = 0.f;
-    hipEventElapsedTime(&ms, t0, t1);
+    (void)hipEventElapsedTime(&ms, t0, t1);
     return ms / reps;
   };
   auto record = [&](const FixedEntry *en, long g0, long g1, long g2, const float (&ms)[2]) {
@@ -204,6 +232,46 @@ int autotune_geometry(cg_ctx *ctx, int nxp, int nyp, int nzp)
       });
     }
     record(es, S, inner, ncols, ms);
+  }
+  // A realisation's colouring and inverse z pass: fused, or the colouring kernel followed by
+  // three z passes.  Posterior covariance spectra: the identity (diagonal A = 1, off-diagonal
+  // B = 0), so every cell takes the factorisation's full path; three separate output grids.
+  const FixedEntry *ez = nzp > 1 ? fixed_find(FIX_STRIDED, nzp) : nullptr;
+  if (!rc && ez && ez->cz) {
+    const float2 *tw = twiddle_table(ctx, nzp);
+    float2 *out[3] = {nullptr, nullptr, nullptr};
+    for (int g = 0; g < 3 && e == hipSuccess; ++g) e = hipMalloc(&out[g], sizeof(float2) * csize);
+    if (e == hipSuccess)
+      rc = fill_constant(ctx, reinterpret_cast<float *>(A), 2 * (size_t)csize, 1, 1, 1.0f);  // every float
+    if (tw && e == hipSuccess && !rc) {
+      ColourParams p{};
+      p.csize = csize;
+      p.cnxp = (int)cnxp;
+      p.nyp = nyp;
+      p.nzp = nzp;
+      for (int g = 0; g < 6; ++g) p.cov[g] = g < 3 ? A : B;
+      for (int g = 0; g < 3; ++g) p.out[g] = out[g];
+      p.seed = 1;
+      const int v = fixed_variant(ctx, ez, plane, plane, plane);
+      StridedArgs sa[3];
+      for (int g = 0; g < 3; ++g) sa[g] = StridedArgs{out[g], plane, 0, (unsigned)plane, (unsigned)plane, 0u, 1.0f};
+      const unsigned nblk = (unsigned)((plane + ez->tile[v] - 1) / ez->tile[v]);
+      float ms[2];
+      ms[0] = time_it([&] {
+        int r = colour(ctx, p);
+        for (int g = 0; g < 3 && !r; ++g) r = ez->s[v](ctx, +1, &sa[g], nullptr, 0, nblk, tw);
+        return r;
+      });
+      ms[1] = time_it([&] { return ez->cz(ctx, p, plane, tw); });
+      if (ms[0] >= 0.f && ms[1] >= 0.f) {
+        std::lock_guard<std::mutex> lock(tune_mu);
+        tune_table[colour_key(ctx, nzp, plane)] = ms[1] < ms[0] ? 1 : 0;
+      }
+    }
+    (void)hipStreamSynchronize(ctx->stream);
+    for (int g = 0; g < 3; ++g)
+      if (out[g]) (void)hipFree(out[g]);
+    if (e != hipSuccess && !rc) rc = hip_fail(ctx, e, "autotune scratch");
   }
   if (t0) (void)hipEventDestroy(t0);
   if (t1) (void)hipEventDestroy(t1);

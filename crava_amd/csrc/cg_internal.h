@@ -67,11 +67,15 @@ typedef int (*StridedFn)(cg_ctx *, int dir, const StridedArgs *one, const Stride
                          unsigned nblocks, const float2 *tw);
 typedef int (*XFn)(cg_ctx *, int forward, int add, const XArgs *one, const XArgs *items, int nitems,
                    unsigned nblocks, const float2 *tw, const float2 *twr);
+struct ColourParams;
+typedef int (*ColourZFn)(cg_ctx *, const ColourParams &p, long plane, const float2 *tw);
 struct FixedEntry {
   int kind, n;
   int tile[2];  // the two tile variants (equal when the length has only one)
   StridedFn s[2];
   XFn x[2];
+  ColourZFn cz;  // strided lengths: colouring fused with the inverse z pass
+  int cz_waves;  // wavefronts of the fused kernel resident per CU
 };
 // Tile variant for a pass geometry: the autotuner's choice (cg_autotune) for this exact
 // geometry, or the context's forced variant, else 0.  `geom` identifies the pass: strided
@@ -115,6 +119,7 @@ struct cg_ctx {
   hipEvent_t join[kSide] = {};
   int fft_engine = CG_FFT_AUTO;
   int fft_variant = -1;  // -1: autotuned choice or 0; 0/1: forced tile variant
+  int fused_colour = -1;  // -1: autotuned or occupancy rule; 0: never; 1: always (compiled nzp)
   bool profiling = false;
   std::vector<cg::TimedLaunch> timed;
   std::vector<hipEvent_t> event_pool;
@@ -175,6 +180,13 @@ int fixed_strided_list(cg_ctx *ctx, const FixedEntry *e, int v, int dir, const S
 int fixed_x_list(cg_ctx *ctx, const FixedEntry *e, int v, int forward, int add, const XArgs *dev_items, int nitems,
                  unsigned nblocks, int nxp);
 int autotune_geometry(cg_ctx *ctx, int nxp, int nyp, int nzp);
+// Colouring of one realisation fused with the inverse z pass of its three grids (counter noise
+// only); returns 1 and sets *rc if it ran: nzp has a compiled plan and the fused kernel is the
+// faster choice for this geometry (autotuned, else when it keeps at least 12 wavefronts per CU
+// resident; with fewer the colouring's loads are poorly hidden).
+int fixed_colour_zinv(cg_ctx *ctx, const ColourParams &p, int nzp, long plane, int *rc);
+// grid_fft_inverse without the z pass (it was fused into the producer).
+int grid_fft_inverse_xy(cg_grid *g, const float *add_real);
 int grid_fft_inverse(cg_grid *g, const float *add_real);
 
 // Elementwise kernels (cg_noise.hip).

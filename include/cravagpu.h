@@ -84,6 +84,13 @@ int cg_ctx_set_fft_engine(cg_ctx *ctx, int engine);
  * choice (-1). */
 int cg_autotune(cg_ctx *ctx, int nxp, int nyp, int nzp);
 int cg_ctx_set_fft_variant(cg_ctx *ctx, int variant);
+/* Realisations with counter noise (cg_simulate, no reference counterpart: a schedule choice):
+ * the colouring can run inside the inverse z pass of the three output grids (one kernel, the
+ * coloured spectra never written to HBM) or as its own kernel followed by three z passes.  Mode
+ * -1 (default): the autotuned choice for the geometry (cg_autotune times both), else fused when
+ * the fused kernel keeps at least 12 wavefronts per CU resident; 0: never fused; 1: fused
+ * whenever nzp has a compiled plan.  Results agree to FP32 rounding either way. */
+int cg_ctx_set_fused_colour(cg_ctx *ctx, int mode);
 
 /* ------------------------------------------------------------------ grids (FFTGrid) */
 /* FFTGrid(nx,ny,nz,nxp,nyp,nzp) + createRealGrid/createComplexGrid (fftgrid.cpp:40, 1048-1063),

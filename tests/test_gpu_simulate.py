@@ -52,10 +52,12 @@ def test_reference_noise_stream_is_bit_identical(ctx, shape):
     g.close()
 
 
-@pytest.mark.parametrize("source", ["grids", "device_reference", "counter"])
-def test_realisation_matches_oracle(ctx, source):
+@pytest.mark.parametrize("source,fused", [("grids", -1), ("device_reference", -1), ("counter", 0), ("counter", 1)])
+def test_realisation_matches_oracle(ctx, source, fused):
     """grids: the reference stream drawn on the host and uploaded; device_reference: the same
-    stream drawn on the device (cg_grid_fill_noise_reference); counter: the counter generator."""
+    stream drawn on the device (cg_grid_fill_noise_reference); counter: the counter generator,
+    with the colouring as its own kernel (fused 0) or inside the inverse z pass (fused 1)."""
+    ctx.set_fused_colour(fused)
     p = P.make_problem(16, 12, 20, seed=11)
     om, oc = _posterior_cov(p)
     nxp, nyp, nzp = p["nxp"], p["nyp"], p["nzp"]
@@ -80,7 +82,38 @@ def test_realisation_matches_oracle(ctx, source):
     fails = C.simulate(ctx, gc, gmean, gout, seed=99, realisation=4,
                        noise_source=C.NOISE_COUNTER if source == "counter" else C.NOISE_GRIDS, stats=True)
     got = np.stack([g.download_real() for g in gout])
+    ctx.set_fused_colour(-1)
     assert fails == nf
     assert rel_l2(got - mean_real, want - mean_real) < 1e-5
+    for g in gc + gmean + gout:
+        g.close()
+
+
+@pytest.mark.parametrize("nz,nzp", [(256, 320), (320, 384)])
+def test_fused_colour_at_production_depths(ctx, nz, nzp):
+    """The colouring fused into the inverse z pass at the z lengths of the single-cube and
+    large-cube workloads (workgroups of 8 columns, the last one partly outside the plane) and the
+    unfused schedule both match the oracle."""
+    p = P.make_problem(11, 7, nz, nzp=nzp, seed=5)
+    om, oc = _posterior_cov(p)
+    nxp, nyp = p["nxp"], p["nyp"]
+    assert ((nxp // 2 + 1) * nyp) % 8 != 0
+    mean_real = np.stack([O.grid_ifft(m, nxp, O.PARAMETER, np.float32) for m in om])
+    noise = np.stack([O.fill_complex_noise_cb(nxp, nyp, nzp, 7, 3 * 2 + g) for g in range(3)])
+    z, nf = O.colour(oc, noise, "f32")
+    want = np.stack([O.grid_ifft(z[g], nxp, O.PARAMETER) + mean_real[g] for g in range(3)])
+    gc, gmean, gout = _grids(ctx, p, 6, C.COVARIANCE), _grids(ctx, p, 3, C.PARAMETER), _grids(ctx, p, 3, C.PARAMETER)
+    for g in range(6):
+        gc[g].upload_complex(oc[g])
+    for g in range(3):
+        gmean[g].upload_real(mean_real[g])
+    for fused in (0, 1):
+        print("fused colour", nz, nzp, fused, flush=True)
+        ctx.set_fused_colour(fused)
+        fails = C.simulate(ctx, gc, gmean, gout, seed=7, realisation=2, noise_source=C.NOISE_COUNTER, stats=True)
+        got = np.stack([g.download_real() for g in gout])
+        assert fails == nf
+        assert rel_l2(got - mean_real, want - mean_real) < 1e-5
+    ctx.set_fused_colour(-1)
     for g in gc + gmean + gout:
         g.close()
