@@ -232,6 +232,17 @@ __device__ __forceinline__ unsigned resolve(const A &one, const A *items, int ni
   return g - a.block0;
 }
 
+// As resolve(), with the tiles of each grid taken in descending order (nblk(a): the grid's
+// tiles).  The y and z passes run this way and the x passes ascending, so that a pass starts on
+// the part of the grid its predecessor wrote last (x forward then y; y inverse then x), which is
+// still in the memory-side cache; with the XCD remap each XCD walks its band of the grid.
+template <typename A, class NB>
+__device__ __forceinline__ unsigned resolve_desc(const A &one, const A *items, int nitems, A &a, NB nblk)
+{
+  const unsigned g = resolve(one, items, nitems, a);
+  return nblk(a) - 1u - g;
+}
+
 // ---------------------------------------------------------------- y / z passes
 // Columns are numbered q = c + inner * o (c < inner the contiguous index, o < outer the slab) and
 // a block takes TI consecutive q, so every tile but the grid's last is full even where inner is
@@ -247,7 +258,7 @@ k_ffts(cg::StridedArgs one, const cg::StridedArgs *__restrict__ items, int nitem
   float2 *tw = smem;
   float2 *buf = smem + N;
   cg::StridedArgs a;
-  const unsigned blk = resolve(one, items, nitems, a);
+  const unsigned blk = resolve_desc(one, items, nitems, a, [](const cg::StridedArgs &x) { return (x.ncols + TI - 1) / TI; });
   const int tid = threadIdx.x;
   const int c = tid % TI, t = tid / TI;
   const unsigned q = blk * TI + c;
