@@ -912,7 +912,7 @@ int cg_simulate(cg_ctx *ctx, const cg_sim_args *a)
   for (int g = 0; g < 3; ++g) p.out[g] = reinterpret_cast<float2 *>(a->out[g]->d);
   p.from_grids = a->noise_source == CG_NOISE_GRIDS;
   p.seed = a->seed;
-  p.stream0 = 3u * a->realisation;
+  p.stream0 = 4u * a->realisation;  // stream pairs 2r (both halves) and 2r + 1 (first half)
   p.fails = a->stats ? reinterpret_cast<unsigned long long *>(ctx->counters) : nullptr;
   int rc = CG_OK;
   if (p.fails) rc = cg::zero_counters(ctx, p.fails, 1);
@@ -1539,7 +1539,7 @@ int cg_batch_create(cg_ctx *ctx, int n, const cg_inversion *inv, cg_batch **out)
       for (int g = 0; g < 6; ++g) it.p.cov[g] = reinterpret_cast<const float2 *>(b->inv[i].post.cov[g]->d);
       for (int g = 0; g < 3; ++g) it.p.out[g] = reinterpret_cast<float2 *>(b->outs[i][3 * r + g]->d);
       it.p.seed = b->inv[i].seed;
-      it.p.stream0 = 3u * (unsigned)r;
+      it.p.stream0 = 4u * (unsigned)r;
       it.block0 = blk;
       blk += (unsigned)((ref->csize + cg::kThreads - 1) / cg::kThreads);
       ci.push_back(it);

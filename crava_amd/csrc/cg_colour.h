@@ -31,43 +31,87 @@ __device__ __forceinline__ void philox(uint32_t (&c)[4], uint32_t k0, uint32_t k
   }
 }
 
-// Two N(0,1) for (seed, stream, cell): u1 = (c0>>8 + 1) 2^-24 in (0, 1], u2 = (c1>>8) 2^-24.
-__device__ __forceinline__ float2 cb_normal2(uint64_t seed, uint32_t stream, uint64_t cell)
+// Box-Muller on two 32-bit words: u1 = (a>>8 + 1) 2^-24 in (0, 1], u2 = (b>>8) 2^-24.
+__device__ __forceinline__ float2 box_muller(uint32_t a, uint32_t b)
 {
-  uint32_t c[4] = {(uint32_t)cell, (uint32_t)(cell >> 32), stream, 0u};
-  philox(c, (uint32_t)seed, (uint32_t)(seed >> 32));
-  const float u1 = (float)((c[0] >> 8) + 1u) * 5.9604644775390625e-08f;
-  const float u2 = (float)(c[1] >> 8) * 5.9604644775390625e-08f;
+  const float u1 = (float)((a >> 8) + 1u) * 5.9604644775390625e-08f;
+  const float u2 = (float)(b >> 8) * 5.9604644775390625e-08f;
   const float r = sqrtf(-2.0f * logf(u1));
   float s, co;
   sincospif(2.0f * u2, &s, &co);
   return make_float2(r * co, r * s);
 }
 
-// Noise value of cell idx in the fillInComplexNoise layout.
-__device__ __forceinline__ float2 noise_cell(uint64_t seed, uint32_t stream, long idx, int cnxp, int nyp, int nzp)
+// One Philox block per (seed, cell, stream pair): counter (cell, stream >> 1, 0), key seed.
+__device__ __forceinline__ void cb_block(uint64_t seed, uint32_t pair, uint64_t cell, uint32_t (&c)[4])
 {
-  const float sd = 0.70710678118654752440f;  // float(1/sqrt(2)), fftgrid.cpp:1002
+  c[0] = (uint32_t)cell;
+  c[1] = (uint32_t)(cell >> 32);
+  c[2] = pair;
+  c[3] = 0u;
+  philox(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+}
+
+// Two N(0,1) for (seed, stream, cell): words 0-1 of the stream pair's block for an even stream,
+// words 2-3 for an odd one.
+__device__ __forceinline__ float2 cb_normal2(uint64_t seed, uint32_t stream, uint64_t cell)
+{
+  uint32_t c[4];
+  cb_block(seed, stream >> 1, cell, c);
+  return (stream & 1u) ? box_muller(c[2], c[3]) : box_muller(c[0], c[1]);
+}
+
+// The normal pairs of streams s0, s0 + 1, s0 + 2 at one cell: two Philox blocks.
+__device__ __forceinline__ void cb_normal2x3(uint64_t seed, uint32_t s0, uint64_t cell, float2 (&z)[3])
+{
+  uint32_t a[4], b[4];
+  cb_block(seed, s0 >> 1, cell, a);
+  cb_block(seed, (s0 >> 1) + 1u, cell, b);
+  if (s0 & 1u) {
+    z[0] = box_muller(a[2], a[3]);
+    z[1] = box_muller(b[0], b[1]);
+    z[2] = box_muller(b[2], b[3]);
+  } else {
+    z[0] = box_muller(a[0], a[1]);
+    z[1] = box_muller(a[2], a[3]);
+    z[2] = box_muller(b[0], b[1]);
+  }
+}
+
+// The fillInComplexNoise layout: cell idx takes its value from the normal pair (z0, z1) drawn at
+// cell *src: kind 0: (z0, z1) / sqrt 2; kind 1 (self-conjugate cell): (z0, 0); kind 2 (the
+// conjugate mirror of an earlier cell): (z0, -z1) / sqrt 2.
+__device__ __forceinline__ int noise_source(long idx, int cnxp, int nyp, int nzp, long *src)
+{
   const int xs = (int)(idx % cnxp);
+  *src = idx;
   if (xs == 0 || (xs == cnxp - 1 && (idx & 1))) {
     const long jk = idx / cnxp;
     const long jind = jk % nyp, kind = jk / nyp;
     const long jcc = jind == 0 ? 0 : nyp - jind;
     const long kcc = kind == 0 ? 0 : nzp - kind;
     const long jkcc = jcc + kcc * nyp;
-    if (jkcc == jk) {
-      const float2 z = cb_normal2(seed, stream, (uint64_t)idx);
-      return make_float2(z.x, 0.f);
-    }
-    if (jkcc > jk) {
-      const float2 z = cb_normal2(seed, stream, (uint64_t)idx);
-      return make_float2(sd * z.x, sd * z.y);
-    }
-    const float2 z = cb_normal2(seed, stream, (uint64_t)(jkcc * cnxp + xs));
-    return make_float2(sd * z.x, -(sd * z.y));
+    if (jkcc == jk) return 1;
+    if (jkcc > jk) return 0;
+    *src = jkcc * cnxp + xs;
+    return 2;
   }
-  const float2 z = cb_normal2(seed, stream, (uint64_t)idx);
-  return make_float2(sd * z.x, sd * z.y);
+  return 0;
+}
+
+__device__ __forceinline__ float2 noise_value(int kind, float2 z)
+{
+  const float sd = 0.70710678118654752440f;  // float(1/sqrt(2)), fftgrid.cpp:1002
+  if (kind == 1) return make_float2(z.x, 0.f);
+  return make_float2(sd * z.x, kind == 2 ? -(sd * z.y) : sd * z.y);
+}
+
+// Noise value of cell idx of one stream.
+__device__ __forceinline__ float2 noise_cell(uint64_t seed, uint32_t stream, long idx, int cnxp, int nyp, int nzp)
+{
+  long src;
+  const int kind = noise_source(idx, cnxp, nyp, nzp, &src);
+  return noise_value(kind, cb_normal2(seed, stream, (uint64_t)src));
 }
 
 struct c32 {
@@ -132,8 +176,11 @@ __device__ __forceinline__ bool colour_cell(const cg::ColourParams &p, long idx,
   // Hermitian completion of the six stored entries (avoinversion.cpp:1425-1437)
   c32 m[6] = {{c00.x, c00.y}, {c01.x, -c01.y}, {c11.x, c11.y}, {c02.x, -c02.y}, {c12.x, -c12.y}, {c22.x, c22.y}};
   if (!p.from_grids) {
+    long src;
+    const int kind = noise_source(idx, p.cnxp, p.nyp, p.nzp, &src);
+    cb_normal2x3(p.seed, p.stream0, (uint64_t)src, z);
 #pragma unroll
-    for (int g = 0; g < 3; ++g) z[g] = noise_cell(p.seed, p.stream0 + g, idx, p.cnxp, p.nyp, p.nzp);
+    for (int g = 0; g < 3; ++g) z[g] = noise_value(kind, z[g]);
   }
 #pragma unroll
   for (int g = 0; g < 3; ++g) o[g] = make_float2(0.f, 0.f);

@@ -192,6 +192,9 @@ typedef struct {
 int cg_posterior(cg_ctx *ctx, const cg_post_args *args);
 
 /* ------------------------------------------------------------------ simulation */
+/* Counter noise: stream s, complex cell c draws the Philox-4x32-10 block with counter
+ * (c, s >> 1, 0) and key seed, and takes a normal pair by Box-Muller from words 0-1 (even s)
+ * or 2-3 (odd s); the fillInComplexNoise layout then places the pairs (Hermitian cells). */
 #define CG_NOISE_COUNTER 0   /* counter-based Philox-4x32-10 + Box-Muller, keyed by seed */
 #define CG_NOISE_GRIDS 1     /* caller-supplied noise spectra in seed[] (e.g. the reference stream) */
 
@@ -201,7 +204,7 @@ typedef struct {
   cg_grid *out[3];          /* realisation grids: receive Vp, Vs, Rho real fields */
   int noise_source;         /* CG_NOISE_* */
   uint64_t seed;            /* CG_NOISE_COUNTER key */
-  uint32_t realisation;     /* CG_NOISE_COUNTER realisation number (streams 3r, 3r+1, 3r+2) */
+  uint32_t realisation;     /* CG_NOISE_COUNTER realisation number (streams 4r, 4r+1, 4r+2) */
   /* CG_NOISE_GRIDS: out[] already hold the noise spectra (fillInComplexNoise result) */
   uint64_t *stats;          /* optional host [1]: Cholesky failures (noise zeroed) */
 } cg_sim_args;
@@ -242,7 +245,7 @@ typedef struct {
   cg_grid *seis_in[8]; /* real-domain seismic cubes (DATA), left unchanged */
   int nreal;           /* posterior realisations */
   cg_grid **out;       /* [nreal][3] realisation grids (real domain after a run) */
-  uint64_t seed;       /* counter-noise key; realisation r uses streams 3r, 3r+1, 3r+2 */
+  uint64_t seed;       /* counter-noise key; realisation r uses streams 4r, 4r+1, 4r+2 */
 } cg_inversion;
 
 int cg_batch_create(cg_ctx *ctx, int n, const cg_inversion *inv, cg_batch **batch);

@@ -97,7 +97,9 @@ void   orc_rnorm01_n(uint32_t *state, double *out, long n);
 /* FFTGrid::fillInComplexNoise (fftgrid.cpp:997-1045), including the Nyquist-column parity rule. */
 void   orc_fill_complex_noise(int nxp, int nyp, int nzp, uint32_t *state, float *cvalue);
 /* Counter-based complex noise with the same Hermitian pattern (the serving/bench generator):
- * normal pair of complex cell `idx` of stream `stream` = orc_cb_normal2(seed, stream, idx). */
+ * normal pair of complex cell `idx` of stream `stream` = orc_cb_normal2(seed, stream, idx), Box-Muller
+ * on words 0-1 (even stream) or 2-3 (odd stream) of the Philox block of counter
+ * (idx, stream >> 1, 0) and key seed. */
 void   orc_cb_normal2(uint64_t seed, uint32_t stream, uint64_t idx, float *z0, float *z1);
 void   orc_fill_complex_noise_cb(int nxp, int nyp, int nzp, uint64_t seed, uint32_t stream, float *cvalue);
 void   orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);

@@ -132,16 +132,18 @@ static void philox(uint32_t ctr[4], const uint32_t key0[2])
   }
 }
 
-/* Two standard normals for (seed, stream, idx): Box-Muller on the 24-bit uniforms
- * u1 = ((c0 >> 8) + 1) 2^-24 in (0, 1] and u2 = (c1 >> 8) 2^-24, evaluated in double here
- * (the device evaluates the same formula in float). */
+/* Two standard normals for (seed, stream, idx): the Philox block of counter
+ * (idx, stream >> 1, 0), key seed; Box-Muller on words (w0, w1) = block words 0-1 for an even
+ * stream, 2-3 for an odd one: u1 = ((w0 >> 8) + 1) 2^-24 in (0, 1], u2 = (w1 >> 8) 2^-24,
+ * evaluated in double here (the device evaluates the same formula in float). */
 void orc_cb_normal2(uint64_t seed, uint32_t stream, uint64_t idx, float *z0, float *z1)
 {
-  uint32_t c[4] = {(uint32_t)idx, (uint32_t)(idx >> 32), stream, 0u};
+  uint32_t c[4] = {(uint32_t)idx, (uint32_t)(idx >> 32), stream >> 1, 0u};
   uint32_t k[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
   philox(c, k);
-  double u1 = (double)((c[0] >> 8) + 1u) * (1.0 / 16777216.0);
-  double u2 = (double)(c[1] >> 8) * (1.0 / 16777216.0);
+  const uint32_t w0 = (stream & 1u) ? c[2] : c[0], w1 = (stream & 1u) ? c[3] : c[1];
+  double u1 = (double)((w0 >> 8) + 1u) * (1.0 / 16777216.0);
+  double u2 = (double)(w1 >> 8) * (1.0 / 16777216.0);
   double r = sqrt(-2.0 * log(u1));
   double th = 2.0 * M_PI * u2;
   *z0 = (float)(r * cos(th));

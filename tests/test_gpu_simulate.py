@@ -66,7 +66,7 @@ def test_realisation_matches_oracle(ctx, source, fused):
         st = np.array([23665], dtype=np.uint32)  # the reference stream (RandomGen default seed)
         noise = np.stack([O.fill_complex_noise(nxp, nyp, nzp, st) for _ in range(3)])
     else:
-        noise = np.stack([O.fill_complex_noise_cb(nxp, nyp, nzp, 99, 3 * 4 + g) for g in range(3)])
+        noise = np.stack([O.fill_complex_noise_cb(nxp, nyp, nzp, 99, 4 * 4 + g) for g in range(3)])
     z, nf = O.colour(oc, noise, "f32")
     want = np.stack([O.grid_ifft(z[g], nxp, O.PARAMETER) + mean_real[g] for g in range(3)])
     gc, gmean, gout = _grids(ctx, p, 6, C.COVARIANCE), _grids(ctx, p, 3, C.PARAMETER), _grids(ctx, p, 3, C.PARAMETER)
@@ -99,7 +99,7 @@ def test_fused_colour_at_production_depths(ctx, nz, nzp):
     nxp, nyp = p["nxp"], p["nyp"]
     assert ((nxp // 2 + 1) * nyp) % 8 != 0
     mean_real = np.stack([O.grid_ifft(m, nxp, O.PARAMETER, np.float32) for m in om])
-    noise = np.stack([O.fill_complex_noise_cb(nxp, nyp, nzp, 7, 3 * 2 + g) for g in range(3)])
+    noise = np.stack([O.fill_complex_noise_cb(nxp, nyp, nzp, 7, 4 * 2 + g) for g in range(3)])
     z, nf = O.colour(oc, noise, "f32")
     want = np.stack([O.grid_ifft(z[g], nxp, O.PARAMETER) + mean_real[g] for g in range(3)])
     gc, gmean, gout = _grids(ctx, p, 6, C.COVARIANCE), _grids(ctx, p, 3, C.PARAMETER), _grids(ctx, p, 3, C.PARAMETER)

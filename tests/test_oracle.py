@@ -284,6 +284,32 @@ def test_philox_known_answers():
         assert list(o) == want
 
 
+def test_counter_noise_stream_pairs():
+    """Stream s at cell idx: Box-Muller on words 0-1 (even s) or 2-3 (odd s) of the Philox block
+    of counter (idx, s >> 1, 0) -- recomputed here from the raw block (itself pinned by the known
+    answers above); the two streams of a pair are uncorrelated."""
+    seed = 0x123456789AB
+    for s in (6, 7, 12):
+        for idx in (0, 5, (1 << 33) + 3):
+            c = np.array([idx & 0xFFFFFFFF, idx >> 32, s >> 1, 0], dtype=np.uint32)
+            k = np.array([seed & 0xFFFFFFFF, seed >> 32], dtype=np.uint32)
+            o = np.zeros(4, dtype=np.uint32)
+            O.lib().orc_philox4x32_10(O._ptr(c, O.ctypes.c_uint32), O._ptr(k, O.ctypes.c_uint32),
+                                      O._ptr(o, O.ctypes.c_uint32))
+            w0, w1 = (int(o[2]), int(o[3])) if s & 1 else (int(o[0]), int(o[1]))
+            u1 = ((w0 >> 8) + 1) / 16777216.0
+            u2 = (w1 >> 8) / 16777216.0
+            r = math.sqrt(-2.0 * math.log(u1))
+            z0, z1 = O.ctypes.c_float(), O.ctypes.c_float()
+            O.lib().orc_cb_normal2(seed, s, idx, O.ctypes.byref(z0), O.ctypes.byref(z1))
+            assert abs(z0.value - r * math.cos(2 * math.pi * u2)) < 1e-6 * max(1.0, r)
+            assert abs(z1.value - r * math.sin(2 * math.pi * u2)) < 1e-6 * max(1.0, r)
+    a = O.fill_complex_noise_cb(32, 32, 32, 77, 8).ravel()
+    b = O.fill_complex_noise_cb(32, 32, 32, 77, 9).ravel()
+    assert abs(np.mean(a * np.conj(b))) < 0.02
+    assert not np.array_equal(a, b)
+
+
 def test_counter_noise_moments():
     z = O.fill_complex_noise_cb(64, 64, 64, 2024, 0)
     v = z[:, 1:-1, 1:-1].ravel()
@@ -299,7 +325,7 @@ def test_sample_covariance_of_coloured_noise_is_the_posterior_covariance():
     n = 4000
     acc = np.zeros((3, 3) + oc.shape[1:], complex)
     for r in range(n):
-        noise = np.stack([O.fill_complex_noise_cb(p["nxp"], p["nyp"], p["nzp"], 99, 3 * r + g) for g in range(3)])
+        noise = np.stack([O.fill_complex_noise_cb(p["nxp"], p["nyp"], p["nzp"], 99, 4 * r + g) for g in range(3)])
         z, _ = O.colour(oc, noise, "f64")
         acc += np.einsum("a...,b...->ab...", z, np.conj(z))
     est = acc / n
