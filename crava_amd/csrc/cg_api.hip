@@ -263,6 +263,7 @@ int cg_ctx_create(cg_ctx **out, int device, uint64_t hbm_budget_bytes)
   DeviceGuard g(device);
   hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipMalloc(&ctx->counters, 256 * sizeof(uint64_t));
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&ctx->num_cu, hipDeviceAttributeMultiprocessorCount, device);
   if (e == hipSuccess) {
     int pools = 0;
     if (hipDeviceGetAttribute(&pools, hipDeviceAttributeMemoryPoolsSupported, device) == hipSuccess && pools) {

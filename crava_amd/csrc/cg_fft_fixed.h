@@ -25,6 +25,26 @@ namespace cgfix {
 
 using namespace cgdft;
 
+// Workgroup barrier for the LDS exchanges of these kernels: each wave's LDS operations are
+// complete (lgkmcnt 0) before it, but outstanding global loads and stores are not waited for
+// (__syncthreads also waits for them), so stores and prefetches into LDS stay in flight across
+// the FFT stages.  No kernel here passes data between threads through global memory.
+__device__ __forceinline__ void lds_sync()
+{
+  __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0); vmcnt and expcnt at their maximum (no wait)
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+
+// Waits for every outstanding global load and store of the wave (vmcnt 0), including loads
+// into LDS.
+__device__ __forceinline__ void vm_wait()
+{
+  __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0); expcnt and lgkmcnt at their maximum
+  asm volatile("" ::: "memory");
+}
+
 // ---------------------------------------------------------------- radix stages
 template <int N, int DIR, int TPS, int ES, int NS, int... RS>
 struct Stages;
@@ -53,7 +73,7 @@ struct Stages<N, DIR, TPS, ES, NS, P, RS...> {
         for (int q = 0; q < P; ++q) v[b][q] = buf[(j + q * NB) * ES];
       }
     }
-    __syncthreads();
+    lds_sync();
 #pragma unroll
     for (int b = 0; b < BPT; ++b) {
       const int j = t + b * TPS;
@@ -73,7 +93,7 @@ struct Stages<N, DIR, TPS, ES, NS, P, RS...> {
         for (int q = 0; q < P; ++q) buf[(d + q * NS) * ES] = v[b][q];
       }
     }
-    __syncthreads();
+    lds_sync();
     Stages<N, DIR, TPS, ES, NS * P, RS...>::run(buf, t, tw);
   }
 };
@@ -123,7 +143,7 @@ struct StagesIO<N, DIR, TPS, ES, NS, P, RS...> {
       }
     }
     if constexpr (!LAST) {
-      __syncthreads();
+      lds_sync();
       StagesIO<N, DIR, TPS, ES, NS * P, RS...>::from_lds(buf, t, tw, sink);
     }
   }
@@ -140,7 +160,7 @@ struct StagesIO<N, DIR, TPS, ES, NS, P, RS...> {
         for (int q = 0; q < P; ++q) v[b][q] = buf[(j + q * NB) * ES];
       }
     }
-    __syncthreads();
+    lds_sync();
     step(v, buf, t, tw, sink);
   }
 };
@@ -192,6 +212,8 @@ constexpr int strided_tile()
 
 // x pass, variant 0: the most rows (up to 128) whose LDS (N twiddles + (N + 1)(RW + 1) slots)
 // fits kLds with at most 512 threads; variant 1: half as many rows (more workgroups per CU).
+// Even nxp, where x_rows_dma allows it: variant 0 is the double-buffered kernel, variant 1 the
+// single-buffered one with x_rows<N, 0>() rows.
 template <int N, int V>
 constexpr int x_rows()
 {
@@ -199,6 +221,14 @@ constexpr int x_rows()
   int rw = 128;
   while (rw > 1 && ((size_t)(N + (N + 1) * (rw + 1)) * 8 > kLds || rw * T > 512)) rw /= 2;
   return (V == 1 && rw > 1) ? rw / 2 : rw;
+}
+template <int N>
+constexpr int x_rows_dma();
+// Rows per tile of variant V of the even-nxp x pass.
+template <int N, int V>
+constexpr int xe_rows()
+{
+  return x_rows_dma<N>() > 0 ? (V == 0 ? x_rows_dma<N>() : x_rows<N, 0>()) : x_rows<N, V>();
 }
 
 __device__ __forceinline__ void load_tw(float2 *tw, const float2 *g, int n, int tid, int nthr)
@@ -275,7 +305,7 @@ k_ffts(cg::StridedArgs one, const cg::StridedArgs *__restrict__ items, int nitem
     for (int r = 0; r < IO::P_; ++r)
       v[b][r] = (live && (IO::EVEN || j < IO::NB)) ? col[(long)(j + r * IO::NB) * a.S] : make_float2(0.f, 0.f);
   }
-  __syncthreads();  // twiddles in LDS
+  lds_sync();  // twiddles in LDS
   const float scale = a.scale;
   auto sink = [&](int e, float2 x) {
     if (live) col[(long)e * a.S] = cscale(x, scale);
@@ -328,7 +358,7 @@ k_fftx_r2c(cg::XArgs one, const cg::XArgs *__restrict__ items, int nitems, const
       if (f < RW * CN) buf[m * ES + r] = cscale(v[i], a.scale);
     }
   }
-  __syncthreads();
+  lds_sync();
   Plan<N>::template run<-1, ES>(buf + tid % RW, tid / RW, tw);
   for (int idx = tid; idx < RW * (N / 2 + 1); idx += NT) {
     const int r = idx % RW, k = idx / RW;
@@ -343,7 +373,7 @@ k_fftx_r2c(cg::XArgs one, const cg::XArgs *__restrict__ items, int nitems, const
     buf[k * ES + r] = xa;
     buf[(N - k) * ES + r] = xb;  // k = 0 writes bin N into the spare slot
   }
-  __syncthreads();
+  lds_sync();
 #pragma unroll
   for (int i = 0; i < PER; ++i) {
     const int f = tid + i * NT;
@@ -390,7 +420,7 @@ k_fftx_c2r(cg::XArgs one, const cg::XArgs *__restrict__ items, int nitems, const
       if (f < RW * CN) buf[m * ES + r] = v[i];
     }
   }
-  __syncthreads();
+  lds_sync();
   for (int idx = tid; idx < RW * (N / 2 + 1); idx += NT) {
     const int r = idx % RW, k = idx / RW;
     float2 xa = buf[k * ES + r], xb = buf[(N - k) * ES + r];  // X[k], X[N-k]
@@ -406,7 +436,7 @@ k_fftx_c2r(cg::XArgs one, const cg::XArgs *__restrict__ items, int nitems, const
       buf[(N - k) * ES + r] = make_float2(eb.x - ob.y, eb.y + ob.x);
     }
   }
-  __syncthreads();
+  lds_sync();
   Plan<N>::template run<+1, ES>(buf + tid % RW, tid / RW, tw);
   float2 v[ADD ? PER : 1];
   if (ADD) {  // every load of the added grid in flight before the first store
@@ -432,6 +462,251 @@ k_fftx_c2r(cg::XArgs one, const cg::XArgs *__restrict__ items, int nitems, const
       io[f] = o;
     }
   }
+}
+
+// ---------------------------------------------------------------- x pass, double-buffered
+// Even nxp, the second tile variant.  Persistent workgroups (a multiple of 8, as many as the CUs
+// hold): workgroup b serves XCD b % 8, which walks its eighth of the tiles (a contiguous band)
+// in steps of gridDim.x / 8.  Two LDS tile buffers hold RW rows each in the grid's own layout
+// (row r at r CN, CN = N + 1 odd for even N, so consecutive rows fall on distinct bank pairs):
+// while the FFT works on one buffer, the next tile's rows stream into the other with
+// global_load_lds (no registers, no LDS store instructions), and the previous tile's stores
+// are in flight.  Iteration: start the next tile's copy; transform; read the tile out to
+// registers; wait for the copy (and the previous stores); barrier; store.
+struct TileWalk {
+  unsigned t, hi, step;
+  __device__ __forceinline__ explicit TileWalk(unsigned ntiles)
+  {
+    const unsigned xcd = blockIdx.x % 8u, q = ntiles / 8u, r = ntiles % 8u;
+    const unsigned lo = xcd * q + (xcd < r ? xcd : r);
+    hi = lo + q + (xcd < r ? 1u : 0u);
+    t = lo + blockIdx.x / 8u;
+    step = gridDim.x / 8u;
+  }
+};
+
+// A tile's rows: source (forward: the real grid, inverse: the grid itself), destination, added
+// grid, scale, first row and float2 slots (nr CN).  The descriptor fields are read with scalar
+// loads (workgroup-uniform), so that looking up the next tile never waits for vector memory.
+struct XTile {
+  const float2 *src;
+  float2 *dst;
+  const float2 *add;
+  float scale;
+  long row0;
+  int n;
+};
+
+template <int RW, int CN, bool FWD>
+__device__ __forceinline__ XTile x_tile(const XTile &one, long one_rows, const cg::XArgs *items, int nitems,
+                                        unsigned tile)
+{
+  XTile x = one;
+  long rows = one_rows;
+  unsigned blk = tile;
+  if (items) {
+    typedef const __attribute__((address_space(4))) cg::XArgs *KArgs;
+    const KArgs it = (KArgs)items + find_item(items, nitems, tile);
+    x.dst = reinterpret_cast<float2 *>(it->dst);
+    x.src = FWD ? reinterpret_cast<const float2 *>(it->src) : x.dst;
+    x.add = reinterpret_cast<const float2 *>(it->add);
+    x.scale = it->scale;
+    rows = it->rows;
+    blk = tile - it->block0;
+  }
+  x.row0 = (long)blk * RW;
+  const long left = rows - x.row0;
+  x.n = (int)(left < RW ? left : RW) * CN;
+  return x;
+}
+
+// Global loads and stores through pointers the compiler cannot trace to a kernel argument
+// (descriptor fields) are flat operations, which also count against the LDS counter that
+// lds_sync waits for; these keep them global.
+// (float2's member operators take generic pointers, hence the plain vector type.)
+typedef float fvec2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float2 gload(const float2 *p)
+{
+  const fvec2 v = *(const __attribute__((address_space(1))) fvec2 *)p;
+  return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ void gstore(float2 *p, float2 v)
+{
+  fvec2 w;
+  w.x = v.x;
+  w.y = v.y;
+  *(__attribute__((address_space(1))) fvec2 *)p = w;
+}
+
+// One 16-byte global_load_lds per lane: lane l of the wave writes LDS bytes [lds + 16 l, +16).
+// Issued as inline assembly so that the compiler does not track it (its own tracking makes
+// later LDS reads of the other buffer wait for the copy); the caller waits with vm_wait and a
+// barrier before reading the destination.  m0 holds the wave's LDS destination; no other code
+// in these kernels uses m0.
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Winline-asm"  // m0 is reserved: declared clobbered on purpose
+__device__ __forceinline__ void lds_copy16(const float2 *g, unsigned lds)
+{
+  asm volatile("s_mov_b32 m0, %1\n\tglobal_load_lds_dwordx4 %0, off" : : "v"(g), "s"(lds) : "memory", "m0");
+}
+#pragma clang diagnostic pop
+
+// n float2 from global src (16-byte aligned) to LDS dst: 16-byte pieces, one per lane (a wave
+// fills 1 KB of consecutive LDS), and an odd last float2 by a plain load.
+template <int NT>
+__device__ __forceinline__ void copy_to_lds(float2 *dst, const float2 *src, int n, int tid)
+{
+  const int pieces = n >> 1;
+  const int wbase = tid & ~63;
+  for (int base = 0; base < pieces; base += NT) {
+    const unsigned lds = __builtin_amdgcn_readfirstlane(
+        (unsigned)(uintptr_t)(__attribute__((address_space(3))) float2 *)(dst + 2 * (base + wbase)));
+    if (base + tid < pieces) lds_copy16(src + 2 * (base + tid), lds);
+  }
+  if ((n & 1) && tid == 0) dst[n - 1] = gload(src + (n - 1));
+}
+
+template <int N, int RW, bool FWD, bool ADD>
+struct XDma {
+  static constexpr int TPS = Plan<N>::TPS;
+  static constexpr int NT = RW * TPS;
+  static constexpr int CN = N + 1;
+  static constexpr int TILE = RW * CN;
+  static constexpr int PER = (TILE + NT - 1) / NT;
+
+  // Transforms tile `cur` (in `buf`), prefetches tile `nxt` (when has_next) into `other`, then
+  // stores `cur`.
+  // Everything the transform reads is in LDS (twiddles included): a global load issued after
+  // the copy would wait for the copy (loads complete in order).
+  static __device__ __forceinline__ void step(float2 *buf, float2 *other, const float2 *tw, const XTile &cur,
+                                              const XTile &nxt, bool has_next, const float2 *tw_row, int tid)
+  {
+    float2 va[ADD ? PER : 1];
+    if (ADD) {  // issued before the copy, used after the transform
+      const float2 *ad = cur.add + cur.row0 * CN;
+#pragma unroll
+      for (int i = 0; i < PER; ++i) {
+        const int f = tid + i * NT;
+        va[ADD ? i : 0] = f < cur.n ? gload(ad + f) : make_float2(0.f, 0.f);
+      }
+    }
+    if (has_next) copy_to_lds<NT>(other, nxt.src + nxt.row0 * CN, nxt.n, tid);
+    const int r = tid % RW, t = tid / RW;
+    if (FWD) {
+      Plan<N>::template run<-1, 1>(buf + r * CN, t, tw);
+      for (int idx = tid; idx < RW * (N / 2 + 1); idx += NT) {
+        const int rr = idx % RW, k = idx / RW;
+        float2 *row = buf + rr * CN;
+        const int kn = k == 0 ? 0 : N - k;
+        const float2 za = row[k], zb = row[kn];
+        const float2 ea = cscale(cadd(za, cconj(zb)), 0.5f), da = csub(za, cconj(zb));
+        const float2 eb = cscale(cadd(zb, cconj(za)), 0.5f), db = csub(zb, cconj(za));
+        const float2 wk = tw_row[k];  // w^(N-k) = -conj(w^k) for w = exp(-i pi / N)
+        const float2 xa = cadd(ea, cmul(wk, make_float2(0.5f * da.y, -0.5f * da.x)));
+        const float2 xb = cadd(eb, cmul(make_float2(-wk.x, wk.y), make_float2(0.5f * db.y, -0.5f * db.x)));
+        row[k] = xa;
+        row[N - k] = xb;  // k = 0 writes bin N into the spare slot
+      }
+      lds_sync();
+    } else {
+      for (int idx = tid; idx < RW * (N / 2 + 1); idx += NT) {
+        const int rr = idx % RW, k = idx / RW;
+        float2 *row = buf + rr * CN;
+        float2 xa = row[k], xb = row[N - k];  // X[k], X[N-k]
+        if (k == 0) {
+          xa.y = 0.f;
+          xb.y = 0.f;
+        }
+        const float2 wk = tw_row[k];
+        const float2 ea = cadd(xa, cconj(xb)), oa = cmul(csub(xa, cconj(xb)), cconj(wk));
+        row[k] = make_float2(ea.x - oa.y, ea.y + oa.x);
+        if (k > 0 && k < N - k) {
+          const float2 eb = cadd(xb, cconj(xa)), ob = cmul(csub(xb, cconj(xa)), make_float2(-wk.x, -wk.y));
+          row[N - k] = make_float2(eb.x - ob.y, eb.y + ob.x);
+        }
+      }
+      lds_sync();
+      Plan<N>::template run<+1, 1>(buf + r * CN, t, tw);
+    }
+    const float scale = cur.scale;
+    float2 o[PER];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int f = tid + i * NT;
+      const int m = f % CN;
+      float2 x = make_float2(0.f, 0.f);
+      if (f < TILE && (FWD || m < N)) x = cscale(buf[f], scale);
+      if (ADD && m < N) {
+        x.x += va[ADD ? i : 0].x;
+        x.y += va[ADD ? i : 0].y;
+      }
+      o[i] = x;
+    }
+    vm_wait();   // the next tile's rows are in `other`; earlier stores are done
+    lds_sync();  // ... for every wave; `buf` is free
+    float2 *out = cur.dst + cur.row0 * CN;
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int f = tid + i * NT;
+      if (f < cur.n) gstore(out + f, o[i]);
+    }
+  }
+};
+
+template <int N, int RW, bool FWD, bool ADD>
+__global__ void __launch_bounds__(RW *Plan<N>::TPS)
+k_fftx_dma(cg::XArgs one, const cg::XArgs *__restrict__ items, int nitems, unsigned ntiles,
+           const float2 *__restrict__ twg, const float2 *__restrict__ tw_row)
+{
+  using K = XDma<N, RW, FWD, ADD>;
+  __shared__ __align__(16) float2 s_tw[N];
+  __shared__ __align__(16) float2 s_twr[N / 2 + 1];
+  __shared__ __align__(16) float2 s_b0[K::TILE];
+  __shared__ __align__(16) float2 s_b1[K::TILE];
+  const int tid = threadIdx.x;
+  TileWalk w(ntiles);
+  if (w.t >= w.hi) return;  // workgroup-uniform: no tile for this workgroup
+  load_tw(s_tw, twg, N, tid, K::NT);
+  load_tw(s_twr, tw_row, N / 2 + 1, tid, K::NT);
+  XTile o1;  // the single grid, from the kernel arguments
+  o1.dst = reinterpret_cast<float2 *>(one.dst);
+  o1.src = FWD ? reinterpret_cast<const float2 *>(one.src) : o1.dst;
+  o1.add = reinterpret_cast<const float2 *>(one.add);
+  o1.scale = one.scale;
+  o1.row0 = 0;
+  o1.n = 0;
+  const long rows1 = one.rows;
+  XTile cur = x_tile<RW, K::CN, FWD>(o1, rows1, items, nitems, w.t);
+  copy_to_lds<K::NT>(s_b0, cur.src + cur.row0 * K::CN, cur.n, tid);
+  vm_wait();
+  lds_sync();
+  for (;;) {  // two tiles per trip, so that each buffer is a fixed array in each half
+    unsigned tn = w.t + w.step;
+    bool more = tn < w.hi;
+    XTile nxt = more ? x_tile<RW, K::CN, FWD>(o1, rows1, items, nitems, tn) : cur;
+    K::step(s_b0, s_b1, s_tw, cur, nxt, more, s_twr, tid);
+    if (!more) break;
+    w.t = tn;
+    cur = nxt;
+    tn = w.t + w.step;
+    more = tn < w.hi;
+    nxt = more ? x_tile<RW, K::CN, FWD>(o1, rows1, items, nitems, tn) : cur;
+    K::step(s_b1, s_b0, s_tw, cur, nxt, more, s_twr, tid);
+    if (!more) break;
+    w.t = tn;
+    cur = nxt;
+  }
+}
+
+// Rows per tile of the double-buffered kernel: the most (up to 32) whose two buffers and the
+// twiddles fit one CU's LDS with at most 1024 threads; 0 where that leaves fewer than 2 rows.
+template <int N>
+constexpr int x_rows_dma()
+{
+  constexpr int T = Plan<N>::TPS;
+  int rw = 32;
+  while (rw >= 2 && ((size_t)(N + N / 2 + 1 + 2 * (size_t)(N + 1) * rw) * 8 > kLdsMax || rw * T > 1024)) rw /= 2;
+  return rw >= 2 ? rw : 0;
 }
 
 // Odd nxp = N: each row holds N reals and one padding float, CN = (N + 1) / 2 float pairs.
@@ -473,7 +748,7 @@ k_fftxo_r2c(cg::XArgs one, const cg::XArgs *__restrict__ items, int nitems, cons
       }
     }
   }
-  __syncthreads();
+  lds_sync();
   Plan<N>::template run<-1, ES>(buf + tid % RW, tid / RW, tw);
 #pragma unroll
   for (int i = 0; i < PER; ++i) {
@@ -526,7 +801,7 @@ k_fftxo_c2r(cg::XArgs one, const cg::XArgs *__restrict__ items, int nitems, cons
       }
     }
   }
-  __syncthreads();
+  lds_sync();
   Plan<N>::template run<+1, ES>(buf + tid % RW, tid / RW, tw);
   float2 v[ADD ? PER : 1];
   if (ADD) {  // every load of the added grid in flight before the first store
@@ -602,7 +877,7 @@ k_colour_zinv(cg::ColourParams p, long plane, const float2 *__restrict__ twg)
       for (int g = 0; g < 3; ++g) buf[(g * N + z) * TI + c] = o[g];
     }
   }
-  __syncthreads();
+  lds_sync();
   const int g = tid / (TI * TPS), r = tid % (TI * TPS), c = r % TI, t = r / TI;
   const bool live = q0 + c < plane;
   float2 *col = p.out[g] + (live ? q0 + c : 0);
@@ -646,12 +921,52 @@ int strided_launch(cg_ctx *ctx, int dir, const cg::StridedArgs *one, const cg::S
   return cg::hip_fail(ctx, hipGetLastError(), "k_ffts");
 }
 
+// Workgroups of a persistent launch over `ntiles` tiles: as many as the CUs hold at once
+// (`per_cu`, the kernel's occupancy), a multiple of 8 (one share per XCD), no more than the tiles
+// need.
+inline unsigned persistent_blocks(const cg_ctx *ctx, unsigned ntiles, int per_cu)
+{
+  const unsigned cap = (unsigned)(per_cu > 0 ? per_cu : 1) * (unsigned)ctx->num_cu;
+  const unsigned want = ntiles < cap ? ntiles : cap;
+  return ((want + 7u) / 8u) * 8u;
+}
+
+// Workgroups of `kernel` one CU holds with `threads` threads (asked once per kernel: every
+// device this library runs on is a gfx950).
+template <typename K>
+int occupancy(K kernel, int threads)
+{
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, threads, 0) != hipSuccess || n < 1) n = 1;
+  return n;
+}
+
+template <int N, bool FWD, bool ADD>
+int xdma_launch(cg_ctx *ctx, const cg::XArgs *one, const cg::XArgs *items, int nitems, unsigned ntiles,
+                const float2 *tw, const float2 *twr)
+{
+  constexpr int RW = x_rows_dma<N>();
+  constexpr int NT = RW * Plan<N>::TPS;
+  if (ntiles == 0) return CG_OK;
+  static const int occ = occupancy(k_fftx_dma<N, RW, FWD, ADD>, NT);
+  const cg::XArgs o = one ? *one : cg::XArgs{};
+  k_fftx_dma<N, RW, FWD, ADD><<<persistent_blocks(ctx, ntiles, occ), NT, 0, ctx->stream>>>(o, items, nitems, ntiles,
+                                                                                            tw, twr);
+  return cg::hip_fail(ctx, hipGetLastError(), "k_fftx_dma");
+}
+
 // `add`: the inverse adds a real grid (every descriptor of a list has one, or none has).
+// `nblocks`: tiles of xe_rows<N, V>() rows (of the grid, or of all descriptors of a list).
 template <int N, int V>
 int xe_launch(cg_ctx *ctx, int forward, int add, const cg::XArgs *one, const cg::XArgs *items, int nitems,
               unsigned nblocks, const float2 *tw, const float2 *twr)
 {
-  constexpr int RW = x_rows<N, V>();
+  if constexpr (V == 0 && x_rows_dma<N>() > 0) {
+    if (forward) return xdma_launch<N, true, false>(ctx, one, items, nitems, nblocks, tw, twr);
+    if (add) return xdma_launch<N, false, true>(ctx, one, items, nitems, nblocks, tw, twr);
+    return xdma_launch<N, false, false>(ctx, one, items, nitems, nblocks, tw, twr);
+  }
+  constexpr int RW = xe_rows<N, V>();
   const size_t lds = sizeof(float2) * (N + (size_t)(N + 1) * (RW + 1));
   const cg::XArgs o = one ? *one : cg::XArgs{};
   if (forward)

@@ -198,7 +198,9 @@ int autotune_geometry(cg_ctx *ctx, int nxp, int nyp, int nzp)
   };
   const bool even = nxp % 2 == 0;
   const FixedEntry *ex = fixed_find(even ? FIX_XE : FIX_XO, even ? nxp / 2 : nxp);
-  if (!rc && ex && ex->tile[0] != ex->tile[1]) {
+  // (The double-buffered even-nxp kernel is not tuned: on warm scratch grids the single-buffered
+  // one times faster, in the transform sequence of an inversion it is slower.)
+  if (!rc && ex && ex->tune && ex->tile[0] != ex->tile[1]) {
     const float2 *tw = twiddle_table(ctx, ex->n), *twr = even ? twiddle_table(ctx, nxp) : nullptr;
     const long rows = (long)nyp * nzp;
     for (int mode = 0; mode < 3 && tw; ++mode) {  // forward; inverse; inverse + add

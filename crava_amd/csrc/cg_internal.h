@@ -76,6 +76,7 @@ struct FixedEntry {
   XFn x[2];
   ColourZFn cz;  // strided lengths: colouring fused with the inverse z pass
   int cz_waves;  // wavefronts of the fused kernel resident per CU
+  int tune;      // 0: variant 0 is kept (the autotuner's timing on scratch grids misjudges it)
 };
 // Tile variant for a pass geometry: the autotuner's choice (cg_autotune) for this exact
 // geometry, or the context's forced variant, else 0.  `geom` identifies the pass: strided
@@ -91,6 +92,7 @@ struct TimedLaunch {
 
 struct cg_ctx {
   int device = 0;
+  int num_cu = 1;  // compute units of the device
   hipStream_t stream = nullptr;
   std::string err;
   uint64_t budget = 0, in_use = 0, peak = 0;
