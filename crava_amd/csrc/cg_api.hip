@@ -909,6 +909,7 @@ int cg_simulate(cg_ctx *ctx, const cg_sim_args *a)
   p.cnxp = ref->cnxp;
   p.nyp = ref->nyp;
   p.nzp = ref->nzp;
+  cg::colour_dims(p);
   for (int g = 0; g < 6; ++g) p.cov[g] = reinterpret_cast<const float2 *>(a->cov[g]->d);
   for (int g = 0; g < 3; ++g) p.out[g] = reinterpret_cast<float2 *>(a->out[g]->d);
   p.from_grids = a->noise_source == CG_NOISE_GRIDS;
@@ -1537,6 +1538,7 @@ int cg_batch_create(cg_ctx *ctx, int n, const cg_inversion *inv, cg_batch **out)
       it.p.cnxp = ref->cnxp;
       it.p.nyp = ref->nyp;
       it.p.nzp = ref->nzp;
+      cg::colour_dims(it.p);
       for (int g = 0; g < 6; ++g) it.p.cov[g] = reinterpret_cast<const float2 *>(b->inv[i].post.cov[g]->d);
       for (int g = 0; g < 3; ++g) it.p.out[g] = reinterpret_cast<float2 *>(b->outs[i][3 * r + g]->d);
       it.p.seed = b->inv[i].seed;

@@ -31,15 +31,16 @@ __device__ __forceinline__ void philox(uint32_t (&c)[4], uint32_t k0, uint32_t k
   }
 }
 
-// Box-Muller on two 32-bit words: u1 = (a>>8 + 1) 2^-24 in (0, 1], u2 = (b>>8) 2^-24.
+// Box-Muller on two 32-bit words: u1 = (a>>8 + 1) 2^-24 in (0, 1], u2 = (b>>8) 2^-24;
+// (r cos 2 pi u2, r sin 2 pi u2) with r = sqrt(-2 ln u1), from the hardware's log2, square root,
+// sine and cosine (the latter two take turns, so 2 pi u2 needs no range reduction).  u1 >= 2^-24
+// keeps every operand normal.
 __device__ __forceinline__ float2 box_muller(uint32_t a, uint32_t b)
 {
   const float u1 = (float)((a >> 8) + 1u) * 5.9604644775390625e-08f;
   const float u2 = (float)(b >> 8) * 5.9604644775390625e-08f;
-  const float r = sqrtf(-2.0f * logf(u1));
-  float s, co;
-  sincospif(2.0f * u2, &s, &co);
-  return make_float2(r * co, r * s);
+  const float r = __builtin_amdgcn_sqrtf(-1.38629436111989061883f * __builtin_amdgcn_logf(u1));  // -2 ln 2 log2 u1
+  return make_float2(r * __builtin_amdgcn_cosf(u2), r * __builtin_amdgcn_sinf(u2));
 }
 
 // One Philox block per (seed, cell, stream pair): counter (cell, stream >> 1, 0), key seed.
@@ -99,6 +100,26 @@ __device__ __forceinline__ int noise_source(long idx, int cnxp, int nyp, int nzp
   return 0;
 }
 
+// noise_source for 32-bit cell indices (grids of fewer than 2^32 complex cells), with the
+// divisions by cnxp and nyp done by multiplication.
+__device__ __forceinline__ int noise_source32(uint32_t idx, const cg::ColourParams &p, uint32_t *src)
+{
+  const uint32_t jk = cg::fast_div(idx, p.by_cnxp);
+  const uint32_t xs = idx - jk * (uint32_t)p.cnxp;
+  *src = idx;
+  if (xs == 0 || (xs == (uint32_t)p.cnxp - 1 && (idx & 1))) {
+    const uint32_t kind = cg::fast_div(jk, p.by_nyp), jind = jk - kind * (uint32_t)p.nyp;
+    const uint32_t jcc = jind == 0 ? 0 : p.nyp - jind;
+    const uint32_t kcc = kind == 0 ? 0 : p.nzp - kind;
+    const uint32_t jkcc = jcc + kcc * (uint32_t)p.nyp;
+    if (jkcc == jk) return 1;
+    if (jkcc > jk) return 0;
+    *src = jkcc * (uint32_t)p.cnxp + xs;
+    return 2;
+  }
+  return 0;
+}
+
 __device__ __forceinline__ float2 noise_value(int kind, float2 z)
 {
   const float sd = 0.70710678118654752440f;  // float(1/sqrt(2)), fftgrid.cpp:1002
@@ -118,7 +139,9 @@ struct c32 {
   float re, im;
 };
 
-// lib_matrCholCpx(3, P) on the lower triangle; returns 0 on success.
+// lib_matrCholCpx(3, P) on the lower triangle; returns 0 on success.  The pivot tests are the
+// reference's (value > 1e-20 in double), as float comparisons with the smallest float above
+// 1e-20; divisions by a pivot are products with its reciprocal.
 __device__ __forceinline__ int chol3(c32 (&m)[6])
 {
   // t(i,j) = i(i+1)/2 + j: 0:(0,0) 1:(1,0) 2:(1,1) 3:(2,0) 4:(2,1) 5:(2,2)
@@ -130,32 +153,34 @@ __device__ __forceinline__ int chol3(c32 (&m)[6])
     m[t].re *= inv;
     m[t].im *= inv;
   }
-  const double tol = 1e-20;
+  const float tol = __uint_as_float(0x1e3ce509u);  // 1.00000005e-20f: x > 1e-20 (double) <=> x >= tol
   // row 0
-  if (!((double)m[0].re > tol)) return 1;
-  m[0].re = sqrtf(m[0].re);
+  if (!(m[0].re >= tol)) return 1;
+  m[0].re = __builtin_amdgcn_sqrtf(m[0].re);
   m[0].im = 0.f;
+  const float r0 = __builtin_amdgcn_rcpf(m[0].re);
   // row 1
-  if (!((double)m[2].re > tol)) return 1;
-  m[1].re = m[1].re / m[0].re;
-  m[1].im = m[1].im / m[0].re;
+  if (!(m[2].re >= tol)) return 1;
+  m[1].re = m[1].re * r0;
+  m[1].im = m[1].im * r0;
   float piv = m[2].re - (m[1].re * m[1].re + m[1].im * m[1].im);
-  if (!((double)piv > tol)) return 1;
-  m[2].re = sqrtf(piv);
+  if (!(piv >= tol)) return 1;
+  m[2].re = __builtin_amdgcn_sqrtf(piv);
   m[2].im = 0.f;
   // row 2
-  if (!((double)m[5].re > tol)) return 1;
-  m[3].re = m[3].re / m[0].re;
-  m[3].im = m[3].im / m[0].re;
+  if (!(m[5].re >= tol)) return 1;
+  m[3].re = m[3].re * r0;
+  m[3].im = m[3].im * r0;
   {
+    const float r2 = __builtin_amdgcn_rcpf(m[2].re);
     const float lr = m[3].re * m[1].re + m[3].im * m[1].im;
     const float li = -(m[3].re * m[1].im) + m[3].im * m[1].re;
-    m[4].re = (m[4].re - lr) / m[2].re;
-    m[4].im = (m[4].im - li) / m[2].re;
+    m[4].re = (m[4].re - lr) * r2;
+    m[4].im = (m[4].im - li) * r2;
   }
   piv = m[5].re - (m[3].re * m[3].re + m[3].im * m[3].im + m[4].re * m[4].re + m[4].im * m[4].im);
-  if (!((double)piv > tol)) return 1;
-  m[5].re = sqrtf(piv);
+  if (!(piv >= tol)) return 1;
+  m[5].re = __builtin_amdgcn_sqrtf(piv);
   m[5].im = 0.f;
   const float sf = sqrtf(factor);
 #pragma unroll
@@ -177,7 +202,14 @@ __device__ __forceinline__ bool colour_cell(const cg::ColourParams &p, long idx,
   c32 m[6] = {{c00.x, c00.y}, {c01.x, -c01.y}, {c11.x, c11.y}, {c02.x, -c02.y}, {c12.x, -c12.y}, {c22.x, c22.y}};
   if (!p.from_grids) {
     long src;
-    const int kind = noise_source(idx, p.cnxp, p.nyp, p.nzp, &src);
+    int kind;
+    if (p.csize <= 0xFFFFFFFFl) {  // workgroup-uniform
+      uint32_t s32;
+      kind = noise_source32((uint32_t)idx, p, &s32);
+      src = s32;
+    } else {
+      kind = noise_source(idx, p.cnxp, p.nyp, p.nzp, &src);
+    }
     cb_normal2x3(p.seed, p.stream0, (uint64_t)src, z);
 #pragma unroll
     for (int g = 0; g < 3; ++g) z[g] = noise_value(kind, z[g]);

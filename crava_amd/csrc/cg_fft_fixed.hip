@@ -251,6 +251,7 @@ int autotune_geometry(cg_ctx *ctx, int nxp, int nyp, int nzp)
       p.cnxp = (int)cnxp;
       p.nyp = nyp;
       p.nzp = nzp;
+      colour_dims(p);
       for (int g = 0; g < 6; ++g) p.cov[g] = g < 3 ? A : B;
       for (int g = 0; g < 3; ++g) p.out[g] = out[g];
       p.seed = 1;

@@ -5,9 +5,27 @@
 
 namespace cg {
 
+// floor(n / d) for 32-bit n by a multiply-high, an add and a shift (Granlund & Montgomery,
+// "Division by invariant integers using multiplication", PLDI 1994, fig. 4.1):
+//   l = ceil(log2 d), m = floor(2^32 (2^l - d) / d) + 1, q = (umulhi(m, n) + n) >> l.
+struct FastDiv {
+  uint32_t m, l;
+};
+inline FastDiv make_fastdiv(uint32_t d)
+{
+  uint32_t l = 0;
+  while ((1ull << l) < d) ++l;
+  return FastDiv{(uint32_t)(((1ull << 32) * ((1ull << l) - d)) / d + 1), l};
+}
+__device__ __forceinline__ uint32_t fast_div(uint32_t n, FastDiv f)
+{
+  return (uint32_t)(((uint64_t)__umulhi(n, f.m) + n) >> f.l);
+}
+
 struct ColourParams {
   long csize;
   int cnxp, nyp, nzp;
+  FastDiv by_cnxp, by_nyp;  // set by colour_dims
   const float2 *cov[6];   // posterior covariance spectra 00 11 22 01 02 12
   float2 *out[3];         // coloured spectra (noise in, if from_grids)
   int from_grids;
@@ -29,6 +47,12 @@ int add_white_noise(cg_ctx *ctx, float *x, int nxp, int rnxp, long rows, float s
 int sum_squares(cg_ctx *ctx, const float *x, int nx, int ny, int nz, int rnxp, int nyp, double *dout);
 int fill_noise(cg_ctx *ctx, float2 *out, long csize, int cnxp, int nyp, int nzp, uint64_t seed, uint32_t stream);
 int colour(cg_ctx *ctx, const ColourParams &p);
+// Fills the divisors of p from p.cnxp and p.nyp.
+inline void colour_dims(ColourParams &p)
+{
+  p.by_cnxp = make_fastdiv((uint32_t)p.cnxp);
+  p.by_nyp = make_fastdiv((uint32_t)p.nyp);
+}
 
 struct ColourItem {
   ColourParams p;
