@@ -574,10 +574,14 @@ struct XDma {
   static constexpr int TILE = RW * CN;
   static constexpr int PER = (TILE + NT - 1) / NT;
 
-  // Transforms tile `cur` (in `buf`), prefetches tile `nxt` (when has_next) into `other`, then
-  // stores `cur`.
-  // Everything the transform reads is in LDS (twiddles included): a global load issued after
-  // the copy would wait for the copy (loads complete in order).
+  // Transforms tile `cur` (in `buf`), prefetches tile `nxt` (when has_next) into `other`, and
+  // stores `cur`.  Everything the transform reads is in LDS (twiddles included): a global load
+  // issued after the copy would wait for the copy (loads complete in order).
+  //   Forward: the R2C post-pass reads the pairs (k, N - k) of a row from LDS with k on
+  //   consecutive lanes and stores X[k] and X[N - k] straight to HBM (both runs coalesced), after
+  //   the wait for the copy, so the stores drain during the next tile's transform.
+  //   Inverse: pre-pass, transform, read-out to registers (+ the added rows), wait, stores.
+  // Ends with a barrier: `buf` is free and `other` holds the next tile for every wave.
   static __device__ __forceinline__ void step(float2 *buf, float2 *other, const float2 *tw, const XTile &cur,
                                               const XTile &nxt, bool has_next, const float2 *tw_row, int tid)
   {
@@ -592,11 +596,16 @@ struct XDma {
     }
     if (has_next) copy_to_lds<NT>(other, nxt.src + nxt.row0 * CN, nxt.n, tid);
     const int r = tid % RW, t = tid / RW;
+    const float scale = cur.scale;
+    float2 *out = cur.dst + cur.row0 * CN;
     if (FWD) {
       Plan<N>::template run<-1, 1>(buf + r * CN, t, tw);
-      for (int idx = tid; idx < RW * (N / 2 + 1); idx += NT) {
-        const int rr = idx % RW, k = idx / RW;
-        float2 *row = buf + rr * CN;
+      vm_wait();  // the next tile's rows are in `other`; the previous tile's stores are done
+      constexpr int H = N / 2 + 1;
+      const int nrows = cur.n / CN;
+      for (int idx = tid; idx < nrows * H; idx += NT) {
+        const int rr = idx / H, k = idx - rr * H;
+        const float2 *row = buf + rr * CN;
         const int kn = k == 0 ? 0 : N - k;
         const float2 za = row[k], zb = row[kn];
         const float2 ea = cscale(cadd(za, cconj(zb)), 0.5f), da = csub(za, cconj(zb));
@@ -604,38 +613,38 @@ struct XDma {
         const float2 wk = tw_row[k];  // w^(N-k) = -conj(w^k) for w = exp(-i pi / N)
         const float2 xa = cadd(ea, cmul(wk, make_float2(0.5f * da.y, -0.5f * da.x)));
         const float2 xb = cadd(eb, cmul(make_float2(-wk.x, wk.y), make_float2(0.5f * db.y, -0.5f * db.x)));
-        row[k] = xa;
-        row[N - k] = xb;  // k = 0 writes bin N into the spare slot
+        float2 *orow = out + rr * CN;
+        gstore(orow + k, cscale(xa, scale));
+        if (N - k != k) gstore(orow + (N - k), cscale(xb, scale));  // k = 0: bin N in the spare slot
       }
       lds_sync();
-    } else {
-      for (int idx = tid; idx < RW * (N / 2 + 1); idx += NT) {
-        const int rr = idx % RW, k = idx / RW;
-        float2 *row = buf + rr * CN;
-        float2 xa = row[k], xb = row[N - k];  // X[k], X[N-k]
-        if (k == 0) {
-          xa.y = 0.f;
-          xb.y = 0.f;
-        }
-        const float2 wk = tw_row[k];
-        const float2 ea = cadd(xa, cconj(xb)), oa = cmul(csub(xa, cconj(xb)), cconj(wk));
-        row[k] = make_float2(ea.x - oa.y, ea.y + oa.x);
-        if (k > 0 && k < N - k) {
-          const float2 eb = cadd(xb, cconj(xa)), ob = cmul(csub(xb, cconj(xa)), make_float2(-wk.x, -wk.y));
-          row[N - k] = make_float2(eb.x - ob.y, eb.y + ob.x);
-        }
-      }
-      lds_sync();
-      Plan<N>::template run<+1, 1>(buf + r * CN, t, tw);
+      return;
     }
-    const float scale = cur.scale;
+    for (int idx = tid; idx < RW * (N / 2 + 1); idx += NT) {
+      const int rr = idx % RW, k = idx / RW;
+      float2 *row = buf + rr * CN;
+      float2 xa = row[k], xb = row[N - k];  // X[k], X[N-k]
+      if (k == 0) {
+        xa.y = 0.f;
+        xb.y = 0.f;
+      }
+      const float2 wk = tw_row[k];
+      const float2 ea = cadd(xa, cconj(xb)), oa = cmul(csub(xa, cconj(xb)), cconj(wk));
+      row[k] = make_float2(ea.x - oa.y, ea.y + oa.x);
+      if (k > 0 && k < N - k) {
+        const float2 eb = cadd(xb, cconj(xa)), ob = cmul(csub(xb, cconj(xa)), make_float2(-wk.x, -wk.y));
+        row[N - k] = make_float2(eb.x - ob.y, eb.y + ob.x);
+      }
+    }
+    lds_sync();
+    Plan<N>::template run<+1, 1>(buf + r * CN, t, tw);
     float2 o[PER];
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
       const int f = tid + i * NT;
       const int m = f % CN;
       float2 x = make_float2(0.f, 0.f);
-      if (f < TILE && (FWD || m < N)) x = cscale(buf[f], scale);
+      if (f < TILE && m < N) x = cscale(buf[f], scale);
       if (ADD && m < N) {
         x.x += va[ADD ? i : 0].x;
         x.y += va[ADD ? i : 0].y;
@@ -644,7 +653,6 @@ struct XDma {
     }
     vm_wait();   // the next tile's rows are in `other`; earlier stores are done
     lds_sync();  // ... for every wave; `buf` is free
-    float2 *out = cur.dst + cur.row0 * CN;
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
       const int f = tid + i * NT;
@@ -704,7 +712,7 @@ template <int N>
 constexpr int x_rows_dma()
 {
   constexpr int T = Plan<N>::TPS;
-  int rw = 32;
+  int rw = 16;
   while (rw >= 2 && ((size_t)(N + N / 2 + 1 + 2 * (size_t)(N + 1) * rw) * 8 > kLdsMax || rw * T > 1024)) rw /= 2;
   return rw >= 2 ? rw : 0;
 }
