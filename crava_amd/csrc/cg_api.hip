@@ -891,11 +891,21 @@ int cg_simulate(cg_ctx *ctx, const cg_sim_args *a)
     if (!a->cov[g] || !same_shape(a->cov[g], ref)) return cg::fail(ctx, CG_EINVAL, "simulate: bad covariance grid");
     if (!a->cov[g]->transformed) return cg::fail(ctx, CG_ESTATE, "simulate: covariance grids must be transformed");
   }
+  // Means in the Fourier domain are added to the coloured spectra (the inverse transform is
+  // linear, and a PARAMETER grid's spectrum carries the same unitary scaling as the
+  // realisation's), which saves reading them again in the x pass; real-domain means are added
+  // after the inverse transform.
+  int spectral = -1;  // -1: no mean, 0: real domain, 1: Fourier domain
   for (int g = 0; g < 3; ++g) {
     if (!a->out[g] || !same_shape(a->out[g], ref)) return cg::fail(ctx, CG_EINVAL, "simulate: bad output grid");
     if (a->mean[g]) {
       if (!same_shape(a->mean[g], ref)) return cg::fail(ctx, CG_EINVAL, "simulate: bad mean grid");
-      if (a->mean[g]->transformed) return cg::fail(ctx, CG_ESTATE, "simulate: mean grids must be in the real domain");
+      const int dom = a->mean[g]->transformed ? 1 : 0;
+      if (spectral >= 0 && dom != spectral)
+        return cg::fail(ctx, CG_ESTATE, "simulate: mean grids must all be in the same domain");
+      spectral = dom;
+      if (dom && a->mean[g]->cubetype == CG_CT_COVARIANCE)
+        return cg::fail(ctx, CG_EINVAL, "simulate: a Fourier-domain mean must be a PARAMETER grid");
     }
     if (a->noise_source == CG_NOISE_GRIDS && !a->out[g]->transformed)
       return cg::fail(ctx, CG_ESTATE, "simulate: noise grids must be in the Fourier domain");
@@ -912,6 +922,8 @@ int cg_simulate(cg_ctx *ctx, const cg_sim_args *a)
   cg::colour_dims(p);
   for (int g = 0; g < 6; ++g) p.cov[g] = reinterpret_cast<const float2 *>(a->cov[g]->d);
   for (int g = 0; g < 3; ++g) p.out[g] = reinterpret_cast<float2 *>(a->out[g]->d);
+  for (int g = 0; g < 3; ++g)
+    p.mean[g] = spectral == 1 && a->mean[g] ? reinterpret_cast<const float2 *>(a->mean[g]->d) : nullptr;
   p.from_grids = a->noise_source == CG_NOISE_GRIDS;
   p.seed = a->seed;
   p.stream0 = 4u * a->realisation;  // stream pairs 2r (both halves) and 2r + 1 (first half)
@@ -929,7 +941,7 @@ int cg_simulate(cg_ctx *ctx, const cg_sim_args *a)
   for (int g = 0; g < 3; ++g) {
     a->out[g]->cubetype = CG_CT_PARAMETER;
     a->out[g]->transformed = 1;
-    const float *add = a->mean[g] ? a->mean[g]->d : nullptr;
+    const float *add = spectral == 0 && a->mean[g] ? a->mean[g]->d : nullptr;
     rc = z_done ? cg::grid_fft_inverse_xy(a->out[g], add) : cg::grid_fft_inverse(a->out[g], add);
     if (rc) return rc;
     a->out[g]->transformed = 0;

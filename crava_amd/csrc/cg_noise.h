@@ -28,6 +28,7 @@ struct ColourParams {
   FastDiv by_cnxp, by_nyp;  // set by colour_dims
   const float2 *cov[6];   // posterior covariance spectra 00 11 22 01 02 12
   float2 *out[3];         // coloured spectra (noise in, if from_grids)
+  const float2 *mean[3];  // posterior mean spectra added to L z (null: the mean is added later)
   int from_grids;
   uint64_t seed;
   uint32_t stream0;       // streams stream0, stream0+1, stream0+2

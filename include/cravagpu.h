@@ -200,7 +200,10 @@ int cg_posterior(cg_ctx *ctx, const cg_post_args *args);
 
 typedef struct {
   cg_grid *cov[6];          /* posterior covariance spectra (Fourier domain, order as above) */
-  cg_grid *mean[3];         /* posterior means, real domain (after cg_fft_c2r_inplace) */
+  cg_grid *mean[3];         /* posterior means (optional), either in the real domain (after
+                             * cg_fft_c2r_inplace: added after the inverse transform) or as
+                             * PARAMETER spectra (before it: added to the coloured spectra, which
+                             * saves one read of each mean; all three in the same domain) */
   cg_grid *out[3];          /* realisation grids: receive Vp, Vs, Rho real fields */
   int noise_source;         /* CG_NOISE_* */
   uint64_t seed;            /* CG_NOISE_COUNTER key */

@@ -52,11 +52,16 @@ def test_reference_noise_stream_is_bit_identical(ctx, shape):
     g.close()
 
 
-@pytest.mark.parametrize("source,fused", [("grids", -1), ("device_reference", -1), ("counter", 0), ("counter", 1)])
-def test_realisation_matches_oracle(ctx, source, fused):
+@pytest.mark.parametrize("source,fused,spectral", [("grids", -1, False), ("device_reference", -1, False),
+                                                   ("counter", 0, False), ("counter", 1, False),
+                                                   ("counter", 0, True), ("counter", 1, True),
+                                                   ("grids", -1, True)])
+def test_realisation_matches_oracle(ctx, source, fused, spectral):
     """grids: the reference stream drawn on the host and uploaded; device_reference: the same
     stream drawn on the device (cg_grid_fill_noise_reference); counter: the counter generator,
-    with the colouring as its own kernel (fused 0) or inside the inverse z pass (fused 1)."""
+    with the colouring as its own kernel (fused 0) or inside the inverse z pass (fused 1).
+    spectral: the posterior means handed over as spectra (added to the coloured spectra) instead
+    of in the real domain (added after the inverse transform)."""
     ctx.set_fused_colour(fused)
     p = P.make_problem(16, 12, 20, seed=11)
     om, oc = _posterior_cov(p)
@@ -74,7 +79,10 @@ def test_realisation_matches_oracle(ctx, source, fused):
         gc[g].upload_complex(oc[g])
     seed = 23665
     for g in range(3):
-        gmean[g].upload_real(mean_real[g])
+        if spectral:
+            gmean[g].upload_complex(om[g])
+        else:
+            gmean[g].upload_real(mean_real[g])
         if source == "grids":
             gout[g].upload_complex(noise[g])
         elif source == "device_reference":
@@ -84,7 +92,10 @@ def test_realisation_matches_oracle(ctx, source, fused):
     got = np.stack([g.download_real() for g in gout])
     ctx.set_fused_colour(-1)
     assert fails == nf
-    assert rel_l2(got - mean_real, want - mean_real) < 1e-5
+    # with the mean inside the transformed spectrum, FP32 rounding is relative to mean + noise
+    ratio = max(1.0, np.linalg.norm(mean_real) / np.linalg.norm(want - mean_real)) if spectral else 1.0
+    assert rel_l2(got - mean_real, want - mean_real) < 1e-5 * ratio
+    assert rel_l2(got, want) < 3e-6
     for g in gc + gmean + gout:
         g.close()
 
