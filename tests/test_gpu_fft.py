@@ -181,6 +181,31 @@ def test_every_compiled_length_matches_oracle(ctx, variant):
     print("compiled lengths, variant %d: %d grids, worst rel L2 %.2e" % (variant, len(shapes), worst), flush=True)
 
 
+@pytest.mark.parametrize("variant", [0, 1])
+def test_x_pass_tails_and_multi_tile_walks(ctx, variant):
+    """Even-nxp x passes on row counts that leave a partial last tile with an odd number of rows
+    (the double-buffered kernel copies an odd float2 count, its last element by a plain load),
+    and on grids with more tiles than the persistent launch has workgroups (each walks several
+    tiles, alternating its two buffers), forward and inverse."""
+    ctx.set_fft_variant(variant)
+    rng = np.random.default_rng(31 + variant)
+    for nzp, nyp, nxp in ((7, 9, 576), (3, 5, 50), (1, 1, 64), (96, 96, 576), (100, 200, 288)):
+        print("x pass", (nzp, nyp, nxp), "variant", variant, flush=True)
+        x = rng.standard_normal((nzp, nyp, nxp)).astype(np.float32)
+        cnxp = nxp // 2 + 1
+        z = (rng.standard_normal((nzp, nyp, cnxp)) + 1j * rng.standard_normal((nzp, nyp, cnxp))).astype(np.complex64)
+        g = _grid(ctx, x.shape, C.DATA)
+        g.upload_real(x)
+        g.fftInPlace()
+        assert rel_l2(g.download_complex(), O.grid_fft(x, C.DATA)) < 2e-6
+        g.upload_complex(z)
+        g.invFFTInPlace()
+        want = O.grid_ifft(z, nxp, C.DATA)
+        assert rel_l2(g.download_real(), want) < 2e-6
+        g.close()
+    ctx.set_fft_variant(-1)
+
+
 def test_autotuned_transforms_match_oracle(ctx):
     """After the autotuner has picked tile variants for a geometry, transforms of that geometry
     (plain and with the inverse's added grid) still match the oracle."""
