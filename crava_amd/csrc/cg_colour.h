@@ -8,6 +8,7 @@
 
 #include <cstdint>
 
+#include "cg_mem.h"
 #include "cg_noise.h"
 
 namespace cgcol {
@@ -196,11 +197,11 @@ __device__ __forceinline__ int chol3(c32 (&m)[6])
 // (streams stream0 .. stream0+2), or `z` as given when from_grids.
 __device__ __forceinline__ bool colour_cell(const cg::ColourParams &p, long idx, float2 (&z)[3], float2 (&o)[3])
 {
-  const float2 c00 = p.cov[0][idx], c11 = p.cov[1][idx], c22 = p.cov[2][idx];
-  const float2 c01 = p.cov[3][idx], c02 = p.cov[4][idx], c12 = p.cov[5][idx];
+  const float2 c00 = cg::gld(p.cov[0] + idx), c11 = cg::gld(p.cov[1] + idx), c22 = cg::gld(p.cov[2] + idx);
+  const float2 c01 = cg::gld(p.cov[3] + idx), c02 = cg::gld(p.cov[4] + idx), c12 = cg::gld(p.cov[5] + idx);
   float2 mv[3];  // loaded with the covariance, before the arithmetic that hides their latency
 #pragma unroll
-  for (int g = 0; g < 3; ++g) mv[g] = p.mean[g] ? p.mean[g][idx] : make_float2(0.f, 0.f);
+  for (int g = 0; g < 3; ++g) mv[g] = p.mean[g] ? cg::gld(p.mean[g] + idx) : make_float2(0.f, 0.f);
   // Hermitian completion of the six stored entries (avoinversion.cpp:1425-1437)
   c32 m[6] = {{c00.x, c00.y}, {c01.x, -c01.y}, {c11.x, c11.y}, {c02.x, -c02.y}, {c12.x, -c12.y}, {c22.x, c22.y}};
   if (!p.from_grids) {

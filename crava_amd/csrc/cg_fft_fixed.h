@@ -303,12 +303,12 @@ k_ffts(cg::StridedArgs one, const cg::StridedArgs *__restrict__ items, int nitem
     const int j = t + b * Plan<N>::TPS;
 #pragma unroll
     for (int r = 0; r < IO::P_; ++r)
-      v[b][r] = (live && (IO::EVEN || j < IO::NB)) ? col[(long)(j + r * IO::NB) * a.S] : make_float2(0.f, 0.f);
+      v[b][r] = (live && (IO::EVEN || j < IO::NB)) ? cg::gld(col + (long)(j + r * IO::NB) * a.S) : make_float2(0.f, 0.f);
   }
   lds_sync();  // twiddles in LDS
   const float scale = a.scale;
   auto sink = [&](int e, float2 x) {
-    if (live) col[(long)e * a.S] = cscale(x, scale);
+    if (live) cg::gst(col + (long)e * a.S, cscale(x, scale));
   };
   IO::step(v, buf + c, t, tw, sink);
 }
@@ -349,7 +349,7 @@ k_fftx_r2c(cg::XArgs one, const cg::XArgs *__restrict__ items, int nitems, const
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
       const int f = tid + i * NT;
-      v[i] = f < nr * CN ? in[f] : make_float2(0.f, 0.f);
+      v[i] = f < nr * CN ? cg::gld(in + f) : make_float2(0.f, 0.f);
     }
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
@@ -378,7 +378,7 @@ k_fftx_r2c(cg::XArgs one, const cg::XArgs *__restrict__ items, int nitems, const
   for (int i = 0; i < PER; ++i) {
     const int f = tid + i * NT;
     const int r = f / CN, m = f - r * CN;
-    if (f < nr * CN) out[f] = buf[m * ES + r];
+    if (f < nr * CN) cg::gst(out + f, buf[m * ES + r]);
   }
 }
 
@@ -411,7 +411,7 @@ k_fftx_c2r(cg::XArgs one, const cg::XArgs *__restrict__ items, int nitems, const
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
       const int f = tid + i * NT;
-      v[i] = f < nr * CN ? io[f] : make_float2(0.f, 0.f);
+      v[i] = f < nr * CN ? cg::gld(io + f) : make_float2(0.f, 0.f);
     }
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
@@ -443,7 +443,7 @@ k_fftx_c2r(cg::XArgs one, const cg::XArgs *__restrict__ items, int nitems, const
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
       const int f = tid + i * NT;
-      v[ADD ? i : 0] = f < nr * CN ? ad[f] : make_float2(0.f, 0.f);
+      v[ADD ? i : 0] = f < nr * CN ? cg::gld(ad + f) : make_float2(0.f, 0.f);
     }
   }
 #pragma unroll
@@ -459,7 +459,7 @@ k_fftx_c2r(cg::XArgs one, const cg::XArgs *__restrict__ items, int nitems, const
           o.y += v[ADD ? i : 0].y;
         }
       }
-      io[f] = o;
+      cg::gst(io + f, o);
     }
   }
 }
@@ -520,23 +520,9 @@ __device__ __forceinline__ XTile x_tile(const XTile &one, long one_rows, const c
   return x;
 }
 
-// Global loads and stores through pointers the compiler cannot trace to a kernel argument
-// (descriptor fields) are flat operations, which also count against the LDS counter that
-// lds_sync waits for; these keep them global.
-// (float2's member operators take generic pointers, hence the plain vector type.)
-typedef float fvec2 __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ float2 gload(const float2 *p)
-{
-  const fvec2 v = *(const __attribute__((address_space(1))) fvec2 *)p;
-  return make_float2(v.x, v.y);
-}
-__device__ __forceinline__ void gstore(float2 *p, float2 v)
-{
-  fvec2 w;
-  w.x = v.x;
-  w.y = v.y;
-  *(__attribute__((address_space(1))) fvec2 *)p = w;
-}
+// Descriptor fields are pointers loaded from memory: their accesses go through cg::gld / cg::gst
+// (cg_mem.h), which keep them global operations rather than flat ones (flat ones also count
+// against the LDS counter that lds_sync waits for).
 
 // One 16-byte global_load_lds per lane: lane l of the wave writes LDS bytes [lds + 16 l, +16).
 // Issued as inline assembly so that the compiler does not track it (its own tracking makes
@@ -563,7 +549,7 @@ __device__ __forceinline__ void copy_to_lds(float2 *dst, const float2 *src, int 
         (unsigned)(uintptr_t)(__attribute__((address_space(3))) float2 *)(dst + 2 * (base + wbase)));
     if (base + tid < pieces) lds_copy16(src + 2 * (base + tid), lds);
   }
-  if ((n & 1) && tid == 0) dst[n - 1] = gload(src + (n - 1));
+  if ((n & 1) && tid == 0) dst[n - 1] = cg::gld(src + (n - 1));
 }
 
 template <int N, int RW, bool FWD, bool ADD>
@@ -591,7 +577,7 @@ struct XDma {
 #pragma unroll
       for (int i = 0; i < PER; ++i) {
         const int f = tid + i * NT;
-        va[ADD ? i : 0] = f < cur.n ? gload(ad + f) : make_float2(0.f, 0.f);
+        va[ADD ? i : 0] = f < cur.n ? cg::gld(ad + f) : make_float2(0.f, 0.f);
       }
     }
     if (has_next) copy_to_lds<NT>(other, nxt.src + nxt.row0 * CN, nxt.n, tid);
@@ -614,8 +600,8 @@ struct XDma {
         const float2 xa = cadd(ea, cmul(wk, make_float2(0.5f * da.y, -0.5f * da.x)));
         const float2 xb = cadd(eb, cmul(make_float2(-wk.x, wk.y), make_float2(0.5f * db.y, -0.5f * db.x)));
         float2 *orow = out + rr * CN;
-        gstore(orow + k, cscale(xa, scale));
-        if (N - k != k) gstore(orow + (N - k), cscale(xb, scale));  // k = 0: bin N in the spare slot
+        cg::gst(orow + k, cscale(xa, scale));
+        if (N - k != k) cg::gst(orow + (N - k), cscale(xb, scale));  // k = 0: bin N in the spare slot
       }
       lds_sync();
       return;
@@ -656,7 +642,7 @@ struct XDma {
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
       const int f = tid + i * NT;
-      if (f < cur.n) gstore(out + f, o[i]);
+      if (f < cur.n) cg::gst(out + f, o[i]);
     }
   }
 };
@@ -744,7 +730,7 @@ k_fftxo_r2c(cg::XArgs one, const cg::XArgs *__restrict__ items, int nitems, cons
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
       const int f = tid + i * NT;
-      v[i] = f < nr * CN ? in[f] : make_float2(0.f, 0.f);
+      v[i] = f < nr * CN ? cg::gld(in + f) : make_float2(0.f, 0.f);
     }
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
@@ -762,7 +748,7 @@ k_fftxo_r2c(cg::XArgs one, const cg::XArgs *__restrict__ items, int nitems, cons
   for (int i = 0; i < PER; ++i) {
     const int f = tid + i * NT;
     const int r = f / CN, m = f - r * CN;
-    if (f < nr * CN) out[f] = buf[m * ES + r];
+    if (f < nr * CN) cg::gst(out + f, buf[m * ES + r]);
   }
 }
 
@@ -793,7 +779,7 @@ k_fftxo_c2r(cg::XArgs one, const cg::XArgs *__restrict__ items, int nitems, cons
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
       const int f = tid + i * NT;
-      v[i] = f < nr * CN ? io[f] : make_float2(0.f, 0.f);
+      v[i] = f < nr * CN ? cg::gld(io + f) : make_float2(0.f, 0.f);
     }
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
@@ -816,7 +802,7 @@ k_fftxo_c2r(cg::XArgs one, const cg::XArgs *__restrict__ items, int nitems, cons
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
       const int f = tid + i * NT;
-      v[ADD ? i : 0] = f < nr * CN ? ad[f] : make_float2(0.f, 0.f);
+      v[ADD ? i : 0] = f < nr * CN ? cg::gld(ad + f) : make_float2(0.f, 0.f);
     }
   }
 #pragma unroll
@@ -831,7 +817,7 @@ k_fftxo_c2r(cg::XArgs one, const cg::XArgs *__restrict__ items, int nitems, cons
         o.x += v[ADD ? i : 0].x;
         if (2 * m + 1 < N) o.y += v[ADD ? i : 0].y;
       }
-      io[f] = o;
+      cg::gst(io + f, o);
     }
   }
 }
@@ -890,7 +876,7 @@ k_colour_zinv(cg::ColourParams p, long plane, const float2 *__restrict__ twg)
   const bool live = q0 + c < plane;
   float2 *col = p.out[g] + (live ? q0 + c : 0);
   auto sink = [&](int e, float2 x) {
-    if (live) col[(long)e * plane] = x;
+    if (live) cg::gst(col + (long)e * plane, x);
   };
   IO::from_lds(buf + g * N * TI + c, t, tw, sink);
 }

@@ -20,9 +20,14 @@
 #include <cmath>
 
 #include "cg_internal.h"
+#include "cg_mem.h"
 #include "cg_post.h"
 
 namespace {
+
+using cg::gld;
+using cg::gst;
+using cg::kat;
 
 template <typename R>
 struct cpx {
@@ -42,6 +47,7 @@ __device__ __forceinline__ int tri(int i, int j) { return i * (i + 1) / 2 + j; }
 // Reciprocal: one v_rcp (about 1 ulp) in FP32, an exact division in FP64.
 __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
+
 
 // Cholesky of the Hermitian M (lower triangle) with lib_matrCholCpx's accept/reject rules
 // (lib_matr.c:78-167): fail if M00 <= 0, or if a diagonal entry or a pivot of M / M00 is
@@ -108,8 +114,8 @@ __device__ __forceinline__ void lsolve(const cpx<R> (&L)[NT * (NT + 1) / 2], con
 // which is the reference's M^-1 solve (lib_matrAXeqBMatCpx, forward then backward
 // substitution) rearranged so that only forward substitutions remain.
 // Returns 0 on success (outputs written), 1 on Cholesky failure (outputs untouched).
-template <int NT, typename R, bool SEP>
-__device__ __forceinline__ int update(const cg::PlaneTab<R> &T, R r, R lam, const R (&P6)[6], cpx<R> (&m3)[3],
+template <int NT, typename R, bool SEP, class Tab>
+__device__ __forceinline__ int update(const Tab &T, R r, R lam, const R (&P6)[6], cpx<R> (&m3)[3],
                                       cpx<R> (&d)[NT], cpx<R> (&cov6)[6], cpx<R> (&res)[NT], bool want_res,
                                       R &cond_lb)
 {
@@ -231,14 +237,14 @@ __device__ __forceinline__ float prior6(const cg::PostParams &p, long idx, int i
 {
   float r = 0.f;
   if (SEP) {
-    const float t = p.rxy[(long)j * p.cnxp + i] * p.rt[k];
+    const float t = gld(p.rxy + (long)j * p.cnxp + i) * kat(p.rt, k);
     r = sqrtf(t * t);
 #pragma unroll
     for (int g = 0; g < 6; ++g) P6[g] = r * p.s6f[g];
   } else {
 #pragma unroll
     for (int g = 0; g < 6; ++g) {
-      float t = p.cov[g][idx].x;
+      float t = gld(p.cov[g] + idx).x;
       if (p.s6f[g] != 0.f) t = t / p.s6f[g];
       P6[g] = sqrtf(t * t) * p.s6f[g];
     }
@@ -256,7 +262,7 @@ __device__ __forceinline__ void post32_body(const cg::PostParams &p, int k, long
   const bool active = c < plane;
   const int i = active ? (int)(c % p.cnxp) : 0, j = active ? (int)(c / p.cnxp) : 0;
   const long idx = (long)k * plane + (active ? c : 0);
-  const bool inv = p.invert[k] != 0;
+  const bool inv = kat(p.invert, k) != 0;
   bool did_inv = false, did_fail = false, defer = false;
   if (active) {
     float P6[6];
@@ -264,21 +270,21 @@ __device__ __forceinline__ void post32_body(const cg::PostParams &p, int k, long
     bool write_prior = !inv;
     if (inv) {
       did_inv = true;
-      const float lt = p.lam_grid ? p.lam_grid[idx].x : p.lam2d[(long)j * p.cnxp + i];
+      const float lt = p.lam_grid ? gld(p.lam_grid + idx).x : gld(p.lam2d + (long)j * p.cnxp + i);
       const float lam = sqrtf(lt * lt);
       cpx<float> m3[3], d[NT], cov6[6], res[NT];
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
-        const float2 v = p.mean[a][idx];
+        const float2 v = gld(p.mean[a] + idx);
         m3[a] = mk<float>(v.x, v.y);
       }
 #pragma unroll
       for (int l = 0; l < NT; ++l) {
-        const float2 v = p.data[l][idx];
+        const float2 v = gld(p.data[l] + idx);
         d[l] = mk<float>(v.x, v.y);
       }
       float cond = 0.f;
-      const int fail = update<NT, float, SEP>(p.tab32[k], r, lam, P6, m3, d, cov6, res, p.want_residual != 0, cond);
+      const int fail = update<NT, float, SEP>(kat(p.tab32, k), r, lam, P6, m3, d, cov6, res, p.want_residual != 0, cond);
       if (MIXED && (fail || cond > p.cond_limit)) {
         defer = true;
       } else if (fail) {  // posterior = prior (avoinversion.cpp:1040-1042)
@@ -286,18 +292,18 @@ __device__ __forceinline__ void post32_body(const cg::PostParams &p, int k, long
         write_prior = true;
       } else {
 #pragma unroll
-        for (int a = 0; a < 3; ++a) p.mean[a][idx] = make_float2(m3[a].re, m3[a].im);
+        for (int a = 0; a < 3; ++a) gst(p.mean[a] + idx, make_float2(m3[a].re, m3[a].im));
 #pragma unroll
-        for (int g = 0; g < 6; ++g) p.cov[g][idx] = make_float2(cov6[g].re, cov6[g].im);
+        for (int g = 0; g < 6; ++g) gst(p.cov[g] + idx, make_float2(cov6[g].re, cov6[g].im));
         if (p.want_residual) {
 #pragma unroll
-          for (int l = 0; l < NT; ++l) p.data[l][idx] = make_float2(res[l].re, res[l].im);
+          for (int l = 0; l < NT; ++l) gst(p.data[l] + idx, make_float2(res[l].re, res[l].im));
         }
       }
     }
     if (write_prior) {
 #pragma unroll
-      for (int g = 0; g < 6; ++g) p.cov[g][idx] = make_float2(P6[g], 0.f);
+      for (int g = 0; g < 6; ++g) gst(p.cov[g] + idx, make_float2(P6[g], 0.f));
     }
   }
   if (MIXED) {  // wave-aggregated append to the FP64 list
@@ -308,7 +314,7 @@ __device__ __forceinline__ void post32_body(const cg::PostParams &p, int k, long
       base = __shfl(base, 0);
       const unsigned lane = __lane_id();
       const unsigned long long below = lane ? (m & ((~0ull) >> (64 - lane))) : 0ull;
-      if (defer) p.list[base + __popcll(below)] = (unsigned)idx;
+      if (defer) gst(p.list + (base + __popcll(below)), (unsigned)idx);
     }
   }
   if (p.stats) {
@@ -331,7 +337,7 @@ __device__ __forceinline__ void post64_body(const cg::PostParams &p, unsigned lo
   const unsigned long long n = *p.list_len;
   const long plane = (long)p.cnxp * p.nyp;
   for (unsigned long long e = e0; e < n; e += stride) {
-    const long idx = p.list[e];
+    const long idx = gld(p.list + e);
     const int k = (int)(idx / plane);
     const long c = idx % plane;
     const int i = (int)(c % p.cnxp), j = (int)(c / p.cnxp);
@@ -339,40 +345,40 @@ __device__ __forceinline__ void post64_body(const cg::PostParams &p, unsigned lo
     prior6<SEP>(p, idx, i, j, k, P6);
     double P6d[6], rd = 0.0;
     if (SEP) {
-      rd = fabs((double)p.rxy[(long)j * p.cnxp + i] * (double)p.rt[k]);
+      rd = fabs((double)gld(p.rxy + (long)j * p.cnxp + i) * (double)kat(p.rt, k));
 #pragma unroll
       for (int g = 0; g < 6; ++g) P6d[g] = rd * p.s6d[g];
     } else {
 #pragma unroll
       for (int g = 0; g < 6; ++g) P6d[g] = (double)P6[g];
     }
-    const float lt = p.lam_grid ? p.lam_grid[idx].x : p.lam2d[(long)j * p.cnxp + i];
+    const float lt = p.lam_grid ? gld(p.lam_grid + idx).x : gld(p.lam2d + (long)j * p.cnxp + i);
     const double lam = fabs((double)lt);
     cpx<double> m3[3], d[NT], cov6[6], res[NT];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-      const float2 v = p.mean[a][idx];
+      const float2 v = gld(p.mean[a] + idx);
       m3[a] = mk<double>(v.x, v.y);
     }
 #pragma unroll
     for (int l = 0; l < NT; ++l) {
-      const float2 v = p.data[l][idx];
+      const float2 v = gld(p.data[l] + idx);
       d[l] = mk<double>(v.x, v.y);
     }
     double cond = 0.0;
-    const int fail = update<NT, double, SEP>(p.tab64[k], rd, lam, P6d, m3, d, cov6, res, p.want_residual != 0, cond);
+    const int fail = update<NT, double, SEP>(kat(p.tab64, k), rd, lam, P6d, m3, d, cov6, res, p.want_residual != 0, cond);
     if (fail) {
       if (p.stats) atomicAdd(p.stats + 64, 1ull);  // rare: the FP64 factor failed too
 #pragma unroll
-      for (int g = 0; g < 6; ++g) p.cov[g][idx] = make_float2(P6[g], 0.f);
+      for (int g = 0; g < 6; ++g) gst(p.cov[g] + idx, make_float2(P6[g], 0.f));
     } else {
 #pragma unroll
-      for (int a = 0; a < 3; ++a) p.mean[a][idx] = make_float2((float)m3[a].re, (float)m3[a].im);
+      for (int a = 0; a < 3; ++a) gst(p.mean[a] + idx, make_float2((float)m3[a].re, (float)m3[a].im));
 #pragma unroll
-      for (int g = 0; g < 6; ++g) p.cov[g][idx] = make_float2((float)cov6[g].re, (float)cov6[g].im);
+      for (int g = 0; g < 6; ++g) gst(p.cov[g] + idx, make_float2((float)cov6[g].re, (float)cov6[g].im));
       if (p.want_residual) {
 #pragma unroll
-        for (int l = 0; l < NT; ++l) p.data[l][idx] = make_float2((float)res[l].re, (float)res[l].im);
+        for (int l = 0; l < NT; ++l) gst(p.data[l] + idx, make_float2((float)res[l].re, (float)res[l].im));
       }
     }
   }
