@@ -1126,10 +1126,20 @@ struct cg_batch {
     unsigned blocks = 0;
   };
   std::vector<PostGroup> post;
+  // Colouring of one realisation number r of every inversion that has it: the colouring kernel
+  // over the inversions whose realisations are not fused, and per z length one launch of the
+  // colouring fused with the inverse z pass over the rest (whose z passes are then skipped).
+  struct FusedColour {
+    const cg::FixedEntry *e = nullptr;
+    const cg::ColourItem *items = nullptr;
+    int nitems = 0;
+    unsigned blocks = 0;
+  };
   struct ColourPass {
     const cg::ColourItem *items = nullptr;
     int nitems = 0;
     unsigned blocks = 0;
+    std::vector<FusedColour> fused;
   };
   std::vector<ColourPass> colour;
   unsigned long long *counters = nullptr;
@@ -1249,7 +1259,13 @@ int enqueue_batch(cg_batch *b)
   for (auto &p : b->inv_mean)
     if (!rc) rc = run_pass(b, p);
   for (size_t r = 0; r < b->sim.size(); ++r) {
-    if (!rc) rc = cg::colour_batch(ctx, b->colour[r].items, b->colour[r].nitems, b->colour[r].blocks);
+    const auto &cp = b->colour[r];
+    if (!rc)
+      rc = fork_join(b, 1 + (int)cp.fused.size(), [&](int k) {
+        if (k == 0) return cg::colour_batch(ctx, cp.items, cp.nitems, cp.blocks);
+        const auto &f = cp.fused[k - 1];
+        return cg::fixed_colour_zinv_list(ctx, f.e, f.items, f.nitems, f.blocks);
+      });
     for (auto &p : b->sim[r])
       if (!rc) rc = run_pass(b, p);
   }
@@ -1311,6 +1327,7 @@ int cg_batch_create(cg_ctx *ctx, int n, const cg_inversion *inv, cg_batch **out)
   std::vector<std::vector<cg::FftItem>> sz(maxR), sy(maxR), sx(maxR);
   std::vector<cg::ColourItem> col_items;
   std::vector<int> col_count(maxR, 0);
+  std::vector<const cg::FixedEntry *> fused_e(n, nullptr);  // per inversion: fused colouring entry
   size_t lds_fx = 0, lds_fy = 0, lds_fz = 0, lds_ix = 0, lds_iy = 0, lds_iz = 0;
   std::vector<size_t> lds_sx(maxR, 0), lds_sy(maxR, 0), lds_sz(maxR, 0);
   int rc = CG_OK;
@@ -1363,10 +1380,12 @@ int cg_batch_create(cg_ctx *ctx, int n, const cg_inversion *inv, cg_batch **out)
       add_s(iy, lds_iy, a.mean[g], 1);
       add_x(ix, lds_ix, cg::FFT_ITEM_X_C2R, nullptr, a.mean[g]->d, ref, si, nullptr);
     }
+    const cg::FixedEntry *fe = cg::fixed_colour_entry(ctx, ref->nzp, (long)ref->cnxp * ref->nyp);
+    fused_e[i] = fe;
     for (int r = 0; r < v.nreal; ++r)
       for (int g = 0; g < 3; ++g) {
         cg_grid *o = b->outs[i][3 * r + g];
-        add_s(sz[r], lds_sz[r], o, 2);
+        if (!fe) add_s(sz[r], lds_sz[r], o, 2);  // fused: the colouring kernel runs the z pass
         add_s(sy[r], lds_sy[r], o, 1);
         add_x(sx[r], lds_sx[r], cg::FFT_ITEM_X_C2R, nullptr, o->d, ref, si, a.mean[g]->d);
       }
@@ -1428,10 +1447,21 @@ int cg_batch_create(cg_ctx *ctx, int n, const cg_inversion *inv, cg_batch **out)
     o_sy[r] = plan_pass(sy[r], cg::FFT_ITEM_STRIDED);
     o_sx[r] = plan_pass(sx[r], cg::FFT_ITEM_X_C2R);
     int cnt = 0;
-    for (int i = 0; i < n; ++i) cnt += b->inv[i].nreal > r;
+    for (int i = 0; i < n; ++i) cnt += b->inv[i].nreal > r && !fused_e[i];
     col_count[r] = cnt;
     o_col[r] = L.add(sizeof(cg::ColourItem) * cnt);
   }
+  // fused colouring: per realisation number, one list per z length (inversions in order)
+  std::map<const cg::FixedEntry *, std::vector<int>> fused_groups;
+  for (int i = 0; i < n; ++i)
+    if (fused_e[i]) fused_groups[fused_e[i]].push_back(i);
+  std::vector<std::vector<size_t>> o_fused(maxR);
+  for (int r = 0; r < maxR; ++r)
+    for (auto &kv : fused_groups) {
+      int c = 0;
+      for (int i : kv.second) c += b->inv[i].nreal > r;
+      o_fused[r].push_back(L.add(sizeof(cg::ColourItem) * std::max(c, 1)));
+    }
   // per-inversion posterior tables
   struct PT {
     std::vector<cg::PlaneTab<float>> t32;
@@ -1541,8 +1571,44 @@ int cg_batch_create(cg_ctx *ctx, int n, const cg_inversion *inv, cg_batch **out)
     b->sim[r].push_back(finish(o_sx[r], +1, CG_K_FFT_X, lds_sx[r]));
     std::vector<cg::ColourItem> ci;
     unsigned blk = 0;
+    auto colour_item = [&](int i) {
+      cg::ColourItem it;
+      std::memset(&it, 0, sizeof(it));
+      const cg_grid *ref = b->inv[i].mean_in[0];
+      it.p.csize = (long)ref->csize;
+      it.p.cnxp = ref->cnxp;
+      it.p.nyp = ref->nyp;
+      it.p.nzp = ref->nzp;
+      cg::colour_dims(it.p);
+      for (int g = 0; g < 6; ++g) it.p.cov[g] = reinterpret_cast<const float2 *>(b->inv[i].post.cov[g]->d);
+      for (int g = 0; g < 3; ++g) it.p.out[g] = reinterpret_cast<float2 *>(b->outs[i][3 * r + g]->d);
+      it.p.seed = b->inv[i].seed;
+      it.p.stream0 = 4u * (unsigned)r;
+      return it;
+    };
+    size_t gi_f = 0;
+    for (auto &kv : fused_groups) {
+      std::vector<cg::ColourItem> fi;
+      unsigned fb = 0;
+      for (int i : kv.second) {
+        if (b->inv[i].nreal <= r) continue;
+        cg::ColourItem it = colour_item(i);
+        it.block0 = fb;
+        fb += (unsigned)(((long)it.p.cnxp * it.p.nyp + kv.first->cz_tile - 1) / kv.first->cz_tile);
+        fi.push_back(it);
+      }
+      const size_t off = o_fused[r][gi_f++];
+      if (fi.empty()) continue;
+      put(off, fi.data(), sizeof(cg::ColourItem) * fi.size());
+      cg_batch::FusedColour fc;
+      fc.e = kv.first;
+      fc.items = reinterpret_cast<const cg::ColourItem *>(dev + off);
+      fc.nitems = (int)fi.size();
+      fc.blocks = fb;
+      b->colour[r].fused.push_back(fc);
+    }
     for (int i = 0; i < n; ++i) {
-      if (b->inv[i].nreal <= r) continue;
+      if (b->inv[i].nreal <= r || fused_e[i]) continue;
       cg::ColourItem it;
       std::memset(&it, 0, sizeof(it));
       const cg_grid *ref = b->inv[i].mean_in[0];

@@ -838,9 +838,10 @@ constexpr int colour_tile()
   return ti;
 }
 
-template <int N, int TI>
+template <int N, int TI, bool LIST>
 __global__ void __launch_bounds__(3 * TI *Plan<N>::TPS)
-k_colour_zinv(cg::ColourParams p, long plane, const float2 *__restrict__ twg)
+k_colour_zinv(cg::ColourParams one, const cg::ColourItem *__restrict__ items, int nitems,
+              const float2 *__restrict__ twg)
 {
   using IO = typename Plan<N>::template IO<+1, TI>;
   constexpr int TPS = Plan<N>::TPS;
@@ -851,8 +852,20 @@ k_colour_zinv(cg::ColourParams p, long plane, const float2 *__restrict__ twg)
   float2 *tw = smem;
   float2 *buf = smem + N;  // grid g, plane z, column c at (g N + z) TI + c
   const int tid = threadIdx.x;
-  const long q0 = (long)cg_xcd_remap(blockIdx.x, gridDim.x) * TI;
+  // one realisation (the kernel argument, in scalar registers), or a list of them (item i owns
+  // tiles [block0_i, block0_{i+1})), whose workgroup's descriptor is staged in LDS
+  __shared__ cg::ColourParams staged;
+  unsigned tile = cg_xcd_remap(blockIdx.x, gridDim.x);
+  if (LIST) {
+    const int it = find_item(items, nitems, tile);
+    tile -= items[it].block0;
+    if (tid == 0) staged = items[it].p;
+  }
+  const cg::ColourParams &p = LIST ? staged : one;
+  const long q0 = (long)tile * TI;
   load_tw(tw, twg, N, tid, NT);
+  if (LIST) lds_sync();  // the staged descriptor (the twiddles are next read after phase 1's barrier)
+  const long plane = (long)p.cnxp * p.nyp;
 #pragma unroll
   for (int i = 0; i < PER; ++i) {
     const int e = tid + i * NT;
@@ -891,13 +904,20 @@ constexpr int colour_waves()
 }
 
 // ---------------------------------------------------------------- host launchers
+// One realisation (`one`; nblocks = its tiles) or a list (`items`, block0 in tiles of
+// colour_tile<N>() columns; nblocks = all tiles).
 template <int N>
-int colour_zinv_launch(cg_ctx *ctx, const cg::ColourParams &p, long plane, const float2 *tw)
+int colour_zinv_launch(cg_ctx *ctx, const cg::ColourParams *one, const cg::ColourItem *items, int nitems,
+                       unsigned nblocks, const float2 *tw)
 {
   constexpr int TI = colour_tile<N>();
+  if (nblocks == 0) return CG_OK;
   const size_t lds = sizeof(float2) * (N + 3 * (size_t)N * TI);
-  const unsigned nblk = (unsigned)((plane + TI - 1) / TI);
-  k_colour_zinv<N, TI><<<nblk, 3 * TI * Plan<N>::TPS, lds, ctx->stream>>>(p, plane, tw);
+  const cg::ColourParams o = one ? *one : cg::ColourParams{};
+  if (items)
+    k_colour_zinv<N, TI, true><<<nblocks, 3 * TI * Plan<N>::TPS, lds, ctx->stream>>>(o, items, nitems, tw);
+  else
+    k_colour_zinv<N, TI, false><<<nblocks, 3 * TI * Plan<N>::TPS, lds, ctx->stream>>>(o, nullptr, 0, tw);
   return cg::hip_fail(ctx, hipGetLastError(), "k_colour_zinv");
 }
 

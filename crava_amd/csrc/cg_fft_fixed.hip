@@ -143,15 +143,22 @@ static TuneKey colour_key(const cg_ctx *ctx, int nzp, long plane)
   return TuneKey{(long)ctx->device, (long)FIX_KINDS, (long)nzp, plane, 0, 0};
 }
 
-int fixed_colour_zinv(cg_ctx *ctx, const ColourParams &p, int nzp, long plane, int *rc)
+const FixedEntry *fixed_colour_entry(cg_ctx *ctx, int nzp, long plane)
 {
   const FixedEntry *e = fixed_find(FIX_STRIDED, nzp);
-  if (!e || !e->cz || p.from_grids || ctx->fused_colour == 0) return 0;
+  if (!e || !e->cz || ctx->fused_colour == 0 || ctx->fft_engine != CG_FFT_AUTO || nzp < 2) return nullptr;
   if (ctx->fused_colour != 1) {
     std::lock_guard<std::mutex> lock(tune_mu);
     const auto it = tune_table.find(colour_key(ctx, nzp, plane));
-    if (it != tune_table.end() ? it->second == 0 : e->cz_waves < 12) return 0;
+    if (it != tune_table.end() ? it->second == 0 : e->cz_waves < 12) return nullptr;
   }
+  return e;
+}
+
+int fixed_colour_zinv(cg_ctx *ctx, const ColourParams &p, int nzp, long plane, int *rc)
+{
+  const FixedEntry *e = p.from_grids ? nullptr : fixed_colour_entry(ctx, nzp, plane);
+  if (!e) return 0;
   const float2 *tw = twiddle_table(ctx, nzp);
   if (!tw) {
     *rc = CG_ENOMEM;
@@ -159,9 +166,22 @@ int fixed_colour_zinv(cg_ctx *ctx, const ColourParams &p, int nzp, long plane, i
   }
   hipEvent_t ev;
   launch_begin(ctx, CG_K_NOISE, &ev);
-  *rc = e->cz(ctx, p, plane, tw);
+  *rc = e->cz(ctx, &p, nullptr, 0, (unsigned)((plane + e->cz_tile - 1) / e->cz_tile), tw);
   launch_end(ctx, CG_K_NOISE, ev);
   return 1;
+}
+
+int fixed_colour_zinv_list(cg_ctx *ctx, const FixedEntry *e, const ColourItem *dev_items, int nitems,
+                           unsigned nblocks)
+{
+  if (nitems <= 0 || nblocks == 0) return CG_OK;
+  const float2 *tw = twiddle_table(ctx, e->n);
+  if (!tw) return CG_ENOMEM;
+  hipEvent_t ev;
+  launch_begin(ctx, CG_K_NOISE, &ev);
+  const int rc = e->cz(ctx, nullptr, dev_items, nitems, nblocks, tw);
+  launch_end(ctx, CG_K_NOISE, ev);
+  return rc;
 }
 
 // Times both tile variants of every pass of a grid of padded size nxp x nyp x nzp (x forward
@@ -265,7 +285,8 @@ int autotune_geometry(cg_ctx *ctx, int nxp, int nyp, int nzp)
         for (int g = 0; g < 3 && !r; ++g) r = ez->s[v](ctx, +1, &sa[g], nullptr, 0, nblk, tw);
         return r;
       });
-      ms[1] = time_it([&] { return ez->cz(ctx, p, plane, tw); });
+      const unsigned czb = (unsigned)((plane + ez->cz_tile - 1) / ez->cz_tile);
+      ms[1] = time_it([&] { return ez->cz(ctx, &p, nullptr, 0, czb, tw); });
       if (ms[0] >= 0.f && ms[1] >= 0.f) {
         std::lock_guard<std::mutex> lock(tune_mu);
         tune_table[colour_key(ctx, nzp, plane)] = ms[1] < ms[0] ? 1 : 0;

@@ -68,7 +68,9 @@ typedef int (*StridedFn)(cg_ctx *, int dir, const StridedArgs *one, const Stride
 typedef int (*XFn)(cg_ctx *, int forward, int add, const XArgs *one, const XArgs *items, int nitems,
                    unsigned nblocks, const float2 *tw, const float2 *twr);
 struct ColourParams;
-typedef int (*ColourZFn)(cg_ctx *, const ColourParams &p, long plane, const float2 *tw);
+struct ColourItem;
+typedef int (*ColourZFn)(cg_ctx *, const ColourParams *one, const ColourItem *items, int nitems, unsigned nblocks,
+                         const float2 *tw);
 struct FixedEntry {
   int kind, n;
   int tile[2];  // the two tile variants (equal when the length has only one)
@@ -76,6 +78,7 @@ struct FixedEntry {
   XFn x[2];
   ColourZFn cz;  // strided lengths: colouring fused with the inverse z pass
   int cz_waves;  // wavefronts of the fused kernel resident per CU
+  int cz_tile;   // its columns per workgroup
   int tune;      // 0: variant 0 is kept (the autotuner's timing on scratch grids misjudges it)
 };
 // Tile variant for a pass geometry: the autotuner's choice (cg_autotune) for this exact
@@ -187,6 +190,12 @@ int autotune_geometry(cg_ctx *ctx, int nxp, int nyp, int nzp);
 // faster choice for this geometry (autotuned, else when it keeps at least 12 wavefronts per CU
 // resident; with fewer the colouring's loads are poorly hidden).
 int fixed_colour_zinv(cg_ctx *ctx, const ColourParams &p, int nzp, long plane, int *rc);
+// The compiled entry whose fused colouring a realisation of nzp planes of `plane` cells would use
+// (same choice as fixed_colour_zinv), or null.
+const FixedEntry *fixed_colour_entry(cg_ctx *ctx, int nzp, long plane);
+// Fused colouring of a list of realisations of one nzp (descriptors in device memory).
+int fixed_colour_zinv_list(cg_ctx *ctx, const FixedEntry *e, const ColourItem *dev_items, int nitems,
+                           unsigned nblocks);
 // grid_fft_inverse without the z pass (it was fused into the producer).
 int grid_fft_inverse_xy(cg_grid *g, const float *add_real);
 int grid_fft_inverse(cg_grid *g, const float *add_real);

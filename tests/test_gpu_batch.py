@@ -132,3 +132,41 @@ def test_batch_compiled_plans_match_runtime_plans(ctx):
         assert rel_l2(ma - p["mean_real"], mg - p["mean_real"]) < 5e-5
         assert rel_l2(ca, cg_) < 5e-5
         assert rel_l2(oa[0] - ma, og[0] - mg) < 5e-5
+
+
+def test_batch_fused_colouring_matches_unfused_and_oracle(ctx):
+    """Realisations of a batch with the colouring fused into the inverse z pass (one launch per z
+    length over every inversion of that length) and with the colouring kernel then z passes: both
+    match the oracle, and the fused schedule really skips the z passes it absorbed.  The mix holds
+    two z lengths of one plan each plus the single-cube depth (256 of 320 planes)."""
+    probs = [P.make_problem(**MIX[0]), P.make_problem(**MIX[1]), P.make_problem(11, 7, 256, nzp=320, seed=5),
+             P.make_problem(9, 6, 256, nzp=320, seed=6)]
+    nreal = 2
+    items = [_item(ctx, p, nreal, 60 + i) for i, p in enumerate(probs)]
+    res, zlaunches = {}, {}
+    for fused in (0, 1):
+        print("batch fused colour", fused, flush=True)
+        ctx.set_fused_colour(fused)
+        b = C.Batch(ctx, items)
+        ctx.set_profiling(True)
+        ctx.reset_kernel_time()
+        b.run()
+        b.wait()
+        zlaunches[fused] = ctx.kernel_time(C.K_FFT_Z)[0]
+        ctx.set_profiling(False)
+        res[fused] = [_results(it) for it in items]
+        b.close()
+    ctx.set_fused_colour(-1)
+    assert zlaunches[1] < zlaunches[0]
+    for i, p in enumerate(probs):
+        mean, cov, outs = res[1][i]
+        _, _, outs0 = res[0][i]
+        ms, ds = P.forward_spectra(p, np.float64)
+        _, _, oc, _, _ = O.posterior(p, ms, ds, None, "f64")
+        for r in range(nreal):
+            noise = np.stack([O.fill_complex_noise_cb(p["nxp"], p["nyp"], p["nzp"], 60 + i, 4 * r + g)
+                              for g in range(3)])
+            z, _ = O.colour(oc, noise, "f64")
+            real = np.stack([O.grid_ifft(z[g], p["nxp"], O.PARAMETER) for g in range(3)])
+            assert rel_l2(outs[r] - mean, real) < 1e-4, (i, r)
+            assert rel_l2(outs[r] - mean, outs0[r] - mean) < 1e-5, (i, r)
