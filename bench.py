@@ -209,6 +209,47 @@ def bench_mix(ctx, n=64, reps=3):
     }
 
 
+def bench_input_padding(ctx, inv, w, reps=5):
+    """The padding fills on the input side at the workload's size, on scratch grids of `inv`
+    (its realisation outputs, rewritten by every step): the background cube fill (nx*ny*nz
+    floats read from HBM, the padded grid written) and the seismic trace interpolation with 5%
+    of the traces zeroed (energies, host walk, trace averages, tapered extrapolation)."""
+    src, dst, seis = inv.out[0]
+    nx, ny, nz = w["nx"], w["ny"], w["nz"]
+    ctx.set_profiling(True)
+    ctx.reset_kernel_time()
+    for _ in range(reps):
+        dst.fill_padded(src.device_ptr())  # src's memory as a device cube of nz*ny*nx floats
+    n, ms = ctx.kernel_time(C.K_ELEMENTWISE)
+    ctx.set_profiling(False)
+    fill_bytes = 4 * (nx * ny * nz + dst.rnxp * w["nyp"] * w["nzp"])
+    fill_gbps = fill_bytes / (ms * 1e-3 / n) / 1e9
+    rv = inv.seis_in[0].download_real(padded=True)
+    rng = np.random.default_rng(3)
+    bad = rng.random((ny, nx)) < 0.05
+    rv[:, :ny, :nx][:, bad] = 0.0
+    seis.setType(C.DATA)
+    seis.upload_real(rv)
+    ctx.set_profiling(True)
+    ctx.reset_kernel_time()
+    t = time.perf_counter()
+    nbad, _ = seis.interpolateSeismic(0.01)
+    ctx.sync()
+    wall = 1000.0 * (time.perf_counter() - t)
+    kn, kms = ctx.kernel_time(C.K_ELEMENTWISE)
+    ctx.set_profiling(False)
+    seis.setType(C.PARAMETER)
+    return {
+        "fill_padded": {"what": "FFTGrid(NRLib::Grid) background fill, %dx%dx%d cube into %dx%dx%d"
+                                % (nx, ny, nz, w["nxp"], w["nyp"], w["nzp"]),
+                        "avg_launch_us": round(1000.0 * ms / n, 2), "bytes_per_launch": fill_bytes,
+                        "gbps": round(fill_gbps, 1), "frac_of_peak": fill_gbps * 1e9 / HBM_PEAK},
+        "interpolate_seismic": {"what": "FFTGrid::interpolateSeismic, 5%% of %d traces zeroed" % (nx * ny),
+                                "bad_traces": nbad, "ms_synchronised": round(wall, 2),
+                                "kernel_ms": round(kms, 3), "launches": kn},
+    }
+
+
 def bench_large(ctx, steps=2):
     """The large single cube with simulation: 1024 x 1024 x 320 padded to 1152 x 1152 x 384, 6
     angles 0-50 deg, Ormsby 5-10-60-80 Hz wavelet, 1500 m lateral range, 20 posterior realisations
@@ -286,6 +327,7 @@ def main():
     ap.add_argument("--no-mix", action="store_true", help="skip the batched-vs-plain uneven mix")
     ap.add_argument("--no-serve", action="store_true", help="skip the continuous-batching serving run")
     ap.add_argument("--no-large", action="store_true", help="skip the 1152x1152x384 cube with 20 realisations")
+    ap.add_argument("--no-pad", action="store_true", help="skip the input padding fills")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a copy-like kernel of known traffic (for scaling hardware byte counters)")
     ap.add_argument("--profile-steps", type=int, default=2)
@@ -421,6 +463,9 @@ def main():
         "posterior_stats": {"inverted": stats[0], "cholesky_failures": stats[1], "fp64_resolves": stats[2]},
         "cpu_baseline": None,
     }
+    if not args.no_pad:
+        result["input_padding"] = bench_input_padding(ctx, inv, w)
+        log("input padding: %s" % result["input_padding"])
     inv.close()
     if not args.no_mix:
         log("uneven mix: plain vs batched ...")

@@ -98,6 +98,9 @@ def lib():
             "cg_autotune": [_vp, i, i, i],
             "cg_arena_create": [_vp, u64, ctypes.POINTER(_vp)], "cg_arena_destroy": [_vp],
             "cg_grid_create_in": [_vp, i, i, i, i, i, i, i, ctypes.POINTER(_vp)],
+            "cg_grid_fill_padded": [_vp, _vp, i],
+            "cg_grid_interpolate_seismic": [_vp, f, _i_p, _i_p],
+            "cg_grid_extrapolate_seismic": [_vp, i, i, i, i],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -283,6 +286,29 @@ class FFTGrid:
     def fillInConstant(self, value):
         """FFTGrid::fillInConstant: value on the cube, RMISSING on the row padding (real domain)."""
         self.ctx.check(lib().cg_grid_fill_constant(self._h, value), "fillInConstant")
+
+    def fill_padded(self, cube):
+        """FFTGrid(const NRLib::Grid<float> *, nxp, nyp, nzp): cube (nz, ny, nx), a host array, or
+        an integer device address of nz*ny*nx floats; the padding is filled by linear edge
+        interpolation and the grid becomes a PARAMETER grid in the real domain."""
+        if isinstance(cube, int):
+            self.ctx.check(lib().cg_grid_fill_padded(self._h, ctypes.c_void_p(cube), 1), "fill_padded")
+            return
+        a = np.ascontiguousarray(cube, dtype=np.float32)
+        if a.shape != (self.nz, self.ny, self.nx):
+            raise ValueError("cube must be (nz, ny, nx) = %s" % ((self.nz, self.ny, self.nx),))
+        self.ctx.check(lib().cg_grid_fill_padded(self._h, a.ctypes.data_as(_vp), 0), "fill_padded")
+
+    def interpolateSeismic(self, energy_threshold):
+        """FFTGrid::interpolateSeismic; returns (bad traces, (imin, imax, jmin, jmax))."""
+        nbad, rect = ctypes.c_int(), (ctypes.c_int * 4)()
+        self.ctx.check(lib().cg_grid_interpolate_seismic(self._h, energy_threshold, ctypes.byref(nbad), rect),
+                       "interpolateSeismic")
+        return nbad.value, tuple(rect)
+
+    def extrapolateSeismic(self, imin, imax, jmin, jmax):
+        """FFTGrid::extrapolateSeismic with the rectangle of good box traces."""
+        self.ctx.check(lib().cg_grid_extrapolate_seismic(self._h, imin, imax, jmin, jmax), "extrapolateSeismic")
 
     def fillInComplexNoiseReference(self, seed):
         """FFTGrid::fillInComplexNoise with the reference's RandomGen stream from state `seed`;

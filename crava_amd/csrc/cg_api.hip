@@ -718,6 +718,263 @@ int cg_grid_fill_constant(cg_grid *g, float value)
   return rc;
 }
 
+int cg_grid_fill_padded(cg_grid *g, const float *cube, int cube_on_device)
+{
+  if (!g || !cube) return CG_EINVAL;
+  cg_ctx *ctx = g->ctx;
+  DeviceGuard dg(ctx->device);
+  const float *src = cube;
+  void *tmp = nullptr;
+  if (!cube_on_device) {
+    const size_t bytes = sizeof(float) * (size_t)g->nx * g->ny * g->nz;
+    hipError_t e = cg::dev_alloc(ctx, &tmp, bytes);
+    if (e != hipSuccess) return cg::hip_fail(ctx, e, "fill_padded: staging the cube");
+    e = hipMemcpyAsync(tmp, cube, bytes, hipMemcpyHostToDevice, ctx->stream);
+    if (e != hipSuccess) {
+      cg::dev_free(ctx, tmp);
+      return cg::hip_fail(ctx, e, "fill_padded: copying the cube");
+    }
+    src = static_cast<const float *>(tmp);
+  }
+  int rc = cg::fill_padded(ctx, g->d, src, g->nx, g->ny, g->nz, g->nxp, g->nyp, g->nzp);
+  if (tmp) {
+    cg::dev_free(ctx, tmp);
+    // the caller may release its host cube on return
+    const hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (rc == CG_OK) rc = cg::hip_fail(ctx, e, "fill_padded");
+  }
+  if (rc == CG_OK) {
+    g->cubetype = CG_CT_PARAMETER;
+    g->transformed = 0;
+  }
+  return rc;
+}
+
+}  // extern "C"
+
+namespace {
+
+// FFTGrid::getFillNumber (fftgrid.cpp:1103-1140) for 0 <= i < np: the box index whose trace a
+// padding index copies, mirrored towards the nearer end and clamped at n / 2.
+int fill_number(int i, int n, int np)
+{
+  if (i < n) return i;
+  const int to_np = np - i - 1, past_n = i - n + 1;
+  return past_n < to_np ? std::max(n - past_n, n / 2) : std::min(to_np, n / 2);
+}
+
+// FFTGrid::getDistToBoundary (fftgrid.cpp:1157-1200) for 0 <= i < np: distance into the padding
+// in taper lengths min(n, np - n) / 2.1, with the reference's float rounding.
+float dist_to_boundary(int i, int n, int np)
+{
+  if (i < n) return 0.0f;
+  const float taper = static_cast<float>(std::min(n, np - n) / 2.1);
+  const int to_np = np - i, past_n = i - (n - 1);
+  return static_cast<float>(past_n < to_np ? past_n : to_np) / taper;
+}
+
+struct ExtrapTables {
+  std::vector<int> ref_i, ref_j;
+  std::vector<float> dx, dy, dz;
+};
+
+ExtrapTables extrap_tables(const cg_grid *g, int imin, int imax, int jmin, int jmax)
+{
+  ExtrapTables t;
+  for (int i = 0; i < g->nxp; ++i) {
+    t.ref_i.push_back(std::min(std::max(fill_number(i, g->nx, g->nxp), imin), imax));
+    t.dx.push_back(dist_to_boundary(i, g->nx, g->nxp));
+  }
+  for (int j = 0; j < g->nyp; ++j) {
+    t.ref_j.push_back(std::min(std::max(fill_number(j, g->ny, g->nyp), jmin), jmax));
+    t.dy.push_back(dist_to_boundary(j, g->ny, g->nyp));
+  }
+  for (int k = 0; k < g->nzp; ++k) t.dz.push_back(dist_to_boundary(k, g->nz, g->nzp));
+  return t;
+}
+
+// A range of the level-sorted trace averages: one dependency level (one workgroup per op) or
+// a run of small levels applied in order by one workgroup.
+struct OpRun {
+  int o0, o1;
+  bool one_level;
+};
+
+// Orders the trace averages by dependency level: an op comes after every earlier op that wrote
+// one of its neighbours or read or wrote its own trace, so the ops of a level touch no column
+// another of them writes and each reads what the sequential order gives it.  Levels of at
+// least kWideLevel ops become one launch each; the smaller ones between them one serial run.
+void level_ops(std::vector<unsigned> &ops, int nx, int ntr, std::vector<OpRun> &runs)
+{
+  constexpr int kWideLevel = 8;
+  std::vector<int> wrote(ntr, -1), read(ntr, -1), lev(ops.size());
+  int nlev = 0;
+  for (size_t o = 0; o < ops.size(); ++o) {
+    const int t = (int)(ops[o] >> 4);
+    const unsigned m = ops[o] & 15u;
+    const int nb[4] = {t - 1, t + 1, t - nx, t + nx};
+    int L = std::max(wrote[t], read[t]);
+    for (int b = 0; b < 4; ++b)
+      if (m >> b & 1u) L = std::max(L, wrote[nb[b]]);
+    lev[o] = ++L;
+    wrote[t] = L;
+    for (int b = 0; b < 4; ++b)
+      if (m >> b & 1u) read[nb[b]] = std::max(read[nb[b]], L);
+    nlev = std::max(nlev, L + 1);
+  }
+  std::vector<int> start(nlev + 1, 0);
+  for (int l : lev) ++start[l + 1];
+  for (int l = 0; l < nlev; ++l) start[l + 1] += start[l];
+  std::vector<unsigned> sorted(ops.size());
+  std::vector<int> fill(start.begin(), start.end() - 1);
+  for (size_t o = 0; o < ops.size(); ++o) sorted[fill[lev[o]]++] = ops[o];
+  ops.swap(sorted);
+  runs.clear();
+  int pending = -1;  // first op of the serial run being gathered
+  for (int l = 0; l < nlev; ++l) {
+    const int o0 = start[l], o1 = start[l + 1];
+    if (o1 - o0 >= kWideLevel) {
+      if (pending >= 0) runs.push_back({pending, o0, false});
+      pending = -1;
+      runs.push_back({o0, o1, true});
+    } else if (pending < 0) {
+      pending = o0;
+    }
+  }
+  if (pending >= 0) runs.push_back({pending, (int)ops.size(), false});
+}
+
+// Stages the extrapolation tables (and, when given, the trace averages before them) and
+// enqueues both passes.
+int seismic_passes(cg_grid *g, const std::vector<unsigned> &ops, const std::vector<OpRun> &runs, int imin, int imax,
+                   int jmin, int jmax)
+{
+  cg_ctx *ctx = g->ctx;
+  const ExtrapTables t = extrap_tables(g, imin, imax, jmin, jmax);
+  std::vector<void *> dp;
+  int rc = stage_tables(ctx,
+                        {{ops.data(), sizeof(unsigned) * ops.size()},
+                         {t.ref_i.data(), sizeof(int) * t.ref_i.size()},
+                         {t.dx.data(), sizeof(float) * t.dx.size()},
+                         {t.ref_j.data(), sizeof(int) * t.ref_j.size()},
+                         {t.dy.data(), sizeof(float) * t.dy.size()},
+                         {t.dz.data(), sizeof(float) * t.dz.size()}},
+                        dp);
+  for (const OpRun &r : runs) {
+    if (rc != CG_OK) break;
+    rc = cg::trace_ops(ctx, g->d, static_cast<const unsigned *>(dp[0]) + r.o0, r.o1 - r.o0, r.one_level, g->nx,
+                       g->rnxp, g->nyp, g->nzp);
+  }
+  if (rc == CG_OK)
+    rc = cg::extrapolate(ctx, g->d, static_cast<const int *>(dp[1]), static_cast<const float *>(dp[2]),
+                         static_cast<const int *>(dp[3]), static_cast<const float *>(dp[4]),
+                         static_cast<const float *>(dp[5]), g->nxp, g->nyp, g->nzp, imin, imax, jmin, jmax);
+  return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cg_grid_extrapolate_seismic(cg_grid *g, int imin, int imax, int jmin, int jmax)
+{
+  if (!g) return CG_EINVAL;
+  if (g->transformed) return cg::fail(g->ctx, CG_ESTATE, "extrapolate_seismic: grid is in the Fourier domain");
+  // outside a non-empty box rectangle the reference's in-place sweep would read cells it has
+  // already overwritten; such rectangles are refused
+  if (imin < 0 || imin > imax || imax >= g->nx || jmin < 0 || jmin > jmax || jmax >= g->ny)
+    return cg::fail(g->ctx, CG_EINVAL, "extrapolate_seismic: the rectangle must lie in the box and be non-empty");
+  DeviceGuard dg(g->ctx->device);
+  return seismic_passes(g, {}, {}, imin, imax, jmin, jmax);
+}
+
+int cg_grid_interpolate_seismic(cg_grid *g, float energy_threshold, int *n_bad, int rect[4])
+{
+  if (!g) return CG_EINVAL;
+  cg_ctx *ctx = g->ctx;
+  if (g->cubetype != CG_CT_DATA) return cg::fail(ctx, CG_ESTATE, "interpolate_seismic: grid is not of type DATA");
+  if (g->transformed) return cg::fail(ctx, CG_ESTATE, "interpolate_seismic: grid is in the Fourier domain");
+  const int nx = g->nx, ny = g->ny, ntr = nx * ny;
+  if ((long)nx * ny >= (1L << 28)) return cg::fail(ctx, CG_EINVAL, "interpolate_seismic: more than 2^28 traces");
+  DeviceGuard dg(ctx->device);
+  // energies over the nz box samples (device), then the flag walk on the host
+  void *de = nullptr;
+  hipError_t e = cg::dev_alloc(ctx, &de, sizeof(float) * ntr);
+  if (e != hipSuccess) return cg::hip_fail(ctx, e, "interpolate_seismic: energy buffer");
+  std::vector<float> energy(ntr);
+  int rc = cg::trace_energy(ctx, g->d, static_cast<float *>(de), nx, ny, g->nz, g->rnxp, g->nyp);
+  if (rc == CG_OK) {
+    e = hipMemcpyAsync(energy.data(), de, sizeof(float) * ntr, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    rc = cg::hip_fail(ctx, e, "interpolate_seismic: energies");
+  }
+  cg::dev_free(ctx, de);
+  if (rc != CG_OK) return rc;
+  float total = 0.0f;
+  for (int t = 0; t < ntr; ++t) total += energy[t];
+  const float limit = energy_threshold * total / static_cast<float>(ntr);
+  // flag bit 0: this trace is bad; bit 1: a bad trace came earlier (fftgrid.cpp:2351, 2376-2399)
+  std::vector<short> flags(ntr);
+  int imin = nx, imax = 0, jmin = ny, jmax = 0, nbad = 0;
+  short earlier = 0;
+  for (int t = 0; t < ntr; ++t) {
+    const bool bad = energy[t] <= limit;
+    flags[t] = static_cast<short>(earlier + (bad ? 1 : 0));
+    if (bad) {
+      ++nbad;
+      earlier = 2;
+    } else {
+      imin = std::min(imin, t % nx);
+      imax = std::max(imax, t % nx);
+      jmin = std::min(jmin, t / nx);
+      jmax = std::max(jmax, t / nx);
+    }
+  }
+  if (nbad == ntr) return cg::fail(ctx, CG_EINVAL, "interpolate_seismic: no trace above the energy limit");
+  // A bad trace with at least two good 4-neighbours becomes their mean (interpolateTrace,
+  // fftgrid.cpp:2440-2481); after a success the pending bad traces before it are retried back
+  // to the last settled one, and the settled run is cleared of bit 1 (fftgrid.cpp:2407-2434).
+  // The walk depends on the flags alone, so it is replayed here and only the averages run on
+  // the device, level by level.
+  std::vector<unsigned> ops;
+  auto try_trace = [&](int t) {
+    const int i = t % nx, j = t / nx;
+    unsigned m = 0;
+    if (i > 0 && flags[t - 1] % 2 == 0) m |= 1u;
+    if (i < nx - 1 && flags[t + 1] % 2 == 0) m |= 2u;
+    if (j > 0 && flags[t - nx] % 2 == 0) m |= 4u;
+    if (j < ny - 1 && flags[t + nx] % 2 == 0) m |= 8u;
+    if (__builtin_popcount(m) > 1) {
+      ops.push_back(static_cast<unsigned>(t) << 4 | m);
+      --flags[t];
+    }
+    return flags[t];
+  };
+  for (int cur = 0; cur < ntr; ++cur) {
+    if (flags[cur] % 2 != 1 || try_trace(cur) % 2 != 0) continue;
+    int t = cur - 1;
+    while (t >= 0 && flags[t] > 1) {
+      if (flags[t] % 2 == 1) try_trace(t);
+      --t;
+    }
+    t = std::max(t, 0) + 1;
+    while (flags[t - 1] == 0 && t <= cur) {
+      if (flags[t] > 1) flags[t] -= 2;
+      ++t;
+    }
+  }
+  if (n_bad) *n_bad = nbad;
+  if (rect) {
+    rect[0] = imin;
+    rect[1] = imax;
+    rect[2] = jmin;
+    rect[3] = jmax;
+  }
+  std::vector<OpRun> runs;
+  level_ops(ops, nx, ntr, runs);
+  return seismic_passes(g, ops, runs, imin, imax, jmin, jmax);
+}
+
 int cg_grid_fill_noise_reference(cg_grid *g, uint32_t *seed)
 {
   if (!g || !seed) return CG_EINVAL;

@@ -206,6 +206,16 @@ int scale_real(cg_ctx *ctx, float *d, size_t n, float s);
 int fill_noise_reference(cg_ctx *ctx, float2 *out, int nxp, int nyp, int nzp, uint32_t *seed);
 int fill_constant(cg_ctx *ctx, float *d, size_t n, int nxp, int rnxp, float v);
 int add_real(cg_ctx *ctx, float *d, const float *s, size_t n);
+// Padding fills (cg_pad.hip): the background cube into the padded grid; per-trace energies;
+// trace averages (op = trace << 4 | neighbour mask: 1 left, 2 right, 4 up, 8 down), either ops
+// of one dependency level (one workgroup each) or a run to apply in order (one workgroup);
+// the tapered extrapolation outside a rectangle of box traces (tables per axis).
+int fill_padded(cg_ctx *ctx, float *rv, const float *cube, int nx, int ny, int nz, int nxp, int nyp, int nzp);
+int trace_energy(cg_ctx *ctx, const float *rv, float *energy, int nx, int ny, int nz, int rnxp, int nyp);
+int trace_ops(cg_ctx *ctx, float *rv, const unsigned *ops, int nops, bool one_level, int nx, int rnxp, int nyp,
+              int nzp);
+int extrapolate(cg_ctx *ctx, float *rv, const int *ref_i, const float *dist_x, const int *ref_j, const float *dist_y,
+                const float *dist_z, int nxp, int nyp, int nzp, int imin, int imax, int jmin, int jmax);
 
 }  // namespace cg
 

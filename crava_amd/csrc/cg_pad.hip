@@ -1,0 +1,210 @@
+// cg_pad.hip -- padding fills of the grids that enter the path: the background model's linear
+// edge interpolation (fftgrid.cpp:125-213) and the seismic data's trace interpolation and
+// tapered extrapolation (fftgrid.cpp:2349-2518).
+//
+// The reference fills the padding in one sequential sweep whose later values read earlier
+// ones.  Here every cell evaluates the chain it depends on directly (z padding from y-padded
+// values, those from x-padded values, those from the cube), so the cells are written in any
+// order, with the same floats.  Every product and sum is rounded on its own, as the
+// reference's x86-64 build computes them (no fused multiply-add).
+#include "cg_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace {
+
+struct PadGeom {
+  int nx, ny, nz, nxp, nyp, nzp, rnxp;
+  float sa, sb, sc;  // 1 / (padding + 1) per axis, rounded on the host as the reference does
+};
+
+__device__ __forceinline__ float blend(float lo, float wlo, float hi, float whi) { return lo * wlo + hi * whi; }
+
+// cube value, or its x-padding blend (j < ny, k < nz, i < nxp)
+__device__ __forceinline__ float pad_x(const PadGeom &g, const float *__restrict__ cube, int i, int j, int k)
+{
+  const float *row = cube + ((long)k * g.ny + j) * g.nx;
+  if (i < g.nx) return row[i];
+  return blend(row[0], g.sa * (float)(i - g.nx + 1), row[g.nx - 1], g.sa * (float)(g.nxp - i));
+}
+
+// ... or its y-padding blend of x-padded values (k < nz, i < nxp)
+__device__ __forceinline__ float pad_xy(const PadGeom &g, const float *__restrict__ cube, int i, int j, int k)
+{
+  if (j < g.ny) return pad_x(g, cube, i, j, k);
+  return blend(pad_x(g, cube, i, 0, k), g.sb * (float)(j - g.ny + 1), pad_x(g, cube, i, g.ny - 1, k),
+               g.sb * (float)(g.nyp - j));
+}
+
+// The padded grid's value at (i, j, k).
+__device__ __forceinline__ float pad_cell(const PadGeom &g, const float *__restrict__ cube, int i, int j, int k)
+{
+  if (i >= g.nxp) return 0.f;  // the row padding of the in-place layout
+  if (k < g.nz) return pad_xy(g, cube, i, j, k);
+  return blend(pad_xy(g, cube, i, j, 0), g.sc * (float)(k - g.nz + 1), pad_xy(g, cube, i, j, g.nz - 1),
+               g.sc * (float)(g.nzp - k));
+}
+
+// Cell pairs (x even, x + 1: rnxp is even, so a pair never leaves its row) stored as float2;
+// each thread computes kPadU pairs a block-width apart before storing any, so that enough
+// loads are in flight to cover HBM latency.  grid.x over a plane's pairs / (kThreads kPadU),
+// grid.y = k.
+constexpr int kPadU = 4;
+
+__global__ void __launch_bounds__(cg::kThreads)
+k_fill_padded(float *__restrict__ rv, const float *__restrict__ cube, PadGeom g)
+{
+  const unsigned pairs = (unsigned)(g.rnxp / 2) * (unsigned)g.nyp;
+  const int k = blockIdx.y;
+  float2 v[kPadU];
+#pragma unroll
+  for (int u = 0; u < kPadU; ++u) {
+    const unsigned p = (blockIdx.x * kPadU + u) * cg::kThreads + threadIdx.x;
+    if (p < pairs) {
+      const unsigned c = 2 * p;
+      const int i = (int)(c % (unsigned)g.rnxp), j = (int)(c / (unsigned)g.rnxp);
+      v[u] = make_float2(pad_cell(g, cube, i, j, k), pad_cell(g, cube, i + 1, j, k));
+    }
+  }
+  float2 *plane = reinterpret_cast<float2 *>(rv + (long)k * g.nyp * g.rnxp);
+#pragma unroll
+  for (int u = 0; u < kPadU; ++u) {
+    const unsigned p = (blockIdx.x * kPadU + u) * cg::kThreads + threadIdx.x;
+    if (p < pairs) plane[p] = v[u];
+  }
+}
+
+// Energy of every box trace over its nz samples, summed in k order (one thread per trace).
+__global__ void __launch_bounds__(cg::kThreads)
+k_trace_energy(const float *__restrict__ rv, float *__restrict__ energy, int nx, int ny, int nz, int rnxp, int nyp)
+{
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nx * ny) return;
+  const int i = t % nx, j = t / nx;
+  const long plane = (long)rnxp * nyp;
+  const float *p = rv + (long)j * rnxp + i;
+  float e = 0.f;
+  for (int k = 0; k < nz; ++k) {
+    const float v = p[k * plane];
+    e += v * v;
+  }
+  energy[t] = e;
+}
+
+// One trace interpolation: op = trace index << 4 | neighbour mask (1 left, 2 right, 4 up,
+// 8 down); the column becomes the mean of the masked neighbours, summed in that order.
+__device__ __forceinline__ void trace_op(float *__restrict__ rv, unsigned op, int nx, int rnxp, long plane, int nzp)
+{
+  const unsigned m = op & 15u, t = op >> 4;
+  const int i = (int)(t % (unsigned)nx), j = (int)(t / (unsigned)nx);
+  const float n = (float)__popc(m);
+  float *col = rv + (long)j * rnxp + i;
+  for (int k = threadIdx.x; k < nzp; k += blockDim.x) {
+    const long z = k * plane;
+    float s = (m & 1u) ? col[z - 1] : 0.f;
+    if (m & 2u) s += col[z + 1];
+    if (m & 4u) s += col[z - rnxp];
+    if (m & 8u) s += col[z + rnxp];
+    col[z] = s / n;
+  }
+}
+
+// A run of ops in order (each may read columns written by the ones before): one workgroup,
+// the barrier after each op makes its column visible to the next.
+__global__ void __launch_bounds__(1024)
+k_trace_ops_serial(float *__restrict__ rv, const unsigned *__restrict__ ops, int nops, int nx, int rnxp, int nyp,
+                   int nzp)
+{
+  for (int o = 0; o < nops; ++o) {
+    trace_op(rv, ops[o], nx, rnxp, (long)rnxp * nyp, nzp);
+    __syncthreads();
+  }
+}
+
+// Ops of one dependency level (none reads or writes a column another of them writes): one
+// workgroup each.
+__global__ void __launch_bounds__(cg::kThreads)
+k_trace_ops_level(float *__restrict__ rv, const unsigned *__restrict__ ops, int nx, int rnxp, int nyp, int nzp)
+{
+  trace_op(rv, ops[blockIdx.x], nx, rnxp, (long)rnxp * nyp, nzp);
+}
+
+// Cells outside the rectangle [imin, imax] x [jmin, jmax] of good traces: the clamped
+// fill-number trace times max(1 - dx^2 - dy^2 - dz^2, 0)^3.  Reads only rectangle traces,
+// writes only others, so it runs in place.  grid.x over nxp * nyp, grid.y = k.
+__global__ void __launch_bounds__(cg::kThreads)
+k_extrapolate(float *__restrict__ rv, const int *__restrict__ ref_i, const float *__restrict__ dist_x,
+              const int *__restrict__ ref_j, const float *__restrict__ dist_y, const float *__restrict__ dist_z,
+              int nxp, int nyp, int rnxp, int imin, int imax, int jmin, int jmax)
+{
+  const unsigned c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= (unsigned)nxp * (unsigned)nyp) return;
+  const int i = (int)(c % (unsigned)nxp), j = (int)(c / (unsigned)nxp), k = blockIdx.y;
+  if (i >= imin && i <= imax && j >= jmin && j <= jmax) return;
+  const long base = (long)k * nyp * rnxp;
+  const float v = rv[base + (long)ref_j[j] * rnxp + ref_i[i]];
+  const float dx = dist_x[i], dy = dist_y[j], dz = dist_z[k];
+  double t = 1.0 - (double)(dx * dx) - (double)(dy * dy) - (double)(dz * dz);
+  t = t > 0.0 ? t : 0.0;
+  const float mult = (float)(t * t * t);
+  rv[base + (long)j * rnxp + i] = mult * v;
+}
+
+}  // namespace
+
+namespace cg {
+
+int fill_padded(cg_ctx *ctx, float *rv, const float *cube, int nx, int ny, int nz, int nxp, int nyp, int nzp)
+{
+  PadGeom g{nx, ny, nz, nxp, nyp, nzp, 2 * (nxp / 2 + 1), 0.f, 0.f, 0.f};
+  g.sa = 1.0f / (float)(nxp - nx + 1);
+  g.sb = 1.0f / (float)(nyp - ny + 1);
+  g.sc = 1.0f / (float)(nzp - nz + 1);
+  const unsigned pairs = (unsigned)(g.rnxp / 2) * (unsigned)nyp, per_block = kThreads * kPadU;
+  hipEvent_t ev;
+  launch_begin(ctx, CG_K_ELEMENTWISE, &ev);
+  k_fill_padded<<<dim3((pairs + per_block - 1) / per_block, (unsigned)nzp), kThreads, 0, ctx->stream>>>(rv, cube,
+                                                                                                         g);
+  launch_end(ctx, CG_K_ELEMENTWISE, ev);
+  return hip_fail(ctx, hipGetLastError(), "k_fill_padded");
+}
+
+int trace_energy(cg_ctx *ctx, const float *rv, float *energy, int nx, int ny, int nz, int rnxp, int nyp)
+{
+  const int n = nx * ny;
+  hipEvent_t ev;
+  launch_begin(ctx, CG_K_ELEMENTWISE, &ev);
+  k_trace_energy<<<(n + kThreads - 1) / kThreads, kThreads, 0, ctx->stream>>>(rv, energy, nx, ny, nz, rnxp, nyp);
+  launch_end(ctx, CG_K_ELEMENTWISE, ev);
+  return hip_fail(ctx, hipGetLastError(), "k_trace_energy");
+}
+
+int trace_ops(cg_ctx *ctx, float *rv, const unsigned *ops, int nops, bool one_level, int nx, int rnxp, int nyp,
+              int nzp)
+{
+  if (nops <= 0) return CG_OK;
+  hipEvent_t ev;
+  launch_begin(ctx, CG_K_ELEMENTWISE, &ev);
+  if (one_level) {
+    k_trace_ops_level<<<(unsigned)nops, kThreads, 0, ctx->stream>>>(rv, ops, nx, rnxp, nyp, nzp);
+  } else {
+    const int threads = nzp >= 1024 ? 1024 : (nzp + 63) / 64 * 64;
+    k_trace_ops_serial<<<1, threads, 0, ctx->stream>>>(rv, ops, nops, nx, rnxp, nyp, nzp);
+  }
+  launch_end(ctx, CG_K_ELEMENTWISE, ev);
+  return hip_fail(ctx, hipGetLastError(), "k_trace_ops");
+}
+
+int extrapolate(cg_ctx *ctx, float *rv, const int *ref_i, const float *dist_x, const int *ref_j, const float *dist_y,
+                const float *dist_z, int nxp, int nyp, int nzp, int imin, int imax, int jmin, int jmax)
+{
+  const unsigned cells = (unsigned)nxp * (unsigned)nyp;
+  hipEvent_t ev;
+  launch_begin(ctx, CG_K_ELEMENTWISE, &ev);
+  k_extrapolate<<<dim3((cells + kThreads - 1) / kThreads, (unsigned)nzp), kThreads, 0, ctx->stream>>>(
+      rv, ref_i, dist_x, ref_j, dist_y, dist_z, nxp, nyp, 2 * (nxp / 2 + 1), imin, imax, jmin, jmax);
+  launch_end(ctx, CG_K_ELEMENTWISE, ev);
+  return hip_fail(ctx, hipGetLastError(), "k_extrapolate");
+}
+
+}  // namespace cg

@@ -146,6 +146,28 @@ int cg_grid_multiply_by_scalar(cg_grid *grid, float scalar);
  * x < nxp, RMISSING (-99999) on the row padding. */
 int cg_grid_fill_constant(cg_grid *grid, float value);
 
+/* ------------------------------------------------------------------ padding fills (input side) */
+/* FFTGrid(const NRLib::Grid<float> *grid, nxp, nyp, nzp) (fftgrid.cpp:125-213), into an
+ * existing grid of those sizes: the unpadded cube (nz*ny*nx floats, i fastest as NRLib::Grid
+ * stores it, grid.hpp:231-236) is copied in and the padding filled by linear interpolation
+ * between the box edges (x first, then y from x-padded values, then z from both), the row
+ * padding x >= nxp with 0.  cube_on_device: 1 if `cube` is device memory, 0 if host memory
+ * (copied in; the call then returns after the fill).  Sets type PARAMETER, real domain. */
+int cg_grid_fill_padded(cg_grid *grid, const float *cube, int cube_on_device);
+/* FFTGrid::interpolateSeismic (fftgrid.cpp:2349-2437) on a DATA grid in the real domain: box
+ * traces whose energy over the nz box samples is at most energy_threshold * mean energy are
+ * replaced by the mean of their good 4-neighbours (interpolateTrace, fftgrid.cpp:2440-2481, in
+ * the reference's order), then the padding is extrapolated (extrapolateSeismic below) from the
+ * rectangle of good traces.  n_bad and rect = {imin, imax, jmin, jmax} may be null.  CG_EINVAL
+ * when every trace is below the limit (the reference's sweep then reads cells it overwrites). */
+int cg_grid_interpolate_seismic(cg_grid *grid, float energy_threshold, int *n_bad, int rect[4]);
+/* FFTGrid::extrapolateSeismic (fftgrid.cpp:2484-2518): every column outside the box rectangle
+ * [imin, imax] x [jmin, jmax] (real domain, x < nxp) becomes the column of its clamped fill
+ * number (getFillNumber, fftgrid.cpp:1103-1140) times max(1 - dx^2 - dy^2 - dz^2, 0)^3, d the
+ * distances into the padding (getDistToBoundary, fftgrid.cpp:1157-1200).  The rectangle must be
+ * non-empty and inside the box (CG_EINVAL otherwise). */
+int cg_grid_extrapolate_seismic(cg_grid *grid, int imin, int imax, int jmin, int jmax);
+
 /* ------------------------------------------------------------------ posterior (AVOInversion) */
 #define CG_PRIOR_GRIDS 0     /* six prior covariance spectra, as SeismicParametersHolder holds */
 #define CG_PRIOR_SEPARABLE 1 /* Sigma_m(k) = |rxy(kx,ky) * rt(kz)| * Sigma0 (no grids read) */

@@ -90,6 +90,14 @@ def lib():
         L.orc_corr_xy.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.c_float] * 5 + [_c_float_p]
         L.orc_circ_corr_t.restype = ctypes.c_int
         L.orc_circ_corr_t.argtypes = [_c_double_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_float_p]
+        L.orc_fill_padded.argtypes = [ctypes.c_int] * 6 + [_c_float_p, _c_float_p]
+        L.orc_fill_number.restype = ctypes.c_int
+        L.orc_fill_number.argtypes = [ctypes.c_int] * 3
+        L.orc_dist_to_boundary.restype = ctypes.c_float
+        L.orc_dist_to_boundary.argtypes = [ctypes.c_int] * 3
+        L.orc_extrapolate_seismic.argtypes = [ctypes.c_int] * 6 + [_c_float_p] + [ctypes.c_int] * 4
+        L.orc_interpolate_seismic.restype = ctypes.c_int
+        L.orc_interpolate_seismic.argtypes = [ctypes.c_int] * 6 + [_c_float_p, ctypes.c_float, _c_int_p]
         L.orc_kz_tables.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, _c_float_p,
                                     _c_float_p, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                     _c_float_p, _c_float_p, _c_float_p, _c_float_p, _c_int_p]
@@ -313,3 +321,40 @@ def circ_corr_t(corr_t, min_int_fq, nzp):
     if err:
         raise ValueError("circular temporal correlation undefined (too few layers)")
     return out
+
+
+def fill_padded(cube, nxp, nyp, nzp):
+    """FFTGrid(const NRLib::Grid<float> *) padding fill: cube (nz, ny, nx) float32 -> padded real
+    grid (nzp, nyp, rnxp)."""
+    c = np.ascontiguousarray(cube, dtype=np.float32)
+    nz, ny, nx = c.shape
+    out = np.zeros((nzp, nyp, 2 * (nxp // 2 + 1)), dtype=np.float32)
+    f = ctypes.c_float
+    lib().orc_fill_padded(nx, ny, nz, nxp, nyp, nzp, _ptr(c, f), _ptr(out, f))
+    return out
+
+
+def fill_number(i, n, np_):
+    return lib().orc_fill_number(i, n, np_)
+
+
+def dist_to_boundary(i, n, np_):
+    return lib().orc_dist_to_boundary(i, n, np_)
+
+
+def extrapolate_seismic(rv, nx, ny, nz, nxp, rect):
+    """FFTGrid::extrapolateSeismic on a copy of the padded real grid rv (nzp, nyp, rnxp)."""
+    out = np.array(rv, dtype=np.float32, order="C", copy=True)
+    nzp, nyp, _ = out.shape
+    lib().orc_extrapolate_seismic(nx, ny, nz, nxp, nyp, nzp, _ptr(out, ctypes.c_float), *rect)
+    return out
+
+
+def interpolate_seismic(rv, nx, ny, nz, nxp, threshold):
+    """FFTGrid::interpolateSeismic on a copy of rv; returns (grid, bad traces, rect)."""
+    out = np.array(rv, dtype=np.float32, order="C", copy=True)
+    nzp, nyp, _ = out.shape
+    rect = np.zeros(4, dtype=np.int32)
+    nbad = lib().orc_interpolate_seismic(nx, ny, nz, nxp, nyp, nzp, _ptr(out, ctypes.c_float), threshold,
+                                         _ptr(rect, ctypes.c_int))
+    return out, nbad, tuple(int(v) for v in rect)

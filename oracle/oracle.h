@@ -24,6 +24,12 @@
  *                                covariance of realisations.
  *   setup (padding, A, Ricker,   closed forms and hand-computed values.
  *   correlations, tables)
+ *   orc_fill_padded              constant cubes, the closed form of the cyclic linear
+ *                                extension of a cube that is a sum of 1D linear profiles.
+ *   orc_fill_number /            the index sequences documented beside getFillNumber and
+ *   orc_dist_to_boundary         getDistToBoundary (fftgrid.cpp:1105-1108, 1160-1163).
+ *   orc_extrapolate_seismic /    the taper's closed form at hand-picked cells, neighbour means of
+ *   orc_interpolate_seismic      isolated bad traces, energies and the rectangle by numpy.
  */
 #ifndef CRAVA_ORACLE_H
 #define CRAVA_ORACLE_H
@@ -109,6 +115,20 @@ void   orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t 
  * cov [6][csize] complex posterior covariance spectra, seed [3][csize] complex in/out. */
 long orc_colour_f32(long csize, const float *cov, float *seed);
 long orc_colour_f64(long csize, const double *cov, double *seed);
+
+/* ---------------------------------------------------------------- padding fills (f2) */
+/* FFTGrid(const NRLib::Grid<float> *, nxp, nyp, nzp) (fftgrid.cpp:125-213): cube [nz][ny][nx]
+ * into the padded real grid rv [nzp][nyp][rnxp], linear edge interpolation in the padding. */
+void  orc_fill_padded(int nx, int ny, int nz, int nxp, int nyp, int nzp, const float *cube, float *rv);
+int   orc_fill_number(int i, int n, int np);        /* fftgrid.cpp:1103-1140 */
+float orc_dist_to_boundary(int i, int n, int np);   /* fftgrid.cpp:1157-1200 */
+/* FFTGrid::extrapolateSeismic (fftgrid.cpp:2484-2518), in place on rv. */
+void  orc_extrapolate_seismic(int nx, int ny, int nz, int nxp, int nyp, int nzp, float *rv, int imin, int imax,
+                              int jmin, int jmax);
+/* FFTGrid::interpolateSeismic (fftgrid.cpp:2349-2437, interpolateTrace 2440-2481), in place;
+ * returns the number of traces below the energy limit, rect = {imin, imax, jmin, jmax}. */
+int   orc_interpolate_seismic(int nx, int ny, int nz, int nxp, int nyp, int nzp, float *rv, float threshold,
+                              int *rect);
 
 /* ---------------------------------------------------------------- setup (a12, a16, a17, f1) */
 double orc_ricker(double t_ms, double peak_hz);                              /* wavelet.cpp:1484-1488 */

@@ -1,0 +1,136 @@
+"""Padding fills of the path's inputs on the device (cg_grid_fill_padded,
+cg_grid_interpolate_seismic, cg_grid_extrapolate_seismic) vs the oracle, bit for bit."""
+import numpy as np
+import pytest
+
+import crava_amd as C
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _grid(ctx, dims, ct=C.PARAMETER):
+    nx, ny, nz, nxp, nyp, nzp = dims
+    return C.FFTGrid(ctx, nx, ny, nz, nxp, nyp, nzp, ct)
+
+
+def _device_cube(ctx, cube):
+    """A scratch grid whose raw memory holds the cube (nz*ny*nx floats) at its start."""
+    nz, ny, nx = cube.shape
+    s = C.FFTGrid(ctx, nx, ny, nz, nx, ny, nz, C.PARAMETER)
+    flat = np.zeros(nz * ny * s.rnxp, dtype=np.float32)
+    flat[:cube.size] = cube.ravel()
+    s.upload_real(flat.reshape(nz, ny, s.rnxp))
+    return s
+
+
+# odd and even nxp, no padding on an axis, one trace wide, one sample deep
+SHAPES = [(7, 5, 6, 12, 8, 10), (9, 4, 5, 9, 6, 8), (6, 6, 3, 10, 6, 3), (1, 3, 4, 4, 5, 7), (33, 1, 20, 45, 1, 24),
+          (20, 17, 1, 25, 18, 2)]
+
+
+@pytest.mark.parametrize("dims", SHAPES)
+@pytest.mark.parametrize("where", ["host", "device"])
+def test_fill_padded_matches_oracle(ctx, dims, where):
+    nx, ny, nz, nxp, nyp, nzp = dims
+    print("fill_padded", dims, where, flush=True)
+    cube = np.random.default_rng(nx * 31 + nz).normal(size=(nz, ny, nx)).astype(np.float32)
+    g = _grid(ctx, dims, C.DATA)
+    g.fillInConstant(7.0)
+    if where == "host":
+        g.fill_padded(cube)
+    else:
+        s = _device_cube(ctx, cube)
+        g.fill_padded(s.device_ptr())
+        ctx.sync()
+        s.close()
+    want = O.fill_padded(cube, nxp, nyp, nzp)
+    got = g.download_real(padded=True)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    assert g.getIsTransformed() == 0
+    g.close()
+
+
+def test_fill_padded_single_cube_size(ctx):
+    """The single-cube workload's background cube, 512 x 512 x 256 into 576 x 576 x 320."""
+    dims = (512, 512, 256, 576, 576, 320)
+    print("fill_padded", dims, flush=True)
+    cube = np.random.default_rng(5).normal(size=(256, 512, 512)).astype(np.float32)
+    g = _grid(ctx, dims)
+    g.fill_padded(cube)
+    got = g.download_real(padded=True)
+    g.close()
+    want = O.fill_padded(cube, 576, 576, 320)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+def _bad_traces(rv, nx, ny, frac, seed, block=None):
+    rng = np.random.default_rng(seed)
+    bad = rng.random((ny, nx)) < frac
+    if block:
+        i0, i1, j0, j1 = block
+        bad[j0:j1, i0:i1] = True
+    rv[:, :ny, :nx][:, bad] = 0.0
+    return int(bad.sum())
+
+
+def _seismic(dims, seed):
+    nx, ny, nz, nxp, nyp, nzp = dims
+    rng = np.random.default_rng(seed)
+    rv = np.zeros((nzp, nyp, 2 * (nxp // 2 + 1)), dtype=np.float32)
+    rv[:, :, :nxp] = rng.normal(size=(nzp, nyp, nxp)).astype(np.float32)
+    return rv
+
+
+@pytest.mark.parametrize("dims,frac,block", [((40, 30, 50, 48, 36, 64), 0.15, (5, 12, 7, 10)),
+                                             ((6, 5, 4, 8, 7, 6), 0.3, None),
+                                             ((33, 1, 20, 45, 1, 24), 0.2, None),
+                                             ((64, 48, 40, 80, 60, 50), 0.05, (0, 64, 0, 3)),
+                                             ((25, 20, 12, 32, 27, 16), 0.6, (20, 25, 0, 20))])
+def test_interpolate_seismic_matches_oracle(ctx, dims, frac, block):
+    """Random bad traces plus blocks of them (a band along an edge, a band of columns): the
+    device's interpolation and extrapolation equal the oracle's sweep bit for bit."""
+    nx, ny, nz, nxp, nyp, nzp = dims
+    print("interpolate_seismic", dims, frac, block, flush=True)
+    rv = _seismic(dims, 11)
+    nbad_set = _bad_traces(rv, nx, ny, frac, 12, block)
+    want, nbad, rect = O.interpolate_seismic(rv, nx, ny, nz, nxp, 0.01)
+    assert nbad >= nbad_set
+    g = _grid(ctx, dims, C.DATA)
+    g.upload_real(rv)
+    got_nbad, got_rect = g.interpolateSeismic(0.01)
+    got = g.download_real(padded=True)
+    g.close()
+    assert (got_nbad, got_rect) == (nbad, rect)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+def test_extrapolate_seismic_inner_rectangle(ctx):
+    dims = (30, 24, 20, 40, 32, 27)
+    nx, ny, nz, nxp, nyp, nzp = dims
+    rv = _seismic(dims, 21)
+    rect = (4, 25, 3, 20)
+    want = O.extrapolate_seismic(rv, nx, ny, nz, nxp, rect)
+    g = _grid(ctx, dims, C.DATA)
+    g.upload_real(rv)
+    g.extrapolateSeismic(*rect)
+    got = g.download_real(padded=True)
+    g.close()
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+def test_padding_fill_errors(ctx):
+    dims = (6, 5, 4, 8, 7, 6)
+    g = _grid(ctx, dims, C.DATA)
+    g.upload_real(np.zeros((6, 7, 10), dtype=np.float32))
+    with pytest.raises(C.CravaError, match="no trace above"):
+        g.interpolateSeismic(0.01)  # every trace has zero energy
+    for rect in ((3, 2, 0, 4), (0, 6, 0, 4), (-1, 3, 0, 4), (0, 5, 2, 1)):
+        with pytest.raises(C.CravaError, match="rectangle"):
+            g.extrapolateSeismic(*rect)
+    g.setType(C.PARAMETER)
+    with pytest.raises(C.CravaError, match="DATA"):
+        g.interpolateSeismic(0.01)
+    with pytest.raises(ValueError):
+        g.fill_padded(np.zeros((4, 5, 5), dtype=np.float32))
+    g.close()

@@ -1,0 +1,124 @@
+"""The oracle's padding fills (FFTGrid's background padding, seismic interpolation and
+extrapolation), pinned by the reference's documented index sequences, closed forms and
+hand-worked trace walks."""
+import numpy as np
+import pytest
+
+import oracle as O
+
+
+def test_fill_number_documented_sequences():
+    # the sequences written beside getFillNumber (fftgrid.cpp:1105-1108)
+    assert [O.fill_number(i, 5, 8) for i in range(8)] == [0, 1, 2, 3, 4, 4, 1, 0]
+    assert [O.fill_number(i, 4, 8) for i in range(8)] == [0, 1, 2, 3, 3, 2, 1, 0]
+    assert [O.fill_number(i, 3, 8) for i in range(8)] == [0, 1, 2, 2, 1, 1, 1, 0]
+    assert O.fill_number(8, 5, 8) == -99999  # beyond np: IMISSING
+
+
+def test_dist_to_boundary_documented_shapes():
+    # fftgrid.cpp:1160-1163: 0 in the box, p in (0, 1) next to either end, r > 1 between
+    for n, shape in ((5, "00000prp"), (4, "0000prrp"), (3, "000prrrp")):
+        d = [O.dist_to_boundary(i, n, 8) for i in range(8)]
+        for v, s in zip(d, shape):
+            assert (v == 0.0) if s == "0" else ((0.0 < v < 1.0) if s == "p" else v > 1.0), (n, d)
+        # distance counts samples past the box end (or to np) in taper lengths min(n, np-n)/2.1
+        taper = np.float32(min(n, 8 - n) / 2.1)
+        assert d[n] == np.float32(1) / taper and d[7] == np.float32(1) / taper
+
+
+@pytest.mark.parametrize("dims", [(7, 5, 6, 12, 8, 10), (9, 4, 5, 9, 6, 8), (6, 6, 3, 10, 6, 3), (1, 3, 4, 4, 5, 7)])
+def test_fill_padded_is_the_cyclic_linear_extension(dims):
+    """For a cube f(i) + g(j) + h(k) every blend uses only edge values, so the padded grid is
+    F(i) + G(j) + H(k), each the cyclic linear extension of its profile: from the last sample
+    towards the first, reached one step past the padded end."""
+    nx, ny, nz, nxp, nyp, nzp = dims
+    rng = np.random.default_rng(3)
+    f, g, h = rng.normal(size=nx), rng.normal(size=ny), rng.normal(size=nz)
+    cube = (f[None, None, :] + g[None, :, None] + h[:, None, None]).astype(np.float32)
+    got = O.fill_padded(cube, nxp, nyp, nzp)
+
+    def ext(p, n, npad):
+        out = np.empty(npad)
+        out[:n] = p
+        for i in range(n, npad):
+            t = (i - n + 1) / (npad - n + 1)
+            out[i] = p[n - 1] + (p[0] - p[n - 1]) * t
+        return out
+
+    want = ext(f, nx, nxp)[None, None, :] + ext(g, ny, nyp)[None, :, None] + ext(h, nz, nzp)[:, None, None]
+    assert np.max(np.abs(got[:, :, :nxp] - want)) < 2e-5 * max(1.0, np.max(np.abs(want)))
+    assert np.all(got[:, :, nxp:] == 0.0)
+    assert np.array_equal(got[:nz, :ny, :nx], cube)
+
+
+def test_fill_padded_constant_cube():
+    cube = np.full((5, 4, 3), 2.5, dtype=np.float32)
+    got = O.fill_padded(cube, 8, 6, 9)
+    assert np.allclose(got[:, :, :8], 2.5, rtol=1e-6, atol=0)
+
+
+def _seismic(nx, ny, nz, nxp, nyp, nzp, seed):
+    rng = np.random.default_rng(seed)
+    rv = np.zeros((nzp, nyp, 2 * (nxp // 2 + 1)), dtype=np.float32)
+    rv[:, :, :nxp] = rng.normal(size=(nzp, nyp, nxp)).astype(np.float32)
+    return rv
+
+
+def test_interpolate_seismic_hand_worked_walk():
+    """6 x 5 traces; bad (zero) traces B0 = (3,0), T1 = (0,3), T2 = (1,3), T3 = (0,4).
+    B0 has three good neighbours.  T1 has one (up), so it waits; T2 then succeeds from its
+    right, up and down neighbours, and the walk back from T2 retries T1 (right = new T2, up).
+    T3 later finds right and up (= new T1) good.  Sums run left, right, up, down from 0."""
+    nx, ny, nz, nxp, nyp, nzp = 6, 5, 4, 8, 7, 6
+    rv = _seismic(nx, ny, nz, nxp, nyp, nzp, 1)
+    for (i, j) in ((3, 0), (0, 3), (1, 3), (0, 4)):
+        rv[:, j, i] = 0.0
+    col = lambda a, i, j: a[:, j, i].astype(np.float32)
+    b0 = ((col(rv, 2, 0) + col(rv, 4, 0)) + col(rv, 3, 1)) / np.float32(3)
+    t2 = ((np.float32(0) + col(rv, 2, 3)) + col(rv, 1, 2) + col(rv, 1, 4)) / np.float32(3)
+    t1 = ((np.float32(0) + t2) + col(rv, 0, 2)) / np.float32(2)
+    t3 = ((np.float32(0) + col(rv, 1, 4)) + t1) / np.float32(2)
+    out, nbad, rect = O.interpolate_seismic(rv, nx, ny, nz, nxp, 0.01)
+    assert nbad == 4 and rect == (0, 5, 0, 4)
+    for (i, j), want in (((3, 0), b0), ((1, 3), t2), ((0, 3), t1), ((0, 4), t3)):
+        assert np.array_equal(out[:, j, i], want), (i, j)
+    # every other box trace is untouched
+    mask = np.ones((ny, nx), bool)
+    for (i, j) in ((3, 0), (0, 3), (1, 3), (0, 4)):
+        mask[j, i] = False
+    assert np.array_equal(out[:, :ny, :nx][:, mask], rv[:, :ny, :nx][:, mask])
+
+
+def test_interpolate_seismic_first_bad_trace_is_not_retried():
+    """The first bad trace carries no 'earlier bad' bit, so the walk back from a later success
+    stops before it: (0,0) with one good neighbour stays as it was."""
+    nx, ny, nz, nxp, nyp, nzp = 5, 4, 3, 6, 5, 4
+    rv = _seismic(nx, ny, nz, nxp, nyp, nzp, 2)
+    rv[:, 0, 0] = 0.0
+    rv[:, 0, 1] = 0.0
+    rv[:, 1, 0] = 0.0
+    out, nbad, rect = O.interpolate_seismic(rv, nx, ny, nz, nxp, 0.01)
+    assert nbad == 3
+    assert np.all(out[:, 0, 0] == 0.0)
+    want01 = ((np.float32(0) + rv[:, 0, 2]) + rv[:, 1, 1]) / np.float32(2)
+    assert np.array_equal(out[:, 0, 1], want01)
+
+
+def test_extrapolate_seismic_taper_closed_form():
+    nx, ny, nz, nxp, nyp, nzp = 6, 5, 4, 10, 8, 7
+    rv = _seismic(nx, ny, nz, nxp, nyp, nzp, 3)
+    out = O.extrapolate_seismic(rv, nx, ny, nz, nxp, (0, nx - 1, 0, ny - 1))
+    tx, ty, tz = (np.float32(min(n, p - n) / 2.1) for n, p in ((nx, nxp), (ny, nyp), (nz, nzp)))
+    # x padding next to the box end: the last box column times (1 - (1/tx)^2)^3
+    for j, k in ((0, 0), (3, 2)):
+        m = (1.0 - float(np.float32(1) / tx) ** 2) ** 3
+        assert abs(out[k, j, nx] - rv[k, j, nx - 1] * m) < 1e-6 * (1 + abs(rv[k, j, nx - 1]))
+    # corner of all three paddings at (nx+1, ny, nz): taper reaches zero when the sum exceeds 1
+    d = [float(np.float32(2) / tx), float(np.float32(1) / ty), float(np.float32(1) / tz)]
+    m = max(1.0 - sum(x * x for x in d), 0.0) ** 3
+    ri, rj = O.fill_number(nx + 1, nx, nxp), O.fill_number(ny, ny, nyp)
+    assert abs(out[nz, ny, nx + 1] - rv[nz, rj, ri] * m) < 1e-6 * (1 + abs(rv[nz, rj, ri]))
+    # columns inside the rectangle keep every sample, their z padding included
+    assert np.array_equal(out[:, :ny, :nx], rv[:, :ny, :nx])
+    # the row padding x >= nxp is not touched
+    assert np.array_equal(out[:, :, nxp:], rv[:, :, nxp:])
