@@ -239,6 +239,19 @@ def bench_input_padding(ctx, inv, w, reps=5):
     kn, kms = ctx.kernel_time(C.K_ELEMENTWISE)
     ctx.set_profiling(False)
     seis.setType(C.PARAMETER)
+    # one grid through a CRAVA binary file and back (byte order swapped on the device)
+    import tempfile
+    geo = dict(x0=0.0, y0=0.0, dx=w["dx"], dy=w["dy"], nx=nx, ny=ny, il0=1.0, xl0=1.0, il_step_x=1.0,
+               il_step_y=0.0, xl_step_x=0.0, xl_step_y=1.0, angle=0.0)
+    with tempfile.TemporaryDirectory() as tmp:
+        name = os.path.join(tmp, "grid")
+        t = time.perf_counter()
+        dst.writeCravaFile(name, geo)
+        t_write = time.perf_counter() - t
+        t = time.perf_counter()
+        src.readCravaFile(name + ".crava")
+        t_read = time.perf_counter() - t
+    file_bytes = 4 * dst.rnxp * w["nyp"] * w["nzp"]
     return {
         "fill_padded": {"what": "FFTGrid(NRLib::Grid) background fill, %dx%dx%d cube into %dx%dx%d"
                                 % (nx, ny, nz, w["nxp"], w["nyp"], w["nzp"]),
@@ -247,6 +260,10 @@ def bench_input_padding(ctx, inv, w, reps=5):
         "interpolate_seismic": {"what": "FFTGrid::interpolateSeismic, 5%% of %d traces zeroed" % (nx * ny),
                                 "bad_traces": nbad, "ms_synchronised": round(wall, 2),
                                 "kernel_ms": round(kms, 3), "launches": kn},
+        "crava_file": {"what": "FFTGrid::writeCravaFile then readCravaFile of one grid (%.0f MB), local disk"
+                               % (file_bytes / 1e6),
+                       "write_ms": round(1000 * t_write, 1), "read_ms": round(1000 * t_read, 1),
+                       "write_gbps": file_bytes / t_write / 1e9, "read_gbps": file_bytes / t_read / 1e9},
     }
 
 

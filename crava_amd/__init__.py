@@ -24,7 +24,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libcravagpu.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "cravagpu.h")
 
-CG_OK, CG_EINVAL, CG_ENOMEM, CG_EHIP, CG_ESTATE = 0, -1, -2, -3, -4
+CG_OK, CG_EINVAL, CG_ENOMEM, CG_EHIP, CG_ESTATE, CG_EIO = 0, -1, -2, -3, -4, -5
 DATA, PARAMETER, COVARIANCE = 1, 2, 3
 PRIOR_GRIDS, PRIOR_SEPARABLE = 0, 1
 PREC_REFERENCE, PREC_MIXED = 0, 1
@@ -101,6 +101,8 @@ def lib():
             "cg_grid_fill_padded": [_vp, _vp, i],
             "cg_grid_interpolate_seismic": [_vp, f, _i_p, _i_p],
             "cg_grid_extrapolate_seismic": [_vp, i, i, i, i],
+            "cg_grid_write_crava": [_vp, ctypes.c_char_p, ctypes.POINTER(CravaGeometry)],
+            "cg_grid_read_crava": [_vp, ctypes.c_char_p, ctypes.POINTER(CravaGeometry)],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -116,6 +118,17 @@ def lib():
 
 class CravaError(RuntimeError):
     pass
+
+
+class CravaGeometry(ctypes.Structure):
+    """cg_crava_geometry: the survey geometry in a CRAVA binary grid file's header."""
+    _fields_ = [("x0", ctypes.c_double), ("y0", ctypes.c_double), ("dx", ctypes.c_double), ("dy", ctypes.c_double),
+                ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("il0", ctypes.c_double), ("xl0", ctypes.c_double),
+                ("il_step_x", ctypes.c_double), ("il_step_y", ctypes.c_double), ("xl_step_x", ctypes.c_double),
+                ("xl_step_y", ctypes.c_double), ("angle", ctypes.c_double)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
 
 
 def _ptr(a, ctype):
@@ -309,6 +322,19 @@ class FFTGrid:
     def extrapolateSeismic(self, imin, imax, jmin, jmax):
         """FFTGrid::extrapolateSeismic with the rectangle of good box traces."""
         self.ctx.check(lib().cg_grid_extrapolate_seismic(self._h, imin, imax, jmin, jmax), "extrapolateSeismic")
+
+    def writeCravaFile(self, file_name, geometry, suffix=".crava"):
+        """FFTGrid::writeCravaFile: file_name + suffix (IO::SuffixCrava); geometry: a dict of the
+        CravaGeometry fields."""
+        geo = CravaGeometry(**geometry)
+        self.ctx.check(lib().cg_grid_write_crava(self._h, (file_name + suffix).encode(), ctypes.byref(geo)),
+                       "writeCravaFile")
+
+    def readCravaFile(self, file_name):
+        """FFTGrid::readCravaFile; returns the header's geometry as a dict."""
+        geo = CravaGeometry()
+        self.ctx.check(lib().cg_grid_read_crava(self._h, file_name.encode(), ctypes.byref(geo)), "readCravaFile")
+        return geo.as_dict()
 
     def fillInComplexNoiseReference(self, seed):
         """FFTGrid::fillInComplexNoise with the reference's RandomGen stream from state `seed`;

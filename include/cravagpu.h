@@ -34,6 +34,7 @@ extern "C" {
 #define CG_ENOMEM (-2)   /* device allocation failed or HBM budget exceeded */
 #define CG_EHIP (-3)     /* HIP runtime error */
 #define CG_ESTATE (-4)   /* operation not valid in the grid's current domain */
+#define CG_EIO (-5)      /* a file could not be opened, read or written */
 
 /* FFTGrid cube types (fftgrid.h:155); the value selects FFT scaling. */
 #define CG_CT_DATA 1
@@ -167,6 +168,24 @@ int cg_grid_interpolate_seismic(cg_grid *grid, float energy_threshold, int *n_ba
  * distances into the padding (getDistToBoundary, fftgrid.cpp:1157-1200).  The rectangle must be
  * non-empty and inside the box (CG_EINVAL otherwise). */
 int cg_grid_extrapolate_seismic(cg_grid *grid, int imin, int imax, int jmin, int jmax);
+
+/* ------------------------------------------------------------------ CRAVA binary grid files */
+/* Survey geometry stored in the file header (Simbox getters written by writeCravaFile). */
+typedef struct {
+  double x0, y0, dx, dy;
+  int nx, ny;
+  double il0, xl0, il_step_x, il_step_y, xl_step_x, xl_step_y, angle;
+} cg_crava_geometry;
+/* FFTGrid::writeCravaFile (fftgrid.cpp:2183-2221) to `path` as given (the reference appends
+ * IO::SuffixCrava()): the tag line "crava_fftgrid_binary", the geometry, rnxp, nyp, nzp and the
+ * grid's rnxp*nyp*nzp floats as stored (either domain), all big-endian.  CG_EIO if the file
+ * cannot be written. */
+int cg_grid_write_crava(cg_grid *grid, const char *path, const cg_crava_geometry *geometry);
+/* FFTGrid::readCravaFile (fftgrid.cpp:2224-2271): the file's floats into the grid, which must have
+ * the file's rnxp, nyp and nzp (CG_EINVAL, "Grid dimension is wrong", otherwise); the tag line is
+ * not checked.  Sets the real domain; geometry (may be null) receives the header.  CG_EIO if the
+ * file cannot be opened or ends early (the grid's contents are then undefined). */
+int cg_grid_read_crava(cg_grid *grid, const char *path, cg_crava_geometry *geometry);
 
 /* ------------------------------------------------------------------ posterior (AVOInversion) */
 #define CG_PRIOR_GRIDS 0     /* six prior covariance spectra, as SeismicParametersHolder holds */

@@ -358,3 +358,35 @@ def interpolate_seismic(rv, nx, ny, nz, nxp, threshold):
     nbad = lib().orc_interpolate_seismic(nx, ny, nz, nxp, nyp, nzp, _ptr(out, ctypes.c_float), threshold,
                                          _ptr(rect, ctypes.c_int))
     return out, nbad, tuple(int(v) for v in rect)
+
+
+CRAVA_TAG = b"crava_fftgrid_binary"
+CRAVA_GEOMETRY = (("x0", "d"), ("y0", "d"), ("dx", "d"), ("dy", "d"), ("nx", "i"), ("ny", "i"), ("il0", "d"),
+                  ("xl0", "d"), ("il_step_x", "d"), ("il_step_y", "d"), ("xl_step_x", "d"), ("xl_step_y", "d"),
+                  ("angle", "d"))
+
+
+def write_crava(path, rv, geometry):
+    """FFTGrid::writeCravaFile (fftgrid.cpp:2183-2221): tag line, geometry, rnxp, nyp, nzp, then the
+    padded real grid rv (nzp, nyp, rnxp), everything big-endian (NRLib's default)."""
+    import struct
+    nzp, nyp, rnxp = rv.shape
+    with open(path, "wb") as f:
+        f.write(CRAVA_TAG + b"\n")
+        for name, code in CRAVA_GEOMETRY:
+            f.write(struct.pack(">" + code, geometry[name]))
+        f.write(struct.pack(">iii", rnxp, nyp, nzp))
+        f.write(np.ascontiguousarray(rv, dtype=">f4").tobytes())
+
+
+def read_crava(path):
+    """FFTGrid::readCravaFile (fftgrid.cpp:2224-2271): returns (geometry, rv (nzp, nyp, rnxp))."""
+    import struct
+    with open(path, "rb") as f:
+        f.readline()
+        geometry = {}
+        for name, code in CRAVA_GEOMETRY:
+            geometry[name] = struct.unpack(">" + code, f.read(struct.calcsize(code)))[0]
+        rnxp, nyp, nzp = struct.unpack(">iii", f.read(12))
+        rv = np.frombuffer(f.read(4 * rnxp * nyp * nzp), dtype=">f4").astype(np.float32)
+    return geometry, rv.reshape(nzp, nyp, rnxp)

@@ -1,5 +1,6 @@
-"""Padding fills of the path's inputs on the device (cg_grid_fill_padded,
-cg_grid_interpolate_seismic, cg_grid_extrapolate_seismic) vs the oracle, bit for bit."""
+"""The path's inputs on the device: padding fills (cg_grid_fill_padded,
+cg_grid_interpolate_seismic, cg_grid_extrapolate_seismic) and CRAVA binary grid files
+(cg_grid_write_crava, cg_grid_read_crava) vs the oracle, bit for bit."""
 import numpy as np
 import pytest
 
@@ -133,4 +134,50 @@ def test_padding_fill_errors(ctx):
         g.interpolateSeismic(0.01)
     with pytest.raises(ValueError):
         g.fill_padded(np.zeros((4, 5, 5), dtype=np.float32))
+    g.close()
+
+
+GEO = dict(x0=451000.5, y0=6780000.25, dx=25.0, dy=12.5, nx=30, ny=24, il0=1001.0, xl0=2002.0, il_step_x=0.5,
+           il_step_y=-0.25, xl_step_x=0.125, xl_step_y=1.5, angle=0.3)
+
+
+@pytest.mark.parametrize("dims", [(30, 24, 20, 40, 32, 27), (9, 7, 5, 9, 7, 5)])
+def test_crava_file_matches_oracle(ctx, tmp_path, dims):
+    """writeCravaFile from the device gives the oracle's bytes; readCravaFile of the oracle's file
+    restores the grid bit for bit and returns the geometry."""
+    nx, ny, nz, nxp, nyp, nzp = dims
+    print("crava file", dims, flush=True)
+    rv = _seismic(dims, 31)
+    g = _grid(ctx, dims)
+    g.upload_real(rv)
+    g.writeCravaFile(str(tmp_path / "dev"), GEO)
+    O.write_crava(str(tmp_path / "orc.crava"), rv, GEO)
+    assert open(tmp_path / "dev.crava", "rb").read() == open(tmp_path / "orc.crava", "rb").read()
+    h = _grid(ctx, dims)
+    geo = h.readCravaFile(str(tmp_path / "orc.crava"))
+    assert geo == GEO
+    got = h.download_real(padded=True)
+    assert np.array_equal(got.view(np.uint32), rv.view(np.uint32))
+    g.close()
+    h.close()
+
+
+def test_crava_file_errors(ctx, tmp_path):
+    dims = (9, 7, 5, 9, 7, 5)
+    rv = _seismic(dims, 32)
+    O.write_crava(str(tmp_path / "a.crava"), rv, GEO)
+    other = _grid(ctx, (9, 7, 5, 10, 7, 5))
+    with pytest.raises(C.CravaError, match="Grid dimension is wrong"):
+        other.readCravaFile(str(tmp_path / "a.crava"))
+    g = _grid(ctx, dims)
+    with pytest.raises(C.CravaError, match="cannot open"):
+        g.readCravaFile(str(tmp_path / "missing.crava"))
+    raw = open(tmp_path / "a.crava", "rb").read()
+    open(tmp_path / "short.crava", "wb").write(raw[:-4])
+    with pytest.raises(C.CravaError, match="ends inside the grid"):
+        g.readCravaFile(str(tmp_path / "short.crava"))
+    open(tmp_path / "head.crava", "wb").write(raw[:60])
+    with pytest.raises(C.CravaError, match="ends inside the header"):
+        g.readCravaFile(str(tmp_path / "head.crava"))
+    other.close()
     g.close()

@@ -122,3 +122,31 @@ def test_extrapolate_seismic_taper_closed_form():
     assert np.array_equal(out[:, :ny, :nx], rv[:, :ny, :nx])
     # the row padding x >= nxp is not touched
     assert np.array_equal(out[:, :, nxp:], rv[:, :, nxp:])
+
+
+GEO = dict(x0=451000.5, y0=6780000.25, dx=25.0, dy=12.5, nx=7, ny=5, il0=1001.0, xl0=2002.0, il_step_x=0.5,
+           il_step_y=-0.25, xl_step_x=0.125, xl_step_y=1.5, angle=0.3)
+
+
+def test_crava_file_layout(tmp_path):
+    """The oracle's CRAVA binary file against its layout: the tag line, 11 doubles and 5 ints of
+    header (108 bytes), then the floats big-endian -- decoded here with numpy's '>f8', '>i4' and
+    '>f4' types at fixed offsets."""
+    rv = np.random.default_rng(1).normal(size=(6, 5, 10)).astype(np.float32)
+    path = str(tmp_path / "g.crava")
+    O.write_crava(path, rv, GEO)
+    raw = open(path, "rb").read()
+    tag = b"crava_fftgrid_binary\n"
+    assert raw.startswith(tag) and len(raw) == len(tag) + 108 + 4 * rv.size
+    h = len(tag)
+    d1 = np.frombuffer(raw, dtype=">f8", count=4, offset=h)
+    ij = np.frombuffer(raw, dtype=">i4", count=2, offset=h + 32)
+    d2 = np.frombuffer(raw, dtype=">f8", count=7, offset=h + 40)
+    dims = np.frombuffer(raw, dtype=">i4", count=3, offset=h + 96)
+    assert list(d1) == [GEO["x0"], GEO["y0"], GEO["dx"], GEO["dy"]] and list(ij) == [7, 5]
+    assert list(d2) == [GEO[k] for k in ("il0", "xl0", "il_step_x", "il_step_y", "xl_step_x", "xl_step_y", "angle")]
+    assert list(dims) == [10, 5, 6]  # rnxp, nyp, nzp
+    body = np.frombuffer(raw, dtype=">f4", offset=h + 108).reshape(rv.shape)
+    assert np.array_equal(body.astype(np.float32), rv)
+    geo, back = O.read_crava(path)
+    assert geo == GEO and np.array_equal(back, rv)
