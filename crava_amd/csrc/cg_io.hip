@@ -13,7 +13,8 @@
 
 #include "cg_internal.h"
 
-namespace {
+// named (not anonymous) so that profilers show the kernel's name
+namespace cgio {
 
 constexpr size_t kChunk = 64u << 20;  // bytes per file read / write
 const char kTag[] = "crava_fftgrid_binary";
@@ -76,7 +77,9 @@ struct File {
   }
 };
 
-}  // namespace
+}  // namespace cgio
+
+using namespace cgio;
 
 extern "C" {
 

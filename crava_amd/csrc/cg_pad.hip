@@ -11,7 +11,8 @@
 
 #pragma clang fp contract(off)
 
-namespace {
+// named (not anonymous) so that profilers show the kernels' names
+namespace cgpad {
 
 struct PadGeom {
   int nx, ny, nz, nxp, nyp, nzp, rnxp;
@@ -20,57 +21,75 @@ struct PadGeom {
 
 __device__ __forceinline__ float blend(float lo, float wlo, float hi, float whi) { return lo * wlo + hi * whi; }
 
-// cube value, or its x-padding blend (j < ny, k < nz, i < nxp)
-__device__ __forceinline__ float pad_x(const PadGeom &g, const float *__restrict__ cube, int i, int j, int k)
-{
-  const float *row = cube + ((long)k * g.ny + j) * g.nx;
-  if (i < g.nx) return row[i];
-  return blend(row[0], g.sa * (float)(i - g.nx + 1), row[g.nx - 1], g.sa * (float)(g.nxp - i));
-}
+// The fill runs as four launches over the row classes (box rows, y-padding rows, z-padding
+// rows, rows in both paddings), so each is branch-free.  A workgroup takes kRows rows of one
+// plane; their source cube rows (row j, or rows 0 and ny - 1 in the y padding, of plane k, or
+// planes 0 and nz - 1 in the z padding) and those rows' edge samples are fixed before the cell
+// loop, so per cell the x blend is arithmetic and the in-row sample (clamped index) an
+// unconditional load: a thread's loads for its cell pair (x even, x + 1; rnxp is even) of every
+// row issue before the first blend.  Blends nest x, then y, then z, as the reference's sweep
+// composes them.
+constexpr int kRows = 4;
 
-// ... or its y-padding blend of x-padded values (k < nz, i < nxp)
-__device__ __forceinline__ float pad_xy(const PadGeom &g, const float *__restrict__ cube, int i, int j, int k)
+template <bool YPAD, bool ZPAD>
+__global__ void __launch_bounds__(1024)
+k_fill_padded(float *__restrict__ rv, const float *__restrict__ cube, PadGeom g, int j_begin, int j_end, int k_begin)
 {
-  if (j < g.ny) return pad_x(g, cube, i, j, k);
-  return blend(pad_x(g, cube, i, 0, k), g.sb * (float)(j - g.ny + 1), pad_x(g, cube, i, g.ny - 1, k),
-               g.sb * (float)(g.nyp - j));
-}
-
-// The padded grid's value at (i, j, k).
-__device__ __forceinline__ float pad_cell(const PadGeom &g, const float *__restrict__ cube, int i, int j, int k)
-{
-  if (i >= g.nxp) return 0.f;  // the row padding of the in-place layout
-  if (k < g.nz) return pad_xy(g, cube, i, j, k);
-  return blend(pad_xy(g, cube, i, j, 0), g.sc * (float)(k - g.nz + 1), pad_xy(g, cube, i, j, g.nz - 1),
-               g.sc * (float)(g.nzp - k));
-}
-
-// Cell pairs (x even, x + 1: rnxp is even, so a pair never leaves its row) stored as float2;
-// each thread computes kPadU pairs a block-width apart before storing any, so that enough
-// loads are in flight to cover HBM latency.  grid.x over a plane's pairs / (kThreads kPadU),
-// grid.y = k.
-constexpr int kPadU = 4;
-
-__global__ void __launch_bounds__(cg::kThreads)
-k_fill_padded(float *__restrict__ rv, const float *__restrict__ cube, PadGeom g)
-{
-  const unsigned pairs = (unsigned)(g.rnxp / 2) * (unsigned)g.nyp;
-  const int k = blockIdx.y;
-  float2 v[kPadU];
+  constexpr int NS = (YPAD ? 2 : 1) * (ZPAD ? 2 : 1);  // source rows per output row
+  const int k = k_begin + blockIdx.y;
+  const float wz0 = g.sc * (float)(k - g.nz + 1), wz1 = g.sc * (float)(g.nzp - k);
+  const long nx = g.nx;
+  const float *src[kRows][NS];
+  float lo[kRows][NS], hi[kRows][NS], wy0[kRows], wy1[kRows];
 #pragma unroll
-  for (int u = 0; u < kPadU; ++u) {
-    const unsigned p = (blockIdx.x * kPadU + u) * cg::kThreads + threadIdx.x;
-    if (p < pairs) {
-      const unsigned c = 2 * p;
-      const int i = (int)(c % (unsigned)g.rnxp), j = (int)(c / (unsigned)g.rnxp);
-      v[u] = make_float2(pad_cell(g, cube, i, j, k), pad_cell(g, cube, i + 1, j, k));
+  for (int r = 0; r < kRows; ++r) {
+    const int j = min(j_begin + (int)blockIdx.x * kRows + r, j_end - 1);  // clamped rows are not stored
+    wy0[r] = g.sb * (float)(j - g.ny + 1);
+    wy1[r] = g.sb * (float)(g.nyp - j);
+#pragma unroll
+    for (int q = 0; q < NS; ++q) {
+      // q: bit 0 selects row ny - 1 (y padding), the next bit plane nz - 1 (z padding)
+      const int jq = YPAD ? ((q & 1) ? g.ny - 1 : 0) : j;
+      const int kq = ZPAD ? ((q >> (YPAD ? 1 : 0)) & 1 ? g.nz - 1 : 0) : k;
+      src[r][q] = cube + ((long)kq * g.ny + jq) * nx;
+      lo[r][q] = src[r][q][0];
+      hi[r][q] = src[r][q][nx - 1];
     }
   }
-  float2 *plane = reinterpret_cast<float2 *>(rv + (long)k * g.nyp * g.rnxp);
+  const int pairs = g.rnxp / 2;
+  for (int p = threadIdx.x; p < pairs; p += blockDim.x) {
+    float2 v[kRows];
 #pragma unroll
-  for (int u = 0; u < kPadU; ++u) {
-    const unsigned p = (blockIdx.x * kPadU + u) * cg::kThreads + threadIdx.x;
-    if (p < pairs) plane[p] = v[u];
+    for (int r = 0; r < kRows; ++r) {
+      float c[2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int i = 2 * p + h;
+        const bool in = i < g.nx;
+        const int ic = in ? i : g.nx - 1;
+        const float wx0 = g.sa * (float)(i - g.nx + 1), wx1 = g.sa * (float)(g.nxp - i);
+        float x[NS];
+#pragma unroll
+        for (int q = 0; q < NS; ++q) {
+          const float s = src[r][q][ic];
+          x[q] = in ? s : blend(lo[r][q], wx0, hi[r][q], wx1);
+        }
+        float val;
+        if (YPAD) {
+          val = blend(x[0], wy0[r], x[1], wy1[r]);
+          if (ZPAD) val = blend(val, wz0, blend(x[2], wy0[r], x[3], wy1[r]), wz1);
+        } else {
+          val = ZPAD ? blend(x[0], wz0, x[1], wz1) : x[0];
+        }
+        c[h] = i < g.nxp ? val : 0.f;  // the row padding of the in-place layout
+      }
+      v[r] = make_float2(c[0], c[1]);
+    }
+#pragma unroll
+    for (int r = 0; r < kRows; ++r) {
+      const int j = j_begin + (int)blockIdx.x * kRows + r;
+      if (j < j_end) reinterpret_cast<float2 *>(rv + ((long)k * g.nyp + j) * g.rnxp)[p] = v[r];
+    }
   }
 }
 
@@ -150,9 +169,11 @@ k_extrapolate(float *__restrict__ rv, const int *__restrict__ ref_i, const float
   rv[base + (long)j * rnxp + i] = mult * v;
 }
 
-}  // namespace
+}  // namespace cgpad
 
 namespace cg {
+
+using namespace cgpad;
 
 int fill_padded(cg_ctx *ctx, float *rv, const float *cube, int nx, int ny, int nz, int nxp, int nyp, int nzp)
 {
@@ -160,11 +181,18 @@ int fill_padded(cg_ctx *ctx, float *rv, const float *cube, int nx, int ny, int n
   g.sa = 1.0f / (float)(nxp - nx + 1);
   g.sb = 1.0f / (float)(nyp - ny + 1);
   g.sc = 1.0f / (float)(nzp - nz + 1);
-  const unsigned pairs = (unsigned)(g.rnxp / 2) * (unsigned)nyp, per_block = kThreads * kPadU;
+  const int pairs = g.rnxp / 2, threads = pairs >= 1024 ? 1024 : (pairs + 63) / 64 * 64;
+  auto grid = [](int j0, int j1, int k0, int k1) {
+    return dim3((unsigned)((j1 - j0 + kRows - 1) / kRows), (unsigned)(k1 - k0));
+  };
   hipEvent_t ev;
   launch_begin(ctx, CG_K_ELEMENTWISE, &ev);
-  k_fill_padded<<<dim3((pairs + per_block - 1) / per_block, (unsigned)nzp), kThreads, 0, ctx->stream>>>(rv, cube,
-                                                                                                         g);
+  hipStream_t s = ctx->stream;
+  k_fill_padded<false, false><<<grid(0, ny, 0, nz), threads, 0, s>>>(rv, cube, g, 0, ny, 0);
+  if (nyp > ny) k_fill_padded<true, false><<<grid(ny, nyp, 0, nz), threads, 0, s>>>(rv, cube, g, ny, nyp, 0);
+  if (nzp > nz) k_fill_padded<false, true><<<grid(0, ny, nz, nzp), threads, 0, s>>>(rv, cube, g, 0, ny, nz);
+  if (nyp > ny && nzp > nz)
+    k_fill_padded<true, true><<<grid(ny, nyp, nz, nzp), threads, 0, s>>>(rv, cube, g, ny, nyp, nz);
   launch_end(ctx, CG_K_ELEMENTWISE, ev);
   return hip_fail(ctx, hipGetLastError(), "k_fill_padded");
 }
