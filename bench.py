@@ -256,6 +256,21 @@ def bench_input_padding(ctx, inv, w, reps=5):
         src.readCravaFile(name + ".crava")
         t_read = time.perf_counter() - t
     file_bytes = 4 * dst.rnxp * w["nyp"] * w["nzp"]
+    # prior covariance grid fill (fillInParamCorr + multiplyByScalar) with a correlation gradient
+    rng = np.random.default_rng(4)
+    cxy = rng.random((w["nyp"], w["nxp"])).astype(np.float32)
+    ct = rng.random(w["nzp"]).astype(np.float32)
+    corr_us = {}
+    for grad in ((0.0, 0.0), (0.1, 0.05)):
+        dst.fillInParamCorr(cxy, ct, grad[0], grad[1], 0.01)  # warm-up
+        ctx.sync()
+        ctx.set_profiling(True)
+        ctx.reset_kernel_time()
+        for _ in range(reps):
+            dst.fillInParamCorr(cxy, ct, grad[0], grad[1], 0.01)
+        pn, pms = ctx.kernel_time(C.K_ELEMENTWISE)
+        ctx.set_profiling(False)
+        corr_us[grad] = 1000.0 * pms / pn
     return {
         "fill_padded": {"what": "FFTGrid(NRLib::Grid) background fill, %dx%dx%d cube into %dx%dx%d"
                                 % (nx, ny, nz, w["nxp"], w["nyp"], w["nzp"]),
@@ -264,6 +279,12 @@ def bench_input_padding(ctx, inv, w, reps=5):
         "interpolate_seismic": {"what": "FFTGrid::interpolateSeismic, 5%% of %d traces zeroed" % (nx * ny),
                                 "bad_traces": nbad, "ms_synchronised": round(wall, 2),
                                 "kernel_ms": round(kms, 3), "launches": kn},
+        "fill_param_corr": {"what": "FFTGrid::fillInParamCorr + multiplyByScalar, %dx%dx%d (one grid written)"
+                                    % (w["nxp"], w["nyp"], w["nzp"]), "bytes_per_launch": file_bytes,
+                            "no_gradient": {"avg_launch_us": round(corr_us[(0.0, 0.0)], 2),
+                                            "gbps": round(file_bytes / corr_us[(0.0, 0.0)] / 1e3, 1)},
+                            "gradients_0.1_0.05": {"avg_launch_us": round(corr_us[(0.1, 0.05)], 2),
+                                                   "gbps": round(file_bytes / corr_us[(0.1, 0.05)] / 1e3, 1)}},
         "crava_file": {"what": "FFTGrid::writeCravaFile then readCravaFile of one grid (%.0f MB), local disk"
                                % (file_bytes / 1e6),
                        "write_ms": round(1000 * t_write, 1), "read_ms": round(1000 * t_read, 1),

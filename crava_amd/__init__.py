@@ -101,6 +101,7 @@ def lib():
             "cg_grid_fill_padded": [_vp, _vp, i],
             "cg_grid_interpolate_seismic": [_vp, f, _i_p, _i_p],
             "cg_grid_extrapolate_seismic": [_vp, i, i, i, i],
+            "cg_grid_fill_param_corr": [_vp, _f_p, _f_p, f, f, f], "cg_grid_fill_err_corr": [_vp, _f_p, f, f],
             "cg_grid_write_crava": [_vp, ctypes.c_char_p, ctypes.POINTER(CravaGeometry)],
             "cg_grid_read_crava": [_vp, ctypes.c_char_p, ctypes.POINTER(CravaGeometry)],
         }
@@ -311,6 +312,24 @@ class FFTGrid:
         if a.shape != (self.nz, self.ny, self.nx):
             raise ValueError("cube must be (nz, ny, nx) = %s" % ((self.nz, self.ny, self.nx),))
         self.ctx.check(lib().cg_grid_fill_padded(self._h, a.ctypes.data_as(_vp), 0), "fill_padded")
+
+    def fillInParamCorr(self, corr_xy, corr_t, grad_i=0.0, grad_j=0.0, scale=1.0):
+        """FFTGrid::fillInParamCorr / FillInLateralCorr, then multiplyByScalar(scale): corr_xy
+        (nyp, nxp), corr_t (nzp,)."""
+        cxy = np.ascontiguousarray(corr_xy, dtype=np.float32)
+        ct = np.ascontiguousarray(corr_t, dtype=np.float32)
+        if cxy.shape != (self.nyp, self.nxp) or ct.shape != (self.nzp,):
+            raise ValueError("corr_xy must be (nyp, nxp) and corr_t (nzp,)")
+        self.ctx.check(lib().cg_grid_fill_param_corr(self._h, _ptr(cxy, ctypes.c_float), _ptr(ct, ctypes.c_float),
+                                                     grad_i, grad_j, scale), "fillInParamCorr")
+
+    def fillInErrCorr(self, corr_xy, grad_i=0.0, grad_j=0.0):
+        """FFTGrid::fillInErrCorr: corr_xy (nyp, nxp)."""
+        cxy = np.ascontiguousarray(corr_xy, dtype=np.float32)
+        if cxy.shape != (self.nyp, self.nxp):
+            raise ValueError("corr_xy must be (nyp, nxp)")
+        self.ctx.check(lib().cg_grid_fill_err_corr(self._h, _ptr(cxy, ctypes.c_float), grad_i, grad_j),
+                       "fillInErrCorr")
 
     def interpolateSeismic(self, energy_threshold):
         """FFTGrid::interpolateSeismic; returns (bad traces, (imin, imax, jmin, jmax))."""

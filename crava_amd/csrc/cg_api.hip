@@ -975,6 +975,34 @@ int cg_grid_interpolate_seismic(cg_grid *g, float energy_threshold, int *n_bad, 
   return seismic_passes(g, ops, runs, imin, imax, jmin, jmax);
 }
 
+int cg_grid_fill_param_corr(cg_grid *g, const float *corr_xy, const float *corr_t, float grad_i, float grad_j,
+                            float scale)
+{
+  if (!g || !corr_xy || !corr_t) return CG_EINVAL;
+  cg_ctx *ctx = g->ctx;
+  DeviceGuard dg(ctx->device);
+  std::vector<void *> dp;
+  int rc = stage_tables(ctx, {{corr_xy, sizeof(float) * g->nxp * g->nyp}, {corr_t, sizeof(float) * g->nzp}}, dp);
+  if (rc == CG_OK)
+    rc = cg::fill_param_corr(ctx, g->d, static_cast<const float *>(dp[0]), static_cast<const float *>(dp[1]), g->nxp,
+                             g->nyp, g->nzp, grad_i, grad_j, scale);
+  if (rc == CG_OK) g->transformed = 0;
+  return rc;
+}
+
+int cg_grid_fill_err_corr(cg_grid *g, const float *corr_xy, float grad_i, float grad_j)
+{
+  if (!g || !corr_xy) return CG_EINVAL;
+  cg_ctx *ctx = g->ctx;
+  DeviceGuard dg(ctx->device);
+  std::vector<void *> dp;
+  int rc = stage_tables(ctx, {{corr_xy, sizeof(float) * g->nxp * g->nyp}}, dp);
+  if (rc == CG_OK)
+    rc = cg::fill_err_corr(ctx, g->d, static_cast<const float *>(dp[0]), g->nxp, g->nyp, g->nzp, grad_i, grad_j);
+  if (rc == CG_OK) g->transformed = 0;
+  return rc;
+}
+
 int cg_grid_fill_noise_reference(cg_grid *g, uint32_t *seed)
 {
   if (!g || !seed) return CG_EINVAL;

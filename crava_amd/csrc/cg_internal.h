@@ -216,6 +216,10 @@ int trace_ops(cg_ctx *ctx, float *rv, const unsigned *ops, int nops, bool one_le
               int nzp);
 int extrapolate(cg_ctx *ctx, float *rv, const int *ref_i, const float *dist_x, const int *ref_j, const float *dist_y,
                 const float *dist_z, int nxp, int nyp, int nzp, int imin, int imax, int jmin, int jmax);
+// Prior / error correlation grids (cg_pad.hip); cxy [nyp][nxp], ct [nzp] in device memory.
+int fill_param_corr(cg_ctx *ctx, float *rv, const float *cxy, const float *ct, int nxp, int nyp, int nzp, float gi,
+                    float gj, float scale);
+int fill_err_corr(cg_ctx *ctx, float *rv, const float *cxy, int nxp, int nyp, int nzp, float gi, float gj);
 
 }  // namespace cg
 

@@ -169,6 +169,84 @@ k_extrapolate(float *__restrict__ rv, const int *__restrict__ ref_i, const float
   rv[base + (long)j * rnxp + i] = mult * v;
 }
 
+// Signed cyclic offset of index i on a lattice of n (FFTGrid's cycleI / cycleJ).
+__device__ __forceinline__ int cyc(int i, int n)
+{
+  const int c = i - n;
+  return i < -c ? i : c;
+}
+
+// FFTGrid::fillInParamCorr / FillInLateralCorr (fftgrid.cpp:830-923) with the multiplyByScalar
+// that follows them (seismicparametersholder.cpp:336-348) applied to every cell: the circular
+// temporal correlation interpolated at k + cycleI gradI + cycleJ gradJ times the lateral
+// correlation, RMISSING on the row padding.  A workgroup takes kRows rows (j) of plane k, a thread
+// cell pairs (float2 stores).  Without gradients (GRAD false) the interpolation point is k for the
+// whole plane, so the table reads are uniform; the arithmetic is the same either way.
+template <bool GRAD>
+__global__ void __launch_bounds__(1024)
+k_fill_param_corr(float *__restrict__ rv, const float *__restrict__ cxy, const float *__restrict__ ct, int nxp,
+                  int nyp, int nzp, int rnxp, float gi, float gj, float scale)
+{
+  const int k = blockIdx.y;
+  // the plane's uniform interpolation (no gradients)
+  const float c0 = ct[k], c1 = ct[k != nzp - 1 ? k + 1 : 0];
+  for (int p = threadIdx.x; p < rnxp / 2; p += blockDim.x) {
+#pragma unroll
+    for (int r = 0; r < kRows; ++r) {
+      const int j = (int)blockIdx.x * kRows + r;
+      if (j >= nyp) break;
+      float v[2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int i = 2 * p + h;
+        float x = -99999.0f;
+        if (i < nxp) {
+          float sub = (float)k + (float)cyc(i, nxp) * gi + (float)cyc(j, nyp) * gj;
+          int base = (int)floorf(sub);
+          sub -= (float)base;
+          float t0 = c0, t1 = c1;
+          if (GRAD) {
+            base %= nzp;  // the reference's cyclic wrap loops
+            if (base < 0) base += nzp;
+            t0 = ct[base];
+            t1 = ct[base != nzp - 1 ? base + 1 : 0];
+          }
+          x = (1.f - sub) * t0;
+          x += sub * t1;
+          x *= cxy[i + nxp * j];
+        }
+        v[h] = x * scale;
+      }
+      reinterpret_cast<float2 *>(rv + ((long)k * nyp + j) * rnxp)[p] = make_float2(v[0], v[1]);
+    }
+  }
+}
+
+// FFTGrid::fillInErrCorr (fftgrid.cpp:777-828): the lateral correlation weighted by the distance
+// of k + cycleI gradI + cycleJ gradJ from the k = 0 plane (cyclically, within one sample), zero
+// elsewhere, RMISSING on the row padding.
+__global__ void __launch_bounds__(cg::kThreads)
+k_fill_err_corr(float *__restrict__ rv, const float *__restrict__ cxy, int nxp, int nyp, int nzp, int rnxp, float gi,
+                float gj)
+{
+  const unsigned c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= (unsigned)rnxp * (unsigned)nyp) return;
+  const int i = (int)(c % (unsigned)rnxp), j = (int)(c / (unsigned)rnxp), k = blockIdx.y;
+  float v = -99999.0f;
+  if (i < nxp) {
+    float sub = (float)k + (float)cyc(i, nxp) * gi + (float)cyc(j, nyp) * gj;
+    v = 0.f;
+    if (fabsf(sub) < 1.0f || fabsf(sub - (float)nzp) < 1.0f) {
+      int base = (int)sub;  // 0 here, or nzp - 1 or nzp near the far end
+      sub -= (float)base;
+      if (base >= 1) base -= nzp;
+      if (base < 0) sub = 1.f - sub;
+      v = (1.f - sub) * cxy[i + nxp * j];
+    }
+  }
+  rv[(long)k * nyp * rnxp + c] = v;
+}
+
 }  // namespace cgpad
 
 namespace cg {
@@ -195,6 +273,34 @@ int fill_padded(cg_ctx *ctx, float *rv, const float *cube, int nx, int ny, int n
     k_fill_padded<true, true><<<grid(ny, nyp, nz, nzp), threads, 0, s>>>(rv, cube, g, ny, nyp, nz);
   launch_end(ctx, CG_K_ELEMENTWISE, ev);
   return hip_fail(ctx, hipGetLastError(), "k_fill_padded");
+}
+
+int fill_param_corr(cg_ctx *ctx, float *rv, const float *cxy, const float *ct, int nxp, int nyp, int nzp, float gi,
+                    float gj, float scale)
+{
+  const int rnxp = 2 * (nxp / 2 + 1), pairs = rnxp / 2;
+  const int threads = pairs >= 1024 ? 1024 : (pairs + 63) / 64 * 64;
+  const dim3 grid((unsigned)((nyp + kRows - 1) / kRows), (unsigned)nzp);
+  hipEvent_t ev;
+  launch_begin(ctx, CG_K_ELEMENTWISE, &ev);
+  if (gi == 0.f && gj == 0.f)
+    k_fill_param_corr<false><<<grid, threads, 0, ctx->stream>>>(rv, cxy, ct, nxp, nyp, nzp, rnxp, gi, gj, scale);
+  else
+    k_fill_param_corr<true><<<grid, threads, 0, ctx->stream>>>(rv, cxy, ct, nxp, nyp, nzp, rnxp, gi, gj, scale);
+  launch_end(ctx, CG_K_ELEMENTWISE, ev);
+  return hip_fail(ctx, hipGetLastError(), "k_fill_param_corr");
+}
+
+int fill_err_corr(cg_ctx *ctx, float *rv, const float *cxy, int nxp, int nyp, int nzp, float gi, float gj)
+{
+  const int rnxp = 2 * (nxp / 2 + 1);
+  const unsigned cells = (unsigned)rnxp * (unsigned)nyp;
+  hipEvent_t ev;
+  launch_begin(ctx, CG_K_ELEMENTWISE, &ev);
+  k_fill_err_corr<<<dim3((cells + kThreads - 1) / kThreads, (unsigned)nzp), kThreads, 0, ctx->stream>>>(
+      rv, cxy, nxp, nyp, nzp, rnxp, gi, gj);
+  launch_end(ctx, CG_K_ELEMENTWISE, ev);
+  return hip_fail(ctx, hipGetLastError(), "k_fill_err_corr");
 }
 
 int trace_energy(cg_ctx *ctx, const float *rv, float *energy, int nx, int ny, int nz, int rnxp, int nyp)

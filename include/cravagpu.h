@@ -168,6 +168,19 @@ int cg_grid_interpolate_seismic(cg_grid *grid, float energy_threshold, int *n_ba
  * distances into the padding (getDistToBoundary, fftgrid.cpp:1157-1200).  The rectangle must be
  * non-empty and inside the box (CG_EINVAL otherwise). */
 int cg_grid_extrapolate_seismic(cg_grid *grid, int imin, int imax, int jmin, int jmax);
+/* FFTGrid::fillInParamCorr / FillInLateralCorr (fftgrid.cpp:830-923) followed by
+ * multiplyByScalar(scale) (seismicparametersholder.cpp:311-348 scales each covariance grid by its
+ * priorVar0 entry): cell (i, j, k), i < nxp, gets the temporal correlation corr_t (nzp floats:
+ * circCorrT or a circ_auto_cov entry) linearly interpolated, cyclically, at k + cycleI grad_i +
+ * cycleJ grad_j (cycleI = i or i - nxp, whichever is nearer 0), times corr_xy[i + nxp j] (the
+ * lateral correlation surface, nyp*nxp floats); the row padding gets RMISSING; then every cell is
+ * multiplied by scale.  Host tables; real domain. */
+int cg_grid_fill_param_corr(cg_grid *grid, const float *corr_xy, const float *corr_t, float grad_i, float grad_j,
+                            float scale);
+/* FFTGrid::fillInErrCorr (fftgrid.cpp:777-828): corr_xy[i + nxp j] times 1 - (the distance of
+ * k + cycleI grad_i + cycleJ grad_j from the k = 0 plane, cyclically) where that distance is
+ * below one sample, 0 elsewhere, RMISSING on the row padding.  Real domain. */
+int cg_grid_fill_err_corr(cg_grid *grid, const float *corr_xy, float grad_i, float grad_j);
 
 /* ------------------------------------------------------------------ CRAVA binary grid files */
 /* Survey geometry stored in the file header (Simbox getters written by writeCravaFile). */

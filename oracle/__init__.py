@@ -98,6 +98,8 @@ def lib():
         L.orc_extrapolate_seismic.argtypes = [ctypes.c_int] * 6 + [_c_float_p] + [ctypes.c_int] * 4
         L.orc_interpolate_seismic.restype = ctypes.c_int
         L.orc_interpolate_seismic.argtypes = [ctypes.c_int] * 6 + [_c_float_p, ctypes.c_float, _c_int_p]
+        L.orc_fill_param_corr.argtypes = [ctypes.c_int] * 3 + [_c_float_p, _c_float_p] + [ctypes.c_float] * 3 + [_c_float_p]
+        L.orc_fill_err_corr.argtypes = [ctypes.c_int] * 3 + [_c_float_p] + [ctypes.c_float] * 2 + [_c_float_p]
         L.orc_kz_tables.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, _c_float_p,
                                     _c_float_p, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                     _c_float_p, _c_float_p, _c_float_p, _c_float_p, _c_int_p]
@@ -390,3 +392,24 @@ def read_crava(path):
         rnxp, nyp, nzp = struct.unpack(">iii", f.read(12))
         rv = np.frombuffer(f.read(4 * rnxp * nyp * nzp), dtype=">f4").astype(np.float32)
     return geometry, rv.reshape(nzp, nyp, rnxp)
+
+
+def fill_param_corr(corr_xy, corr_t, grad_i=0.0, grad_j=0.0, scale=1.0):
+    """fillInParamCorr / FillInLateralCorr then multiplyByScalar: corr_xy (nyp, nxp), corr_t (nzp,)
+    -> padded real grid (nzp, nyp, rnxp)."""
+    cxy = np.ascontiguousarray(corr_xy, dtype=np.float32)
+    ct = np.ascontiguousarray(corr_t, dtype=np.float32)
+    nyp, nxp = cxy.shape
+    nzp = len(ct)
+    out = np.zeros((nzp, nyp, 2 * (nxp // 2 + 1)), dtype=np.float32)
+    f = ctypes.c_float
+    lib().orc_fill_param_corr(nxp, nyp, nzp, _ptr(cxy, f), _ptr(ct, f), grad_i, grad_j, scale, _ptr(out, f))
+    return out
+
+
+def fill_err_corr(corr_xy, nzp, grad_i=0.0, grad_j=0.0):
+    cxy = np.ascontiguousarray(corr_xy, dtype=np.float32)
+    nyp, nxp = cxy.shape
+    out = np.zeros((nzp, nyp, 2 * (nxp // 2 + 1)), dtype=np.float32)
+    lib().orc_fill_err_corr(nxp, nyp, nzp, _ptr(cxy, ctypes.c_float), grad_i, grad_j, _ptr(out, ctypes.c_float))
+    return out

@@ -30,6 +30,9 @@
  *   orc_dist_to_boundary         getDistToBoundary (fftgrid.cpp:1105-1108, 1160-1163).
  *   orc_extrapolate_seismic /    the taper's closed form at hand-picked cells, neighbour means of
  *   orc_interpolate_seismic      isolated bad traces, energies and the rectangle by numpy.
+ *   orc_fill_param_corr /        closed forms: no gradient gives corr_t[k] corr_xy; integer
+ *   orc_fill_err_corr            gradients give exact cyclic shifts; fractional ones linear
+ *                                interpolation (numpy, double); the error grid's k = 0 plane.
  */
 #ifndef CRAVA_ORACLE_H
 #define CRAVA_ORACLE_H
@@ -129,6 +132,12 @@ void  orc_extrapolate_seismic(int nx, int ny, int nz, int nxp, int nyp, int nzp,
  * returns the number of traces below the energy limit, rect = {imin, imax, jmin, jmax}. */
 int   orc_interpolate_seismic(int nx, int ny, int nz, int nxp, int nyp, int nzp, float *rv, float threshold,
                               int *rect);
+/* FFTGrid::fillInParamCorr / FillInLateralCorr (fftgrid.cpp:830-923) then multiplyByScalar(scale);
+ * corr_xy [nyp][nxp], corr_t [nzp]; rv [nzp][nyp][rnxp]. */
+void  orc_fill_param_corr(int nxp, int nyp, int nzp, const float *corr_xy, const float *corr_t, float grad_i,
+                          float grad_j, float scale, float *rv);
+/* FFTGrid::fillInErrCorr (fftgrid.cpp:777-828). */
+void  orc_fill_err_corr(int nxp, int nyp, int nzp, const float *corr_xy, float grad_i, float grad_j, float *rv);
 
 /* ---------------------------------------------------------------- setup (a12, a16, a17, f1) */
 double orc_ricker(double t_ms, double peak_hz);                              /* wavelet.cpp:1484-1488 */

@@ -1,10 +1,11 @@
 /*
  * orc_pad.c -- padding fills on the input side of the path (test infrastructure only).
  *
- * Sequential restatements, in the reference's loop order, of how FFTGrid fills the padded
- * cube from an unpadded one: the background model's linear edge interpolation and the
- * seismic data's trace interpolation and tapered extrapolation.  Float arithmetic as the
- * reference's x86-64 build does it (no FMA; the Makefile passes -ffp-contract=off).
+ * Sequential restatements, in the reference's loop order, of how FFTGrid fills the grids that
+ * enter the path: the background model's linear edge interpolation, the seismic data's trace
+ * interpolation and tapered extrapolation, and the prior and error correlation grids.  Float
+ * arithmetic as the reference's x86-64 build does it (no FMA; the Makefile passes
+ * -ffp-contract=off).
  */
 #include <math.h>
 #include <stdlib.h>
@@ -178,4 +179,69 @@ int orc_interpolate_seismic(int nx, int ny, int nz, int nxp, int nyp, int nzp, f
     rect[3] = jmax;
   }
   return nbad;
+}
+
+/* Signed cyclic offset of index i on a lattice of n (FFTGrid's cycleI / cycleJ). */
+static int cyc(int i, int n)
+{
+  int c = i - n;
+  if (i < -c) c = i;
+  return c;
+}
+
+/* FFTGrid::fillInParamCorr / FillInLateralCorr (fftgrid.cpp:830-923), followed by
+ * multiplyByScalar(scale) (seismicparametersholder.cpp:336-348 scales by priorVar0 after the
+ * fill): the circular temporal correlation interpolated at k + cycleI gradI + cycleJ gradJ
+ * (cyclically) times the lateral correlation corr_xy[i + nxp j]; RMISSING on the row padding. */
+void orc_fill_param_corr(int nxp, int nyp, int nzp, const float *corr_xy, const float *corr_t, float grad_i,
+                         float grad_j, float scale, float *rv)
+{
+  const int rnxp = 2 * (nxp / 2 + 1);
+  for (int k = 0; k < nzp; ++k)
+    for (int j = 0; j < nyp; ++j)
+      for (int i = 0; i < rnxp; ++i) {
+        float v;
+        if (i < nxp) {
+          float sub = k + cyc(i, nxp) * grad_i + cyc(j, nyp) * grad_j;
+          int base = (int)floorf(sub);
+          sub -= base;
+          while (base < 0) base += nzp;
+          while (base >= nzp) base -= nzp;
+          v = (1 - sub) * corr_t[base];
+          v += sub * corr_t[base != nzp - 1 ? base + 1 : 0];
+          v *= corr_xy[i + nxp * j];
+        } else {
+          v = ORC_RMISSING;
+        }
+        rv[ridx(rnxp, nyp, i, j, k)] = v * scale;
+      }
+}
+
+/* FFTGrid::fillInErrCorr (fftgrid.cpp:777-828): the lateral correlation, weighted by the
+ * distance of k + cycleI gradI + cycleJ gradJ from the k = 0 plane (cyclically, within one
+ * sample), zero elsewhere; RMISSING on the row padding. */
+void orc_fill_err_corr(int nxp, int nyp, int nzp, const float *corr_xy, float grad_i, float grad_j, float *rv)
+{
+  const int rnxp = 2 * (nxp / 2 + 1);
+  for (int k = 0; k < nzp; ++k)
+    for (int j = 0; j < nyp; ++j)
+      for (int i = 0; i < rnxp; ++i) {
+        float v;
+        if (i < nxp) {
+          float sub = k + cyc(i, nxp) * grad_i + cyc(j, nyp) * grad_j;
+          if (fabsf(sub) < 1.0f || fabsf(sub - nzp) < 1.0f) {
+            int base = (int)sub;
+            sub -= base;
+            while (base < -1) base += nzp;
+            while (base >= 1) base -= nzp;
+            if (base < 0) sub = 1 - sub;
+            v = (1 - sub) * corr_xy[i + nxp * j];
+          } else {
+            v = 0.0f;
+          }
+        } else {
+          v = ORC_RMISSING;
+        }
+        rv[ridx(rnxp, nyp, i, j, k)] = v;
+      }
 }

@@ -1,5 +1,6 @@
 """The path's inputs on the device: padding fills (cg_grid_fill_padded,
-cg_grid_interpolate_seismic, cg_grid_extrapolate_seismic) and CRAVA binary grid files
+cg_grid_interpolate_seismic, cg_grid_extrapolate_seismic), prior and error correlation grids
+(cg_grid_fill_param_corr, cg_grid_fill_err_corr) and CRAVA binary grid files
 (cg_grid_write_crava, cg_grid_read_crava) vs the oracle, bit for bit."""
 import numpy as np
 import pytest
@@ -180,4 +181,27 @@ def test_crava_file_errors(ctx, tmp_path):
     with pytest.raises(C.CravaError, match="ends inside the header"):
         g.readCravaFile(str(tmp_path / "head.crava"))
     other.close()
+    g.close()
+
+
+@pytest.mark.parametrize("dims,gi,gj,scale", [((9, 6, 10), 0.0, 0.0, 1.0), ((9, 6, 10), 2.0, 0.0, 0.7),
+                                              ((16, 12, 20), 0.25, -0.5, 3.0), ((15, 7, 9), -1.75, 0.3, 1.0),
+                                              ((576, 576, 320), 0.1, 0.05, 0.01)])
+def test_correlation_fills_match_oracle(ctx, dims, gi, gj, scale):
+    """fillInParamCorr (+ multiplyByScalar) and fillInErrCorr with no, integer, fractional and
+    negative gradients, bit for bit; the last case at the single-cube padded size."""
+    nxp, nyp, nzp = dims
+    print("correlation fills", dims, gi, gj, flush=True)
+    rng = np.random.default_rng(nxp + nzp)
+    cxy = rng.random((nyp, nxp)).astype(np.float32)
+    ct = rng.random(nzp).astype(np.float32)
+    g = C.FFTGrid(ctx, nxp, nyp, nzp, nxp, nyp, nzp, C.COVARIANCE)
+    g.fillInParamCorr(cxy, ct, gi, gj, scale)
+    got = g.download_real(padded=True)
+    want = O.fill_param_corr(cxy, ct, gi, gj, scale)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    g.fillInErrCorr(cxy, gi, gj)
+    got = g.download_real(padded=True)
+    want = O.fill_err_corr(cxy, nzp, gi, gj)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
     g.close()
