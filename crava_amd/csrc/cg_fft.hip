@@ -13,7 +13,8 @@
 #include "cg_dft.h"
 #include "cg_internal.h"
 
-namespace {
+// named (not anonymous) so that profilers show the kernels' names
+namespace cgfft {
 
 using namespace cgdft;
 
@@ -314,9 +315,11 @@ __global__ void __launch_bounds__(cg::kThreads) k_fft_batch(const cg::FftItem *_
 
 constexpr size_t kLdsBudget = 80 * 1024;  // bytes per workgroup: two workgroups per CU (160 KB LDS)
 
-}  // namespace
+}  // namespace cgfft
 
 namespace cg {
+
+using namespace cgfft;
 
 bool make_plan(int n, FftPlan *plan)
 {

@@ -23,7 +23,8 @@
 #include "cg_mem.h"
 #include "cg_post.h"
 
-namespace {
+// named (not anonymous) so that profilers show the kernels' names
+namespace cgpost {
 
 using cg::gld;
 using cg::gst;
@@ -455,9 +456,11 @@ void launch_nt64(const cg::PostParams &p, unsigned blocks, hipStream_t s, bool s
   else k_posterior64<NT, false><<<blocks, cg::kThreads, 0, s>>>(p);
 }
 
-}  // namespace
+}  // namespace cgpost
 
 namespace cg {
+
+using namespace cgpost;
 
 int launch_posterior_batch(cg_ctx *ctx, int nt, bool sep, bool mixed, const PostItem *dev_items, int n,
                            unsigned blocks)

@@ -26,7 +26,8 @@
 
 #include "cg_internal.h"
 
-namespace {
+// named (not anonymous) so that profilers show the kernels' names
+namespace cgrng {
 
 constexpr uint32_t kMul = 69069u;    // random.h:11
 constexpr uint32_t kAdd = 1u;        // random.h:12
@@ -283,9 +284,11 @@ __global__ void k_noise_cells(float2 *__restrict__ out, long csize, int cnxp, in
   out[i] = v;
 }
 
-}  // namespace
+}  // namespace cgrng
 
 namespace cg {
+
+using namespace cgrng;
 
 // Fills `out` (cnxp x nyp x nzp complex) as FFTGrid::fillInComplexNoise does from the state
 // *seed and advances *seed past the uniforms consumed.  Synchronises the context stream.
