@@ -89,6 +89,7 @@ def lib():
             "cg_grid_fill_noise_reference": [_vp, ctypes.POINTER(ctypes.c_uint32)],
             "cg_build_kz_tables": [i, i, d, i, _f_p, _f_p, f, f, f, _f_p, _f_p, _f_p, _f_p, _i_p],
             "cg_factorable": [i],
+            "cg_make_circ_auto_cov_pos_def": [i, _f_p],
             "cg_fft_r2c": [_vp, _vp],
             "cg_forward_model": [_vp, i, _f_p, _vp * 3, _vp * 8],
             "cg_grid_add_white_noise": [_vp, f, u64, ctypes.c_uint32],
@@ -622,3 +623,15 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def make_circ_auto_cov_pos_def(circ_auto_cov):
+    """SeismicParametersHolder::MakeCircAutoCovPosDef on the nine circular auto- and
+    cross-covariance series (3, 3, nzp); returns the adjusted series (host computation)."""
+    a = np.array(circ_auto_cov, dtype=np.float32, order="C", copy=True)
+    if a.ndim != 3 or a.shape[:2] != (3, 3):
+        raise ValueError("circ_auto_cov must be (3, 3, nzp)")
+    rc = lib().cg_make_circ_auto_cov_pos_def(a.shape[2], _ptr(a, ctypes.c_float))
+    if rc != CG_OK:
+        raise CravaError("cg_make_circ_auto_cov_pos_def failed (%d): a lag-0 variance is not positive" % rc)
+    return a

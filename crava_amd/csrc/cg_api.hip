@@ -1296,6 +1296,130 @@ int cg_grid_sum_squares(cg_grid *g, int ext, double *sum)
 }
 
 // ---------------------------------------------------------------------------- host setup
+}  // extern "C"
+
+namespace {
+
+// |M| = V |diag(w)| V^T of a real symmetric n x n matrix (row-major, overwritten), by cyclic
+// Jacobi rotations.
+template <int N>
+void abs_sym(double (&a)[N][N])
+{
+  double v[N][N] = {};
+  for (int i = 0; i < N; ++i) v[i][i] = 1.0;
+  for (int sweep = 0; sweep < 64; ++sweep) {
+    double off = 0.0, all = 0.0;
+    for (int p = 0; p < N; ++p)
+      for (int q = 0; q < N; ++q) {
+        all += a[p][q] * a[p][q];
+        if (p != q) off += a[p][q] * a[p][q];
+      }
+    if (off <= 1e-30 * all) break;
+    for (int p = 0; p < N - 1; ++p)
+      for (int q = p + 1; q < N; ++q) {
+        if (a[p][q] == 0.0) continue;
+        // rotation in the (p, q) plane that zeroes a[p][q]: t = tan of the angle
+        const double th = (a[q][q] - a[p][p]) / (2.0 * a[p][q]);
+        const double t = (th >= 0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1.0));
+        const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+        for (int r = 0; r < N; ++r) {  // columns p, q
+          const double rp = a[r][p], rq = a[r][q];
+          a[r][p] = c * rp - s * rq;
+          a[r][q] = s * rp + c * rq;
+        }
+        for (int r = 0; r < N; ++r) {  // rows p, q
+          const double pr = a[p][r], qr = a[q][r];
+          a[p][r] = c * pr - s * qr;
+          a[q][r] = s * pr + c * qr;
+        }
+        for (int r = 0; r < N; ++r) {
+          const double rp = v[r][p], rq = v[r][q];
+          v[r][p] = c * rp - s * rq;
+          v[r][q] = s * rp + c * rq;
+        }
+      }
+  }
+  double w[N];
+  for (int i = 0; i < N; ++i) w[i] = std::fabs(a[i][i]);
+  for (int i = 0; i < N; ++i)
+    for (int j = 0; j < N; ++j) {
+      double x = 0.0;
+      for (int k = 0; k < N; ++k) x += v[i][k] * w[k] * v[j][k];
+      a[i][j] = x;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int cg_make_circ_auto_cov_pos_def(int nzp, float *cac)
+{
+  if (nzp < 1 || !cac) return CG_EINVAL;
+  const int nf = nzp / 2 + 1;
+  const double two_pi = 6.283185307179586476925286766559;
+  auto series = [&](int j, int k) { return cac + (size_t)(3 * j + k) * nzp; };
+  // lag-0 standard deviations, as the reference rounds them (square root of a float)
+  double d[3], dinv[3];
+  for (int i = 0; i < 3; ++i) {
+    const float v0 = series(i, i)[0];
+    if (!(v0 > 0.0f)) return CG_EINVAL;
+    d[i] = (double)std::sqrt(v0);
+    dinv[i] = 1.0 / (double)std::sqrt(v0);
+  }
+  // spectra of the nine series (sign -1, unnormalised, bins 0..nzp/2), rounded to float as
+  // FFTGrid::fft1DzInPlace returns them
+  std::vector<std::complex<float>> spec((size_t)9 * nf);
+  for (int s = 0; s < 9; ++s) {
+    const float *x = cac + (size_t)s * nzp;
+    for (int m = 0; m < nf; ++m) {
+      double re = 0.0, im = 0.0;
+      for (int t = 0; t < nzp; ++t) {
+        const double ang = two_pi * (double)((long)m * t % nzp) / nzp;
+        re += x[t] * std::cos(ang);
+        im -= x[t] * std::sin(ang);
+      }
+      spec[(size_t)s * nf + m] = std::complex<float>((float)re, (float)im);
+    }
+  }
+  // per frequency: M = D^-1 C D^-1 (Hermitian for Hermitian pairs of cross-covariances),
+  // C' = D |M| D with |M| from its real symmetric 6 x 6 form [[Re, -Im], [Im, Re]]
+  for (int m = 0; m < nf; ++m) {
+    double r[6][6];
+    for (int j = 0; j < 3; ++j)
+      for (int k = 0; k < 3; ++k) {
+        const std::complex<float> c = spec[(size_t)(3 * j + k) * nf + m];
+        const double re = dinv[j] * (double)c.real() * dinv[k], im = dinv[j] * (double)c.imag() * dinv[k];
+        r[j][k] = re;
+        r[j + 3][k + 3] = re;
+        r[j][k + 3] = -im;
+        r[j + 3][k] = im;
+      }
+    abs_sym<6>(r);
+    for (int j = 0; j < 3; ++j)
+      for (int k = 0; k < 3; ++k)
+        spec[(size_t)(3 * j + k) * nf + m] =
+            std::complex<float>((float)(d[j] * r[j][k] * d[k]), (float)(d[j] * r[j + 3][k] * d[k]));
+  }
+  // back to lags: half-complex to real (imaginary parts of bins 0 and nzp/2 for even nzp ignored,
+  // as FFTW's c2r does), unnormalised, then times 1/nzp in float
+  const float factor = (float)(1.0 / (double)nzp);
+  for (int s = 0; s < 9; ++s) {
+    const std::complex<float> *X = &spec[(size_t)s * nf];
+    float *x = cac + (size_t)s * nzp;
+    for (int t = 0; t < nzp; ++t) {
+      double acc = X[0].real();
+      for (int m = 1; m < nf; ++m) {
+        const double ang = two_pi * (double)((long)m * t % nzp) / nzp;
+        const double term = X[m].real() * std::cos(ang) - X[m].imag() * std::sin(ang);
+        acc += (2 * m == nzp) ? X[m].real() * std::cos(ang) : 2.0 * term;
+      }
+      x[t] = (float)acc * factor;
+    }
+  }
+  return CG_OK;
+}
+
 int cg_factorable(int leastint)
 {
   if (leastint <= 1) return 1;

@@ -338,6 +338,14 @@ int cg_build_kz_tables(int nz, int nzp, double dt_ms, int ntheta, const float *w
                        float *e3, int *invert);
 /* FFTGrid::findClosestFactorableNumber (fftgrid.cpp:2630-2658). */
 int cg_factorable(int leastint);
+/* SeismicParametersHolder::MakeCircAutoCovPosDef (seismicparametersholder.cpp:661-767), in place on
+ * the nine circular auto- and cross-covariance series circ_auto_cov[3][3][nzp] (host memory):
+ * per frequency the 3 x 3 spectrum, scaled by the lag-0 standard deviations, has its eigenvalues
+ * replaced by their absolute values, then the series are transformed back (x 1/nzp).  The
+ * cross-covariances come in Hermitian pairs (c_jk(t) = c_kj(-t)), as the reference builds them,
+ * so each spectrum is Hermitian.  CG_EINVAL if a lag-0 variance is not positive (the
+ * reference's result is then NaN).  Host only. */
+int cg_make_circ_auto_cov_pos_def(int nzp, float *circ_auto_cov);
 
 #ifdef __cplusplus
 }

@@ -413,3 +413,19 @@ def fill_err_corr(corr_xy, nzp, grad_i=0.0, grad_j=0.0):
     out = np.zeros((nzp, nyp, 2 * (nxp // 2 + 1)), dtype=np.float32)
     lib().orc_fill_err_corr(nxp, nyp, nzp, _ptr(cxy, ctypes.c_float), grad_i, grad_j, _ptr(out, ctypes.c_float))
     return out
+
+
+def make_circ_auto_cov_pos_def(cac):
+    """SeismicParametersHolder::MakeCircAutoCovPosDef (seismicparametersholder.cpp:661-767) in
+    double: spectra of the nine series, per frequency D V |diag(w)| V^H D with (w, V) the general
+    complex eigendecomposition of D^-1 C D^-1 (D = lag-0 standard deviations), back to lags / nzp."""
+    a = np.asarray(cac, dtype=np.float64)
+    nzp = a.shape[2]
+    d = np.sqrt(np.array([a[i, i, 0] for i in range(3)]))
+    spec = np.fft.rfft(a, axis=2)  # sign -1, unnormalised, bins 0..nzp/2
+    out = np.empty_like(spec)
+    for m in range(spec.shape[2]):
+        M = spec[:, :, m] / d[:, None] / d[None, :]
+        w, V = np.linalg.eig(M)
+        out[:, :, m] = d[:, None] * (V @ np.diag(np.abs(w)) @ V.conj().T) * d[None, :]
+    return np.fft.irfft(out, n=nzp, axis=2)
