@@ -102,7 +102,7 @@ def lib():
             "cg_grid_fill_padded": [_vp, _vp, i],
             "cg_grid_interpolate_seismic": [_vp, f, _i_p, _i_p],
             "cg_grid_extrapolate_seismic": [_vp, i, i, i, i],
-            "cg_grid_fill_param_corr": [_vp, _f_p, _f_p, f, f, f], "cg_grid_fill_err_corr": [_vp, _f_p, f, f],
+            "cg_grid_fill_param_corr": [_vp, _f_p, _f_p, f, f, f], "cg_grid_fill_trace_taper": [_vp, _vp, i], "cg_grid_fill_err_corr": [_vp, _f_p, f, f],
             "cg_grid_write_crava": [_vp, ctypes.c_char_p, ctypes.POINTER(CravaGeometry)],
             "cg_grid_read_crava": [_vp, ctypes.c_char_p, ctypes.POINTER(CravaGeometry)],
         }
@@ -313,6 +313,16 @@ class FFTGrid:
         if a.shape != (self.nz, self.ny, self.nx):
             raise ValueError("cube must be (nz, ny, nx) = %s" % ((self.nz, self.ny, self.nx),))
         self.ctx.check(lib().cg_grid_fill_padded(self._h, a.ctypes.data_as(_vp), 0), "fill_padded")
+
+    def fill_trace_taper(self, cube):
+        """ComputeSyntSeismic's trace taper: cube (nz, ny, nx) host array, or a device address."""
+        if isinstance(cube, int):
+            self.ctx.check(lib().cg_grid_fill_trace_taper(self._h, ctypes.c_void_p(cube), 1), "fill_trace_taper")
+            return
+        a = np.ascontiguousarray(cube, dtype=np.float32)
+        if a.shape != (self.nz, self.ny, self.nx):
+            raise ValueError("cube must be (nz, ny, nx) = %s" % ((self.nz, self.ny, self.nx),))
+        self.ctx.check(lib().cg_grid_fill_trace_taper(self._h, a.ctypes.data_as(_vp), 0), "fill_trace_taper")
 
     def fillInParamCorr(self, corr_xy, corr_t, grad_i=0.0, grad_j=0.0, scale=1.0):
         """FFTGrid::fillInParamCorr / FillInLateralCorr, then multiplyByScalar(scale): corr_xy

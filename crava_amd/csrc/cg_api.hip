@@ -975,6 +975,36 @@ int cg_grid_interpolate_seismic(cg_grid *g, float energy_threshold, int *n_bad, 
   return seismic_passes(g, ops, runs, imin, imax, jmin, jmax);
 }
 
+int cg_grid_fill_trace_taper(cg_grid *g, const float *cube, int cube_on_device)
+{
+  if (!g || !cube) return CG_EINVAL;
+  cg_ctx *ctx = g->ctx;
+  if (g->nzp < g->nz + 2)
+    return cg::fail(ctx, CG_EINVAL, "fill_trace_taper: needs nzp >= nz + 2 (the taper divides by nzp - nz - 1)");
+  DeviceGuard dg(ctx->device);
+  const float *src = cube;
+  void *tmp = nullptr;
+  if (!cube_on_device) {
+    const size_t bytes = sizeof(float) * (size_t)g->nx * g->ny * g->nz;
+    hipError_t e = cg::dev_alloc(ctx, &tmp, bytes);
+    if (e != hipSuccess) return cg::hip_fail(ctx, e, "fill_trace_taper: staging the cube");
+    e = hipMemcpyAsync(tmp, cube, bytes, hipMemcpyHostToDevice, ctx->stream);
+    if (e != hipSuccess) {
+      cg::dev_free(ctx, tmp);
+      return cg::hip_fail(ctx, e, "fill_trace_taper: copying the cube");
+    }
+    src = static_cast<const float *>(tmp);
+  }
+  int rc = cg::fill_trace_taper(ctx, g->d, src, g->nx, g->ny, g->nz, g->nxp, g->nyp, g->nzp);
+  if (tmp) {
+    cg::dev_free(ctx, tmp);
+    const hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (rc == CG_OK) rc = cg::hip_fail(ctx, e, "fill_trace_taper");
+  }
+  if (rc == CG_OK) g->transformed = 0;
+  return rc;
+}
+
 int cg_grid_fill_param_corr(cg_grid *g, const float *corr_xy, const float *corr_t, float grad_i, float grad_j,
                             float scale)
 {

@@ -220,6 +220,8 @@ int extrapolate(cg_ctx *ctx, float *rv, const int *ref_i, const float *dist_x, c
 int fill_param_corr(cg_ctx *ctx, float *rv, const float *cxy, const float *ct, int nxp, int nyp, int nzp, float gi,
                     float gj, float scale);
 int fill_err_corr(cg_ctx *ctx, float *rv, const float *cxy, int nxp, int nyp, int nzp, float gi, float gj);
+// The forward model's trace taper (cg_pad.hip); cube [nz][ny][nx] in device memory.
+int fill_trace_taper(cg_ctx *ctx, float *rv, const float *cube, int nx, int ny, int nz, int nxp, int nyp, int nzp);
 
 }  // namespace cg
 

@@ -247,6 +247,31 @@ k_fill_err_corr(float *__restrict__ rv, const float *__restrict__ cxy, int nxp, 
   rv[(long)k * nyp * rnxp + c] = v;
 }
 
+// The trace taper of CravaResult::ComputeSyntSeismic (cravaresult.cpp:2338-2346): box traces
+// keep their nz samples and run linearly from the last sample back to the first over the z
+// padding (fac ((k - nz) r_0 + (nzp - k - 1) r_{nz-1}), fac = 1 / (nzp - nz - 1)); columns
+// outside the box and the row padding are 0 (the z-only operators that follow never mix
+// columns).  grid.x over a plane's rnxp * nyp cells, grid.y = k.
+__global__ void __launch_bounds__(cg::kThreads)
+k_fill_trace_taper(float *__restrict__ rv, const float *__restrict__ cube, int nx, int ny, int nz, int nyp, int rnxp,
+                   int nzp, float fac)
+{
+  const unsigned c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= (unsigned)rnxp * (unsigned)nyp) return;
+  const int i = (int)(c % (unsigned)rnxp), j = (int)(c / (unsigned)rnxp), k = blockIdx.y;
+  float v = 0.f;
+  if (i < nx && j < ny) {
+    const long plane = (long)nx * ny, col = (long)j * nx + i;
+    if (k < nz) {
+      v = cube[k * plane + col];
+    } else {
+      const float r0 = cube[col], r1 = cube[(nz - 1) * plane + col];
+      v = fac * ((float)(k - nz) * r0 + (float)(nzp - k - 1) * r1);
+    }
+  }
+  rv[(long)k * nyp * rnxp + c] = v;
+}
+
 }  // namespace cgpad
 
 namespace cg {
@@ -289,6 +314,19 @@ int fill_param_corr(cg_ctx *ctx, float *rv, const float *cxy, const float *ct, i
     k_fill_param_corr<true><<<grid, threads, 0, ctx->stream>>>(rv, cxy, ct, nxp, nyp, nzp, rnxp, gi, gj, scale);
   launch_end(ctx, CG_K_ELEMENTWISE, ev);
   return hip_fail(ctx, hipGetLastError(), "k_fill_param_corr");
+}
+
+int fill_trace_taper(cg_ctx *ctx, float *rv, const float *cube, int nx, int ny, int nz, int nxp, int nyp, int nzp)
+{
+  const int rnxp = 2 * (nxp / 2 + 1);
+  const float fac = 1.0f / (float)(nzp - nz - 1);
+  const unsigned cells = (unsigned)rnxp * (unsigned)nyp;
+  hipEvent_t ev;
+  launch_begin(ctx, CG_K_ELEMENTWISE, &ev);
+  k_fill_trace_taper<<<dim3((cells + kThreads - 1) / kThreads, (unsigned)nzp), kThreads, 0, ctx->stream>>>(
+      rv, cube, nx, ny, nz, nyp, rnxp, nzp, fac);
+  launch_end(ctx, CG_K_ELEMENTWISE, ev);
+  return hip_fail(ctx, hipGetLastError(), "k_fill_trace_taper");
 }
 
 int fill_err_corr(cg_ctx *ctx, float *rv, const float *cxy, int nxp, int nyp, int nzp, float gi, float gj)

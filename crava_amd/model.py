@@ -128,3 +128,44 @@ def inversion_tables(nz, nzp, dt_ms, angles_deg, peak_hz, wavelet_scales=None, v
     wav = np.stack([wavelet_fft_order(nzp, dt_ms, peak_hz, s, ormsby_hz) for s in scales])
     K, e1, e2, e3, inv = build_kz_tables(nz, nzp, dt_ms, wav, A, low_cut, high_cut, 1.0)
     return dict(A=A, wavelets=wav, K=K, e1=e1, e2=e2, e3=e3, invert=inv)
+
+
+def synthetic_operator(wavelets, A):
+    """Per-k_z operator of CravaResult::ComputeSyntSeismic (cravaresult.cpp:2312-2377) for
+    cg_forward_model: K[k, l, j] = A[l, j] (1 - e^{-2 pi i k / nzp}) conj(W_l(k)), the backward
+    difference of the impedance (Wavelet1D FIRSTORDERBACKWARDDIFF, wavelet1D.cpp:462-467) and the
+    product with the conjugate wavelet spectrum.  wavelets (ntheta, nzp) real, in FFT order;
+    A (ntheta, 3) the reflection coefficients.  Returns (nzp, ntheta, 3) complex64."""
+    w = np.asarray(wavelets, dtype=np.float64)
+    a = np.asarray(A, dtype=np.float64)
+    nzp = w.shape[1]
+    k = np.arange(nzp)
+    diff = 1.0 - np.exp(-2j * np.pi * k / nzp)
+    W = np.fft.fft(w, axis=1)  # sign -1, unnormalised, every k
+    K = diff[:, None, None] * np.conj(W).T[:, :, None] * a[None, :, :]
+    return K.astype(np.complex64)
+
+
+def synthetic_seismic(ctx, vp, vs, rho, wavelets, A, nxp, nyp, nzp):
+    """CravaResult::ComputeSyntSeismic on the device: vp, vs, rho cubes (nz, ny, nx) -> one
+    synthetic seismic cube (nz, ny, nx) per angle.  Each parameter gets the reference's trace
+    taper over the z padding, is transformed (3D, the x / y transforms cancel for an operator
+    acting along z only), combined by the operator of synthetic_operator and transformed back."""
+    from . import FFTGrid, PARAMETER, DATA, forward_model
+    nz, ny, nx = np.shape(vp)
+    nt = len(wavelets)
+    m = [FFTGrid(ctx, nx, ny, nz, nxp, nyp, nzp, PARAMETER) for _ in range(3)]
+    d = [FFTGrid(ctx, nx, ny, nz, nxp, nyp, nzp, DATA) for _ in range(nt)]
+    try:
+        for g, cube in zip(m, (vp, vs, rho)):
+            g.fill_trace_taper(cube)
+            g.fftInPlace()
+        forward_model(ctx, synthetic_operator(wavelets, A), m, d)
+        out = []
+        for g in d:
+            g.invFFTInPlace()
+            out.append(g.download_real()[:nz, :ny, :nx])
+        return out
+    finally:
+        for g in m + d:
+            g.close()

@@ -177,6 +177,14 @@ int cg_grid_extrapolate_seismic(cg_grid *grid, int imin, int imax, int jmin, int
  * multiplied by scale.  Host tables; real domain. */
 int cg_grid_fill_param_corr(cg_grid *grid, const float *corr_xy, const float *corr_t, float grad_i, float grad_j,
                             float scale);
+/* The trace taper of CravaResult::ComputeSyntSeismic (cravaresult.cpp:2338-2346): box columns of
+ * the cube (nz*ny*nx floats, i fastest) keep their nz samples, and over the z padding run from the
+ * last sample back to the first, fac ((k - nz) r_0 + (nzp - k - 1) r_{nz-1}) with
+ * fac = 1 / (nzp - nz - 1); other columns and the row padding are 0.  With cg_fft_r2c,
+ * cg_forward_model (K = A D conj(W), D the backward difference 1 - e^{-2 pi i k / nzp}) and
+ * cg_fft_c2r_inplace it gives ComputeSyntSeismic's synthetic seismic (the operators act along z
+ * only).  Needs nzp >= nz + 2.  cube_on_device as in cg_grid_fill_padded.  Real domain. */
+int cg_grid_fill_trace_taper(cg_grid *grid, const float *cube, int cube_on_device);
 /* FFTGrid::fillInErrCorr (fftgrid.cpp:777-828): corr_xy[i + nxp j] times 1 - (the distance of
  * k + cycleI grad_i + cycleJ grad_j from the k = 0 plane, cyclically) where that distance is
  * below one sample, 0 elsewhere, RMISSING on the row padding.  Real domain. */

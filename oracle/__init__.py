@@ -429,3 +429,25 @@ def make_circ_auto_cov_pos_def(cac):
         w, V = np.linalg.eig(M)
         out[:, :, m] = d[:, None] * (V @ np.diag(np.abs(w)) @ V.conj().T) * d[None, :]
     return np.fft.irfft(out, n=nzp, axis=2)
+
+
+def synthetic_seismic(vp, vs, rho, wavelets, A, nzp):
+    """CravaResult::ComputeSyntSeismic (cravaresult.cpp:2312-2406) trace by trace, in double:
+    impedance A_l . (vp, vs, rho), the trace taper over the z padding
+    (fac ((k - nz) r_0 + (nzp - k - 1) r_{nz-1}), fac = 1 / (nzp - nz - 1)), backward difference
+    (cyclic), forward FFT, product with the conjugate wavelet spectrum, inverse FFT / nzp, the
+    first nz samples."""
+    p = np.stack([np.asarray(x, dtype=np.float64) for x in (vp, vs, rho)])
+    nz = p.shape[1]
+    out = []
+    for l in range(len(wavelets)):
+        imp = np.tensordot(np.asarray(A[l], dtype=np.float64), p, axes=1)  # (nz, ny, nx)
+        tr = np.zeros((nzp,) + imp.shape[1:])
+        tr[:nz] = imp
+        k = np.arange(nz, nzp)[:, None, None]
+        tr[nz:] = ((k - nz) * imp[0] + (nzp - k - 1) * imp[nz - 1]) / (nzp - nz - 1)
+        r = tr - np.roll(tr, 1, axis=0)
+        W = np.fft.rfft(np.asarray(wavelets[l], dtype=np.float64))
+        s = np.fft.rfft(r, axis=0) * np.conj(W)[:, None, None]
+        out.append(np.fft.irfft(s, n=nzp, axis=0)[:nz])
+    return out

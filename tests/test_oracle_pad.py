@@ -197,3 +197,24 @@ def test_err_corr_closed_forms():
     # the oracle keeps; it is compared on the device, not restated here)
     keep = _cyc(nxp) >= 0
     assert np.max(np.abs(g[:, :, :nxp][:, :, keep] - np.broadcast_to(want, g[:, :, :nxp].shape)[:, :, keep])) < 1e-6
+
+
+def test_synthetic_seismic_oracle_time_domain():
+    """The oracle's ComputeSyntSeismic against the same model in the time domain: the tapered
+    impedance trace, differenced backwards (cyclically), cross-correlated cyclically with the
+    wavelet, y[t] = sum_u r'[u] w[u - t] (the product with the conjugate spectrum)."""
+    rng = np.random.default_rng(6)
+    nx, ny, nz, nzp = 3, 2, 7, 12
+    vp, vs, rho = (rng.normal(size=(nz, ny, nx)) for _ in range(3))
+    A = rng.normal(size=(2, 3))
+    w = rng.normal(size=(2, nzp))
+    got = O.synthetic_seismic(vp, vs, rho, w, A, nzp)
+    for l in range(2):
+        for j in range(ny):
+            for i in range(nx):
+                imp = [A[l, 0] * vp[k, j, i] + A[l, 1] * vs[k, j, i] + A[l, 2] * rho[k, j, i] for k in range(nz)]
+                tr = imp + [((k - nz) * imp[0] + (nzp - k - 1) * imp[nz - 1]) / (nzp - nz - 1)
+                            for k in range(nz, nzp)]
+                d = [tr[u] - tr[u - 1] for u in range(nzp)]
+                y = [sum(d[u] * w[l, (u - t) % nzp] for u in range(nzp)) for t in range(nz)]
+                assert np.allclose(got[l][:, j, i], y, rtol=1e-9, atol=1e-9)
