@@ -82,6 +82,7 @@ def lib():
             "cg_grid_upload_real": [_vp, _f_p], "cg_grid_upload_complex": [_vp, _f_p],
             "cg_grid_download_real": [_vp, _f_p], "cg_grid_download_complex": [_vp, _f_p],
             "cg_grid_get_complex": [_vp, i, i, i, i, _f_p], "cg_grid_set_complex": [_vp, i, i, i, i, _f_p],
+            "cg_grid_get_real_value": [_vp, i, i, i, i, _f_p], "cg_grid_get_real_trace": [_vp, i, i, i, _f_p],
             "cg_fft_r2c_inplace": [_vp], "cg_fft_c2r_inplace": [_vp],
             "cg_grid_add": [_vp, _vp], "cg_grid_multiply_by_scalar": [_vp, f], "cg_grid_fill_constant": [_vp, f],
             "cg_posterior": [_vp, ctypes.POINTER(PostArgs)], "cg_simulate": [_vp, ctypes.POINTER(SimArgs)],
@@ -313,6 +314,18 @@ class FFTGrid:
         if a.shape != (self.nz, self.ny, self.nx):
             raise ValueError("cube must be (nz, ny, nx) = %s" % ((self.nz, self.ny, self.nx),))
         self.ctx.check(lib().cg_grid_fill_padded(self._h, a.ctypes.data_as(_vp), 0), "fill_padded")
+
+    def getRealValue(self, i, j, k, extSimbox=False):
+        v = ctypes.c_float()
+        self.ctx.check(lib().cg_grid_get_real_value(self._h, i, j, k, 1 if extSimbox else 0, ctypes.byref(v)),
+                       "getRealValue")
+        return v.value
+
+    def getRealTrace(self, i, j, add_padding=False):
+        out = np.empty(self.nzp if add_padding else self.nz, dtype=np.float32)
+        self.ctx.check(lib().cg_grid_get_real_trace(self._h, i, j, 1 if add_padding else 0, _ptr(out, ctypes.c_float)),
+                       "getRealTrace")
+        return out
 
     def fill_trace_taper(self, cube):
         """ComputeSyntSeismic's trace taper: cube (nz, ny, nx) host array, or a device address."""

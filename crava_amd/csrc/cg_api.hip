@@ -660,6 +660,35 @@ int cg_grid_set_complex(cg_grid *g, int i, int j, int k, int ext, const float v[
   return copy_sync(g, g->d + 2 * idx, v, 2 * sizeof(float), hipMemcpyHostToDevice);
 }
 
+int cg_grid_get_real_value(cg_grid *g, int i, int j, int k, int ext, float *v)
+{
+  if (!g || !v) return CG_EINVAL;
+  if (g->transformed) return cg::fail(g->ctx, CG_ESTATE, "get_real_value: grid is in the Fourier domain");
+  if (!in_box(g, i, j, k, ext)) {
+    *v = kRMissing;
+    return CG_OK;
+  }
+  const size_t idx = (size_t)i + (size_t)g->rnxp * ((size_t)j + (size_t)g->nyp * k);
+  return copy_sync(g, v, g->d + idx, sizeof(float), hipMemcpyDeviceToHost);
+}
+
+int cg_grid_get_real_trace(cg_grid *g, int i, int j, int add_padding, float *v)
+{
+  if (!g || !v) return CG_EINVAL;
+  if (g->transformed) return cg::fail(g->ctx, CG_ESTATE, "get_real_trace: grid is in the Fourier domain");
+  const int n = add_padding ? g->nzp : g->nz;
+  // getRealValue(i, j, k) tests the unpadded box, so the padding part of a padded trace and every
+  // sample of a column outside the box read RMISSING
+  for (int k = 0; k < n; ++k) v[k] = kRMissing;
+  if (!in_box(g, i, j, 0, 0)) return CG_OK;
+  DeviceGuard dg(g->ctx->device);
+  const size_t pitch = sizeof(float) * (size_t)g->rnxp * g->nyp;
+  hipError_t e = hipMemcpy2DAsync(v, sizeof(float), g->d + (size_t)i + (size_t)g->rnxp * j, pitch, sizeof(float),
+                                  (size_t)g->nz, hipMemcpyDeviceToHost, g->ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(g->ctx->stream);
+  return cg::hip_fail(g->ctx, e, "get_real_trace");
+}
+
 int cg_fft_r2c_inplace(cg_grid *g)
 {
   if (!g) return CG_EINVAL;

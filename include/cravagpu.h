@@ -129,6 +129,14 @@ int cg_grid_download_complex(cg_grid *grid, float *host_cvalue);
  * ext_simbox = 0 the range is the unpadded nx*ny*nz box, else the padded box. */
 int cg_grid_get_complex(cg_grid *grid, int i, int j, int k, int ext_simbox, float value[2]);
 int cg_grid_set_complex(cg_grid *grid, int i, int j, int k, int ext_simbox, const float value[2]);
+/* getRealValue (fftgrid.cpp:1237-1263) in the real domain: RMISSING outside the box (ext_simbox = 0)
+ * or the padded box; SeismicParametersHolder::getOrigin reads (0, 0, 0) with it. */
+int cg_grid_get_real_value(cg_grid *grid, int i, int j, int k, int ext_simbox, float *value);
+/* getRealTrace (fftgrid.cpp:1266-1283): value[k] = getRealValue(i, j, k) for k < nz, or k < nzp
+ * with add_padding (whose padding part then reads RMISSING, as getRealValue tests the unpadded
+ * box); SeismicParametersHolder::createPostCov00 (seismicparametersholder.cpp:1056-1067) is the
+ * trace at (0, 0).  Real domain. */
+int cg_grid_get_real_trace(cg_grid *grid, int i, int j, int add_padding, float *value);
 
 /* FFTGrid::fftInPlace (fftgrid.cpp:1587-1612): DATA/PARAMETER are scaled by 1/sqrt(N) before
  * the forward transform, COVARIANCE is not.  Sign -1, FFTW pass order x (r2c), y, z. */

@@ -228,3 +228,23 @@ def test_synthetic_seismic_matches_oracle(ctx, dims, wavelet):
     want = O.synthetic_seismic(vp, vs, rho, w, A, nzp)
     for g, o in zip(got, want):
         assert rel_l2(g, o) < 1e-5
+
+
+def test_real_accessors(ctx):
+    """getRealValue / getRealTrace (getOrigin and createPostCov00 read through them): box values,
+    RMISSING outside the box, and the padding part of a padded trace reads RMISSING as well."""
+    dims = (7, 5, 6, 10, 8, 9)
+    nx, ny, nz, nxp, nyp, nzp = dims
+    rv = _seismic(dims, 41)
+    g = _grid(ctx, dims, C.COVARIANCE)
+    g.upload_real(rv)
+    assert g.getRealValue(0, 0, 0) == rv[0, 0, 0]
+    assert g.getRealValue(3, 2, 4) == rv[4, 2, 3]
+    assert g.getRealValue(8, 2, 4) == -99999.0 and g.getRealValue(8, 2, 4, extSimbox=True) == rv[4, 2, 8]
+    assert g.getRealValue(-1, 0, 0, extSimbox=True) == -99999.0
+    assert np.array_equal(g.getRealTrace(0, 0), rv[:nz, 0, 0])
+    assert np.array_equal(g.getRealTrace(4, 3), rv[:nz, 3, 4])
+    t = g.getRealTrace(4, 3, add_padding=True)
+    assert len(t) == nzp and np.array_equal(t[:nz], rv[:nz, 3, 4]) and np.all(t[nz:] == -99999.0)
+    assert np.all(g.getRealTrace(nx, 0) == -99999.0)
+    g.close()
