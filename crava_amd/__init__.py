@@ -86,6 +86,7 @@ def lib():
             "cg_fft_r2c_inplace": [_vp], "cg_fft_c2r_inplace": [_vp],
             "cg_grid_add": [_vp, _vp], "cg_grid_multiply_by_scalar": [_vp, f], "cg_grid_fill_constant": [_vp, f],
             "cg_posterior": [_vp, ctypes.POINTER(PostArgs)], "cg_simulate": [_vp, ctypes.POINTER(SimArgs)],
+            "cg_simulate_pair": [_vp, ctypes.POINTER(SimArgs), _vp * 3],
             "cg_grid_fill_noise_counter": [_vp, u64, ctypes.c_uint32],
             "cg_grid_fill_noise_reference": [_vp, ctypes.POINTER(ctypes.c_uint32)],
             "cg_build_kz_tables": [i, i, d, i, _f_p, _f_p, f, f, f, _f_p, _f_p, _f_p, _f_p, _i_p],
@@ -522,8 +523,10 @@ def posterior(ctx, tables, mean, data, cov, prior_mode=PRIOR_SEPARABLE, rxy=None
     return tuple(int(x) for x in stats)
 
 
-def simulate(ctx, cov, mean, out, seed=0, realisation=0, noise_source=NOISE_COUNTER, stats=False):
-    """One posterior realisation into out[3] (real domain).  Returns Cholesky failures."""
+def simulate(ctx, cov, mean, out, seed=0, realisation=0, noise_source=NOISE_COUNTER, stats=False, next_out=None):
+    """One posterior realisation into out[3] (real domain), or with next_out two: realisations
+    `realisation` (into out) and `realisation` + 1 (into next_out), coloured together where the
+    fused colouring applies (cg_simulate_pair).  Returns Cholesky failures (of both)."""
     a = SimArgs()
     for g in range(6):
         a.cov[g] = cov[g]._h
@@ -535,7 +538,11 @@ def simulate(ctx, cov, mean, out, seed=0, realisation=0, noise_source=NOISE_COUN
     a.realisation = realisation
     st = np.zeros(1, dtype=np.uint64)
     a.stats = _ptr(st, ctypes.c_uint64) if stats else None
-    ctx.check(lib().cg_simulate(ctx._h, ctypes.byref(a)), "cg_simulate")
+    if next_out is None:
+        ctx.check(lib().cg_simulate(ctx._h, ctypes.byref(a)), "cg_simulate")
+    else:
+        nx = (_vp * 3)(*[g._h for g in next_out])
+        ctx.check(lib().cg_simulate_pair(ctx._h, ctypes.byref(a), nx), "cg_simulate_pair")
     return int(st[0])
 
 

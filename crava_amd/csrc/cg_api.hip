@@ -1226,7 +1226,9 @@ int cg_posterior(cg_ctx *ctx, const cg_post_args *a)
 }
 
 // ---------------------------------------------------------------------------- simulation
-int cg_simulate(cg_ctx *ctx, const cg_sim_args *a)
+// One realisation (next null), or realisations r and r + 1 of one posterior (next: the second's
+// output grids), coloured in one fused launch where the length has one, else one at a time.
+static int simulate_impl(cg_ctx *ctx, const cg_sim_args *a, cg_grid *const *next)
 {
   if (!ctx || !a) return CG_EINVAL;
   const cg_grid *ref = a->cov[0];
@@ -1256,6 +1258,13 @@ int cg_simulate(cg_ctx *ctx, const cg_sim_args *a)
   }
   if (a->noise_source != CG_NOISE_COUNTER && a->noise_source != CG_NOISE_GRIDS)
     return cg::fail(ctx, CG_EINVAL, "simulate: unknown noise source");
+  if (next) {
+    for (int g = 0; g < 3; ++g) {
+      if (!next[g] || !same_shape(next[g], ref)) return cg::fail(ctx, CG_EINVAL, "simulate_pair: bad output grid");
+      for (int h = 0; h < 3; ++h)
+        if (next[g] == a->out[h]) return cg::fail(ctx, CG_EINVAL, "simulate_pair: output grids overlap");
+    }
+  }
   DeviceGuard dg(ctx->device);
   cg::ColourParams p;
   std::memset(&p, 0, sizeof(p));
@@ -1278,17 +1287,41 @@ int cg_simulate(cg_ctx *ctx, const cg_sim_args *a)
   // Counter noise with a compiled z plan: the colouring runs inside the inverse z pass, so the
   // coloured spectra never make a round trip through HBM.
   bool z_done = false;
-  if (!p.from_grids && ctx->fft_engine == CG_FFT_AUTO && p.nzp > 1)
-    z_done = cg::fixed_colour_zinv(ctx, p, p.nzp, (long)p.cnxp * p.nyp, &rc) != 0;
-  if (!z_done) rc = cg::colour(ctx, p);
-  if (rc) return rc;
-  for (int g = 0; g < 3; ++g) {
-    a->out[g]->cubetype = CG_CT_PARAMETER;
-    a->out[g]->transformed = 1;
-    const float *add = spectral == 0 && a->mean[g] ? a->mean[g]->d : nullptr;
-    rc = z_done ? cg::grid_fft_inverse_xy(a->out[g], add) : cg::grid_fft_inverse(a->out[g], add);
+  const bool fusable = !p.from_grids && ctx->fft_engine == CG_FFT_AUTO && p.nzp > 1;
+  if (next) {
+    // both realisations in one launch, or one after the other
+    for (int g = 0; g < 3; ++g) p.out2[g] = reinterpret_cast<float2 *>(next[g]->d);
+    p.stream1 = 4u * (a->realisation + 1u);
+    if (!(fusable && cg::fixed_colour_zinv2(ctx, p, p.nzp, (long)p.cnxp * p.nyp, &rc) != 0)) {
+      uint64_t s1 = 0, s2 = 0;
+      cg_sim_args b = *a;
+      b.stats = a->stats ? &s1 : nullptr;
+      rc = simulate_impl(ctx, &b, nullptr);
+      if (rc) return rc;
+      for (int g = 0; g < 3; ++g) b.out[g] = next[g];
+      b.realisation = a->realisation + 1u;
+      b.stats = a->stats ? &s2 : nullptr;
+      rc = simulate_impl(ctx, &b, nullptr);
+      if (!rc && a->stats) a->stats[0] = s1 + s2;
+      return rc;
+    }
     if (rc) return rc;
-    a->out[g]->transformed = 0;
+    z_done = true;
+  } else {
+    if (fusable) z_done = cg::fixed_colour_zinv(ctx, p, p.nzp, (long)p.cnxp * p.nyp, &rc) != 0;
+    if (!z_done) rc = cg::colour(ctx, p);
+    if (rc) return rc;
+  }
+  for (int s = 0; s < (next ? 2 : 1); ++s) {
+    for (int g = 0; g < 3; ++g) {
+      cg_grid *o = s ? next[g] : a->out[g];
+      o->cubetype = CG_CT_PARAMETER;
+      o->transformed = 1;
+      const float *add = spectral == 0 && a->mean[g] ? a->mean[g]->d : nullptr;
+      rc = z_done ? cg::grid_fft_inverse_xy(o, add) : cg::grid_fft_inverse(o, add);
+      if (rc) return rc;
+      o->transformed = 0;
+    }
   }
   if (a->stats) {
     uint64_t h = 0;
@@ -1298,6 +1331,14 @@ int cg_simulate(cg_ctx *ctx, const cg_sim_args *a)
     a->stats[0] = h;
   }
   return CG_OK;
+}
+
+int cg_simulate(cg_ctx *ctx, const cg_sim_args *a) { return simulate_impl(ctx, a, nullptr); }
+
+int cg_simulate_pair(cg_ctx *ctx, const cg_sim_args *a, cg_grid *next_out[3])
+{
+  if (!next_out) return CG_EINVAL;
+  return simulate_impl(ctx, a, next_out);
 }
 
 // ---------------------------------------------------------------------------- forward modelling

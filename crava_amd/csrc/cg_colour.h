@@ -240,4 +240,58 @@ __device__ __forceinline__ bool colour_cell(const cg::ColourParams &p, long idx,
   return failed;
 }
 
+// colour_cell for two realisations of one posterior (counter noise only): the covariance and
+// mean are read and the 3x3 factor computed once; each realisation's arithmetic is colour_cell's,
+// operation for operation, so the pair equals two single cells bit for bit.
+__device__ __forceinline__ bool colour_cell2(const cg::ColourParams &p, long idx, float2 (&o)[3], float2 (&o2)[3])
+{
+  const float2 c00 = cg::gld(p.cov[0] + idx), c11 = cg::gld(p.cov[1] + idx), c22 = cg::gld(p.cov[2] + idx);
+  const float2 c01 = cg::gld(p.cov[3] + idx), c02 = cg::gld(p.cov[4] + idx), c12 = cg::gld(p.cov[5] + idx);
+  float2 mv[3];
+#pragma unroll
+  for (int g = 0; g < 3; ++g) mv[g] = p.mean[g] ? cg::gld(p.mean[g] + idx) : make_float2(0.f, 0.f);
+  c32 m[6] = {{c00.x, c00.y}, {c01.x, -c01.y}, {c11.x, c11.y}, {c02.x, -c02.y}, {c12.x, -c12.y}, {c22.x, c22.y}};
+  long src;
+  int kind;
+  if (p.csize <= 0xFFFFFFFFl) {
+    uint32_t s32;
+    kind = noise_source32((uint32_t)idx, p, &s32);
+    src = s32;
+  } else {
+    kind = noise_source(idx, p.cnxp, p.nyp, p.nzp, &src);
+  }
+  float2 z[3], z2[3];
+  cb_normal2x3(p.seed, p.stream0, (uint64_t)src, z);
+  cb_normal2x3(p.seed, p.stream1, (uint64_t)src, z2);
+#pragma unroll
+  for (int g = 0; g < 3; ++g) {
+    z[g] = noise_value(kind, z[g]);
+    z2[g] = noise_value(kind, z2[g]);
+    o[g] = make_float2(0.f, 0.f);
+    o2[g] = make_float2(0.f, 0.f);
+  }
+  const bool failed = chol3(m) != 0;
+  if (!failed) {
+    const int row[3] = {0, 1, 3};
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int b = 0; b <= a; ++b) {
+        const c32 l = m[row[a] + b];
+        o[a].x += l.re * z[b].x - l.im * z[b].y;
+        o[a].y += l.re * z[b].y + l.im * z[b].x;
+        o2[a].x += l.re * z2[b].x - l.im * z2[b].y;
+        o2[a].y += l.re * z2[b].y + l.im * z2[b].x;
+      }
+  }
+#pragma unroll
+  for (int g = 0; g < 3; ++g) {
+    o[g].x += mv[g].x;
+    o[g].y += mv[g].y;
+    o2[g].x += mv[g].x;
+    o2[g].y += mv[g].y;
+  }
+  return failed;
+}
+
 }  // namespace cgcol

@@ -894,6 +894,67 @@ k_colour_zinv(cg::ColourParams one, const cg::ColourItem *__restrict__ items, in
   IO::from_lds(buf + g * N * TI + c, t, tw, sink);
 }
 
+// Two realisations of one posterior per workgroup (k_colour_zinv2): six column sets in LDS.
+template <int N>
+constexpr int colour_tile2()
+{
+  constexpr int T = Plan<N>::TPS;
+  int ti = 16;
+  while (ti > 1 && ((size_t)(N + 6 * (size_t)N * ti) * 8 > kLds || 6 * ti * T > 1024)) ti /= 2;
+  return ti;
+}
+
+// k_colour_zinv for realisations stream0 and stream1 of one posterior: the covariance is read
+// and factorised once per cell (colour_cell2), both realisations' columns are transformed from
+// LDS; grid g < 3 goes to out[g], g >= 3 to out2[g - 3].
+template <int N, int TI>
+__global__ void __launch_bounds__(6 * TI *Plan<N>::TPS)
+k_colour_zinv2(cg::ColourParams p, const float2 *__restrict__ twg)
+{
+  using IO = typename Plan<N>::template IO<+1, TI>;
+  constexpr int TPS = Plan<N>::TPS;
+  constexpr int NT = 6 * TI * TPS;
+  constexpr int CELLS = N * TI;
+  constexpr int PER = (CELLS + NT - 1) / NT;
+  extern __shared__ float2 smem[];
+  float2 *tw = smem;
+  float2 *buf = smem + N;  // grid g (0..5), plane z, column c at (g N + z) TI + c
+  const int tid = threadIdx.x;
+  const long q0 = (long)cg_xcd_remap(blockIdx.x, gridDim.x) * TI;
+  load_tw(tw, twg, N, tid, NT);
+  const long plane = (long)p.cnxp * p.nyp;
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const int e = tid + i * NT;
+    const int c = e % TI, z = e / TI;
+    const bool live = e < CELLS && q0 + c < plane;
+    float2 o[3] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f), make_float2(0.f, 0.f)};
+    float2 o2[3] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f), make_float2(0.f, 0.f)};
+    bool failed = false;
+    if (live) failed = cgcol::colour_cell2(p, q0 + c + (long)z * plane, o, o2);
+    if (p.fails) {  // a failed cell zeroes the noise of both realisations: counted twice
+      const unsigned long long msk = __ballot(failed);
+      if (failed && (int)__lane_id() == __ffsll((long long)msk) - 1)
+        atomicAdd(p.fails, 2ull * (unsigned long long)__popcll(msk));
+    }
+    if (e < CELLS) {
+#pragma unroll
+      for (int g = 0; g < 3; ++g) {
+        buf[(g * N + z) * TI + c] = o[g];
+        buf[((g + 3) * N + z) * TI + c] = o2[g];
+      }
+    }
+  }
+  lds_sync();
+  const int g = tid / (TI * TPS), r = tid % (TI * TPS), c = r % TI, t = r / TI;
+  const bool live = q0 + c < plane;
+  float2 *col = (g < 3 ? p.out[g] : p.out2[g - 3]) + (live ? q0 + c : 0);
+  auto sink = [&](int e, float2 x) {
+    if (live) cg::gst(col + (long)e * plane, x);
+  };
+  IO::from_lds(buf + g * N * TI + c, t, tw, sink);
+}
+
 // Wavefronts of the fused kernel one CU holds (LDS-limited).
 template <int N>
 constexpr int colour_waves()
@@ -919,6 +980,18 @@ int colour_zinv_launch(cg_ctx *ctx, const cg::ColourParams *one, const cg::Colou
   else
     k_colour_zinv<N, TI, false><<<nblocks, 3 * TI * Plan<N>::TPS, lds, ctx->stream>>>(o, nullptr, 0, tw);
   return cg::hip_fail(ctx, hipGetLastError(), "k_colour_zinv");
+}
+
+// Two realisations of one posterior (p.out / p.stream0 and p.out2 / p.stream1), single launch.
+template <int N>
+int colour_zinv2_launch(cg_ctx *ctx, const cg::ColourParams &p, long plane, const float2 *tw)
+{
+  constexpr int TI = colour_tile2<N>();
+  const size_t lds = sizeof(float2) * (N + 6 * (size_t)N * TI);
+  const unsigned nblk = (unsigned)((plane + TI - 1) / TI);
+  if (nblk == 0) return CG_OK;
+  k_colour_zinv2<N, TI><<<nblk, 6 * TI * Plan<N>::TPS, lds, ctx->stream>>>(p, tw);
+  return cg::hip_fail(ctx, hipGetLastError(), "k_colour_zinv2");
 }
 
 template <int N, int V>

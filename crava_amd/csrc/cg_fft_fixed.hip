@@ -171,6 +171,22 @@ int fixed_colour_zinv(cg_ctx *ctx, const ColourParams &p, int nzp, long plane, i
   return 1;
 }
 
+int fixed_colour_zinv2(cg_ctx *ctx, const ColourParams &p, int nzp, long plane, int *rc)
+{
+  const FixedEntry *e = p.from_grids ? nullptr : fixed_colour_entry(ctx, nzp, plane);
+  if (!e || !e->cz2) return 0;
+  const float2 *tw = twiddle_table(ctx, nzp);
+  if (!tw) {
+    *rc = CG_ENOMEM;
+    return 1;
+  }
+  hipEvent_t ev;
+  launch_begin(ctx, CG_K_NOISE, &ev);
+  *rc = e->cz2(ctx, p, plane, tw);
+  launch_end(ctx, CG_K_NOISE, ev);
+  return 1;
+}
+
 int fixed_colour_zinv_list(cg_ctx *ctx, const FixedEntry *e, const ColourItem *dev_items, int nitems,
                            unsigned nblocks)
 {

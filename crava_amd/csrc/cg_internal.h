@@ -71,6 +71,7 @@ struct ColourParams;
 struct ColourItem;
 typedef int (*ColourZFn)(cg_ctx *, const ColourParams *one, const ColourItem *items, int nitems, unsigned nblocks,
                          const float2 *tw);
+typedef int (*ColourZ2Fn)(cg_ctx *, const ColourParams &p, long plane, const float2 *tw);
 struct FixedEntry {
   int kind, n;
   int tile[2];  // the two tile variants (equal when the length has only one)
@@ -80,6 +81,7 @@ struct FixedEntry {
   int cz_waves;  // wavefronts of the fused kernel resident per CU
   int cz_tile;   // its columns per workgroup
   int tune;      // 0: variant 0 is kept (the autotuner's timing on scratch grids misjudges it)
+  ColourZ2Fn cz2;  // strided lengths: two realisations per launch (fused colouring)
 };
 // Tile variant for a pass geometry: the autotuner's choice (cg_autotune) for this exact
 // geometry, or the context's forced variant, else 0.  `geom` identifies the pass: strided
@@ -190,6 +192,8 @@ int autotune_geometry(cg_ctx *ctx, int nxp, int nyp, int nzp);
 // faster choice for this geometry (autotuned, else when it keeps at least 12 wavefronts per CU
 // resident; with fewer the colouring's loads are poorly hidden).
 int fixed_colour_zinv(cg_ctx *ctx, const ColourParams &p, int nzp, long plane, int *rc);
+// The same for two realisations of one posterior (p.out2 / p.stream1 set): one launch.
+int fixed_colour_zinv2(cg_ctx *ctx, const ColourParams &p, int nzp, long plane, int *rc);
 // The compiled entry whose fused colouring a realisation of nzp planes of `plane` cells would use
 // (same choice as fixed_colour_zinv), or null.
 const FixedEntry *fixed_colour_entry(cg_ctx *ctx, int nzp, long plane);

@@ -33,6 +33,9 @@ struct ColourParams {
   uint64_t seed;
   uint32_t stream0;       // streams stream0, stream0+1, stream0+2
   unsigned long long *fails;
+  // the second realisation of a pair (k_colour_zinv2): outputs and streams stream1 .. stream1+2
+  float2 *out2[3];
+  uint32_t stream1;
 };
 
 struct ForwardParams {

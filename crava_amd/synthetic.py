@@ -179,8 +179,17 @@ class Problem:
                        lam2d=self.rxy, want_residual=False, precision=precision, cond_limit=1e6, stats=stats)
         for g in self.mean_w:
             g.invFFTInPlace()
-        for r in range(R):  # realisations beyond the kept sets are streamed through them
-            simulate(self.ctx, self.cov, self.mean_w, self.out[r % len(self.out)], seed=self.s["seed"], realisation=r)
+        # realisations beyond the kept sets are streamed through them; with two or more sets,
+        # consecutive realisations are coloured in pairs (one read of the covariance per pair)
+        n, r = len(self.out), 0
+        while r < R:
+            if r + 1 < R and n > 1:
+                simulate(self.ctx, self.cov, self.mean_w, self.out[r % n], seed=self.s["seed"], realisation=r,
+                         next_out=self.out[(r + 1) % n])
+                r += 2
+            else:
+                simulate(self.ctx, self.cov, self.mean_w, self.out[r % n], seed=self.s["seed"], realisation=r)
+                r += 1
         return st
 
     def item(self, realisations=None):

@@ -128,3 +128,31 @@ def test_fused_colour_at_production_depths(ctx, nz, nzp):
     ctx.set_fused_colour(-1)
     for g in gc + gmean + gout:
         g.close()
+
+
+@pytest.mark.parametrize("fused,nz,nzp", [(1, 256, 320), (-1, 256, 320), (0, 256, 320), (1, 20, None)])
+def test_realisation_pair_equals_two_singles(ctx, fused, nz, nzp):
+    """cg_simulate_pair: realisations r and r + 1 coloured in one launch (fused) or one after the
+    other (unfused), bit for bit the two single calls, failures summed."""
+    ctx.set_fused_colour(fused)
+    p = P.make_problem(11, 7, nz, nzp=nzp, seed=9)
+    print("realisation pair", fused, nz, p["nzp"], flush=True)
+    om, oc = _posterior_cov(p)
+    nxp = p["nxp"]
+    mean_real = np.stack([O.grid_ifft(m, nxp, O.PARAMETER, np.float32) for m in om])
+    gc, gmean = _grids(ctx, p, 6, C.COVARIANCE), _grids(ctx, p, 3, C.PARAMETER)
+    outs = [_grids(ctx, p, 3, C.PARAMETER) for _ in range(4)]
+    for g in range(6):
+        gc[g].upload_complex(oc[g])
+    for g in range(3):
+        gmean[g].upload_real(mean_real[g])
+    fp = C.simulate(ctx, gc, gmean, outs[0], seed=5, realisation=2, stats=True, next_out=outs[1])
+    f1 = C.simulate(ctx, gc, gmean, outs[2], seed=5, realisation=2, stats=True)
+    f2 = C.simulate(ctx, gc, gmean, outs[3], seed=5, realisation=3, stats=True)
+    ctx.set_fused_colour(-1)
+    assert fp == f1 + f2
+    for a, b in ((outs[0], outs[2]), (outs[1], outs[3])):
+        for ga, gb in zip(a, b):
+            assert np.array_equal(ga.download_real().view(np.uint32), gb.download_real().view(np.uint32))
+    for g in gc + gmean + [g for o in outs for g in o]:
+        g.close()
