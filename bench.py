@@ -55,7 +55,9 @@ def kernel_bytes(w, nt, R, n_inv_planes):
         # inverted cells: read 3 + nt spectra, write 3 means + 6 covariance entries;
         # pass-through cells: write the 6 prior covariance entries
         C.K_POSTERIOR: n_inv_planes * plane * (3 + nt + 9) * 8 + (w["nzp"] - n_inv_planes) * plane * 6 * 8,
-        C.K_NOISE: R * 9 * G,  # colouring: read 6 covariance, write 3 (noise generated in registers)
+        # colouring: read 6 covariance, write 3 per realisation (noise generated in registers); the
+        # step colours realisations in pairs, which read the covariance once for both
+        C.K_NOISE: (R // 2) * 12 * G + (R % 2) * 9 * G,
     }
 
 
