@@ -377,8 +377,9 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=2)
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (the slab-decomposed "
-              "all-to-all design is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path (independent requests would "
+              "run one per GPU, a single cube as z slabs with an all-to-all transpose); run with "
+              "--gpus 1", file=sys.stderr)
         sys.exit(2)
     w = dict(WORKLOAD)
     R = args.realisations
