@@ -297,10 +297,11 @@ def bench_input_padding(ctx, inv, w, reps=5):
 def bench_large(ctx, steps=2):
     """The large single cube with simulation: 1024 x 1024 x 320 padded to 1152 x 1152 x 384, 6
     angles 0-50 deg, Ormsby 5-10-60-80 Hz wavelet, 1500 m lateral range, 20 posterior realisations
-    (streamed through one set of three output grids), covariance kept as the 6 Hermitian spectra."""
+    (streamed through two sets of three output grids, coloured in pairs), covariance kept as the 6
+    Hermitian spectra."""
     sp = S.spec(1024, 1024, 320, tuple(float(a) for a in range(0, 60, 10)), 25.0, seed=20261205, nxp=1152,
                 nyp=1152, nzp=384, lateral_range=1500.0, ormsby_hz=(5.0, 10.0, 60.0, 80.0), realisations=20,
-                out_sets=1)
+                out_sets=2)
     t0 = time.perf_counter()
     prob = S.Problem(ctx, sp)
     ctx.autotune(1152, 1152, 384)

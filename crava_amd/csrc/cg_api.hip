@@ -1292,7 +1292,11 @@ static int simulate_impl(cg_ctx *ctx, const cg_sim_args *a, cg_grid *const *next
     // both realisations in one launch, or one after the other
     for (int g = 0; g < 3; ++g) p.out2[g] = reinterpret_cast<float2 *>(next[g]->d);
     p.stream1 = 4u * (a->realisation + 1u);
-    if (!(fusable && cg::fixed_colour_zinv2(ctx, p, p.nzp, (long)p.cnxp * p.nyp, &rc) != 0)) {
+    if (fusable && cg::fixed_colour_zinv2(ctx, p, p.nzp, (long)p.cnxp * p.nyp, &rc) != 0) {
+      z_done = true;
+    } else if (!p.from_grids) {
+      rc = cg::colour2(ctx, p);  // unfused, then the z passes of all six grids
+    } else {
       uint64_t s1 = 0, s2 = 0;
       cg_sim_args b = *a;
       b.stats = a->stats ? &s1 : nullptr;
@@ -1306,7 +1310,6 @@ static int simulate_impl(cg_ctx *ctx, const cg_sim_args *a, cg_grid *const *next
       return rc;
     }
     if (rc) return rc;
-    z_done = true;
   } else {
     if (fusable) z_done = cg::fixed_colour_zinv(ctx, p, p.nzp, (long)p.cnxp * p.nyp, &rc) != 0;
     if (!z_done) rc = cg::colour(ctx, p);

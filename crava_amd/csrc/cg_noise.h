@@ -51,6 +51,8 @@ int add_white_noise(cg_ctx *ctx, float *x, int nxp, int rnxp, long rows, float s
 int sum_squares(cg_ctx *ctx, const float *x, int nx, int ny, int nz, int rnxp, int nyp, double *dout);
 int fill_noise(cg_ctx *ctx, float2 *out, long csize, int cnxp, int nyp, int nzp, uint64_t seed, uint32_t stream);
 int colour(cg_ctx *ctx, const ColourParams &p);
+// Two realisations of one posterior (out2 / stream1 set, counter noise), unfused.
+int colour2(cg_ctx *ctx, const ColourParams &p);
 // Fills the divisors of p from p.cnxp and p.nyp.
 inline void colour_dims(ColourParams &p)
 {

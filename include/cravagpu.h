@@ -288,10 +288,11 @@ typedef struct {
 int cg_simulate(cg_ctx *ctx, const cg_sim_args *args);
 /* Realisations args->realisation and args->realisation + 1 of the same posterior (no reference
  * counterpart: AVOInversion::simulate loops over realisations one at a time): the first into
- * args->out, the second into next_out.  With counter noise and a compiled z plan where the
- * colouring is fused, one launch reads and factorises each wavenumber's covariance once for both
- * (results equal two cg_simulate calls bit for bit); otherwise it makes those two calls.  stats
- * receives the failures of both. */
+ * args->out, the second into next_out.  With counter noise one colouring launch (fused with the
+ * inverse z pass where cg_simulate would fuse it) reads and factorises each wavenumber's
+ * covariance once for both; results equal two cg_simulate calls bit for bit.  With noise grids it
+ * makes those two calls (next_out then holds the second noise).  stats receives the failures of
+ * both. */
 int cg_simulate_pair(cg_ctx *ctx, const cg_sim_args *args, cg_grid *next_out[3]);
 
 /* FFTGrid::fillInComplexNoise layout (fftgrid.cpp:997-1045) with the counter-based generator:
