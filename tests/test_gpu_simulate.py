@@ -156,3 +156,44 @@ def test_realisation_pair_equals_two_singles(ctx, fused, nz, nzp):
             assert np.array_equal(ga.download_real().view(np.uint32), gb.download_real().view(np.uint32))
     for g in gc + gmean + [g for o in outs for g in o]:
         g.close()
+
+
+@pytest.mark.parametrize("fused", [1, 0])
+def test_realisation_pair_spectral_means_and_noise_grids(ctx, fused):
+    """cg_simulate_pair with the means handed over as spectra (added to both coloured spectra)
+    equals two single calls bit for bit; with noise grids (the second in next_out) it makes the
+    two single calls itself and matches the oracle."""
+    ctx.set_fused_colour(fused)
+    p = P.make_problem(11, 7, 256, nzp=320, seed=13)
+    print("realisation pair, spectral means / noise grids", fused, flush=True)
+    om, oc = _posterior_cov(p)
+    nxp, nyp, nzp = p["nxp"], p["nyp"], p["nzp"]
+    gc, gmean = _grids(ctx, p, 6, C.COVARIANCE), _grids(ctx, p, 3, C.PARAMETER)
+    outs = [_grids(ctx, p, 3, C.PARAMETER) for _ in range(4)]
+    for g in range(6):
+        gc[g].upload_complex(oc[g])
+    for g in range(3):
+        gmean[g].upload_complex(om[g])
+    C.simulate(ctx, gc, gmean, outs[0], seed=8, realisation=6, next_out=outs[1])
+    C.simulate(ctx, gc, gmean, outs[2], seed=8, realisation=6)
+    C.simulate(ctx, gc, gmean, outs[3], seed=8, realisation=7)
+    for a, b in ((outs[0], outs[2]), (outs[1], outs[3])):
+        for ga, gb in zip(a, b):
+            assert np.array_equal(ga.download_real().view(np.uint32), gb.download_real().view(np.uint32))
+    # noise grids: the reference stream for realisation 0 in outs[0], realisation 1 in outs[1]
+    st = np.array([23665], dtype=np.uint32)
+    noise = [np.stack([O.fill_complex_noise(nxp, nyp, nzp, st) for _ in range(3)]) for _ in range(2)]
+    mean_real = np.stack([O.grid_ifft(m, nxp, O.PARAMETER, np.float32) for m in om])
+    for g in range(3):
+        gmean[g].upload_real(mean_real[g])
+        outs[0][g].upload_complex(noise[0][g])
+        outs[1][g].upload_complex(noise[1][g])
+    C.simulate(ctx, gc, gmean, outs[0], noise_source=C.NOISE_GRIDS, next_out=outs[1])
+    ctx.set_fused_colour(-1)
+    for s in range(2):
+        z, _ = O.colour(oc, noise[s], "f32")
+        want = np.stack([O.grid_ifft(z[g], nxp, O.PARAMETER) + mean_real[g] for g in range(3)])
+        got = np.stack([g.download_real() for g in outs[s]])
+        assert rel_l2(got - mean_real, want - mean_real) < 1e-5
+    for g in gc + gmean + [g for o in outs for g in o]:
+        g.close()
