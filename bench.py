@@ -49,15 +49,17 @@ def kernel_bytes(w, nt, R, n_inv_planes):
     plane = (w["nxp"] // 2 + 1) * w["nyp"]
     nfwd, ninv = 3 + nt, 3 + 3 * R
     return {
-        C.K_FFT_X: nfwd * (Nr + G) + 3 * (G + Nr) + 3 * R * (G + 2 * Nr),
+        # forward: real cube in, spectrum out; inverse (means, realisations): spectrum in, real out
+        # (the realisations get the means as spectra in the colouring, so no add in the x pass)
+        C.K_FFT_X: nfwd * (Nr + G) + 3 * (G + Nr) + 3 * R * (G + Nr),
         C.K_FFT_Y: (nfwd + ninv) * 2 * G,
         C.K_FFT_Z: (nfwd + ninv) * 2 * G,
         # inverted cells: read 3 + nt spectra, write 3 means + 6 covariance entries;
         # pass-through cells: write the 6 prior covariance entries
         C.K_POSTERIOR: n_inv_planes * plane * (3 + nt + 9) * 8 + (w["nzp"] - n_inv_planes) * plane * 6 * 8,
-        # colouring: read 6 covariance, write 3 per realisation (noise generated in registers); the
-        # step colours realisations in pairs, which read the covariance once for both
-        C.K_NOISE: (R // 2) * 12 * G + (R % 2) * 9 * G,
+        # colouring: read 6 covariance and 3 mean spectra, write 3 per realisation (noise generated
+        # in registers); the step colours realisations in pairs, which read both once
+        C.K_NOISE: (R // 2) * 15 * G + (R % 2) * 12 * G,
     }
 
 

@@ -166,10 +166,10 @@ class Problem:
         ctx.sync()
 
     def step(self, realisations=None, stats=False, precision=PREC_MIXED):
-        """The plain per-request path: forward FFTs, posterior, inverse FFTs of the means,
-        realisations (the real-domain means are added in the inverse x pass: adding the mean
-        spectra in the colouring instead measured slower, the fused colouring being the more
-        bandwidth-starved kernel)."""
+        """The plain per-request path: forward FFTs, posterior, realisations, inverse FFTs of the
+        means.  The realisations run while the means are still spectra, which the colouring adds
+        (one read per realisation pair, instead of one per realisation in the inverse x pass:
+        measured faster with paired colouring, slower when realisations were coloured singly)."""
         R = self.R if realisations is None else realisations
         for g, src in zip(self.mean_w, self.mean_in):
             g.fft_from(src)
@@ -177,8 +177,6 @@ class Problem:
             g.fft_from(src)
         st = posterior(self.ctx, self.tables, self.mean_w, self.data_w, self.cov, rxy=self.rxy, rt=self.rt,
                        lam2d=self.rxy, want_residual=False, precision=precision, cond_limit=1e6, stats=stats)
-        for g in self.mean_w:
-            g.invFFTInPlace()
         # realisations beyond the kept sets are streamed through them; with two or more sets,
         # consecutive realisations are coloured in pairs (one read of the covariance per pair)
         n, r = len(self.out), 0
@@ -190,6 +188,8 @@ class Problem:
             else:
                 simulate(self.ctx, self.cov, self.mean_w, self.out[r % n], seed=self.s["seed"], realisation=r)
                 r += 1
+        for g in self.mean_w:
+            g.invFFTInPlace()
         return st
 
     def item(self, realisations=None):
