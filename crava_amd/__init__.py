@@ -95,7 +95,7 @@ def lib():
             "cg_fft_r2c": [_vp, _vp],
             "cg_forward_model": [_vp, i, _f_p, _vp * 3, _vp * 8],
             "cg_grid_add_white_noise": [_vp, f, u64, ctypes.c_uint32],
-            "cg_grid_sum_squares": [_vp, i, _d_p],
+            "cg_grid_sum_squares": [_vp, i, _d_p], "cg_grid_data_variance": [_vp, _f_p, _u64_p],
             "cg_ctx_set_fft_engine": [_vp, i], "cg_ctx_set_fft_variant": [_vp, i],
             "cg_ctx_set_fused_colour": [_vp, i],
             "cg_autotune": [_vp, i, i, i],
@@ -293,6 +293,12 @@ class FFTGrid:
         v = ctypes.c_double()
         self.ctx.check(lib().cg_grid_sum_squares(self._h, int(extSimbox), ctypes.byref(v)), "sum_squares")
         return v.value
+
+    def data_variance(self):
+        """ModelAVODynamic::ComputeDataVariance for this stack: (variance as float, non-zero samples)."""
+        v, n = ctypes.c_float(), ctypes.c_uint64()
+        self.ctx.check(lib().cg_grid_data_variance(self._h, ctypes.byref(v), ctypes.byref(n)), "data_variance")
+        return v.value, n.value
 
     def add(self, other):
         self.ctx.check(lib().cg_grid_add(self._h, other._h), "add")

@@ -1389,13 +1389,42 @@ int cg_grid_sum_squares(cg_grid *g, int ext, double *sum)
   if (g->transformed) return cg::fail(g->ctx, CG_ESTATE, "sum_squares: grid is in the Fourier domain");
   cg_ctx *ctx = g->ctx;
   DeviceGuard dg(ctx->device);
-  double *d = reinterpret_cast<double *>(ctx->counters + 200);  // a double slot past the statistics
-  int rc = cg::sum_squares(ctx, g->d, ext ? g->nxp : g->nx, ext ? g->nyp : g->ny, ext ? g->nzp : g->nz, g->rnxp,
-                           g->nyp, d);
+  const int ny = ext ? g->nyp : g->ny, nz = ext ? g->nzp : g->nz;
+  char *work = static_cast<char *>(cg::table_space(ctx, cg::box_sum_squares_work(ny, nz) + 256));
+  if (!work) return CG_ENOMEM;
+  double *d = reinterpret_cast<double *>(work);
+  int rc = cg::box_sum_squares(ctx, g->d, ext ? g->nxp : g->nx, ny, nz, g->rnxp, g->nyp, false, work + 256, d,
+                               nullptr);
   if (rc) return rc;
   hipError_t e = hipMemcpyAsync(sum, d, sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   return cg::hip_fail(ctx, e, "sum_squares");
+}
+
+int cg_grid_data_variance(cg_grid *g, float *variance, uint64_t *count)
+{
+  if (!g || !variance) return CG_EINVAL;
+  if (g->transformed) return cg::fail(g->ctx, CG_ESTATE, "data_variance: grid is in the Fourier domain");
+  cg_ctx *ctx = g->ctx;
+  DeviceGuard dg(ctx->device);
+  char *work = static_cast<char *>(cg::table_space(ctx, cg::box_sum_squares_work(g->ny, g->nz) + 256));
+  if (!work) return CG_ENOMEM;
+  double *d = reinterpret_cast<double *>(work);
+  unsigned long long *n = reinterpret_cast<unsigned long long *>(work + 64);
+  int rc = cg::box_sum_squares(ctx, g->d, g->nx, g->ny, g->nz, g->rnxp, g->nyp, true, work + 256, d, n);
+  if (rc) return rc;
+  struct {
+    double sum;
+    unsigned long long n;
+  } h;
+  hipError_t e = hipMemcpyAsync(&h.sum, d, sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(&h.n, n, sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) return cg::hip_fail(ctx, e, "data_variance");
+  // modelavodynamic.cpp:612-617: zero when every sample is zero
+  *variance = h.n == 0 ? 0.0f : static_cast<float>(h.sum / static_cast<double>(h.n));
+  if (count) *count = h.n;
+  return CG_OK;
 }
 
 // ---------------------------------------------------------------------------- host setup

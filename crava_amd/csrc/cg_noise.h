@@ -48,7 +48,13 @@ struct ForwardParams {
 
 int forward_model(cg_ctx *ctx, const ForwardParams &p);
 int add_white_noise(cg_ctx *ctx, float *x, int nxp, int rnxp, long rows, float sigma, uint64_t seed, uint32_t stream);
-int sum_squares(cg_ctx *ctx, const float *x, int nx, int ny, int nz, int rnxp, int nyp, double *dout);
+// Sum of squares over the nx * ny * nz box of a real grid in a fixed order (rows in i order, rows
+// of a plane in j order, planes in k order); ref: the data variance's definition (float products,
+// zero samples skipped and counted in *dcount).  work: box_sum_squares_work(ny, nz) bytes of
+// device memory; dout / dcount device memory.
+int box_sum_squares(cg_ctx *ctx, const float *x, int nx, int ny, int nz, int rnxp, int nyp, bool ref, void *work,
+                    double *dout, unsigned long long *dcount);
+size_t box_sum_squares_work(int ny, int nz);
 int fill_noise(cg_ctx *ctx, float2 *out, long csize, int cnxp, int nyp, int nzp, uint64_t seed, uint32_t stream);
 int colour(cg_ctx *ctx, const ColourParams &p);
 // Two realisations of one posterior (out2 / stream1 set, counter noise), unfused.

@@ -44,6 +44,23 @@ def spec(nx, ny, nz, angles, peak_hz, seed, nxp=None, nyp=None, nzp=None, dx=25.
                 out_sets=out_sets)
 
 
+def single_cube_spec(realisations=2):
+    """The production single cube: 512 x 512 x 256 padded to 576 x 576 x 320, dx = dy = 25 m,
+    dt = 4 ms, 5 angles 0-40 deg, a 30 Hz Ricker wavelet scaled U(0.8, 1.2) per angle, S/N U(3, 8)
+    per angle, the default prior (spherical 1000 m lateral, exponential 40 ms temporal)."""
+    return spec(512, 512, 256, (0.0, 10.0, 20.0, 30.0, 40.0), 30.0, seed=20261202, nxp=576, nyp=576, nzp=320,
+                realisations=realisations)
+
+
+def large_cube_spec(realisations=20):
+    """The large cube with simulation: 1024 x 1024 x 320 padded to 1152 x 1152 x 384, 6 angles
+    0-50 deg, an Ormsby 5-10-60-80 Hz wavelet, 1500 m lateral range, 20 posterior realisations
+    streamed through two sets of output grids (coloured in pairs)."""
+    return spec(1024, 1024, 320, tuple(float(a) for a in range(0, 60, 10)), 25.0, seed=20261205, nxp=1152,
+                nyp=1152, nzp=384, lateral_range=1500.0, ormsby_hz=(5.0, 10.0, 60.0, 80.0),
+                realisations=realisations, out_sets=2)
+
+
 def cells(s):
     """Unpadded cells of one problem (the unit of the cells/s throughput)."""
     return s["nx"] * s["ny"] * s["nz"]
@@ -170,15 +187,31 @@ class Problem:
         means.  The realisations run while the means are still spectra, which the colouring adds
         (one read per realisation pair, instead of one per realisation in the inverse x pass:
         measured faster with paired colouring, slower when realisations were coloured singly)."""
-        R = self.R if realisations is None else realisations
+        self.forward()
+        st = self.posterior(stats=stats, precision=precision)
+        self.realisations(realisations)
+        self.inverse_means()
+        return st
+
+    def forward(self):
+        """Forward FFTs of the resident real-domain inputs into the work spectra."""
         for g, src in zip(self.mean_w, self.mean_in):
             g.fft_from(src)
         for g, src in zip(self.data_w, self.seis_in):
             g.fft_from(src)
-        st = posterior(self.ctx, self.tables, self.mean_w, self.data_w, self.cov, rxy=self.rxy, rt=self.rt,
-                       lam2d=self.rxy, want_residual=False, precision=precision, cond_limit=1e6, stats=stats)
-        # realisations beyond the kept sets are streamed through them; with two or more sets,
-        # consecutive realisations are coloured in pairs (one read of the covariance per pair)
+
+    def posterior(self, stats=False, precision=PREC_MIXED):
+        """Per-wavenumber posterior on the work spectra; returns (inverted, failures, FP64 re-solves)
+        when stats (a synchronisation), zeros otherwise."""
+        return posterior(self.ctx, self.tables, self.mean_w, self.data_w, self.cov, rxy=self.rxy, rt=self.rt,
+                         lam2d=self.rxy, want_residual=False, precision=precision, cond_limit=1e6, stats=stats)
+
+    def realisations(self, realisations=None):
+        """Posterior realisations 0..R-1 from the spectral means; realisation r ends in
+        self.out[r % len(self.out)].  Realisations beyond the kept sets are streamed through them;
+        with two or more sets, consecutive realisations are coloured in pairs (one read of the
+        covariance per pair)."""
+        R = self.R if realisations is None else realisations
         n, r = len(self.out), 0
         while r < R:
             if r + 1 < R and n > 1:
@@ -188,9 +221,10 @@ class Problem:
             else:
                 simulate(self.ctx, self.cov, self.mean_w, self.out[r % n], seed=self.s["seed"], realisation=r)
                 r += 1
+
+    def inverse_means(self):
         for g in self.mean_w:
             g.invFFTInPlace()
-        return st
 
     def item(self, realisations=None):
         """Batch descriptor: the same inputs, work grids and outputs as step()."""

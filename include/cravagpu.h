@@ -349,9 +349,17 @@ int cg_forward_model(cg_ctx *ctx, int ntheta, const float *K, cg_grid *m[3], cg_
 /* Real-domain white noise x += sigma * N(0,1) on every padded cell (counter-based generator). */
 int cg_grid_add_white_noise(cg_grid *grid, float sigma, uint64_t seed, uint32_t stream);
 /* Sum of squares of a real-domain grid over the unpadded box (ext_simbox = 0) or the padded
- * box; the basis of the data variance of ModelAVODynamic::ComputeDataVariance
- * (modelavodynamic.cpp:575-620).  Synchronises. */
+ * box, sum over cells of double(x)^2 in a fixed order (rows in x order, the rows of a plane in y
+ * order, the planes in z order), so that equal grids give equal sums on every run.  Synchronises.
+ * No reference counterpart (the synthetic inputs' signal variance). */
 int cg_grid_sum_squares(cg_grid *grid, int ext_simbox, double *sum);
+/* ModelAVODynamic::ComputeDataVariance (modelavodynamic.cpp:575-620) for one stack: the mean of
+ * float(x * x) over the samples of the unpadded box with x != 0, summed in double in the
+ * reference's order (each row in x order, the rows of a plane in y order, the planes in z order),
+ * returned as float (0 when every sample is zero); count (may be null) receives the number of
+ * non-zero samples.  Bit-identical to the reference's sequential sweep.  Real domain;
+ * synchronises. */
+int cg_grid_data_variance(cg_grid *grid, float *variance, uint64_t *count);
 
 /* ------------------------------------------------------------------ host-side setup */
 /* Per-k_z tables of the constant-thickness branch (avoinversion.cpp:944-975; Wavelet1D

@@ -77,6 +77,8 @@ def lib():
         L.orc_fill_complex_noise.argtypes = [ctypes.c_int] * 3 + [_c_u32_p, _c_float_p]
         L.orc_cb_normal2.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64, _c_float_p, _c_float_p]
         L.orc_fill_complex_noise_cb.argtypes = [ctypes.c_int] * 3 + [ctypes.c_uint64, ctypes.c_uint32, _c_float_p]
+        L.orc_fill_complex_noise_cb_planes.argtypes = ([ctypes.c_int] * 3 + [ctypes.c_uint64, ctypes.c_uint32, _c_int_p,
+                                                       ctypes.c_int, _c_float_p])
         L.orc_philox4x32_10.argtypes = [_c_u32_p, _c_u32_p, _c_u32_p]
         L.orc_colour_f32.restype = ctypes.c_long
         L.orc_colour_f32.argtypes = [ctypes.c_long, _c_float_p, _c_float_p]
@@ -100,6 +102,8 @@ def lib():
         L.orc_interpolate_seismic.argtypes = [ctypes.c_int] * 6 + [_c_float_p, ctypes.c_float, _c_int_p]
         L.orc_fill_param_corr.argtypes = [ctypes.c_int] * 3 + [_c_float_p, _c_float_p] + [ctypes.c_float] * 3 + [_c_float_p]
         L.orc_fill_err_corr.argtypes = [ctypes.c_int] * 3 + [_c_float_p] + [ctypes.c_float] * 2 + [_c_float_p]
+        L.orc_data_variance.restype = ctypes.c_float
+        L.orc_data_variance.argtypes = [ctypes.c_int] * 6 + [_c_float_p, ctypes.POINTER(ctypes.c_long)]
         L.orc_kz_tables.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, _c_float_p,
                                     _c_float_p, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                     _c_float_p, _c_float_p, _c_float_p, _c_float_p, _c_int_p]
@@ -291,6 +295,66 @@ def fill_complex_noise_cb(nxp, nyp, nzp, seed, stream):
     out = np.zeros((nzp, nyp, nxp // 2 + 1), dtype=np.complex64)
     lib().orc_fill_complex_noise_cb(nxp, nyp, nzp, seed, stream, _ptr(out.view(np.float32), ctypes.c_float))
     return out
+
+
+def data_variance(rv, nx, ny, nz, nxp):
+    """ModelAVODynamic::ComputeDataVariance of a padded real grid rv (nzp, nyp, rnxp): (float variance,
+    non-zero box samples)."""
+    a = np.ascontiguousarray(rv, dtype=np.float32)
+    nzp, nyp, _ = a.shape
+    n = ctypes.c_long()
+    v = lib().orc_data_variance(nx, ny, nz, nxp, nyp, nzp, _ptr(a, ctypes.c_float), ctypes.byref(n))
+    return float(np.float32(v)), n.value
+
+
+def fill_complex_noise_cb_planes(nxp, nyp, nzp, seed, stream, ks):
+    """Counter noise of fill_complex_noise_cb on the k planes `ks` only: (len(ks), nyp, cnxp)."""
+    kk = np.ascontiguousarray(ks, dtype=np.int32)
+    out = np.zeros((len(kk), nyp, nxp // 2 + 1), dtype=np.complex64)
+    lib().orc_fill_complex_noise_cb_planes(nxp, nyp, nzp, seed, stream, _ptr(kk, ctypes.c_int), len(kk),
+                                           _ptr(out.view(np.float32), ctypes.c_float))
+    return out
+
+
+def spectrum_planes(real, ks, cubetype, nxp=None):
+    """k planes `ks` of FFTGrid::fftInPlace of a real grid, in FP64 (numpy), without the whole 3D
+    transform: the z transform at those k is a matrix product over the planes, then a 2D FFT.
+    real: (nzp, nyp, nxp) or padded (nzp, nyp, rnxp) with nxp given.  Returns (len(ks), nyp, cnxp)."""
+    nzp, nyp = real.shape[:2]
+    nxp = real.shape[2] if nxp is None else nxp
+    cnxp = nxp // 2 + 1
+    z = np.arange(nzp)
+    E = np.exp(-2j * np.pi * np.outer(np.asarray(ks), z) / nzp)
+    out = np.empty((len(ks), nyp, cnxp), dtype=np.complex128)
+    for p in range(len(ks)):
+        acc = np.zeros((nyp, nxp), dtype=np.complex128)
+        for z0 in range(0, nzp, 32):  # bounded memory: 32 planes at a time
+            sl = np.asarray(real[z0:z0 + 32, :, :nxp], dtype=np.float64)
+            acc += np.tensordot(E[p, z0:z0 + 32], sl, axes=(0, 0))
+        out[p] = np.fft.fft2(acc)[:, :cnxp]
+    if cubetype != COVARIANCE:  # FFTGrid's float scale factor (fftgrid.cpp:1601)
+        out *= float(np.float32(1.0) / np.sqrt(np.float32(float(nxp) * nyp * nzp)))
+    return out
+
+
+def real_planes(spec, zs, nxp, cubetype):
+    """z planes `zs` of FFTGrid::invFFTInPlace of a (nzp, nyp, cnxp) spectrum, in FP64 (numpy): the
+    inverse z transform at those z as a matrix product, then inverse y and the half-complex x
+    inverse (imaginary parts of the x DC and Nyquist bins ignored, as FFTW's hc2r).  Returns
+    (len(zs), nyp, nxp)."""
+    nzp, nyp, cnxp = spec.shape
+    k = np.arange(nzp)
+    E = np.exp(2j * np.pi * np.outer(np.asarray(zs), k) / nzp)
+    out = np.empty((len(zs), nyp, nxp))
+    for p in range(len(zs)):
+        acc = np.zeros((nyp, cnxp), dtype=np.complex128)
+        for k0 in range(0, nzp, 32):
+            acc += np.tensordot(E[p, k0:k0 + 32], np.asarray(spec[k0:k0 + 32], dtype=np.complex128), axes=(0, 0))
+        out[p] = np.fft.irfft(np.fft.ifft(acc, axis=0) * nyp, n=nxp, axis=1) * nxp  # unnormalised
+    N = float(nxp) * nyp * nzp
+    if cubetype == COVARIANCE:
+        return out / N
+    return out * float(np.float32(1.0 / np.sqrt(np.float32(N))))  # fftgrid.cpp:1634
 
 
 def kz_tables(nz, nzp, dt_ms, wavelets, A, low_cut=5.0, high_cut=55.0, min_rel_thick=1.0):

@@ -111,6 +111,9 @@ void   orc_fill_complex_noise(int nxp, int nyp, int nzp, uint32_t *state, float 
  * (idx, stream >> 1, 0) and key seed. */
 void   orc_cb_normal2(uint64_t seed, uint32_t stream, uint64_t idx, float *z0, float *z1);
 void   orc_fill_complex_noise_cb(int nxp, int nyp, int nzp, uint64_t seed, uint32_t stream, float *cvalue);
+/* the same noise on the listed k planes only (nk planes of (nxp/2+1)*nyp cells) */
+void   orc_fill_complex_noise_cb_planes(int nxp, int nyp, int nzp, uint64_t seed, uint32_t stream, const int *ks,
+                                        int nk, float *cvalue);
 void   orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 
 /* ---------------------------------------------------------------- colouring (a9) */
@@ -151,6 +154,8 @@ int    orc_circ_corr_t(const double *corr_t, int n, int min_int_fq, int nzp, flo
 /* Per-k_z tables of the constant-thickness branch (avoinversion.cpp:944-975 with fillkW,
  * fillkWNorm, Wavelet1D difference operators wavelet1D.cpp:389-500, findNorm wavelet.cpp:882-900).
  * wavelets [ntheta][nzp] real time-domain wavelets in FFT order; A [ntheta][3]. */
+/* ModelAVODynamic::ComputeDataVariance for one stack (padded real grid rv), the reference's order */
+float  orc_data_variance(int nx, int ny, int nz, int nxp, int nyp, int nzp, const float *rv, long *count);
 void   orc_kz_tables(int nz, int nzp, double dt_ms, int ntheta, const float *wavelets,
                      const float *A, float low_cut, float high_cut, float min_rel_thick,
                      float *K, float *e1, float *e2, float *e3, int *invert);

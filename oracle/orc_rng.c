@@ -160,38 +160,51 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
 
 /* Same Hermitian pattern as the reference fill; value of cell i from its own counter, the
  * conjugate-mirror cells copy their partner's value. */
-void orc_fill_complex_noise_cb(int nxp, int nyp, int nzp, uint64_t seed, uint32_t stream, float *cv)
+static void cb_cell(int nxp, int nyp, int nzp, uint64_t seed, uint32_t stream, long i, float *out)
 {
   const long cnxp = nxp / 2 + 1;
-  const long csize = cnxp * nyp * nzp;
   const double sd = 1 / sqrt(2.0);
-  for (long i = 0; i < csize; i++) {
-    long xs = i % cnxp;
-    float z0, z1;
-    if (xs == 0 || (xs == cnxp - 1 && (i % 2) == 1)) {
-      long jk = (i - xs) / cnxp;
-      long jind = jk % nyp, kind = jk / nyp;
-      long jcc = (jind == 0) ? 0 : nyp - jind;
-      long kcc = (kind == 0) ? 0 : nzp - kind;
-      long jkcc = jcc + kcc * nyp;
-      if (jkcc == jk) {
-        orc_cb_normal2(seed, stream, (uint64_t)i, &z0, &z1);
-        cv[2 * i] = z0;
-        cv[2 * i + 1] = 0;
-      } else if (jkcc > jk) {
-        orc_cb_normal2(seed, stream, (uint64_t)i, &z0, &z1);
-        cv[2 * i] = (float)(sd * z0);
-        cv[2 * i + 1] = (float)(sd * z1);
-      } else {
-        long cci = jkcc * cnxp + xs;
-        orc_cb_normal2(seed, stream, (uint64_t)cci, &z0, &z1);
-        cv[2 * i] = (float)(sd * z0);
-        cv[2 * i + 1] = -(float)(sd * z1);
-      }
-    } else {
+  long xs = i % cnxp;
+  float z0, z1;
+  if (xs == 0 || (xs == cnxp - 1 && (i % 2) == 1)) {
+    long jk = (i - xs) / cnxp;
+    long jind = jk % nyp, kind = jk / nyp;
+    long jcc = (jind == 0) ? 0 : nyp - jind;
+    long kcc = (kind == 0) ? 0 : nzp - kind;
+    long jkcc = jcc + kcc * nyp;
+    if (jkcc == jk) {
       orc_cb_normal2(seed, stream, (uint64_t)i, &z0, &z1);
-      cv[2 * i] = (float)(sd * z0);
-      cv[2 * i + 1] = (float)(sd * z1);
+      out[0] = z0;
+      out[1] = 0;
+    } else if (jkcc > jk) {
+      orc_cb_normal2(seed, stream, (uint64_t)i, &z0, &z1);
+      out[0] = (float)(sd * z0);
+      out[1] = (float)(sd * z1);
+    } else {
+      long cci = jkcc * cnxp + xs;
+      orc_cb_normal2(seed, stream, (uint64_t)cci, &z0, &z1);
+      out[0] = (float)(sd * z0);
+      out[1] = -(float)(sd * z1);
     }
+  } else {
+    orc_cb_normal2(seed, stream, (uint64_t)i, &z0, &z1);
+    out[0] = (float)(sd * z0);
+    out[1] = (float)(sd * z1);
   }
+}
+
+void orc_fill_complex_noise_cb(int nxp, int nyp, int nzp, uint64_t seed, uint32_t stream, float *cv)
+{
+  const long csize = (long)(nxp / 2 + 1) * nyp * nzp;
+  for (long i = 0; i < csize; i++) cb_cell(nxp, nyp, nzp, seed, stream, i, cv + 2 * i);
+}
+
+/* The same noise on the k planes ks[0..nk) only: cv holds nk planes of (nxp/2+1) * nyp cells. */
+void orc_fill_complex_noise_cb_planes(int nxp, int nyp, int nzp, uint64_t seed, uint32_t stream, const int *ks,
+                                      int nk, float *cv)
+{
+  const long plane = (long)(nxp / 2 + 1) * nyp;
+  for (int p = 0; p < nk; p++)
+    for (long c = 0; c < plane; c++)
+      cb_cell(nxp, nyp, nzp, seed, stream, (long)ks[p] * plane + c, cv + 2 * ((long)p * plane + c));
 }

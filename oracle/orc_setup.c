@@ -169,3 +169,31 @@ void orc_kz_tables(int nz, int nzp, double dt_ms, int nt, const float *wav, cons
   }
   free(w); free(s); free(fd); free(bd); free(cd); free(wn); free(sn); free(diff);
 }
+
+/* ModelAVODynamic::ComputeDataVariance (modelavodynamic.cpp:575-620) for one stack, in the
+ * reference's order: the sequential sweep k, j, i over the padded real grid rv (rnxp floats per
+ * row), float products of the non-zero box samples summed in double per row, rows per plane,
+ * planes in total; returns float(total / count), 0 when every sample is zero. */
+float orc_data_variance(int nx, int ny, int nz, int nxp, int nyp, int nzp, const float *rv, long *count)
+{
+  const int rnxp = 2 * (nxp / 2 + 1);
+  double totvar = 0;
+  long ndata = 0;
+  for (int k = 0; k < nzp; k++) {
+    double tmpvar1 = 0;
+    for (int j = 0; j < nyp; j++) {
+      double tmpvar2 = 0;
+      for (int i = 0; i < rnxp; i++) {
+        float tmp = rv[(long)rnxp * (j + (long)nyp * k) + i];
+        if (k < nz && j < ny && i < nx && tmp != 0.0) {
+          tmpvar2 += (double)(tmp * tmp);
+          ndata++;
+        }
+      }
+      tmpvar1 += tmpvar2;
+    }
+    totvar += tmpvar1;
+  }
+  if (count) *count = ndata;
+  return ndata == 0 ? 0.0f : (float)(totvar / (double)ndata);
+}

@@ -1,0 +1,341 @@
+"""Every benchmark workload configuration run on the GPU and checked, output by output.
+
+* plain case, 64 x 64 x 128 (padded 84 x 84 x 180 by the reference rule), 3 angles, 1
+  realisation: the whole inversion (forward FFTs, posterior, inverse FFTs, one realisation)
+  against the FP64 oracle, and against the reference's own FP32 pipeline (FFTW 2.1.2 and lib_matr
+  compiled from the CRAVA tree) where that library was built;
+* production cube, 512 x 512 x 256 (576 x 576 x 320), 5 angles, 2 realisations, and the large cube,
+  1024 x 1024 x 320 (1152 x 1152 x 384), 6 angles, 20 realisations: the benchmark's own call
+  sequence at full size, checked on sampled k and z planes -- forward spectra against FP64
+  transforms of the resident inputs, the posterior against the FP64 oracle on the GPU's own
+  spectra, pass-through planes equal to the prior, Sigma_post <= Sigma_m, the inverse transform
+  of the means, and realisations against the FP64 colouring of the same counter noise;
+* the 64-inversion uneven mix: one batch against the plain per-request path, and sampled cubes
+  against the FP64 oracle;
+* served requests at the serving sizes: each equal alone and in the mix, sampled requests
+  against the FP64 oracle.
+
+Tolerances (relative L2, stated per output): mean update 1e-4, covariance 1e-5, realisation noise
+part 1e-4 (it goes through a 3 x 3 Cholesky of the FP32 covariance and a further FP32 inverse
+transform), forward spectra 5e-6, inverse transform of the means 2e-5 of the update (FP32 rounding
+of the background, ~8, against an update of ~0.05).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import crava_amd as C
+import oracle as O
+import problem as P
+from crava_amd import synthetic as S
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+needs_ref = pytest.mark.skipif(not O.ref_available(), reason="reference-built oracle/_ref absent")
+
+COV_ORDER = [(0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2)]
+
+
+def _herm3(c6):
+    """(6, n) covariance entries -> (n, 3, 3) Hermitian matrices."""
+    n = c6.shape[1]
+    M = np.zeros((n, 3, 3), dtype=np.complex128)
+    for g, (a, b) in enumerate(COV_ORDER):
+        M[:, a, b] = c6[g]
+        M[:, b, a] = np.conj(c6[g])
+    return M
+
+
+# ------------------------------------------------------------------------------ plain case
+def test_plain_case_full_inversion_vs_fp64_oracle_and_reference(ctx):
+    p = P.make_problem(64, 64, 128, seed=2026, angles_deg=(0.0, 15.0, 30.0), peak_hz=25.0, sn=5.0)
+    nx, ny, nz, nxp, nyp, nzp, nt = (p[k] for k in ("nx", "ny", "nz", "nxp", "nyp", "nzp", "ntheta"))
+    assert (nxp, nyp, nzp) == (84, 84, 180)
+    print("plain case %dx%dx%d -> %dx%dx%d, %d angles" % (nx, ny, nz, nxp, nyp, nzp, nt), flush=True)
+    seed = 77
+    mk = lambda ct: C.FFTGrid(ctx, nx, ny, nz, nxp, nyp, nzp, ct)
+    mean = [mk(C.PARAMETER) for _ in range(3)]
+    data = [mk(C.DATA) for _ in range(nt)]
+    cov = [mk(C.COVARIANCE) for _ in range(6)]
+    out = [mk(C.PARAMETER) for _ in range(3)]
+    for g, r in zip(mean, p["mean_real"]):
+        g.upload_real(r)
+        g.fftInPlace()
+    for g, r in zip(data, p["seis_real"]):
+        g.upload_real(r)
+        g.fftInPlace()
+    gms = np.stack([g.download_complex() for g in mean])
+    gds = np.stack([g.download_complex() for g in data])
+    ninv, nfail, n64 = C.posterior(ctx, p, mean, data, cov, rxy=p["rxy"], rt=p["rt"], lam2d=p["lam2d"],
+                                   want_residual=True, precision=C.PREC_MIXED)
+    res = np.stack([g.download_complex() for g in data])
+    cpost = np.stack([g.download_complex() for g in cov])
+    C.simulate(ctx, cov, mean, out, seed=seed, realisation=0)  # spectral means, as the benchmark
+    for g in mean:
+        g.invFFTInPlace()
+    gmean = np.stack([g.download_real() for g in mean])
+    greal = np.stack([g.download_real() for g in out])
+    for g in mean + data + cov + out:
+        g.close()
+    assert ninv == int(p["invert"].sum()) * nyp * (nxp // 2 + 1) and nfail == 0
+    # forward spectra
+    ms, ds = P.forward_spectra(p, np.float64)
+    assert rel_l2(gms, ms) < 5e-6 and rel_l2(gds, ds) < 5e-6
+    # FP64 oracle from the FP64 spectra of the same inputs
+    om, od, oc, _, onf = O.posterior(p, ms, ds, None, "f64")
+    assert onf == 0
+    mu_rt = np.stack([O.grid_ifft(m, nxp, O.PARAMETER) for m in ms])  # the oracle's own round trip
+    want = np.stack([O.grid_ifft(m, nxp, O.PARAMETER) for m in om])
+    e_mean = rel_l2(gmean - p["mean_real"], want - mu_rt)
+    e_cov = rel_l2(cpost, oc)
+    e_res = rel_l2(res, od)
+    noise = np.stack([O.fill_complex_noise_cb(nxp, nyp, nzp, seed, g) for g in range(3)])
+    z, zf = O.colour(oc, noise, "f64")
+    wantr = np.stack([O.grid_ifft(z[g], nxp, O.PARAMETER) for g in range(3)])
+    e_real = rel_l2(greal - gmean, wantr)
+    print("plain case vs FP64: mean update %.2e, covariance %.2e, residual %.2e, realisation %.2e (FP64 re-solves %d)"
+          % (e_mean, e_cov, e_res, e_real, n64), flush=True)
+    assert e_mean < 1e-4 and e_cov < 1e-5 and e_res < 1e-4 and e_real < 1e-4
+    if O.ref_available():
+        # the reference's own FP32 pipeline on the same float inputs
+        rms = np.stack([O.ref_grid_fft(r, O.PARAMETER) for r in p["mean_real"]])
+        rds = np.stack([O.ref_grid_fft(r, O.DATA) for r in p["seis_real"]])
+        rm, _, rc, _, rnf = O.ref_posterior(p, rms, rds)
+        rmean = np.stack([O.ref_grid_ifft(m, nxp, O.PARAMETER) for m in rm])
+        rmu = np.stack([O.ref_grid_ifft(m, nxp, O.PARAMETER) for m in rms])
+        e_ref = rel_l2(gmean - p["mean_real"], rmean - rmu)
+        e_refc = rel_l2(cpost, rc)
+        e_ref_fp64 = rel_l2(rmean - rmu, want - mu_rt)
+        print("plain case vs reference FP32 pipeline: mean update %.2e, covariance %.2e "
+              "(reference vs FP64: %.2e)" % (e_ref, e_refc, e_ref_fp64), flush=True)
+        assert rnf == 0 and e_ref < 1e-4 and e_refc < 1e-5
+
+
+# ------------------------------------------------------------------------------ single cubes
+def _sampled_planes(invert, n_inv=4, n_pass=3):
+    inv_k = np.where(invert != 0)[0]
+    pass_k = np.where(invert == 0)[0]
+    ks = list(inv_k[np.linspace(0, len(inv_k) - 1, n_inv).astype(int)])
+    ks += list(pass_k[np.linspace(0, len(pass_k) - 1, n_pass).astype(int)])
+    return sorted(set(int(k) for k in ks))
+
+
+def _check_cube(ctx, sp, label, rel=(0, 1)):
+    """The benchmark's call sequence on one cube at full size, checked on sampled planes."""
+    prob = S.Problem(ctx, sp)
+    w = sp
+    nxp, nyp, nzp, nt = w["nxp"], w["nyp"], w["nzp"], prob.nt
+    cnxp = nxp // 2 + 1
+    ctx.autotune(nxp, nyp, nzp)
+    tb = prob.tables
+    ks = _sampled_planes(tb["invert"])
+    kinv = [k for k in ks if tb["invert"][k]]
+    kpass = [k for k in ks if not tb["invert"][k]]
+    print("%s: sampled k planes %s" % (label, ks), flush=True)
+    prob.forward()
+    ms = np.stack([g.download_complex()[ks] for g in prob.mean_w])
+    ds = np.stack([g.download_complex()[ks] for g in prob.data_w])
+    # forward spectra against FP64 transforms of the resident real inputs (one seismic cube)
+    real0 = prob.seis_in[0].download_real()
+    want0 = O.spectrum_planes(real0, ks, O.DATA)
+    e_fwd = rel_l2(ds[0], want0)
+    del real0
+    ninv, nfail, n64 = prob.posterior(stats=True)
+    mpost_full = [g.download_complex() for g in prob.mean_w]
+    mp = np.stack([m[ks] for m in mpost_full])
+    cp = np.stack([g.download_complex()[ks] for g in prob.cov])
+    # FP64 oracle on the sampled planes, from the GPU's own forward spectra
+    sub = dict(nxp=nxp, nyp=nyp, nzp=len(ks), ntheta=nt, K=tb["K"][ks], e1=tb["e1"][ks], e2=tb["e2"][ks],
+               e3=tb["e3"][ks], invert=tb["invert"][ks], sigma0e=tb["sigma0e"], wnc=tb["wnc"],
+               prior_var0=tb["prior_var0"], prior_mode=1, rxy=prob.rxy, rt=prob.rt[ks], lam2d=prob.rxy)
+    om, _, oc, _, onf = O.posterior(sub, ms.astype(np.complex128), ds.astype(np.complex128), None, "f64")
+    ii = [ks.index(k) for k in kinv]
+    pp = [ks.index(k) for k in kpass]
+    e_upd = rel_l2(mp[:, ii] - ms[:, ii], om[:, ii] - ms[:, ii])
+    e_cov = rel_l2(cp, oc)
+    # pass-through planes: the means untouched, the covariance the prior r(k) Sigma0
+    assert np.array_equal(mp[:, pp], ms[:, pp])
+    e_prior = rel_l2(cp[:, pp], oc[:, pp])
+    # Sigma_post is PSD and Sigma_m - Sigma_post is PSD at sampled inverted wavenumbers
+    rng = np.random.default_rng(5)
+    cells = rng.integers(0, nyp * cnxp, 4096)
+    S0 = np.asarray(tb["prior_var0"])
+    worst_post, worst_gain = 0.0, 0.0
+    for i in ii:
+        c6 = cp[:, i].reshape(6, -1)[:, cells].astype(np.complex128)
+        r = np.abs(prob.rxy.reshape(-1)[cells].astype(np.float64) * float(prob.rt[ks[i]]))
+        Sp = _herm3(c6)
+        Sm = r[:, None, None] * S0[None]
+        scale = np.trace(Sm, axis1=1, axis2=2).real + 1e-30
+        worst_post = min(worst_post, float(np.min(np.linalg.eigvalsh(Sp)[:, 0] / scale)))
+        worst_gain = min(worst_gain, float(np.min(np.linalg.eigvalsh(Sm - Sp)[:, 0] / scale)))
+    # realisations (the benchmark's sequence: coloured from the spectral means, then the means'
+    # inverse transforms)
+    R = prob.R
+    prob.realisations()
+    prob.inverse_means()
+    zs = sorted({0, nzp // 3, nzp // 2 + 1, nzp - 1})
+    e_inv = 0.0
+    gmean = [g.download_real() for g in prob.mean_w]
+    for a in range(3):
+        want = O.real_planes(mpost_full[a], zs, nxp, O.PARAMETER)
+        bg = S.BACKGROUND[a]
+        e_inv = max(e_inv, rel_l2(gmean[a][zs] - bg, want - bg))
+    del mpost_full
+    # the last realisations in each output set: r = R - 2 and R - 1 (or 0 alone)
+    e_real = 0.0
+    last = [r for r in range(R) if r >= R - len(prob.out)]
+    ocov_inv = oc[:, ii].astype(np.complex128)
+    for r in last:
+        outs = prob.out[r % len(prob.out)]
+        noise = np.stack([O.fill_complex_noise_cb_planes(nxp, nyp, nzp, sp["seed"], 4 * r + g, [ks[i] for i in ii])
+                          for g in range(3)])
+        z, _ = O.colour(ocov_inv, noise, "f64")
+        got = np.stack([O.spectrum_planes(outs[g].download_real() - gmean[g], [ks[i] for i in ii], O.PARAMETER)
+                        for g in range(3)])
+        e_real = max(e_real, rel_l2(got, z))
+    print("%s: inverted %d, failures %d, FP64 re-solves %d; forward %.2e, mean update %.2e, covariance %.2e, "
+          "prior planes %.2e, min eig(Sigma_post)/tr %.1e, min eig(Sigma_m - Sigma_post)/tr %.1e, inverse means "
+          "%.2e, realisations %s %.2e" % (label, ninv, nfail, n64, e_fwd, e_upd, e_cov, e_prior, worst_post,
+                                           worst_gain, e_inv, last, e_real), flush=True)
+    prob.close()
+    assert onf == 0 and nfail == 0
+    assert ninv == prob.n_inverted_planes * nyp * cnxp
+    assert e_fwd < 5e-6
+    assert e_upd < 1e-4 and e_cov < 1e-5 and e_prior < 1e-6
+    assert worst_post > -1e-6 and worst_gain > -1e-6
+    assert e_inv < 2e-5
+    assert e_real < 1e-4
+    return dict(forward=e_fwd, mean_update=e_upd, covariance=e_cov, realisations=e_real)
+
+
+@pytest.mark.timeout(600)
+def test_production_cube_576x576x320_sampled_planes(ctx):
+    _check_cube(ctx, S.single_cube_spec(realisations=2), "576x576x320")
+
+
+@pytest.mark.timeout(900)
+def test_large_cube_1152x1152x384_sampled_planes(ctx):
+    _check_cube(ctx, S.large_cube_spec(realisations=20), "1152x1152x384")
+
+
+# ------------------------------------------------------------------------------ uneven mix
+def _inputs_real(prob):
+    return (np.stack([g.download_real() for g in prob.mean_in]), np.stack([g.download_real() for g in prob.seis_in]))
+
+
+@pytest.mark.timeout(600)
+def test_uneven_mix_batched_equals_plain_and_oracle(ctx):
+    specs = S.uneven_mix(64)
+    probs = [S.Problem(ctx, s) for s in specs]
+    for shape in sorted({(s["nxp"], s["nyp"], s["nzp"]) for s in specs}):
+        ctx.autotune(*shape)
+    print("uneven mix: 64 inversions, %.1f M cells" % (sum(S.cells(s) for s in specs) / 1e6), flush=True)
+    # plain per-request path
+    plain = []
+    for p in probs:
+        p.step(realisations=0)
+        ctx.sync()
+        plain.append((np.stack([g.download_real() for g in p.mean_w]), np.stack([g.download_complex() for g in p.cov])))
+    # one batch of all of them, replayed as a graph
+    b = C.Batch(ctx, [p.item(realisations=0) for p in probs])
+    b.use_graph(True)
+    for _ in range(2):
+        for p in probs:
+            for g in p.mean_w:
+                g.fillInConstant(0.0)
+        b.run()
+        b.wait()
+        nbit = 0
+        for p, (pm, pc) in zip(probs, plain):
+            bm = np.stack([g.download_real() for g in p.mean_w])
+            bc = np.stack([g.download_complex() for g in p.cov])
+            nbit += int(np.array_equal(bm, pm) and np.array_equal(bc, pc))
+        print("uneven mix: batched equal to plain bit for bit in %d of 64 inversions" % nbit, flush=True)
+        assert nbit == 64
+    b.close()
+    # sampled cubes (the smallest and two others) against the FP64 oracle
+    order = np.argsort([S.cells(s) for s in specs])
+    for i in (int(order[0]), int(order[len(order) // 3]), int(order[len(order) // 2])):
+        p, s = probs[i], specs[i]
+        mreal, sreal = _inputs_real(p)
+        q = dict(nxp=s["nxp"], nyp=s["nyp"], nzp=s["nzp"], ntheta=p.nt, **{k: p.tables[k] for k in
+                 ("K", "e1", "e2", "e3", "invert", "sigma0e", "wnc", "prior_var0")}, prior_mode=1, rxy=p.rxy,
+                 rt=p.rt, lam2d=p.rxy)
+        ms = np.stack([O.grid_fft(m, O.PARAMETER) for m in mreal])
+        ds = np.stack([O.grid_fft(x, O.DATA) for x in sreal])
+        om, _, oc, _, onf = O.posterior(q, ms, ds, None, "f64")
+        mu_rt = np.stack([O.grid_ifft(m, s["nxp"], O.PARAMETER) for m in ms])
+        want = np.stack([O.grid_ifft(m, s["nxp"], O.PARAMETER) for m in om])
+        e_m = rel_l2(plain[i][0] - mreal, want - mu_rt)
+        e_c = rel_l2(plain[i][1], oc)
+        print("uneven mix cube %d (%dx%dx%d, %d angles): mean update %.2e, covariance %.2e"
+              % (i, s["nxp"], s["nyp"], s["nzp"], p.nt, e_m, e_c), flush=True)
+        assert onf == 0 and e_m < 1e-4 and e_c < 1e-5
+    for p in probs:
+        p.close()
+
+
+# ------------------------------------------------------------------------------ serving
+@pytest.mark.timeout(600)
+def test_served_requests_at_serving_sizes():
+    from crava_amd import serve
+    specs = S.serving_requests(6, seed=20261210)
+    print("serving: %s" % [(s["nx"], s["ny"], s["nz"], len(s["angles"]), s["realisations"]) for s in specs],
+          flush=True)
+
+    def run(group, lanes, max_batch):
+        ex = serve.GpuExecutor(lanes=lanes)
+        srv = serve.Server(ex, hbm_budget=200e9, lanes=lanes, max_batch=max_batch, ingest_per_step=len(group))
+        got = {}
+
+        def on_done(r):
+            p = r.work
+            got[r.spec["seed"]] = dict(
+                mean=np.stack([g.download_real() for g in p.mean_w]),
+                cov=np.stack([g.download_complex() for g in p.cov]),
+                real=[np.stack([g.download_real() for g in o]) for o in p.out[:p.R]],
+                inputs=_inputs_real(p), tables=p.tables, rxy=p.rxy, rt=p.rt, nt=p.nt)
+
+        for s in group:
+            srv.submit(s)
+        srv.drain(on_done)
+        ex.close()
+        return got, srv
+
+    mix, srv = run(specs, lanes=2, max_batch=4)
+    assert max(srv.batch_sizes) > 1
+    for s in specs:
+        alone, _ = run([s], lanes=1, max_batch=1)
+        a, m = alone[s["seed"]], mix[s["seed"]]
+        assert np.array_equal(a["mean"], m["mean"]) and np.array_equal(a["cov"], m["cov"])
+        assert len(a["real"]) == len(m["real"]) == s["realisations"]
+        for x, y in zip(a["real"], m["real"]):
+            assert np.array_equal(x, y)
+    # the two smallest requests against the FP64 oracle
+    for s in sorted(specs, key=S.cells)[:2]:
+        m = mix[s["seed"]]
+        mreal, sreal = m["inputs"]
+        tb = m["tables"]
+        q = dict(nxp=s["nxp"], nyp=s["nyp"], nzp=s["nzp"], ntheta=m["nt"], **{k: tb[k] for k in
+                 ("K", "e1", "e2", "e3", "invert", "sigma0e", "wnc", "prior_var0")}, prior_mode=1, rxy=m["rxy"],
+                 rt=m["rt"], lam2d=m["rxy"])
+        ms = np.stack([O.grid_fft(x, O.PARAMETER) for x in mreal])
+        ds = np.stack([O.grid_fft(x, O.DATA) for x in sreal])
+        om, _, oc, _, onf = O.posterior(q, ms, ds, None, "f64")
+        mu_rt = np.stack([O.grid_ifft(x, s["nxp"], O.PARAMETER) for x in ms])
+        want = np.stack([O.grid_ifft(x, s["nxp"], O.PARAMETER) for x in om])
+        e_m = rel_l2(m["mean"] - mreal, want - mu_rt)
+        e_c = rel_l2(m["cov"], oc)
+        e_r = 0.0
+        for r, real in enumerate(m["real"]):
+            noise = np.stack([O.fill_complex_noise_cb(s["nxp"], s["nyp"], s["nzp"], s["seed"], 4 * r + g)
+                              for g in range(3)])
+            z, _ = O.colour(oc, noise, "f64")
+            wr = np.stack([O.grid_ifft(z[g], s["nxp"], O.PARAMETER) for g in range(3)])
+            e_r = max(e_r, rel_l2(real - m["mean"], wr))
+        print("served request %dx%dx%d, %d angles, %d realisations: mean update %.2e, covariance %.2e, "
+              "realisations %.2e" % (s["nx"], s["ny"], s["nz"], m["nt"], s["realisations"], e_m, e_c, e_r), flush=True)
+        assert onf == 0 and e_m < 1e-4 and e_c < 1e-5 and e_r < 1e-4

@@ -5,7 +5,6 @@ import pytest
 
 from crava_amd import serve
 from crava_amd import synthetic as S
-from conftest import rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -16,10 +15,9 @@ SPECS = [S.spec(20, 16, 24, (0.0, 15.0, 30.0), 25.0, seed=101, realisations=2),
          S.spec(18, 18, 18, (0.0, 15.0, 30.0), 35.0, seed=104, realisations=3),
          S.spec(30, 10, 16, (0.0, 9.0, 18.0, 27.0, 36.0, 45.0), 25.0, seed=105, realisations=1)]
 
-# Inputs are generated on the device from each request's seed; the S/N scaling uses a
-# floating-point reduction whose order may vary between runs, so alone-vs-mix agreement is
-# stated as relative L2 <= 1e-6 (the batch itself is order-independent).
-TOL = 1e-6
+# Inputs are generated on the device from each request's seed (the S/N scaling's reduction runs
+# in a fixed order) and a batch computes every inversion with the same kernels and descriptors,
+# so each result is the same alone and in the mix bit for bit.
 
 
 def _serve(specs, lanes, max_batch):
@@ -45,12 +43,10 @@ def test_results_alone_and_in_the_mix_agree():
     mix, srv = _serve(SPECS, lanes=2, max_batch=3)
     assert len(mix) == len(SPECS) and srv.in_use == 0
     assert max(srv.batch_sizes) > 1
-    bg = np.asarray(S.BACKGROUND, dtype=np.float32)[:, None, None, None]
     for s in SPECS:
         alone, _ = _serve([s], lanes=1, max_batch=1)
         (ma, ca, ra), (mm, cm, rm) = alone[s["seed"]], mix[s["seed"]]
-        assert rel_l2(mm - bg, ma - bg) <= TOL
-        assert rel_l2(cm, ca) <= TOL
+        assert np.array_equal(mm, ma) and np.array_equal(cm, ca)
         assert len(ra) == len(rm) == s["realisations"]
         for x, y in zip(rm, ra):
-            assert rel_l2(x - mm, y - ma) <= TOL
+            assert np.array_equal(x, y)
