@@ -32,28 +32,26 @@ from crava_amd import synthetic as S  # noqa: E402
 
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
 
-WORKLOAD = dict(nx=512, ny=512, nz=256, nxp=576, nyp=576, nzp=320, dx=25.0, dy=25.0, dt=4.0,
-                angles=(0.0, 10.0, 20.0, 30.0, 40.0), peak_hz=30.0, std=(0.10, 0.15, 0.05),
-                corr=(0.7, 0.4, 0.3), lateral_range=1000.0, temporal_range=40.0, wnc=0.1, vsvp=0.5,
-                low_cut=5.0, high_cut=55.0, seed=20261202)
-
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def kernel_bytes(w, nt, R, n_inv_planes):
-    """Algorithmic HBM bytes per kernel class for one step (what each kernel must read + write)."""
+def kernel_bytes(w, nt, R, n_inv_planes, launches):
+    """Algorithmic HBM bytes per kernel class for one step (what each kernel must read + write).
+    `launches`: launches per step of each FFT class, as counted on the context stream -- every FFT
+    launch of the plain path transforms one grid (the realisations' inverse z passes run inside
+    the fused colouring and are not z launches)."""
     G = (w["nxp"] // 2 + 1) * w["nyp"] * w["nzp"] * 8  # one complex grid
     Nr = w["nxp"] * w["nyp"] * w["nzp"] * 4  # one real cube (without row padding)
     plane = (w["nxp"] // 2 + 1) * w["nyp"]
-    nfwd, ninv = 3 + nt, 3 + 3 * R
     return {
-        # forward: real cube in, spectrum out; inverse (means, realisations): spectrum in, real out
-        # (the realisations get the means as spectra in the colouring, so no add in the x pass)
-        C.K_FFT_X: nfwd * (Nr + G) + 3 * (G + Nr) + 3 * R * (G + Nr),
-        C.K_FFT_Y: (nfwd + ninv) * 2 * G,
-        C.K_FFT_Z: (nfwd + ninv) * 2 * G,
+        # x: forward reads the real cube and writes the spectrum, inverse the reverse (the
+        # realisations get the means as spectra in the colouring, so no add in the x pass)
+        C.K_FFT_X: launches[C.K_FFT_X] * (Nr + G),
+        # y / z: one grid read and written per launch
+        C.K_FFT_Y: launches[C.K_FFT_Y] * 2 * G,
+        C.K_FFT_Z: launches[C.K_FFT_Z] * 2 * G,
         # inverted cells: read 3 + nt spectra, write 3 means + 6 covariance entries;
         # pass-through cells: write the 6 prior covariance entries
         C.K_POSTERIOR: n_inv_planes * plane * (3 + nt + 9) * 8 + (w["nzp"] - n_inv_planes) * plane * 6 * 8,
@@ -149,6 +147,21 @@ def measured_traffic(kernel):
         with open(TRAFFIC) as f:
             return json.load(f)["bytes_per_launch"].get(kernel)
     except (OSError, ValueError, KeyError):
+        return None
+
+
+ACCURACY = os.path.join(ROOT, "profiles", "accuracy_configs.json")
+
+
+def accuracy_summary():
+    """Per-output relative L2 of the GPU results against the FP64 oracle (and the reference's own FP32
+    pipeline for the plain case) for each workload, as measured by tests/test_gpu_configs.py on the
+    GPU with CG_ACCURACY_OUT set (summary committed under profiles/)."""
+    try:
+        with open(ACCURACY) as f:
+            return {"source": "tests/test_gpu_configs.py (CG_ACCURACY_OUT), relative L2 per output",
+                    "workloads": json.load(f)}
+    except (OSError, ValueError):
         return None
 
 
@@ -301,9 +314,7 @@ def bench_large(ctx, steps=2):
     angles 0-50 deg, Ormsby 5-10-60-80 Hz wavelet, 1500 m lateral range, 20 posterior realisations
     (streamed through two sets of three output grids, coloured in pairs), covariance kept as the 6
     Hermitian spectra."""
-    sp = S.spec(1024, 1024, 320, tuple(float(a) for a in range(0, 60, 10)), 25.0, seed=20261205, nxp=1152,
-                nyp=1152, nzp=384, lateral_range=1500.0, ormsby_hz=(5.0, 10.0, 60.0, 80.0), realisations=20,
-                out_sets=2)
+    sp = S.large_cube_spec(realisations=20)
     t0 = time.perf_counter()
     prob = S.Problem(ctx, sp)
     ctx.autotune(1152, 1152, 384)
@@ -384,8 +395,8 @@ def main():
               "run one per GPU, a single cube as z slabs with an all-to-all transpose); run with "
               "--gpus 1", file=sys.stderr)
         sys.exit(2)
-    w = dict(WORKLOAD)
     R = args.realisations
+    w = S.single_cube_spec(realisations=R)
     ctx = C.Context(0)
     if args.copy_calibration:
         # a copy-like kernel of known traffic for scaling hardware byte counters: FFTGrid::add
@@ -399,7 +410,7 @@ def main():
         ctx.close()
         return
     t0 = time.perf_counter()
-    inv = S.Problem(ctx, S.spec(**w, realisations=R))
+    inv = S.Problem(ctx, w)
     ctx.autotune(w["nxp"], w["nyp"], w["nzp"])
     log("setup %.1f s (incl. autotuning the FFT tiles of the grid)" % (time.perf_counter() - t0))
     for _ in range(args.warmup):
@@ -455,7 +466,8 @@ def main():
     for g in inv.out[0]:
         seed = g.fillInComplexNoiseReference(seed)
     ms_refnoise = 1000.0 * (time.perf_counter() - t) / 3
-    kb = kernel_bytes(w, inv.nt, R, inv.n_inverted_planes)
+    kb = kernel_bytes(w, inv.nt, R, inv.n_inverted_planes,
+                      {k: kt[k][0] // args.profile_steps for k in (C.K_FFT_X, C.K_FFT_Y, C.K_FFT_Z)})
     kernels = {}
     for k, (n, ms) in kt.items():
         if n == 0:
@@ -509,6 +521,7 @@ def main():
         "stream_ceiling": {"kernel": "k_add (read 2 grids, write 1)", "gbps": round(stream_gbps, 1),
                            "frac_of_peak": stream_gbps * 1e9 / HBM_PEAK},
         "posterior_stats": {"inverted": stats[0], "cholesky_failures": stats[1], "fp64_resolves": stats[2]},
+        "accuracy": accuracy_summary(),
         "cpu_baseline": None,
     }
     if not args.no_pad:

@@ -58,6 +58,17 @@ class SimArgs(ctypes.Structure):
     ]
 
 
+class ScaleWaveletArgs(ctypes.Structure):
+    _fields_ = [
+        ("mode", ctypes.c_int), ("dt_ms", ctypes.c_double), ("wavelet", _f_p), ("rel_thick", _d_p),
+        ("shift_ms", _f_p), ("gain", _f_p), ("stretch", _f_p), ("low_cut", ctypes.c_float),
+        ("high_cut", ctypes.c_float), ("wnc", ctypes.c_float), ("error_var", ctypes.c_float), ("corr_t", _f_p),
+        ("A", _f_p), ("prior_var0", _d_p),
+    ]
+
+
+SCALE_WAVELET_DIVIDE, SCALE_WAVELET_MULTIPLY = 0, 1
+
 _lib = None
 
 
@@ -90,6 +101,7 @@ def lib():
             "cg_grid_fill_noise_counter": [_vp, u64, ctypes.c_uint32],
             "cg_grid_fill_noise_reference": [_vp, ctypes.POINTER(ctypes.c_uint32)],
             "cg_build_kz_tables": [i, i, d, i, _f_p, _f_p, f, f, f, _f_p, _f_p, _f_p, _f_p, _i_p],
+            "cg_build_kz_tables_variable_thickness": [i, i, d, i, _f_p, _f_p, f, f, f, _f_p, _f_p, _f_p, _f_p, _i_p],
             "cg_factorable": [i],
             "cg_make_circ_auto_cov_pos_def": [i, _f_p],
             "cg_fft_r2c": [_vp, _vp],
@@ -106,6 +118,9 @@ def lib():
             "cg_grid_extrapolate_seismic": [_vp, i, i, i, i],
             "cg_grid_fill_param_corr": [_vp, _f_p, _f_p, f, f, f], "cg_grid_fill_trace_taper": [_vp, _vp, i], "cg_grid_fill_err_corr": [_vp, _f_p, f, f],
             "cg_grid_write_crava": [_vp, ctypes.c_char_p, ctypes.POINTER(CravaGeometry)],
+            "cg_grid_scale_wavelet": [_vp, ctypes.POINTER(ScaleWaveletArgs)],
+            "cg_grid_write_storm": [_vp, ctypes.c_char_p, ctypes.POINTER(StormGeometry), i],
+            "cg_grid_read_storm": [_vp, ctypes.c_char_p, ctypes.POINTER(StormGeometry)],
             "cg_grid_read_crava": [_vp, ctypes.c_char_p, ctypes.POINTER(CravaGeometry)],
         }
         for name, args in sig.items():
@@ -122,6 +137,13 @@ def lib():
 
 class CravaError(RuntimeError):
     pass
+
+
+class StormGeometry(ctypes.Structure):
+    """cg_storm_geometry: the simbox volume of a STORM grid header."""
+    _fields_ = [("x_min", ctypes.c_double), ("lx", ctypes.c_double), ("y_min", ctypes.c_double),
+                ("ly", ctypes.c_double), ("lz", ctypes.c_double), ("angle", ctypes.c_double),
+                ("top_name", ctypes.c_char_p), ("bot_name", ctypes.c_char_p)]
 
 
 class CravaGeometry(ctypes.Structure):
@@ -362,6 +384,38 @@ class FFTGrid:
         self.ctx.check(lib().cg_grid_fill_err_corr(self._h, _ptr(cxy, ctypes.c_float), grad_i, grad_j),
                        "fillInErrCorr")
 
+    def scale_wavelet(self, mode, dt_ms, wavelet, rel_thick, low_cut=5.0, high_cut=55.0, wnc=0.1, error_var=1.0,
+                      corr_t=None, A=None, prior_var0=None, shift_ms=None, gain=None, stretch=None):
+        """The non-constant-thickness data rescaling on this real-domain grid, trace by trace:
+        SCALE_WAVELET_DIVIDE (divideDataByScaleWavelet with computeAdjustmentFactor) or
+        SCALE_WAVELET_MULTIPLY (multiplyDataByScaleWaveletAndWriteToFile before the write).
+        wavelet (nzp,) in FFT order; rel_thick (nyp, nxp); shift_ms, gain, stretch (ny, nx) or None;
+        corr_t (nzp,) the filtered prior temporal correlation, A (3,), prior_var0 (3, 3) (divide)."""
+        a = ScaleWaveletArgs()
+        keep = []
+
+        def arr(x, dt, shape=None):
+            if x is None:
+                return None
+            y = np.ascontiguousarray(x, dtype=dt)
+            if shape is not None and y.shape != shape:
+                raise ValueError("expected shape %s, got %s" % (shape, y.shape))
+            keep.append(y)
+            return y
+
+        a.mode = mode
+        a.dt_ms = dt_ms
+        a.wavelet = _ptr(arr(wavelet, np.float32, (self.nzp,)), ctypes.c_float)
+        a.rel_thick = _ptr(arr(rel_thick, np.float64, (self.nyp, self.nxp)), ctypes.c_double)
+        a.shift_ms = _ptr(arr(shift_ms, np.float32, (self.ny, self.nx)), ctypes.c_float)
+        a.gain = _ptr(arr(gain, np.float32, (self.ny, self.nx)), ctypes.c_float)
+        a.stretch = _ptr(arr(stretch, np.float32, (self.ny, self.nx)), ctypes.c_float)
+        a.low_cut, a.high_cut, a.wnc, a.error_var = low_cut, high_cut, wnc, error_var
+        a.corr_t = _ptr(arr(corr_t, np.float32, (self.nzp,)), ctypes.c_float)
+        a.A = _ptr(arr(A, np.float32, (3,)), ctypes.c_float)
+        a.prior_var0 = _ptr(arr(prior_var0, np.float64, (3, 3)), ctypes.c_double)
+        self.ctx.check(lib().cg_grid_scale_wavelet(self._h, ctypes.byref(a)), "scale_wavelet")
+
     def interpolateSeismic(self, energy_threshold):
         """FFTGrid::interpolateSeismic; returns (bad traces, (imin, imax, jmin, jmax))."""
         nbad, rect = ctypes.c_int(), (ctypes.c_int * 4)()
@@ -379,6 +433,22 @@ class FFTGrid:
         geo = CravaGeometry(**geometry)
         self.ctx.check(lib().cg_grid_write_crava(self._h, (file_name + suffix).encode(), ctypes.byref(geo)),
                        "writeCravaFile")
+
+    def writeStormFile(self, file_name, geometry, padding=False, suffix=".storm"):
+        """FFTGrid::writeStormFile, binary: file_name + suffix (IO::SuffixStormBinary); geometry: a
+        dict of x_min, lx, y_min, ly, lz, angle (radians) and optionally top_name, bot_name."""
+        g = StormGeometry(geometry["x_min"], geometry["lx"], geometry["y_min"], geometry["ly"], geometry["lz"],
+                          geometry["angle"], (geometry.get("top_name") or "").encode() or None,
+                          (geometry.get("bot_name") or "").encode() or None)
+        self.ctx.check(lib().cg_grid_write_storm(self._h, (file_name + suffix).encode(), ctypes.byref(g),
+                                                 1 if padding else 0), "writeStormFile")
+
+    def readStormFile(self, file_name):
+        """A STORM binary grid of this grid's box size, then the padding fill of
+        FFTGrid(const NRLib::Grid *); returns the header's volume as a dict."""
+        g = StormGeometry()
+        self.ctx.check(lib().cg_grid_read_storm(self._h, file_name.encode(), ctypes.byref(g)), "readStormFile")
+        return dict(x_min=g.x_min, lx=g.lx, y_min=g.y_min, ly=g.ly, lz=g.lz, angle=g.angle)
 
     def readCravaFile(self, file_name):
         """FFTGrid::readCravaFile; returns the header's geometry as a dict."""
@@ -451,8 +521,10 @@ class FFTGrid:
             pass
 
 
-def build_kz_tables(nz, nzp, dt_ms, wavelets, A, low_cut=5.0, high_cut=55.0, min_rel_thick=1.0):
-    """Host-side per-k_z tables (K, e1, e2, e3, invert) of avoinversion.cpp:944-975."""
+def build_kz_tables(nz, nzp, dt_ms, wavelets, A, low_cut=5.0, high_cut=55.0, min_rel_thick=1.0,
+                    constant_thickness=True):
+    """Host-side per-k_z tables (K, e1, e2, e3, invert) of avoinversion.cpp:944-975, or with
+    constant_thickness=False those of the non-constant-thickness branch (:976-1014)."""
     w = np.ascontiguousarray(wavelets, dtype=np.float32)
     nt = w.shape[0]
     a = np.ascontiguousarray(A, dtype=np.float32)
@@ -462,7 +534,8 @@ def build_kz_tables(nz, nzp, dt_ms, wavelets, A, low_cut=5.0, high_cut=55.0, min
     e3 = np.zeros((nzp, nt), dtype=np.complex64)
     inv = np.zeros(nzp, dtype=np.int32)
     f = ctypes.c_float
-    rc = lib().cg_build_kz_tables(nz, nzp, dt_ms, nt, _ptr(w, f), _ptr(a, f), low_cut, high_cut, min_rel_thick,
+    fn = lib().cg_build_kz_tables if constant_thickness else lib().cg_build_kz_tables_variable_thickness
+    rc = fn(nz, nzp, dt_ms, nt, _ptr(w, f), _ptr(a, f), low_cut, high_cut, min_rel_thick,
                                   _ptr(K.view(np.float32), f), _ptr(e1.view(np.float32), f),
                                   _ptr(e2.view(np.float32), f), _ptr(e3.view(np.float32), f),
                                   _ptr(inv, ctypes.c_int))

@@ -13,6 +13,7 @@
 #include "cg_internal.h"
 #include "cg_noise.h"
 #include "cg_post.h"
+#include "cg_trace.h"
 
 namespace {
 
@@ -1088,7 +1089,7 @@ int cg_grid_fill_noise_counter(cg_grid *g, uint64_t seed, uint32_t stream)
 }
 
 // ---------------------------------------------------------------------------- posterior
-int cg_posterior(cg_ctx *ctx, const cg_post_args *a)
+static int post_validate(cg_ctx *ctx, const cg_post_args *a)
 {
   if (!ctx || !a) return CG_EINVAL;
   const int nt = a->ntheta;
@@ -1126,6 +1127,59 @@ int cg_posterior(cg_ctx *ctx, const cg_post_args *a)
   } else if (!a->lam2d) {
     return cg::fail(ctx, CG_EINVAL, "posterior: need err_corr grid or lam2d table");
   }
+  return CG_OK;
+}
+
+// Device parameters of a posterior call; the caller sets the table, list/mask and counter
+// pointers.
+static cg::PostParams post_params(const cg_post_args *a)
+{
+  const cg_grid *ref = a->mean[0];
+  cg::PostParams p;
+  std::memset(&p, 0, sizeof(p));
+  p.nxp = ref->nxp; p.nyp = ref->nyp; p.nzp = ref->nzp; p.cnxp = ref->cnxp; p.nt = a->ntheta;
+  p.csize = (long)ref->csize;
+  for (int g = 0; g < 3; ++g) p.mean[g] = reinterpret_cast<float2 *>(a->mean[g]->d);
+  for (int l = 0; l < a->ntheta; ++l) p.data[l] = reinterpret_cast<float2 *>(a->data[l]->d);
+  for (int g = 0; g < 6; ++g) p.cov[g] = reinterpret_cast<float2 *>(a->cov[g]->d);
+  p.lam_grid = a->err_corr ? reinterpret_cast<const float2 *>(a->err_corr->d) : nullptr;
+  const double *pv = a->prior_var0;
+  const double s6[6] = {pv[0], pv[4], pv[8], pv[1], pv[2], pv[5]};
+  for (int g = 0; g < 6; ++g) {
+    p.s6f[g] = (float)s6[g];
+    p.s6d[g] = s6[g];
+  }
+  p.prior_mode = a->prior_mode;
+  p.want_residual = a->want_residual;
+  p.precision = a->precision;
+  p.cond_limit = (float)(a->cond_limit > 0 ? a->cond_limit : 1e6);
+  return p;
+}
+
+// Statistics of a finished posterior call from its counters (device: [0] deferred, [8, 72)
+// inverted slots, [72, 136) failure slots).  Synchronises.
+static int post_stats(cg_ctx *ctx, const unsigned long long *dev_counters, bool mixed, uint64_t *stats)
+{
+  uint64_t h[136];
+  hipError_t e = hipMemcpyAsync(h, dev_counters, sizeof(h), hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) return cg::hip_fail(ctx, e, "posterior stats");
+  uint64_t inv_n = 0, fail_n = 0;
+  for (int s = 0; s < 64; ++s) {
+    inv_n += h[8 + s];
+    fail_n += h[8 + 64 + s];
+  }
+  stats[0] = inv_n;
+  stats[1] = fail_n;
+  stats[2] = mixed ? h[0] : 0;  // FP64 re-solves = deferred wavenumbers
+  return CG_OK;
+}
+
+int cg_posterior(cg_ctx *ctx, const cg_post_args *a)
+{
+  int rc0 = post_validate(ctx, a);
+  if (rc0) return rc0;
+  const cg_grid *ref = a->mean[0];
   DeviceGuard dg(ctx->device);
   const int nyp = ref->nyp, nzp = ref->nzp, cnxp = ref->cnxp;
   const size_t plane = (size_t)cnxp * nyp;
@@ -1169,30 +1223,13 @@ int cg_posterior(cg_ctx *ctx, const cg_post_args *a)
     }
     list = ctx->list;
   }
-  cg::PostParams p;
-  std::memset(&p, 0, sizeof(p));
-  p.nxp = ref->nxp; p.nyp = nyp; p.nzp = nzp; p.cnxp = cnxp; p.nt = nt;
-  p.csize = (long)ref->csize;
-  for (int g = 0; g < 3; ++g) p.mean[g] = reinterpret_cast<float2 *>(a->mean[g]->d);
-  for (int l = 0; l < nt; ++l) p.data[l] = reinterpret_cast<float2 *>(a->data[l]->d);
-  for (int g = 0; g < 6; ++g) p.cov[g] = reinterpret_cast<float2 *>(a->cov[g]->d);
+  cg::PostParams p = post_params(a);
   p.tab32 = static_cast<const cg::PlaneTab<float> *>(dp[0]);
   p.tab64 = static_cast<const cg::PlaneTab<double> *>(dp[1]);
   p.invert = static_cast<const int *>(dp[2]);
   p.rxy = static_cast<const float *>(dp[3]);
   p.rt = static_cast<const float *>(dp[4]);
   p.lam2d = static_cast<const float *>(dp[5]);
-  p.lam_grid = a->err_corr ? reinterpret_cast<const float2 *>(a->err_corr->d) : nullptr;
-  const double *pv = a->prior_var0;
-  const double s6[6] = {pv[0], pv[4], pv[8], pv[1], pv[2], pv[5]};
-  for (int g = 0; g < 6; ++g) {
-    p.s6f[g] = (float)s6[g];
-    p.s6d[g] = s6[g];
-  }
-  p.prior_mode = a->prior_mode;
-  p.want_residual = a->want_residual;
-  p.precision = a->precision;
-  p.cond_limit = (float)(a->cond_limit > 0 ? a->cond_limit : 1e6);
   p.list = list;
   // counters[0]: FP64 list length; counters[8, 136): statistics slots
   unsigned long long *cnt = reinterpret_cast<unsigned long long *>(ctx->counters);
@@ -1208,20 +1245,7 @@ int cg_posterior(cg_ctx *ctx, const cg_post_args *a)
     a->cov[g]->transformed = 1;
     a->cov[g]->cubetype = CG_CT_COVARIANCE;
   }
-  if (a->stats) {
-    uint64_t h[136];
-    hipError_t e = hipMemcpyAsync(h, ctx->counters, sizeof(h), hipMemcpyDeviceToHost, ctx->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    if (e != hipSuccess) return cg::hip_fail(ctx, e, "posterior stats");
-    uint64_t inv_n = 0, fail_n = 0;
-    for (int s = 0; s < 64; ++s) {
-      inv_n += h[8 + s];
-      fail_n += h[8 + 64 + s];
-    }
-    a->stats[0] = inv_n;
-    a->stats[1] = fail_n;
-    a->stats[2] = mixed ? h[0] : 0;  // FP64 re-solves = deferred wavenumbers
-  }
+  if (a->stats) return post_stats(ctx, cnt, mixed, a->stats);
   return CG_OK;
 }
 
@@ -1621,6 +1645,206 @@ int cg_build_kz_tables(int nz, int nzp, double dt_ms, int nt, const float *wav, 
       e2[2 * (k * nt + l)] = (float)sr / sn[l];
       e2[2 * (k * nt + l) + 1] = -((float)si / sn[l]);
       e3[2 * (k * nt + l)] = 1.0f;
+      e3[2 * (k * nt + l) + 1] = 0.0f;
+    }
+  }
+  return CG_OK;
+}
+
+// Wavelet::getCAmp(k, scale) (wavelet.cpp:337-394) on the half spectrum c[0, nzp/2]: the spectrum
+// of the wavelet stretched by `scale`, c(k / scale) / scale, linearly interpolated between bins
+// (zero beyond nzp/2), the Hermitian mirror for k > nzp/2; float arithmetic as the reference.
+static std::complex<float> stretched_bin(const std::vector<std::complex<float>> &c, int nzp, int k, float scale)
+{
+  const int cnzp = nzp / 2 + 1;
+  const bool mirror = k >= cnzp;
+  const float omega = float(mirror ? nzp - k : k) / scale;
+  if (omega >= cnzp) return std::complex<float>(0.0f, 0.0f);
+  const int omL = int(std::floor(omega));
+  int omU = omL + 1;
+  if (omU >= cnzp) omU -= 1;
+  const float d = omega - float(omL);
+  float re = c[omL].real() * (1.0f - d) + c[omU].real() * d;
+  float im = c[omL].imag() * (1.0f - d) + c[omU].imag() * d;
+  if (mirror) im = -im;
+  return std::complex<float>(re / scale, im / scale);
+}
+
+int cg_grid_scale_wavelet(cg_grid *g, const cg_scale_wavelet_args *a)
+{
+  if (!g || !a) return CG_EINVAL;
+  cg_ctx *ctx = g->ctx;
+  if (a->mode != CG_SCALE_WAVELET_DIVIDE && a->mode != CG_SCALE_WAVELET_MULTIPLY)
+    return cg::fail(ctx, CG_EINVAL, "scale_wavelet: unknown mode");
+  if (!a->wavelet || !a->rel_thick || !(a->dt_ms > 0))
+    return cg::fail(ctx, CG_EINVAL, "scale_wavelet: missing wavelet, relative thickness or sample interval");
+  if (a->mode == CG_SCALE_WAVELET_DIVIDE && (!a->corr_t || !a->A || !a->prior_var0))
+    return cg::fail(ctx, CG_EINVAL, "scale_wavelet: divide needs corr_t, A and prior_var0");
+  if (g->transformed) return cg::fail(ctx, CG_ESTATE, "scale_wavelet: grid is in the Fourier domain");
+  const int nzp = g->nzp, cnzp = nzp / 2 + 1, nz = g->nz;
+  for (long t = 0; t < (long)g->nxp * g->nyp; ++t)
+    if (!(a->rel_thick[t] > 0)) return cg::fail(ctx, CG_EINVAL, "scale_wavelet: relative thickness must be positive");
+  DeviceGuard dg(ctx->device);
+  std::vector<double> ct(nzp), st(nzp);
+  for (int m = 0; m < nzp; ++m) {
+    ct[m] = std::cos(2.0 * M_PI * m / nzp);
+    st[m] = std::sin(2.0 * M_PI * m / nzp);
+  }
+  std::vector<float> wv(a->wavelet, a->wavelet + nzp);
+  std::vector<double> wd(wv.begin(), wv.end());
+  std::vector<float2> W(cnzp), D3(cnzp);
+  std::vector<float> tau2(cnzp, 0.0f);
+  for (int m = 0; m < cnzp; ++m) {
+    double wr, wi;
+    dft_bin(wd, ct, st, m, wr, wi);
+    W[m] = make_float2((float)wr, (float)wi);
+    const double om = 2.0 * M_PI * m / nzp;
+    D3[m] = make_float2(0.0f, (float)(std::sin(om) * (2.0 * std::cos(om) - 2.0)));  // i sin(w) (2 cos(w) - 2)
+  }
+  const float wnorm = find_norm(wv);
+  // band norm of errorSmooth3's spectrum (Wavelet::findNormWithinFrequencyBand, wavelet.cpp:903-925)
+  const float dOmega = (float)(1.0 / (nzp * a->dt_ms * 0.001));
+  const int loI = std::max<int>(0, (int)std::floor(a->low_cut / dOmega));
+  const int hiI = std::min<int>(cnzp, (int)std::ceil(a->high_cut / dOmega));
+  double n2 = 0.0;
+  const double fac = 1.0 / (double)nzp;
+  for (int m = loI; m < hiI; ++m) {
+    const float sr = -D3[m].y * W[m].y, si = D3[m].y * W[m].x;  // (i d) (wr + i wi)
+    n2 += 2.0 * fac * (sr * sr + si * si);
+  }
+  if (loI == 0) {
+    const float sr = -D3[0].y * W[0].y, si = D3[0].y * W[0].x;
+    n2 -= fac * (sr * sr + si * si);
+  }
+  const float norm_f3 = (float)std::sqrt(n2);
+  const double lz = nz * a->dt_ms;
+  const float maxfrequency = (float)((nzp / 2) * 1000.0 * nz) / (float)(lz * nzp);
+  const float modw = wnorm * wnorm * (maxfrequency / a->high_cut);
+  if (a->mode == CG_SCALE_WAVELET_DIVIDE) {
+    // the reflection-coefficient time covariance (avoinversion.cpp:368-401) and its operator FFT
+    double eiVar = 0.0;
+    for (int l = 0; l < 3; ++l)
+      for (int m = 0; m < 3; ++m) eiVar += a->A[l] * (float)a->prior_var0[3 * l + m] * a->A[m];
+    std::vector<float> ei(nzp), rc(nzp);
+    for (int k = 0; k < nzp; ++k) ei[k] = (float)(eiVar * a->corr_t[k]);
+    for (int i = 0; i < nzp; ++i) {
+      const float next = ei[(i + 1) % nzp], prev = ei[(i + nzp - 1) % nzp];
+      rc[i] = 2 * ei[i] - next - prev;
+    }
+    std::vector<double> rcd(rc.begin(), rc.end());
+    for (int m = 0; m < cnzp; ++m) {
+      double r, im;
+      dft_bin(rcd, ct, st, m, r, im);
+      tau2[m] = std::fabs((float)r);
+    }
+  }
+  std::vector<float> taper;
+  for (int k = nz + 1; k < nzp; ++k) {
+    const float d = dist_to_boundary(k, nz, nzp);
+    taper.push_back(std::max<float>(1 - d * d, 0));
+  }
+  const size_t box = (size_t)g->nx * g->ny;
+  std::vector<void *> dp;
+  int rc = stage_tables(ctx,
+                        {{W.data(), sizeof(float2) * cnzp},
+                         {D3.data(), sizeof(float2) * cnzp},
+                         {tau2.data(), sizeof(float) * cnzp},
+                         {taper.empty() ? nullptr : taper.data(), sizeof(float) * taper.size()},
+                         {a->rel_thick, sizeof(double) * g->nxp * g->nyp},
+                         {a->shift_ms, a->shift_ms ? sizeof(float) * box : 0},
+                         {a->gain, a->gain ? sizeof(float) * box : 0},
+                         {a->stretch, a->stretch ? sizeof(float) * box : 0}},
+                        dp);
+  if (rc) return rc;
+  cg::ScaleWaveletParams p;
+  std::memset(&p, 0, sizeof(p));
+  p.data = reinterpret_cast<float2 *>(g->d);
+  p.mode = a->mode;
+  p.nx = g->nx; p.ny = g->ny; p.nz = nz; p.nxp = g->nxp; p.nyp = g->nyp; p.nzp = nzp; p.cnxp = g->cnxp;
+  p.lz = lz;
+  p.dz = (float)a->dt_ms;
+  p.low_cut = a->low_cut; p.high_cut = a->high_cut; p.wnc = a->wnc; p.error_var = a->error_var;
+  p.wnorm = wnorm; p.norm_f3 = norm_f3; p.modw = modw;
+  p.W = static_cast<const float2 *>(dp[0]);
+  p.D3 = static_cast<const float2 *>(dp[1]);
+  p.tau2 = static_cast<const float *>(dp[2]);
+  p.rel_thick = static_cast<const double *>(dp[4]);
+  p.shift = static_cast<const float *>(dp[5]);
+  p.gain = static_cast<const float *>(dp[6]);
+  p.stretch = static_cast<const float *>(dp[7]);
+  return cg::scale_wavelet_filter(ctx, p, static_cast<const float *>(dp[3]));
+}
+
+int cg_build_kz_tables_variable_thickness(int nz, int nzp, double dt_ms, int nt, const float *wav, const float *A,
+                                          float low_cut, float high_cut, float min_rel_thick, float *K, float *e1,
+                                          float *e2, float *e3, int *invert)
+{
+  if (nz < 1 || nzp < nz || nt < 1 || nt > cg::kMaxTheta || !wav || !A || !K || !e1 || !e2 || !e3 || !invert)
+    return CG_EINVAL;
+  std::vector<double> ct(nzp), st(nzp);
+  for (int m = 0; m < nzp; ++m) {
+    ct[m] = std::cos(2.0 * M_PI * m / nzp);
+    st[m] = std::sin(2.0 * M_PI * m / nzp);
+  }
+  // difference operators (wavelet1D.cpp:389-500): forward [-1, 0, ..., 0, +1]; D3 its backward then
+  // central difference, as sequences
+  std::vector<double> d1(nzp, 0.0), d2(nzp), d3(nzp);
+  d1[0] = -1.0;
+  d1[nzp - 1] += 1.0;
+  for (int i = 0; i < nzp; ++i) d2[i] = (float)(i == 0 ? d1[0] - d1[nzp - 1] : d1[i] - d1[i - 1]);
+  for (int i = 0; i < nzp; ++i) {
+    const double hi = i == nzp - 1 ? d2[0] : d2[i + 1], lo = i == 0 ? d2[nzp - 1] : d2[i - 1];
+    d3[i] = (float)(0.5 * (hi - lo));
+  }
+  const int cnzp = nzp / 2 + 1;
+  std::vector<std::vector<std::complex<float>>> Wc(nt, std::vector<std::complex<float>>(cnzp));
+  std::vector<float> wn(nt), sn(nt);
+  for (int l = 0; l < nt; ++l) {
+    const float *wl = wav + (size_t)l * nzp;
+    std::vector<float> w(wl, wl + nzp), f(nzp), b(nzp), c(nzp);
+    wn[l] = find_norm(w);
+    for (int i = 0; i < nzp; ++i) f[i] = (i == nzp - 1) ? w[0] - w[i] : w[i + 1] - w[i];
+    for (int i = 0; i < nzp; ++i) b[i] = (i == 0) ? f[0] - f[nzp - 1] : f[i] - f[i - 1];
+    for (int i = 0; i < nzp; ++i) {
+      const float hi = (i == nzp - 1) ? b[0] : b[i + 1];
+      const float lo = (i == 0) ? b[nzp - 1] : b[i - 1];
+      c[i] = (float)(0.5 * (hi - lo));
+    }
+    sn[l] = find_norm(c);  // errorSmooth3's norm
+    std::vector<double> wd(w.begin(), w.end());
+    for (int m = 0; m < cnzp; ++m) {  // seisWaveletForNorm's spectrum
+      double wr, wi;
+      dft_bin(wd, ct, st, m, wr, wi);
+      Wc[l][m] = std::complex<float>((float)wr, (float)wi);
+    }
+  }
+  const double lz = nz * dt_ms;
+  for (int k = 0; k < nzp; ++k) {
+    const float rf = (float)((nz * 1000.0f) / (lz * nzp) * std::min(k, nzp - k));
+    invert[k] = (rf > low_cut * (double)min_rel_thick) && (rf < high_cut);
+    double kdr, kdi, k3r, k3i;
+    dft_bin(d1, ct, st, k, kdr, kdi);
+    dft_bin(d3, ct, st, k, k3r, k3i);
+    for (int l = 0; l < nt; ++l) {
+      // K = D A (avoinversion.cpp:990-1000): the data were divided by the scale wavelet
+      for (int j = 0; j < 3; ++j) {
+        K[2 * ((k * nt + l) * 3 + j)] = (float)kdr * A[l * 3 + j];
+        K[2 * ((k * nt + l) * 3 + j) + 1] = (float)kdi * A[l * 3 + j];
+      }
+      // errMult1 = 1 / |w_l|, errMult2 = D3 / |errorSmooth3_l| (avoinversion.cpp:1002-1013)
+      e1[2 * (k * nt + l)] = 1.0f / wn[l];
+      e1[2 * (k * nt + l) + 1] = 0.0f;
+      const float sm = 1.0f / sn[l];
+      e2[2 * (k * nt + l)] = (float)k3r * sm;
+      e2[2 * (k * nt + l) + 1] = (float)k3i * sm;
+      // errMult3 = fillInverseAbskWRobust (avoinversion.cpp:1296-1322): 1 / the larger modulus of
+      // the wavelet spectrum at k and at k / 0.999
+      std::complex<float> v = k < cnzp ? Wc[l][k] : std::conj(Wc[l][nzp - k]);
+      const std::complex<float> vf = stretched_bin(Wc[l], nzp, k, 0.999f);
+      const float mod = v.real() * v.real() + v.imag() * v.imag();
+      const float modf = vf.real() * vf.real() + vf.imag() * vf.imag();
+      const float mx = std::max(mod, modf);
+      e3[2 * (k * nt + l)] = mx > 0.0f ? (float)(1.0 / std::sqrt(mx)) : wn[l] * nzp * nzp * 100.0f;
       e3[2 * (k * nt + l) + 1] = 0.0f;
     }
   }

@@ -168,6 +168,9 @@ int fft_x_r2c(cg_ctx *ctx, const float *src, float *grid, int nxp, long rows, fl
 int fft_x_c2r(cg_ctx *ctx, float *grid, int nxp, long rows, float scale, const float *add);
 int fft_strided(cg_ctx *ctx, float2 *data, int n, long elem_stride, long inner, long outer, long outer_stride,
                 int dir, float scale);
+// The same, timed as kernel class kclass (CG_K_*).
+int fft_strided(cg_ctx *ctx, float2 *data, int n, long elem_stride, long inner, long outer, long outer_stride,
+                int dir, float scale, int kclass);
 int grid_fft_forward(cg_grid *g, const float *src);
 // Batched passes: plan_fft_item completes an item whose geometry fields are set (kind x:
 // src/grid/add/nxp/rows/scale; strided: data/n/S/inner/outer/OS/scale) and returns the LDS

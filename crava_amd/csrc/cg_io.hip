@@ -6,6 +6,9 @@
 // The byte order is swapped on the device (one pass over the grid in HBM); the file moves
 // through a pinned host buffer in chunks.
 #include <algorithm>
+#include <cctype>
+#include <cmath>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -58,6 +61,38 @@ T get_be(const unsigned char *&p)
   T v;
   std::memcpy(&v, r, sizeof(T));
   return v;
+}
+
+// Big-endian copy of the box [0, nx) x [0, ny) x [0, nz) of a real grid (row pitch rnxp, nyp rows
+// per plane) into a contiguous nz * ny * nx array, i fastest (the STORM file order), and back.
+__global__ void __launch_bounds__(cg::kThreads)
+k_box_to_be(const float *rv, unsigned *out, int nx, int ny, int nz, int rnxp, int nyp)
+{
+  const size_t n = (size_t)nx * ny * nz;
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
+    const size_t i = t % nx, jk = t / nx, j = jk % ny, k = jk / ny;
+    out[t] = __builtin_bswap32(__float_as_uint(rv[i + (size_t)rnxp * (j + (size_t)nyp * k)]));
+  }
+}
+
+__global__ void __launch_bounds__(cg::kThreads) k_be_to_host32(const unsigned *in, float *out, size_t n)
+{
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)
+    out[t] = __uint_as_float(__builtin_bswap32(in[t]));
+}
+
+unsigned stream_blocks(size_t n)
+{
+  size_t b = (n + cg::kThreads - 1) / cg::kThreads;
+  if (b > 65536) b = 65536;
+  return (unsigned)(b ? b : 1);
+}
+
+std::string fixed6(double v)
+{
+  char buf[64];
+  snprintf(buf, sizeof(buf), "%.6f", v);  // NRLib::ToString(v, 6): std::fixed, precision 6
+  return buf;
 }
 
 // A pinned host buffer for one call.
@@ -183,6 +218,137 @@ int cg_grid_read_crava(cg_grid *g, const char *path, cg_crava_geometry *geo_out)
   if (rc == CG_OK) {
     g->transformed = 0;
     if (geo_out) *geo_out = geo;
+  }
+  return rc;
+}
+
+int cg_grid_write_storm(cg_grid *g, const char *path, const cg_storm_geometry *geo, int padding)
+{
+  if (!g || !path || !geo) return CG_EINVAL;
+  cg_ctx *ctx = g->ctx;
+  if (g->transformed) return cg::fail(ctx, CG_ESTATE, "write_storm: grid is in the Fourier domain");
+  (void)hipSetDevice(ctx->device);
+  const int nx = padding ? g->nxp : g->nx, ny = padding ? g->nyp : g->ny, nz = padding ? g->nzp : g->nz;
+  // Simbox::getStormHeader (simbox.cpp:1012-1034), binary form
+  std::string head = "storm_petro_binary\n";
+  head += "0 " + std::to_string(g->cubetype) + " " + fixed6(-99999.0) + "\n";
+  head += "FFTGrid\n";
+  if (geo->top_name && geo->bot_name)
+    head += fixed6(geo->x_min) + " " + fixed6(geo->lx) + " " + fixed6(geo->y_min) + " " + fixed6(geo->ly) + " " +
+            geo->top_name + " " + geo->bot_name + " 0.0 0.0\n";
+  else
+    head += fixed6(geo->x_min) + " " + fixed6(geo->lx) + " " + fixed6(geo->y_min) + " " + fixed6(geo->ly) +
+            " 0.0 " + fixed6(geo->lz) + " 0.0 0.0\n";
+  head += fixed6(geo->lz) + " " + fixed6(geo->angle * 180.0 / M_PI) + "\n\n";
+  head += std::to_string(nx) + " " + std::to_string(ny) + " " + std::to_string(nz) + "\n";
+  File file;
+  file.f = fopen(path, "wb");
+  if (!file.f) return cg::fail(ctx, CG_EIO, std::string("write_storm: cannot open '") + path + "'");
+  if (fwrite(head.data(), 1, head.size(), file.f) != head.size())
+    return cg::fail(ctx, CG_EIO, std::string("write_storm: cannot write '") + path + "'");
+  const size_t n = (size_t)nx * ny * nz, bytes = n * sizeof(float);
+  void *packed = nullptr;
+  hipError_t e = cg::dev_alloc(ctx, &packed, std::max<size_t>(bytes, 4));
+  if (e != hipSuccess) return cg::hip_fail(ctx, e, "write_storm: staging buffer");
+  hipEvent_t ev;
+  cg::launch_begin(ctx, CG_K_ELEMENTWISE, &ev);
+  k_box_to_be<<<stream_blocks(n), cg::kThreads, 0, ctx->stream>>>(g->d, static_cast<unsigned *>(packed), nx, ny, nz,
+                                                                   g->rnxp, g->nyp);
+  cg::launch_end(ctx, CG_K_ELEMENTWISE, ev);
+  int rc = cg::hip_fail(ctx, hipGetLastError(), "k_box_to_be");
+  Pinned host;
+  if (rc == CG_OK) {
+    e = hipHostMalloc(&host.p, kChunk, hipHostMallocDefault);
+    rc = cg::hip_fail(ctx, e, "write_storm: pinned buffer");
+  }
+  for (size_t off = 0; rc == CG_OK && off < bytes; off += kChunk) {
+    const size_t m = std::min(kChunk, bytes - off);
+    e = hipMemcpyAsync(host.p, static_cast<char *>(packed) + off, m, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    rc = cg::hip_fail(ctx, e, "write_storm: copy out");
+    if (rc == CG_OK && fwrite(host.p, 1, m, file.f) != m)
+      rc = cg::fail(ctx, CG_EIO, std::string("write_storm: cannot write '") + path + "'");
+  }
+  cg::dev_free(ctx, packed);
+  if (rc == CG_OK && (fputs("0\n", file.f) < 0 || fflush(file.f) != 0))
+    rc = cg::fail(ctx, CG_EIO, std::string("write_storm: cannot write '") + path + "'");
+  return rc;
+}
+
+// The next whitespace-separated token of the header (NRLib::ReadNext), or "" at the end.
+static std::string next_token(FILE *f)
+{
+  std::string t;
+  int c;
+  while ((c = fgetc(f)) != EOF && isspace(c)) {
+  }
+  while (c != EOF && !isspace(c)) {
+    t.push_back((char)c);
+    c = fgetc(f);
+  }
+  if (c != EOF) ungetc(c, f);
+  return t;
+}
+
+int cg_grid_read_storm(cg_grid *g, const char *path, cg_storm_geometry *geo_out)
+{
+  if (!g || !path) return CG_EINVAL;
+  cg_ctx *ctx = g->ctx;
+  (void)hipSetDevice(ctx->device);
+  File file;
+  file.f = fopen(path, "rb");
+  if (!file.f) return cg::fail(ctx, CG_EIO, std::string("read_storm: cannot open '") + path + "'");
+  // StormContGrid::ReadFromFile (stormcontgrid.cpp:82-170): format, zone number, model file name,
+  // missing code, variable name, the volume (x0 lx y0 ly top base erosion-top erosion-base, lz angle),
+  // nx ny nz, the rest of that line, then the big-endian floats and the barrier count
+  const std::string fmt = next_token(file.f);
+  if (fmt != "storm_petro_binary")
+    return cg::fail(ctx, CG_EINVAL, "read_storm: '" + std::string(path) + "' is not a STORM binary grid ('" + fmt + "')");
+  std::vector<std::string> tok;
+  for (int t = 0; t < 4 + 8 + 2 + 3; ++t) tok.push_back(next_token(file.f));
+  if (tok.back().empty()) return cg::fail(ctx, CG_EIO, std::string("read_storm: '") + path + "' ends inside the header");
+  const int nx = std::atoi(tok[14].c_str()), ny = std::atoi(tok[15].c_str()), nz = std::atoi(tok[16].c_str());
+  if (nx != g->nx || ny != g->ny || nz != g->nz)
+    return cg::fail(ctx, CG_EINVAL, "read_storm: the file's grid " + std::to_string(nx) + " x " + std::to_string(ny) +
+                                        " x " + std::to_string(nz) + " is not the grid's box " + std::to_string(g->nx) +
+                                        " x " + std::to_string(g->ny) + " x " + std::to_string(g->nz));
+  int c;
+  while ((c = fgetc(file.f)) != EOF && c != '\n') {  // DiscardRestOfLine
+  }
+  const size_t n = (size_t)nx * ny * nz, bytes = n * sizeof(float);
+  void *cube = nullptr;
+  hipError_t e = cg::dev_alloc(ctx, &cube, std::max<size_t>(bytes, 4));
+  if (e != hipSuccess) return cg::hip_fail(ctx, e, "read_storm: staging buffer");
+  Pinned host;
+  e = hipHostMalloc(&host.p, kChunk, hipHostMallocDefault);
+  int rc = cg::hip_fail(ctx, e, "read_storm: pinned buffer");
+  for (size_t off = 0; rc == CG_OK && off < bytes; off += kChunk) {
+    const size_t m = std::min(kChunk, bytes - off);
+    if (fread(host.p, 1, m, file.f) != m) {
+      rc = cg::fail(ctx, CG_EIO, std::string("read_storm: '") + path + "' ends inside the grid");
+      break;
+    }
+    e = hipMemcpyAsync(static_cast<char *>(cube) + off, host.p, m, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    rc = cg::hip_fail(ctx, e, "read_storm: copy in");
+  }
+  if (rc == CG_OK) {
+    k_be_to_host32<<<stream_blocks(n), cg::kThreads, 0, ctx->stream>>>(static_cast<const unsigned *>(cube),
+                                                                        static_cast<float *>(cube), n);
+    rc = cg::hip_fail(ctx, hipGetLastError(), "k_be_to_host32");
+  }
+  // FFTGrid(const NRLib::Grid<float> *, nxp, nyp, nzp): the cube and its padding fill
+  if (rc == CG_OK) rc = cg_grid_fill_padded(g, static_cast<const float *>(cube), 1);
+  cg::dev_free(ctx, cube);
+  if (rc == CG_OK && geo_out) {
+    geo_out->x_min = std::atof(tok[4].c_str());
+    geo_out->lx = std::atof(tok[5].c_str());
+    geo_out->y_min = std::atof(tok[6].c_str());
+    geo_out->ly = std::atof(tok[7].c_str());
+    geo_out->lz = std::atof(tok[12].c_str());
+    geo_out->angle = std::atof(tok[13].c_str()) * M_PI / 180.0;
+    geo_out->top_name = nullptr;
+    geo_out->bot_name = nullptr;
   }
   return rc;
 }

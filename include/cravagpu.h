@@ -198,6 +198,42 @@ int cg_grid_fill_trace_taper(cg_grid *grid, const float *cube, int cube_on_devic
  * below one sample, 0 elsewhere, RMISSING on the row padding.  Real domain. */
 int cg_grid_fill_err_corr(cg_grid *grid, const float *corr_xy, float grad_i, float grad_j);
 
+/* ------------------------------------------------------------------ non-constant thickness */
+/* The seismic data rescaling of the non-constant-thickness branch, trace by trace along z on a
+ * real-domain DATA grid (every trace transformed, filtered and transformed back on the device):
+ *  CG_SCALE_WAVELET_DIVIDE   AVOInversion::divideDataByScaleWavelet (avoinversion.cpp:485-579)
+ *      with computeAdjustmentFactor (:582-719): every padded trace, the z padding tapered by
+ *      max(1 - dist^2, 0) beyond nz (getDistToBoundary), the spectrum multiplied by the adjustment
+ *      factor of its local wavelet inside lowCut < deltaF k < highCut (deltaF from the trace's
+ *      relative thickness) and set to zero outside.  The reference then calls interpolateSeismic
+ *      (cg_grid_interpolate_seismic);
+ *  CG_SCALE_WAVELET_MULTIPLY multiplyDataByScaleWaveletAndWriteToFile (:721-790) before the write:
+ *      every box trace's spectrum times the conjugate of its local wavelet stretched by relThick *
+ *      localStretch; the padding traces are left (up to FP32 rounding of the transform pair).
+ * The local wavelet of a trace is the angle's wavelet shifted by shift_ms and scaled by gain
+ * (Wavelet1D::shiftAndScale, wavelet1D.cpp:580-602) and stretched (Wavelet::getCAmp(k, scale),
+ * wavelet.cpp:337-394); divide looks it up at the box trace the reference mirrors a padded trace
+ * to (avoinversion.cpp:516-533).  No reference counterpart in the signature: the arrays replace
+ * Simbox and Wavelet queries. */
+#define CG_SCALE_WAVELET_DIVIDE 0
+#define CG_SCALE_WAVELET_MULTIPLY 1
+typedef struct {
+  int mode;                  /* CG_SCALE_WAVELET_* */
+  double dt_ms;              /* sample interval (lz = nz dt) */
+  const float *wavelet;      /* [nzp] the angle's wavelet, time samples in FFT order */
+  const double *rel_thick;   /* [nyp][nxp] Simbox::getRelThick at every padded trace */
+  const float *shift_ms;     /* [ny][nx] local time shifts (Wavelet::getLocalTimeshift) or null */
+  const float *gain;         /* [ny][nx] local gain factors (getLocalGainFactor) or null */
+  const float *stretch;      /* [ny][nx] local stretch (getLocalStretch) or null */
+  float low_cut, high_cut;   /* Hz */
+  float wnc;                 /* white-noise component */
+  float error_var;           /* errThetaCov[l][l] of the angle */
+  const float *corr_t;       /* [nzp] SeismicParametersHolder::getPriorCorrTFiltered(nz, nzp) */
+  const float *A;            /* [3] the angle's reflection coefficients */
+  const double *prior_var0;  /* [3][3] */
+} cg_scale_wavelet_args;
+int cg_grid_scale_wavelet(cg_grid *grid, const cg_scale_wavelet_args *args);
+
 /* ------------------------------------------------------------------ CRAVA binary grid files */
 /* Survey geometry stored in the file header (Simbox getters written by writeCravaFile). */
 typedef struct {
@@ -215,6 +251,27 @@ int cg_grid_write_crava(cg_grid *grid, const char *path, const cg_crava_geometry
  * not checked.  Sets the real domain; geometry (may be null) receives the header.  CG_EIO if the
  * file cannot be opened or ends early (the grid's contents are then undefined). */
 int cg_grid_read_crava(cg_grid *grid, const char *path, cg_crava_geometry *geometry);
+
+/* ------------------------------------------------------------------ STORM grid files */
+/* Simbox geometry in a STORM header (Simbox::getStormHeader, simbox.cpp:1012-1034). */
+typedef struct {
+  double x_min, lx, y_min, ly;  /* GetXMin, GetLX, GetYMin, GetLY */
+  double lz;                    /* GetLZ */
+  double angle;                 /* GetAngle, radians (the header stores degrees) */
+  const char *top_name;         /* surface names of a non-flat simbox, or null for a flat one */
+  const char *bot_name;
+} cg_storm_geometry;
+/* FFTGrid::writeStormFile (fftgrid.cpp:1889-1957), binary form, to `path` as given (the reference
+ * appends IO::SuffixStormBinary()): the header of getStormHeader with the grid's cube type, then
+ * the box (padding = 0) or the padded cube (padding = 1) of the real grid, k, j, i order, i fastest,
+ * big-endian floats (packed and byte-swapped on the device), then "0\n".  Real domain; CG_EIO if the
+ * file cannot be written. */
+int cg_grid_write_storm(cg_grid *grid, const char *path, const cg_storm_geometry *geometry, int padding);
+/* StormContGrid::ReadFromFile (stormcontgrid.cpp:82-170) of a STORM binary grid whose nx, ny, nz are
+ * the grid's box, followed by the padding fill of FFTGrid(const NRLib::Grid *) (as
+ * cg_grid_fill_padded).  geometry (may be null) receives the volume (surface names are not
+ * returned).  CG_EINVAL for another format or size, CG_EIO if the file cannot be read or ends early. */
+int cg_grid_read_storm(cg_grid *grid, const char *path, cg_storm_geometry *geometry);
 
 /* ------------------------------------------------------------------ posterior (AVOInversion) */
 #define CG_PRIOR_GRIDS 0     /* six prior covariance spectra, as SeismicParametersHolder holds */
@@ -368,6 +425,16 @@ int cg_grid_data_variance(cg_grid *grid, float *variance, uint64_t *count);
 int cg_build_kz_tables(int nz, int nzp, double dt_ms, int ntheta, const float *wavelets, const float *A,
                        float low_cut, float high_cut, float min_rel_thick, float *K, float *e1, float *e2,
                        float *e3, int *invert);
+/* The same tables for the non-constant-thickness branch (avoinversion.cpp:976-1014, used when the
+ * simbox is not of constant thickness and the seismic data have been divided by the scale wavelet,
+ * cg_grid_divide_by_scale_wavelet): K = D A with D the forward difference operator, e1 = 1 / |w_l|,
+ * e2 = D3 / |errorSmooth3_l| with D3 the central, backward and forward differences, and
+ * e3 = fillInverseAbskWRobust (:1296-1322), 1 / max(|W_l(k)|, |W_l(k / 0.999)| / 0.999) with the
+ * stretched spectrum of Wavelet::getCAmp(k, scale) (wavelet.cpp:337-394).  min_rel_thick: the
+ * simbox's minimum relative thickness (the band test is f > low_cut min_rel_thick). */
+int cg_build_kz_tables_variable_thickness(int nz, int nzp, double dt_ms, int ntheta, const float *wavelets,
+                                          const float *A, float low_cut, float high_cut, float min_rel_thick,
+                                          float *K, float *e1, float *e2, float *e3, int *invert);
 /* FFTGrid::findClosestFactorableNumber (fftgrid.cpp:2630-2658). */
 int cg_factorable(int leastint);
 /* SeismicParametersHolder::MakeCircAutoCovPosDef (seismicparametersholder.cpp:661-767), in place on

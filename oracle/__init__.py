@@ -373,6 +373,167 @@ def kz_tables(nz, nzp, dt_ms, wavelets, A, low_cut=5.0, high_cut=55.0, min_rel_t
     return K, e1, e2, e3, inv
 
 
+def stretched_spectrum(half, nzp, k, scale):
+    """Wavelet::getCAmp(k, scale) (wavelet.cpp:337-394): the half spectrum `half` (bins 0..nzp/2)
+    read at k / scale by linear interpolation between bins (zero at or beyond nzp/2 + 1), the
+    Hermitian mirror for k > nzp/2, divided by scale."""
+    cnzp = nzp // 2 + 1
+    mirror = k >= cnzp
+    om = (nzp - k if mirror else k) / scale
+    if om >= cnzp:
+        return 0j
+    lo = int(np.floor(om))
+    hi = lo + 1 if lo + 1 < cnzp else lo
+    d = om - lo
+    v = half[lo] * (1 - d) + half[hi] * d
+    return (np.conj(v) if mirror else v) / scale
+
+
+def kz_tables_variable_thickness(nz, nzp, dt_ms, wavelets, A, low_cut=5.0, high_cut=55.0, min_rel_thick=1.0):
+    """Per-k_z tables of the non-constant-thickness branch (avoinversion.cpp:976-1014), in double:
+    K = D A, e1 = 1 / |w|, e2 = D3 / |errorSmooth3|, e3 = fillInverseAbskWRobust (:1296-1322);
+    D = FFT of the forward-difference sequence, D3 = FFT of central(backward(forward-difference))
+    (wavelet1D.cpp:389-500)."""
+    w = np.asarray(wavelets, dtype=np.float64)
+    nt = w.shape[0]
+    d1 = np.zeros(nzp)
+    d1[0], d1[nzp - 1] = -1.0, 1.0
+    d2 = d1 - np.roll(d1, 1)
+    d3 = 0.5 * (np.roll(d2, -1) - np.roll(d2, 1))
+    D, D3 = np.fft.fft(d1), np.fft.fft(d3)
+    K = np.zeros((nzp, nt, 3), dtype=np.complex128)
+    e1 = np.zeros((nzp, nt), dtype=np.complex128)
+    e2 = np.zeros((nzp, nt), dtype=np.complex128)
+    e3 = np.zeros((nzp, nt), dtype=np.complex128)
+    for l in range(nt):
+        f = np.roll(w[l], -1) - w[l]
+        b = f - np.roll(f, 1)
+        c = 0.5 * (np.roll(b, -1) - np.roll(b, 1))
+        W = np.fft.fft(w[l])
+        half = W[:nzp // 2 + 1]
+        K[:, l, :] = D[:, None] * np.asarray(A[l], dtype=np.float64)[None, :]
+        e1[:, l] = 1.0 / np.linalg.norm(w[l])
+        e2[:, l] = D3 / np.linalg.norm(c)
+        for k in range(nzp):
+            m2 = max(abs(W[k]) ** 2, abs(stretched_spectrum(half, nzp, k, 0.999)) ** 2)
+            e3[k, l] = 1.0 / np.sqrt(m2) if m2 > 0 else np.linalg.norm(w[l]) * nzp * nzp * 100.0
+    k = np.arange(nzp)
+    rf = nz * 1000.0 / (nz * dt_ms * nzp) * np.minimum(k, nzp - k)
+    inv = ((rf > low_cut * min_rel_thick) & (rf < high_cut)).astype(np.int32)
+    return K, e1, e2, e3, inv
+
+
+def reflection_time_covariance_spectrum(corr_t_filtered, A, prior_var0):
+    """AVOInversion::computeReflectionCoefficientTimeCovariance (avoinversion.cpp:368-401) and its
+    operator FFT (:649-650): the elastic-impedance variance A S0 A^T times the filtered circular
+    temporal correlation, the second difference 2 c[i] - c[i+1] - c[i-1] (cyclic), half spectrum."""
+    ct = np.asarray(corr_t_filtered, dtype=np.float64)
+    A = np.asarray(A, dtype=np.float64)
+    ei = float(A @ np.asarray(prior_var0, dtype=np.float64) @ A) * ct
+    rc = 2 * ei - np.roll(ei, -1) - np.roll(ei, 1)
+    return np.fft.rfft(rc)
+
+
+def scale_wavelet_filter(rv, nx, ny, nz, nxp, dt_ms, wavelet, mode, rel_thick, low_cut=5.0, high_cut=55.0,
+                         wnc=0.1, error_var=1.0, tau_spec=None, shift=None, gain=None, stretch=None):
+    """AVOInversion::divideDataByScaleWavelet (mode "divide", avoinversion.cpp:485-579, with
+    computeAdjustmentFactor :582-719) or multiplyDataByScaleWaveletAndWriteToFile (mode "multiply",
+    :721-790) on a copy of the padded real grid rv (nzp, nyp, rnxp), trace by trace in double.
+    rel_thick (nyp, nxp): the simbox's relative thickness at every padded trace; shift (ms), gain,
+    stretch (ny, nx): the local wavelet grids (None: 0, 1, 1); tau_spec: the half spectrum of the
+    reflection-coefficient time covariance (reflection_time_covariance_spectrum).  The wavelet is
+    (nzp,) in FFT order; its time-domain norm is |w| (shiftAndScale leaves the norm unchanged)."""
+    out = np.array(rv, dtype=np.float64, copy=True)
+    nzp, nyp, _ = out.shape
+    cnzp = nzp // 2 + 1
+    w = np.asarray(wavelet, dtype=np.float64)
+    W = np.fft.rfft(w)
+    wnorm = np.linalg.norm(w)
+    om = 2 * np.pi * np.arange(cnzp) / nzp
+    D3 = 1j * np.sin(om) * (2 * np.cos(om) - 2)  # central(backward(forward)) difference operator
+    lz = nz * dt_ms
+    tol_fac = 0.05
+    maxfreq = (nzp // 2) * 1000.0 * nz / (lz * nzp)
+    modW = wnorm ** 2 * maxfreq / high_cut
+    d_omega = 1.0 / (nzp * dt_ms * 0.001)  # findNormWithinFrequencyBand's bins (wavelet.cpp:903-925)
+    lo_i = max(0, int(np.floor(low_cut / d_omega)))
+    hi_i = min(cnzp, int(np.ceil(high_cut / d_omega)))
+    ks = np.arange(cnzp)
+
+    def band_norm(c):
+        n2 = 2.0 / nzp * np.sum(np.abs(c[lo_i:hi_i]) ** 2)
+        if lo_i == 0:
+            n2 -= np.abs(c[0]) ** 2 / nzp
+        return np.sqrt(n2)
+
+    def stretched(c, sf):
+        return np.array([stretched_spectrum(c, nzp, k, sf) for k in range(cnzp)])
+
+    def local(i, j):
+        ts = 0.0 if shift is None or not (0 <= i < nx and 0 <= j < ny) else float(shift[j, i])
+        g = 1.0 if gain is None or not (0 <= i < nx and 0 <= j < ny) else float(gain[j, i])
+        st = 1.0 if stretch is None or not (0 <= i < nx and 0 <= j < ny) else float(stretch[j, i])
+        # Wavelet1D::shiftAndScale (wavelet1D.cpp:580-602)
+        amp = g * np.exp(-2j * np.pi * (ts / dt_ms) * ks / nzp)
+        return W * amp, st
+
+    taper = np.ones(nzp)
+    for k in range(nz + 1, nzp):
+        d = dist_to_boundary(k, nz, nzp)
+        taper[k] = max(1.0 - d * d, 0.0)
+    xs = range(nxp) if mode == "divide" else range(nx)
+    ys = range(nyp) if mode == "divide" else range(ny)
+    for j in ys:
+        for i in xs:
+            if mode == "divide":  # the local wavelet of the mirrored box trace
+                ii = 0 if i > 3 * nx - 1 else i
+                jj = 0 if j > 3 * ny - 1 else j
+                if ii > (nxp + nx) // 2:
+                    ii = nxp - ii
+                if ii >= nx:
+                    ii = 2 * nx - ii - 1
+                if jj > (nyp + ny) // 2:
+                    jj = nyp - jj
+                if jj >= ny:
+                    jj = 2 * ny - jj - 1
+            else:
+                ii, jj = i, j
+            Wl, st = local(ii, jj)
+            rel = float(rel_thick[j, i])
+            sf = rel * st
+            tr = out[:, j, i] / np.sqrt(nzp)
+            if mode == "divide":
+                tr = tr * taper
+            X = np.fft.rfft(tr)
+            if mode == "multiply":
+                Y = X * np.conj(stretched(Wl, sf))
+            else:
+                Sl = D3 * Wl
+                Sl = Sl / band_norm(Sl)
+                wloc = stretched(Wl, sf)
+                sloc = stretched(Sl, sf)
+                size_loc = np.abs(wloc) ** 2 / wnorm ** 2 + np.abs(sloc) ** 2
+                size_glob = np.abs(Wl) ** 2 / wnorm ** 2 + np.abs(Sl) ** 2
+                s2loc = error_var * (wnc + (1 - wnc) * 0.5 * size_loc)
+                s2glob = error_var * (wnc + (1 - wnc) * 0.5 * size_glob)
+                tau2 = np.abs(np.real(tau_spec)) if tau_spec is not None else np.zeros(cnzp)
+                wl2 = np.abs(wloc) ** 2
+                wg2 = np.abs(W) ** 2
+                wg2a = np.where(wg2 < tol_fac * modW, 0.5 * (wg2 + tol_fac * modW), wg2)
+                wl2a = np.where(wl2 < tol_fac * modW, 0.5 * (wl2 + tol_fac * modW), wl2)
+                low = wl2a < tol_fac * wg2a
+                wl2a_new = np.where(low, 0.5 * (wl2 + tol_fac * wg2a), wl2a)
+                with np.errstate(divide="ignore", invalid="ignore"):
+                    s2loc = np.where(low, s2loc * wl2a_new / wl2, s2loc)
+                wl2a = wl2a_new
+                factor = (wloc / (tau2 * wl2a + s2loc)) / (wg2a / (tau2 * wg2 + s2glob))
+                deltaF = nz * 1000.0 / (rel * lz * nzp)
+                band = (deltaF * ks < high_cut) & (deltaF * ks > low_cut)
+                Y = np.where(band, X * factor, 0.0)
+            out[:, j, i] = np.fft.irfft(Y, n=nzp) * nzp / np.sqrt(nzp)
+    return out
+
+
 def corr_xy(nxp, nyp, dx, dy, range1, range2=None, angle=0.0):
     out = np.zeros((nyp, nxp), dtype=np.float32)
     lib().orc_corr_xy(nxp, nyp, dx, dy, range1, range1 if range2 is None else range2, angle,
@@ -456,6 +617,52 @@ def read_crava(path):
         rnxp, nyp, nzp = struct.unpack(">iii", f.read(12))
         rv = np.frombuffer(f.read(4 * rnxp * nyp * nzp), dtype=">f4").astype(np.float32)
     return geometry, rv.reshape(nzp, nyp, rnxp)
+
+
+def storm_header(cubetype, nx, ny, nz, geometry):
+    """Simbox::getStormHeader (simbox.cpp:1012-1034), binary form: numbers with std::fixed and
+    precision 6 (NRLib::ToString(v, 6)); a flat simbox (no surface names) writes top 0.0 and base lz."""
+    f6 = lambda v: "%.6f" % v
+    g = geometry
+    h = "storm_petro_binary\n0 %d %s\nFFTGrid\n" % (cubetype, f6(-99999.0))
+    if g.get("top_name") and g.get("bot_name"):
+        h += " ".join([f6(g["x_min"]), f6(g["lx"]), f6(g["y_min"]), f6(g["ly"]), g["top_name"], g["bot_name"],
+                       "0.0", "0.0"]) + "\n"
+    else:
+        h += " ".join([f6(g["x_min"]), f6(g["lx"]), f6(g["y_min"]), f6(g["ly"]), "0.0", f6(g["lz"]), "0.0", "0.0"]) + "\n"
+    h += f6(g["lz"]) + " " + f6(g["angle"] * 180.0 / np.pi) + "\n\n"
+    h += "%d %d %d\n" % (nx, ny, nz)
+    return h
+
+
+def write_storm(path, cube, cubetype, geometry):
+    """FFTGrid::writeStormFile (fftgrid.cpp:1889-1957), binary: the header, then the cube (nz, ny, nx)
+    as big-endian floats, i fastest, then "0\n"."""
+    nz, ny, nx = cube.shape
+    with open(path, "wb") as f:
+        f.write(storm_header(cubetype, nx, ny, nz, geometry).encode())
+        f.write(np.ascontiguousarray(cube, dtype=">f4").tobytes())
+        f.write(b"0\n")
+
+
+def read_storm(path):
+    """StormContGrid::ReadFromFile (stormcontgrid.cpp:82-170) of a binary grid: (tokens of the header,
+    cube (nz, ny, nx))."""
+    with open(path, "rb") as f:
+        data = f.read()
+    toks, pos = [], 0
+    while len(toks) < 18:  # format, 4 more, the volume (8 + 2), nx ny nz
+        while data[pos:pos + 1].isspace():
+            pos += 1
+        end = pos
+        while end < len(data) and not data[end:end + 1].isspace():
+            end += 1
+        toks.append(data[pos:end].decode())
+        pos = end
+    pos = data.index(b"\n", pos) + 1  # the rest of the nx ny nz line
+    nx, ny, nz = int(toks[15]), int(toks[16]), int(toks[17])
+    cube = np.frombuffer(data[pos:pos + 4 * nx * ny * nz], dtype=">f4").astype(np.float32).reshape(nz, ny, nx)
+    return toks, cube
 
 
 def fill_param_corr(corr_xy, corr_t, grad_i=0.0, grad_j=0.0, scale=1.0):

@@ -248,3 +248,51 @@ def test_real_accessors(ctx):
     assert len(t) == nzp and np.array_equal(t[:nz], rv[:nz, 3, 4]) and np.all(t[nz:] == -99999.0)
     assert np.all(g.getRealTrace(nx, 0) == -99999.0)
     g.close()
+
+
+STORM_GEO = dict(x_min=431250.5, lx=12800.0, y_min=6472100.25, ly=9600.0, lz=1024.0, angle=0.5235987755982988)
+
+
+@pytest.mark.parametrize("dims", [(7, 5, 6, 12, 8, 10), (9, 4, 5, 9, 6, 8), (64, 48, 40, 80, 60, 48)])
+@pytest.mark.parametrize("padding", [False, True])
+def test_storm_file_matches_oracle(ctx, tmp_path, dims, padding):
+    """writeStormFile from the device gives the oracle's bytes (box or padded cube); reading the
+    oracle's file of the box restores the cube with the padding fill of FFTGrid(NRLib::Grid)."""
+    nx, ny, nz, nxp, nyp, nzp = dims
+    print("storm file", dims, padding, flush=True)
+    rv = _seismic(dims, 41)
+    g = _grid(ctx, dims, C.DATA)
+    g.upload_real(rv)
+    g.writeStormFile(str(tmp_path / "dev"), STORM_GEO, padding=padding)
+    cube = rv[:nzp, :nyp, :nxp] if padding else rv[:nz, :ny, :nx]
+    O.write_storm(str(tmp_path / "orc.storm"), cube, C.DATA, STORM_GEO)
+    assert open(tmp_path / "dev.storm", "rb").read() == open(tmp_path / "orc.storm", "rb").read()
+    if not padding:
+        h = _grid(ctx, dims)
+        geo = h.readStormFile(str(tmp_path / "orc.storm"))
+        assert all(abs(geo[k] - STORM_GEO[k]) < 1e-6 for k in STORM_GEO)
+        got = h.download_real(padded=True)
+        want = O.fill_padded(cube, nxp, nyp, nzp)
+        assert np.array_equal(got[:, :, :nxp].view(np.uint32), want[:, :, :nxp].view(np.uint32))
+        h.close()
+    g.close()
+
+
+def test_storm_file_errors(ctx, tmp_path):
+    dims = (9, 7, 5, 9, 7, 5)
+    O.write_storm(str(tmp_path / "a.storm"), np.zeros((5, 7, 9), np.float32), 1, STORM_GEO)
+    other = _grid(ctx, (8, 7, 5, 9, 7, 5))
+    with pytest.raises(C.CravaError, match="not the grid's box"):
+        other.readStormFile(str(tmp_path / "a.storm"))
+    g = _grid(ctx, dims)
+    with pytest.raises(C.CravaError, match="cannot open"):
+        g.readStormFile(str(tmp_path / "missing.storm"))
+    raw = open(tmp_path / "a.storm", "rb").read()
+    open(tmp_path / "short.storm", "wb").write(raw[:-40])
+    with pytest.raises(C.CravaError, match="ends inside the grid"):
+        g.readStormFile(str(tmp_path / "short.storm"))
+    open(tmp_path / "crava.storm", "wb").write(b"crava_fftgrid_binary\n" + raw)
+    with pytest.raises(C.CravaError, match="not a STORM binary grid"):
+        g.readStormFile(str(tmp_path / "crava.storm"))
+    other.close()
+    g.close()

@@ -218,3 +218,27 @@ def test_synthetic_seismic_oracle_time_domain():
                 d = [tr[u] - tr[u - 1] for u in range(nzp)]
                 y = [sum(d[u] * w[l, (u - t) % nzp] for u in range(nzp)) for t in range(nz)]
                 assert np.allclose(got[l][:, j, i], y, rtol=1e-9, atol=1e-9)
+
+
+STORM_GEO = dict(x_min=431250.5, lx=12800.0, y_min=6472100.25, ly=9600.0, lz=1024.0, angle=0.5235987755982988)
+
+
+def test_storm_file_layout(tmp_path):
+    """The oracle's STORM binary grid against the layout getStormHeader writes (hand-written text for
+    these numbers: fixed notation with six decimals, the angle in degrees, a blank line before the
+    dimensions), then big-endian floats i fastest and the barrier count 0."""
+    cube = np.random.default_rng(2).normal(size=(4, 3, 5)).astype(np.float32)
+    path = str(tmp_path / "g.storm")
+    O.write_storm(path, cube, 3, STORM_GEO)
+    raw = open(path, "rb").read()
+    head = (b"storm_petro_binary\n0 3 -99999.000000\nFFTGrid\n"
+            b"431250.500000 12800.000000 6472100.250000 9600.000000 0.0 1024.000000 0.0 0.0\n"
+            b"1024.000000 30.000000\n\n5 3 4\n")
+    assert raw.startswith(head) and raw.endswith(b"0\n") and len(raw) == len(head) + 4 * cube.size + 2
+    body = np.frombuffer(raw, dtype=">f4", count=cube.size, offset=len(head)).reshape(cube.shape)
+    assert np.array_equal(body.astype(np.float32), cube)
+    toks, back = O.read_storm(path)
+    assert toks[0] == "storm_petro_binary" and toks[3] == "-99999.000000" and toks[4] == "FFTGrid"
+    assert np.array_equal(back, cube)
+    named = dict(STORM_GEO, top_name="top.irap", bot_name="base.irap")
+    assert O.storm_header(2, 5, 3, 4, named).splitlines()[3].split()[4:] == ["top.irap", "base.irap", "0.0", "0.0"]
