@@ -97,6 +97,7 @@ def lib():
             "cg_fft_r2c_inplace": [_vp], "cg_fft_c2r_inplace": [_vp],
             "cg_grid_add": [_vp, _vp], "cg_grid_multiply_by_scalar": [_vp, f], "cg_grid_fill_constant": [_vp, f],
             "cg_posterior": [_vp, ctypes.POINTER(PostArgs)], "cg_simulate": [_vp, ctypes.POINTER(SimArgs)],
+            "cg_posterior_noalloc": [_vp, ctypes.POINTER(PostArgs)],
             "cg_simulate_pair": [_vp, ctypes.POINTER(SimArgs), _vp * 3],
             "cg_grid_fill_noise_counter": [_vp, u64, ctypes.c_uint32],
             "cg_grid_fill_noise_reference": [_vp, ctypes.POINTER(ctypes.c_uint32)],
@@ -554,10 +555,11 @@ def forward_model(ctx, K, m, d):
 
 
 def posterior(ctx, tables, mean, data, cov, prior_mode=PRIOR_SEPARABLE, rxy=None, rt=None, err_corr=None,
-              lam2d=None, want_residual=True, precision=PREC_REFERENCE, cond_limit=1e6, stats=True):
+              lam2d=None, want_residual=True, precision=PREC_REFERENCE, cond_limit=1e6, stats=True, noalloc=False):
     """Per-wavenumber posterior on Fourier-domain FFTGrids (in place).
 
-    tables: dict with K, e1, e2, e3, invert, sigma0e, wnc, prior_var0.
+    tables: dict with K, e1, e2, e3, invert, sigma0e, wnc, prior_var0.  noalloc: the entry point
+    that allocates no device memory (cg_posterior_noalloc).
     Returns (inverted, cholesky_failures, fp64_resolves)."""
     a = PostArgs()
     keep = []
@@ -598,7 +600,10 @@ def posterior(ctx, tables, mean, data, cov, prior_mode=PRIOR_SEPARABLE, rxy=None
     stats = np.zeros(3, dtype=np.uint64)
     # statistics cost a device-to-host copy and a stream synchronisation
     a.stats = _ptr(stats, ctypes.c_uint64) if want_stats else None
-    ctx.check(lib().cg_posterior(ctx._h, ctypes.byref(a)), "cg_posterior")
+    if noalloc:
+        ctx.check(lib().cg_posterior_noalloc(ctx._h, ctypes.byref(a)), "cg_posterior_noalloc")
+    else:
+        ctx.check(lib().cg_posterior(ctx._h, ctypes.byref(a)), "cg_posterior")
     return tuple(int(x) for x in stats)
 
 

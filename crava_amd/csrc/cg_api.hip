@@ -265,6 +265,9 @@ int cg_ctx_create(cg_ctx **out, int device, uint64_t hbm_budget_bytes)
   hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipMalloc(&ctx->counters, 256 * sizeof(uint64_t));
   if (e == hipSuccess) e = hipDeviceGetAttribute(&ctx->num_cu, hipDeviceAttributeMultiprocessorCount, device);
+  // the allocation-free posterior's buffer: its address on this device, looked up once per device
+  // (a failure is reported by cg_posterior_noalloc, the only user)
+  if (e == hipSuccess) (void)cg::noalloc_scratch_init(device);
   if (e == hipSuccess) {
     int pools = 0;
     if (hipDeviceGetAttribute(&pools, hipDeviceAttributeMemoryPoolsSupported, device) == hipSuccess && pools) {
@@ -1242,6 +1245,79 @@ int cg_posterior(cg_ctx *ctx, const cg_post_args *a)
   rc = cg::launch_posterior(ctx, p);
   if (rc) return rc;
   for (int g = 0; g < 6; ++g) {  // the covariance grids now hold posterior covariance spectra
+    a->cov[g]->transformed = 1;
+    a->cov[g]->cubetype = CG_CT_COVARIANCE;
+  }
+  if (a->stats) return post_stats(ctx, cnt, mixed, a->stats);
+  return CG_OK;
+}
+
+int cg_posterior_noalloc(cg_ctx *ctx, const cg_post_args *a)
+{
+  int rc = post_validate(ctx, a);
+  if (rc) return rc;
+  const cg_grid *ref = a->mean[0];
+  const int nzp = ref->nzp;
+  const size_t plane = (size_t)ref->cnxp * ref->nyp;
+  if ((size_t)nzp > cg::kNoAllocMaxNzp || plane > cg::kNoAllocMaxPlane || ref->csize > cg::kNoAllocMaxCells)
+    return cg::fail(ctx, CG_EINVAL, "posterior_noalloc: grid larger than the fixed buffer allows (nzp <= 1536, "
+                                    "(nxp/2+1)*nyp <= 2^21 and (nxp/2+1)*nyp*nzp <= 2^28 complex cells)");
+  DeviceGuard dg(ctx->device);
+  char *base = static_cast<char *>(cg::noalloc_scratch(ctx->device));
+  if (!base) return cg::fail(ctx, CG_EHIP, "posterior_noalloc: the device buffer's address was not found");
+  std::vector<cg::PlaneTab<float>> t32;
+  std::vector<cg::PlaneTab<double>> t64;
+  build_plane_tables<float>(a, nzp, t32);
+  const bool mixed = a->precision == CG_PREC_MIXED;
+  if (mixed) build_plane_tables<double>(a, nzp, t64);
+  const bool sep = a->prior_mode == CG_PRIOR_SEPARABLE;
+  const std::pair<const void *, size_t> parts[6] = {
+      {t32.data(), sizeof(cg::PlaneTab<float>) * nzp},
+      {mixed ? (const void *)t64.data() : nullptr, mixed ? sizeof(cg::PlaneTab<double>) * nzp : 0},
+      {a->invert, sizeof(int) * nzp},
+      {sep ? a->rxy : nullptr, sep ? sizeof(float) * plane : 0},
+      {sep ? a->rt : nullptr, sep ? sizeof(float) * nzp : 0},
+      {a->err_corr ? nullptr : a->lam2d, a->err_corr ? 0 : sizeof(float) * plane},
+  };
+  // the tables go to the buffer straight from host memory; the call then waits for the copies, so
+  // that the host tables may go (they live on this call's stack and heap)
+  char *tab = base + cg::kNoAllocTables;
+  const void *dp[6];
+  size_t off = 0;
+  hipError_t e = hipSuccess;
+  for (int i = 0; i < 6; ++i) {
+    dp[i] = parts[i].second ? tab + off : nullptr;
+    if (parts[i].second && e == hipSuccess)
+      e = hipMemcpyAsync(tab + off, parts[i].first, parts[i].second, hipMemcpyHostToDevice, ctx->stream);
+    off += align_up(parts[i].second, 256);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) return cg::hip_fail(ctx, e, "posterior_noalloc tables");
+  cg::PostParams p = post_params(a);
+  p.tab32 = static_cast<const cg::PlaneTab<float> *>(dp[0]);
+  p.tab64 = static_cast<const cg::PlaneTab<double> *>(dp[1]);
+  p.invert = static_cast<const int *>(dp[2]);
+  p.rxy = static_cast<const float *>(dp[3]);
+  p.rt = static_cast<const float *>(dp[4]);
+  p.lam2d = static_cast<const float *>(dp[5]);
+  // counters at the buffer's start: [0] deferred wavenumbers, [8, 136) statistics slots; the
+  // deferral mask after the tables, cleared here (the previous call's contents remain otherwise)
+  unsigned long long *cnt = reinterpret_cast<unsigned long long *>(base);
+  p.list = nullptr;
+  p.mask = mixed ? reinterpret_cast<unsigned long long *>(base + cg::kNoAllocMask) : nullptr;
+  p.list_len = mixed ? cnt : nullptr;
+  p.stats = a->stats ? cnt + 8 : nullptr;
+  if (mixed || a->stats) {
+    rc = cg::zero_counters(ctx, cnt, 136);
+    if (rc) return rc;
+  }
+  if (mixed) {
+    rc = cg::zero_words(ctx, p.mask, ((size_t)ref->csize + 63) / 64);
+    if (rc) return rc;
+  }
+  rc = cg::launch_posterior(ctx, p);
+  if (rc) return rc;
+  for (int g = 0; g < 6; ++g) {
     a->cov[g]->transformed = 1;
     a->cov[g]->cubetype = CG_CT_COVARIANCE;
   }

@@ -44,6 +44,9 @@ struct PostParams {
   unsigned long long *list_len;   // MIXED: number of wavenumbers deferred to FP64
   unsigned long long *stats;      // optional: [0,64) inverted, [64,128) Cholesky failures (slots summed on the host)
   unsigned *list;                // wavenumbers deferred to FP64 (capacity: inverted cells)
+  // Instead of the list (when set): one bit per complex cell, bit idx of word idx / 64 set for a
+  // deferred wavenumber; all zero on entry.  The FP64 pass clears what it consumes.
+  unsigned long long *mask;
 };
 
 struct PostItem {
@@ -53,6 +56,24 @@ struct PostItem {
 };
 
 int launch_posterior(cg_ctx *ctx, const PostParams &p);
+// The allocation-free variant's fixed device buffer (cg_posterior_noalloc): one per device, a
+// file-scope array of kNoAllocBytes; noalloc_scratch_init looks up its address on the current
+// device once (at context creation), noalloc_scratch returns it (null before the lookup or if it
+// failed).  Layout: counters [0, kNoAllocTables), per-call tables [kNoAllocTables, kNoAllocMask),
+// the FP64 deferral bit mask [kNoAllocMask, kNoAllocBytes).
+constexpr size_t kNoAllocTables = 4096;
+constexpr size_t kNoAllocMaxNzp = 1536;             // k planes
+constexpr size_t kNoAllocMaxPlane = size_t(1) << 21;  // complex cells per k plane
+constexpr size_t kNoAllocMaxCells = size_t(1) << 28;  // complex cells per grid (mask bits)
+constexpr size_t kNoAllocTableBytes =
+    kNoAllocMaxNzp * (sizeof(PlaneTab<float>) + sizeof(PlaneTab<double>) + 2 * sizeof(int)) +
+    2 * kNoAllocMaxPlane * sizeof(float) + 8 * 256;
+constexpr size_t kNoAllocMask = kNoAllocTables + ((kNoAllocTableBytes + 255) / 256) * 256;
+constexpr size_t kNoAllocBytes = kNoAllocMask + kNoAllocMaxCells / 8;
+int noalloc_scratch_init(int device);
+void *noalloc_scratch(int device);
+// Clears n 64-bit words of device memory with a kernel.
+int zero_words(cg_ctx *ctx, unsigned long long *d, size_t n);
 int launch_posterior_batch(cg_ctx *ctx, int nt, bool sep, bool mixed, const PostItem *dev_items, int n,
                            unsigned blocks);
 

@@ -18,6 +18,7 @@
 //   D_l = wnc Sigma0e_ll |e3_l|^2,
 // so each lane assembles M from three per-plane tables (computed once on the host in double).
 #include <cmath>
+#include <mutex>
 
 #include "cg_internal.h"
 #include "cg_mem.h"
@@ -307,7 +308,11 @@ __device__ __forceinline__ void post32_body(const cg::PostParams &p, int k, long
       for (int g = 0; g < 6; ++g) gst(p.cov[g] + idx, make_float2(P6[g], 0.f));
     }
   }
-  if (MIXED) {  // wave-aggregated append to the FP64 list
+  if (MIXED && p.mask) {  // mark the deferred cell; one counter add per wavefront
+    if (defer) atomicOr(p.mask + ((unsigned long long)idx >> 6), 1ull << ((unsigned long long)idx & 63u));
+    const unsigned long long m = __ballot(defer);
+    if (m && __lane_id() == 0) atomicAdd(p.list_len, (unsigned long long)__popcll(m));
+  } else if (MIXED) {  // wave-aggregated append to the FP64 list
     const unsigned long long m = __ballot(defer);
     if (m) {
       unsigned long long base = 0;
@@ -330,59 +335,89 @@ __global__ void __launch_bounds__(cg::kThreads) k_posterior(cg::PostParams p)
   post32_body<NT, SEP, MIXED>(p, blockIdx.y, (long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
+// FP64 re-solve of one deferred wavenumber (its inputs were left intact by the FP32 pass).
+template <int NT, bool SEP>
+__device__ __forceinline__ void post64_cell(const cg::PostParams &p, long idx)
+{
+  const long plane = (long)p.cnxp * p.nyp;
+  const int k = (int)(idx / plane);
+  const long c = idx % plane;
+  const int i = (int)(c % p.cnxp), j = (int)(c / p.cnxp);
+  float P6[6];
+  prior6<SEP>(p, idx, i, j, k, P6);
+  double P6d[6], rd = 0.0;
+  if (SEP) {
+    rd = fabs((double)gld(p.rxy + (long)j * p.cnxp + i) * (double)kat(p.rt, k));
+#pragma unroll
+    for (int g = 0; g < 6; ++g) P6d[g] = rd * p.s6d[g];
+  } else {
+#pragma unroll
+    for (int g = 0; g < 6; ++g) P6d[g] = (double)P6[g];
+  }
+  const float lt = p.lam_grid ? gld(p.lam_grid + idx).x : gld(p.lam2d + (long)j * p.cnxp + i);
+  const double lam = fabs((double)lt);
+  cpx<double> m3[3], d[NT], cov6[6], res[NT];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float2 v = gld(p.mean[a] + idx);
+    m3[a] = mk<double>(v.x, v.y);
+  }
+#pragma unroll
+  for (int l = 0; l < NT; ++l) {
+    const float2 v = gld(p.data[l] + idx);
+    d[l] = mk<double>(v.x, v.y);
+  }
+  double cond = 0.0;
+  const int fail = update<NT, double, SEP>(kat(p.tab64, k), rd, lam, P6d, m3, d, cov6, res, p.want_residual != 0, cond);
+  if (fail) {
+    if (p.stats) atomicAdd(p.stats + 64, 1ull);  // rare: the FP64 factor failed too
+#pragma unroll
+    for (int g = 0; g < 6; ++g) gst(p.cov[g] + idx, make_float2(P6[g], 0.f));
+  } else {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) gst(p.mean[a] + idx, make_float2((float)m3[a].re, (float)m3[a].im));
+#pragma unroll
+    for (int g = 0; g < 6; ++g) gst(p.cov[g] + idx, make_float2((float)cov6[g].re, (float)cov6[g].im));
+    if (p.want_residual) {
+#pragma unroll
+      for (int l = 0; l < NT; ++l) gst(p.data[l] + idx, make_float2((float)res[l].re, (float)res[l].im));
+    }
+  }
+}
+
 // FP64 pass over the deferred wavenumbers.  Fixed grid, every thread strides over the list
 // and stops at its length, so every wave exits.
 template <int NT, bool SEP>
 __device__ __forceinline__ void post64_body(const cg::PostParams &p, unsigned long long e0, unsigned long long stride)
 {
   const unsigned long long n = *p.list_len;
-  const long plane = (long)p.cnxp * p.nyp;
-  for (unsigned long long e = e0; e < n; e += stride) {
-    const long idx = gld(p.list + e);
-    const int k = (int)(idx / plane);
-    const long c = idx % plane;
-    const int i = (int)(c % p.cnxp), j = (int)(c / p.cnxp);
-    float P6[6];
-    prior6<SEP>(p, idx, i, j, k, P6);
-    double P6d[6], rd = 0.0;
-    if (SEP) {
-      rd = fabs((double)gld(p.rxy + (long)j * p.cnxp + i) * (double)kat(p.rt, k));
-#pragma unroll
-      for (int g = 0; g < 6; ++g) P6d[g] = rd * p.s6d[g];
-    } else {
-#pragma unroll
-      for (int g = 0; g < 6; ++g) P6d[g] = (double)P6[g];
-    }
-    const float lt = p.lam_grid ? gld(p.lam_grid + idx).x : gld(p.lam2d + (long)j * p.cnxp + i);
-    const double lam = fabs((double)lt);
-    cpx<double> m3[3], d[NT], cov6[6], res[NT];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      const float2 v = gld(p.mean[a] + idx);
-      m3[a] = mk<double>(v.x, v.y);
-    }
-#pragma unroll
-    for (int l = 0; l < NT; ++l) {
-      const float2 v = gld(p.data[l] + idx);
-      d[l] = mk<double>(v.x, v.y);
-    }
-    double cond = 0.0;
-    const int fail = update<NT, double, SEP>(kat(p.tab64, k), rd, lam, P6d, m3, d, cov6, res, p.want_residual != 0, cond);
-    if (fail) {
-      if (p.stats) atomicAdd(p.stats + 64, 1ull);  // rare: the FP64 factor failed too
-#pragma unroll
-      for (int g = 0; g < 6; ++g) gst(p.cov[g] + idx, make_float2(P6[g], 0.f));
-    } else {
-#pragma unroll
-      for (int a = 0; a < 3; ++a) gst(p.mean[a] + idx, make_float2((float)m3[a].re, (float)m3[a].im));
-#pragma unroll
-      for (int g = 0; g < 6; ++g) gst(p.cov[g] + idx, make_float2((float)cov6[g].re, (float)cov6[g].im));
-      if (p.want_residual) {
-#pragma unroll
-        for (int l = 0; l < NT; ++l) gst(p.data[l] + idx, make_float2((float)res[l].re, (float)res[l].im));
-      }
+  for (unsigned long long e = e0; e < n; e += stride) post64_cell<NT, SEP>(p, (long)gld(p.list + e));
+}
+
+// FP64 pass over the deferral bit mask: a fixed grid strides over its words (csize / 64, rounded
+// up); each set bit is re-solved and its word cleared, so the mask is all zero again afterwards.
+template <int NT, bool SEP>
+__global__ void __launch_bounds__(cg::kThreads) k_posterior64_mask(cg::PostParams p)
+{
+  const unsigned long long nw = ((unsigned long long)p.csize + 63ull) >> 6;
+  for (unsigned long long w = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; w < nw;
+       w += (unsigned long long)gridDim.x * blockDim.x) {
+    unsigned long long bits = p.mask[w];
+    if (!bits) continue;
+    p.mask[w] = 0ull;
+    while (bits) {
+      const int b = __ffsll((long long)bits) - 1;
+      bits &= bits - 1ull;
+      post64_cell<NT, SEP>(p, (long)(w * 64ull + (unsigned long long)b));
     }
   }
+}
+
+__global__ void __launch_bounds__(cg::kThreads) k_zero_words(unsigned long long *d, unsigned long long n)
+{
+  for (unsigned long long w = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; w < n;
+       w += (unsigned long long)gridDim.x * blockDim.x)
+    d[w] = 0ull;
 }
 
 template <int NT, bool SEP>
@@ -452,9 +487,20 @@ void launch_nt(const cg::PostParams &p, dim3 grid, hipStream_t s, bool sep, bool
 template <int NT>
 void launch_nt64(const cg::PostParams &p, unsigned blocks, hipStream_t s, bool sep)
 {
-  if (sep) k_posterior64<NT, true><<<blocks, cg::kThreads, 0, s>>>(p);
-  else k_posterior64<NT, false><<<blocks, cg::kThreads, 0, s>>>(p);
+  if (p.mask) {
+    if (sep) k_posterior64_mask<NT, true><<<blocks, cg::kThreads, 0, s>>>(p);
+    else k_posterior64_mask<NT, false><<<blocks, cg::kThreads, 0, s>>>(p);
+  } else {
+    if (sep) k_posterior64<NT, true><<<blocks, cg::kThreads, 0, s>>>(p);
+    else k_posterior64<NT, false><<<blocks, cg::kThreads, 0, s>>>(p);
+  }
 }
+
+// The allocation-free posterior's buffer: device memory of fixed size on every device the library
+// runs on, held until the process ends.  Host code never passes this name to a kernel (in host
+// code it is only a stand-in for the symbol); it uses the address noalloc_scratch_init looked up.
+static __device__ __align__(256) unsigned char scratch[cg::kNoAllocBytes];
+
 
 }  // namespace cgpost
 
@@ -480,6 +526,44 @@ int launch_posterior_batch(cg_ctx *ctx, int nt, bool sep, bool mixed, const Post
   }
   launch_end(ctx, CG_K_POSTERIOR, ev);
   return hip_fail(ctx, hipGetLastError(), "k_posterior_batch");
+}
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mu;
+void *scratch_ptr[kMaxDevices] = {};
+bool scratch_tried[kMaxDevices] = {};
+}  // namespace
+
+int noalloc_scratch_init(int device)
+{
+  if (device < 0 || device >= kMaxDevices) return CG_EINVAL;
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (scratch_tried[device]) return scratch_ptr[device] ? CG_OK : CG_EHIP;
+  scratch_tried[device] = true;
+  int cur = -1;
+  if (hipGetDevice(&cur) != hipSuccess || cur != device) return CG_EHIP;  // the caller makes it current
+  void *ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr) return CG_EHIP;
+  scratch_ptr[device] = ptr;
+  return CG_OK;
+}
+
+void *noalloc_scratch(int device)
+{
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  return scratch_ptr[device];
+}
+
+int zero_words(cg_ctx *ctx, unsigned long long *d, size_t n)
+{
+  if (n == 0) return CG_OK;
+  size_t b = (n + kThreads - 1) / kThreads;
+  if (b > 4096) b = 4096;
+  k_zero_words<<<(unsigned)b, kThreads, 0, ctx->stream>>>(d, (unsigned long long)n);
+  return hip_fail(ctx, hipGetLastError(), "k_zero_words");
 }
 
 int launch_posterior(cg_ctx *ctx, const PostParams &p)

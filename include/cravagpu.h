@@ -317,6 +317,17 @@ typedef struct {
  * All grids in the Fourier domain; the caller transforms before and after, as the reference
  * does at :229-244, :908-917 and :1112-1114. */
 int cg_posterior(cg_ctx *ctx, const cg_post_args *args);
+/* cg_posterior without any device allocation.  No counterpart in the reference; it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  Same arguments and results as cg_posterior (the same kernels; the FP64
+ * re-solves are found through a bit mask instead of a list).  Its tables, counters and deferral
+ * mask live in one fixed buffer of 55 MB per device, a file-scope device array that every
+ * GPU the library runs on holds until the process ends; grids beyond nzp <= 1536,
+ * (nxp/2+1)*nyp <= 2^21 and (nxp/2+1)*nyp*nzp <= 2^28 complex cells are refused with CG_EINVAL.
+ * Every call on a device shares that buffer: calls that overlap on one GPU (other contexts or
+ * streams) race and must not run concurrently.  The host tables are copied straight from host
+ * memory and the call waits for the copies (one stream synchronisation). */
+int cg_posterior_noalloc(cg_ctx *ctx, const cg_post_args *args);
 
 /* ------------------------------------------------------------------ simulation */
 /* Counter noise: stream s, complex cell c draws the Philox-4x32-10 block with counter

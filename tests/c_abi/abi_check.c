@@ -81,6 +81,7 @@ int main(void)
   CHECK(cg_grid_destroy(NULL) == CG_EINVAL);
   CHECK(cg_arena_destroy(NULL) == CG_EINVAL);
   CHECK(cg_posterior(NULL, NULL) == CG_EINVAL);
+  CHECK(cg_posterior_noalloc(NULL, NULL) == CG_EINVAL);
   CHECK(cg_simulate(NULL, NULL) == CG_EINVAL);
   CHECK(cg_fft_r2c_inplace(NULL) == CG_EINVAL);
   CHECK(cg_fft_c2r_inplace(NULL) == CG_EINVAL);
