@@ -159,8 +159,8 @@ def accuracy_summary():
     GPU with CG_ACCURACY_OUT set (summary committed under profiles/)."""
     try:
         with open(ACCURACY) as f:
-            return {"source": "tests/test_gpu_configs.py (CG_ACCURACY_OUT), relative L2 per output",
-                    "workloads": json.load(f)}
+            d = json.load(f)
+        return {"source": d.get("command"), "unit": d.get("unit"), "workloads": d.get("workloads")}
     except (OSError, ValueError):
         return None
 

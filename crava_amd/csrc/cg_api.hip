@@ -2245,15 +2245,18 @@ int cg_batch_create(cg_ctx *ctx, int n, const cg_inversion *inv, cg_batch **out)
     size_t off_generic = 0;
     struct G {
       const cg::FixedEntry *e;
+      int variant;
       std::vector<cg::FftItem> items;
       size_t off;
     };
     std::vector<G> groups;
   };
+  // Grids are grouped by compiled length and by the tile variant their own geometry uses alone
+  // (the autotuned choice), so that every grid is transformed by the same kernel as in a plain call.
   auto plan_pass = [&](const std::vector<cg::FftItem> &v, int kind) {
     PassBuild pb;
     pb.kind = kind;
-    std::map<const cg::FixedEntry *, size_t> index;
+    std::map<std::pair<const cg::FixedEntry *, int>, size_t> index;
     for (const auto &it : v) {
       const cg::FixedEntry *e = nullptr;
       if (ctx->fft_engine == CG_FFT_AUTO) {
@@ -2265,10 +2268,14 @@ int cg_batch_create(cg_ctx *ctx, int n, const cg_inversion *inv, cg_batch **out)
         pb.generic.push_back(it);
         continue;
       }
-      auto f = index.find(e);
+      const int var = kind == cg::FFT_ITEM_STRIDED
+                          ? cg::fixed_variant(ctx, e, it.S, it.inner, it.inner * it.outer)
+                          : cg::fixed_variant(ctx, e, it.rows, kind == cg::FFT_ITEM_X_R2C,
+                                              kind != cg::FFT_ITEM_X_R2C && it.add != nullptr);
+      auto f = index.find({e, var});
       if (f == index.end()) {
-        f = index.emplace(e, pb.groups.size()).first;
-        pb.groups.push_back({e, {}, 0});
+        f = index.emplace(std::make_pair(e, var), pb.groups.size()).first;
+        pb.groups.push_back({e, var, {}, 0});
       }
       pb.groups[f->second].items.push_back(it);
     }
@@ -2368,7 +2375,7 @@ int cg_batch_create(cg_ctx *ctx, int n, const cg_inversion *inv, cg_batch **out)
       unsigned gb = 0;
       const auto &f0 = g.items.front();
       if (pb.kind == cg::FFT_ITEM_STRIDED) {
-        fg.variant = cg::fixed_variant(ctx, g.e, f0.S, f0.inner, f0.inner * f0.outer);
+        fg.variant = g.variant;
         const int tile = g.e->tile[fg.variant];
         std::vector<cg::StridedArgs> a;
         for (const auto &it : g.items) {
@@ -2380,7 +2387,7 @@ int cg_batch_create(cg_ctx *ctx, int n, const cg_inversion *inv, cg_batch **out)
       } else {
         const bool fwd = pb.kind == cg::FFT_ITEM_X_R2C;
         fg.add = !fwd && f0.add != nullptr;
-        fg.variant = cg::fixed_variant(ctx, g.e, f0.rows, fwd, fg.add);
+        fg.variant = g.variant;
         const int tile = g.e->tile[fg.variant];
         std::vector<cg::XArgs> a;
         for (const auto &it : g.items) {

@@ -220,14 +220,22 @@ def _check_cube(ctx, sp, label, rel=(0, 1)):
     else:
         C.simulate(ctx, prob.cov, None, prob.out[last[0] % len(prob.out)], seed=sp["seed"], realisation=last[0])
     e_real, e_sum = 0.0, 0.0
-    ocov_inv = oc[:, ii].astype(np.complex128)
-    kk_inv = [ks[i] for i in ii]
+    # every sampled inverted plane with its mirror nzp - k: the real-domain realisation only holds the
+    # Hermitian part of the coloured noise in the x = 0 and x = nxp/2 columns (the half-complex
+    # inverse reads the real parts there), which pairs k with nzp - k
+    kk_inv = sorted({ks[i] for i in ii} | {(nzp - ks[i]) % nzp for i in ii})
+    covs = np.stack([g.download_complex()[kk_inv] for g in prob.cov]).astype(np.complex128)
+    mirror = [kk_inv.index((nzp - k) % nzp) for k in kk_inv]
+    xcols = [0] + ([cnxp - 1] if nxp % 2 == 0 else [])
+    jm = (-np.arange(nyp)) % nyp
     for r in last:
         outs = prob.out[r % len(prob.out)]
         pure = [g.download_real() for g in outs]
         noise = np.stack([O.fill_complex_noise_cb_planes(nxp, nyp, nzp, sp["seed"], 4 * r + g, kk_inv)
                           for g in range(3)])
-        z, _ = O.colour(ocov_inv, noise, "f64")
+        z, _ = O.colour(covs, noise, "f64")
+        for xc in xcols:  # the Hermitian part of those columns
+            z[:, :, :, xc] = 0.5 * (z[:, :, :, xc] + np.conj(z[:, mirror][:, :, jm, xc]))
         got = np.stack([O.spectrum_planes(pure[g], kk_inv, O.PARAMETER) for g in range(3)])
         e_real = max(e_real, rel_l2(got, z))
         for g in range(3):  # benchmark realisation = posterior mean + colouring
