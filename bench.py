@@ -344,7 +344,7 @@ def bench_large(ctx, steps=2):
     }
 
 
-def bench_serving(n=48, lanes=3, load=0.7):
+def bench_serving(n=48, lanes=2, load=0.7, ingest_workers=2):
     """Continuous batching under Poisson arrivals (crava_amd.serve): requests of log-uniform sizes
     64^2 x 64 .. 384^2 x 256, 3-8 angles, 0-10 realisations, admitted against a 200 GB HBM
     budget.  A first pass with every request arriving at once measures the server's capacity
@@ -353,7 +353,7 @@ def bench_serving(n=48, lanes=3, load=0.7):
     device, batched compute)."""
     from crava_amd import serve
     specs = S.serving_requests(n)
-    ex = serve.GpuExecutor(lanes=lanes)
+    ex = serve.GpuExecutor(lanes=lanes, ingest_workers=ingest_workers)
     try:
         srv = serve.Server(ex, hbm_budget=200e9, lanes=lanes, max_batch=16)
         cap = serve.run_open_loop(srv, specs, np.zeros(n))
@@ -366,12 +366,12 @@ def bench_serving(n=48, lanes=3, load=0.7):
     return {
         "workload": "%d requests, sizes log-uniform 64x64x64..384x384x256 (padded by the reference rule), 3-8 "
                     "angles, 0-10 realisations, Poisson arrivals at %.1f req/s (%.0f%% of capacity), 200 GB HBM "
-                    "budget, %d lanes" % (n, rate, 100 * load, lanes),
+                    "budget, %d lanes, %d ingest threads" % (n, rate, 100 * load, lanes, ingest_workers),
         "cells_per_s": rep["cells_per_s"], "latency_ms": rep["latency_ms"], "requests": rep["requests"],
         "batches": rep["batches"], "mean_batch": rep["mean_batch"], "peak_admitted_gb": rep["peak_admitted_gb"],
         "mean_ingest_ms": float(ingest),
         "capacity": {"cells_per_s": cap["cells_per_s"], "requests_per_s": cap["requests"] / cap["seconds"],
-                     "latency_ms": cap["latency_ms"]},
+                     "latency_ms": cap["latency_ms"], "batches": cap["batches"], "mean_batch": cap["mean_batch"]},
     }
 
 

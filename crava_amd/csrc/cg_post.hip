@@ -46,6 +46,11 @@ constexpr double kTol = 1e-20;  // lib_matr.c:19
 
 __device__ __forceinline__ int tri(int i, int j) { return i * (i + 1) / 2 + j; }  // j <= i
 
+// Fused multiply-add in the working precision: the complex products below are written as chains
+// of fma(-a, b, acc) so that each real product costs one instruction.
+__device__ __forceinline__ float fmr(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmr(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 // Reciprocal: one v_rcp (about 1 ulp) in FP32, an exact division in FP64.
 __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
@@ -72,15 +77,16 @@ __device__ __forceinline__ int chol(cpx<R> (&m)[NT * (NT + 1) / 2], R (&rd)[NT],
       R lr = m[tri(i, j)].re, li = m[tri(i, j)].im;
 #pragma unroll
       for (int k = 0; k < j; ++k) {  // a_ij - sum_k L_ik conj(L_jk)
-        lr -= m[tri(i, k)].re * m[tri(j, k)].re + m[tri(i, k)].im * m[tri(j, k)].im;
-        li -= m[tri(i, k)].im * m[tri(j, k)].re - m[tri(i, k)].re * m[tri(j, k)].im;
+        const cpx<R> a = m[tri(i, k)], b = m[tri(j, k)];
+        lr = fmr(-a.re, b.re, fmr(-a.im, b.im, lr));
+        li = fmr(-a.im, b.re, fmr(a.re, b.im, li));
       }
       m[tri(i, j)].re = lr * rd[j];
       m[tri(i, j)].im = li * rd[j];
     }
     R s = 0;
 #pragma unroll
-    for (int k = 0; k < i; ++k) s += m[tri(i, k)].re * m[tri(i, k)].re + m[tri(i, k)].im * m[tri(i, k)].im;
+    for (int k = 0; k < i; ++k) s = fmr(m[tri(i, k)].re, m[tri(i, k)].re, fmr(m[tri(i, k)].im, m[tri(i, k)].im, s));
     const R piv = m[tri(i, i)].re - s;
     if (!((double)piv > tol)) return 1;
     pmin = piv < pmin ? piv : pmin;
@@ -103,8 +109,9 @@ __device__ __forceinline__ void lsolve(const cpx<R> (&L)[NT * (NT + 1) / 2], con
     R xr = x[i].re, xi = x[i].im;
 #pragma unroll
     for (int j = 0; j < i; ++j) {
-      xr -= x[j].re * L[tri(i, j)].re - x[j].im * L[tri(i, j)].im;
-      xi -= x[j].im * L[tri(i, j)].re + x[j].re * L[tri(i, j)].im;
+      const cpx<R> l = L[tri(i, j)];
+      xr = fmr(-x[j].re, l.re, fmr(x[j].im, l.im, xr));
+      xi = fmr(-x[j].im, l.re, fmr(-x[j].re, l.im, xi));
     }
     x[i].re = xr * rd[i];
     x[i].im = xi * rd[i];
@@ -121,21 +128,22 @@ __device__ __forceinline__ int update(const Tab &T, R r, R lam, const R (&P6)[6]
                                       cpx<R> (&d)[NT], cpx<R> (&cov6)[6], cpx<R> (&res)[NT], bool want_res,
                                       R &cond_lb)
 {
-  cpx<R> Y[3][NT];  // columns of K Sigma_m, then L^-1 of them
+  // Separable prior: Y holds L^-1 S (S = K Sigma0), and K Sigma_m = r S enters the outputs as
+  // Y^H Y r^2 and Y^H z r; the grid prior: Y holds L^-1 K Sigma_m itself (scale 1).
+  cpx<R> Y[3][NT];
   cpx<R> M[NT * (NT + 1) / 2];
   if (SEP) {
 #pragma unroll
     for (int l = 0; l < NT; ++l)
 #pragma unroll
-      for (int a = 0; a < 3; ++a) Y[a][l] = mk<R>(r * T.S[l][a].re, r * T.S[l][a].im);
+      for (int a = 0; a < 3; ++a) Y[a][l] = mk<R>(T.S[l][a].re, T.S[l][a].im);
 #pragma unroll
     for (int l = 0; l < NT; ++l)
 #pragma unroll
       for (int m = 0; m <= l; ++m) {
         const int t = tri(l, m);
-        M[t].re = r * T.G[t].re + lam * T.Q[t].re;
-        M[t].im = r * T.G[t].im + lam * T.Q[t].im;
-        if (l == m) M[t].re += T.D[l];
+        M[t].re = fmr(r, T.G[t].re, fmr(lam, T.Q[t].re, l == m ? T.D[l] : R(0)));
+        M[t].im = fmr(r, T.G[t].im, lam * T.Q[t].im);
       }
   } else {
     const R Pm[3][3] = {{P6[0], P6[3], P6[4]}, {P6[3], P6[1], P6[5]}, {P6[4], P6[5], P6[2]}};
@@ -146,8 +154,8 @@ __device__ __forceinline__ int update(const Tab &T, R r, R lam, const R (&P6)[6]
         R xr = 0, xi = 0;
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-          xr += T.K[l][a].re * Pm[a][b];
-          xi += T.K[l][a].im * Pm[a][b];
+          xr = fmr(T.K[l][a].re, Pm[a][b], xr);
+          xi = fmr(T.K[l][a].im, Pm[a][b], xi);
         }
         Y[b][l] = mk<R>(xr, xi);
       }
@@ -155,16 +163,15 @@ __device__ __forceinline__ int update(const Tab &T, R r, R lam, const R (&P6)[6]
     for (int l = 0; l < NT; ++l)
 #pragma unroll
       for (int m = 0; m <= l; ++m) {
-        R xr = 0, xi = 0;
+        const int t = tri(l, m);
+        R xr = fmr(lam, T.Q[t].re, l == m ? T.D[l] : R(0)), xi = lam * T.Q[t].im;
 #pragma unroll
         for (int a = 0; a < 3; ++a) {  // (K Sigma_m K^H)_lm = sum_a KS_la conj(K_ma)
-          xr += Y[a][l].re * T.K[m][a].re + Y[a][l].im * T.K[m][a].im;
-          xi += Y[a][l].im * T.K[m][a].re - Y[a][l].re * T.K[m][a].im;
+          xr = fmr(Y[a][l].re, T.K[m][a].re, fmr(Y[a][l].im, T.K[m][a].im, xr));
+          xi = fmr(Y[a][l].im, T.K[m][a].re, fmr(-Y[a][l].re, T.K[m][a].im, xi));
         }
-        const int t = tri(l, m);
-        M[t].re = xr + lam * T.Q[t].re;
-        M[t].im = xi + lam * T.Q[t].im;
-        if (l == m) M[t].re += T.D[l];
+        M[t].re = xr;
+        M[t].im = xi;
       }
   }
   R rd[NT];
@@ -176,15 +183,16 @@ __device__ __forceinline__ int update(const Tab &T, R r, R lam, const R (&P6)[6]
     R xr = d[l].re, xi = d[l].im;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-      xr -= T.K[l][a].re * m3[a].re - T.K[l][a].im * m3[a].im;
-      xi -= T.K[l][a].im * m3[a].re + T.K[l][a].re * m3[a].im;
+      xr = fmr(-T.K[l][a].re, m3[a].re, fmr(T.K[l][a].im, m3[a].im, xr));
+      xi = fmr(-T.K[l][a].im, m3[a].re, fmr(-T.K[l][a].re, m3[a].im, xi));
     }
     z[l] = mk<R>(xr, xi);
   }
   lsolve<NT, R>(M, rd, z);
 #pragma unroll
   for (int a = 0; a < 3; ++a) lsolve<NT, R>(M, rd, Y[a]);
-  // Sigma' = Sigma_m - Y^H Y (entries 00 11 22 01 02 12), m' = m + Y^H z
+  // Sigma' = Sigma_m - (r^2) Y^H Y (entries 00 11 22 01 02 12), m' = m + (r) Y^H z
+  const R s2 = SEP ? r * r : R(1), s1 = SEP ? r : R(1);
   const int ia[6] = {0, 1, 2, 0, 0, 1}, ib[6] = {0, 1, 2, 1, 2, 2};
 #pragma unroll
   for (int g = 0; g < 6; ++g) {
@@ -192,21 +200,21 @@ __device__ __forceinline__ int update(const Tab &T, R r, R lam, const R (&P6)[6]
 #pragma unroll
     for (int l = 0; l < NT; ++l) {
       const cpx<R> u = Y[ia[g]][l], v = Y[ib[g]][l];
-      xr += u.re * v.re + u.im * v.im;
-      if (g >= 3) xi += u.re * v.im - u.im * v.re;
+      xr = fmr(u.re, v.re, fmr(u.im, v.im, xr));
+      if (g >= 3) xi = fmr(u.re, v.im, fmr(-u.im, v.re, xi));
     }
-    cov6[g] = mk<R>(P6[g] - xr, -xi);
+    cov6[g] = mk<R>(fmr(-s2, xr, P6[g]), -s2 * xi);
   }
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     R xr = 0, xi = 0;
 #pragma unroll
     for (int l = 0; l < NT; ++l) {
-      xr += Y[a][l].re * z[l].re + Y[a][l].im * z[l].im;
-      xi += Y[a][l].re * z[l].im - Y[a][l].im * z[l].re;
+      xr = fmr(Y[a][l].re, z[l].re, fmr(Y[a][l].im, z[l].im, xr));
+      xi = fmr(Y[a][l].re, z[l].im, fmr(-Y[a][l].im, z[l].re, xi));
     }
-    m3[a].re += xr;
-    m3[a].im += xi;
+    m3[a].re = fmr(s1, xr, m3[a].re);
+    m3[a].im = fmr(s1, xi, m3[a].im);
   }
   if (want_res) {
 #pragma unroll
@@ -214,8 +222,8 @@ __device__ __forceinline__ int update(const Tab &T, R r, R lam, const R (&P6)[6]
       R xr = d[l].re, xi = d[l].im;
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
-        xr -= T.K[l][a].re * m3[a].re - T.K[l][a].im * m3[a].im;
-        xi -= T.K[l][a].im * m3[a].re + T.K[l][a].re * m3[a].im;
+        xr = fmr(-T.K[l][a].re, m3[a].re, fmr(T.K[l][a].im, m3[a].im, xr));
+        xi = fmr(-T.K[l][a].im, m3[a].re, fmr(-T.K[l][a].re, m3[a].im, xi));
       }
       res[l] = mk<R>(xr, xi);
     }
@@ -235,11 +243,11 @@ __device__ __forceinline__ void wave_count(unsigned long long *slots, bool pred)
 // Loads the prior covariance entries of one wavenumber (getNextParameterCovariance:
 // |Re(c_ab)/s_ab| * s_ab with zero imaginary part).
 template <bool SEP>
-__device__ __forceinline__ float prior6(const cg::PostParams &p, long idx, int i, int j, int k, float (&P6)[6])
+__device__ __forceinline__ float prior6(const cg::PostParams &p, long idx, long c, int k, float (&P6)[6])
 {
   float r = 0.f;
   if (SEP) {
-    const float t = gld(p.rxy + (long)j * p.cnxp + i) * kat(p.rt, k);
+    const float t = gld(p.rxy + c) * kat(p.rt, k);
     r = sqrtf(t * t);
 #pragma unroll
     for (int g = 0; g < 6; ++g) P6[g] = r * p.s6f[g];
@@ -262,17 +270,18 @@ __device__ __forceinline__ void post32_body(const cg::PostParams &p, int k, long
 {
   const long plane = (long)p.cnxp * p.nyp;
   const bool active = c < plane;
-  const int i = active ? (int)(c % p.cnxp) : 0, j = active ? (int)(c / p.cnxp) : 0;
-  const long idx = (long)k * plane + (active ? c : 0);
+  // the lateral tables rxy and lam2d are [nyp][cnxp]: cell c of the plane is their entry c
+  const long cc = active ? c : 0;
+  const long idx = (long)k * plane + cc;
   const bool inv = kat(p.invert, k) != 0;
   bool did_inv = false, did_fail = false, defer = false;
   if (active) {
     float P6[6];
-    const float r = prior6<SEP>(p, idx, i, j, k, P6);
+    const float r = prior6<SEP>(p, idx, cc, k, P6);
     bool write_prior = !inv;
     if (inv) {
       did_inv = true;
-      const float lt = p.lam_grid ? gld(p.lam_grid + idx).x : gld(p.lam2d + (long)j * p.cnxp + i);
+      const float lt = p.lam_grid ? gld(p.lam_grid + idx).x : gld(p.lam2d + cc);
       const float lam = sqrtf(lt * lt);
       cpx<float> m3[3], d[NT], cov6[6], res[NT];
 #pragma unroll
@@ -342,19 +351,18 @@ __device__ __forceinline__ void post64_cell(const cg::PostParams &p, long idx)
   const long plane = (long)p.cnxp * p.nyp;
   const int k = (int)(idx / plane);
   const long c = idx % plane;
-  const int i = (int)(c % p.cnxp), j = (int)(c / p.cnxp);
   float P6[6];
-  prior6<SEP>(p, idx, i, j, k, P6);
+  prior6<SEP>(p, idx, c, k, P6);
   double P6d[6], rd = 0.0;
   if (SEP) {
-    rd = fabs((double)gld(p.rxy + (long)j * p.cnxp + i) * (double)kat(p.rt, k));
+    rd = fabs((double)gld(p.rxy + c) * (double)kat(p.rt, k));
 #pragma unroll
     for (int g = 0; g < 6; ++g) P6d[g] = rd * p.s6d[g];
   } else {
 #pragma unroll
     for (int g = 0; g < 6; ++g) P6d[g] = (double)P6[g];
   }
-  const float lt = p.lam_grid ? gld(p.lam_grid + idx).x : gld(p.lam2d + (long)j * p.cnxp + i);
+  const float lt = p.lam_grid ? gld(p.lam_grid + idx).x : gld(p.lam2d + c);
   const double lam = fabs((double)lt);
   cpx<double> m3[3], d[NT], cov6[6], res[NT];
 #pragma unroll

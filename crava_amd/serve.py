@@ -4,8 +4,9 @@ Requests arrive at any time.  The server keeps
   * a waiting queue, from which requests are admitted in arrival order while the device bytes
     of every admitted request (its grids, held from admission to release) stay within an HBM
     budget;
-  * an ingest stage that materialises an admitted request's inputs on the device (on its own
-    stream), after which the request is ready;
+  * an ingest stage that materialises an admitted request's inputs on the device, after which the
+    request is ready.  An executor with `ingest_async` runs ingests off the scheduling loop (each
+    on its own stream), so that dispatch, completions and admission go on while inputs are built;
   * `lanes` execution lanes (one HIP stream each).  Whenever a lane is idle, every ready request
     (up to `max_batch`) is gathered into one batch (cg_batch: each FFT pass, the posterior and
     the colouring of all of them share launches) and enqueued on that lane.  Lanes run
@@ -60,6 +61,7 @@ class Server:
         self.max_batch = max_batch
         self.clock = clock
         self.waiting = collections.deque()
+        self.ingesting = collections.deque()  # (request, handle) of asynchronous ingests, admission order
         self.ready = collections.deque()
         self.running = [None] * lanes  # per lane: (batch handle, [requests]) or None
         self.done = []
@@ -78,7 +80,8 @@ class Server:
         return r
 
     def idle(self):
-        return not self.waiting and not self.ready and all(x is None for x in self.running)
+        return (not self.waiting and not self.ingesting and not self.ready
+                and all(x is None for x in self.running))
 
     # ---------------------------------------------------------------- scheduling step
     def step(self, on_done=None):
@@ -86,20 +89,37 @@ class Server:
         to `ingest_per_step` waiting requests (in arrival order, while they fit the budget),
         dispatch again.  Returns True if anything moved."""
         moved = self._complete(on_done)
+        moved |= self._ingested()
         moved |= self._dispatch()
+        asynchronous = hasattr(self.ex, "ingest_async")
         n = 0
         while n < self.ingest_per_step and self.waiting and self.in_use + self.waiting[0].bytes <= self.budget:
             r = self.waiting.popleft()
             r.admitted = self.clock()
             self.in_use += r.bytes
             self.peak = max(self.peak, self.in_use)
-            r.work = self.ex.ingest(r)
-            r.ready = self.clock()
-            self.ready.append(r)
+            if asynchronous:
+                self.ingesting.append((r, self.ex.ingest_async(r)))
+            else:
+                r.work = self.ex.ingest(r)
+                r.ready = self.clock()
+                self.ready.append(r)
             n += 1
         if n:
             moved = True
+            moved |= self._ingested()
             moved |= self._dispatch()
+        return moved
+
+    def _ingested(self):
+        """Ready requests from finished asynchronous ingests, in admission order."""
+        moved = False
+        while self.ingesting and self.ingesting[0][1].done():
+            r, h = self.ingesting.popleft()
+            r.work = h.result()
+            r.ready = self.clock()
+            self.ready.append(r)
+            moved = True
         return moved
 
     def _dispatch(self):
@@ -158,16 +178,58 @@ class Server:
                 "peak_admitted_gb": self.peak / 1e9, "budget_gb": self.budget / 1e9}
 
 
-class GpuExecutor:
-    """Device side of the server over the C ABI: inputs are materialised on an ingest context
-    (crava_amd.synthetic.Problem: seeded synthetic seismic and priors), batches run on one
-    context per lane."""
+class _Worker:
+    """A host thread that owns one device context: it runs the tasks given to it in order, so a
+    context is only ever used by its own thread (contexts are not re-entrant)."""
 
-    def __init__(self, device=0, lanes=3, graph=False):
+    def __init__(self, device):
+        import queue
+        import threading
         from . import Context
-        self.ingest_ctx = Context(device)
+        self.ctx = Context(device)
+        self.q = queue.Queue()
+        self.pending = 0
+        self.t = threading.Thread(target=self._run, daemon=True)
+        self.t.start()
+
+    def _run(self):
+        while True:
+            task = self.q.get()
+            if task is None:
+                return
+            fn, fut = task
+            if fut.set_running_or_notify_cancel():
+                try:
+                    fut.set_result(fn(self.ctx))
+                except BaseException as e:  # reported to the caller through the future
+                    fut.set_exception(e)
+            self.pending -= 1
+
+    def submit(self, fn):
+        from concurrent.futures import Future
+        fut = Future()
+        self.pending += 1
+        self.q.put((fn, fut))
+        return fut
+
+    def close(self):
+        self.q.put(None)
+        self.t.join()
+        self.ctx.close()
+
+
+class GpuExecutor:
+    """Device side of the server over the C ABI: inputs are materialised by `ingest_workers` host
+    threads, each with its own context and stream (crava_amd.synthetic.Problem: seeded synthetic
+    seismic and priors, built on the device), concurrently with the batches, which run on one
+    context per lane.  A request's grids are released by the worker that built them."""
+
+    def __init__(self, device=0, lanes=3, graph=False, ingest_workers=2):
+        from . import Context
+        self.workers = [_Worker(device) for _ in range(max(1, ingest_workers))]
         self.lane_ctx = [Context(device) for _ in range(lanes)]
         self.graph = graph
+        self.owner = {}
 
     def footprint(self, spec):
         from . import synthetic as S
@@ -177,10 +239,15 @@ class GpuExecutor:
         from . import synthetic as S
         return S.cells(spec)
 
-    def ingest(self, r):
+    def ingest_async(self, r):
         from . import synthetic as S
-        p = S.Problem(self.ingest_ctx, r.spec)  # synchronises the ingest stream at the end
-        return p
+        w = min(self.workers, key=lambda x: x.pending)
+        self.owner[r.rid] = w
+        spec = r.spec
+        return w.submit(lambda ctx: S.Problem(ctx, spec))  # synchronises the worker's stream at the end
+
+    def ingest(self, r):
+        return self.ingest_async(r).result()
 
     def launch(self, lane, group):
         from . import Batch
@@ -197,13 +264,15 @@ class GpuExecutor:
         b.close()
 
     def release(self, r):
-        r.work.close()
+        work = r.work
         r.work = None
+        self.owner.pop(r.rid).submit(lambda ctx: work.close())
 
     def close(self):
+        for w in self.workers:
+            w.close()
         for c in self.lane_ctx:
             c.close()
-        self.ingest_ctx.close()
 
 
 def poisson_arrivals(n, rate_hz, seed):
