@@ -123,11 +123,17 @@ def lib():
             "cg_grid_write_storm": [_vp, ctypes.c_char_p, ctypes.POINTER(StormGeometry), i],
             "cg_grid_read_storm": [_vp, ctypes.c_char_p, ctypes.POINTER(StormGeometry)],
             "cg_grid_read_crava": [_vp, ctypes.c_char_p, ctypes.POINTER(CravaGeometry)],
+            "cg_local_noise_create": [_vp, ctypes.POINTER(LocalNoiseArgs), ctypes.POINTER(_vp)],
+            "cg_local_noise_matrices": [_vp, _d_p, _d_p],
+            "cg_local_noise_correct": [_vp, _vp * 3, _vp * 3],
+            "cg_local_noise_correct_realisation": [_vp, _vp * 3, _vp * 3],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
             fn.argtypes = args
             fn.restype = ctypes.c_int
+        L.cg_local_noise_destroy.argtypes = [_vp]
+        L.cg_local_noise_destroy.restype = None
         L.cg_arena_grid_bytes.argtypes = [i, i, i]
         L.cg_arena_grid_bytes.restype = ctypes.c_uint64
         L.cg_last_error.argtypes = [_vp]
@@ -145,6 +151,14 @@ class StormGeometry(ctypes.Structure):
     _fields_ = [("x_min", ctypes.c_double), ("lx", ctypes.c_double), ("y_min", ctypes.c_double),
                 ("ly", ctypes.c_double), ("lz", ctypes.c_double), ("angle", ctypes.c_double),
                 ("top_name", ctypes.c_char_p), ("bot_name", ctypes.c_char_p)]
+
+
+class LocalNoiseArgs(ctypes.Structure):
+    """cg_local_noise_args."""
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("ntheta", ctypes.c_int),
+                ("G", ctypes.POINTER(ctypes.c_double)), ("prior_var0", ctypes.POINTER(ctypes.c_double)),
+                ("post_var0", ctypes.POINTER(ctypes.c_double)), ("err_theta_cov", ctypes.POINTER(ctypes.c_double)),
+                ("noise_scale", ctypes.POINTER(ctypes.POINTER(ctypes.c_double)))]
 
 
 class CravaGeometry(ctypes.Structure):
@@ -749,3 +763,54 @@ def make_circ_auto_cov_pos_def(circ_auto_cov):
     if rc != CG_OK:
         raise CravaError("cg_make_circ_auto_cov_pos_def failed (%d): a lag-0 variance is not positive" % rc)
     return a
+
+
+class LocalNoise:
+    """AVOInversion::correctVpVsRho (avoinversion.cpp:2146-2302) on the device: the per-trace
+    3 x 3 matrices of newPosteriorCovPointwise from the local noise scale grids (n_theta arrays
+    (ny, nx), Grid2D (i, j) at [j, i]) and the operator G of AVOInversion::computeG
+    (crava_amd.model.compute_g), then applied to the real-domain posterior means (correct) and to
+    realisations' noise fields before the posterior is added (correct_realisation)."""
+
+    def __init__(self, ctx, G, prior_var0, post_var0, err_theta_cov, noise_scales):
+        sc = [np.ascontiguousarray(a, dtype=np.float64) for a in noise_scales]
+        ny, nx = sc[0].shape
+        nt = len(sc)
+        self.ctx, self.nx, self.ny = ctx, nx, ny
+        keep = [np.ascontiguousarray(G, dtype=np.float64).reshape(nt, 3),
+                np.ascontiguousarray(prior_var0, dtype=np.float64).reshape(3, 3),
+                np.ascontiguousarray(post_var0, dtype=np.float64).reshape(3, 3),
+                np.ascontiguousarray(err_theta_cov, dtype=np.float64).reshape(nt, nt)]
+        ptrs = (ctypes.POINTER(ctypes.c_double) * nt)(*[_ptr(a, ctypes.c_double) for a in sc])
+        args = LocalNoiseArgs(nx, ny, nt, *[_ptr(a, ctypes.c_double) for a in keep], ptrs)
+        self._h = _vp()
+        ctx.check(lib().cg_local_noise_create(ctx._h, ctypes.byref(args), ctypes.byref(self._h)),
+                  "cg_local_noise_create")
+
+    def matrices(self):
+        """(sigmanew, kept), each (ny, nx, 3, 3)."""
+        a = np.empty((self.ny, self.nx, 3, 3))
+        b = np.empty((self.ny, self.nx, 3, 3))
+        self.ctx.check(lib().cg_local_noise_matrices(self._h, _ptr(a, ctypes.c_double), _ptr(b, ctypes.c_double)),
+                       "cg_local_noise_matrices")
+        return a, b
+
+    def correct(self, post, mean):
+        self.ctx.check(lib().cg_local_noise_correct(self._h, (_vp * 3)(*[g._h for g in post]),
+                                                    (_vp * 3)(*[g._h for g in mean])), "cg_local_noise_correct")
+
+    def correct_realisation(self, real, post):
+        self.ctx.check(lib().cg_local_noise_correct_realisation(self._h, (_vp * 3)(*[g._h for g in real]),
+                                                                (_vp * 3)(*[g._h for g in post])),
+                       "cg_local_noise_correct_realisation")
+
+    def close(self):
+        if self._h:
+            lib().cg_local_noise_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

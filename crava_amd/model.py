@@ -169,3 +169,46 @@ def synthetic_seismic(ctx, vp, vs, rho, wavelets, A, nxp, nyp, nzp):
     finally:
         for g in m + d:
             g.close()
+
+
+def _eig_general(m):
+    """Eigenvalues (real parts) and right eigenvectors (columns) by LAPACK dgeev, the routine
+    NRLib::ComputeEigenVectors reaches through flens::ev (libs/nrlib/flens/nrlib_flens.cpp:114-121)."""
+    w, v = np.linalg.eig(np.asarray(m, dtype=np.float64))
+    return np.real(w).copy(), np.real(v).copy()
+
+
+def compute_g(prior_var0, post_var0, err_theta_cov):
+    """AVOInversion::computeG (avoinversion.cpp:1943-2026), the host side of the local-noise
+    correction (crava_amd.LocalNoise): the n_theta x 3 operator
+    G = Sigma_e^1/2 Lg V^T Sigma_m^-1/2, where V, lambda are the eigen pairs of
+    A = Sigma_m^-1/2 (Sigma_m - postVar0) Sigma_m^-1/2 in decreasing order (NRLib::Sort3x3) and
+    Lg = diag(sqrt(lambda / (1 - lambda))), so that the posterior of G with noise Sigma_e is
+    postVar0.  V's signs are LAPACK's (they matter once the scales vary between angles)."""
+    sm = np.asarray(prior_var0, dtype=np.float64)
+    se = np.asarray(err_theta_cov, dtype=np.float64)
+    nt = se.shape[0]
+
+    def vfv(v, f):
+        return (v * f[None, :]) @ v.T
+
+    w, v = _eig_general(sm - np.asarray(post_var0, dtype=np.float64))
+    sdelta = vfv(v, np.maximum(w, 0.0))
+    w, v = _eig_general(sm)
+    sinv = vfv(v, np.array([1.0 / math.sqrt(x) if x > 1e-7 else 0.0 for x in w]))
+    w, v = _eig_general((sinv @ sdelta) @ sinv)
+    order = []
+    for _ in range(3):  # Sort3x3: decreasing, the first of equal values first
+        best, bi = -100000.0, -1
+        for i in range(3):
+            if i not in order and w[i] > best:
+                best, bi = w[i], i
+        order.append(bi)
+    w, v = w[order], v[:, order]
+    lg = np.zeros((nt, 3))
+    for i in range(min(nt, 3)):
+        if w[i] > 0:
+            lg[i, i] = math.sqrt(w[i] / (1.0 - w[i]))
+    we, ve = _eig_general(se)
+    esq = vfv(ve, np.array([math.sqrt(x) if x > 0.0 else 0.0 for x in we]))
+    return ((esq @ lg) @ v.T) @ sinv

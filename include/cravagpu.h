@@ -345,7 +345,7 @@ typedef struct {
   cg_grid *out[3];          /* realisation grids: receive Vp, Vs, Rho real fields */
   int noise_source;         /* CG_NOISE_* */
   uint64_t seed;            /* CG_NOISE_COUNTER key */
-  uint32_t realisation;     /* CG_NOISE_COUNTER realisation number (streams 4r, 4r+1, 4r+2) */
+  uint32_t realisation;     /* CG_NOISE_COUNTER realisation number (streams 3r, 3r+1, 3r+2) */
   /* CG_NOISE_GRIDS: out[] already hold the noise spectra (fillInComplexNoise result) */
   uint64_t *stats;          /* optional host [1]: Cholesky failures (noise zeroed) */
 } cg_sim_args;
@@ -394,7 +394,7 @@ typedef struct {
   cg_grid *seis_in[8]; /* real-domain seismic cubes (DATA), left unchanged */
   int nreal;           /* posterior realisations */
   cg_grid **out;       /* [nreal][3] realisation grids (real domain after a run) */
-  uint64_t seed;       /* counter-noise key; realisation r uses streams 4r, 4r+1, 4r+2 */
+  uint64_t seed;       /* counter-noise key; realisation r uses streams 3r, 3r+1, 3r+2 */
 } cg_inversion;
 
 int cg_batch_create(cg_ctx *ctx, int n, const cg_inversion *inv, cg_batch **batch);
@@ -428,6 +428,38 @@ int cg_grid_sum_squares(cg_grid *grid, int ext_simbox, double *sum);
  * non-zero samples.  Bit-identical to the reference's sequential sweep.  Real domain;
  * synchronises. */
 int cg_grid_data_variance(cg_grid *grid, float *variance, uint64_t *count);
+
+/* ------------------------------------------------------------------ local noise */
+/* Laterally varying data noise (AVOInversion::correctVpVsRho, avoinversion.cpp:2146-2302): per
+ * trace (i, j) the 3 x 3 matrices of newPosteriorCovPointwise (:2028-2124) from the angle scales
+ * s_l = noise_scale_l(i, j) / min noise_scale_l (Grid2D::FindMin, -99999 marking missing values),
+ * computed on the device in FP64 by cg_local_noise_create (synchronises):
+ *   sigmanew = (Sigma_m G^T M^+ G Sigma_m)^1/2 (Sigma_m - postVar0)^-1/2,
+ *   kept     = (Sigma_m - Sigma_m G^T M^+ G Sigma_m)^1/2 postVar0^-1/2   (sigmamdnew_, :2232-2241),
+ * M = G Sigma_m G^T + D Sigma_e D, D = diag(sqrt s), M^+ without eigenvalues <= 1e-8.  G is the
+ * caller's AVOInversion::computeG (:1943-2026), whose LAPACK eigenvectors it keeps. */
+typedef struct cg_local_noise cg_local_noise;
+typedef struct {
+  int nx, ny, ntheta;                 /* the box's traces; 1 to 8 angles */
+  const double *G;                    /* ntheta x 3, row-major */
+  const double *prior_var0;           /* Sigma_m, 3 x 3 row-major (priorVar0_) */
+  const double *post_var0;            /* 3 x 3 (postVar0_, the lag-0 posterior covariance) */
+  const double *err_theta_cov;        /* Sigma_e, ntheta x ntheta (errThetaCov_) */
+  const double *const *noise_scale;   /* ntheta host arrays of nx*ny, (i, j) at i + nx*j */
+} cg_local_noise_args;
+int cg_local_noise_create(cg_ctx *ctx, const cg_local_noise_args *args, cg_local_noise **ln);
+/* Host copies of the per-trace matrices, trace t = i + nx*j at [9t, 9t + 9) row-major; either
+ * pointer may be null.  Synchronises. */
+int cg_local_noise_matrices(const cg_local_noise *ln, double *sigmanew, double *kept);
+/* The posterior update of correctVpVsRho (:2243-2263) in place on the real-domain posterior
+ * means post[3] with the prior means mean[3] (meanVp2_ ...): inside the box
+ * p = float(m + s_a0 d0 + s_a1 d1 + s_a2 d2), d = p - m in float, the sum in double. */
+int cg_local_noise_correct(cg_local_noise *ln, cg_grid *const post[3], cg_grid *const mean[3]);
+/* simulate's local-noise scaling (:1485-1504) and the posterior added after it: inside the box
+ * the real-domain noise fields real[3] become float(k_a0 r0 + k_a1 r1 + k_a2 r2) (double), then
+ * every padded cell gets post[3] added in float (FFTGrid::add). */
+int cg_local_noise_correct_realisation(cg_local_noise *ln, cg_grid *const real[3], cg_grid *const post[3]);
+void cg_local_noise_destroy(cg_local_noise *ln);
 
 /* ------------------------------------------------------------------ host-side setup */
 /* Per-k_z tables of the constant-thickness branch (avoinversion.cpp:944-975; Wavelet1D

@@ -10,6 +10,7 @@ It wraps two shared libraries built by oracle/Makefile:
   reference tree was not available at build time).
 """
 import ctypes
+import math
 import os
 
 import numpy as np
@@ -135,6 +136,8 @@ def ref():
         R.ref_rnorm01.restype = ctypes.c_double
         R.ref_rnorm01_n.argtypes = [_c_double_p, ctypes.c_long]
         R.ref_fill_complex_noise.argtypes = [ctypes.c_int] * 3 + [_c_float_p]
+        R.ref_matr_eigen.restype = ctypes.c_int
+        R.ref_matr_eigen.argtypes = [ctypes.c_int, _c_double_p, _c_double_p, _c_double_p]
         _ref = R
     return _ref
 
@@ -722,3 +725,156 @@ def synthetic_seismic(vp, vs, rho, wavelets, A, nzp):
         s = np.fft.rfft(r, axis=0) * np.conj(W)[:, None, None]
         out.append(np.fft.irfft(s, n=nzp, axis=0)[:nz])
     return out
+
+
+# ---------------------------------------------------------------- local noise (correctVpVsRho)
+RMISSING = -99999.0
+
+
+def _ev(m):
+    """NRLib::ComputeEigenVectors (libs/nrlib/flens/nrlib_flens.cpp:114-121): flens::ev on a
+    general matrix, i.e. LAPACK dgeev, keeping the real parts of the eigenvalues and the right
+    eigenvectors as columns.  numpy.linalg.eig calls the same LAPACK driver."""
+    w, v = np.linalg.eig(np.asarray(m, dtype=np.float64))
+    return np.real(w).copy(), np.real(v).copy()
+
+
+def _vfv(v, f):
+    """(V diag(f)) V^T: the reference's `H = Evec * EvalMat; X = H * EvecT`."""
+    return (v * np.asarray(f, dtype=np.float64)[None, :]) @ v.T
+
+
+def sort3x3(w, v):
+    """NRLib::Sort3x3 (nrlib_flens.cpp:247-280): eigenvalues in decreasing order (the first of
+    equal ones first), eigenvector columns permuted alike."""
+    idx = []
+    for _ in range(3):
+        best, bi = -100000.0, -1
+        for i in range(3):
+            if i not in idx and w[i] > best:
+                best, bi = w[i], i
+        idx.append(bi)
+    return np.array([w[i] for i in idx]), v[:, idx].copy()
+
+
+def compute_g(prior_var0, post_var0, err_theta_cov):
+    """AVOInversion::computeG (avoinversion.cpp:1943-2026): the n_theta x 3 operator G with
+    Sigma_m - Sigma_m G^T (G Sigma_m G^T + Sigma_e)^-1 G Sigma_m = postVar0 (the posterior the
+    full inversion gave at lag 0), from the eigen decompositions of Sigma_m - postVar0, Sigma_m,
+    A = Sigma_m^-1/2 (Sigma_m - postVar0) Sigma_m^-1/2 (sorted, Sort3x3) and Sigma_e."""
+    Sm = np.asarray(prior_var0, dtype=np.float64)
+    E = np.asarray(err_theta_cov, dtype=np.float64)
+    nt = E.shape[0]
+    w, V = _ev(Sm - np.asarray(post_var0, dtype=np.float64))
+    Sdelta = _vfv(V, np.where(w > 0.0, w, 0.0))
+    w, V = _ev(Sm)
+    Sinv = _vfv(V, np.where(w > 1e-7, 1.0 / np.sqrt(np.where(w > 1e-7, w, 1.0)), 0.0))
+    A = (Sinv @ Sdelta) @ Sinv
+    w, V = sort3x3(*_ev(A))
+    Lg = np.zeros((nt, 3))
+    for i in range(min(nt, 3)):
+        if w[i] > 0:
+            Lg[i, i] = math.sqrt(w[i] / (1.0 - w[i]))
+    we, Ve = _ev(E)
+    Esq = _vfv(Ve, np.where(we > 0.0, np.sqrt(np.where(we > 0.0, we, 0.0)), 0.0))
+    return ((Esq @ Lg) @ V.T) @ Sinv
+
+
+def post_var0_inv_sqrt(post_var0, use_ref=False):
+    """correctVpVsRho's postVar0^-1/2 (avoinversion.cpp:2182-2191): lib_matr_eigen, 1/sqrt of the
+    positive eigenvalues, V D V^T.  use_ref: through the reference-compiled lib_matr_eigen."""
+    a = np.ascontiguousarray(post_var0, dtype=np.float64)
+    if use_ref:
+        vec, val = np.zeros((3, 3)), np.zeros(3)
+        ref().ref_matr_eigen(3, _ptr(a, ctypes.c_double), _ptr(vec, ctypes.c_double), _ptr(val, ctypes.c_double))
+    else:
+        val, vec = np.linalg.eigh(a)
+    return _vfv(vec, np.where(val > 0.0, 1.0 / np.sqrt(np.where(val > 0.0, val, 1.0)), 0.0))
+
+
+def new_posterior_cov_pointwise(G, scales, prior_var0, post_var0, err_theta_cov):
+    """AVOInversion::newPosteriorCovPointwise (avoinversion.cpp:2028-2124) for one trace:
+    sigmanew = sqrt(Sigma_m G^T M^-1 G Sigma_m) (Sigma_m - postVar0)^-1/2 with
+    M = G Sigma_m G^T + D Sigma_e D, D = diag(sqrt(scales)) (eigenvalues <= 1e-8 of M dropped from
+    its inverse, <= 1e-7 of Sigma_m - postVar0 from its inverse root), and
+    sqrt(Sigma_m - Sigma_m G^T M^-1 G Sigma_m).  Returns (sigmanew, sigmamdnew)."""
+    G = np.asarray(G, dtype=np.float64)
+    Sm = np.asarray(prior_var0, dtype=np.float64)
+    E = np.asarray(err_theta_cov, dtype=np.float64)
+    nt = E.shape[0]
+    D = np.zeros((nt, nt))
+    for i in range(nt):
+        D[i, i] = math.sqrt(scales[i])
+    sigmaenew = (D @ E) @ D
+    help_ = ((G @ Sm) @ G.T) + sigmaenew
+    we, Ve = _ev(help_)
+    Minv = _vfv(Ve, np.where(we > 1e-8, 1.0 / np.where(we > 1e-8, we, 1.0), 0.0))
+    deltanew = (((Sm @ G.T) @ Minv) @ G) @ Sm
+    w, V = _ev(Sm - deltanew)
+    sigmamdnew = _vfv(V, np.where(w > 0.0, np.sqrt(np.where(w > 0.0, w, 0.0)), 0.0))
+    w, V = _ev(deltanew)
+    dsq = _vfv(V, np.where(w > 0.0, np.sqrt(np.where(w > 0.0, w, 0.0)), 0.0))
+    w, V = _ev(Sm - np.asarray(post_var0, dtype=np.float64))
+    sdi = _vfv(V, np.where(w > 1e-7, 1.0 / np.sqrt(np.where(w > 1e-7, w, 1.0)), 0.0))
+    return dsq @ sdi, sigmamdnew
+
+
+def find_min(a, missing=RMISSING):
+    """NRLib::Grid2D::FindMin (libs/nrlib/grid/grid2d.hpp:182-191) over a grid in storage order."""
+    flat = np.asarray(a, dtype=np.float64).ravel()
+    m = flat[0]
+    for v in flat:
+        if (m == missing or v < m) and v != missing:
+            m = v
+    return m
+
+
+def local_noise_matrices(G, scale_grids, prior_var0, post_var0, err_theta_cov):
+    """correctVpVsRho's per-trace matrices (avoinversion.cpp:2206-2241).  scale_grids: (n_theta,
+    ny, nx) local noise scales (Grid2D (i, j) at [j, i]).  Returns (sigmanew, stored), each
+    (ny, nx, 3, 3): the posterior correction and sqrt(local posterior) postVar0^-1/2 (kept for
+    the realisations, sigmamdnew_)."""
+    sg = np.asarray(scale_grids, dtype=np.float64)
+    nt, ny, nx = sg.shape
+    mins = [find_min(sg[l]) for l in range(nt)]
+    old = post_var0_inv_sqrt(post_var0)
+    sn = np.zeros((ny, nx, 3, 3))
+    st = np.zeros((ny, nx, 3, 3))
+    for j in range(ny):
+        for i in range(nx):
+            sc = [sg[l, j, i] / mins[l] for l in range(nt)]
+            a, b = new_posterior_cov_pointwise(G, sc, prior_var0, post_var0, err_theta_cov)
+            sn[j, i] = a
+            st[j, i] = b @ old
+    return sn, st
+
+
+def correct_vp_vs_rho(post, mean, nx, ny, nz, sigmanew):
+    """The per-sample update of correctVpVsRho (avoinversion.cpp:2243-2263) on real-domain padded
+    grids (3, nzp, nyp, rnxp) float32: diff = post - mean in float, then
+    float(mean + s00 d0 + s01 d1 + s02 d2) evaluated in double, left to right."""
+    out = np.array(post, dtype=np.float32, copy=True)
+    p = np.asarray(post, dtype=np.float32)[:, :nz, :ny, :nx]
+    m = np.asarray(mean, dtype=np.float32)[:, :nz, :ny, :nx]
+    d = (p - m).astype(np.float32)  # float subtraction
+    s = np.asarray(sigmanew, dtype=np.float64)[None]  # (1, ny, nx, 3, 3)
+    for a in range(3):
+        acc = m[a].astype(np.float64) + s[..., a, 0] * d[0].astype(np.float64)
+        acc = acc + s[..., a, 1] * d[1].astype(np.float64)
+        acc = acc + s[..., a, 2] * d[2].astype(np.float64)
+        out[a, :nz, :ny, :nx] = acc.astype(np.float32)
+    return out
+
+
+def correct_realisation(seed, post, nx, ny, nz, stored):
+    """simulate with local noise (avoinversion.cpp:1485-1504): inside the box the realisation's
+    noise fields become float(s0 vp + s1 vs + s2 rho) (double, left to right), then the whole
+    padded grid gets the posterior added in float (FFTGrid::add, fftgrid.cpp:1659-1678)."""
+    out = np.array(seed, dtype=np.float32, copy=True)
+    z = np.asarray(seed, dtype=np.float32)[:, :nz, :ny, :nx].astype(np.float64)
+    s = np.asarray(stored, dtype=np.float64)[None]
+    for a in range(3):
+        acc = s[..., a, 0] * z[0] + s[..., a, 1] * z[1]
+        acc = acc + s[..., a, 2] * z[2]
+        out[a, :nz, :ny, :nx] = acc.astype(np.float32)
+    return (out + np.asarray(post, dtype=np.float32)).astype(np.float32)

@@ -2,7 +2,7 @@
  * ref_harness.cpp -- drives the reference's own compiled code (oracle/_ref, test
  * infrastructure only).  Linked against, and compiled from their places in the CRAVA tree:
  *   libs/fft/fftw/*.c, libs/fft/rfftw/*.c   FFTW 2.1.2, float (libs/fft/include/config.h:90)
- *   libs/lib/lib_matr.c                     small complex linear algebra
+ *   libs/lib/lib_matr.c                     small complex linear algebra, symmetric eigen
  *   libs/lib/random.cpp                     RandomGen (LCG + Marsaglia-Bray)
  * The harness restates only the call sequences (FFTGrid::fftInPlace/invFFTInPlace,
  * the per-wavenumber loop of avoinversion.cpp:1016-1083, simulate's colouring 1421-1460,
@@ -256,6 +256,29 @@ void ref_fill_complex_noise(int nxp, int nyp, int nzp, float *cv)
       cv[2 * i + 1] = float(sd * RandomGen::rnorm01());
     }
   }
+}
+
+/* lib_matr_eigen (lib_matr.c:920-1000, tred2 + tqli): a is n x n row-major, vec receives the
+ * eigenvectors as columns (row-major), val the eigenvalues; returns the routine's error flag. */
+int ref_matr_eigen(int n, const double *a, double *vec, double *val)
+{
+  double **m = (double **)malloc(sizeof(double *) * n);
+  double **v = (double **)malloc(sizeof(double *) * n);
+  for (int i = 0; i < n; i++) {
+    m[i] = (double *)malloc(sizeof(double) * n);
+    v[i] = (double *)malloc(sizeof(double) * n);
+    for (int j = 0; j < n; j++) m[i][j] = a[i * n + j];
+  }
+  int err = 0;
+  lib_matr_eigen(m, n, v, val, &err);
+  for (int i = 0; i < n; i++) {
+    for (int j = 0; j < n; j++) vec[i * n + j] = v[i][j];
+    free(m[i]);
+    free(v[i]);
+  }
+  free(m);
+  free(v);
+  return err;
 }
 
 } /* extern "C" */

@@ -91,6 +91,30 @@ def test_mixed_precision_matches_fp64_oracle(ctx):
     assert n64 <= ninv
 
 
+def test_ill_conditioned_wavenumbers_resolved_in_fp64(ctx):
+    """Seismic with S/N 1e8 on five angles: M = K Sigma K^H + noise is a rank-3 signal over a
+    tiny noise floor.  FP32 throughout (the reference's arithmetic) fails the Cholesky
+    factorisation at thousands of wavenumbers (keeping the prior there) and loses the update and
+    the covariance; the mixed mode re-solves every wavenumber whose pivot ratio exceeds 1e6 (or
+    whose FP32 factorisation fails) in FP64.  The covariance then matches the FP64 oracle; the
+    update keeps an error of about eps |d| / |d - K m| from the residual of the wavenumbers left
+    in FP32 (cancellation in the data, not the conditioning of M)."""
+    sn = 1e8
+    p = P.make_problem(20, 10, 30, seed=6, angles_deg=(0.0, 10.0, 20.0, 30.0, 40.0), peak_hz=30.0, sn=sn)
+    mean, data = P.forward_spectra(p, np.float32)
+    om, od, oc, _, _ = O.posterior(p, mean.astype(np.complex128), data.astype(np.complex128), None, "f64")
+    rm, rd, rc, (ninv, rfail, _) = _run_gpu(ctx, p, mean, data, precision=C.PREC_REFERENCE)
+    gm, gd, gc, (ninv2, nfail, n64) = _run_gpu(ctx, p, mean, data, precision=C.PREC_MIXED)
+    e_ref = rel_l2(rm - mean, om - mean), rel_l2(rc, oc)
+    e_mix = rel_l2(gm - mean, om - mean), rel_l2(gc, oc), rel_l2(gd, od)
+    print("S/N %.0e: FP32 update %.2e cov %.2e (%d failures); mixed update %.2e cov %.2e residual %.2e, "
+          "%d of %d wavenumbers re-solved in FP64" % (sn, e_ref[0], e_ref[1], rfail, e_mix[0], e_mix[1], e_mix[2],
+                                                       n64, ninv2), flush=True)
+    assert ninv == ninv2 and rfail > 1000 and nfail == 0 and n64 > ninv // 2
+    assert e_ref[1] > 1.0 and e_mix[1] < 1e-5
+    assert e_ref[0] > 0.5 and e_mix[0] < 5e-3 and e_mix[2] < 2e-3
+
+
 def test_all_wavenumbers_through_fp64_pass(ctx):
     p = P.make_problem(16, 12, 20, seed=7)
     mean, data = P.forward_spectra(p, np.float32)
