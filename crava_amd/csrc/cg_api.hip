@@ -1379,7 +1379,9 @@ static int simulate_impl(cg_ctx *ctx, const cg_sim_args *a, cg_grid *const *next
     p.mean[g] = spectral == 1 && a->mean[g] ? reinterpret_cast<const float2 *>(a->mean[g]->d) : nullptr;
   p.from_grids = a->noise_source == CG_NOISE_GRIDS;
   p.seed = a->seed;
-  p.stream0 = 4u * a->realisation;  // stream pairs 2r (both halves) and 2r + 1 (first half)
+  // streams 3r, 3r + 1, 3r + 2: a pair of consecutive realisations from an even one takes three
+  // whole Philox blocks per cell
+  p.stream0 = 3u * a->realisation;
   p.fails = a->stats ? reinterpret_cast<unsigned long long *>(ctx->counters) : nullptr;
   int rc = CG_OK;
   if (p.fails) rc = cg::zero_counters(ctx, p.fails, 1);
@@ -1391,7 +1393,7 @@ static int simulate_impl(cg_ctx *ctx, const cg_sim_args *a, cg_grid *const *next
   if (next) {
     // both realisations in one launch, or one after the other
     for (int g = 0; g < 3; ++g) p.out2[g] = reinterpret_cast<float2 *>(next[g]->d);
-    p.stream1 = 4u * (a->realisation + 1u);
+    p.stream1 = 3u * (a->realisation + 1u);
     if (fusable && cg::fixed_colour_zinv2(ctx, p, p.nzp, (long)p.cnxp * p.nyp, &rc) != 0) {
       z_done = true;
     } else if (!p.from_grids) {
@@ -2431,7 +2433,7 @@ int cg_batch_create(cg_ctx *ctx, int n, const cg_inversion *inv, cg_batch **out)
       for (int g = 0; g < 6; ++g) it.p.cov[g] = reinterpret_cast<const float2 *>(b->inv[i].post.cov[g]->d);
       for (int g = 0; g < 3; ++g) it.p.out[g] = reinterpret_cast<float2 *>(b->outs[i][3 * r + g]->d);
       it.p.seed = b->inv[i].seed;
-      it.p.stream0 = 4u * (unsigned)r;
+      it.p.stream0 = 3u * (unsigned)r;
       return it;
     };
     size_t gi_f = 0;
@@ -2468,7 +2470,7 @@ int cg_batch_create(cg_ctx *ctx, int n, const cg_inversion *inv, cg_batch **out)
       for (int g = 0; g < 6; ++g) it.p.cov[g] = reinterpret_cast<const float2 *>(b->inv[i].post.cov[g]->d);
       for (int g = 0; g < 3; ++g) it.p.out[g] = reinterpret_cast<float2 *>(b->outs[i][3 * r + g]->d);
       it.p.seed = b->inv[i].seed;
-      it.p.stream0 = 4u * (unsigned)r;
+      it.p.stream0 = 3u * (unsigned)r;
       it.block0 = blk;
       blk += (unsigned)((ref->csize + cg::kThreads - 1) / cg::kThreads);
       ci.push_back(it);

@@ -38,6 +38,7 @@ __device__ __forceinline__ void philox(uint32_t (&c)[4], uint32_t k0, uint32_t k
 // keeps every operand normal.
 __device__ __forceinline__ float2 box_muller(uint32_t a, uint32_t b)
 {
+#pragma clang fp contract(off)
   const float u1 = (float)((a >> 8) + 1u) * 5.9604644775390625e-08f;
   const float u2 = (float)(b >> 8) * 5.9604644775390625e-08f;
   const float r = __builtin_amdgcn_sqrtf(-1.38629436111989061883f * __builtin_amdgcn_logf(u1));  // -2 ln 2 log2 u1
@@ -145,6 +146,7 @@ struct c32 {
 // 1e-20; divisions by a pivot are products with its reciprocal.
 __device__ __forceinline__ int chol3(c32 (&m)[6])
 {
+#pragma clang fp contract(off)
   // t(i,j) = i(i+1)/2 + j: 0:(0,0) 1:(1,0) 2:(1,1) 3:(2,0) 4:(2,1) 5:(2,2)
   const float factor = m[0].re;
   if (!(factor > 0.f)) return 1;
@@ -192,11 +194,23 @@ __device__ __forceinline__ int chol3(c32 (&m)[6])
   return 0;
 }
 
+// o + l z with explicitly placed fused multiply-adds (the per-cell functions compile with
+// contraction off, so a single and a paired cell round alike whatever kernel inlines them).
+__device__ __forceinline__ float2 lz(const c32 &l, const float2 &z, float2 o)
+{
+  o.x = __builtin_fmaf(l.re, z.x, o.x);
+  o.x = __builtin_fmaf(-l.im, z.y, o.x);
+  o.y = __builtin_fmaf(l.re, z.y, o.y);
+  o.y = __builtin_fmaf(l.im, z.x, o.y);
+  return o;
+}
+
 // One cell of the colouring: out[g] = L z_g (+ the mean spectrum, when given) for the posterior
 // covariance at idx (L z = 0 where the factorisation fails, which the return value reports).  Noise from the counter generator
 // (streams stream0 .. stream0+2), or `z` as given when from_grids.
 __device__ __forceinline__ bool colour_cell(const cg::ColourParams &p, long idx, float2 (&z)[3], float2 (&o)[3])
 {
+#pragma clang fp contract(off)
   const float2 c00 = cg::gld(p.cov[0] + idx), c11 = cg::gld(p.cov[1] + idx), c22 = cg::gld(p.cov[2] + idx);
   const float2 c01 = cg::gld(p.cov[3] + idx), c02 = cg::gld(p.cov[4] + idx), c12 = cg::gld(p.cov[5] + idx);
   float2 mv[3];  // loaded with the covariance, before the arithmetic that hides their latency
@@ -228,8 +242,7 @@ __device__ __forceinline__ bool colour_cell(const cg::ColourParams &p, long idx,
 #pragma unroll
       for (int b = 0; b <= a; ++b) {
         const c32 l = m[row[a] + b];
-        o[a].x += l.re * z[b].x - l.im * z[b].y;
-        o[a].y += l.re * z[b].y + l.im * z[b].x;
+        o[a] = lz(l, z[b], o[a]);
       }
   }
 #pragma unroll
@@ -240,11 +253,20 @@ __device__ __forceinline__ bool colour_cell(const cg::ColourParams &p, long idx,
   return failed;
 }
 
+// Whether a pair's six streams (stream0 .. stream0 + 5) are three whole Philox blocks.
+__device__ __host__ __forceinline__ bool pair_in_three_blocks(const cg::ColourParams &p)
+{
+  return p.stream1 == p.stream0 + 3u && !(p.stream0 & 1u);
+}
+
 // colour_cell for two realisations of one posterior (counter noise only): the covariance and
 // mean are read and the 3x3 factor computed once; each realisation's arithmetic is colour_cell's,
-// operation for operation, so the pair equals two single cells bit for bit.
-__device__ __forceinline__ bool colour_cell2(const cg::ColourParams &p, long idx, float2 (&o)[3], float2 (&o2)[3])
+// operation for operation, so the pair equals two single cells bit for bit.  THREE: the streams
+// are three whole Philox blocks (pair_in_three_blocks), drawn once each.
+template <bool THREE>
+__device__ __forceinline__ bool colour_cell2t(const cg::ColourParams &p, long idx, float2 (&o)[3], float2 (&o2)[3])
 {
+#pragma clang fp contract(off)
   const float2 c00 = cg::gld(p.cov[0] + idx), c11 = cg::gld(p.cov[1] + idx), c22 = cg::gld(p.cov[2] + idx);
   const float2 c01 = cg::gld(p.cov[3] + idx), c02 = cg::gld(p.cov[4] + idx), c12 = cg::gld(p.cov[5] + idx);
   float2 mv[3];
@@ -261,8 +283,24 @@ __device__ __forceinline__ bool colour_cell2(const cg::ColourParams &p, long idx
     kind = noise_source(idx, p.cnxp, p.nyp, p.nzp, &src);
   }
   float2 z[3], z2[3];
-  cb_normal2x3(p.seed, p.stream0, (uint64_t)src, z);
-  cb_normal2x3(p.seed, p.stream1, (uint64_t)src, z2);
+  if (THREE) {
+    // six consecutive streams from an even one: exactly three Philox blocks, the same words
+    // cb_normal2x3 takes for each realisation
+    uint32_t a[4], b[4], d[4];
+    const uint32_t q = p.stream0 >> 1;
+    cb_block(p.seed, q, (uint64_t)src, a);
+    cb_block(p.seed, q + 1u, (uint64_t)src, b);
+    cb_block(p.seed, q + 2u, (uint64_t)src, d);
+    z[0] = box_muller(a[0], a[1]);
+    z[1] = box_muller(a[2], a[3]);
+    z[2] = box_muller(b[0], b[1]);
+    z2[0] = box_muller(b[2], b[3]);
+    z2[1] = box_muller(d[0], d[1]);
+    z2[2] = box_muller(d[2], d[3]);
+  } else {
+    cb_normal2x3(p.seed, p.stream0, (uint64_t)src, z);
+    cb_normal2x3(p.seed, p.stream1, (uint64_t)src, z2);
+  }
 #pragma unroll
   for (int g = 0; g < 3; ++g) {
     z[g] = noise_value(kind, z[g]);
@@ -278,10 +316,8 @@ __device__ __forceinline__ bool colour_cell2(const cg::ColourParams &p, long idx
 #pragma unroll
       for (int b = 0; b <= a; ++b) {
         const c32 l = m[row[a] + b];
-        o[a].x += l.re * z[b].x - l.im * z[b].y;
-        o[a].y += l.re * z[b].y + l.im * z[b].x;
-        o2[a].x += l.re * z2[b].x - l.im * z2[b].y;
-        o2[a].y += l.re * z2[b].y + l.im * z2[b].x;
+        o[a] = lz(l, z[b], o[a]);
+        o2[a] = lz(l, z2[b], o2[a]);
       }
   }
 #pragma unroll
@@ -292,6 +328,11 @@ __device__ __forceinline__ bool colour_cell2(const cg::ColourParams &p, long idx
     o2[g].y += mv[g].y;
   }
   return failed;
+}
+
+__device__ __forceinline__ bool colour_cell2(const cg::ColourParams &p, long idx, float2 (&o)[3], float2 (&o2)[3])
+{
+  return pair_in_three_blocks(p) ? colour_cell2t<true>(p, idx, o, o2) : colour_cell2t<false>(p, idx, o, o2);
 }
 
 }  // namespace cgcol

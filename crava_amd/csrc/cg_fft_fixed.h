@@ -894,20 +894,34 @@ k_colour_zinv(cg::ColourParams one, const cg::ColourItem *__restrict__ items, in
   IO::from_lds(buf + g * N * TI + c, t, tw, sink);
 }
 
-// Two realisations of one posterior per workgroup (k_colour_zinv2): six column sets in LDS.
+// Two realisations of one posterior per workgroup.  Wide tiles matter most here (the covariance
+// and mean columns are read in TI-cell row segments): 16 columns with the second realisation
+// held in registers (k_colour_zinv2r, three column sets in LDS) where a thread then keeps at
+// most 6 cells, else six column sets (k_colour_zinv2) with the widest tile that fits the CU's
+// LDS and 1024 threads.  Measured on the 576 and 1152 cubes against 4-column tiles at two
+// workgroups per CU: 2.14 -> 1.72 ms (nzp 320, registers, 16 columns), 13.5 -> 10.3 ms
+// (nzp 384, six sets, 8 columns).
+template <int N>
+constexpr bool colour2_in_registers()
+{
+  constexpr int T = Plan<N>::TPS;
+  return (size_t)(N + 3 * (size_t)N * 16) * 8 <= kLdsMax && 3 * 16 * T <= 1024 && (N + 3 * T - 1) / (3 * T) <= 6;
+}
+
 template <int N>
 constexpr int colour_tile2()
 {
   constexpr int T = Plan<N>::TPS;
+  if (colour2_in_registers<N>()) return 16;
   int ti = 16;
-  while (ti > 1 && ((size_t)(N + 6 * (size_t)N * ti) * 8 > kLds || 6 * ti * T > 1024)) ti /= 2;
+  while (ti > 1 && ((size_t)(N + 6 * (size_t)N * ti) * 8 > kLdsMax || 6 * ti * T > 1024)) ti /= 2;
   return ti;
 }
 
 // k_colour_zinv for realisations stream0 and stream1 of one posterior: the covariance is read
 // and factorised once per cell (colour_cell2), both realisations' columns are transformed from
 // LDS; grid g < 3 goes to out[g], g >= 3 to out2[g - 3].
-template <int N, int TI>
+template <int N, int TI, bool THREE>
 __global__ void __launch_bounds__(6 * TI *Plan<N>::TPS)
 k_colour_zinv2(cg::ColourParams p, const float2 *__restrict__ twg)
 {
@@ -931,7 +945,7 @@ k_colour_zinv2(cg::ColourParams p, const float2 *__restrict__ twg)
     float2 o[3] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f), make_float2(0.f, 0.f)};
     float2 o2[3] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f), make_float2(0.f, 0.f)};
     bool failed = false;
-    if (live) failed = cgcol::colour_cell2(p, q0 + c + (long)z * plane, o, o2);
+    if (live) failed = cgcol::colour_cell2t<THREE>(p, q0 + c + (long)z * plane, o, o2);
     if (p.fails) {  // a failed cell zeroes the noise of both realisations: counted twice
       const unsigned long long msk = __ballot(failed);
       if (failed && (int)__lane_id() == __ffsll((long long)msk) - 1)
@@ -953,6 +967,83 @@ k_colour_zinv2(cg::ColourParams p, const float2 *__restrict__ twg)
     if (live) cg::gst(col + (long)e * plane, x);
   };
   IO::from_lds(buf + g * N * TI + c, t, tw, sink);
+}
+
+// k_colour_zinv2 with half the LDS per column: the first realisation's three columns go through
+// LDS and their inverse z transforms while the second's coloured values wait in registers, then
+// take the same LDS and transforms.  Twice the columns per workgroup at the same LDS (wider row
+// segments of the covariance reads), the same cells and arithmetic (bit for bit).
+// (At least four waves per SIMD: a 16-column tile is 960 threads, one workgroup per CU, which
+// needs at most 128 registers per lane.)
+template <int N, int TI, bool THREE>
+__global__ void __launch_bounds__(3 * TI *Plan<N>::TPS) __attribute__((amdgpu_waves_per_eu(4)))
+k_colour_zinv2r(cg::ColourParams p, const float2 *__restrict__ twg)
+{
+  using IO = typename Plan<N>::template IO<+1, TI>;
+  constexpr int TPS = Plan<N>::TPS;
+  constexpr int NT = 3 * TI * TPS;
+  constexpr int CELLS = N * TI;
+  constexpr int PER = (CELLS + NT - 1) / NT;
+  extern __shared__ float2 smem[];
+  float2 *tw = smem;
+  float2 *buf = smem + N;  // grid g (0..2), plane z, column c at (g N + z) TI + c
+  const int tid = threadIdx.x;
+  const long q0 = (long)cg_xcd_remap(blockIdx.x, gridDim.x) * TI;
+  load_tw(tw, twg, N, tid, NT);
+  const long plane = (long)p.cnxp * p.nyp;
+  float2 keep[PER][3];
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const int e = tid + i * NT;
+    const int c = e % TI, z = e / TI;
+    const bool live = e < CELLS && q0 + c < plane;
+    float2 o[3] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f), make_float2(0.f, 0.f)};
+#pragma unroll
+    for (int g = 0; g < 3; ++g) keep[i][g] = make_float2(0.f, 0.f);
+    bool failed = false;
+    if (live) failed = cgcol::colour_cell2t<THREE>(p, q0 + c + (long)z * plane, o, keep[i]);
+    if (p.fails) {  // a failed cell zeroes the noise of both realisations: counted twice
+      const unsigned long long msk = __ballot(failed);
+      if (failed && (int)__lane_id() == __ffsll((long long)msk) - 1)
+        atomicAdd(p.fails, 2ull * (unsigned long long)__popcll(msk));
+    }
+    if (e < CELLS) {
+#pragma unroll
+      for (int g = 0; g < 3; ++g) buf[(g * N + z) * TI + c] = o[g];
+    }
+    // one cell's loads in flight at a time: the kept values already take 6 PER registers
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  lds_sync();
+  const int g = tid / (TI * TPS), r = tid % (TI * TPS), c = r % TI, t = r / TI;
+  const bool live = q0 + c < plane;
+  const long off = live ? q0 + c : 0;
+  {
+    float2 *col = p.out[g] + off;
+    auto sink = [&](int e, float2 x) {
+      if (live) cg::gst(col + (long)e * plane, x);
+    };
+    IO::from_lds(buf + g * N * TI + c, t, tw, sink);
+  }
+  // The last stage of from_lds reads its inputs and passes a barrier before it computes and
+  // stores, so no thread reads the column sets any more: the second realisation may overwrite them.
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const int e = tid + i * NT;
+    const int cc = e % TI, z = e / TI;
+    if (e < CELLS) {
+#pragma unroll
+      for (int gg = 0; gg < 3; ++gg) buf[(gg * N + z) * TI + cc] = keep[i][gg];
+    }
+  }
+  lds_sync();
+  {
+    float2 *col = p.out2[g] + off;
+    auto sink = [&](int e, float2 x) {
+      if (live) cg::gst(col + (long)e * plane, x);
+    };
+    IO::from_lds(buf + g * N * TI + c, t, tw, sink);
+  }
 }
 
 // Wavefronts of the fused kernel one CU holds (LDS-limited).
@@ -982,16 +1073,30 @@ int colour_zinv_launch(cg_ctx *ctx, const cg::ColourParams *one, const cg::Colou
   return cg::hip_fail(ctx, hipGetLastError(), "k_colour_zinv");
 }
 
+// k_colour_zinv2 (REG false) or k_colour_zinv2r (REG true) with TI columns per workgroup.
+template <int N, int TI, bool REG>
+int colour_zinv2_go(cg_ctx *ctx, const cg::ColourParams &p, long plane, const float2 *tw)
+{
+  constexpr int SETS = REG ? 3 : 6;
+  const size_t lds = sizeof(float2) * (N + SETS * (size_t)N * TI);
+  const unsigned nblk = (unsigned)((plane + TI - 1) / TI);
+  if (nblk == 0) return CG_OK;
+  const bool three = cgcol::pair_in_three_blocks(p);
+  if constexpr (REG) {
+    if (three) k_colour_zinv2r<N, TI, true><<<nblk, SETS * TI * Plan<N>::TPS, lds, ctx->stream>>>(p, tw);
+    else k_colour_zinv2r<N, TI, false><<<nblk, SETS * TI * Plan<N>::TPS, lds, ctx->stream>>>(p, tw);
+  } else {
+    if (three) k_colour_zinv2<N, TI, true><<<nblk, SETS * TI * Plan<N>::TPS, lds, ctx->stream>>>(p, tw);
+    else k_colour_zinv2<N, TI, false><<<nblk, SETS * TI * Plan<N>::TPS, lds, ctx->stream>>>(p, tw);
+  }
+  return cg::hip_fail(ctx, hipGetLastError(), REG ? "k_colour_zinv2r" : "k_colour_zinv2");
+}
+
 // Two realisations of one posterior (p.out / p.stream0 and p.out2 / p.stream1), single launch.
 template <int N>
 int colour_zinv2_launch(cg_ctx *ctx, const cg::ColourParams &p, long plane, const float2 *tw)
 {
-  constexpr int TI = colour_tile2<N>();
-  const size_t lds = sizeof(float2) * (N + 6 * (size_t)N * TI);
-  const unsigned nblk = (unsigned)((plane + TI - 1) / TI);
-  if (nblk == 0) return CG_OK;
-  k_colour_zinv2<N, TI><<<nblk, 6 * TI * Plan<N>::TPS, lds, ctx->stream>>>(p, tw);
-  return cg::hip_fail(ctx, hipGetLastError(), "k_colour_zinv2");
+  return colour_zinv2_go<N, colour_tile2<N>(), colour2_in_registers<N>()>(ctx, p, plane, tw);
 }
 
 template <int N, int V>

@@ -171,10 +171,31 @@ int fixed_colour_zinv(cg_ctx *ctx, const ColourParams &p, int nzp, long plane, i
   return 1;
 }
 
+// Tune-table key of the fused-or-not choice for a pair of realisations.
+static TuneKey colour2_key(const cg_ctx *ctx, int nzp, long plane)
+{
+  return TuneKey{(long)ctx->device, (long)FIX_KINDS + 1, (long)nzp, plane, 0, 0};
+}
+
+// The paired colouring is fused where the autotuner timed it faster than the colouring kernel +
+// six z passes for this geometry; untuned, where its tile has at least 8 columns (64-byte row
+// segments; narrower tiles, at the longest lengths, read the covariance too finely).
+static const FixedEntry *fixed_colour2_entry(cg_ctx *ctx, int nzp, long plane)
+{
+  const FixedEntry *e = fixed_find(FIX_STRIDED, nzp);
+  if (!e || !e->cz2 || ctx->fused_colour == 0 || ctx->fft_engine != CG_FFT_AUTO || nzp < 2) return nullptr;
+  if (ctx->fused_colour != 1) {
+    std::lock_guard<std::mutex> lock(tune_mu);
+    const auto it = tune_table.find(colour2_key(ctx, nzp, plane));
+    if (it != tune_table.end() ? it->second == 0 : e->cz2_tile < 8) return nullptr;
+  }
+  return e;
+}
+
 int fixed_colour_zinv2(cg_ctx *ctx, const ColourParams &p, int nzp, long plane, int *rc)
 {
-  const FixedEntry *e = p.from_grids ? nullptr : fixed_colour_entry(ctx, nzp, plane);
-  if (!e || !e->cz2) return 0;
+  const FixedEntry *e = p.from_grids ? nullptr : fixed_colour2_entry(ctx, nzp, plane);
+  if (!e) return 0;
   const float2 *tw = twiddle_table(ctx, nzp);
   if (!tw) {
     *rc = CG_ENOMEM;
@@ -277,7 +298,7 @@ int autotune_geometry(cg_ctx *ctx, int nxp, int nyp, int nzp)
   const FixedEntry *ez = nzp > 1 ? fixed_find(FIX_STRIDED, nzp) : nullptr;
   if (!rc && ez && ez->cz) {
     const float2 *tw = twiddle_table(ctx, nzp);
-    float2 *out[3] = {nullptr, nullptr, nullptr};
+    float2 *out[3] = {nullptr, nullptr, nullptr}, *out2[3] = {nullptr, nullptr, nullptr};
     for (int g = 0; g < 3 && e == hipSuccess; ++g) e = hipMalloc(&out[g], sizeof(float2) * csize);
     if (e == hipSuccess)
       rc = fill_constant(ctx, reinterpret_cast<float *>(A), 2 * (size_t)csize, 1, 1, 1.0f);  // every float
@@ -307,10 +328,32 @@ int autotune_geometry(cg_ctx *ctx, int nxp, int nyp, int nzp)
         std::lock_guard<std::mutex> lock(tune_mu);
         tune_table[colour_key(ctx, nzp, plane)] = ms[1] < ms[0] ? 1 : 0;
       }
+      // two realisations per launch: the pair colouring kernel + six z passes, or fused
+      for (int g = 0; g < 3 && e == hipSuccess && ez->cz2; ++g) e = hipMalloc(&out2[g], sizeof(float2) * csize);
+      if (e == hipSuccess && ez->cz2) {
+        for (int g = 0; g < 3; ++g) p.out2[g] = out2[g];
+        p.stream1 = 3;
+        StridedArgs sb[3];
+        for (int g = 0; g < 3; ++g)
+          sb[g] = StridedArgs{out2[g], plane, 0, (unsigned)plane, (unsigned)plane, 0u, 1.0f};
+        ms[0] = time_it([&] {
+          int r = colour2(ctx, p);
+          for (int g = 0; g < 3 && !r; ++g) r = ez->s[v](ctx, +1, &sa[g], nullptr, 0, nblk, tw);
+          for (int g = 0; g < 3 && !r; ++g) r = ez->s[v](ctx, +1, &sb[g], nullptr, 0, nblk, tw);
+          return r;
+        });
+        ms[1] = time_it([&] { return ez->cz2(ctx, p, plane, tw); });
+        if (ms[0] >= 0.f && ms[1] >= 0.f) {
+          std::lock_guard<std::mutex> lock(tune_mu);
+          tune_table[colour2_key(ctx, nzp, plane)] = ms[1] < ms[0] ? 1 : 0;
+        }
+      }
     }
     (void)hipStreamSynchronize(ctx->stream);
-    for (int g = 0; g < 3; ++g)
+    for (int g = 0; g < 3; ++g) {
       if (out[g]) (void)hipFree(out[g]);
+      if (out2[g]) (void)hipFree(out2[g]);
+    }
     if (e != hipSuccess && !rc) rc = hip_fail(ctx, e, "autotune scratch");
   }
   if (t0) (void)hipEventDestroy(t0);

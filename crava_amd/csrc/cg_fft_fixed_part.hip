@@ -16,13 +16,13 @@ namespace {
   {cg::FIX_STRIDED, n, {cgfix::strided_tile<n, 0>(), cgfix::strided_tile<n, 1>()},           \
    {&cgfix::strided_launch<n, 0>, &cgfix::strided_launch<n, 1>}, {nullptr, nullptr},         \
    &cgfix::colour_zinv_launch<n>, cgfix::colour_waves<n>(), cgfix::colour_tile<n>(), 1,     \
-   &cgfix::colour_zinv2_launch<n>},
+   &cgfix::colour_zinv2_launch<n>, cgfix::colour_tile2<n>()},
 #define XE(n)                                                                        \
   {cg::FIX_XE, n, {cgfix::xe_rows<n, 0>(), cgfix::xe_rows<n, 1>()}, {nullptr, nullptr}, \
-   {&cgfix::xe_launch<n, 0>, &cgfix::xe_launch<n, 1>}, nullptr, 0, 0, cgfix::x_rows_dma<n>() > 0 ? 0 : 1, nullptr},
+   {&cgfix::xe_launch<n, 0>, &cgfix::xe_launch<n, 1>}, nullptr, 0, 0, cgfix::x_rows_dma<n>() > 0 ? 0 : 1, nullptr, 0},
 #define XO(n)                                                                        \
   {cg::FIX_XO, n, {cgfix::x_rows<n, 0>(), cgfix::x_rows<n, 1>()}, {nullptr, nullptr}, \
-   {&cgfix::xo_launch<n, 0>, &cgfix::xo_launch<n, 1>}, nullptr, 0, 0, 1, nullptr},
+   {&cgfix::xo_launch<n, 0>, &cgfix::xo_launch<n, 1>}, nullptr, 0, 0, 1, nullptr, 0},
 const cg::FixedEntry entries[] = {CG_CAT(CG_FFT_PART, CG_PART)(S, XE, XO)};
 #undef S
 #undef XE

@@ -82,6 +82,7 @@ struct FixedEntry {
   int cz_tile;   // its columns per workgroup
   int tune;      // 0: variant 0 is kept (the autotuner's timing on scratch grids misjudges it)
   ColourZ2Fn cz2;  // strided lengths: two realisations per launch (fused colouring)
+  int cz2_tile;    // its columns per workgroup
 };
 // Tile variant for a pass geometry: the autotuner's choice (cg_autotune) for this exact
 // geometry, or the context's forced variant, else 0.  `geom` identifies the pass: strided

@@ -164,7 +164,7 @@ def test_batch_fused_colouring_matches_unfused_and_oracle(ctx):
         ms, ds = P.forward_spectra(p, np.float64)
         _, _, oc, _, _ = O.posterior(p, ms, ds, None, "f64")
         for r in range(nreal):
-            noise = np.stack([O.fill_complex_noise_cb(p["nxp"], p["nyp"], p["nzp"], 60 + i, 4 * r + g)
+            noise = np.stack([O.fill_complex_noise_cb(p["nxp"], p["nyp"], p["nzp"], 60 + i, 3 * r + g)
                               for g in range(3)])
             z, _ = O.colour(oc, noise, "f64")
             real = np.stack([O.grid_ifft(z[g], p["nxp"], O.PARAMETER) for g in range(3)])

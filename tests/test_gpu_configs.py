@@ -231,7 +231,7 @@ def _check_cube(ctx, sp, label, rel=(0, 1)):
     for r in last:
         outs = prob.out[r % len(prob.out)]
         pure = [g.download_real() for g in outs]
-        noise = np.stack([O.fill_complex_noise_cb_planes(nxp, nyp, nzp, sp["seed"], 4 * r + g, kk_inv)
+        noise = np.stack([O.fill_complex_noise_cb_planes(nxp, nyp, nzp, sp["seed"], 3 * r + g, kk_inv)
                           for g in range(3)])
         z, _ = O.colour(covs, noise, "f64")
         for xc in xcols:  # the Hermitian part of those columns
@@ -389,7 +389,7 @@ def test_served_requests_at_serving_sizes():
         e_c = rel_l2(m["cov"], oc)
         e_r = 0.0
         for r, real in enumerate(m["real"]):
-            noise = np.stack([O.fill_complex_noise_cb(s["nxp"], s["nyp"], s["nzp"], s["seed"], 4 * r + g)
+            noise = np.stack([O.fill_complex_noise_cb(s["nxp"], s["nyp"], s["nzp"], s["seed"], 3 * r + g)
                               for g in range(3)])
             z, _ = O.colour(oc, noise, "f64")
             wr = np.stack([O.grid_ifft(z[g], s["nxp"], O.PARAMETER) for g in range(3)])

@@ -71,7 +71,7 @@ def test_realisation_matches_oracle(ctx, source, fused, spectral):
         st = np.array([23665], dtype=np.uint32)  # the reference stream (RandomGen default seed)
         noise = np.stack([O.fill_complex_noise(nxp, nyp, nzp, st) for _ in range(3)])
     else:
-        noise = np.stack([O.fill_complex_noise_cb(nxp, nyp, nzp, 99, 4 * 4 + g) for g in range(3)])
+        noise = np.stack([O.fill_complex_noise_cb(nxp, nyp, nzp, 99, 3 * 4 + g) for g in range(3)])
     z, nf = O.colour(oc, noise, "f32")
     want = np.stack([O.grid_ifft(z[g], nxp, O.PARAMETER) + mean_real[g] for g in range(3)])
     gc, gmean, gout = _grids(ctx, p, 6, C.COVARIANCE), _grids(ctx, p, 3, C.PARAMETER), _grids(ctx, p, 3, C.PARAMETER)
@@ -110,7 +110,7 @@ def test_fused_colour_at_production_depths(ctx, nz, nzp):
     nxp, nyp = p["nxp"], p["nyp"]
     assert ((nxp // 2 + 1) * nyp) % 8 != 0
     mean_real = np.stack([O.grid_ifft(m, nxp, O.PARAMETER, np.float32) for m in om])
-    noise = np.stack([O.fill_complex_noise_cb(nxp, nyp, nzp, 7, 4 * 2 + g) for g in range(3)])
+    noise = np.stack([O.fill_complex_noise_cb(nxp, nyp, nzp, 7, 3 * 2 + g) for g in range(3)])
     z, nf = O.colour(oc, noise, "f32")
     want = np.stack([O.grid_ifft(z[g], nxp, O.PARAMETER) + mean_real[g] for g in range(3)])
     gc, gmean, gout = _grids(ctx, p, 6, C.COVARIANCE), _grids(ctx, p, 3, C.PARAMETER), _grids(ctx, p, 3, C.PARAMETER)
@@ -130,13 +130,15 @@ def test_fused_colour_at_production_depths(ctx, nz, nzp):
         g.close()
 
 
-@pytest.mark.parametrize("fused,nz,nzp", [(1, 256, 320), (-1, 256, 320), (0, 256, 320), (1, 20, None)])
-def test_realisation_pair_equals_two_singles(ctx, fused, nz, nzp):
+@pytest.mark.parametrize("fused,nz,nzp,r0", [(1, 256, 320, 2), (1, 256, 320, 3), (-1, 256, 320, 2), (0, 256, 320, 2),
+                                           (0, 256, 320, 3), (1, 20, None, 2)])
+def test_realisation_pair_equals_two_singles(ctx, fused, nz, nzp, r0):
     """cg_simulate_pair: realisations r and r + 1 coloured in one launch (fused) or one after the
-    other (unfused), bit for bit the two single calls, failures summed."""
+    other (unfused), bit for bit the two single calls, failures summed.  From an even r the pair's
+    six streams are three whole Philox blocks; from an odd r they straddle four."""
     ctx.set_fused_colour(fused)
     p = P.make_problem(11, 7, nz, nzp=nzp, seed=9)
-    print("realisation pair", fused, nz, p["nzp"], flush=True)
+    print("realisation pair", fused, nz, p["nzp"], r0, flush=True)
     om, oc = _posterior_cov(p)
     nxp = p["nxp"]
     mean_real = np.stack([O.grid_ifft(m, nxp, O.PARAMETER, np.float32) for m in om])
@@ -146,9 +148,9 @@ def test_realisation_pair_equals_two_singles(ctx, fused, nz, nzp):
         gc[g].upload_complex(oc[g])
     for g in range(3):
         gmean[g].upload_real(mean_real[g])
-    fp = C.simulate(ctx, gc, gmean, outs[0], seed=5, realisation=2, stats=True, next_out=outs[1])
-    f1 = C.simulate(ctx, gc, gmean, outs[2], seed=5, realisation=2, stats=True)
-    f2 = C.simulate(ctx, gc, gmean, outs[3], seed=5, realisation=3, stats=True)
+    fp = C.simulate(ctx, gc, gmean, outs[0], seed=5, realisation=r0, stats=True, next_out=outs[1])
+    f1 = C.simulate(ctx, gc, gmean, outs[2], seed=5, realisation=r0, stats=True)
+    f2 = C.simulate(ctx, gc, gmean, outs[3], seed=5, realisation=r0 + 1, stats=True)
     ctx.set_fused_colour(-1)
     assert fp == f1 + f2
     for a, b in ((outs[0], outs[2]), (outs[1], outs[3])):

@@ -325,7 +325,7 @@ def test_sample_covariance_of_coloured_noise_is_the_posterior_covariance():
     n = 4000
     acc = np.zeros((3, 3) + oc.shape[1:], complex)
     for r in range(n):
-        noise = np.stack([O.fill_complex_noise_cb(p["nxp"], p["nyp"], p["nzp"], 99, 4 * r + g) for g in range(3)])
+        noise = np.stack([O.fill_complex_noise_cb(p["nxp"], p["nyp"], p["nzp"], 99, 3 * r + g) for g in range(3)])
         z, _ = O.colour(oc, noise, "f64")
         acc += np.einsum("a...,b...->ab...", z, np.conj(z))
     est = acc / n
