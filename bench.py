@@ -65,27 +65,42 @@ NAMES = {C.K_FFT_X: "fft_x", C.K_FFT_Y: "fft_y", C.K_FFT_Z: "fft_z", C.K_POSTERI
          C.K_NOISE: "colour", C.K_ELEMENTWISE: "elementwise"}
 
 
-def cpu_baseline(w, inv, R, seconds_hint=20.0):
-    """The reference's own code (oracle/_ref: FFTW 2.1.2, lib_matr, RandomGen) on one host core, on a
-    bounded sample, extrapolated to the reference path for this workload: 15 forward FFTs (3 means,
-    5 seismic, 6 prior covariance, errCorr), 9 inverse (3 means, 6 covariance), the posterior loop,
-    and per realisation 3 noise fills, the colouring loop and 3 inverse FFTs."""
+def _on_threads(fn, T):
+    """Runs fn(i), i < T, on T threads at once and returns the wall time (the reference's code is
+    called through ctypes, which releases the interpreter lock, so the threads run in parallel)."""
+    import threading
+    ths = [threading.Thread(target=fn, args=(i,)) for i in range(T)]
+    t = time.perf_counter()
+    for th in ths:
+        th.start()
+    for th in ths:
+        th.join()
+    return time.perf_counter() - t
+
+
+def cpu_baseline(w, inv, R, threads=1):
+    """The reference's own code (oracle/_ref: FFTW 2.1.2, lib_matr, RandomGen) on `threads` host
+    cores, on a bounded sample, extrapolated to the reference path for this workload: 15 forward
+    FFTs (3 means, 5 seismic, 6 prior covariance, errCorr), 9 inverse (3 means, 6 covariance), the
+    posterior loop, and per realisation 3 noise fills, the colouring loop and 3 inverse FFTs.  With
+    threads > 1 (the reference itself is single-threaded; this is the all-cores figure), every
+    sampled piece runs once per thread at the same time on its own data and counts as that many
+    pieces done; the noise fill stays serial (the reference's RandomGen is one process-wide
+    sequential stream)."""
     import oracle as O
     if not O.ref_available():
         return None
     R_ = O.ref()
+    T = max(1, threads)
     nxp, nyp, nzp = w["nxp"], w["nyp"], w["nzp"]
     cnxp = nxp // 2 + 1
     plane = cnxp * nyp
     rng = np.random.default_rng(1)
-    x = rng.standard_normal((nzp, nyp, 2 * cnxp)).astype(np.float32)
-    t = time.perf_counter()
-    R_.ref_grid_fft(nxp, nyp, nzp, O.DATA, O._ptr(x, O.ctypes.c_float))
-    t_fwd = time.perf_counter() - t
-    t = time.perf_counter()
-    R_.ref_grid_ifft(nxp, nyp, nzp, O.PARAMETER, O._ptr(x, O.ctypes.c_float))
-    t_inv = time.perf_counter() - t
-    del x
+    xs = [rng.standard_normal((nzp, nyp, 2 * cnxp)).astype(np.float32) for _ in range(T)]
+    t_fwd = _on_threads(lambda i: R_.ref_grid_fft(nxp, nyp, nzp, O.DATA, O._ptr(xs[i], O.ctypes.c_float)), T) / T
+    t_inv = _on_threads(lambda i: R_.ref_grid_ifft(nxp, nyp, nzp, O.PARAMETER, O._ptr(xs[i], O.ctypes.c_float)),
+                        T) / T
+    del xs
     # posterior loop on NPL planes of inverted and NPL planes of pass-through wavenumbers, with the
     # workload's tables, in the reference's covariance-grid form
     NPL = 4  # planes sampled per kind
@@ -107,15 +122,13 @@ def cpu_baseline(w, inv, R, seconds_hint=20.0):
         p["lam_grid"] = np.broadcast_to(inv.rxy[None], shape).astype(np.complex64)
         mean = (rng.standard_normal((3,) + shape) + 1j * rng.standard_normal((3,) + shape)).astype(np.complex64)
         data = (rng.standard_normal((inv.nt,) + shape) + 1j * rng.standard_normal((inv.nt,) + shape)).astype(np.complex64)
-        t = time.perf_counter()
-        _, _, c, _, _ = O.ref_posterior(p, mean, data, covs)
-        times[label] = (time.perf_counter() - t) / (NPL * plane)
+        out = [None] * T
+        wall = _on_threads(lambda i: out.__setitem__(i, O.ref_posterior(p, mean, data, covs)), T)
+        times[label] = wall / (T * NPL * plane)
         if label == "inv":
-            cov_post = c
-    seed = (rng.standard_normal((3,) + cov_post.shape[1:]) + 0j).astype(np.complex64)
-    t = time.perf_counter()
-    O.ref_colour(cov_post, seed)
-    t_col = (time.perf_counter() - t) / (NPL * plane)
+            cov_post = out[0][2]
+    seeds = [(rng.standard_normal((3,) + cov_post.shape[1:]) + 0j).astype(np.complex64) for _ in range(T)]
+    t_col = _on_threads(lambda i: O.ref_colour(cov_post, seeds[i]), T) / (T * NPL * plane)
     R_.ref_rng_seed(23665)
     noise = np.zeros((NPL, nyp, cnxp), dtype=np.complex64)
     t = time.perf_counter()
@@ -126,14 +139,15 @@ def cpu_baseline(w, inv, R, seconds_hint=20.0):
     total = (15 * t_fwd + 9 * t_inv + n_inv * times["inv"] + (n_all - n_inv) * times["pass"]
              + R * (3 * n_all * t_noise + n_all * t_col + 3 * t_inv))
     cells = w["nx"] * w["ny"] * w["nz"]
+    each = "" if T == 1 else ", %d at a time on %d threads (noise fill serial)" % (T, T)
     return {
-        "value": cells / total, "unit": "cells/s", "cores": 1, "kind": "reference",
+        "value": cells / total, "unit": "cells/s", "cores": T, "kind": "reference",
         "seconds_per_inversion": total,
         "sample": ("reference FFTW 2.1.2 rfftw3d forward (%.2f s) and inverse (%.2f s) of one 576x576x320 grid; "
                    "reference lib_matr posterior loop on %d inverted + %d pass-through k planes (%.2f / %.3f us "
-                   "per wavenumber); lib_matr colouring and RandomGen noise fill on %d planes; extrapolated to "
+                   "per wavenumber); lib_matr colouring and RandomGen noise fill on %d planes%s; extrapolated to "
                    "15 forward + 9 inverse FFTs, the full loop and %d realisations" %
-                   (t_fwd, t_inv, NPL, NPL, 1e6 * times["inv"], 1e6 * times["pass"], NPL, R)),
+                   (t_fwd, t_inv, NPL, NPL, 1e6 * times["inv"], 1e6 * times["pass"], NPL, each, R)),
     }
 
 
@@ -330,6 +344,15 @@ def bench_large(ctx, steps=2):
     prob.step(realisations=0)
     ctx.sync()
     ms0 = 1000.0 * (time.perf_counter() - t)
+    # which colouring the realisation pairs took (the autotuned choice at this length): the fused
+    # kernel runs no z launches of its own
+    ctx.set_profiling(True)
+    ctx.reset_kernel_time()
+    prob.realisations(2)
+    ctx.sync()
+    nz_launch = ctx.kernel_time(C.K_FFT_Z)[0]
+    n_col, ms_col = ctx.kernel_time(C.K_NOISE)
+    ctx.set_profiling(False)
     G = (sp["nxp"] // 2 + 1) * sp["nyp"] * sp["nzp"] * 8
     nt = len(sp["angles"])
     cells = S.cells(sp)
@@ -340,6 +363,7 @@ def bench_large(ctx, steps=2):
         "ms_per_step": ms, "cells_per_s": cells / (ms * 1e-3), "mean_cov_only_ms": ms0,
         "model_v1_frac": G * (5 * (3 + nt) + 21 + 24 * 20) / (ms * 1e-3) / HBM_PEAK,
         "mean_cov_model_v1_frac": G * (5 * (3 + nt) + 21) / (ms0 * 1e-3) / HBM_PEAK,
+        "pair_colouring": {"fused": nz_launch == 0, "colour_ms": round(ms_col / max(n_col, 1), 3)},
         "steps": steps,
     }
 
@@ -541,6 +565,9 @@ def main():
         log("cpu baseline ...")
         try:
             result["cpu_baseline"] = cpu_baseline(w, inv, R)
+            ncores = min(16, int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1)
+            if ncores > 1:
+                result["cpu_baseline_all_cores"] = cpu_baseline(w, inv, R, threads=ncores)
         except Exception as e:  # reported, not fatal: the GPU figure stands on its own
             result["cpu_baseline"] = {"error": repr(e)}
     ctx.close()

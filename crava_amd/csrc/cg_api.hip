@@ -1971,12 +1971,14 @@ struct cg_batch {
   std::vector<PostGroup> post;
   // Colouring of one realisation number r of every inversion that has it: the colouring kernel
   // over the inversions whose realisations are not fused, and per z length one launch of the
-  // colouring fused with the inverse z pass over the rest (whose z passes are then skipped).
+  // colouring fused with the inverse z pass over the rest (whose z passes are then skipped); for
+  // inversions whose pairs are fused, the launch at even r colours r and r + 1 together.
   struct FusedColour {
     const cg::FixedEntry *e = nullptr;
     const cg::ColourItem *items = nullptr;
     int nitems = 0;
     unsigned blocks = 0;
+    bool pair = false;  // realisations r and r + 1 per item (the paired kernel)
   };
   struct ColourPass {
     const cg::ColourItem *items = nullptr;
@@ -2107,7 +2109,8 @@ int enqueue_batch(cg_batch *b)
       rc = fork_join(b, 1 + (int)cp.fused.size(), [&](int k) {
         if (k == 0) return cg::colour_batch(ctx, cp.items, cp.nitems, cp.blocks);
         const auto &f = cp.fused[k - 1];
-        return cg::fixed_colour_zinv_list(ctx, f.e, f.items, f.nitems, f.blocks);
+        return f.pair ? cg::fixed_colour_zinv2_list(ctx, f.e, f.items, f.nitems, f.blocks)
+                      : cg::fixed_colour_zinv_list(ctx, f.e, f.items, f.nitems, f.blocks);
       });
     for (auto &p : b->sim[r])
       if (!rc) rc = run_pass(b, p);
@@ -2171,6 +2174,7 @@ int cg_batch_create(cg_ctx *ctx, int n, const cg_inversion *inv, cg_batch **out)
   std::vector<cg::ColourItem> col_items;
   std::vector<int> col_count(maxR, 0);
   std::vector<const cg::FixedEntry *> fused_e(n, nullptr);  // per inversion: fused colouring entry
+  std::vector<char> paired(n, 0);  // realisations 2m and 2m + 1 coloured by one paired launch
   size_t lds_fx = 0, lds_fy = 0, lds_fz = 0, lds_ix = 0, lds_iy = 0, lds_iz = 0;
   std::vector<size_t> lds_sx(maxR, 0), lds_sy(maxR, 0), lds_sz(maxR, 0);
   int rc = CG_OK;
@@ -2225,6 +2229,7 @@ int cg_batch_create(cg_ctx *ctx, int n, const cg_inversion *inv, cg_batch **out)
     }
     const cg::FixedEntry *fe = cg::fixed_colour_entry(ctx, ref->nzp, (long)ref->cnxp * ref->nyp);
     fused_e[i] = fe;
+    paired[i] = fe && cg::fixed_colour2_entry(ctx, v.mean_in[0]->nzp, (long)v.mean_in[0]->cnxp * v.mean_in[0]->nyp);
     for (int r = 0; r < v.nreal; ++r)
       for (int g = 0; g < 3; ++g) {
         cg_grid *o = b->outs[i][3 * r + g];
@@ -2301,16 +2306,29 @@ int cg_batch_create(cg_ctx *ctx, int n, const cg_inversion *inv, cg_batch **out)
     col_count[r] = cnt;
     o_col[r] = L.add(sizeof(cg::ColourItem) * cnt);
   }
-  // fused colouring: per realisation number, one list per z length (inversions in order)
+  // fused colouring: per realisation number, one list per z length (inversions in order), and
+  // at even r one list of pairs (r, r + 1) per z length for the paired inversions that have both
   std::map<const cg::FixedEntry *, std::vector<int>> fused_groups;
   for (int i = 0; i < n; ++i)
     if (fused_e[i]) fused_groups[fused_e[i]].push_back(i);
-  std::vector<std::vector<size_t>> o_fused(maxR);
+  // 0: not coloured at r by a fused launch; 1: single; 2: the first of a pair; 3: the second
+  auto fused_kind = [&](int i, int r) {
+    const int nr = b->inv[i].nreal;
+    if (!fused_e[i] || nr <= r) return 0;
+    if (!paired[i]) return 1;
+    if (r % 2 == 1) return 3;
+    return r + 1 < nr ? 2 : 1;
+  };
+  std::vector<std::vector<size_t>> o_fused(maxR), o_fused2(maxR);
   for (int r = 0; r < maxR; ++r)
     for (auto &kv : fused_groups) {
-      int c = 0;
-      for (int i : kv.second) c += b->inv[i].nreal > r;
+      int c = 0, c2 = 0;
+      for (int i : kv.second) {
+        c += fused_kind(i, r) == 1;
+        c2 += fused_kind(i, r) == 2;
+      }
       o_fused[r].push_back(L.add(sizeof(cg::ColourItem) * std::max(c, 1)));
+      o_fused2[r].push_back(L.add(sizeof(cg::ColourItem) * std::max(c2, 1)));
     }
   // per-inversion posterior tables
   struct PT {
@@ -2438,24 +2456,33 @@ int cg_batch_create(cg_ctx *ctx, int n, const cg_inversion *inv, cg_batch **out)
     };
     size_t gi_f = 0;
     for (auto &kv : fused_groups) {
-      std::vector<cg::ColourItem> fi;
-      unsigned fb = 0;
-      for (int i : kv.second) {
-        if (b->inv[i].nreal <= r) continue;
-        cg::ColourItem it = colour_item(i);
-        it.block0 = fb;
-        fb += (unsigned)(((long)it.p.cnxp * it.p.nyp + kv.first->cz_tile - 1) / kv.first->cz_tile);
-        fi.push_back(it);
+      for (int kind = 1; kind <= 2; ++kind) {
+        std::vector<cg::ColourItem> fi;
+        unsigned fb = 0;
+        const int tile = kind == 2 ? kv.first->cz2_tile : kv.first->cz_tile;
+        for (int i : kv.second) {
+          if (fused_kind(i, r) != kind) continue;
+          cg::ColourItem it = colour_item(i);
+          if (kind == 2) {  // streams 3r .. 3r + 5 with r even: three whole Philox blocks
+            for (int g = 0; g < 3; ++g) it.p.out2[g] = reinterpret_cast<float2 *>(b->outs[i][3 * (r + 1) + g]->d);
+            it.p.stream1 = 3u * (unsigned)(r + 1);
+          }
+          it.block0 = fb;
+          fb += (unsigned)(((long)it.p.cnxp * it.p.nyp + tile - 1) / tile);
+          fi.push_back(it);
+        }
+        const size_t off = kind == 2 ? o_fused2[r][gi_f] : o_fused[r][gi_f];
+        if (fi.empty()) continue;
+        put(off, fi.data(), sizeof(cg::ColourItem) * fi.size());
+        cg_batch::FusedColour fc;
+        fc.e = kv.first;
+        fc.items = reinterpret_cast<const cg::ColourItem *>(dev + off);
+        fc.nitems = (int)fi.size();
+        fc.blocks = fb;
+        fc.pair = kind == 2;
+        b->colour[r].fused.push_back(fc);
       }
-      const size_t off = o_fused[r][gi_f++];
-      if (fi.empty()) continue;
-      put(off, fi.data(), sizeof(cg::ColourItem) * fi.size());
-      cg_batch::FusedColour fc;
-      fc.e = kv.first;
-      fc.items = reinterpret_cast<const cg::ColourItem *>(dev + off);
-      fc.nitems = (int)fi.size();
-      fc.blocks = fb;
-      b->colour[r].fused.push_back(fc);
+      ++gi_f;
     }
     for (int i = 0; i < n; ++i) {
       if (b->inv[i].nreal <= r || fused_e[i]) continue;

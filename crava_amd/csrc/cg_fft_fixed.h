@@ -920,10 +920,12 @@ constexpr int colour_tile2()
 
 // k_colour_zinv for realisations stream0 and stream1 of one posterior: the covariance is read
 // and factorised once per cell (colour_cell2), both realisations' columns are transformed from
-// LDS; grid g < 3 goes to out[g], g >= 3 to out2[g - 3].
-template <int N, int TI, bool THREE>
+// LDS; grid g < 3 goes to out[g], g >= 3 to out2[g - 3].  One pair (`one`), or a list of them
+// (item i owns tiles [block0_i, block0_{i+1}); its descriptor is staged in LDS).
+template <int N, int TI, bool THREE, bool LIST>
 __global__ void __launch_bounds__(6 * TI *Plan<N>::TPS)
-k_colour_zinv2(cg::ColourParams p, const float2 *__restrict__ twg)
+k_colour_zinv2(cg::ColourParams one, const cg::ColourItem *__restrict__ items, int nitems,
+               const float2 *__restrict__ twg)
 {
   using IO = typename Plan<N>::template IO<+1, TI>;
   constexpr int TPS = Plan<N>::TPS;
@@ -934,8 +936,17 @@ k_colour_zinv2(cg::ColourParams p, const float2 *__restrict__ twg)
   float2 *tw = smem;
   float2 *buf = smem + N;  // grid g (0..5), plane z, column c at (g N + z) TI + c
   const int tid = threadIdx.x;
-  const long q0 = (long)cg_xcd_remap(blockIdx.x, gridDim.x) * TI;
+  __shared__ cg::ColourParams staged;
+  unsigned tile = cg_xcd_remap(blockIdx.x, gridDim.x);
+  if (LIST) {
+    const int it = find_item(items, nitems, tile);
+    tile -= items[it].block0;
+    if (tid == 0) staged = items[it].p;
+  }
+  const cg::ColourParams &p = LIST ? staged : one;
+  const long q0 = (long)tile * TI;
   load_tw(tw, twg, N, tid, NT);
+  if (LIST) lds_sync();  // the staged descriptor
   const long plane = (long)p.cnxp * p.nyp;
 #pragma unroll
   for (int i = 0; i < PER; ++i) {
@@ -975,9 +986,10 @@ k_colour_zinv2(cg::ColourParams p, const float2 *__restrict__ twg)
 // segments of the covariance reads), the same cells and arithmetic (bit for bit).
 // (At least four waves per SIMD: a 16-column tile is 960 threads, one workgroup per CU, which
 // needs at most 128 registers per lane.)
-template <int N, int TI, bool THREE>
+template <int N, int TI, bool THREE, bool LIST>
 __global__ void __launch_bounds__(3 * TI *Plan<N>::TPS) __attribute__((amdgpu_waves_per_eu(4)))
-k_colour_zinv2r(cg::ColourParams p, const float2 *__restrict__ twg)
+k_colour_zinv2r(cg::ColourParams one, const cg::ColourItem *__restrict__ items, int nitems,
+                const float2 *__restrict__ twg)
 {
   using IO = typename Plan<N>::template IO<+1, TI>;
   constexpr int TPS = Plan<N>::TPS;
@@ -988,8 +1000,17 @@ k_colour_zinv2r(cg::ColourParams p, const float2 *__restrict__ twg)
   float2 *tw = smem;
   float2 *buf = smem + N;  // grid g (0..2), plane z, column c at (g N + z) TI + c
   const int tid = threadIdx.x;
-  const long q0 = (long)cg_xcd_remap(blockIdx.x, gridDim.x) * TI;
+  __shared__ cg::ColourParams staged;
+  unsigned tile = cg_xcd_remap(blockIdx.x, gridDim.x);
+  if (LIST) {
+    const int it = find_item(items, nitems, tile);
+    tile -= items[it].block0;
+    if (tid == 0) staged = items[it].p;
+  }
+  const cg::ColourParams &p = LIST ? staged : one;
+  const long q0 = (long)tile * TI;
   load_tw(tw, twg, N, tid, NT);
+  if (LIST) lds_sync();  // the staged descriptor
   const long plane = (long)p.cnxp * p.nyp;
   float2 keep[PER][3];
 #pragma unroll
@@ -1073,30 +1094,38 @@ int colour_zinv_launch(cg_ctx *ctx, const cg::ColourParams *one, const cg::Colou
   return cg::hip_fail(ctx, hipGetLastError(), "k_colour_zinv");
 }
 
-// k_colour_zinv2 (REG false) or k_colour_zinv2r (REG true) with TI columns per workgroup.
+// k_colour_zinv2 (REG false) or k_colour_zinv2r (REG true) with TI columns per workgroup: one
+// pair (`one`; nblocks = its tiles), or a list of pairs whose six streams are three whole Philox
+// blocks each (even first realisations; nblocks = all tiles).
 template <int N, int TI, bool REG>
-int colour_zinv2_go(cg_ctx *ctx, const cg::ColourParams &p, long plane, const float2 *tw)
+int colour_zinv2_go(cg_ctx *ctx, const cg::ColourParams *one, const cg::ColourItem *items, int nitems,
+                    unsigned nblk, const float2 *tw)
 {
   constexpr int SETS = REG ? 3 : 6;
+  constexpr unsigned NT = SETS * TI * Plan<N>::TPS;
   const size_t lds = sizeof(float2) * (N + SETS * (size_t)N * TI);
-  const unsigned nblk = (unsigned)((plane + TI - 1) / TI);
   if (nblk == 0) return CG_OK;
-  const bool three = cgcol::pair_in_three_blocks(p);
-  if constexpr (REG) {
-    if (three) k_colour_zinv2r<N, TI, true><<<nblk, SETS * TI * Plan<N>::TPS, lds, ctx->stream>>>(p, tw);
-    else k_colour_zinv2r<N, TI, false><<<nblk, SETS * TI * Plan<N>::TPS, lds, ctx->stream>>>(p, tw);
+  const cg::ColourParams o = one ? *one : cg::ColourParams{};
+  if (items) {
+    if constexpr (REG) k_colour_zinv2r<N, TI, true, true><<<nblk, NT, lds, ctx->stream>>>(o, items, nitems, tw);
+    else k_colour_zinv2<N, TI, true, true><<<nblk, NT, lds, ctx->stream>>>(o, items, nitems, tw);
+  } else if (cgcol::pair_in_three_blocks(o)) {
+    if constexpr (REG) k_colour_zinv2r<N, TI, true, false><<<nblk, NT, lds, ctx->stream>>>(o, nullptr, 0, tw);
+    else k_colour_zinv2<N, TI, true, false><<<nblk, NT, lds, ctx->stream>>>(o, nullptr, 0, tw);
   } else {
-    if (three) k_colour_zinv2<N, TI, true><<<nblk, SETS * TI * Plan<N>::TPS, lds, ctx->stream>>>(p, tw);
-    else k_colour_zinv2<N, TI, false><<<nblk, SETS * TI * Plan<N>::TPS, lds, ctx->stream>>>(p, tw);
+    if constexpr (REG) k_colour_zinv2r<N, TI, false, false><<<nblk, NT, lds, ctx->stream>>>(o, nullptr, 0, tw);
+    else k_colour_zinv2<N, TI, false, false><<<nblk, NT, lds, ctx->stream>>>(o, nullptr, 0, tw);
   }
   return cg::hip_fail(ctx, hipGetLastError(), REG ? "k_colour_zinv2r" : "k_colour_zinv2");
 }
 
-// Two realisations of one posterior (p.out / p.stream0 and p.out2 / p.stream1), single launch.
+// Two realisations of one posterior (p.out / p.stream0 and p.out2 / p.stream1), or a list of
+// such pairs (block0 in tiles of colour_tile2<N>() columns).
 template <int N>
-int colour_zinv2_launch(cg_ctx *ctx, const cg::ColourParams &p, long plane, const float2 *tw)
+int colour_zinv2_launch(cg_ctx *ctx, const cg::ColourParams *one, const cg::ColourItem *items, int nitems,
+                        unsigned nblocks, const float2 *tw)
 {
-  return colour_zinv2_go<N, colour_tile2<N>(), colour2_in_registers<N>()>(ctx, p, plane, tw);
+  return colour_zinv2_go<N, colour_tile2<N>(), colour2_in_registers<N>()>(ctx, one, items, nitems, nblocks, tw);
 }
 
 template <int N, int V>

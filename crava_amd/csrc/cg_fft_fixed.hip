@@ -180,7 +180,7 @@ static TuneKey colour2_key(const cg_ctx *ctx, int nzp, long plane)
 // The paired colouring is fused where the autotuner timed it faster than the colouring kernel +
 // six z passes for this geometry; untuned, where its tile has at least 8 columns (64-byte row
 // segments; narrower tiles, at the longest lengths, read the covariance too finely).
-static const FixedEntry *fixed_colour2_entry(cg_ctx *ctx, int nzp, long plane)
+const FixedEntry *fixed_colour2_entry(cg_ctx *ctx, int nzp, long plane)
 {
   const FixedEntry *e = fixed_find(FIX_STRIDED, nzp);
   if (!e || !e->cz2 || ctx->fused_colour == 0 || ctx->fft_engine != CG_FFT_AUTO || nzp < 2) return nullptr;
@@ -203,9 +203,23 @@ int fixed_colour_zinv2(cg_ctx *ctx, const ColourParams &p, int nzp, long plane, 
   }
   hipEvent_t ev;
   launch_begin(ctx, CG_K_NOISE, &ev);
-  *rc = e->cz2(ctx, p, plane, tw);
+  *rc = e->cz2(ctx, &p, nullptr, 0, (unsigned)((plane + e->cz2_tile - 1) / e->cz2_tile), tw);
   launch_end(ctx, CG_K_NOISE, ev);
   return 1;
+}
+
+int fixed_colour_zinv2_list(cg_ctx *ctx, const FixedEntry *e, const ColourItem *dev_items, int nitems,
+                            unsigned nblocks)
+{
+  if (nitems <= 0 || nblocks == 0) return CG_OK;
+  if (!e->cz2) return fail(ctx, CG_EINVAL, "paired colouring: length without a paired kernel");
+  const float2 *tw = twiddle_table(ctx, e->n);
+  if (!tw) return CG_ENOMEM;
+  hipEvent_t ev;
+  launch_begin(ctx, CG_K_NOISE, &ev);
+  const int rc = e->cz2(ctx, nullptr, dev_items, nitems, nblocks, tw);
+  launch_end(ctx, CG_K_NOISE, ev);
+  return rc;
 }
 
 int fixed_colour_zinv_list(cg_ctx *ctx, const FixedEntry *e, const ColourItem *dev_items, int nitems,
@@ -342,7 +356,8 @@ int autotune_geometry(cg_ctx *ctx, int nxp, int nyp, int nzp)
           for (int g = 0; g < 3 && !r; ++g) r = ez->s[v](ctx, +1, &sb[g], nullptr, 0, nblk, tw);
           return r;
         });
-        ms[1] = time_it([&] { return ez->cz2(ctx, p, plane, tw); });
+        const unsigned cz2b = (unsigned)((plane + ez->cz2_tile - 1) / ez->cz2_tile);
+        ms[1] = time_it([&] { return ez->cz2(ctx, &p, nullptr, 0, cz2b, tw); });
         if (ms[0] >= 0.f && ms[1] >= 0.f) {
           std::lock_guard<std::mutex> lock(tune_mu);
           tune_table[colour2_key(ctx, nzp, plane)] = ms[1] < ms[0] ? 1 : 0;

@@ -71,7 +71,8 @@ struct ColourParams;
 struct ColourItem;
 typedef int (*ColourZFn)(cg_ctx *, const ColourParams *one, const ColourItem *items, int nitems, unsigned nblocks,
                          const float2 *tw);
-typedef int (*ColourZ2Fn)(cg_ctx *, const ColourParams &p, long plane, const float2 *tw);
+typedef int (*ColourZ2Fn)(cg_ctx *, const ColourParams *one, const ColourItem *items, int nitems, unsigned nblocks,
+                          const float2 *tw);
 struct FixedEntry {
   int kind, n;
   int tile[2];  // the two tile variants (equal when the length has only one)
@@ -198,6 +199,11 @@ int autotune_geometry(cg_ctx *ctx, int nxp, int nyp, int nzp);
 int fixed_colour_zinv(cg_ctx *ctx, const ColourParams &p, int nzp, long plane, int *rc);
 // The same for two realisations of one posterior (p.out2 / p.stream1 set): one launch.
 int fixed_colour_zinv2(cg_ctx *ctx, const ColourParams &p, int nzp, long plane, int *rc);
+// The paired kernel's entry for an nzp x plane geometry (null where pairs are not fused), and a
+// list launch of pairs (even first realisations) of that entry's length.
+const FixedEntry *fixed_colour2_entry(cg_ctx *ctx, int nzp, long plane);
+int fixed_colour_zinv2_list(cg_ctx *ctx, const FixedEntry *e, const ColourItem *dev_items, int nitems,
+                            unsigned nblocks);
 // The compiled entry whose fused colouring a realisation of nzp planes of `plane` cells would use
 // (same choice as fixed_colour_zinv), or null.
 const FixedEntry *fixed_colour_entry(cg_ctx *ctx, int nzp, long plane);

@@ -138,10 +138,11 @@ def test_batch_fused_colouring_matches_unfused_and_oracle(ctx):
     """Realisations of a batch with the colouring fused into the inverse z pass (one launch per z
     length over every inversion of that length) and with the colouring kernel then z passes: both
     match the oracle, and the fused schedule really skips the z passes it absorbed.  The mix holds
-    two z lengths of one plan each plus the single-cube depth (256 of 320 planes)."""
+    two z lengths of one plan each plus the single-cube depth (256 of 320 planes); with three
+    realisations the fused schedule colours the first two in one paired launch per z length."""
     probs = [P.make_problem(**MIX[0]), P.make_problem(**MIX[1]), P.make_problem(11, 7, 256, nzp=320, seed=5),
              P.make_problem(9, 6, 256, nzp=320, seed=6)]
-    nreal = 2
+    nreal = 3  # fused: realisations 0 and 1 by the paired kernel, 2 alone
     items = [_item(ctx, p, nreal, 60 + i) for i, p in enumerate(probs)]
     res, zlaunches = {}, {}
     for fused in (0, 1):

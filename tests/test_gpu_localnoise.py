@@ -27,7 +27,7 @@ def _case(nt, seed, nx, ny):
     return sm, post, e, scales, rng
 
 
-@pytest.mark.parametrize("nt,seed", [(3, 1), (5, 2), (8, 3)])
+@pytest.mark.parametrize("nt,seed", [(1, 6), (2, 7), (3, 1), (5, 2), (8, 3)])
 def test_trace_matrices_match_oracle(ctx, nt, seed):
     nx, ny = 13, 9
     sm, post, e, scales, _ = _case(nt, seed, nx, ny)
@@ -37,9 +37,14 @@ def test_trace_matrices_match_oracle(ctx, nt, seed):
     sn, kept = ln.matrices()
     ln.close()
     wn, wk = O.local_noise_matrices(G, scales, sm, post, e)
-    assert np.abs(sn - wn).max() < 1e-11 * np.abs(wn).max()
+    # with fewer than three angles Sigma_m G^T M^+ G Sigma_m has rank nt < 3: its square root
+    # takes sqrt of eigenvalues at rounding level (~1e-17), which any two eigen solvers place
+    # differently (~1e-8 in sigmanew)
+    tol = 1e-11 if nt >= 3 else 1e-7
+    assert np.abs(sn - wn).max() < tol * np.abs(wn).max()
     assert np.abs(kept - wk).max() < 1e-11 * np.abs(wk).max()
-    assert np.abs(sn[ny // 2, nx // 3] - np.eye(3)).max() < 1e-9  # the trace at every minimum
+    if nt >= 3:  # G reproduces the lag-0 posterior only with three or more angles
+        assert np.abs(sn[ny // 2, nx // 3] - np.eye(3)).max() < 1e-9  # the trace at every minimum
 
 
 def test_posterior_and_realisation_corrections_match_oracle(ctx):

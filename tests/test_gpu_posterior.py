@@ -44,6 +44,7 @@ CASES = [
          wavelet_scales=[0.9, 1.1, 1.0, 0.85, 1.15], sn_per_angle=[3.5, 7.0, 5.0, 4.0, 6.5]),
     dict(nx=12, ny=12, nz=24, seed=3, angles_deg=(0.0, 8.0, 16.0, 24.0, 32.0, 40.0, 48.0, 56.0)),
     dict(nx=9, ny=7, nz=25, seed=4, angles_deg=(0.0, 30.0)),
+    dict(nx=10, ny=6, nz=18, seed=12, angles_deg=(20.0,)),  # one angle: 1 x 1 systems
 ]
 
 
@@ -57,8 +58,10 @@ def test_separable_prior_matches_oracle_f32(ctx, case):
     gm, gd, gc, (ninv, nfail, n64) = _run_gpu(ctx, p, mean, data)
     assert ninv == int(p["invert"].sum()) * p["nyp"] * p["cnxp"]
     assert nfail == onf
+    e_upd = rel_l2(gm - mean, om - mean)
+    print("update vs the reference-order FP32 oracle %.2e" % e_upd, flush=True)
     assert rel_l2(gm, om) < 1e-5
-    assert rel_l2(gm - mean, om - mean) < 2e-4  # the update itself
+    assert e_upd < 1e-5  # the update itself (the north star asks for 1e-4)
     assert rel_l2(gc, oc) < 1e-5
     assert rel_l2(gd, od) < 1e-5
     # pass-through planes are untouched / the prior
@@ -85,7 +88,8 @@ def test_mixed_precision_matches_fp64_oracle(ctx):
     om, od, oc, _, _ = O.posterior(p, mean.astype(np.complex128), data.astype(np.complex128), None, "f64")
     gm, gd, gc, (ninv, nfail, n64) = _run_gpu(ctx, p, mean, data, precision=C.PREC_MIXED)
     assert nfail == 0
-    assert rel_l2(gm - mean, om - mean) < 1e-4
+    print("mixed update vs FP64 %.2e" % rel_l2(gm - mean, om - mean), flush=True)
+    assert rel_l2(gm - mean, om - mean) < 1e-5
     assert rel_l2(gc, oc) < 1e-5
     assert rel_l2(gd, od) < 1e-4
     assert n64 <= ninv
