@@ -93,6 +93,14 @@ int main(void)
   CHECK(cg_batch_create(NULL, 0, NULL, NULL) == CG_EINVAL);
   CHECK(cg_batch_run(NULL) == CG_EINVAL);
   {
+    cg_local_noise *ln = (cg_local_noise *)1;
+    CHECK(cg_local_noise_create(NULL, NULL, &ln) == CG_EINVAL && ln == (cg_local_noise *)1);
+    CHECK(cg_local_noise_matrices(NULL, NULL, NULL) == CG_EINVAL);
+    CHECK(cg_local_noise_correct(NULL, NULL, NULL) == CG_EINVAL);
+    CHECK(cg_local_noise_correct_realisation(NULL, NULL, NULL) == CG_EINVAL);
+    cg_local_noise_destroy(NULL); /* a no-op */
+  }
+  {
     int dims[6];
     CHECK(cg_grid_dims(NULL, dims) == CG_EINVAL);
   }
