@@ -984,10 +984,12 @@ k_colour_zinv2(cg::ColourParams one, const cg::ColourItem *__restrict__ items, i
 // LDS and their inverse z transforms while the second's coloured values wait in registers, then
 // take the same LDS and transforms.  Twice the columns per workgroup at the same LDS (wider row
 // segments of the covariance reads), the same cells and arithmetic (bit for bit).
-// (At least four waves per SIMD: a 16-column tile is 960 threads, one workgroup per CU, which
-// needs at most 128 registers per lane.)
+// (Register budget: the workgroup's waves spread over the CU's four SIMDs -- a 16-column tile at
+// nzp 320 is 960 threads, four waves per SIMD, at most 128 registers per lane.  A 12-column tile
+// (three waves per SIMD) fits without spilling but reads 96-byte row segments that straddle
+// cache lines: 2.18 ms per pair against 1.72 ms.)
 template <int N, int TI, bool THREE, bool LIST>
-__global__ void __launch_bounds__(3 * TI *Plan<N>::TPS) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ void __launch_bounds__(3 * TI *Plan<N>::TPS) __attribute__((amdgpu_waves_per_eu((3 * TI * Plan<N>::TPS + 255) / 256)))
 k_colour_zinv2r(cg::ColourParams one, const cg::ColourItem *__restrict__ items, int nitems,
                 const float2 *__restrict__ twg)
 {
