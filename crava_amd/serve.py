@@ -188,7 +188,8 @@ class _Worker:
         from . import Context
         self.ctx = Context(device)
         self.q = queue.Queue()
-        self.pending = 0
+        self.pending = 0  # tasks submitted and not finished (for choosing the least busy worker)
+        self._lock = threading.Lock()
         self.t = threading.Thread(target=self._run, daemon=True)
         self.t.start()
 
@@ -203,12 +204,14 @@ class _Worker:
                     fut.set_result(fn(self.ctx))
                 except BaseException as e:  # reported to the caller through the future
                     fut.set_exception(e)
-            self.pending -= 1
+            with self._lock:
+                self.pending -= 1
 
     def submit(self, fn):
         from concurrent.futures import Future
         fut = Future()
-        self.pending += 1
+        with self._lock:
+            self.pending += 1
         self.q.put((fn, fut))
         return fut
 

@@ -11,12 +11,29 @@
  */
 #include <math.h>
 #include <stdint.h>
+
+#include <mutex>
 #include <stdlib.h>
 #include <string.h>
 #include "fftw.h"
 #include "rfftw.h"
 #include "lib/lib_matr.h"
 #include "lib/random.h"
+
+/* FFTW 2's planner keeps process-wide state (the twiddle cache): plans are created and destroyed
+ * under one lock, so that timings on several threads at once are safe; executing a plan is
+ * reentrant. */
+static std::mutex plan_mu;
+static rfftwnd_plan make_plan(int nzp, int nyp, int nxp, fftw_direction dir)
+{
+  std::lock_guard<std::mutex> lock(plan_mu);
+  return rfftw3d_create_plan(nzp, nyp, nxp, dir, FFTW_ESTIMATE | FFTW_IN_PLACE);
+}
+static void drop_plan(rfftwnd_plan p)
+{
+  std::lock_guard<std::mutex> lock(plan_mu);
+  rfftwnd_destroy_plan(p);
+}
 
 extern "C" {
 
@@ -28,9 +45,9 @@ void ref_grid_fft(int nxp, int nyp, int nzp, int cubetype, float *rvalue)
     float s = 1.0f / sqrtf((float)((size_t)nxp * nyp * nzp));
     for (size_t i = 0; i < rsize; i++) rvalue[i] *= s;
   }
-  rfftwnd_plan plan = rfftw3d_create_plan(nzp, nyp, nxp, FFTW_REAL_TO_COMPLEX, FFTW_ESTIMATE | FFTW_IN_PLACE);
+  rfftwnd_plan plan = make_plan(nzp, nyp, nxp, FFTW_REAL_TO_COMPLEX);
   rfftwnd_one_real_to_complex(plan, rvalue, (fftw_complex *)rvalue);
-  rfftwnd_destroy_plan(plan);
+  drop_plan(plan);
 }
 
 /* FFTGrid::invFFTInPlace, fftgrid.cpp:1615-1645 */
@@ -39,9 +56,9 @@ void ref_grid_ifft(int nxp, int nyp, int nzp, int cubetype, float *rvalue)
   size_t n = (size_t)nxp * nyp * nzp;
   size_t rsize = (size_t)2 * (nxp / 2 + 1) * nyp * nzp;
   float scale = (cubetype == 3) ? float(1.0 / n) : float(1.0 / sqrtf(float(n)));
-  rfftwnd_plan plan = rfftw3d_create_plan(nzp, nyp, nxp, FFTW_COMPLEX_TO_REAL, FFTW_ESTIMATE | FFTW_IN_PLACE);
+  rfftwnd_plan plan = make_plan(nzp, nyp, nxp, FFTW_COMPLEX_TO_REAL);
   rfftwnd_one_complex_to_real(plan, (fftw_complex *)rvalue, rvalue);
-  rfftwnd_destroy_plan(plan);
+  drop_plan(plan);
   for (size_t i = 0; i < rsize; i++) rvalue[i] *= scale;
 }
 
