@@ -122,6 +122,7 @@ def lib():
             "cg_grid_scale_wavelet": [_vp, ctypes.POINTER(ScaleWaveletArgs)],
             "cg_grid_write_storm": [_vp, ctypes.c_char_p, ctypes.POINTER(StormGeometry), i],
             "cg_grid_read_storm": [_vp, ctypes.c_char_p, ctypes.POINTER(StormGeometry)],
+            "cg_grid_write_segy": [_vp, ctypes.c_char_p, ctypes.POINTER(SegyGeometry)],
             "cg_grid_read_crava": [_vp, ctypes.c_char_p, ctypes.POINTER(CravaGeometry)],
             "cg_local_noise_create": [_vp, ctypes.POINTER(LocalNoiseArgs), ctypes.POINTER(_vp)],
             "cg_local_noise_matrices": [_vp, _d_p, _d_p],
@@ -159,6 +160,17 @@ class LocalNoiseArgs(ctypes.Structure):
                 ("G", ctypes.POINTER(ctypes.c_double)), ("prior_var0", ctypes.POINTER(ctypes.c_double)),
                 ("post_var0", ctypes.POINTER(ctypes.c_double)), ("err_theta_cov", ctypes.POINTER(ctypes.c_double)),
                 ("noise_scale", ctypes.POINTER(ctypes.POINTER(ctypes.c_double)))]
+
+
+class SegyGeometry(ctypes.Structure):
+    """cg_segy_geometry: the simbox geometry FFTGrid::writeSegyFile reads."""
+    _fields_ = [("x0", ctypes.c_double), ("y0", ctypes.c_double), ("dx", ctypes.c_double), ("dy", ctypes.c_double),
+                ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("angle", ctypes.c_double),
+                ("il0", ctypes.c_double), ("xl0", ctypes.c_double), ("il_step_x", ctypes.c_double),
+                ("il_step_y", ctypes.c_double), ("xl_step_x", ctypes.c_double), ("xl_step_y", ctypes.c_double),
+                ("dz", ctypes.c_double), ("z0", ctypes.c_float), ("zmax", ctypes.c_double),
+                ("top", ctypes.POINTER(ctypes.c_double)), ("rel_thick", ctypes.POINTER(ctypes.c_double)),
+                ("text", ctypes.POINTER(ctypes.c_char_p)), ("ntext", ctypes.c_int)]
 
 
 class CravaGeometry(ctypes.Structure):
@@ -457,6 +469,20 @@ class FFTGrid:
                           (geometry.get("bot_name") or "").encode() or None)
         self.ctx.check(lib().cg_grid_write_storm(self._h, (file_name + suffix).encode(), ctypes.byref(g),
                                                  1 if padding else 0), "writeStormFile")
+
+    def writeSegyFile(self, file_name, geometry):
+        """FFTGrid::writeSegyFile (fftgrid.cpp:1960-2050) to file_name as given; geometry: a dict of
+        x0, y0, dx, dy, angle, il0, xl0, il_step_x, il_step_y, xl_step_x, xl_step_y, dz, z0, zmax,
+        top (ny, nx), rel_thick (ny, nx) and optionally text (header lines from the second on)."""
+        g = geometry
+        top = np.ascontiguousarray(g["top"], dtype=np.float64)
+        rel = np.ascontiguousarray(g["rel_thick"], dtype=np.float64)
+        text = [t.encode() for t in g.get("text", [])]
+        tarr = (ctypes.c_char_p * max(len(text), 1))(*text)
+        geo = SegyGeometry(g["x0"], g["y0"], g["dx"], g["dy"], self.nx, self.ny, g["angle"], g["il0"], g["xl0"],
+                           g["il_step_x"], g["il_step_y"], g["xl_step_x"], g["xl_step_y"], g["dz"], float(g["z0"]),
+                           g["zmax"], _ptr(top, ctypes.c_double), _ptr(rel, ctypes.c_double), tarr, len(text))
+        self.ctx.check(lib().cg_grid_write_segy(self._h, file_name.encode(), ctypes.byref(geo)), "writeSegyFile")
 
     def readStormFile(self, file_name):
         """A STORM binary grid of this grid's box size, then the padding fill of

@@ -112,6 +112,126 @@ struct File {
   }
 };
 
+// ---------------------------------------------------------------- SEG-Y
+// One output trace of FFTGrid::writeSegyFile: grid column (i, j), the samples [first, end) taken
+// from the grid (zero elsewhere), the trace's top time and cell thickness.
+struct SegyTrace {
+  int i, j, first, end;
+  double top, gdz;
+};
+
+// IEEE single -> IBM single (hexadecimal) float by its definition, truncating the fraction:
+// value = (-1)^s 0.F 16^(E - 64) with a 24-bit F in [1/16, 1).  Both zeros give 0.  (Normal floats
+// only are exact: an IEEE denormal becomes an unnormalised IBM fraction.)
+__host__ __device__ inline unsigned ieee_to_ibm(float f)
+{
+  const unsigned u = __builtin_bit_cast(unsigned, f);
+  if ((u & 0x7fffffffu) == 0) return 0u;
+  const unsigned e = (u >> 23) & 0xffu;
+  const unsigned mant = e ? ((u & 0x7fffffu) | 0x800000u) : (u & 0x7fffffu);  // f = mant 2^(p - 24)
+  const int p = (e ? (int)e : 1) - 126;
+  const int q = (p + 3) >> 2;            // ceil(p / 4): f = (mant >> (4q - p)) 2^-24 16^q
+  const unsigned frac = mant >> (unsigned)(4 * q - p);
+  return (u & 0x80000000u) | ((unsigned)(q + 64) << 24) | frac;
+}
+
+// Samples of the sorted output traces: FFTGrid::getRegularZInterpolatedRealValue
+// (fftgrid.cpp:2275-2306) for k in [first, end), with its float/double roundings, the truncation
+// of the position to an index and the clamps at both ends; getRealValue's RMISSING outside the
+// box (fftgrid.cpp:1237-1262).  Written as big-endian IBM floats, trace after trace.
+__global__ void __launch_bounds__(cg::kThreads)
+k_segy_traces(const float *__restrict__ g, int rnxp, int nyp, int nz, const SegyTrace *__restrict__ tr, int ns,
+              double z0, double dz, unsigned *__restrict__ out)
+{
+#pragma clang fp contract(off)
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= ns) return;
+  const SegyTrace t = tr[blockIdx.y];
+  float v = 0.f;
+  if (k >= t.first && k < t.end) {
+    const float z = (float)(z0 + dz * (double)k);
+    float tt = (float)(((double)z - t.top) / t.gdz);
+    int kg = (int)tt;
+    tt -= (float)kg;
+    bool missing = false;
+    if (kg < 0) {
+      if (kg < -1) missing = true;
+      else {
+        kg = 0;
+        tt = 0.f;
+      }
+    } else if (kg > nz - 2) {
+      if (kg > nz - 1) missing = true;
+      else {
+        kg = nz - 2;
+        tt = 1.f;
+      }
+    }
+    if (missing) {
+      v = -99999.f;
+    } else {
+      const size_t col = (size_t)t.i + (size_t)rnxp * (size_t)t.j, slab = (size_t)rnxp * nyp;
+      const float v1 = kg >= 0 && kg < nz ? g[col + slab * kg] : -99999.f;
+      const float v2 = kg + 1 >= 0 && kg + 1 < nz ? g[col + slab * (kg + 1)] : -99999.f;
+      v = (1.f - tt) * v1 + tt * v2;
+    }
+  }
+  out[(size_t)blockIdx.y * ns + k] = __builtin_bswap32(ieee_to_ibm(v));
+}
+
+void put_be16(unsigned char *b, int v)
+{
+  b[0] = (unsigned char)((v >> 8) & 0xff);
+  b[1] = (unsigned char)(v & 0xff);
+}
+
+void put_be32(unsigned char *b, int v)
+{
+  const unsigned u = (unsigned)v;
+  b[0] = (unsigned char)(u >> 24);
+  b[1] = (unsigned char)(u >> 16);
+  b[2] = (unsigned char)(u >> 8);
+  b[3] = (unsigned char)u;
+}
+
+// ASCII -> EBCDIC (code page 037) for the printable characters a textual header uses; others
+// become EBCDIC spaces.
+unsigned char ebcdic(char c)
+{
+  static const char *lower = "abcdefghijklmnopqrstuvwxyz", *upper = "ABCDEFGHIJKLMNOPQRSTUVWXYZ";
+  static const unsigned char lo[26] = {0x81, 0x82, 0x83, 0x84, 0x85, 0x86, 0x87, 0x88, 0x89, 0x91, 0x92, 0x93, 0x94,
+                                       0x95, 0x96, 0x97, 0x98, 0x99, 0xa2, 0xa3, 0xa4, 0xa5, 0xa6, 0xa7, 0xa8, 0xa9};
+  for (int i = 0; i < 26; ++i) {
+    if (c == lower[i]) return lo[i];
+    if (c == upper[i]) return (unsigned char)(lo[i] + 0x40);
+  }
+  if (c >= '0' && c <= '9') return (unsigned char)(0xf0 + (c - '0'));
+  switch (c) {
+    case '.': return 0x4b;
+    case '<': return 0x4c;
+    case '(': return 0x4d;
+    case '+': return 0x4e;
+    case '&': return 0x50;
+    case '*': return 0x5c;
+    case ')': return 0x5d;
+    case ';': return 0x5e;
+    case '-': return 0x60;
+    case '/': return 0x61;
+    case ',': return 0x6b;
+    case '%': return 0x6c;
+    case '_': return 0x6d;
+    case '>': return 0x6e;
+    case '?': return 0x6f;
+    case ':': return 0x7a;
+    case '#': return 0x7b;
+    case '@': return 0x7c;
+    case '\'': return 0x7d;
+    case '=': return 0x7e;
+    case '"': return 0x7f;
+  }
+  return 0x40;
+}
+
 }  // namespace cgio
 
 using namespace cgio;
@@ -350,6 +470,138 @@ int cg_grid_read_storm(cg_grid *g, const char *path, cg_storm_geometry *geo_out)
     geo_out->top_name = nullptr;
     geo_out->bot_name = nullptr;
   }
+  return rc;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int cg_grid_write_segy(cg_grid *g, const char *path, const cg_segy_geometry *geo)
+{
+  if (!g || !path || !geo) return CG_EINVAL;
+  cg_ctx *ctx = g->ctx;
+  if (g->transformed) return cg::fail(ctx, CG_ESTATE, "write_segy: grid is in the Fourier domain");
+  if (geo->nx != g->nx || geo->ny != g->ny || !geo->top || !geo->rel_thick || geo->ntext < 0 || geo->ntext > 38 ||
+      (geo->ntext > 0 && !geo->text))
+    return cg::fail(ctx, CG_EINVAL, "write_segy: geometry of another box, or missing tables");
+  (void)hipSetDevice(ctx->device);
+  // FFTGrid::writeSegyFile (fftgrid.cpp:1960-2050): sample interval rounded to whole ms, enough
+  // samples from z0 to the simbox's bottom
+  float dzf = (float)std::floor(geo->dz + 0.5);
+  if (dzf == 0.f) dzf = 1.f;
+  const int ns = (int)std::ceil((geo->zmax - (double)geo->z0) / (double)dzf);
+  if (ns < 1 || ns > 32767) return cg::fail(ctx, CG_EINVAL, "write_segy: sample count outside 1..32767");
+  // traces: cell centres (Simbox::getXYCoord), inline / crossline of the float coordinates
+  // (SegyGeometry::FindILXL), sorted by inline then crossline as SegY::WriteAllTracesToFile does;
+  // traces with a missing top are not written
+  struct Out {
+    SegyTrace t;
+    int il, xl, ux, uy;
+  };
+  std::vector<Out> tr;
+  tr.reserve((size_t)g->nx * g->ny);
+  const double cr = std::cos(geo->angle), sr = std::sin(geo->angle);
+  for (int j = 0; j < g->ny; ++j)
+    for (int i = 0; i < g->nx; ++i) {
+      const double top = geo->top[(size_t)j * g->nx + i];
+      if (top == -99999.0 || top == -88888.0) continue;  // RMISSING, WELLMISSING
+      const double rx = ((double)i + 0.5) * geo->dx, ry = ((double)j + 0.5) * geo->dy;
+      const double x = rx * cr - ry * sr + geo->x0, y = rx * sr + ry * cr + geo->y0;
+      const float xf = (float)x, yf = (float)y;
+      Out o;
+      o.t.i = i;
+      o.t.j = j;
+      o.t.top = top;
+      o.t.gdz = geo->dz * geo->rel_thick[(size_t)j * g->nx + i];
+      o.t.first = (int)std::floor(0.5 + (top - (double)geo->z0) / (double)dzf);
+      o.t.end = std::min(ns, (int)std::floor(0.5 + ((top - (double)geo->z0) + g->nz * o.t.gdz) / (double)dzf));
+      o.il = (int)(0.5 + geo->il0 + ((double)xf - geo->x0) * geo->il_step_x + ((double)yf - geo->y0) * geo->il_step_y);
+      o.xl = (int)(0.5 + geo->xl0 + ((double)xf - geo->x0) * geo->xl_step_x + ((double)yf - geo->y0) * geo->xl_step_y);
+      o.ux = (int)(double)xf;  // TraceHeader::SetUtmx with scalco 1, written as int
+      o.uy = (int)(double)yf;
+      tr.push_back(o);
+    }
+  std::stable_sort(tr.begin(), tr.end(),
+                   [](const Out &a, const Out &b) { return a.il < b.il || (a.il == b.il && a.xl < b.xl); });
+  File file;
+  file.f = fopen(path, "wb");
+  if (!file.f) return cg::fail(ctx, CG_EIO, std::string("write_segy: cannot open '") + path + "'");
+  // textual header: 40 lines of 80 EBCDIC characters, "C" + line number, the caller's lines from
+  // the second line on, 75 characters each (TextualHeader::SetLine(i + 1, text), as
+  // writeSegyFile places them)
+  std::vector<unsigned char> head(3600, 0);
+  for (int l = 0; l < 40; ++l) {
+    char line[96];
+    const char *text = l >= 1 && l - 1 < geo->ntext && geo->text[l - 1] ? geo->text[l - 1] : (l == 39 ? "END EBCDIC" : "");
+    snprintf(line, sizeof(line), "C%2d  %-75.75s", l + 1, text);
+    for (int c = 0; c < 80; ++c) head[80 * l + c] = ebcdic(line[c]);
+  }
+  // binary header (BinaryHeader::Write, commonheaders.cpp:234-270)
+  unsigned char *bh = head.data() + 3200;
+  put_be32(bh + 0, 9999);
+  put_be32(bh + 4, 1);
+  put_be32(bh + 8, 1);
+  put_be16(bh + 12, (short)g->nx);
+  put_be16(bh + 16, (short)(1000 * dzf));
+  put_be16(bh + 20, (short)ns);
+  put_be16(bh + 24, 1);  // IBM floats
+  put_be16(bh + 26, 1);
+  put_be16(bh + 28, 4);
+  put_be16(bh + 54, 1);
+  put_be16(bh + 302, 1);  // fixed trace length
+  if (fwrite(head.data(), 1, head.size(), file.f) != head.size())
+    return cg::fail(ctx, CG_EIO, std::string("write_segy: cannot write '") + path + "'");
+  if (tr.empty()) return fflush(file.f) == 0 ? CG_OK : cg::fail(ctx, CG_EIO, "write_segy: flush");
+  // samples on the device, traces in chunks of at most kChunk bytes
+  const size_t per = (size_t)ns * 4;
+  const size_t chunk = std::max<size_t>(1, std::min(tr.size(), kChunk / per));
+  void *dtr = nullptr, *dout = nullptr;
+  hipError_t e = cg::dev_alloc(ctx, &dtr, sizeof(SegyTrace) * chunk);
+  if (e == hipSuccess) e = cg::dev_alloc(ctx, &dout, per * chunk);
+  int rc = cg::hip_fail(ctx, e, "write_segy: staging buffers");
+  Pinned host;
+  if (rc == CG_OK) rc = cg::hip_fail(ctx, hipHostMalloc(&host.p, sizeof(SegyTrace) * chunk + per * chunk, hipHostMallocDefault),
+                                     "write_segy: pinned buffer");
+  std::vector<unsigned char> th(240);
+  for (size_t c0 = 0; rc == CG_OK && c0 < tr.size(); c0 += chunk) {
+    const size_t m = std::min(chunk, tr.size() - c0);
+    SegyTrace *ht = static_cast<SegyTrace *>(host.p);
+    unsigned char *hs = static_cast<unsigned char *>(host.p) + sizeof(SegyTrace) * chunk;
+    for (size_t t = 0; t < m; ++t) ht[t] = tr[c0 + t].t;
+    e = hipMemcpyAsync(dtr, ht, sizeof(SegyTrace) * m, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) {
+      hipEvent_t ev;
+      cg::launch_begin(ctx, CG_K_ELEMENTWISE, &ev);
+      k_segy_traces<<<dim3((unsigned)((ns + cg::kThreads - 1) / cg::kThreads), (unsigned)m), cg::kThreads, 0,
+                      ctx->stream>>>(g->d, g->rnxp, g->nyp, g->nz, static_cast<const SegyTrace *>(dtr), ns,
+                                     (double)geo->z0, (double)dzf, static_cast<unsigned *>(dout));
+      cg::launch_end(ctx, CG_K_ELEMENTWISE, ev);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(hs, dout, per * m, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    rc = cg::hip_fail(ctx, e, "write_segy: samples");
+    for (size_t t = 0; rc == CG_OK && t < m; ++t) {
+      // trace header (TraceHeader::Write, SeisWorks positions, traceheader.cpp:527-586)
+      const Out &o = tr[c0 + t];
+      std::fill(th.begin(), th.end(), 0);
+      put_be32(&th[8], o.il);
+      put_be32(&th[20], o.xl);
+      put_be32(&th[36], 0);  // offset
+      put_be16(&th[70], 1);  // scalco
+      put_be32(&th[72], o.ux);
+      put_be32(&th[76], o.uy);
+      put_be16(&th[108], (short)geo->z0);
+      put_be16(&th[114], ns);
+      put_be16(&th[116], (unsigned short)(dzf * 1000));
+      if (fwrite(th.data(), 1, 240, file.f) != 240 || fwrite(hs + per * t, 1, per, file.f) != per)
+        rc = cg::fail(ctx, CG_EIO, std::string("write_segy: cannot write '") + path + "'");
+    }
+  }
+  if (dtr) cg::dev_free(ctx, dtr);
+  if (dout) cg::dev_free(ctx, dout);
+  if (rc == CG_OK && fflush(file.f) != 0) rc = cg::fail(ctx, CG_EIO, "write_segy: flush");
   return rc;
 }
 

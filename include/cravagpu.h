@@ -429,6 +429,31 @@ int cg_grid_sum_squares(cg_grid *grid, int ext_simbox, double *sum);
  * synchronises. */
 int cg_grid_data_variance(cg_grid *grid, float *variance, uint64_t *count);
 
+/* ------------------------------------------------------------------ SEG-Y grid files */
+/* The simbox geometry FFTGrid::writeSegyFile reads (fftgrid.cpp:1960-2050). */
+typedef struct {
+  double x0, y0, dx, dy;      /* Simbox::getx0, gety0, getdx, getdy */
+  int nx, ny;                 /* the box's traces (the grid's nx, ny) */
+  double angle;               /* Simbox::getAngle, radians */
+  double il0, xl0, il_step_x, il_step_y, xl_step_x, xl_step_y;  /* getIL0 ... getXLStepY */
+  double dz;                  /* Simbox::getdz, ms */
+  float z0;                   /* time of the file's first sample */
+  double zmax;                /* Simbox::getMinMaxZ's maximum */
+  const double *top;          /* [ny][nx] Simbox::getTop at each trace centre; -99999 (missing): trace not written */
+  const double *rel_thick;    /* [ny][nx] Simbox::getRelThick(i, j) */
+  const char *const *text;    /* textual header lines (from the second line on), or null */
+  int ntext;                  /* 0..38 */
+} cg_segy_geometry;
+/* FFTGrid::writeSegyFile (fftgrid.cpp:1960-2050) with SegY's writer (libs/nrlib/segy: BinaryHeader,
+ * SeisWorks trace headers, SegY::WriteAllTracesToFile with scalco 1 and offset 0), to `path` as
+ * given: sample interval floor(dz + 0.5) ms (1 if 0), ceil((zmax - z0) / that) samples per trace;
+ * trace (i, j) holds, from the sample nearest its top to the one nearest its bottom, the grid's
+ * trace interpolated linearly at the sample times (getRegularZInterpolatedRealValue) and 0 above
+ * and below; traces sorted by inline then crossline (of the float cell-centre coordinates), big-endian
+ * IBM floats (truncated).  The 3200-byte textual header is "C" + line number + the given text, in
+ * EBCDIC (not the reference's standard header text).  Real domain; samples computed on the device. */
+int cg_grid_write_segy(cg_grid *grid, const char *path, const cg_segy_geometry *geometry);
+
 /* ------------------------------------------------------------------ local noise */
 /* Laterally varying data noise (AVOInversion::correctVpVsRho, avoinversion.cpp:2146-2302): per
  * trace (i, j) the 3 x 3 matrices of newPosteriorCovPointwise (:2028-2124) from the angle scales

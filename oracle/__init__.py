@@ -668,6 +668,109 @@ def read_storm(path):
     return toks, cube
 
 
+def ieee_to_ibm(x):
+    """IBM single (hexadecimal) floats of float32 values by the definition, fraction truncated:
+    (-1)^s 0.F 16^(E - 64), F a 24-bit fraction in [1/16, 1); both zeros give 0 (as the writer of
+    libs/nrlib/iotools/fileio.hpp:1127-1139 does for normal floats)."""
+    u = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32).astype(np.int64)
+    e = (u >> 23) & 0xFF
+    mant = np.where(e > 0, (u & 0x7FFFFF) | 0x800000, u & 0x7FFFFF)
+    p = np.where(e > 0, e, 1) - 126
+    q = (p + 3) >> 2
+    frac = mant >> (4 * q - p)
+    out = (u & 0x80000000) | ((q + 64) << 24) | frac
+    return np.where((u & 0x7FFFFFFF) == 0, 0, out).astype(np.uint32)
+
+
+def ibm_to_float(b):
+    """Values (float64) of IBM single floats given as uint32."""
+    b = np.asarray(b, dtype=np.uint64)
+    sign = np.where(b & 0x80000000, -1.0, 1.0)
+    expo = ((b >> 24) & 0x7F).astype(np.int64) - 64
+    frac = (b & 0xFFFFFF).astype(np.float64) / float(1 << 24)
+    return sign * frac * np.power(16.0, expo)
+
+
+def segy_write(real, nx, ny, nz, geo):
+    """FFTGrid::writeSegyFile (fftgrid.cpp:1960-2050) with NRLib's SegY writer for a real padded grid
+    (nzp, nyp, rnxp): the file as bytes.  geo: x0, y0, dx, dy, angle, il0, xl0, il_step_x,
+    il_step_y, xl_step_x, xl_step_y, dz, z0 (float), zmax, top (ny, nx), rel_thick (ny, nx), text
+    (lines from the second on).  Binary header as BinaryHeader::Write (commonheaders.cpp:234-270);
+    trace headers at the SeisWorks positions with scalco 1 (traceheader.cpp:38-47, 527-586);
+    traces sorted by inline then crossline (segy.cpp:1364-1430); samples as
+    getRegularZInterpolatedRealValue (fftgrid.cpp:2275-2306), big-endian IBM floats."""
+    import struct
+    f32 = np.float32
+    grid = np.asarray(real, dtype=np.float32)
+    dzf = f32(math.floor(geo["dz"] + 0.5))
+    if dzf == 0:
+        dzf = f32(1.0)
+    z0 = f32(geo["z0"])
+    ns = int(math.ceil((geo["zmax"] - float(z0)) / float(dzf)))
+    text = list(geo.get("text", []))
+    lines = []
+    for l in range(40):
+        t = text[l - 1] if 1 <= l <= len(text) else ("END EBCDIC" if l == 39 else "")
+        lines.append(("C%2d  %-75.75s" % (l + 1, t))[:80])
+    head = "".join(lines).encode("cp037")
+    bh = bytearray(400)
+    struct.pack_into(">iii", bh, 0, 9999, 1, 1)
+    struct.pack_into(">h", bh, 12, nx)
+    struct.pack_into(">h", bh, 16, int(1000 * dzf))
+    struct.pack_into(">h", bh, 20, ns)
+    struct.pack_into(">hhh", bh, 24, 1, 1, 4)
+    struct.pack_into(">h", bh, 54, 1)
+    struct.pack_into(">h", bh, 302, 1)
+    cr, sr = math.cos(geo["angle"]), math.sin(geo["angle"])
+    traces = []
+    for j in range(ny):
+        for i in range(nx):
+            top = float(geo["top"][j, i])
+            if top in (-99999.0, -88888.0):
+                continue
+            rx, ry = (i + 0.5) * geo["dx"], (j + 0.5) * geo["dy"]
+            xf = float(f32(rx * cr - ry * sr + geo["x0"]))
+            yf = float(f32(rx * sr + ry * cr + geo["y0"]))
+            il = int(0.5 + geo["il0"] + (xf - geo["x0"]) * geo["il_step_x"] + (yf - geo["y0"]) * geo["il_step_y"])
+            xl = int(0.5 + geo["xl0"] + (xf - geo["x0"]) * geo["xl_step_x"] + (yf - geo["y0"]) * geo["xl_step_y"])
+            gdz = geo["dz"] * float(geo["rel_thick"][j, i])
+            first = int(math.floor(0.5 + (top - float(z0)) / float(dzf)))
+            end = min(ns, int(math.floor(0.5 + ((top - float(z0)) + nz * gdz) / float(dzf))))
+            traces.append((il, xl, i, j, top, gdz, first, end, int(xf), int(yf)))
+    traces.sort(key=lambda t: (t[0], t[1]))  # stable: equal pairs keep the scan order
+    out = [head, bytes(bh)]
+    k = np.arange(ns)
+    for il, xl, i, j, top, gdz, first, end, ux, uy in traces:
+        z = (np.float64(z0) + np.float64(dzf) * k).astype(f32)
+        t = ((z.astype(np.float64) - top) / gdz).astype(f32)
+        kg = np.trunc(t).astype(np.int64)
+        t = (t - kg.astype(f32)).astype(f32)
+        miss = (kg < -1) | (kg > nz - 1)
+        lo = (kg < 0) & ~miss
+        hi = (kg > nz - 2) & (kg >= 0) & ~miss
+        kg = np.where(lo, 0, np.where(hi, nz - 2, kg))
+        t = np.where(lo, f32(0), np.where(hi, f32(1), t)).astype(f32)
+        col = grid[:, j, i]
+
+        def at(kk):
+            ok = (kk >= 0) & (kk < nz)
+            return np.where(ok, col[np.clip(kk, 0, max(nz - 1, 0))], f32(-99999.0)).astype(f32)
+
+        v = ((f32(1) - t) * at(kg) + t * at(kg + 1)).astype(f32)
+        v = np.where(miss, f32(-99999.0), v).astype(f32)
+        v = np.where((k >= first) & (k < end), v, f32(0)).astype(f32)
+        th = bytearray(240)
+        struct.pack_into(">i", th, 8, il)
+        struct.pack_into(">i", th, 20, xl)
+        struct.pack_into(">h", th, 70, 1)
+        struct.pack_into(">ii", th, 72, ux, uy)
+        struct.pack_into(">h", th, 108, int(z0))
+        struct.pack_into(">hH", th, 114, ns, int(dzf * 1000))
+        out.append(bytes(th))
+        out.append(ieee_to_ibm(v).astype(">u4").tobytes())
+    return b"".join(out)
+
+
 def fill_param_corr(corr_xy, corr_t, grad_i=0.0, grad_j=0.0, scale=1.0):
     """fillInParamCorr / FillInLateralCorr then multiplyByScalar: corr_xy (nyp, nxp), corr_t (nzp,)
     -> padded real grid (nzp, nyp, rnxp)."""

@@ -89,6 +89,7 @@ int main(void)
   CHECK(cg_grid_scale_wavelet(NULL, NULL) == CG_EINVAL);
   CHECK(cg_grid_write_storm(NULL, NULL, NULL, 0) == CG_EINVAL);
   CHECK(cg_grid_read_storm(NULL, NULL, NULL) == CG_EINVAL);
+  CHECK(cg_grid_write_segy(NULL, NULL, NULL) == CG_EINVAL);
   CHECK(cg_grid_sum_squares(NULL, 0, NULL) == CG_EINVAL);
   CHECK(cg_batch_create(NULL, 0, NULL, NULL) == CG_EINVAL);
   CHECK(cg_batch_run(NULL) == CG_EINVAL);

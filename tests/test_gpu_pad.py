@@ -296,3 +296,28 @@ def test_storm_file_errors(ctx, tmp_path):
         g.readStormFile(str(tmp_path / "crava.storm"))
     other.close()
     g.close()
+
+
+@pytest.mark.parametrize("dims,angle", [((7, 5, 6, 12, 8, 10), 0.3), ((33, 20, 40, 40, 24, 48), 0.0),
+                                        ((64, 48, 40, 80, 60, 48), -0.7)])
+def test_segy_file_matches_oracle(ctx, tmp_path, dims, angle):
+    """writeSegyFile from the device (samples interpolated and converted to IBM floats there) gives
+    the oracle's bytes: headers, inline/crossline order, a missing trace left out, tops and
+    thicknesses varying by trace."""
+    nx, ny, nz, nxp, nyp, nzp = dims
+    print("segy file", dims, angle, flush=True)
+    rng = np.random.default_rng(nx + ny)
+    rv = _seismic(dims, 43)
+    g = _grid(ctx, dims, C.DATA)
+    g.upload_real(rv)
+    geo = dict(x0=431250.5, y0=6472100.25, dx=25.0, dy=20.0, angle=angle, il0=1000.0, xl0=2000.0,
+               il_step_x=-np.sin(angle) / 20.0, il_step_y=np.cos(angle) / 20.0,
+               xl_step_x=np.cos(angle) / 25.0, xl_step_y=np.sin(angle) / 25.0, dz=4.0, z0=np.float32(1996.0),
+               zmax=2000.0 + 4.0 * nz * 1.2 + 40.0, top=2000.0 + rng.uniform(-6, 30, (ny, nx)),
+               rel_thick=rng.uniform(0.8, 1.2, (ny, nx)), text=["cravagpu", "MI355X"])
+    geo["top"][ny // 2, nx // 3] = -99999.0
+    g.writeSegyFile(str(tmp_path / "dev.segy"), geo)
+    want = O.segy_write(rv, nx, ny, nz, geo)
+    got = open(tmp_path / "dev.segy", "rb").read()
+    assert len(got) == len(want) and got == want
+    g.close()
