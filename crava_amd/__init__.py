@@ -477,6 +477,8 @@ class FFTGrid:
         g = geometry
         top = np.ascontiguousarray(g["top"], dtype=np.float64)
         rel = np.ascontiguousarray(g["rel_thick"], dtype=np.float64)
+        if top.shape != (self.ny, self.nx) or rel.shape != (self.ny, self.nx):
+            raise ValueError("writeSegyFile: top and rel_thick must be (ny, nx) = (%d, %d)" % (self.ny, self.nx))
         text = [t.encode() for t in g.get("text", [])]
         tarr = (ctypes.c_char_p * max(len(text), 1))(*text)
         geo = SegyGeometry(g["x0"], g["y0"], g["dx"], g["dy"], self.nx, self.ny, g["angle"], g["il0"], g["xl0"],

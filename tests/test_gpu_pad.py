@@ -320,4 +320,30 @@ def test_segy_file_matches_oracle(ctx, tmp_path, dims, angle):
     want = O.segy_write(rv, nx, ny, nz, geo)
     got = open(tmp_path / "dev.segy", "rb").read()
     assert len(got) == len(want) and got == want
+    # every top missing: the headers alone
+    geo["top"][:] = -99999.0
+    g.writeSegyFile(str(tmp_path / "none.segy"), geo)
+    assert open(tmp_path / "none.segy", "rb").read() == O.segy_write(rv, nx, ny, nz, geo)
+    assert len(O.segy_write(rv, nx, ny, nz, geo)) == 3600
+    g.close()
+
+
+def test_segy_file_one_layer_and_errors(ctx, tmp_path):
+    """A grid of one layer (the interpolation clamps to index -1 and reads RMISSING there, as the
+    reference's getRealValue does) matches the oracle; tables of another box and a file with no
+    samples are refused."""
+    dims = (6, 4, 1, 8, 6, 4)
+    nx, ny, nz = dims[:3]
+    rv = _seismic(dims, 44)
+    g = _grid(ctx, dims, C.DATA)
+    g.upload_real(rv)
+    geo = dict(x0=0.0, y0=0.0, dx=10.0, dy=10.0, angle=0.0, il0=1.0, xl0=1.0, il_step_x=0.0, il_step_y=0.1,
+               xl_step_x=0.1, xl_step_y=0.0, dz=4.0, z0=np.float32(100.0), zmax=120.0,
+               top=np.full((ny, nx), 100.0), rel_thick=np.ones((ny, nx)))
+    g.writeSegyFile(str(tmp_path / "one.segy"), geo)
+    assert open(tmp_path / "one.segy", "rb").read() == O.segy_write(rv, nx, ny, nz, geo)
+    with pytest.raises(ValueError, match="must be"):
+        g.writeSegyFile(str(tmp_path / "bad.segy"), dict(geo, top=np.full((ny + 1, nx), 100.0)))
+    with pytest.raises(C.CravaError, match="sample count"):
+        g.writeSegyFile(str(tmp_path / "bad.segy"), dict(geo, zmax=90.0))
     g.close()
