@@ -50,10 +50,13 @@ class Server:
     batches of up to `max_batch` ready requests.  `clock` returns seconds."""
 
     def __init__(self, executor, hbm_budget=200e9, lanes=3, max_batch=32, ingest_per_step=1,
-                 clock=time.perf_counter):
+                 clock=time.perf_counter, asynchronous_ingest=True):
         if lanes < 1 or max_batch < 1 or ingest_per_step < 1:
             raise ValueError("lanes, max_batch and ingest_per_step must be >= 1")
         self.ingest_per_step = ingest_per_step
+        # ingest off the scheduling loop when the executor can (asynchronous_ingest=False: every
+        # request admitted in a step is ready before that step dispatches)
+        self.asynchronous = asynchronous_ingest and hasattr(executor, "ingest_async")
         self.ex = executor
         self.budget = float(hbm_budget)
         self.in_use = 0.0
@@ -91,7 +94,7 @@ class Server:
         moved = self._complete(on_done)
         moved |= self._ingested()
         moved |= self._dispatch()
-        asynchronous = hasattr(self.ex, "ingest_async")
+        asynchronous = self.asynchronous
         n = 0
         while n < self.ingest_per_step and self.waiting and self.in_use + self.waiting[0].bytes <= self.budget:
             r = self.waiting.popleft()

@@ -346,7 +346,9 @@ def test_served_requests_at_serving_sizes():
 
     def run(group, lanes, max_batch):
         ex = serve.GpuExecutor(lanes=lanes)
-        srv = serve.Server(ex, hbm_budget=200e9, lanes=lanes, max_batch=max_batch, ingest_per_step=len(group))
+        # every request ingested before the first dispatch: the lanes get mixed batches
+        srv = serve.Server(ex, hbm_budget=200e9, lanes=lanes, max_batch=max_batch, ingest_per_step=len(group),
+                           asynchronous_ingest=False)
         got = {}
 
         def on_done(r):

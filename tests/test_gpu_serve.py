@@ -23,7 +23,8 @@ SPECS = [S.spec(20, 16, 24, (0.0, 15.0, 30.0), 25.0, seed=101, realisations=2),
 def _serve(specs, lanes, max_batch):
     ex = serve.GpuExecutor(lanes=lanes)
     # all requests are ingested before the first dispatch, so the lanes receive mixed batches
-    srv = serve.Server(ex, hbm_budget=8e9, lanes=lanes, max_batch=max_batch, ingest_per_step=len(specs))
+    srv = serve.Server(ex, hbm_budget=8e9, lanes=lanes, max_batch=max_batch, ingest_per_step=len(specs),
+                       asynchronous_ingest=False)
     out = {}
 
     def on_done(r):

@@ -99,3 +99,47 @@ def test_oversized_request_is_refused():
     srv = serve.Server(FakeExecutor(time.perf_counter), hbm_budget=10.0)
     with pytest.raises(ValueError):
         srv.submit(dict(bytes=11.0, cells=1, work=0.0))
+
+
+class AsyncFakeExecutor(FakeExecutor):
+    """Ingest runs off the scheduling loop and takes `ingest_s` seconds."""
+
+    class _Handle:
+        def __init__(self, clock, t, value):
+            self.clock, self.t, self.value = clock, t, value
+
+        def done(self):
+            return self.clock() >= self.t
+
+        def result(self):
+            return self.value
+
+    def __init__(self, clock, ingest_s):
+        super().__init__(clock)
+        self.ingest_s = ingest_s
+        self.async_calls = 0
+
+    def ingest_async(self, r):
+        self.async_calls += 1
+        return self._Handle(self.clock, self.clock() + self.ingest_s, self.ingest(r))
+
+
+def test_asynchronous_ingest_and_the_synchronous_switch():
+    clock = time.perf_counter
+    ex = AsyncFakeExecutor(clock, ingest_s=0.002)
+    srv = serve.Server(ex, hbm_budget=1e9, lanes=2, max_batch=4, ingest_per_step=12, clock=clock)
+    for s in _specs(12, work=0.002):
+        srv.submit(s)
+    srv.drain()
+    assert ex.async_calls == 12 and len(srv.done) == 12 and sum(srv.batch_sizes) == 12
+    assert all(r.started >= r.ready > r.admitted for r in srv.done)
+
+    ex = AsyncFakeExecutor(clock, ingest_s=0.002)
+    srv = serve.Server(ex, hbm_budget=1e9, lanes=2, max_batch=4, ingest_per_step=12, clock=clock,
+                       asynchronous_ingest=False)
+    for s in _specs(12, work=0.002):
+        srv.submit(s)
+    srv.drain()
+    # every request ingested in the first step: the two lanes start with full batches of 4
+    assert ex.async_calls == 0 and len(srv.done) == 12
+    assert srv.batch_sizes[:2] == [4, 4] and sum(srv.batch_sizes) == 12
