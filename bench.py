@@ -7,9 +7,11 @@ Workload (one step = one inversion, inputs resident in HBM when the timed region
   ln(3000, 1500, 2.3), std (0.10, 0.15, 0.05), correlations (0.7, 0.4, 0.3), spherical 1000 m
   lateral and exponential 40 ms temporal correlation; a seeded Gaussian elastic field drawn from
   the prior and forward-modelled to seismic on the GPU.
-  Step: 3 + 5 forward FFTs, per-wavenumber posterior (FP32, FP64 re-solve where the pivot ratio of
-  the Cholesky factor exceeds 1e6), 3 inverse FFTs of the posterior means, 2 posterior
-  realisations (counter-based noise, colouring, 3 inverse FFTs + mean each).
+  Step: one cg_invert call -- 3 + 5 forward FFTs, per-wavenumber posterior (FP32, FP64 re-solve
+  where the pivot ratio of the Cholesky factor exceeds 1e6), 3 inverse FFTs of the posterior
+  means, at this geometry with the z transform split into two stages carried by the x passes and
+  the posterior pass -- then 2 posterior realisations (counter-based noise, colouring, 3 inverse
+  FFTs + the real-domain mean each).
 
 Prints one JSON line.  Also reported: per-kernel HIP-event times, the roofline of the dominant
 kernel, p50/p99 step latency, and the reference's own CPU code (FFTW 2.1.2, lib_matr,
@@ -37,27 +39,36 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def kernel_bytes(w, nt, R, n_inv_planes, launches):
+def kernel_bytes(w, nt, R, n_inv_planes, launches, split):
     """Algorithmic HBM bytes per kernel class for one step (what each kernel must read + write).
     `launches`: launches per step of each FFT class, as counted on the context stream -- every FFT
-    launch of the plain path transforms one grid (the realisations' inverse z passes run inside
-    the fused colouring and are not z launches)."""
+    launch transforms one grid (the realisations' inverse z passes run inside the fused colouring
+    and are not z launches).  split: the request ran the split-z passes (cg_invert) -- the x
+    passes carry the z stages, the posterior pass reads and writes whole grids, and the
+    realisations take the real-domain means in their inverse x passes; otherwise the drop-in
+    sequence (the posterior reads the spectra of inverted planes only, the colouring adds the
+    spectral means)."""
     G = (w["nxp"] // 2 + 1) * w["nyp"] * w["nzp"] * 8  # one complex grid
     Nr = w["nxp"] * w["nyp"] * w["nzp"] * 4  # one real cube (without row padding)
     plane = (w["nxp"] // 2 + 1) * w["nyp"]
+    if split:
+        post = (3 + nt + 3 + 6) * G  # z stage 2 needs every plane of the inputs and of the means
+    else:
+        post = n_inv_planes * plane * (3 + nt + 9) * 8 + (w["nzp"] - n_inv_planes) * plane * 6 * 8
     return {
-        # x: forward reads the real cube and writes the spectrum, inverse the reverse (the
-        # realisations get the means as spectra in the colouring, so no add in the x pass)
-        C.K_FFT_X: launches[C.K_FFT_X] * (Nr + G),
+        # x: forward reads the real cube and writes the spectrum, inverse the reverse, plus the
+        # real-domain mean each realisation grid adds (split)
+        C.K_FFT_X: launches[C.K_FFT_X] * (Nr + G) + (3 * R * Nr if split else 0),
         # y / z: one grid read and written per launch
         C.K_FFT_Y: launches[C.K_FFT_Y] * 2 * G,
         C.K_FFT_Z: launches[C.K_FFT_Z] * 2 * G,
-        # inverted cells: read 3 + nt spectra, write 3 means + 6 covariance entries;
-        # pass-through cells: write the 6 prior covariance entries
-        C.K_POSTERIOR: n_inv_planes * plane * (3 + nt + 9) * 8 + (w["nzp"] - n_inv_planes) * plane * 6 * 8,
-        # colouring: read 6 covariance and 3 mean spectra, write 3 per realisation (noise generated
-        # in registers); the step colours realisations in pairs, which read both once
-        C.K_NOISE: (R // 2) * 15 * G + (R % 2) * 12 * G,
+        # split: read 3 + nt grids, write 3 means + 6 covariance grids; drop-in: inverted cells read
+        # 3 + nt spectra and write 3 means + 6 covariance entries, pass-through cells write the 6
+        # prior covariance entries
+        C.K_POSTERIOR: post,
+        # colouring: read 6 covariance spectra (+ 3 mean spectra, drop-in), write 3 per realisation
+        # (noise generated in registers); pairs read their inputs once
+        C.K_NOISE: (R // 2) * ((12 if split else 15) * G) + (R % 2) * ((9 if split else 12) * G),
     }
 
 
@@ -490,8 +501,9 @@ def main():
     for g in inv.out[0]:
         seed = g.fillInComplexNoiseReference(seed)
     ms_refnoise = 1000.0 * (time.perf_counter() - t) / 3
+    split = kt[C.K_FFT_Z][0] == 0  # the split-z request launches no z pass of its own
     kb = kernel_bytes(w, inv.nt, R, inv.n_inverted_planes,
-                      {k: kt[k][0] // args.profile_steps for k in (C.K_FFT_X, C.K_FFT_Y, C.K_FFT_Z)})
+                      {k: kt[k][0] // args.profile_steps for k in (C.K_FFT_X, C.K_FFT_Y, C.K_FFT_Z)}, split)
     kernels = {}
     for k, (n, ms) in kt.items():
         if n == 0:

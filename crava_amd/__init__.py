@@ -717,6 +717,35 @@ def _post_args(tables, mean, data, cov, rxy, rt, lam2d, precision, cond_limit, w
     return a
 
 
+def invert(ctx, item, precision=PREC_MIXED, cond_limit=1e6, want_residual=False, stats=False):
+    """One whole inversion (cg_invert: AVOInversion::computePostMeanResidAndFFTCov followed by
+    simulate): `item` as for Batch (tables, rxy, rt, lam2d, mean_in[3], seis_in[nt], mean[3],
+    data[nt], cov[6], out, seed).  On return mean[] hold the real-domain posterior means, cov[]
+    the covariance spectra, out[] the realisations.  With stats, returns (inverted, cholesky
+    failures, FP64 re-solves)."""
+    keep = []
+    d = InversionDesc()
+    d.post = _post_args(item["tables"], item["mean"], item["data"], item["cov"], item["rxy"], item["rt"],
+                        item["lam2d"], precision, cond_limit, want_residual, keep)
+    st = (ctypes.c_uint64 * 3)()
+    if stats:
+        d.post.stats = ctypes.cast(st, ctypes.POINTER(ctypes.c_uint64))
+    for g in range(3):
+        d.mean_in[g] = item["mean_in"][g]._h
+    for l, s_ in enumerate(item["seis_in"]):
+        d.seis_in[l] = s_._h
+    outs = item.get("out", [])
+    d.nreal = len(outs)
+    arr = (_vp * max(1, 3 * len(outs)))(*[g._h for r in outs for g in r])
+    d.out = ctypes.cast(arr, ctypes.POINTER(_vp))
+    d.seed = int(item.get("seed", 0))
+    L = lib()
+    L.cg_invert.argtypes = [_vp, ctypes.POINTER(InversionDesc)]
+    L.cg_invert.restype = ctypes.c_int
+    ctx.check(L.cg_invert(ctx._h, ctypes.byref(d)), "cg_invert")
+    return tuple(int(x) for x in st) if stats else None
+
+
 class Batch:
     """Many inversions of any sizes planned together (cg_batch): every FFT pass, the posterior
     (per angle count) and the colouring of all of them share launches.  Each item is a dict

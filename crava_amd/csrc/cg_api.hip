@@ -1446,6 +1446,171 @@ int cg_simulate_pair(cg_ctx *ctx, const cg_sim_args *a, cg_grid *next_out[3])
   return simulate_impl(ctx, a, next_out);
 }
 
+// ---------------------------------------------------------------------------- one inversion
+// AVOInversion::computePostMeanResidAndFFTCov (avoinversion.cpp:229-244, 908-1114) and simulate
+// (:1401-1511) for one inversion.  With a compiled split of the z transform (cg_invert.hip) the
+// transform's stages ride in the x passes and the posterior pass; otherwise the drop-in
+// sequence (forward transforms, cg_posterior, inverse transforms) runs.
+int cg_invert(cg_ctx *ctx, const cg_inversion *v)
+{
+  if (!ctx || !v) return CG_EINVAL;
+  const cg_post_args &a = v->post;
+  const int nt = a.ntheta;
+  if (nt < 1 || nt > cg::kMaxTheta) return cg::fail(ctx, CG_EINVAL, "invert: ntheta must be 1..8");
+  const cg_grid *ref = v->mean_in[0];
+  if (!ref) return cg::fail(ctx, CG_EINVAL, "invert: missing input grid");
+  {
+    std::vector<const cg_grid *> all;
+    for (int g = 0; g < 3; ++g) all.insert(all.end(), {v->mean_in[g], a.mean[g]});
+    for (int l = 0; l < nt; ++l) all.insert(all.end(), {v->seis_in[l], a.data[l]});
+    for (int g = 0; g < 6; ++g) all.push_back(a.cov[g]);
+    if (v->nreal < 0 || (v->nreal > 0 && !v->out)) return cg::fail(ctx, CG_EINVAL, "invert: bad realisations");
+    for (int r = 0; r < 3 * v->nreal; ++r) all.push_back(v->out[r]);
+    for (const cg_grid *g : all)
+      if (!g || g->ctx != ctx || !same_shape(g, ref)) return cg::fail(ctx, CG_EINVAL, "invert: bad grid");
+    for (int g = 0; g < 3; ++g)
+      if (v->mean_in[g]->transformed) return cg::fail(ctx, CG_ESTATE, "invert: inputs must be in the real domain");
+    for (int l = 0; l < nt; ++l)
+      if (v->seis_in[l]->transformed) return cg::fail(ctx, CG_ESTATE, "invert: inputs must be in the real domain");
+    for (int g = 0; g < 3; ++g)
+      for (int l = 0; l < nt; ++l)
+        if (a.mean[g] == a.data[l]) return cg::fail(ctx, CG_EINVAL, "invert: work grids overlap");
+  }
+  DeviceGuard dg(ctx->device);
+  const int nxp = ref->nxp, nyp = ref->nyp, nzp = ref->nzp, cnxp = ref->cnxp;
+  const size_t plane = (size_t)cnxp * nyp;
+  int n1 = 0, n2 = 0;
+  const bool split = ctx->fft_engine == CG_FFT_AUTO && !a.want_residual && a.prior_mode == CG_PRIOR_SEPARABLE &&
+                     !a.err_corr && a.rxy && a.rt && a.lam2d && cg::split_z_supported(nxp, nzp, &n1, &n2);
+  int rc = CG_OK;
+  std::vector<cg_grid *> work;
+  for (int g = 0; g < 3; ++g) work.push_back(a.mean[g]);
+  for (int l = 0; l < nt; ++l) work.push_back(a.data[l]);
+  auto input = [&](int i) { return i < 3 ? v->mean_in[i] : v->seis_in[i - 3]; };
+  if (!split) {
+    for (int i = 0; i < 3 + nt && !rc; ++i) {
+      work[i]->cubetype = input(i)->cubetype;
+      rc = cg::grid_fft_forward(work[i], input(i)->d);
+      if (!rc) work[i]->transformed = 1;
+    }
+    if (!rc) rc = cg_posterior(ctx, &a);
+    for (int g = 0; g < 3 && !rc; ++g) {
+      rc = cg::grid_fft_inverse(a.mean[g], nullptr);
+      if (!rc) a.mean[g]->transformed = 0;
+    }
+  } else {
+    if (a.precision != CG_PREC_REFERENCE && a.precision != CG_PREC_MIXED)
+      return cg::fail(ctx, CG_EINVAL, "invert: unknown precision");
+    if (!a.K || !a.e1 || !a.e2 || !a.e3 || !a.invert || !a.sigma0e || !a.prior_var0)
+      return cg::fail(ctx, CG_EINVAL, "invert: missing table");
+    const bool mixed = a.precision == CG_PREC_MIXED;
+    std::vector<cg::PlaneTab<float>> t32;
+    std::vector<cg::PlaneTab<double>> t64;
+    build_plane_tables<float>(&a, nzp, t32);
+    if (mixed) build_plane_tables<double>(&a, nzp, t64);
+    std::vector<int> inv(a.invert, a.invert + nzp);
+    std::vector<std::pair<const void *, size_t>> parts = {
+        {t32.data(), sizeof(cg::PlaneTab<float>) * nzp},
+        {mixed ? (const void *)t64.data() : nullptr, mixed ? sizeof(cg::PlaneTab<double>) * nzp : 0},
+        {inv.data(), sizeof(int) * nzp},
+        {a.rxy, sizeof(float) * plane},
+        {a.rt, sizeof(float) * nzp},
+        {a.lam2d, sizeof(float) * plane},
+    };
+    std::vector<void *> dp;
+    rc = stage_tables(ctx, parts, dp);
+    if (rc) return rc;
+    const unsigned ctiles = (unsigned)((plane + 63) / 64);
+    const size_t ntiles = (size_t)ctiles * n2;
+    unsigned *list = nullptr;
+    if (mixed) {  // the deferred-tile list: capacity all tiles
+      const size_t bytes = align_up(ntiles * sizeof(unsigned), 256);
+      if (ctx->list_bytes < bytes) {
+        (void)hipStreamSynchronize(ctx->stream);
+        if (ctx->list) {
+          (void)hipFree(ctx->list);
+          ctx->in_use -= ctx->list_bytes;
+        }
+        ctx->list = nullptr;
+        ctx->list_bytes = 0;
+        hipError_t e = hipMalloc(&ctx->list, bytes);
+        if (e != hipSuccess) return cg::hip_fail(ctx, e, "hipMalloc(deferred tiles)");
+        ctx->list_bytes = bytes;
+        ctx->in_use += bytes;
+        ctx->peak = std::max(ctx->peak, ctx->in_use);
+      }
+      list = ctx->list;
+    }
+    const float N = (float)nxp * (float)nyp * (float)nzp;
+    const float s_fwd = 1.0f / sqrtf(N);
+    for (int i = 0; i < 3 + nt && !rc; ++i) {
+      const cg_grid *in = input(i);
+      cg_grid *w = work[i];
+      const float s = in->cubetype == CG_CT_COVARIANCE ? 1.0f : s_fwd;
+      rc = cg::fft_xz(ctx, 1, in->d, w->d, nullptr, nxp, nyp, nzp, s);
+      if (!rc && nyp > 1)
+        rc = cg::fft_strided(ctx, reinterpret_cast<float2 *>(w->d), nyp, cnxp, cnxp, nzp, (long)plane, -1, 1.0f,
+                             CG_K_FFT_Y);
+      w->cubetype = in->cubetype;
+      w->transformed = 1;
+    }
+    if (rc) return rc;
+    cg::PostParams p = post_params(&a);
+    p.tab32 = static_cast<const cg::PlaneTab<float> *>(dp[0]);
+    p.tab64 = static_cast<const cg::PlaneTab<double> *>(dp[1]);
+    p.invert = static_cast<const int *>(dp[2]);
+    p.rxy = static_cast<const float *>(dp[3]);
+    p.rt = static_cast<const float *>(dp[4]);
+    p.lam2d = static_cast<const float *>(dp[5]);
+    // counters[0]: FP64 re-solves; [1]: deferred tiles; [8, 136): statistics slots
+    unsigned long long *cnt = reinterpret_cast<unsigned long long *>(ctx->counters);
+    p.list_len = mixed ? cnt : nullptr;
+    p.stats = a.stats ? cnt + 8 : nullptr;
+    if (mixed || a.stats) {
+      rc = cg::zero_counters(ctx, cnt, 136);
+      if (rc) return rc;
+    }
+    cg::PostZArgs z;
+    z.p = p;
+    z.n2 = n2;
+    z.ctiles = ctiles;
+    z.redo = list;
+    z.redo_len = mixed ? cnt + 1 : nullptr;
+    rc = cg::launch_posterior_z(ctx, z, n1, mixed);
+    if (rc) return rc;
+    for (int g = 0; g < 6; ++g) {
+      a.cov[g]->transformed = 1;
+      a.cov[g]->cubetype = CG_CT_COVARIANCE;
+    }
+    for (int g = 0; g < 3 && !rc; ++g) {
+      float2 *c = reinterpret_cast<float2 *>(a.mean[g]->d);
+      if (nyp > 1) rc = cg::fft_strided(ctx, c, nyp, cnxp, cnxp, nzp, (long)plane, +1, 1.0f, CG_K_FFT_Y);
+      const float s = a.mean[g]->cubetype == CG_CT_COVARIANCE ? float(1.0 / (double)N) : s_fwd;
+      if (!rc) rc = cg::fft_xz(ctx, 0, nullptr, a.mean[g]->d, nullptr, nxp, nyp, nzp, s);
+      a.mean[g]->transformed = 0;
+    }
+    if (!rc && a.stats) rc = post_stats(ctx, cnt, mixed, a.stats);
+  }
+  if (rc) return rc;
+  // realisations: pairs (2m, 2m + 1) as cg_simulate_pair, a last odd one alone; the posterior
+  // means are added in the real domain
+  for (int r = 0; r < v->nreal && !rc; r += 2) {
+    cg_sim_args s;
+    std::memset(&s, 0, sizeof(s));
+    for (int g = 0; g < 6; ++g) s.cov[g] = a.cov[g];
+    for (int g = 0; g < 3; ++g) {
+      s.mean[g] = a.mean[g];
+      s.out[g] = v->out[3 * r + g];
+    }
+    s.noise_source = CG_NOISE_COUNTER;
+    s.seed = v->seed;
+    s.realisation = (uint32_t)r;
+    if (r + 1 < v->nreal) rc = simulate_impl(ctx, &s, v->out + 3 * (r + 1));
+    else rc = simulate_impl(ctx, &s, nullptr);
+  }
+  return rc;
+}
+
 // ---------------------------------------------------------------------------- forward modelling
 int cg_forward_model(cg_ctx *ctx, int nt, const float *K, cg_grid *m[3], cg_grid *d[])
 {

@@ -1217,4 +1217,204 @@ int xo_launch(cg_ctx *ctx, int forward, int add, const cg::XArgs *one, const cg:
   return cg::hip_fail(ctx, hipGetLastError(), "k_fftxo");
 }
 
+
+// ---------------------------------------------------------------- x pass with the first z stage
+// The z transform of length nzp = N1 N2 in two stages (n = n1 + N1 n2, k = k2 + N2 k1,
+// w = exp(-2 pi i / nzp)):
+//   stage 1: for each n1, a length-N2 DFT over n2 (planes n1 + N1 n2), times w^(n1 k2), into
+//            plane n1 + N1 k2;
+//   stage 2: for each k2, a length-N1 DFT over the N1 consecutive planes n1 + N1 k2, after which
+//            plane k1 + N1 k2 holds frequency k2 + N2 k1.
+// A tile of this x pass is the N2 rows of one (y, n1), which it transforms along x in LDS as
+// k_fftx_r2c does and then along z (stage 1) before anything reaches HBM; stage 2 belongs to a
+// pass whose tiles are N1 consecutive planes (the posterior, which also starts the inverse
+// transform with the inverse of stage 2).  The y pass commutes with both stages.  Inverse: the
+// inverse stage 1 (times w^-(n1 k2), then a length-N2 inverse DFT over k2) and the x c2r.
+// Even nxp = 2N only (CN = N + 1 complex slots per row).  Tile t: y = t mod nyp, n1 = t / nyp.
+struct XZArgs {
+  const float *src;  // forward: the real grid read (may be the grid itself)
+  float *dst;        // the grid
+  const float *add;  // inverse: real grid added to the output, or null
+  int nyp, n1;       // rows per plane; N1
+  float scale;
+};
+
+template <int N, int N2>
+struct XZ {
+  static constexpr int TPS = Plan<N>::TPS;
+  static constexpr int NT = N2 * TPS;
+  static constexpr int ES = N2 + 1;  // odd: slot m of row r at m ES + r
+  static constexpr int CN = N + 1;
+  static constexpr int PER = (N2 * CN + NT - 1) / NT;
+};
+
+template <int N, int N2>
+__global__ void __launch_bounds__(N2 * Plan<N>::TPS)
+k_fftxz_fwd(XZArgs a, const float2 *__restrict__ twg, const float2 *__restrict__ tw_row,
+            const float2 *__restrict__ twz)
+{
+  using K = XZ<N, N2>;
+  constexpr int NT = K::NT, ES = K::ES, CN = K::CN, PER = K::PER;
+  __shared__ float2 s_tw[N];
+  __shared__ float2 s_twr[N / 2 + 1];
+  __shared__ float2 s_twz[N2];
+  __shared__ float2 buf[CN * ES];
+  const int tid = threadIdx.x;
+  const unsigned t = cg_xcd_remap(blockIdx.x, gridDim.x);
+  const int y = (int)(t % (unsigned)a.nyp), n1 = (int)(t / (unsigned)a.nyp);
+  const long pitch = (long)a.n1 * a.nyp * CN;  // float2 from row n2 to row n2 + 1 (N1 planes)
+  const long first = ((long)n1 * a.nyp + y) * CN;
+  const float2 *in = reinterpret_cast<const float2 *>(a.src) + first;
+  float2 *out = reinterpret_cast<float2 *>(a.dst) + first;
+  load_tw(s_tw, twg, N, tid, NT);
+  load_tw(s_twr, tw_row, N / 2 + 1, tid, NT);
+  if (tid < N2) s_twz[tid] = twz[n1 * tid];  // n1 k2 < nzp
+  {
+    float2 v[PER];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int f = tid + i * NT;
+      const int r = f / CN, m = f - r * CN;
+      v[i] = f < N2 * CN ? cg::gld(in + r * pitch + m) : make_float2(0.f, 0.f);
+    }
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int f = tid + i * NT;
+      const int r = f / CN, m = f - r * CN;
+      if (f < N2 * CN) buf[m * ES + r] = cscale(v[i], a.scale);
+    }
+  }
+  lds_sync();
+  Plan<N>::template run<-1, ES>(buf + tid % N2, tid / N2, s_tw);
+  for (int idx = tid; idx < N2 * (N / 2 + 1); idx += NT) {  // R2C post-pass, as k_fftx_r2c
+    const int r = idx % N2, k = idx / N2;
+    const int kn = k == 0 ? 0 : N - k;
+    const float2 za = buf[k * ES + r], zb = buf[kn * ES + r];
+    const float2 ea = cscale(cadd(za, cconj(zb)), 0.5f), da = csub(za, cconj(zb));
+    const float2 eb = cscale(cadd(zb, cconj(za)), 0.5f), db = csub(zb, cconj(za));
+    const float2 w = s_twr[k];
+    const float2 xa = cadd(ea, cmul(w, make_float2(0.5f * da.y, -0.5f * da.x)));
+    const float2 xb = cadd(eb, cmul(make_float2(-w.x, w.y), make_float2(0.5f * db.y, -0.5f * db.x)));
+    buf[k * ES + r] = xa;
+    buf[(N - k) * ES + r] = xb;
+  }
+  lds_sync();
+  // stage 1 along z: bin m of the N2 rows, one sequence per thread, stored row by row (lanes on
+  // consecutive bins: coalesced)
+  for (int m = tid; m < CN; m += NT) {
+    float2 c[N2];
+#pragma unroll
+    for (int r = 0; r < N2; ++r) c[r] = buf[m * ES + r];
+    dft<-1, N2>(c);
+#pragma unroll
+    for (int k2 = 0; k2 < N2; ++k2) cg::gst(out + k2 * pitch + m, k2 == 0 ? c[0] : cmul(c[k2], s_twz[k2]));
+  }
+}
+
+template <int N, int N2, bool ADD>
+__global__ void __launch_bounds__(N2 * Plan<N>::TPS)
+k_fftxz_inv(XZArgs a, const float2 *__restrict__ twg, const float2 *__restrict__ tw_row,
+            const float2 *__restrict__ twz)
+{
+  using K = XZ<N, N2>;
+  constexpr int NT = K::NT, ES = K::ES, CN = K::CN, PER = K::PER;
+  __shared__ float2 s_tw[N];
+  __shared__ float2 s_twr[N / 2 + 1];
+  __shared__ float2 s_twz[N2];
+  __shared__ float2 buf[CN * ES];
+  const int tid = threadIdx.x;
+  const unsigned t = cg_xcd_remap(blockIdx.x, gridDim.x);
+  const int y = (int)(t % (unsigned)a.nyp), n1 = (int)(t / (unsigned)a.nyp);
+  const long pitch = (long)a.n1 * a.nyp * CN;
+  const long first = ((long)n1 * a.nyp + y) * CN;
+  float2 *io = reinterpret_cast<float2 *>(a.dst) + first;
+  const float2 *ad = ADD ? reinterpret_cast<const float2 *>(a.add) + first : nullptr;
+  load_tw(s_tw, twg, N, tid, NT);
+  load_tw(s_twr, tw_row, N / 2 + 1, tid, NT);
+  if (tid < N2) s_twz[tid] = cconj(twz[n1 * tid]);
+  {
+    float2 v[PER];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int f = tid + i * NT;
+      const int r = f / CN, m = f - r * CN;
+      v[i] = f < N2 * CN ? cg::gld(io + r * pitch + m) : make_float2(0.f, 0.f);
+    }
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int f = tid + i * NT;
+      const int r = f / CN, m = f - r * CN;
+      if (f < N2 * CN) buf[m * ES + r] = v[i];
+    }
+  }
+  lds_sync();
+  // inverse stage 1: row k2 times w^-(n1 k2), then the inverse DFT over k2 into rows n2
+  for (int m = tid; m < CN; m += NT) {
+    float2 c[N2];
+#pragma unroll
+    for (int r = 0; r < N2; ++r) c[r] = r == 0 ? buf[m * ES] : cmul(buf[m * ES + r], s_twz[r]);
+    dft<+1, N2>(c);
+#pragma unroll
+    for (int r = 0; r < N2; ++r) buf[m * ES + r] = c[r];
+  }
+  lds_sync();
+  for (int idx = tid; idx < N2 * (N / 2 + 1); idx += NT) {  // C2R pre-pass, as k_fftx_c2r
+    const int r = idx % N2, k = idx / N2;
+    float2 xa = buf[k * ES + r], xb = buf[(N - k) * ES + r];
+    if (k == 0) {
+      xa.y = 0.f;
+      xb.y = 0.f;
+    }
+    const float2 w = s_twr[k];
+    const float2 ea = cadd(xa, cconj(xb)), oa = cmul(csub(xa, cconj(xb)), cconj(w));
+    buf[k * ES + r] = make_float2(ea.x - oa.y, ea.y + oa.x);
+    if (k > 0 && k < N - k) {
+      const float2 eb = cadd(xb, cconj(xa)), ob = cmul(csub(xb, cconj(xa)), make_float2(-w.x, -w.y));
+      buf[(N - k) * ES + r] = make_float2(eb.x - ob.y, eb.y + ob.x);
+    }
+  }
+  lds_sync();
+  Plan<N>::template run<+1, ES>(buf + tid % N2, tid / N2, s_tw);
+  float2 va[ADD ? PER : 1];
+  if (ADD) {
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int f = tid + i * NT;
+      const int r = f / CN, m = f - r * CN;
+      va[ADD ? i : 0] = f < N2 * CN ? cg::gld(ad + r * pitch + m) : make_float2(0.f, 0.f);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const int f = tid + i * NT;
+    const int r = f / CN, m = f - r * CN;
+    if (f < N2 * CN) {
+      float2 o = make_float2(0.f, 0.f);
+      if (m < N) {
+        o = cscale(buf[m * ES + r], a.scale);
+        if (ADD) {
+          o.x += va[ADD ? i : 0].x;
+          o.y += va[ADD ? i : 0].y;
+        }
+      }
+      cg::gst(io + r * pitch + m, o);
+    }
+  }
+}
+
+template <int N, int N2>
+int xz_launch(cg_ctx *ctx, int forward, const XZArgs &a, int nzp, const float2 *tw, const float2 *twr,
+              const float2 *twz)
+{
+  const unsigned nblk = (unsigned)a.nyp * (unsigned)a.n1;
+  if (forward)
+    k_fftxz_fwd<N, N2><<<nblk, XZ<N, N2>::NT, 0, ctx->stream>>>(a, tw, twr, twz);
+  else if (a.add)
+    k_fftxz_inv<N, N2, true><<<nblk, XZ<N, N2>::NT, 0, ctx->stream>>>(a, tw, twr, twz);
+  else
+    k_fftxz_inv<N, N2, false><<<nblk, XZ<N, N2>::NT, 0, ctx->stream>>>(a, tw, twr, twz);
+  (void)nzp;
+  return cg::hip_fail(ctx, hipGetLastError(), "k_fftxz");
+}
+
 }  // namespace cgfix

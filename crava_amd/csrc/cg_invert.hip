@@ -1,0 +1,319 @@
+// cg_invert.hip -- kernels of the split-z inversion (cg_invert): the posterior means and
+// covariance of one inversion with the z transform cut into two stages, so that the posterior
+// pass also carries the z transform's last forward stage and the inverse's first stage.
+//
+// Pass sequence of one inversion (nzp = N1 N2; G = 3 + ntheta input grids):
+//   G x  k_fftxz_fwd   x r2c + z stage 1                  read the real cube, write the grid
+//   G x  y pass (k_ffts, forward)
+//   1 x  k_posterior_z z stage 2 of G grids, posterior, inverse z stage 2 of the 3 means
+//   3 x  y pass (inverse)
+//   3 x  k_fftxz_inv   inverse z stage 1 + x c2r
+// against x, y, z, posterior, z, y, x of the unsplit sequence: the forward z passes (2 G grids
+// of traffic), the posterior's round trip of the spectra and the means' inverse z passes
+// (6 grids) are replaced by one read of the G grids and one write of the means.
+//
+// k_posterior_z: tile = N1 consecutive planes (n1 + N1 k2, one k2) x 64 consecutive cells of a
+// plane.  The G grids' tile rows go to LDS; one thread per (grid, column) runs the length-N1
+// DFT along z in registers; then wave w of 8 takes planes k1 = w, w + 8, ... -- frequency
+// k = k2 + N2 k1, one k per wave, so the per-plane tables are scalar loads as in k_posterior --
+// and the 64 lanes the 64 columns; the covariance goes straight to HBM at plane k (the natural
+// spectrum layout), the means back to LDS, and one thread per (mean, column) runs the inverse
+// length-N1 DFT and stores the column's N1 planes.  Per-cell arithmetic is cg_post_dev.h's
+// update, as in k_posterior (avoinversion.cpp:944-1083).
+//
+// FP64 re-solves (CG_PREC_MIXED): a tile with a deferred wavenumber writes the covariance of its
+// other cells, leaves its means unwritten (the prior means stay in HBM, and the seismic grids
+// are only ever read) and is appended to a tile list; k_posterior_z<.., REDO> then runs each
+// listed tile again with the deferred cells in FP64.  Every tile is written by exactly one
+// workgroup, so the result does not depend on the list's order.
+#include "cg_fft_fixed.h"
+#include "cg_post_dev.h"
+
+namespace cgpost {
+
+using cgdft::dft;
+using cgfix::lds_sync;
+
+constexpr int kZCols = 64;     // cells of a plane per tile (one per lane)
+constexpr int kZThreads = 512;  // 8 waves
+
+// One 64-column tile.  buf: [G][N1][64] float2 in LDS; s_defer: the workgroup's deferral flag.
+template <int NT, int N1, bool MIXED, bool REDO>
+__device__ __forceinline__ void post_z_tile(const cg::PostZArgs &z, unsigned t, float2 *buf, int *s_defer)
+{
+  constexpr int G = 3 + NT, TC = kZCols, NTH = kZThreads, W = NTH / 64;
+  static_assert((TC * N1) % NTH == 0, "a tile's rows of one grid must be whole rounds of the threads");
+  constexpr int PER = G * N1 * TC / NTH;
+  const cg::PostParams &p = z.p;
+  const int tid = threadIdx.x;
+  const long plane = (long)p.cnxp * p.nyp;
+  const unsigned k2 = t / z.ctiles, ct = t % z.ctiles;
+  const long c0 = (long)ct * TC;
+  // 1. the G grids' rows of planes n1 + N1 k2, cells c0 .. c0 + 63 (512-byte segments)
+  {
+    float2 v[PER];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      constexpr int RPG = TC * N1 / NTH;  // rounds per grid
+      const int g = i / RPG;
+      const int e = (i % RPG) * NTH + tid;  // within the grid's tile
+      const int c = e % TC, n1 = e / TC;
+      const float2 *src = g < 3 ? p.mean[g < 3 ? g : 0] : p.data[g >= 3 ? g - 3 : 0];
+      v[i] = c0 + c < plane ? cg::gld(src + (long)(n1 + N1 * (long)k2) * plane + c0 + c) : make_float2(0.f, 0.f);
+    }
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      constexpr int RPG = TC * N1 / NTH;
+      const int g = i / RPG;
+      const int e = (i % RPG) * NTH + tid;
+      buf[g * N1 * TC + e] = v[i];
+    }
+  }
+  if (MIXED && !REDO && tid == 0) *s_defer = 0;
+  lds_sync();
+  // 2. z stage 2, forward: one (grid, column) per thread
+  for (int s = tid; s < G * TC; s += NTH) {
+    const int g = s / TC, c = s % TC;
+    float2 w[N1];
+#pragma unroll
+    for (int n = 0; n < N1; ++n) w[n] = buf[(g * N1 + n) * TC + c];
+    dft<-1, N1>(w);
+#pragma unroll
+    for (int n = 0; n < N1; ++n) buf[(g * N1 + n) * TC + c] = w[n];
+  }
+  lds_sync();
+  // 3. the posterior of each cell: wave wv takes planes k1 = wv, wv + 8, ...
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6), c = tid & 63;
+  const long cc = c0 + c;
+  const bool active = cc < plane;
+  const long ccs = active ? cc : 0;
+  bool any_defer = false;
+#pragma unroll 1
+  for (int k1 = wv; k1 < N1; k1 += W) {
+    const int k = (int)k2 + z.n2 * k1;  // wavefront-uniform
+    const long idx = (long)k * plane + ccs;
+    const bool inv = kat(p.invert, k) != 0;
+    bool did_inv = false, did_fail = false, defer = false;
+    if (active) {
+      const float tr = gld(p.rxy + ccs) * kat(p.rt, k);
+      const float r = sqrtf(tr * tr);
+      float P6[6];
+#pragma unroll
+      for (int g = 0; g < 6; ++g) P6[g] = r * p.s6f[g];
+      bool write_prior = !inv, write_cov = false;
+      cpx<float> cov6[6];
+      if (inv) {
+        did_inv = true;
+        const float lt = gld(p.lam2d + ccs);
+        const float lam = sqrtf(lt * lt);
+        cpx<float> m3[3], d[NT], res[NT];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          const float2 v = buf[(a * N1 + k1) * TC + c];
+          m3[a] = mk<float>(v.x, v.y);
+        }
+#pragma unroll
+        for (int l = 0; l < NT; ++l) {
+          const float2 v = buf[((3 + l) * N1 + k1) * TC + c];
+          d[l] = mk<float>(v.x, v.y);
+        }
+        float cond = 0.f;
+        const int fail = update<NT, float, true>(kat(p.tab32, k), r, lam, P6, m3, d, cov6, res, false, cond);
+        if (MIXED && (fail || cond > p.cond_limit)) {
+          defer = true;
+        } else if (fail) {  // posterior = prior (avoinversion.cpp:1040-1042)
+          did_fail = true;
+          write_prior = true;
+        } else {
+          write_cov = true;
+#pragma unroll
+          for (int a = 0; a < 3; ++a) buf[(a * N1 + k1) * TC + c] = make_float2(m3[a].re, m3[a].im);
+        }
+        if (REDO && defer) {  // the FP64 re-solve of k_posterior64 on the same inputs
+          const double rd = fabs((double)gld(p.rxy + ccs) * (double)kat(p.rt, k));
+          double P6d[6];
+#pragma unroll
+          for (int g = 0; g < 6; ++g) P6d[g] = rd * p.s6d[g];
+          cpx<double> m3d[3], dd[NT], cov6d[6], resd[NT];
+#pragma unroll
+          for (int a = 0; a < 3; ++a) {
+            const float2 v = buf[(a * N1 + k1) * TC + c];
+            m3d[a] = mk<double>(v.x, v.y);
+          }
+#pragma unroll
+          for (int l = 0; l < NT; ++l) {
+            const float2 v = buf[((3 + l) * N1 + k1) * TC + c];
+            dd[l] = mk<double>(v.x, v.y);
+          }
+          double cond64 = 0.0;
+          const int fail64 = update<NT, double, true>(kat(p.tab64, k), rd, fabs((double)lt), P6d, m3d, dd, cov6d, resd,
+                                                      false, cond64);
+          if (fail64) {
+            did_fail = true;  // rare: the FP64 factor failed too
+            write_prior = true;
+          } else {
+#pragma unroll
+            for (int g = 0; g < 6; ++g) cov6[g] = mk<float>((float)cov6d[g].re, (float)cov6d[g].im);
+            write_cov = true;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) buf[(a * N1 + k1) * TC + c] = make_float2((float)m3d[a].re, (float)m3d[a].im);
+          }
+        }
+      }
+      if (write_cov) {
+#pragma unroll
+        for (int g = 0; g < 6; ++g) gst(p.cov[g] + idx, make_float2(cov6[g].re, cov6[g].im));
+      } else if (write_prior) {
+#pragma unroll
+        for (int g = 0; g < 6; ++g) gst(p.cov[g] + idx, make_float2(P6[g], 0.f));
+      }
+    }
+    any_defer |= defer;
+    if (p.stats) {
+      if (!REDO) wave_count(p.stats, did_inv);
+      // REDO counts only the re-solved cells' outcome; their tile's other cells were counted
+      // by the first pass
+      wave_count(p.stats + 64, REDO ? (defer && did_fail) : did_fail);
+    }
+    if (REDO) {
+      const unsigned long long m = __ballot(defer);
+      if (m && __lane_id() == 0) atomicAdd(p.list_len, (unsigned long long)__popcll(m));
+    }
+  }
+  if (MIXED && !REDO && any_defer) *s_defer = 1;
+  lds_sync();
+  if (MIXED && !REDO && *s_defer) {
+    if (tid == 0) {
+      const unsigned long long slot = atomicAdd(z.redo_len, 1ull);
+      z.redo[slot] = t;
+    }
+    return;
+  }
+  // 4. inverse z stage 2 of the means: one (mean, column) per thread, the column's N1 planes
+  // stored with lanes on consecutive cells
+  for (int s = tid; s < 3 * TC; s += NTH) {
+    const int a = s / TC, cl = s % TC;
+    float2 w[N1];
+#pragma unroll
+    for (int n = 0; n < N1; ++n) w[n] = buf[(a * N1 + n) * TC + cl];
+    dft<+1, N1>(w);
+    if (c0 + cl < plane) {
+      float2 *dst = p.mean[a] + (long)N1 * k2 * plane + c0 + cl;
+#pragma unroll
+      for (int n = 0; n < N1; ++n) gst(dst + (long)n * plane, w[n]);
+    }
+  }
+}
+
+template <int NT, int N1, bool MIXED>
+__global__ void __launch_bounds__(kZThreads) k_posterior_z(cg::PostZArgs z)
+{
+  __shared__ float2 buf[(3 + NT) * N1 * kZCols];
+  __shared__ int s_defer;
+  post_z_tile<NT, N1, MIXED, false>(z, cg_xcd_remap(blockIdx.x, gridDim.x), buf, &s_defer);
+}
+
+// The deferred tiles again, FP64 where the first pass deferred (workgroups stride over the list;
+// the loop bound is the same for every thread of a workgroup).
+template <int NT, int N1>
+__global__ void __launch_bounds__(kZThreads) k_posterior_z_redo(cg::PostZArgs z)
+{
+  __shared__ float2 buf[(3 + NT) * N1 * kZCols];
+  __shared__ int s_defer;
+  const unsigned long long n = *z.redo_len;
+  for (unsigned long long i = blockIdx.x; i < n; i += gridDim.x) {
+    post_z_tile<NT, N1, true, true>(z, z.redo[i], buf, &s_defer);
+    lds_sync();  // the tile's LDS is read to the end before the next tile's rows arrive
+  }
+}
+
+template <int NT, int N1>
+int launch_z_nt(cg_ctx *ctx, const cg::PostZArgs &z, unsigned tiles, bool mixed)
+{
+  if (mixed) {
+    k_posterior_z<NT, N1, true><<<tiles, kZThreads, 0, ctx->stream>>>(z);
+    int rc = cg::hip_fail(ctx, hipGetLastError(), "k_posterior_z");
+    if (rc) return rc;
+    const unsigned blocks = tiles < 1024u ? tiles : 1024u;
+    k_posterior_z_redo<NT, N1><<<blocks, kZThreads, 0, ctx->stream>>>(z);
+    return cg::hip_fail(ctx, hipGetLastError(), "k_posterior_z_redo");
+  }
+  k_posterior_z<NT, N1, false><<<tiles, kZThreads, 0, ctx->stream>>>(z);
+  return cg::hip_fail(ctx, hipGetLastError(), "k_posterior_z");
+}
+
+}  // namespace cgpost
+
+namespace {
+
+// Compiled splits of the split-z inversion: x half-length N (nxp = 2N), N1, N2 (nzp = N1 N2).
+struct XZEntry {
+  int n, n1, n2;
+  int (*launch)(cg_ctx *, int, const cgfix::XZArgs &, int, const float2 *, const float2 *, const float2 *);
+};
+const XZEntry kXZ[] = {
+    {288, 16, 20, &cgfix::xz_launch<288, 20>},
+    {576, 16, 24, &cgfix::xz_launch<576, 24>},
+};
+
+const XZEntry *xz_find(int nxp, int nzp)
+{
+  if (nxp % 2) return nullptr;
+  for (const XZEntry &e : kXZ)
+    if (e.n == nxp / 2 && e.n1 * e.n2 == nzp) return &e;
+  return nullptr;
+}
+
+}  // namespace
+
+namespace cg {
+
+bool split_z_supported(int nxp, int nzp, int *n1, int *n2)
+{
+  const XZEntry *e = xz_find(nxp, nzp);
+  if (!e) return false;
+  if (n1) *n1 = e->n1;
+  if (n2) *n2 = e->n2;
+  return true;
+}
+
+int fft_xz(cg_ctx *ctx, int forward, const float *src, float *grid, const float *add, int nxp, int nyp, int nzp,
+           float scale)
+{
+  const XZEntry *e = xz_find(nxp, nzp);
+  if (!e) return fail(ctx, CG_EINVAL, "split-z x pass: no compiled split for this grid");
+  const float2 *tw = twiddle_table(ctx, e->n);
+  const float2 *twr = twiddle_table(ctx, nxp);
+  const float2 *twz = twiddle_table(ctx, nzp);
+  if (!tw || !twr || !twz) return CG_ENOMEM;
+  cgfix::XZArgs a{forward ? src : nullptr, grid, forward ? nullptr : add, nyp, e->n1, scale};
+  hipEvent_t ev;
+  launch_begin(ctx, CG_K_FFT_X, &ev);
+  const int rc = e->launch(ctx, forward, a, nzp, tw, twr, twz);
+  launch_end(ctx, CG_K_FFT_X, ev);
+  return rc;
+}
+
+int launch_posterior_z(cg_ctx *ctx, const PostZArgs &z, int n1, bool mixed)
+{
+  const long plane = (long)z.p.cnxp * z.p.nyp;
+  const unsigned long tiles = (unsigned long)z.n2 * (unsigned long)((plane + cgpost::kZCols - 1) / cgpost::kZCols);
+  if (tiles >= (1ul << 31) || n1 != 16) return fail(ctx, CG_EINVAL, "split-z posterior: unsupported split");
+  hipEvent_t ev;
+  launch_begin(ctx, CG_K_POSTERIOR, &ev);
+  int rc;
+  switch (z.p.nt) {
+    case 1: rc = cgpost::launch_z_nt<1, 16>(ctx, z, (unsigned)tiles, mixed); break;
+    case 2: rc = cgpost::launch_z_nt<2, 16>(ctx, z, (unsigned)tiles, mixed); break;
+    case 3: rc = cgpost::launch_z_nt<3, 16>(ctx, z, (unsigned)tiles, mixed); break;
+    case 4: rc = cgpost::launch_z_nt<4, 16>(ctx, z, (unsigned)tiles, mixed); break;
+    case 5: rc = cgpost::launch_z_nt<5, 16>(ctx, z, (unsigned)tiles, mixed); break;
+    case 6: rc = cgpost::launch_z_nt<6, 16>(ctx, z, (unsigned)tiles, mixed); break;
+    case 7: rc = cgpost::launch_z_nt<7, 16>(ctx, z, (unsigned)tiles, mixed); break;
+    default: rc = cgpost::launch_z_nt<8, 16>(ctx, z, (unsigned)tiles, mixed); break;
+  }
+  launch_end(ctx, CG_K_POSTERIOR, ev);
+  return rc;
+}
+
+}  // namespace cg

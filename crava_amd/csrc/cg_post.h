@@ -56,6 +56,25 @@ struct PostItem {
 };
 
 int launch_posterior(cg_ctx *ctx, const PostParams &p);
+
+// The split-z posterior pass (cg_invert.hip): mean[] and data[] hold the spectra with the z
+// transform's first stage done (plane n1 + N1 k2 of frequency index k2), cov[] receive the
+// natural-layout covariance spectra, mean[] the posterior means with the inverse z transform's
+// first stage done.
+struct PostZArgs {
+  PostParams p;
+  int n2;                          // N2 (nzp = N1 N2)
+  unsigned ctiles;                 // 64-cell tiles per plane
+  unsigned *redo;                  // MIXED: tiles left to the FP64 pass (capacity: all tiles)
+  unsigned long long *redo_len;    // MIXED: their number (zero on entry)
+};
+int launch_posterior_z(cg_ctx *ctx, const PostZArgs &z, int n1, bool mixed);
+// A compiled split nzp = n1 n2 for this x length (even nxp only).
+bool split_z_supported(int nxp, int nzp, int *n1, int *n2);
+// x pass with z stage 1 (forward: src real grid -> grid) or inverse z stage 1 with the x c2r
+// (grid in place, + add).
+int fft_xz(cg_ctx *ctx, int forward, const float *src, float *grid, const float *add, int nxp, int nyp, int nzp,
+           float scale);
 // The allocation-free variant's fixed device buffer (cg_posterior_noalloc): one per device, a
 // file-scope array of kNoAllocBytes; noalloc_scratch_init looks up its address on the current
 // device once (at context creation), noalloc_scratch returns it (null before the lookup or if it

@@ -15,7 +15,8 @@ import math
 
 import numpy as np
 
-from . import (COVARIANCE, DATA, PARAMETER, PREC_MIXED, Arena, FFTGrid, forward_model, posterior, simulate)
+from . import (COVARIANCE, DATA, PARAMETER, PREC_MIXED, Arena, FFTGrid, forward_model, invert, posterior,
+               simulate)
 from . import model as M
 
 BACKGROUND = (math.log(3000.0), math.log(1500.0), math.log(2.3))
@@ -183,10 +184,18 @@ class Problem:
         ctx.sync()
 
     def step(self, realisations=None, stats=False, precision=PREC_MIXED):
-        """The plain per-request path: forward FFTs, posterior, realisations, inverse FFTs of the
-        means.  The realisations run while the means are still spectra, which the colouring adds
-        (one read per realisation pair, instead of one per realisation in the inverse x pass:
-        measured faster with paired colouring, slower when realisations were coloured singly)."""
+        """One request: cg_invert (forward transforms, posterior, inverse transforms of the means;
+        the split-z passes where the grid has them), then the realisations with the real-domain
+        posterior means (added in the inverse x pass).  Returns (inverted, failures, FP64
+        re-solves) when stats (a synchronisation), zeros otherwise."""
+        st = invert(self.ctx, self.item(realisations=0), precision=precision, cond_limit=1e6, stats=stats)
+        self.realisations(realisations)
+        return st if stats else (0, 0, 0)
+
+    def step_dropin(self, realisations=None, stats=False, precision=PREC_MIXED):
+        """The same request as the reference's call sequence, one drop-in call each: forward FFTs,
+        posterior, realisations, inverse FFTs of the means.  The realisations run while the means
+        are still spectra, which the colouring adds."""
         self.forward()
         st = self.posterior(stats=stats, precision=precision)
         self.realisations(realisations)
@@ -207,7 +216,8 @@ class Problem:
                          lam2d=self.rxy, want_residual=False, precision=precision, cond_limit=1e6, stats=stats)
 
     def realisations(self, realisations=None):
-        """Posterior realisations 0..R-1 from the spectral means; realisation r ends in
+        """Posterior realisations 0..R-1 from the posterior means (spectral or real-domain, as
+        the means are at the time); realisation r ends in
         self.out[r % len(self.out)].  Realisations beyond the kept sets are streamed through them;
         with two or more sets, consecutive realisations are coloured in pairs (one read of the
         covariance per pair)."""

@@ -408,6 +408,21 @@ int cg_batch_query(cg_batch *batch, int *done);
 int cg_batch_wait(cg_batch *batch);
 int cg_batch_destroy(cg_batch *batch);
 
+/* AVOInversion::computePostMeanResidAndFFTCov (avoinversion.cpp:229-244, 908-1114) followed by
+ * AVOInversion::simulate (:1401-1511), for one inversion in one call on the context stream:
+ * forward transforms of the real-domain inputs into post.mean[] / post.data[], the
+ * per-wavenumber posterior, inverse transforms of the posterior means, then inv->nreal
+ * realisations (counter noise, streams 3r..3r+2 for realisation r; consecutive pairs as
+ * cg_simulate_pair; the posterior means added in the real domain).  Where the grid has a compiled
+ * split nzp = 16 N2 for its x length and the call uses the separable prior with host tables
+ * (rxy, rt, lam2d) and no residual, the z transform runs in two stages carried by the x passes
+ * and the posterior pass (fewer passes over HBM, same results to FP32 rounding); otherwise the
+ * drop-in sequence (cg_fft_r2c, cg_posterior, cg_fft_c2r_inplace) runs.  On return post.mean[]
+ * hold the real-domain posterior means and post.cov[] the posterior covariance spectra;
+ * post.data[] are work grids whose contents are unspecified unless want_residual (then the
+ * residual spectra).  post.stats as for cg_posterior. */
+int cg_invert(cg_ctx *ctx, const cg_inversion *inv);
+
 /* ------------------------------------------------------------------ forward modelling */
 /* Synthetic seismic spectra d_l(k) = sum_j K_lj(kz) m_j(k) from Fourier-domain elastic grids
  * m[3], with the per-k_z operator K of cg_post_args (host table [nzp][ntheta][3] complex).

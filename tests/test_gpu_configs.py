@@ -146,7 +146,8 @@ def _sampled_planes(invert, n_inv=4, n_pass=3):
 
 
 def _check_cube(ctx, sp, label, rel=(0, 1)):
-    """The benchmark's call sequence on one cube at full size, checked on sampled planes."""
+    """The drop-in call sequence on one cube at full size, checked on sampled planes against the
+    FP64 oracle; then the benchmark's request (cg_invert) against it."""
     prob = S.Problem(ctx, sp)
     w = sp
     nxp, nyp, nzp, nt = w["nxp"], w["nyp"], w["nzp"], prob.nt
@@ -194,7 +195,7 @@ def _check_cube(ctx, sp, label, rel=(0, 1)):
         scale = np.trace(Sm, axis1=1, axis2=2).real + 1e-30
         worst_post = min(worst_post, float(np.min(np.linalg.eigvalsh(Sp)[:, 0] / scale)))
         worst_gain = min(worst_gain, float(np.min(np.linalg.eigvalsh(Sm - Sp)[:, 0] / scale)))
-    # realisations (the benchmark's sequence: coloured from the spectral means, then the means'
+    # realisations (the drop-in sequence: coloured from the spectral means, then the means'
     # inverse transforms)
     R = prob.R
     prob.realisations()
@@ -240,18 +241,38 @@ def _check_cube(ctx, sp, label, rel=(0, 1)):
         e_real = max(e_real, rel_l2(got, z))
         for g in range(3):  # benchmark realisation = posterior mean + colouring
             e_sum = max(e_sum, float(np.max(np.abs(bench_real[r][g] - gmean[g] - pure[g]))))
+    # the benchmark's request: cg_invert (the split-z passes at this geometry) and the
+    # realisations with the real-domain means, against the drop-in sequence above
+    prob.step()
+    ctx.sync()
+    e_split_mean = 0.0
+    for a in range(3):
+        sm = prob.mean_w[a].download_real()
+        bg = S.BACKGROUND[a]
+        e_split_mean = max(e_split_mean, rel_l2(sm - bg, gmean[a] - bg))
+        del sm
+    split_cov_same = bool(np.array_equal(np.stack([g.download_complex()[ks] for g in prob.cov]), cp))
+    e_split_real = 0.0
+    for r in last:
+        for g in range(3):
+            got = prob.out[r % len(prob.out)][g].download_real()
+            e_split_real = max(e_split_real, float(np.max(np.abs(got - bench_real[r][g]))))
+            del got
     del bench_real
     print("%s: inverted %d, failures %d, FP64 re-solves %d; forward %.2e, mean update %.2e, covariance %.2e, "
           "prior planes %.2e, min eig(Sigma_post)/tr %.1e, min eig(Sigma_m - Sigma_post)/tr %.1e, inverse means "
-          "%.2e, colouring of realisations %s %.2e, realisation - mean - colouring max %.2e"
+          "%.2e, colouring of realisations %s %.2e, realisation - mean - colouring max %.2e; split-z request vs "
+          "drop-in: mean update %.2e, covariance identical %s, realisations max %.2e"
           % (label, ninv, nfail, n64, e_fwd, e_upd, e_cov, e_prior, worst_post, worst_gain, e_inv, last, e_real,
-             e_sum), flush=True)
+             e_sum, e_split_mean, split_cov_same, e_split_real), flush=True)
     prob.close()
     _record(label, dict(vs="FP64 oracle on sampled planes %s" % ks, forward_spectra=e_fwd, mean_update=e_upd,
                         covariance=e_cov, prior_planes=e_prior, inverse_means=e_inv, realisation_colouring=e_real,
                         realisation_minus_mean_minus_colouring_max_abs=e_sum,
                         realisations_checked=last, min_eig_post_over_trace=worst_post,
-                        min_eig_prior_minus_post_over_trace=worst_gain, inverted=ninv, fp64_resolves=n64))
+                        min_eig_prior_minus_post_over_trace=worst_gain, inverted=ninv, fp64_resolves=n64,
+                        split_z_vs_dropin_mean_update=e_split_mean, split_z_covariance_identical=split_cov_same,
+                        split_z_vs_dropin_realisations_max_abs=e_split_real))
     assert onf == 0 and nfail == 0
     assert ninv == prob.n_inverted_planes * nyp * cnxp
     assert e_fwd < 5e-6
@@ -260,6 +281,7 @@ def _check_cube(ctx, sp, label, rel=(0, 1)):
     assert e_inv < 2e-5
     assert e_real < 1e-4
     assert e_sum < 2e-5  # ~20 units in the last place of the background ln Vp ~ 8
+    assert e_split_mean < 5e-5 and split_cov_same and e_split_real < 5e-5
     return dict(forward=e_fwd, mean_update=e_upd, covariance=e_cov, realisations=e_real)
 
 
