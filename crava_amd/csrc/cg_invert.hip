@@ -14,7 +14,7 @@
 //
 // k_posterior_z: tile = N1 consecutive planes (n1 + N1 k2, one k2) x 64 consecutive cells of a
 // plane.  The G grids' tile rows go to LDS; one thread per (grid, column) runs the length-N1
-// DFT along z in registers; then wave w of 8 takes planes k1 = w, w + 8, ... -- frequency
+// DFT along z in registers; then wave w of W takes planes k1 = w, w + W, ... -- frequency
 // k = k2 + N2 k1, one k per wave, so the per-plane tables are scalar loads as in k_posterior --
 // and the 64 lanes the 64 columns; the covariance goes straight to HBM at plane k (the natural
 // spectrum layout), the means back to LDS, and one thread per (mean, column) runs the inverse
@@ -22,10 +22,11 @@
 // update, as in k_posterior (avoinversion.cpp:944-1083).
 //
 // FP64 re-solves (CG_PREC_MIXED): a tile with a deferred wavenumber writes the covariance of its
-// other cells, leaves its means unwritten (the prior means stay in HBM, and the seismic grids
-// are only ever read) and is appended to a tile list; k_posterior_z<.., REDO> then runs each
-// listed tile again with the deferred cells in FP64.  Every tile is written by exactly one
-// workgroup, so the result does not depend on the list's order.
+// other cells, leaves the means of the columns holding one unwritten (the prior means stay in
+// HBM, and the seismic grids are only ever read) and is appended to a tile list;
+// k_posterior_z_redo then runs each listed tile again with the deferred cells in FP64.  Every
+// tile of the list is written by exactly one workgroup, so the result does not depend on the
+// list's order.
 #include "cg_fft_fixed.h"
 #include "cg_post_dev.h"
 
@@ -34,14 +35,32 @@ namespace cgpost {
 using cgdft::dft;
 using cgfix::lds_sync;
 
-constexpr int kZCols = 64;     // cells of a plane per tile (one per lane)
-constexpr int kZThreads = 512;  // 8 waves
+constexpr int kZCols = 64;  // cells of a plane per tile (one per lane)
 
-// One 64-column tile.  buf: [G][N1][64] float2 in LDS; s_defer: the workgroup's deferral flag.
-template <int NT, int N1, bool MIXED, bool REDO>
-__device__ __forceinline__ void post_z_tile(const cg::PostZArgs &z, unsigned t, float2 *buf, int *s_defer)
+// Waves per workgroup: the smallest divisor of N1 from 8 up (N1 itself below 8), so that every
+// wave takes N1 / W planes.
+constexpr int z_waves(int n1)
 {
-  constexpr int G = 3 + NT, TC = kZCols, NTH = kZThreads, W = NTH / 64;
+  if (n1 <= 8) return n1;
+  for (int w = 8; w < n1; ++w)
+    if (n1 % w == 0) return w;
+  return n1;
+}
+template <int N1>
+constexpr int z_threads()
+{
+  return 64 * z_waves(N1);
+}
+
+// Marks a deferred wavenumber's mean entries in LDS (a NaN with a payload no input carries): the
+// inverse stage-2 transform of its column is then not stored.
+constexpr unsigned kDeferMark = 0x7fc0defeu;
+
+// One 64-column tile.  buf: [G][N1][64] float2 in LDS.
+template <int NT, int N1, bool MIXED, bool REDO>
+__device__ __forceinline__ void post_z_tile(const cg::PostZArgs &z, unsigned t, float2 *buf)
+{
+  constexpr int G = 3 + NT, TC = kZCols, NTH = z_threads<N1>(), W = NTH / 64;
   static_assert((TC * N1) % NTH == 0, "a tile's rows of one grid must be whole rounds of the threads");
   constexpr int PER = G * N1 * TC / NTH;
   const cg::PostParams &p = z.p;
@@ -69,7 +88,6 @@ __device__ __forceinline__ void post_z_tile(const cg::PostZArgs &z, unsigned t, 
       buf[g * N1 * TC + e] = v[i];
     }
   }
-  if (MIXED && !REDO && tid == 0) *s_defer = 0;
   lds_sync();
   // 2. z stage 2, forward: one (grid, column) per thread
   for (int s = tid; s < G * TC; s += NTH) {
@@ -87,7 +105,6 @@ __device__ __forceinline__ void post_z_tile(const cg::PostZArgs &z, unsigned t, 
   const long cc = c0 + c;
   const bool active = cc < plane;
   const long ccs = active ? cc : 0;
-  bool any_defer = false;
 #pragma unroll 1
   for (int k1 = wv; k1 < N1; k1 += W) {
     const int k = (int)k2 + z.n2 * k1;  // wavefront-uniform
@@ -168,7 +185,11 @@ __device__ __forceinline__ void post_z_tile(const cg::PostZArgs &z, unsigned t, 
         for (int g = 0; g < 6; ++g) gst(p.cov[g] + idx, make_float2(P6[g], 0.f));
       }
     }
-    any_defer |= defer;
+    if (MIXED && !REDO && defer) {
+      const float mark = __uint_as_float(kDeferMark);
+#pragma unroll
+      for (int a = 0; a < 3; ++a) buf[(a * N1 + k1) * TC + c] = make_float2(mark, mark);
+    }
     if (p.stats) {
       if (!REDO) wave_count(p.stats, did_inv);
       // REDO counts only the re-solved cells' outcome; their tile's other cells were counted
@@ -180,22 +201,24 @@ __device__ __forceinline__ void post_z_tile(const cg::PostZArgs &z, unsigned t, 
       if (m && __lane_id() == 0) atomicAdd(p.list_len, (unsigned long long)__popcll(m));
     }
   }
-  if (MIXED && !REDO && any_defer) *s_defer = 1;
   lds_sync();
-  if (MIXED && !REDO && *s_defer) {
-    if (tid == 0) {
-      const unsigned long long slot = atomicAdd(z.redo_len, 1ull);
-      z.redo[slot] = t;
-    }
-    return;
-  }
   // 4. inverse z stage 2 of the means: one (mean, column) per thread, the column's N1 planes
-  // stored with lanes on consecutive cells
+  // stored with lanes on consecutive cells.  A column holding a deferred wavenumber keeps its
+  // prior means in HBM; the wave of mean 0 lists the tile for the FP64 pass.
   for (int s = tid; s < 3 * TC; s += NTH) {
     const int a = s / TC, cl = s % TC;
     float2 w[N1];
+    bool marked = false;
 #pragma unroll
-    for (int n = 0; n < N1; ++n) w[n] = buf[(a * N1 + n) * TC + cl];
+    for (int n = 0; n < N1; ++n) {
+      w[n] = buf[(a * N1 + n) * TC + cl];
+      if (MIXED && !REDO) marked |= __float_as_uint(w[n].x) == kDeferMark;
+    }
+    if (MIXED && !REDO) {
+      const unsigned long long m = __ballot(marked);
+      if (a == 0 && m && __lane_id() == 0) z.redo[atomicAdd(z.redo_len, 1ull)] = t;
+      if (marked) continue;
+    }
     dft<+1, N1>(w);
     if (c0 + cl < plane) {
       float2 *dst = p.mean[a] + (long)N1 * k2 * plane + c0 + cl;
@@ -206,23 +229,21 @@ __device__ __forceinline__ void post_z_tile(const cg::PostZArgs &z, unsigned t, 
 }
 
 template <int NT, int N1, bool MIXED>
-__global__ void __launch_bounds__(kZThreads) k_posterior_z(cg::PostZArgs z)
+__global__ void __launch_bounds__(z_threads<N1>()) k_posterior_z(cg::PostZArgs z)
 {
   __shared__ float2 buf[(3 + NT) * N1 * kZCols];
-  __shared__ int s_defer;
-  post_z_tile<NT, N1, MIXED, false>(z, cg_xcd_remap(blockIdx.x, gridDim.x), buf, &s_defer);
+  post_z_tile<NT, N1, MIXED, false>(z, cg_xcd_remap(blockIdx.x, gridDim.x), buf);
 }
 
 // The deferred tiles again, FP64 where the first pass deferred (workgroups stride over the list;
 // the loop bound is the same for every thread of a workgroup).
 template <int NT, int N1>
-__global__ void __launch_bounds__(kZThreads) k_posterior_z_redo(cg::PostZArgs z)
+__global__ void __launch_bounds__(z_threads<N1>()) k_posterior_z_redo(cg::PostZArgs z)
 {
   __shared__ float2 buf[(3 + NT) * N1 * kZCols];
-  __shared__ int s_defer;
   const unsigned long long n = *z.redo_len;
   for (unsigned long long i = blockIdx.x; i < n; i += gridDim.x) {
-    post_z_tile<NT, N1, true, true>(z, z.redo[i], buf, &s_defer);
+    post_z_tile<NT, N1, true, true>(z, z.redo[i], buf);
     lds_sync();  // the tile's LDS is read to the end before the next tile's rows arrive
   }
 }
@@ -231,14 +252,14 @@ template <int NT, int N1>
 int launch_z_nt(cg_ctx *ctx, const cg::PostZArgs &z, unsigned tiles, bool mixed)
 {
   if (mixed) {
-    k_posterior_z<NT, N1, true><<<tiles, kZThreads, 0, ctx->stream>>>(z);
+    k_posterior_z<NT, N1, true><<<tiles, z_threads<N1>(), 0, ctx->stream>>>(z);
     int rc = cg::hip_fail(ctx, hipGetLastError(), "k_posterior_z");
     if (rc) return rc;
     const unsigned blocks = tiles < 1024u ? tiles : 1024u;
-    k_posterior_z_redo<NT, N1><<<blocks, kZThreads, 0, ctx->stream>>>(z);
+    k_posterior_z_redo<NT, N1><<<blocks, z_threads<N1>(), 0, ctx->stream>>>(z);
     return cg::hip_fail(ctx, hipGetLastError(), "k_posterior_z_redo");
   }
-  k_posterior_z<NT, N1, false><<<tiles, kZThreads, 0, ctx->stream>>>(z);
+  k_posterior_z<NT, N1, false><<<tiles, z_threads<N1>(), 0, ctx->stream>>>(z);
   return cg::hip_fail(ctx, hipGetLastError(), "k_posterior_z");
 }
 
@@ -251,6 +272,8 @@ struct XZEntry {
   int n, n1, n2;
   int (*launch)(cg_ctx *, int, const cgfix::XZArgs &, int, const float2 *, const float2 *, const float2 *);
 };
+// (nzp 320 was also measured split 20 x 16: the posterior pass at 20 planes per tile holds one
+// workgroup per CU and took 1.87 ms against 1.56 ms, the 16-row x passes 224 us against 215 us.)
 const XZEntry kXZ[] = {
     {288, 16, 20, &cgfix::xz_launch<288, 20>},
     {576, 16, 24, &cgfix::xz_launch<576, 24>},
@@ -294,6 +317,21 @@ int fft_xz(cg_ctx *ctx, int forward, const float *src, float *grid, const float 
   return rc;
 }
 
+template <int N1>
+static int launch_z_n1(cg_ctx *ctx, const PostZArgs &z, unsigned tiles, bool mixed)
+{
+  switch (z.p.nt) {
+    case 1: return cgpost::launch_z_nt<1, N1>(ctx, z, tiles, mixed);
+    case 2: return cgpost::launch_z_nt<2, N1>(ctx, z, tiles, mixed);
+    case 3: return cgpost::launch_z_nt<3, N1>(ctx, z, tiles, mixed);
+    case 4: return cgpost::launch_z_nt<4, N1>(ctx, z, tiles, mixed);
+    case 5: return cgpost::launch_z_nt<5, N1>(ctx, z, tiles, mixed);
+    case 6: return cgpost::launch_z_nt<6, N1>(ctx, z, tiles, mixed);
+    case 7: return cgpost::launch_z_nt<7, N1>(ctx, z, tiles, mixed);
+    default: return cgpost::launch_z_nt<8, N1>(ctx, z, tiles, mixed);
+  }
+}
+
 int launch_posterior_z(cg_ctx *ctx, const PostZArgs &z, int n1, bool mixed)
 {
   const long plane = (long)z.p.cnxp * z.p.nyp;
@@ -301,17 +339,7 @@ int launch_posterior_z(cg_ctx *ctx, const PostZArgs &z, int n1, bool mixed)
   if (tiles >= (1ul << 31) || n1 != 16) return fail(ctx, CG_EINVAL, "split-z posterior: unsupported split");
   hipEvent_t ev;
   launch_begin(ctx, CG_K_POSTERIOR, &ev);
-  int rc;
-  switch (z.p.nt) {
-    case 1: rc = cgpost::launch_z_nt<1, 16>(ctx, z, (unsigned)tiles, mixed); break;
-    case 2: rc = cgpost::launch_z_nt<2, 16>(ctx, z, (unsigned)tiles, mixed); break;
-    case 3: rc = cgpost::launch_z_nt<3, 16>(ctx, z, (unsigned)tiles, mixed); break;
-    case 4: rc = cgpost::launch_z_nt<4, 16>(ctx, z, (unsigned)tiles, mixed); break;
-    case 5: rc = cgpost::launch_z_nt<5, 16>(ctx, z, (unsigned)tiles, mixed); break;
-    case 6: rc = cgpost::launch_z_nt<6, 16>(ctx, z, (unsigned)tiles, mixed); break;
-    case 7: rc = cgpost::launch_z_nt<7, 16>(ctx, z, (unsigned)tiles, mixed); break;
-    default: rc = cgpost::launch_z_nt<8, 16>(ctx, z, (unsigned)tiles, mixed); break;
-  }
+  const int rc = launch_z_n1<16>(ctx, z, (unsigned)tiles, mixed);
   launch_end(ctx, CG_K_POSTERIOR, ev);
   return rc;
 }
