@@ -1387,10 +1387,20 @@ static int simulate_impl(cg_ctx *ctx, const cg_sim_args *a, cg_grid *const *next
   if (p.fails) rc = cg::zero_counters(ctx, p.fails, 1);
   if (rc) return rc;
   // Counter noise with a compiled z plan: the colouring runs inside the inverse z pass, so the
-  // coloured spectra never make a round trip through HBM.
-  bool z_done = false;
+  // coloured spectra never make a round trip through HBM.  Where the grid has a compiled split
+  // of the z transform, the colouring carries the inverse's stage 2 and the x pass its stage 1.
+  bool z_done = false, split = false;
   const bool fusable = !p.from_grids && ctx->fft_engine == CG_FFT_AUTO && p.nzp > 1;
-  if (next) {
+  int sn1 = 0, sn2 = 0;
+  if (fusable && ctx->fused_colour != 0 && cg::split_z_supported(ref->nxp, ref->nzp, &sn1, &sn2)) {
+    if (next) {
+      for (int g = 0; g < 3; ++g) p.out2[g] = reinterpret_cast<float2 *>(next[g]->d);
+      p.stream1 = 3u * (a->realisation + 1u);
+    }
+    rc = cg::colour_split(ctx, p, sn1, sn2, next != nullptr);
+    if (rc) return rc;
+    split = true;
+  } else if (next) {
     // both realisations in one launch, or one after the other
     for (int g = 0; g < 3; ++g) p.out2[g] = reinterpret_cast<float2 *>(next[g]->d);
     p.stream1 = 3u * (a->realisation + 1u);
@@ -1423,7 +1433,17 @@ static int simulate_impl(cg_ctx *ctx, const cg_sim_args *a, cg_grid *const *next
       o->cubetype = CG_CT_PARAMETER;
       o->transformed = 1;
       const float *add = spectral == 0 && a->mean[g] ? a->mean[g]->d : nullptr;
-      rc = z_done ? cg::grid_fft_inverse_xy(o, add) : cg::grid_fft_inverse(o, add);
+      if (split) {  // y pass, then the inverse stage 1 with the x c2r (PARAMETER: unitary scale)
+        const long plane = (long)o->cnxp * o->nyp;
+        if (o->nyp > 1)
+          rc = cg::fft_strided(ctx, reinterpret_cast<float2 *>(o->d), o->nyp, o->cnxp, o->cnxp, o->nzp, plane, +1,
+                               1.0f, CG_K_FFT_Y);
+        const size_t n = (size_t)o->nxp * o->nyp * o->nzp;
+        const float sc = float(1.0 / sqrtf(float(n)));  // as grid_fft_inverse
+        if (!rc) rc = cg::fft_xz(ctx, 0, nullptr, o->d, add, o->nxp, o->nyp, o->nzp, sc);
+      } else {
+        rc = z_done ? cg::grid_fft_inverse_xy(o, add) : cg::grid_fft_inverse(o, add);
+      }
       if (rc) return rc;
       o->transformed = 0;
     }
@@ -1541,8 +1561,10 @@ int cg_invert(cg_ctx *ctx, const cg_inversion *v)
       }
       list = ctx->list;
     }
-    const float N = (float)nxp * (float)nyp * (float)nzp;
-    const float s_fwd = 1.0f / sqrtf(N);
+    // the scales of grid_fft_forward / grid_fft_inverse
+    const long N = (long)nxp * nyp * nzp;
+    const float s_fwd = 1.0f / sqrtf((float)N);
+    const float s_inv = float(1.0 / sqrtf(float((size_t)N)));
     for (int i = 0; i < 3 + nt && !rc; ++i) {
       const cg_grid *in = input(i);
       cg_grid *w = work[i];
@@ -1582,16 +1604,53 @@ int cg_invert(cg_ctx *ctx, const cg_inversion *v)
       a.cov[g]->transformed = 1;
       a.cov[g]->cubetype = CG_CT_COVARIANCE;
     }
+    // the realisations while the means are still in the split layout: the colouring adds them
+    // there, and the realisations' inverse passes read nothing else
+    if (v->nreal > 0) {
+      cg::ColourParams cp;
+      std::memset(&cp, 0, sizeof(cp));
+      cp.csize = (long)ref->csize;
+      cp.cnxp = cnxp;
+      cp.nyp = nyp;
+      cp.nzp = nzp;
+      cg::colour_dims(cp);
+      for (int g = 0; g < 6; ++g) cp.cov[g] = reinterpret_cast<const float2 *>(a.cov[g]->d);
+      cp.seed = v->seed;
+      const float2 *ms[3];
+      for (int g = 0; g < 3; ++g) ms[g] = reinterpret_cast<const float2 *>(a.mean[g]->d);
+      for (int r = 0; r < v->nreal && !rc; r += 2) {
+        const bool pair = r + 1 < v->nreal;
+        for (int g = 0; g < 3; ++g) {
+          cp.out[g] = reinterpret_cast<float2 *>(v->out[3 * r + g]->d);
+          cp.out2[g] = pair ? reinterpret_cast<float2 *>(v->out[3 * (r + 1) + g]->d) : nullptr;
+        }
+        cp.stream0 = 3u * (uint32_t)r;
+        cp.stream1 = 3u * (uint32_t)(r + 1);
+        rc = cg::colour_split(ctx, cp, n1, n2, pair, ms);
+        for (int q = r; q < (pair ? r + 2 : r + 1) && !rc; ++q) {
+          for (int g = 0; g < 3 && !rc; ++g) {
+            cg_grid *o = v->out[3 * q + g];
+            o->cubetype = CG_CT_PARAMETER;
+            if (nyp > 1)
+              rc = cg::fft_strided(ctx, reinterpret_cast<float2 *>(o->d), nyp, cnxp, cnxp, nzp, (long)plane, +1, 1.0f,
+                                   CG_K_FFT_Y);
+            if (!rc) rc = cg::fft_xz(ctx, 0, nullptr, o->d, nullptr, nxp, nyp, nzp, s_inv);
+            o->transformed = 0;
+          }
+        }
+      }
+      if (rc) return rc;
+    }
     for (int g = 0; g < 3 && !rc; ++g) {
       float2 *c = reinterpret_cast<float2 *>(a.mean[g]->d);
       if (nyp > 1) rc = cg::fft_strided(ctx, c, nyp, cnxp, cnxp, nzp, (long)plane, +1, 1.0f, CG_K_FFT_Y);
-      const float s = a.mean[g]->cubetype == CG_CT_COVARIANCE ? float(1.0 / (double)N) : s_fwd;
+      const float s = a.mean[g]->cubetype == CG_CT_COVARIANCE ? float(1.0 / (double)N) : s_inv;
       if (!rc) rc = cg::fft_xz(ctx, 0, nullptr, a.mean[g]->d, nullptr, nxp, nyp, nzp, s);
       a.mean[g]->transformed = 0;
     }
     if (!rc && a.stats) rc = post_stats(ctx, cnt, mixed, a.stats);
   }
-  if (rc) return rc;
+  if (rc || split) return rc;
   // realisations: pairs (2m, 2m + 1) as cg_simulate_pair, a last odd one alone; the posterior
   // means are added in the real domain
   for (int r = 0; r < v->nreal && !rc; r += 2) {

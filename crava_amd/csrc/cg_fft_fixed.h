@@ -1347,6 +1347,15 @@ k_fftxz_inv(XZArgs a, const float2 *__restrict__ twg, const float2 *__restrict__
       if (f < N2 * CN) buf[m * ES + r] = v[i];
     }
   }
+  float2 va[ADD ? PER : 1];
+  if (ADD) {  // the added rows' loads in flight during the transforms
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int f = tid + i * NT;
+      const int r = f / CN, m = f - r * CN;
+      va[ADD ? i : 0] = f < N2 * CN ? cg::gld(ad + r * pitch + m) : make_float2(0.f, 0.f);
+    }
+  }
   lds_sync();
   // inverse stage 1: row k2 times w^-(n1 k2), then the inverse DFT over k2 into rows n2
   for (int m = tid; m < CN; m += NT) {
@@ -1375,15 +1384,6 @@ k_fftxz_inv(XZArgs a, const float2 *__restrict__ twg, const float2 *__restrict__
   }
   lds_sync();
   Plan<N>::template run<+1, ES>(buf + tid % N2, tid / N2, s_tw);
-  float2 va[ADD ? PER : 1];
-  if (ADD) {
-#pragma unroll
-    for (int i = 0; i < PER; ++i) {
-      const int f = tid + i * NT;
-      const int r = f / CN, m = f - r * CN;
-      va[ADD ? i : 0] = f < N2 * CN ? cg::gld(ad + r * pitch + m) : make_float2(0.f, 0.f);
-    }
-  }
 #pragma unroll
   for (int i = 0; i < PER; ++i) {
     const int f = tid + i * NT;

@@ -28,6 +28,7 @@
 // tile of the list is written by exactly one workgroup, so the result does not depend on the
 // list's order.
 #include "cg_fft_fixed.h"
+#include "cg_noise.h"
 #include "cg_post_dev.h"
 
 namespace cgpost {
@@ -263,6 +264,81 @@ int launch_z_nt(cg_ctx *ctx, const cg::PostZArgs &z, unsigned tiles, bool mixed)
   return cg::hip_fail(ctx, hipGetLastError(), "k_posterior_z");
 }
 
+// Realisations with the split z transform (cg_simulate / cg_simulate_pair on such grids):
+// colouring of the N1 planes k2 + N2 k1 (one k2) x 64 cells into LDS -- wave w takes
+// k1 = w, w + W, ...; the same per-cell functions as the unsplit colouring kernels, so the noise
+// and the coloured values are those of any other path -- then the inverse stage 2 over k1 into
+// planes n1 + N1 k2 of the output grids.  The y pass and the inverse stage 1 with the x c2r (and
+// the real-domain mean) follow.  PAIR: realisations stream0 and stream1, out and out2.
+// mean_split: optional posterior means in the same split layout (the posterior pass's output
+// before the means' own inverse y and x passes), added at the store.
+struct SplitMeans {
+  const float2 *m[3];
+};
+
+template <int N1, bool PAIR, bool THREE>
+__global__ void __launch_bounds__(z_threads<N1>())
+k_colour_z2(cg::ColourParams p, int n2, unsigned ctiles, SplitMeans mean_split)
+{
+  constexpr int TC = kZCols, NTH = z_threads<N1>(), W = NTH / 64, S = PAIR ? 6 : 3;
+  __shared__ float2 buf[S * N1 * TC];
+  const int tid = threadIdx.x;
+  const unsigned t = cg_xcd_remap(blockIdx.x, gridDim.x);
+  const unsigned k2 = t / ctiles, ct = t % ctiles;
+  const long plane = (long)p.cnxp * p.nyp;
+  const long c0 = (long)ct * TC;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6), c = tid & 63;
+  const bool active = c0 + c < plane;
+#pragma unroll 1
+  for (int k1 = wv; k1 < N1; k1 += W) {
+    const long idx = (long)((int)k2 + n2 * k1) * plane + (active ? c0 + c : 0);
+    float2 o[3] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f), make_float2(0.f, 0.f)};
+    float2 o2[3] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f), make_float2(0.f, 0.f)};
+    bool failed = false;
+    if (active) {
+      if (PAIR) {
+        failed = cgcol::colour_cell2t<THREE>(p, idx, o, o2);
+      } else {
+        float2 zn[3];
+        failed = cgcol::colour_cell(p, idx, zn, o);
+      }
+    }
+    if (p.fails) {  // one atomic per wavefront; a failed cell zeroes the noise of both of a pair
+      const unsigned long long msk = __ballot(failed);
+      if (failed && (int)__lane_id() == __ffsll((long long)msk) - 1)
+        atomicAdd(p.fails, (PAIR ? 2ull : 1ull) * (unsigned long long)__popcll(msk));
+    }
+#pragma unroll
+    for (int g = 0; g < 3; ++g) {
+      buf[(g * N1 + k1) * TC + c] = o[g];
+      if (PAIR) buf[((g + 3) * N1 + k1) * TC + c] = o2[g];
+    }
+  }
+  lds_sync();
+  for (int s = tid; s < S * TC; s += NTH) {
+    const int g = s / TC, cl = s % TC;
+    float2 w[N1];
+#pragma unroll
+    for (int n = 0; n < N1; ++n) w[n] = buf[(g * N1 + n) * TC + cl];
+    dft<+1, N1>(w);
+    if (c0 + cl < plane) {
+      const long off = (long)N1 * k2 * plane + c0 + cl;
+      float2 *dst = (g < 3 ? p.out[g] : p.out2[g - 3]) + off;
+      const float2 *ms = mean_split.m[g < 3 ? g : g - 3];
+      if (ms) {
+#pragma unroll
+        for (int n = 0; n < N1; ++n) {
+          const float2 m = gld(ms + off + (long)n * plane);
+          w[n].x += m.x;
+          w[n].y += m.y;
+        }
+      }
+#pragma unroll
+      for (int n = 0; n < N1; ++n) gst(dst + (long)n * plane, w[n]);
+    }
+  }
+}
+
 }  // namespace cgpost
 
 namespace {
@@ -342,6 +418,28 @@ int launch_posterior_z(cg_ctx *ctx, const PostZArgs &z, int n1, bool mixed)
   const int rc = launch_z_n1<16>(ctx, z, (unsigned)tiles, mixed);
   launch_end(ctx, CG_K_POSTERIOR, ev);
   return rc;
+}
+
+
+int colour_split(cg_ctx *ctx, const ColourParams &p, int n1, int n2, bool pair, const float2 *const *mean_split)
+{
+  cgpost::SplitMeans ms{{nullptr, nullptr, nullptr}};
+  for (int g = 0; g < 3 && mean_split; ++g) ms.m[g] = mean_split[g];
+  const long plane = (long)p.cnxp * p.nyp;
+  const unsigned long ctiles = (unsigned long)((plane + cgpost::kZCols - 1) / cgpost::kZCols);
+  const unsigned long tiles = ctiles * (unsigned long)n2;
+  if (tiles >= (1ul << 31) || n1 != 16) return fail(ctx, CG_EINVAL, "split-z colouring: unsupported split");
+  constexpr int NTH = cgpost::z_threads<16>();
+  hipEvent_t ev;
+  launch_begin(ctx, CG_K_NOISE, &ev);
+  if (!pair)
+    cgpost::k_colour_z2<16, false, false><<<(unsigned)tiles, NTH, 0, ctx->stream>>>(p, n2, (unsigned)ctiles, ms);
+  else if (cgcol::pair_in_three_blocks(p))
+    cgpost::k_colour_z2<16, true, true><<<(unsigned)tiles, NTH, 0, ctx->stream>>>(p, n2, (unsigned)ctiles, ms);
+  else
+    cgpost::k_colour_z2<16, true, false><<<(unsigned)tiles, NTH, 0, ctx->stream>>>(p, n2, (unsigned)ctiles, ms);
+  launch_end(ctx, CG_K_NOISE, ev);
+  return hip_fail(ctx, hipGetLastError(), "k_colour_z2");
 }
 
 }  // namespace cg

@@ -75,6 +75,12 @@ bool split_z_supported(int nxp, int nzp, int *n1, int *n2);
 // (grid in place, + add).
 int fft_xz(cg_ctx *ctx, int forward, const float *src, float *grid, const float *add, int nxp, int nyp, int nzp,
            float scale);
+struct ColourParams;
+// Colouring (one realisation, or a pair: out2 / stream1) with the inverse z transform's stage 2,
+// into the output grids' split layout (cg_invert.hip); the y pass and fft_xz follow.  mean_split
+// (optional, [3]): posterior means in that layout, added.
+int colour_split(cg_ctx *ctx, const ColourParams &p, int n1, int n2, bool pair,
+                 const float2 *const *mean_split = nullptr);
 // The allocation-free variant's fixed device buffer (cg_posterior_noalloc): one per device, a
 // file-scope array of kNoAllocBytes; noalloc_scratch_init looks up its address on the current
 // device once (at context creation), noalloc_scratch returns it (null before the lookup or if it

@@ -184,12 +184,16 @@ class Problem:
         ctx.sync()
 
     def step(self, realisations=None, stats=False, precision=PREC_MIXED):
-        """One request: cg_invert (forward transforms, posterior, inverse transforms of the means;
-        the split-z passes where the grid has them), then the realisations with the real-domain
-        posterior means (added in the inverse x pass).  Returns (inverted, failures, FP64
-        re-solves) when stats (a synchronisation), zeros otherwise."""
-        st = invert(self.ctx, self.item(realisations=0), precision=precision, cond_limit=1e6, stats=stats)
-        self.realisations(realisations)
+        """One request: cg_invert (forward transforms, posterior, realisations, inverse transforms
+        of the means; the split-z passes where the grid has them).  Realisations beyond the kept
+        output sets are streamed through them afterwards, with the real-domain means.  Returns
+        (inverted, failures, FP64 re-solves) when stats (a synchronisation), zeros otherwise."""
+        R = self.R if realisations is None else realisations
+        if R <= len(self.out):  # the realisations inside cg_invert (one output set each)
+            st = invert(self.ctx, self.item(realisations=R), precision=precision, cond_limit=1e6, stats=stats)
+        else:  # more realisations than output sets: streamed through them afterwards
+            st = invert(self.ctx, self.item(realisations=0), precision=precision, cond_limit=1e6, stats=stats)
+            self.realisations(R)
         return st if stats else (0, 0, 0)
 
     def step_dropin(self, realisations=None, stats=False, precision=PREC_MIXED):
