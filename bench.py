@@ -45,9 +45,9 @@ def kernel_bytes(w, nt, R, n_inv_planes, launches, split):
     launch transforms one grid (the realisations' inverse z passes run inside the fused colouring
     and are not z launches).  split: the request ran the split-z passes (cg_invert) -- the x
     passes carry the z stages, the posterior pass reads and writes whole grids, and the
-    realisations take the real-domain means in their inverse x passes; otherwise the drop-in
-    sequence (the posterior reads the spectra of inverted planes only, the colouring adds the
-    spectral means)."""
+    colouring adds the means in their split layout; otherwise the drop-in sequence (the
+    posterior reads the spectra of inverted planes only, the colouring adds the spectral
+    means).  Either way a pair of realisations reads the means once."""
     G = (w["nxp"] // 2 + 1) * w["nyp"] * w["nzp"] * 8  # one complex grid
     Nr = w["nxp"] * w["nyp"] * w["nzp"] * 4  # one real cube (without row padding)
     plane = (w["nxp"] // 2 + 1) * w["nyp"]
@@ -56,9 +56,8 @@ def kernel_bytes(w, nt, R, n_inv_planes, launches, split):
     else:
         post = n_inv_planes * plane * (3 + nt + 9) * 8 + (w["nzp"] - n_inv_planes) * plane * 6 * 8
     return {
-        # x: forward reads the real cube and writes the spectrum, inverse the reverse, plus the
-        # real-domain mean each realisation grid adds (split)
-        C.K_FFT_X: launches[C.K_FFT_X] * (Nr + G) + (3 * R * Nr if split else 0),
+        # x: forward reads the real cube and writes the spectrum, inverse the reverse
+        C.K_FFT_X: launches[C.K_FFT_X] * (Nr + G),
         # y / z: one grid read and written per launch
         C.K_FFT_Y: launches[C.K_FFT_Y] * 2 * G,
         C.K_FFT_Z: launches[C.K_FFT_Z] * 2 * G,
@@ -66,9 +65,9 @@ def kernel_bytes(w, nt, R, n_inv_planes, launches, split):
         # 3 + nt spectra and write 3 means + 6 covariance entries, pass-through cells write the 6
         # prior covariance entries
         C.K_POSTERIOR: post,
-        # colouring: read 6 covariance spectra (+ 3 mean spectra, drop-in), write 3 per realisation
-        # (noise generated in registers); pairs read their inputs once
-        C.K_NOISE: (R // 2) * ((12 if split else 15) * G) + (R % 2) * ((9 if split else 12) * G),
+        # colouring: read 6 covariance and 3 mean spectra, write 3 per realisation (noise
+        # generated in registers); pairs read their inputs once
+        C.K_NOISE: (R // 2) * 15 * G + (R % 2) * 12 * G,
     }
 
 

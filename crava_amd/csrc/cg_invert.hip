@@ -270,18 +270,22 @@ int launch_z_nt(cg_ctx *ctx, const cg::PostZArgs &z, unsigned tiles, bool mixed)
 // and the coloured values are those of any other path -- then the inverse stage 2 over k1 into
 // planes n1 + N1 k2 of the output grids.  The y pass and the inverse stage 1 with the x c2r (and
 // the real-domain mean) follow.  PAIR: realisations stream0 and stream1, out and out2.
-// mean_split: optional posterior means in the same split layout (the posterior pass's output
-// before the means' own inverse y and x passes), added at the store.
+// MEAN: posterior means in the same split layout (the posterior pass's output before the means'
+// own inverse y and x passes), mean_split, are added at the store; their tile rows are requested
+// at the start and wait in LDS, so the reads overlap the colouring.
 struct SplitMeans {
   const float2 *m[3];
 };
 
-template <int N1, bool PAIR, bool THREE>
+template <int N1, bool PAIR, bool THREE, bool MEAN>
 __global__ void __launch_bounds__(z_threads<N1>())
 k_colour_z2(cg::ColourParams p, int n2, unsigned ctiles, SplitMeans mean_split)
 {
   constexpr int TC = kZCols, NTH = z_threads<N1>(), W = NTH / 64, S = PAIR ? 6 : 3;
+  static_assert((3 * N1 * TC) % NTH == 0, "the means' tile rows must be whole rounds of the threads");
+  constexpr int MPER = 3 * N1 * TC / NTH;
   __shared__ float2 buf[S * N1 * TC];
+  __shared__ float2 mbuf[MEAN ? 3 * N1 * TC : 1];  // [3][N1][64]: the means' tile rows
   const int tid = threadIdx.x;
   const unsigned t = cg_xcd_remap(blockIdx.x, gridDim.x);
   const unsigned k2 = t / ctiles, ct = t % ctiles;
@@ -289,6 +293,17 @@ k_colour_z2(cg::ColourParams p, int n2, unsigned ctiles, SplitMeans mean_split)
   const long c0 = (long)ct * TC;
   const int wv = __builtin_amdgcn_readfirstlane(tid >> 6), c = tid & 63;
   const bool active = c0 + c < plane;
+  // the means' rows (planes n1 + N1 k2) requested first, in flight during the colouring
+  float2 mv[MEAN ? MPER : 1];
+  if (MEAN) {
+#pragma unroll
+    for (int i = 0; i < MPER; ++i) {
+      const int e = i * NTH + tid;
+      const int g = e / (N1 * TC), n1 = (e / TC) % N1, cl = e % TC;
+      mv[MEAN ? i : 0] = c0 + cl < plane ? gld(mean_split.m[g] + (long)(n1 + N1 * (long)k2) * plane + c0 + cl)
+                                        : make_float2(0.f, 0.f);
+    }
+  }
 #pragma unroll 1
   for (int k1 = wv; k1 < N1; k1 += W) {
     const long idx = (long)((int)k2 + n2 * k1) * plane + (active ? c0 + c : 0);
@@ -314,27 +329,29 @@ k_colour_z2(cg::ColourParams p, int n2, unsigned ctiles, SplitMeans mean_split)
       if (PAIR) buf[((g + 3) * N1 + k1) * TC + c] = o2[g];
     }
   }
+  if (MEAN) {
+#pragma unroll
+    for (int i = 0; i < MPER; ++i) mbuf[i * NTH + tid] = mv[MEAN ? i : 0];
+  }
   lds_sync();
+  // inverse stage 2 of the coloured columns (+ the split-layout means)
   for (int s = tid; s < S * TC; s += NTH) {
-    const int g = s / TC, cl = s % TC;
+    const int g = s / TC, cl = s % TC, gm = g < 3 ? g : g - 3;
     float2 w[N1];
 #pragma unroll
     for (int n = 0; n < N1; ++n) w[n] = buf[(g * N1 + n) * TC + cl];
     dft<+1, N1>(w);
     if (c0 + cl < plane) {
-      const long off = (long)N1 * k2 * plane + c0 + cl;
-      float2 *dst = (g < 3 ? p.out[g] : p.out2[g - 3]) + off;
-      const float2 *ms = mean_split.m[g < 3 ? g : g - 3];
-      if (ms) {
+      float2 *dst = (g < 3 ? p.out[g] : p.out2[g - 3]) + (long)N1 * k2 * plane + c0 + cl;
 #pragma unroll
-        for (int n = 0; n < N1; ++n) {
-          const float2 m = gld(ms + off + (long)n * plane);
+      for (int n = 0; n < N1; ++n) {
+        if (MEAN) {
+          const float2 m = mbuf[(gm * N1 + n) * TC + cl];
           w[n].x += m.x;
           w[n].y += m.y;
         }
+        gst(dst + (long)n * plane, w[n]);
       }
-#pragma unroll
-      for (int n = 0; n < N1; ++n) gst(dst + (long)n * plane, w[n]);
     }
   }
 }
@@ -432,12 +449,17 @@ int colour_split(cg_ctx *ctx, const ColourParams &p, int n1, int n2, bool pair, 
   constexpr int NTH = cgpost::z_threads<16>();
   hipEvent_t ev;
   launch_begin(ctx, CG_K_NOISE, &ev);
-  if (!pair)
-    cgpost::k_colour_z2<16, false, false><<<(unsigned)tiles, NTH, 0, ctx->stream>>>(p, n2, (unsigned)ctiles, ms);
-  else if (cgcol::pair_in_three_blocks(p))
-    cgpost::k_colour_z2<16, true, true><<<(unsigned)tiles, NTH, 0, ctx->stream>>>(p, n2, (unsigned)ctiles, ms);
-  else
-    cgpost::k_colour_z2<16, true, false><<<(unsigned)tiles, NTH, 0, ctx->stream>>>(p, n2, (unsigned)ctiles, ms);
+  const unsigned nb = (unsigned)tiles, ctl = (unsigned)ctiles;
+  const bool three = pair && cgcol::pair_in_three_blocks(p);
+  if (ms.m[0]) {
+    if (!pair) cgpost::k_colour_z2<16, false, false, true><<<nb, NTH, 0, ctx->stream>>>(p, n2, ctl, ms);
+    else if (three) cgpost::k_colour_z2<16, true, true, true><<<nb, NTH, 0, ctx->stream>>>(p, n2, ctl, ms);
+    else cgpost::k_colour_z2<16, true, false, true><<<nb, NTH, 0, ctx->stream>>>(p, n2, ctl, ms);
+  } else {
+    if (!pair) cgpost::k_colour_z2<16, false, false, false><<<nb, NTH, 0, ctx->stream>>>(p, n2, ctl, ms);
+    else if (three) cgpost::k_colour_z2<16, true, true, false><<<nb, NTH, 0, ctx->stream>>>(p, n2, ctl, ms);
+    else cgpost::k_colour_z2<16, true, false, false><<<nb, NTH, 0, ctx->stream>>>(p, n2, ctl, ms);
+  }
   launch_end(ctx, CG_K_NOISE, ev);
   return hip_fail(ctx, hipGetLastError(), "k_colour_z2");
 }
