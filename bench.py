@@ -336,8 +336,8 @@ def bench_input_padding(ctx, inv, w, reps=5):
 def bench_large(ctx, steps=2):
     """The large single cube with simulation: 1024 x 1024 x 320 padded to 1152 x 1152 x 384, 6
     angles 0-50 deg, Ormsby 5-10-60-80 Hz wavelet, 1500 m lateral range, 20 posterior realisations
-    (streamed through two sets of three output grids, coloured in pairs), covariance kept as the 6
-    Hermitian spectra."""
+    (all resident, coloured in pairs inside cg_invert), covariance kept as the 6 Hermitian
+    spectra."""
     sp = S.large_cube_spec(realisations=20)
     t0 = time.perf_counter()
     prob = S.Problem(ctx, sp)

@@ -56,10 +56,10 @@ def single_cube_spec(realisations=2):
 def large_cube_spec(realisations=20):
     """The large cube with simulation: 1024 x 1024 x 320 padded to 1152 x 1152 x 384, 6 angles
     0-50 deg, an Ormsby 5-10-60-80 Hz wavelet, 1500 m lateral range, 20 posterior realisations
-    streamed through two sets of output grids (coloured in pairs)."""
+    (coloured in pairs), every one kept resident: 84 grids of 2.04 GB, 171 GB of HBM."""
     return spec(1024, 1024, 320, tuple(float(a) for a in range(0, 60, 10)), 25.0, seed=20261205, nxp=1152,
                 nyp=1152, nzp=384, lateral_range=1500.0, ormsby_hz=(5.0, 10.0, 60.0, 80.0),
-                realisations=realisations, out_sets=2)
+                realisations=realisations)
 
 
 def cells(s):

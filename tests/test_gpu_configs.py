@@ -5,11 +5,12 @@
   against the FP64 oracle, and against the reference's own FP32 pipeline (FFTW 2.1.2 and lib_matr
   compiled from the CRAVA tree) where that library was built;
 * production cube, 512 x 512 x 256 (576 x 576 x 320), 5 angles, 2 realisations, and the large cube,
-  1024 x 1024 x 320 (1152 x 1152 x 384), 6 angles, 20 realisations: the benchmark's own call
-  sequence at full size, checked on sampled k and z planes -- forward spectra against FP64
-  transforms of the resident inputs, the posterior against the FP64 oracle on the GPU's own
-  spectra, pass-through planes equal to the prior, Sigma_post <= Sigma_m, the inverse transform
-  of the means, and realisations against the FP64 colouring of the same counter noise;
+  1024 x 1024 x 320 (1152 x 1152 x 384), 6 angles, 20 realisations: the drop-in call sequence at
+  full size, checked on sampled k and z planes -- forward spectra against FP64 transforms of the
+  resident inputs, the posterior against the FP64 oracle on the GPU's own spectra, pass-through
+  planes equal to the prior, Sigma_post <= Sigma_m, the inverse transform of the means, and
+  realisations against the FP64 colouring of the same counter noise -- then the benchmark's
+  request (cg_invert, split z) against the drop-in sequence;
 * the 64-inversion uneven mix: one batch against the plain per-request path, and sampled cubes
   against the FP64 oracle;
 * served requests at the serving sizes: each equal alone and in the mix, sampled requests
@@ -208,12 +209,12 @@ def _check_cube(ctx, sp, label, rel=(0, 1)):
         bg = S.BACKGROUND[a]
         e_inv = max(e_inv, rel_l2(gmean[a][zs] - bg, want - bg))
     del mpost_full
-    # the last realisations in each output set: r = R - 2 and R - 1 (or 0 alone).  They hold
+    # the last two realisations: r = R - 2 and R - 1 (or 0 alone).  They hold
     # mean + colouring, whose FP32 rounding of the background (~8) is far above the coloured noise at
     # well-resolved wavenumbers; so the colouring is checked on its own (the same pair of
     # realisations coloured again without the means) against the FP64 oracle, and the benchmark's
     # realisations against mean + that colouring
-    last = [r for r in range(R) if r >= R - len(prob.out)]
+    last = [r for r in range(R) if r >= R - min(2, len(prob.out))]
     bench_real = {r: [g.download_real() for g in prob.out[r % len(prob.out)]] for r in last}
     if len(last) == 2:
         C.simulate(ctx, prob.cov, None, prob.out[last[0] % len(prob.out)], seed=sp["seed"], realisation=last[0],
