@@ -90,7 +90,10 @@ int cg_ctx_set_fft_variant(cg_ctx *ctx, int variant);
  * coloured spectra never written to HBM) or as its own kernel followed by three z passes.  Mode
  * -1 (default): the autotuned choice for the geometry (cg_autotune times both), else fused when
  * the fused kernel keeps at least 12 wavefronts per CU resident; 0: never fused; 1: fused
- * whenever nzp has a compiled plan.  Results agree to FP32 rounding either way. */
+ * whenever nzp has a compiled plan.  On grids with a compiled split of the z transform (see
+ * cg_invert) modes -1 and 1 colour with the split kernels (the colouring carries the inverse z
+ * transform's second stage, the x pass its first); 0 keeps the unsplit, unfused colouring.
+ * Results agree to FP32 rounding either way. */
 int cg_ctx_set_fused_colour(cg_ctx *ctx, int mode);
 
 /* ------------------------------------------------------------------ grids (FFTGrid) */
