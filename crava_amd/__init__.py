@@ -123,6 +123,7 @@ def lib():
             "cg_grid_write_storm": [_vp, ctypes.c_char_p, ctypes.POINTER(StormGeometry), i],
             "cg_grid_read_storm": [_vp, ctypes.c_char_p, ctypes.POINTER(StormGeometry)],
             "cg_grid_write_segy": [_vp, ctypes.c_char_p, ctypes.POINTER(SegyGeometry)],
+            "cg_grid_fill_from_traces": [_vp, ctypes.POINTER(TraceFillArgs)],
             "cg_grid_read_crava": [_vp, ctypes.c_char_p, ctypes.POINTER(CravaGeometry)],
             "cg_local_noise_create": [_vp, ctypes.POINTER(LocalNoiseArgs), ctypes.POINTER(_vp)],
             "cg_local_noise_matrices": [_vp, _d_p, _d_p],
@@ -160,6 +161,14 @@ class LocalNoiseArgs(ctypes.Structure):
                 ("G", ctypes.POINTER(ctypes.c_double)), ("prior_var0", ctypes.POINTER(ctypes.c_double)),
                 ("post_var0", ctypes.POINTER(ctypes.c_double)), ("err_theta_cov", ctypes.POINTER(ctypes.c_double)),
                 ("noise_scale", ctypes.POINTER(ctypes.POINTER(ctypes.c_double)))]
+
+
+class TraceFillArgs(ctypes.Structure):
+    _fields_ = [("samples", ctypes.POINTER(ctypes.c_float)), ("length", ctypes.POINTER(ctypes.c_int)),
+                ("z0_data", ctypes.POINTER(ctypes.c_float)), ("ntraces", ctypes.c_int), ("stride", ctypes.c_int),
+                ("dz_data", ctypes.c_float), ("smooth_length", ctypes.c_float),
+                ("column_trace", ctypes.POINTER(ctypes.c_int)), ("z0_grid", ctypes.POINTER(ctypes.c_float)),
+                ("dz_grid", ctypes.POINTER(ctypes.c_float))]
 
 
 class SegyGeometry(ctypes.Structure):
@@ -485,6 +494,30 @@ class FFTGrid:
                            g["il_step_x"], g["il_step_y"], g["xl_step_x"], g["xl_step_y"], g["dz"], float(g["z0"]),
                            g["zmax"], _ptr(top, ctypes.c_double), _ptr(rel, ctypes.c_double), tarr, len(text))
         self.ctx.check(lib().cg_grid_write_segy(self._h, file_name.encode(), ctypes.byref(geo)), "writeSegyFile")
+
+    def fillInData(self, traces, z0_data, dz_data, column_trace, z0_grid, dz_grid, smooth_length):
+        """CommonData::FillInData (commondata.cpp:6451-6775), the SEG-Y seismic branch, with the
+        columns resolved by the caller: traces (a list of 1-D arrays, at most 1024 samples),
+        z0_data (time of each trace's first sample, ms), dz_data (ms), column_trace (nyp, rnxp)
+        trace index per column or -1 for zeros, z0_grid / dz_grid (nyp, rnxp) the centre of each
+        column's first cell and its thickness (ms), smooth_length the guard-zone taper (ms)."""
+        n = len(traces)
+        stride = max(len(t) for t in traces)
+        samples = np.zeros((n, stride), dtype=np.float32)
+        for t, tr in enumerate(traces):
+            samples[t, :len(tr)] = np.asarray(tr, dtype=np.float32)
+        length = np.array([len(t) for t in traces], dtype=np.int32)
+        z0d = np.ascontiguousarray(z0_data, dtype=np.float32)
+        shape = (self.nyp, self.rnxp)
+        ct = np.ascontiguousarray(column_trace, dtype=np.int32)
+        z0g = np.ascontiguousarray(z0_grid, dtype=np.float32)
+        dzg = np.ascontiguousarray(dz_grid, dtype=np.float32)
+        if ct.shape != shape or z0g.shape != shape or dzg.shape != shape or z0d.shape != (n,):
+            raise ValueError("fillInData: columns must be (nyp, rnxp) = %s, z0_data (%d,)" % (shape, n))
+        a = TraceFillArgs(_ptr(samples, ctypes.c_float), _ptr(length, ctypes.c_int), _ptr(z0d, ctypes.c_float), n,
+                          stride, float(dz_data), float(smooth_length), _ptr(ct, ctypes.c_int),
+                          _ptr(z0g, ctypes.c_float), _ptr(dzg, ctypes.c_float))
+        self.ctx.check(lib().cg_grid_fill_from_traces(self._h, ctypes.byref(a)), "fillInData")
 
     def readStormFile(self, file_name):
         """A STORM binary grid of this grid's box size, then the padding fill of

@@ -19,11 +19,23 @@ namespace {
 
 constexpr float kRMissing = -99999.0f;  // RMISSING, definitions.h:35
 
+size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+// Offsets of the parts of one device allocation (256-byte aligned).
+struct Layout {
+  size_t size = 0;
+  size_t add(size_t bytes)
+  {
+    const size_t off = size;
+    size += align_up(bytes ? bytes : 1, 256);
+    return off;
+  }
+};
+
 struct DeviceGuard {
   explicit DeviceGuard(int dev) { (void)hipSetDevice(dev); }
 };
 
-size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
 // Pinned staging of small host tables into one device buffer on the context stream.
 int stage_tables(cg_ctx *ctx, const std::vector<std::pair<const void *, size_t>> &parts, std::vector<void *> &dptrs)
@@ -1878,6 +1890,100 @@ int cg_make_circ_auto_cov_pos_def(int nzp, float *cac)
   return CG_OK;
 }
 
+// CommonData::FillInData (commondata.cpp:6451-6775), the SEG-Y DATA branch, with the columns
+// resolved by the caller: the traces tapered and zero-padded, their length-nt R2C spectra (the x
+// pass), the bins spread into length-mt spectra, the length-mt C2R (the x pass inverse, scale
+// 1 / nt), every column interpolated from its trace's fine samples.
+int cg_grid_fill_from_traces(cg_grid *g, const cg_trace_fill_args *a)
+{
+  if (!g || !a || !a->samples || !a->length || !a->z0_data || !a->column_trace || !a->z0_grid || !a->dz_grid)
+    return CG_EINVAL;
+  cg_ctx *ctx = g->ctx;
+  if (a->ntraces < 1 || a->stride < 1) return cg::fail(ctx, CG_EINVAL, "fill_from_traces: no traces");
+  if (!(a->dz_data > 0.0f) || !(a->smooth_length >= 0.0f))
+    return cg::fail(ctx, CG_EINVAL, "fill_from_traces: bad sample interval or smoothing length");
+  const int ns = (int)std::floor(a->smooth_length / a->dz_data);
+  int maxlen = 0;
+  for (int t = 0; t < a->ntraces; ++t) {
+    const int len = a->length[t];
+    if (len < 1 || len > a->stride) return cg::fail(ctx, CG_EINVAL, "fill_from_traces: trace length outside 1..stride");
+    if (len < ns) return cg::fail(ctx, CG_EINVAL, "fill_from_traces: a trace is shorter than its guard-zone taper");
+    maxlen = std::max(maxlen, len);
+  }
+  const long ncol = (long)g->nyp * g->rnxp;
+  for (long c = 0; c < ncol; ++c)
+    if (a->column_trace[c] < -1 || a->column_trace[c] >= a->ntraces)
+      return cg::fail(ctx, CG_EINVAL, "fill_from_traces: column trace index out of range");
+  const int nt = cg_factorable(maxlen);
+  if (nt > 1024) return cg::fail(ctx, CG_EINVAL, "fill_from_traces: traces longer than 1024 samples");
+  DeviceGuard dg(ctx->device);
+  cg::TraceFillParams p;
+  std::memset(&p, 0, sizeof(p));
+  p.ntraces = a->ntraces;
+  p.stride = a->stride;
+  p.dz_data = a->dz_data;
+  p.ns = ns;
+  p.nt = nt;
+  p.rnt = 2 * (nt / 2 + 1);
+  p.mt = 10 * nt;
+  p.rmt = 2 * (p.mt / 2 + 1);
+  p.dz_fine = a->dz_data / 10.0f;
+  p.nz = g->nz;
+  p.nyp = g->nyp;
+  p.nzp = g->nzp;
+  p.rnxp = g->rnxp;
+  p.grid = g->d;
+  // one device block: samples, lengths, z0_data, columns (trace, z0, dz), the two trace buffers
+  Layout L;
+  const size_t o_s = L.add(sizeof(float) * (size_t)a->ntraces * a->stride);
+  const size_t o_len = L.add(sizeof(int) * a->ntraces);
+  const size_t o_z0d = L.add(sizeof(float) * a->ntraces);
+  const size_t o_ct = L.add(sizeof(int) * ncol);
+  const size_t o_z0g = L.add(sizeof(float) * ncol);
+  const size_t o_dzg = L.add(sizeof(float) * ncol);
+  const size_t o_a = L.add(sizeof(float) * (size_t)a->ntraces * p.rnt);
+  const size_t o_b = L.add(sizeof(float) * (size_t)a->ntraces * p.rmt);
+  char *dev = nullptr;
+  hipError_t e = cg::dev_alloc(ctx, reinterpret_cast<void **>(&dev), L.size);
+  if (e != hipSuccess) return cg::hip_fail(ctx, e, "fill_from_traces: device buffers");
+  const struct {
+    size_t off;
+    const void *src;
+    size_t bytes;
+  } up[] = {{o_s, a->samples, sizeof(float) * (size_t)a->ntraces * a->stride},
+            {o_len, a->length, sizeof(int) * a->ntraces},
+            {o_z0d, a->z0_data, sizeof(float) * a->ntraces},
+            {o_ct, a->column_trace, sizeof(int) * ncol},
+            {o_z0g, a->z0_grid, sizeof(float) * ncol},
+            {o_dzg, a->dz_grid, sizeof(float) * ncol}};
+  for (const auto &u : up) {
+    if (e == hipSuccess) e = hipMemcpyAsync(dev + u.off, u.src, u.bytes, hipMemcpyHostToDevice, ctx->stream);
+  }
+  int rc = cg::hip_fail(ctx, e, "fill_from_traces: copying the traces");
+  p.samples = reinterpret_cast<const float *>(dev + o_s);
+  p.length = reinterpret_cast<const int *>(dev + o_len);
+  p.z0_data = reinterpret_cast<const float *>(dev + o_z0d);
+  p.column_trace = reinterpret_cast<const int *>(dev + o_ct);
+  p.z0_grid = reinterpret_cast<const float *>(dev + o_z0g);
+  p.dz_grid = reinterpret_cast<const float *>(dev + o_dzg);
+  p.a = reinterpret_cast<float *>(dev + o_a);
+  p.b = reinterpret_cast<float *>(dev + o_b);
+  if (!rc) rc = cg::pack_traces(ctx, p);
+  if (!rc) rc = cg::fft_x_r2c(ctx, p.a, p.a, nt, a->ntraces, 1.0f);
+  if (!rc) rc = cg::spread_spectra(ctx, p);
+  if (!rc) rc = cg::fft_x_c2r(ctx, p.b, p.mt, a->ntraces, 1 / static_cast<float>(nt), nullptr);
+  if (!rc) rc = cg::fill_columns(ctx, p);
+  cg::dev_free(ctx, dev);
+  // the caller may release its host arrays on return
+  e = hipStreamSynchronize(ctx->stream);
+  if (!rc) rc = cg::hip_fail(ctx, e, "fill_from_traces");
+  if (!rc) {
+    g->cubetype = CG_CT_DATA;
+    g->transformed = 0;
+  }
+  return rc;
+}
+
 int cg_factorable(int leastint)
 {
   if (leastint <= 1) return 1;
@@ -2220,16 +2326,6 @@ struct cg_batch {
 };
 
 namespace {
-
-struct Layout {
-  size_t size = 0;
-  size_t add(size_t bytes)
-  {
-    const size_t off = size;
-    size += align_up(bytes ? bytes : 1, 256);
-    return off;
-  }
-};
 
 // Side streams (default priority) of a context, created once: the launches of one batched pass
 // (one per compiled length) and the posterior groups are independent and run concurrently;

@@ -30,4 +30,30 @@ struct ScaleWaveletParams {
 // max(1 - dist^2, 0) for k = nz + 1 .. nzp - 1.
 int scale_wavelet_filter(cg_ctx *ctx, const ScaleWaveletParams &p, const float *taper_dev);
 
+// CommonData::FillInData (commondata.cpp:6451-6775), the SEG-Y DATA branch: traces tapered in
+// the guard zone and zero-padded (k_pack_traces), their length-nt spectra (the x pass), spread
+// into length-mt spectra (k_spread_spectra), fine traces (the x pass inverse, scale 1 / nt), the
+// grid's columns interpolated from them (k_fill_columns).
+struct TraceFillParams {
+  const float *samples;    // [ntraces][stride]
+  const int *length;       // [ntraces]
+  long stride;
+  int ntraces;
+  float dz_data;
+  int ns;                  // samples tapered at each end: floor(smooth_length / dz_data)
+  int nt, rnt, mt, rmt;    // rnt = 2 (nt / 2 + 1), mt = 10 nt, rmt = 2 (mt / 2 + 1)
+  float *a;                // [ntraces][rnt]: tapered traces, then their spectra
+  float *b;                // [ntraces][rmt]: fine spectra, then fine traces
+  const int *column_trace;  // [nyp][rnxp]
+  const float *z0_grid;     // [nyp][rnxp]
+  const float *dz_grid;     // [nyp][rnxp]
+  const float *z0_data;     // [ntraces]
+  float dz_fine;
+  float *grid;             // real grid, in-place layout
+  int nz, nyp, nzp, rnxp;
+};
+int pack_traces(cg_ctx *ctx, const TraceFillParams &p);
+int spread_spectra(cg_ctx *ctx, const TraceFillParams &p);
+int fill_columns(cg_ctx *ctx, const TraceFillParams &p);
+
 }  // namespace cg

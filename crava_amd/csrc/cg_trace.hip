@@ -227,3 +227,120 @@ int scale_wavelet_filter(cg_ctx *ctx, const ScaleWaveletParams &p, const float *
 }
 
 }  // namespace cg
+
+// ---------------------------------------------------------------- seismic traces onto the grid
+namespace cgtrace {
+
+// CommonData::SmoothTraceInGuardZone (commondata.cpp:6847-6885) and the zero padding of
+// ResampleTrace (commondata.h:148-160): sample n of trace t, in float as the reference (the
+// factors from double angles, then squared in float; the start factor first where both apply).
+__global__ void __launch_bounds__(cg::kThreads) k_pack_traces(cg::TraceFillParams p)
+{
+#pragma clang fp contract(off)
+  const long total = (long)p.ntraces * p.rnt;
+  for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
+    const int t = (int)(idx / p.rnt), n = (int)(idx - (long)t * p.rnt);
+    const int len = p.length[t];
+    float v = 0.f;
+    if (n < len) {
+      v = p.samples[(long)t * p.stride + n];
+      if (n < p.ns) {
+        const float s = (float)sin(1.5707963267948966 * ((double)n / (double)p.ns));
+        v *= s * s;
+      }
+      const int ke = n - (len - p.ns);
+      if (ke >= 0) {
+        const float c = (float)cos(1.5707963267948966 * ((double)(ke + 1) / (double)p.ns));
+        v *= c * c;
+      }
+    }
+    p.a[idx] = v;
+  }
+}
+
+// The cnt = rnt / 2 bins of each trace's spectrum as the lowest bins of a zero spectrum of
+// cmt = rmt / 2 bins (ResampleTrace, commondata.h:166-176).
+__global__ void __launch_bounds__(cg::kThreads) k_spread_spectra(cg::TraceFillParams p)
+{
+  const long total = (long)p.ntraces * p.rmt;
+  for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
+    const int t = (int)(idx / p.rmt), f = (int)(idx - (long)t * p.rmt);
+    p.b[idx] = f < p.rnt ? p.a[(long)t * p.rnt + f] : 0.f;
+  }
+}
+
+// CommonData::InterpolateGridValues (commondata.cpp:6947-6995) with GetZSimboxIndex (:7052-7062)
+// and SetTrace (:7084-7092): real cell (i, j, k) of the padded grid from the fine trace of its
+// column's trace (zero for columns without one), in the reference's float arithmetic.
+__global__ void __launch_bounds__(cg::kThreads) k_fill_columns(cg::TraceFillParams p)
+{
+#pragma clang fp contract(off)
+  const long plane = (long)p.nyp * p.rnxp;
+  const long total = plane * p.nzp;
+  const float inv = 1.0f / p.dz_fine;
+  for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
+    const int k = (int)(idx / plane);
+    const long col = idx - (long)k * plane;
+    const int t = p.column_trace[col];
+    float v = 0.f;
+    if (t >= 0) {
+      const float z0_shift = p.z0_grid[col] - p.z0_data[t];
+      const int refk = k < (p.nz + p.nzp) / 2 ? k : k - p.nzp;
+      const float dl = (z0_shift + (float)refk * p.dz_grid[col]) * inv;
+      const float fl = floorf(dl), cl = ceilf(dl);
+      const int l1 = (int)fl, l2 = (int)cl;
+      const float *fine = p.b + (long)t * p.rmt;
+      const int n_fine = p.rmt;
+      if (l2 < 0 || l1 > n_fine - 1) {
+        v = 0.f;
+      } else if (l1 < 0) {
+        v = fine[l2];
+      } else if (l2 > n_fine - 1 || l1 == l2) {
+        v = fine[l1];
+      } else {
+        const float w1 = cl - dl, w2 = dl - fl;
+        v = w1 * fine[l1] + w2 * fine[l2];
+      }
+    }
+    p.grid[idx] = v;
+  }
+}
+
+}  // namespace cgtrace
+
+namespace cg {
+
+static unsigned grid_stride_blocks(long total)
+{
+  long b = (total + kThreads - 1) / kThreads;
+  return (unsigned)(b < 8192 ? (b > 0 ? b : 1) : 8192);
+}
+
+int pack_traces(cg_ctx *ctx, const TraceFillParams &p)
+{
+  hipEvent_t ev;
+  launch_begin(ctx, CG_K_ELEMENTWISE, &ev);
+  cgtrace::k_pack_traces<<<grid_stride_blocks((long)p.ntraces * p.rnt), kThreads, 0, ctx->stream>>>(p);
+  launch_end(ctx, CG_K_ELEMENTWISE, ev);
+  return hip_fail(ctx, hipGetLastError(), "k_pack_traces");
+}
+
+int spread_spectra(cg_ctx *ctx, const TraceFillParams &p)
+{
+  hipEvent_t ev;
+  launch_begin(ctx, CG_K_ELEMENTWISE, &ev);
+  cgtrace::k_spread_spectra<<<grid_stride_blocks((long)p.ntraces * p.rmt), kThreads, 0, ctx->stream>>>(p);
+  launch_end(ctx, CG_K_ELEMENTWISE, ev);
+  return hip_fail(ctx, hipGetLastError(), "k_spread_spectra");
+}
+
+int fill_columns(cg_ctx *ctx, const TraceFillParams &p)
+{
+  hipEvent_t ev;
+  launch_begin(ctx, CG_K_ELEMENTWISE, &ev);
+  cgtrace::k_fill_columns<<<grid_stride_blocks((long)p.nyp * p.rnxp * p.nzp), kThreads, 0, ctx->stream>>>(p);
+  launch_end(ctx, CG_K_ELEMENTWISE, ev);
+  return hip_fail(ctx, hipGetLastError(), "k_fill_columns");
+}
+
+}  // namespace cg

@@ -276,6 +276,30 @@ int cg_grid_write_storm(cg_grid *grid, const char *path, const cg_storm_geometry
  * returned).  CG_EINVAL for another format or size, CG_EIO if the file cannot be read or ends early. */
 int cg_grid_read_storm(cg_grid *grid, const char *path, cg_storm_geometry *geometry);
 
+/* CommonData::FillInData (commondata.cpp:6451-6775), the SEG-Y seismic (DATA) branch, with the
+ * columns resolved by the caller as FillInData resolves them (GetFillNumber, Simbox::getCoord /
+ * getdz, SegyGeometry::IsInside, SegY::GetNearestTrace): column (i, j) of the padded grid,
+ * i < rnxp, takes trace column_trace[j][i], or zeros where that is -1 (outside the data, or a
+ * dead trace).  Each trace is tapered over floor(smooth_length / dz_data) samples at each end
+ * (SmoothTraceInGuardZone, :6847-6885), resampled ten times finer through its spectrum of length
+ * nt = the closest 2^a 3^b 5^c 7^d >= the longest trace (ResampleTrace, commondata.h:136-190),
+ * and interpolated at the cell centres z0_grid + refk dz_grid, refk = GetZSimboxIndex(k)
+ * (InterpolateGridValues, :6947-6995).  The grid becomes a real-domain DATA grid.  Traces at most
+ * 1024 samples, none shorter than its taper; host arrays, copied before the call returns. */
+typedef struct {
+  const float *samples;      /* [ntraces][stride] */
+  const int *length;         /* [ntraces] samples of each trace, 1..stride */
+  const float *z0_data;      /* [ntraces] time of the first sample, ms: GetNearestTrace's z0
+                              * less half a sample (commondata.cpp:6584-6585) */
+  int ntraces, stride;
+  float dz_data;             /* sample interval, ms */
+  float smooth_length;       /* guard-zone taper, ms */
+  const int *column_trace;   /* [nyp][rnxp] */
+  const float *z0_grid;      /* [nyp][rnxp] centre of the column's first cell, ms (z0 + dz / 2) */
+  const float *dz_grid;      /* [nyp][rnxp] */
+} cg_trace_fill_args;
+int cg_grid_fill_from_traces(cg_grid *grid, const cg_trace_fill_args *args);
+
 /* ------------------------------------------------------------------ posterior (AVOInversion) */
 #define CG_PRIOR_GRIDS 0     /* six prior covariance spectra, as SeismicParametersHolder holds */
 #define CG_PRIOR_SEPARABLE 1 /* Sigma_m(k) = |rxy(kx,ky) * rt(kz)| * Sigma0 (no grids read) */

@@ -981,3 +981,96 @@ def correct_realisation(seed, post, nx, ny, nz, stored):
         acc = acc + s[..., a, 2] * z[2]
         out[a, :nz, :ny, :nx] = acc.astype(np.float32)
     return (out + np.asarray(post, dtype=np.float32)).astype(np.float32)
+
+
+# ------------------------------------------------------------------ seismic traces onto the grid
+# CommonData::FillInData (commondata.cpp:6451-6775), the is_segy, DATA (seismic) branch: every
+# column of the padded grid takes its nearest SEG-Y trace, tapered in the guard zone, resampled
+# ten times finer through its spectrum and interpolated at the grid's cell centres.
+
+def smooth_trace_in_guard_zone(trace, dz_data, smooth_length):
+    """CommonData::SmoothTraceInGuardZone (commondata.cpp:6847-6885): sin^2 taper over the first
+    floor(smooth_length / dz) samples, cos^2 over the last as many (float products, the factors
+    from double angles)."""
+    t = np.array(trace, dtype=np.float32, copy=True)
+    ns = int(math.floor(np.float32(smooth_length) / np.float32(dz_data)))
+    n = len(t)
+    for k in range(ns):
+        s = np.float32(math.sin(0.5 * math.pi * (k / ns)))
+        t[k] = np.float32(t[k] * np.float32(s * s))
+    for k in range(ns):
+        c = np.float32(math.cos(0.5 * math.pi * ((k + 1) / ns)))
+        t[n - ns + k] = np.float32(t[n - ns + k] * np.float32(c * c))
+    return t
+
+
+def resample_trace(trace, nt):
+    """CommonData::ResampleTrace (commondata.h:136-190) in double: the trace zero-padded to nt,
+    its nt/2 + 1 bins as the lowest bins of a length-10 nt spectrum, inverse (Hermitian: a bin
+    below the fine Nyquist counts twice, the imaginary part of DC is dropped), scaled by 1 / nt.
+    Returns the rmt = 2 (5 nt + 1) floats the reference's in-place buffer holds: the fine trace
+    and, past it, the zeros of the (always empty) fine Nyquist bin."""
+    mt = 10 * nt
+    x = np.zeros(nt)
+    x[:len(trace)] = np.asarray(trace, dtype=np.float64)
+    c = np.fft.rfft(x)
+    cf = np.zeros(mt // 2 + 1, dtype=complex)
+    cf[:len(c)] = c
+    out = np.zeros(2 * (mt // 2 + 1))
+    out[:mt] = np.fft.irfft(cf, mt) * (mt / nt)
+    return out
+
+
+def z_simbox_index(k, nz, nzp):
+    """CommonData::GetZSimboxIndex (commondata.cpp:7052-7062)."""
+    return k if k < (nz + nzp) // 2 else k - nzp
+
+
+def interpolate_grid_values(fine, z0_grid, dz_grid, z0_data, dz_fine, nz, nzp):
+    """CommonData::InterpolateGridValues (commondata.cpp:6947-6995): the value at cell k of a
+    column (refk = GetZSimboxIndex(k)) from the fine trace, linear between the two fine samples
+    around it, in the reference's float arithmetic; zero outside the fine trace."""
+    n_fine = len(fine)
+    f32 = np.float32
+    z0_shift = f32(f32(z0_grid) - f32(z0_data))
+    inv = f32(f32(1.0) / f32(dz_fine))
+    out = np.zeros(nzp, dtype=np.float32)
+    for k in range(nzp):
+        refk = z_simbox_index(k, nz, nzp)
+        dl = f32(f32(z0_shift + f32(f32(refk) * f32(dz_grid))) * inv)
+        l1, l2 = int(math.floor(dl)), int(math.ceil(dl))
+        if l2 < 0 or l1 > n_fine - 1:
+            v = 0.0
+        elif l1 < 0:
+            v = fine[l2]
+        elif l2 > n_fine - 1 or l1 == l2:
+            v = fine[l1]
+        else:
+            w1, w2 = f32(math.ceil(dl) - dl), f32(dl - math.floor(dl))
+            v = f32(w1 * f32(fine[l1])) + f32(w2 * f32(fine[l2]))
+        out[k] = f32(v)
+    return out
+
+
+def fill_from_traces(traces, z0_data, dz_data, column_trace, z0_grid, dz_grid, nz, nzp, smooth_length):
+    """The columns of FillInData's DATA branch: column (j, i) of the padded real grid
+    (nzp, nyp, rnxp) takes trace column_trace[j, i] (zeros where negative) with its first sample
+    at z0_data[t] (ms), tapered (smooth_trace_in_guard_zone), resampled (resample_trace with nt
+    the closest factorable length of the longest trace, rounded to float as the reference's
+    buffer is) and interpolated at z0_grid[j, i] + refk dz_grid[j, i] (interpolate_grid_values,
+    dz_fine = dz_data / 10).  Returns (grid, nt)."""
+    nyp, rnxp = column_trace.shape
+    nt = lib().orc_factorable(max(len(t) for t in traces))
+    dz_fine = np.float32(np.float32(dz_data) / np.float32(10.0))
+    fine = {}
+    out = np.zeros((nzp, nyp, rnxp), dtype=np.float32)
+    for j in range(nyp):
+        for i in range(rnxp):
+            t = int(column_trace[j, i])
+            if t < 0:
+                continue
+            if t not in fine:
+                sm = smooth_trace_in_guard_zone(traces[t], dz_data, smooth_length)
+                fine[t] = resample_trace(sm, nt).astype(np.float32)
+            out[:, j, i] = interpolate_grid_values(fine[t], z0_grid[j, i], dz_grid[j, i], z0_data[t], dz_fine, nz, nzp)
+    return out, nt

@@ -13,6 +13,7 @@
 #include <stdint.h>
 
 #include <mutex>
+#include <vector>
 #include <stdlib.h>
 #include <string.h>
 #include "fftw.h"
@@ -68,6 +69,31 @@ void ref_fft1d_r2c(int n, float *buf)
   rfftwnd_plan plan = rfftwnd_create_plan(1, &n, FFTW_REAL_TO_COMPLEX, FFTW_ESTIMATE | FFTW_IN_PLACE);
   rfftwnd_one_real_to_complex(plan, buf, (fftw_complex *)buf);
   rfftwnd_destroy_plan(plan);
+}
+
+/* CommonData::ResampleTrace (commondata.h:136-190) with the reference's FFTW 2.1.2 plans as
+ * FillInData makes them (commondata.cpp:6531-6538): the n_data samples zero-padded to nt, r2c,
+ * the cnt bins into a zero spectrum of cmt bins, c2r of length mt = 10 nt, scaled by 1 / nt.
+ * fine receives the rmt = 2 (mt / 2 + 1) floats of the in-place buffer (the reference reads up
+ * to index rmt - 1). */
+void ref_resample_trace(int n_data, const float *data, int nt, float *fine)
+{
+  const int mt = 10 * nt, cnt = nt / 2 + 1, rnt = 2 * cnt, cmt = mt / 2 + 1, rmt = 2 * cmt;
+  std::vector<float> amp(rnt), ampf(rmt);
+  rfftwnd_plan p1, p2;
+  {
+    std::lock_guard<std::mutex> lock(plan_mu);
+    p1 = rfftwnd_create_plan(1, &nt, FFTW_REAL_TO_COMPLEX, FFTW_ESTIMATE | FFTW_IN_PLACE);
+    p2 = rfftwnd_create_plan(1, &mt, FFTW_COMPLEX_TO_REAL, FFTW_ESTIMATE | FFTW_IN_PLACE);
+  }
+  for (int i = 0; i < rnt; i++) amp[i] = i < n_data ? data[i] : 0.0f;
+  rfftwnd_one_real_to_complex(p1, amp.data(), (fftw_complex *)amp.data());
+  for (int i = 0; i < rmt; i++) ampf[i] = i < 2 * cnt ? amp[i] : 0.0f;
+  rfftwnd_one_complex_to_real(p2, (fftw_complex *)ampf.data(), ampf.data());
+  const float scale = 1 / static_cast<float>(nt);
+  for (int i = 0; i < rmt; i++) fine[i] = scale * ampf[i];
+  drop_plan(p1);
+  drop_plan(p2);
 }
 
 static fftw_complex **cmat(int n1, int n2)
