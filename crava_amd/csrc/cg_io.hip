@@ -605,4 +605,102 @@ int cg_grid_write_segy(cg_grid *g, const char *path, const cg_segy_geometry *geo
   return rc;
 }
 
+
+// NRLib SegY reading (libs/nrlib/segy: the binary header's interval, sample count and format;
+// each trace's header fields and samples), for SegY::GetNearestTrace's callers: the traces in
+// file order, IBM (format 1) or IEEE (format 5) big-endian samples converted to floats (an IBM
+// float's 24-bit fraction fits a float's significand: the conversion is exact inside float's
+// range).  Two calls: the first (null buffers) returns the counts, the second fills what the
+// caller asks for.
+static int segy_field(const unsigned char *h, int pos, int bytes)
+{
+  const unsigned char *p = h + pos - 1;
+  return bytes == 2 ? (int)(short)(((unsigned)p[0] << 8) | p[1])
+                    : (int)(((unsigned)p[0] << 24) | ((unsigned)p[1] << 16) | ((unsigned)p[2] << 8) | p[3]);
+}
+
+static float ibm_to_ieee(uint32_t w)
+{
+  const uint32_t frac = w & 0x00ffffffu;
+  if (frac == 0) return (w >> 31) ? -0.0f : 0.0f;
+  const int e16 = (int)((w >> 24) & 0x7f) - 64;  // value = 0.frac (base 16) * 16^e16
+  const double v = std::ldexp((double)frac, 4 * e16 - 24);
+  return (float)((w >> 31) ? -v : v);
+}
+
+int cg_segy_read(const char *path, const cg_segy_format *fmt, cg_segy_traces *out)
+{
+  if (!path || !out) return CG_EINVAL;
+  const int il_pos = fmt && fmt->il_pos ? fmt->il_pos : 9, xl_pos = fmt && fmt->xl_pos ? fmt->xl_pos : 21;
+  const int x_pos = fmt && fmt->x_pos ? fmt->x_pos : 73, y_pos = fmt && fmt->y_pos ? fmt->y_pos : 77;
+  const int sc_pos = fmt && fmt->scalco_pos ? fmt->scalco_pos : 71;
+  const int t0_pos = fmt && fmt->start_time_pos ? fmt->start_time_pos : 109;
+  for (int pos : {il_pos, xl_pos, x_pos, y_pos})
+    if (pos < 1 || pos > 237) return CG_EINVAL;
+  for (int pos : {sc_pos, t0_pos})
+    if (pos < 1 || pos > 239) return CG_EINVAL;
+  FILE *f = std::fopen(path, "rb");
+  if (!f) return CG_EIO;
+  struct Closer {
+    FILE *f;
+    ~Closer() { std::fclose(f); }
+  } closer{f};
+  unsigned char bh[400];
+  if (std::fseek(f, 3200, SEEK_SET) != 0 || std::fread(bh, 1, 400, f) != 400) return CG_EIO;
+  const int dt_us = segy_field(bh, 17, 2), ns = segy_field(bh, 21, 2), code = segy_field(bh, 25, 2);
+  if (ns <= 0 || dt_us <= 0 || (code != 1 && code != 5)) return CG_EINVAL;
+  if (std::fseek(f, 0, SEEK_END) != 0) return CG_EIO;
+  const long size = std::ftell(f);
+  const long tb = 240L + 4L * ns;
+  if (size < 3600 || (size - 3600) % tb != 0) return CG_EINVAL;
+  const long ntr = (size - 3600) / tb;
+  if (ntr > INT32_MAX) return CG_EINVAL;
+  const bool fill = out->samples || out->il || out->xl || out->x || out->y || out->start_time;
+  if (fill && (out->ntraces != (int)ntr || out->nsamples != ns)) return CG_EINVAL;
+  out->ntraces = (int)ntr;
+  out->nsamples = ns;
+  out->dz = (float)dt_us / 1000.0f;
+  out->format_code = code;
+  if (!fill) return CG_OK;
+  if (std::fseek(f, 3600, SEEK_SET) != 0) return CG_EIO;
+  std::vector<unsigned char> buf((size_t)tb);
+  for (long t = 0; t < ntr; ++t) {
+    if (std::fread(buf.data(), 1, (size_t)tb, f) != (size_t)tb) return CG_EIO;
+    const unsigned char *h = buf.data();
+    if (out->il) out->il[t] = segy_field(h, il_pos, 4);
+    if (out->xl) out->xl[t] = segy_field(h, xl_pos, 4);
+    // TraceHeader::Read (traceheader.cpp:499-518): scalco -10000..-10 and 10..10000 in powers of
+    // ten, as a float factor; any other value leaves the factor 1 (GetUtmx: factor * utmx)
+    const int sc = segy_field(h, sc_pos, 2);
+    float factor = 1.0f;
+    switch (sc) {
+      case -10000: factor = 0.0001f; break;
+      case -1000: factor = 0.001f; break;
+      case -100: factor = 0.01f; break;
+      case -10: factor = 0.1f; break;
+      case 10: case 100: case 1000: case 10000: factor = (float)sc; break;
+      default: break;
+    }
+    const double scale = factor;
+    if (out->x) out->x[t] = segy_field(h, x_pos, 4) * scale;
+    if (out->y) out->y[t] = segy_field(h, y_pos, 4) * scale;
+    if (out->start_time) out->start_time[t] = (float)segy_field(h, t0_pos, 2);
+    if (out->samples) {
+      float *dst = out->samples + t * (long)ns;
+      for (int k = 0; k < ns; ++k) {
+        const unsigned char *q = h + 240 + 4 * k;
+        const uint32_t w = ((uint32_t)q[0] << 24) | ((uint32_t)q[1] << 16) | ((uint32_t)q[2] << 8) | q[3];
+        if (code == 1) {
+          dst[k] = ibm_to_ieee(w);
+        } else {
+          float v;
+          std::memcpy(&v, &w, 4);
+          dst[k] = v;
+        }
+      }
+    }
+  }
+  return CG_OK;
+}
+
 }  // extern "C"

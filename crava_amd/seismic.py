@@ -1,0 +1,139 @@
+"""Seismic input from SEG-Y files onto the padded grid: NRLib's SegY reading (cg_segy_read) and
+CommonData::FillInData's column loop (commondata.cpp:6451-6775) for the seismic (DATA) branch,
+whose per-trace work runs on the device (FFTGrid.fillInData, cg_grid_fill_from_traces).
+
+The column loop follows the reference: column (i, j) of the padded grid, i < rnxp, takes the box
+cell refi = GetFillNumber(i, nx, nxp), refj = GetFillNumber(j, ny, nyp) (no cell for i >= nxp),
+its centre from Simbox::getCoord(refi, refj, 0) (simbox.cpp:814-842), the SEG-Y trace nearest to
+that centre (SegyGeometry::FindContIndex / FindIndex and SegY::GetNearestTrace, segy.cpp:1031-1050,
+segygeometry.cpp:955-1035) and z0 + dz / 2 and dz of the cell column (commondata.cpp:6555-6640).
+The lattice of the SEG-Y volume is the one its inline and crossline numbers span, with the
+first inline and crossline the smallest ones and inlines along the first axis (NRLib's
+SegyGeometry when IL0 == minIL, XL0 == minXL and firstAxisIL); the continuous inline / crossline
+of a point is the caller's linear map, as SegyGeometry::FindContILXL computes it."""
+import ctypes
+import math
+
+import numpy as np
+
+from . import FFTGrid, CravaError, lib  # noqa: F401
+
+RMISSING = -99999.0
+
+
+class SegyFormat(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in ("il_pos", "xl_pos", "x_pos", "y_pos", "scalco_pos", "start_time_pos")]
+
+
+class SegyTraces(ctypes.Structure):
+    _fields_ = [("ntraces", ctypes.c_int), ("nsamples", ctypes.c_int), ("dz", ctypes.c_float),
+                ("format_code", ctypes.c_int), ("samples", ctypes.POINTER(ctypes.c_float)),
+                ("il", ctypes.POINTER(ctypes.c_int)), ("xl", ctypes.POINTER(ctypes.c_int)),
+                ("x", ctypes.POINTER(ctypes.c_double)), ("y", ctypes.POINTER(ctypes.c_double)),
+                ("start_time", ctypes.POINTER(ctypes.c_float))]
+
+
+def read_segy(path, header_positions=None):
+    """Every trace of a SEG-Y file (cg_segy_read): dict of samples (ntraces, nsamples), il, xl,
+    x, y, start_time (ms), dz (ms).  header_positions: dict of 1-based trace-header byte
+    positions il_pos, xl_pos, x_pos, y_pos, scalco_pos, start_time_pos (SeisWorks by default)."""
+    L = lib()
+    L.cg_segy_read.argtypes = [ctypes.c_char_p, ctypes.POINTER(SegyFormat), ctypes.POINTER(SegyTraces)]
+    L.cg_segy_read.restype = ctypes.c_int
+    fmt = SegyFormat(**(header_positions or {}))
+    t = SegyTraces()
+    rc = L.cg_segy_read(path.encode(), ctypes.byref(fmt), ctypes.byref(t))
+    if rc != 0:
+        raise CravaError("cg_segy_read(%s) failed (%d)" % (path, rc))
+    n, ns = t.ntraces, t.nsamples
+    out = dict(samples=np.zeros((n, ns), np.float32), il=np.zeros(n, np.int32), xl=np.zeros(n, np.int32),
+               x=np.zeros(n), y=np.zeros(n), start_time=np.zeros(n, np.float32), dz=float(t.dz))
+    t.samples = out["samples"].ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+    t.il = out["il"].ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+    t.xl = out["xl"].ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+    t.x = out["x"].ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    t.y = out["y"].ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    t.start_time = out["start_time"].ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+    rc = L.cg_segy_read(path.encode(), ctypes.byref(fmt), ctypes.byref(t))
+    if rc != 0:
+        raise CravaError("cg_segy_read(%s) failed (%d)" % (path, rc))
+    return out
+
+
+def fill_number(i, n, np_):
+    """CommonData::GetFillNumber (commondata.cpp:6777-6814): the box index whose trace fills
+    padded index i, None at or past np_."""
+    if i >= np_:
+        return None
+    if i < n:
+        return i
+    below, above = np_ - i - 1, i - n + 1
+    return max(n - above, n // 2) if above < below else min(below, n // 2)
+
+
+def segy_lattice(traces):
+    """The inline / crossline lattice of the traces (steps the smallest positive differences) and
+    the trace at each lattice node ((nx, ny) array, -1 where none)."""
+    def step(v):
+        u = np.unique(v)
+        d = np.diff(u)
+        return int(d[d > 0].min()) if len(u) > 1 else 1
+    il, xl = traces["il"], traces["xl"]
+    lat = dict(min_il=int(il.min()), max_il=int(il.max()), il_step=step(il),
+               min_xl=int(xl.min()), max_xl=int(xl.max()), xl_step=step(xl))
+    lat["nx"] = (lat["max_il"] - lat["min_il"]) // lat["il_step"] + 1
+    lat["ny"] = (lat["max_xl"] - lat["min_xl"]) // lat["xl_step"] + 1
+    node = np.full((lat["nx"], lat["ny"]), -1, dtype=np.int64)
+    i = (il - lat["min_il"]) // lat["il_step"]
+    j = (xl - lat["min_xl"]) // lat["xl_step"]
+    node[i, j] = np.arange(len(il))
+    lat["node"] = node
+    return lat
+
+
+def resolve_columns(traces, lattice, transform, simbox, nx, ny, nz, nxp, nyp):
+    """FillInData's column loop for the seismic branch: (column_trace, z0_grid, dz_grid), each
+    (nyp, rnxp), and the traces' z0_data (ms).  transform: in_line0, cross_line0, x0, y0,
+    il_step_x, il_step_y, xl_step_x, xl_step_y (SegyGeometry::FindContILXL); simbox: x0, y0, dx,
+    dy, angle and top, bot (ny, nx) of the box cells (RMISSING where the top is missing)."""
+    rnxp = 2 * (nxp // 2 + 1)
+    ct = np.full((nyp, rnxp), -1, dtype=np.int32)
+    z0g = np.zeros((nyp, rnxp), np.float32)
+    dzg = np.ones((nyp, rnxp), np.float32)
+    cr, sr = math.cos(simbox["angle"]), math.sin(simbox["angle"])
+    tf = transform
+    for j in range(nyp):
+        rj = fill_number(j, ny, nyp)
+        for i in range(rnxp):
+            ri = fill_number(i, nx, nxp)
+            if ri is None or rj is None:
+                continue
+            rx, ry = (ri + 0.5) * simbox["dx"], (rj + 0.5) * simbox["dy"]
+            xf = float(np.float32(rx * cr - ry * sr + simbox["x0"]))
+            yf = float(np.float32(rx * sr + ry * cr + simbox["y0"]))
+            il = tf["in_line0"] + (xf - tf["x0"]) * tf["il_step_x"] + (yf - tf["y0"]) * tf["il_step_y"]
+            xl = tf["cross_line0"] + (xf - tf["x0"]) * tf["xl_step_x"] + (yf - tf["y0"]) * tf["xl_step_y"]
+            xind = 0.5 + (il - lattice["min_il"]) / lattice["il_step"]
+            yind = 0.5 + (xl - lattice["min_xl"]) / lattice["xl_step"]
+            if not (0.0 <= xind < lattice["nx"] and 0.0 <= yind < lattice["ny"]):
+                continue  # outside the data: zeros
+            t = int(lattice["node"][int(xind), int(yind)])
+            top, bot = float(simbox["top"][rj, ri]), float(simbox["bot"][rj, ri])
+            if t < 0 or top == RMISSING or bot == RMISSING:
+                continue  # dead trace: zeros
+            dz = (bot - top) / nz
+            z0 = top + 0.5 * dz  # Simbox::getCoord(refi, refj, 0)
+            ct[j, i] = t
+            z0g[j, i] = np.float32(np.float32(z0) + np.float32(0.5) * np.float32(dz))
+            dzg[j, i] = np.float32(dz)
+    return ct, z0g, dzg, np.asarray(traces["start_time"], np.float32)
+
+
+def fill_in_data_from_segy(grid, path, transform, simbox, smooth_length, header_positions=None):
+    """CommonData::FillInData for a seismic FFTGrid from a SEG-Y file: the traces read, the
+    columns resolved on the host, the per-trace work on the device."""
+    tr = read_segy(path, header_positions)
+    lat = segy_lattice(tr)
+    ct, z0g, dzg, z0d = resolve_columns(tr, lat, transform, simbox, grid.nx, grid.ny, grid.nz, grid.nxp, grid.nyp)
+    grid.fillInData(list(tr["samples"]), z0d, tr["dz"], ct, z0g, dzg, smooth_length)
+    return tr, ct

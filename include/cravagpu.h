@@ -496,6 +496,29 @@ typedef struct {
  * EBCDIC (not the reference's standard header text).  Real domain; samples computed on the device. */
 int cg_grid_write_segy(cg_grid *grid, const char *path, const cg_segy_geometry *geometry);
 
+/* NRLib's SegY reading (libs/nrlib/segy: BinaryHeader::Update, TraceHeader, SegY::ReadTrace) for
+ * SegY::GetNearestTrace's callers such as CommonData::FillInData: every trace of a SEG-Y file in
+ * file order.  The binary header gives the sample interval (bytes 3217-3218, microseconds), the
+ * samples per trace (3221-3222) and the format (3225-3226: 1 IBM or 5 IEEE floats, big-endian);
+ * each trace header gives inline, crossline, x, y (times scalco's factor as TraceHeader::Read sets
+ * it: 10..10000 multiply, -10..-10000 divide, any other value 1) and the start time (ms).  Header byte positions (1-based; 0 or a null format: the SeisWorks
+ * positions the SEG-Y writer uses, 9, 21, 73, 77, 71, 109).  Call with null buffers to learn
+ * ntraces and nsamples, then with buffers of that size (any may be null).  CG_EIO if the file
+ * cannot be read, CG_EINVAL for another format or a size that is not whole traces. */
+typedef struct {
+  int il_pos, xl_pos, x_pos, y_pos, scalco_pos, start_time_pos;
+} cg_segy_format;
+typedef struct {
+  int ntraces, nsamples; /* from the file (first call); must match on the second */
+  float dz;              /* sample interval, ms */
+  int format_code;
+  float *samples;        /* [ntraces][nsamples] */
+  int *il, *xl;
+  double *x, *y;
+  float *start_time;
+} cg_segy_traces;
+int cg_segy_read(const char *path, const cg_segy_format *format, cg_segy_traces *traces);
+
 /* ------------------------------------------------------------------ local noise */
 /* Laterally varying data noise (AVOInversion::correctVpVsRho, avoinversion.cpp:2146-2302): per
  * trace (i, j) the 3 x 3 matrices of newPosteriorCovPointwise (:2028-2124) from the angle scales

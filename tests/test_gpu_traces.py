@@ -61,3 +61,38 @@ def test_fill_from_traces_refuses_short_traces(ctx):
     with pytest.raises(C.CravaError):
         g.fillInData(traces, [0.0], 4.0, ct, z, z + 4.0, 40.0)  # a 10-sample taper on 8 samples
     g.close()
+
+
+def test_segy_file_onto_the_grid(ctx, tmp_path):
+    """A SEG-Y file written from a grid (FFTGrid::writeSegyFile on the device), read back and
+    resampled onto a grid over the same box (CommonData::FillInData): the device grid equals the
+    oracle's columns from the same traces."""
+    import os
+    from crava_amd import seismic as SE
+    rng = np.random.default_rng(11)
+    nx, ny, nz, nxp, nyp, nzp = 10, 8, 50, 12, 10, 64
+    src = C.FFTGrid(ctx, nx, ny, nz, nxp, nyp, nzp, C.DATA)
+    kk = np.arange(nzp, dtype=np.float32)[:, None, None]
+    field = (np.sin(0.25 * kk + rng.uniform(0, 3, (1, nyp, 2 * (nxp // 2 + 1))))).astype(np.float32)
+    src.upload_real(field)
+    top = 1000.0 + rng.uniform(0.0, 30.0, (ny, nx))
+    geo = dict(x0=4000.0, y0=7000.0, dx=25.0, dy=25.0, angle=0.2, il0=100.2, xl0=200.3,
+               il_step_x=np.cos(0.2) / 25.0, il_step_y=np.sin(0.2) / 25.0, xl_step_x=-np.sin(0.2) / 25.0,
+               xl_step_y=np.cos(0.2) / 25.0, dz=4.0, z0=900.0, zmax=1400.0, top=top, rel_thick=np.ones((ny, nx)))
+    path = os.path.join(str(tmp_path), "in.segy")
+    src.writeSegyFile(path, geo)
+    transform = dict(in_line0=geo["il0"], cross_line0=geo["xl0"], x0=geo["x0"], y0=geo["y0"],
+                     il_step_x=geo["il_step_x"], il_step_y=geo["il_step_y"], xl_step_x=geo["xl_step_x"],
+                     xl_step_y=geo["xl_step_y"])
+    simbox = dict(x0=geo["x0"], y0=geo["y0"], dx=25.0, dy=25.0, angle=0.2, top=top, bot=top + nz * 4.0)
+    dst = C.FFTGrid(ctx, nx, ny, nz, nxp, nyp, nzp, C.DATA)
+    tr, ct = SE.fill_in_data_from_segy(dst, path, transform, simbox, 20.0)
+    got = dst.download_real(padded=True)
+    ct2, z0g, dzg, z0d = SE.resolve_columns(tr, SE.segy_lattice(tr), transform, simbox, nx, ny, nz, nxp, nyp)
+    want, _ = O.fill_from_traces(list(tr["samples"]), z0d, tr["dz"], ct2, z0g, dzg, nz, nzp, 20.0)
+    err = float(np.max(np.abs(got - want))) / float(np.max(np.abs(want)))
+    print("SEG-Y onto the grid: %d traces of %d samples, %d columns filled; max |device - oracle| / max %.2e"
+          % (len(tr["il"]), tr["samples"].shape[1], int((ct >= 0).sum()), err), flush=True)
+    assert err < 2e-6 and int((ct >= 0).sum()) == nxp * nyp
+    src.close()
+    dst.close()
