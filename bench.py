@@ -312,7 +312,35 @@ def bench_input_padding(ctx, inv, w, reps=5):
         pn, pms = ctx.kernel_time(C.K_ELEMENTWISE)
         ctx.set_profiling(False)
         corr_us[grad] = 1000.0 * pms / pn
+    # CommonData::FillInData's seismic branch: one trace per box column (nz + 2 x 25 samples of
+    # 4 ms, a 100 ms guard zone), the padding columns from their fill cells, onto the padded grid
+    from crava_amd import seismic as SE
+    ns_tr = nz + 50
+    tr = rng.standard_normal((nx * ny, ns_tr)).astype(np.float32)
+    ri = np.array([-1 if SE.fill_number(i, nx, w["nxp"]) is None else SE.fill_number(i, nx, w["nxp"])
+                   for i in range(dst.rnxp)])
+    rj = np.array([SE.fill_number(j, ny, w["nyp"]) for j in range(w["nyp"])])
+    col = np.where(ri[None, :] >= 0, rj[:, None] * nx + ri[None, :], -1).astype(np.int32)
+    z0g = np.full(col.shape, 100.0 + 2.0, np.float32)
+    dzg = np.full(col.shape, 4.0, np.float32)
+    z0d = np.zeros(nx * ny, np.float32)
+    seis.setType(C.DATA)
+    seis.fillInData(list(tr), z0d, 4.0, col, z0g, dzg, 100.0)  # warm-up
+    ctx.sync()
+    ctx.set_profiling(True)
+    ctx.reset_kernel_time()
+    t = time.perf_counter()
+    seis.fillInData(list(tr), z0d, 4.0, col, z0g, dzg, 100.0)
+    wall_tr = 1000.0 * (time.perf_counter() - t)
+    tk = sum(ctx.kernel_time(k)[1] for k in (C.K_ELEMENTWISE, C.K_FFT_X))
+    ctx.set_profiling(False)
+    seis.setType(C.PARAMETER)
+    nt_tr = C.lib().cg_factorable(ns_tr)
     return {
+        "fill_from_traces": {"what": "CommonData::FillInData (SEG-Y seismic branch): %d traces of %d samples "
+                                     "(guard-zone taper, ten-fold spectral resampling: R2C of %d, C2R of %d) "
+                                     "onto %dx%dx%d" % (nx * ny, ns_tr, nt_tr, 10 * nt_tr, w["nxp"], w["nyp"], w["nzp"]),
+                             "ms_synchronised_incl_host_copies": round(wall_tr, 2), "kernel_ms": round(tk, 3)},
         "fill_padded": {"what": "FFTGrid(NRLib::Grid) background fill, %dx%dx%d cube into %dx%dx%d"
                                 % (nx, ny, nz, w["nxp"], w["nyp"], w["nzp"]),
                         "avg_launch_us": round(1000.0 * ms / n, 2), "bytes_per_launch": fill_bytes,

@@ -94,6 +94,8 @@ int main(void)
   CHECK(cg_batch_create(NULL, 0, NULL, NULL) == CG_EINVAL);
   CHECK(cg_batch_run(NULL) == CG_EINVAL);
   CHECK(cg_invert(NULL, NULL) == CG_EINVAL);
+  CHECK(cg_grid_fill_from_traces(NULL, NULL) == CG_EINVAL);
+  CHECK(cg_segy_read(NULL, NULL, NULL) == CG_EINVAL);
   {
     cg_local_noise *ln = (cg_local_noise *)1;
     CHECK(cg_local_noise_create(NULL, NULL, &ln) == CG_EINVAL && ln == (cg_local_noise *)1);
