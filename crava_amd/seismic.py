@@ -95,37 +95,36 @@ def resolve_columns(traces, lattice, transform, simbox, nx, ny, nz, nxp, nyp):
     """FillInData's column loop for the seismic branch: (column_trace, z0_grid, dz_grid), each
     (nyp, rnxp), and the traces' z0_data (ms).  transform: in_line0, cross_line0, x0, y0,
     il_step_x, il_step_y, xl_step_x, xl_step_y (SegyGeometry::FindContILXL); simbox: x0, y0, dx,
-    dy, angle and top, bot (ny, nx) of the box cells (RMISSING where the top is missing)."""
+    dy, angle and top, bot (ny, nx) of the box cells (RMISSING where missing).  Vectorised over the
+    columns, each in the reference's arithmetic (the cell centre in double, rounded to float as the
+    reference passes it on; the lattice index truncated)."""
     rnxp = 2 * (nxp // 2 + 1)
-    ct = np.full((nyp, rnxp), -1, dtype=np.int32)
-    z0g = np.zeros((nyp, rnxp), np.float32)
-    dzg = np.ones((nyp, rnxp), np.float32)
+    ri = np.array([-1 if fill_number(i, nx, nxp) is None else fill_number(i, nx, nxp) for i in range(rnxp)])
+    rj = np.array([fill_number(j, ny, nyp) for j in range(nyp)])
+    RI, RJ = np.meshgrid(ri, rj)  # (nyp, rnxp)
+    has_cell = RI >= 0
+    RIc = np.where(has_cell, RI, 0)
     cr, sr = math.cos(simbox["angle"]), math.sin(simbox["angle"])
+    rx, ry = (RIc + 0.5) * simbox["dx"], (RJ + 0.5) * simbox["dy"]
+    xf = (rx * cr - ry * sr + simbox["x0"]).astype(np.float32).astype(np.float64)
+    yf = (rx * sr + ry * cr + simbox["y0"]).astype(np.float32).astype(np.float64)
     tf = transform
-    for j in range(nyp):
-        rj = fill_number(j, ny, nyp)
-        for i in range(rnxp):
-            ri = fill_number(i, nx, nxp)
-            if ri is None or rj is None:
-                continue
-            rx, ry = (ri + 0.5) * simbox["dx"], (rj + 0.5) * simbox["dy"]
-            xf = float(np.float32(rx * cr - ry * sr + simbox["x0"]))
-            yf = float(np.float32(rx * sr + ry * cr + simbox["y0"]))
-            il = tf["in_line0"] + (xf - tf["x0"]) * tf["il_step_x"] + (yf - tf["y0"]) * tf["il_step_y"]
-            xl = tf["cross_line0"] + (xf - tf["x0"]) * tf["xl_step_x"] + (yf - tf["y0"]) * tf["xl_step_y"]
-            xind = 0.5 + (il - lattice["min_il"]) / lattice["il_step"]
-            yind = 0.5 + (xl - lattice["min_xl"]) / lattice["xl_step"]
-            if not (0.0 <= xind < lattice["nx"] and 0.0 <= yind < lattice["ny"]):
-                continue  # outside the data: zeros
-            t = int(lattice["node"][int(xind), int(yind)])
-            top, bot = float(simbox["top"][rj, ri]), float(simbox["bot"][rj, ri])
-            if t < 0 or top == RMISSING or bot == RMISSING:
-                continue  # dead trace: zeros
-            dz = (bot - top) / nz
-            z0 = top + 0.5 * dz  # Simbox::getCoord(refi, refj, 0)
-            ct[j, i] = t
-            z0g[j, i] = np.float32(np.float32(z0) + np.float32(0.5) * np.float32(dz))
-            dzg[j, i] = np.float32(dz)
+    il = tf["in_line0"] + (xf - tf["x0"]) * tf["il_step_x"] + (yf - tf["y0"]) * tf["il_step_y"]
+    xl = tf["cross_line0"] + (xf - tf["x0"]) * tf["xl_step_x"] + (yf - tf["y0"]) * tf["xl_step_y"]
+    xind = 0.5 + (il - lattice["min_il"]) / lattice["il_step"]
+    yind = 0.5 + (xl - lattice["min_xl"]) / lattice["xl_step"]
+    inside = has_cell & (xind >= 0.0) & (xind < lattice["nx"]) & (yind >= 0.0) & (yind < lattice["ny"])
+    xi = np.where(inside, xind, 0.0).astype(np.int64)
+    yi = np.where(inside, yind, 0.0).astype(np.int64)
+    t = np.where(inside, lattice["node"][xi, yi], -1)
+    top = np.asarray(simbox["top"], dtype=np.float64)[RJ, RIc]
+    bot = np.asarray(simbox["bot"], dtype=np.float64)[RJ, RIc]
+    live = (t >= 0) & (top != RMISSING) & (bot != RMISSING)  # else outside or dead: zeros
+    dz = np.where(live, (bot - top) / nz, 1.0)
+    z0 = top + 0.5 * dz  # Simbox::getCoord(refi, refj, 0)
+    ct = np.where(live, t, -1).astype(np.int32)
+    z0g = np.where(live, z0.astype(np.float32) + np.float32(0.5) * dz.astype(np.float32), 0.0).astype(np.float32)
+    dzg = dz.astype(np.float32)
     return ct, z0g, dzg, np.asarray(traces["start_time"], np.float32)
 
 
