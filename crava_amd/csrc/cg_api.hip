@@ -1902,18 +1902,27 @@ int cg_grid_fill_from_traces(cg_grid *g, const cg_trace_fill_args *a)
   if (a->ntraces < 1 || a->stride < 1) return cg::fail(ctx, CG_EINVAL, "fill_from_traces: no traces");
   if (!(a->dz_data > 0.0f) || !(a->smooth_length >= 0.0f))
     return cg::fail(ctx, CG_EINVAL, "fill_from_traces: bad sample interval or smoothing length");
+  if (a->grid_type != 0 && a->grid_type != CG_CT_DATA && a->grid_type != CG_CT_PARAMETER)
+    return cg::fail(ctx, CG_EINVAL, "fill_from_traces: grid_type must be CG_CT_DATA or CG_CT_PARAMETER");
+  const bool detrend = a->grid_type == CG_CT_PARAMETER;
   const int ns = (int)std::floor(a->smooth_length / a->dz_data);
   int maxlen = 0;
   for (int t = 0; t < a->ntraces; ++t) {
     const int len = a->length[t];
     if (len < 1 || len > a->stride) return cg::fail(ctx, CG_EINVAL, "fill_from_traces: trace length outside 1..stride");
-    if (len < ns) return cg::fail(ctx, CG_EINVAL, "fill_from_traces: a trace is shorter than its guard-zone taper");
+    if (!detrend && len < ns)
+      return cg::fail(ctx, CG_EINVAL, "fill_from_traces: a trace is shorter than its guard-zone taper");
     maxlen = std::max(maxlen, len);
   }
   const long ncol = (long)g->nyp * g->rnxp;
   for (long c = 0; c < ncol; ++c)
-    if (a->column_trace[c] < -1 || a->column_trace[c] >= a->ntraces)
+    if (a->column_trace[c] < -2 || a->column_trace[c] >= a->ntraces)
       return cg::fail(ctx, CG_EINVAL, "fill_from_traces: column trace index out of range");
+  if (detrend)  // FillInData stops at the first box column outside the data (commondata.cpp:6767-6770)
+    for (int j = 0; j < g->ny; ++j)
+      for (int i = 0; i < g->nx; ++i)
+        if (a->column_trace[(long)j * g->rnxp + i] == -1)
+          return cg::fail(ctx, CG_EINVAL, "fill_from_traces: the traces do not cover the box");
   const int nt = cg_factorable(maxlen);
   if (nt > 1024) return cg::fail(ctx, CG_EINVAL, "fill_from_traces: traces longer than 1024 samples");
   DeviceGuard dg(ctx->device);
@@ -1933,7 +1942,9 @@ int cg_grid_fill_from_traces(cg_grid *g, const cg_trace_fill_args *a)
   p.nzp = g->nzp;
   p.rnxp = g->rnxp;
   p.grid = g->d;
-  // one device block: samples, lengths, z0_data, columns (trace, z0, dz), the two trace buffers
+  p.detrend = detrend ? 1 : 0;
+  // one device block: samples, lengths, z0_data, columns (trace, z0, dz), the two trace buffers,
+  // the detrended traces' TraceInfo
   Layout L;
   const size_t o_s = L.add(sizeof(float) * (size_t)a->ntraces * a->stride);
   const size_t o_len = L.add(sizeof(int) * a->ntraces);
@@ -1943,6 +1954,7 @@ int cg_grid_fill_from_traces(cg_grid *g, const cg_trace_fill_args *a)
   const size_t o_dzg = L.add(sizeof(float) * ncol);
   const size_t o_a = L.add(sizeof(float) * (size_t)a->ntraces * p.rnt);
   const size_t o_b = L.add(sizeof(float) * (size_t)a->ntraces * p.rmt);
+  const size_t o_info = L.add(detrend ? sizeof(cg::TraceInfo) * a->ntraces : 0);
   char *dev = nullptr;
   hipError_t e = cg::dev_alloc(ctx, reinterpret_cast<void **>(&dev), L.size);
   if (e != hipSuccess) return cg::hip_fail(ctx, e, "fill_from_traces: device buffers");
@@ -1968,7 +1980,8 @@ int cg_grid_fill_from_traces(cg_grid *g, const cg_trace_fill_args *a)
   p.dz_grid = reinterpret_cast<const float *>(dev + o_dzg);
   p.a = reinterpret_cast<float *>(dev + o_a);
   p.b = reinterpret_cast<float *>(dev + o_b);
-  if (!rc) rc = cg::pack_traces(ctx, p);
+  p.info = detrend ? reinterpret_cast<cg::TraceInfo *>(dev + o_info) : nullptr;
+  if (!rc) rc = detrend ? cg::pack_detrend(ctx, p) : cg::pack_traces(ctx, p);
   if (!rc) rc = cg::fft_x_r2c(ctx, p.a, p.a, nt, a->ntraces, 1.0f);
   if (!rc) rc = cg::spread_spectra(ctx, p);
   if (!rc) rc = cg::fft_x_c2r(ctx, p.b, p.mt, a->ntraces, 1 / static_cast<float>(nt), nullptr);
@@ -1978,7 +1991,7 @@ int cg_grid_fill_from_traces(cg_grid *g, const cg_trace_fill_args *a)
   e = hipStreamSynchronize(ctx->stream);
   if (!rc) rc = cg::hip_fail(ctx, e, "fill_from_traces");
   if (!rc) {
-    g->cubetype = CG_CT_DATA;
+    g->cubetype = detrend ? CG_CT_PARAMETER : CG_CT_DATA;
     g->transformed = 0;
   }
   return rc;

@@ -30,10 +30,17 @@ struct ScaleWaveletParams {
 // max(1 - dist^2, 0) for k = nz + 1 .. nzp - 1.
 int scale_wavelet_filter(cg_ctx *ctx, const ScaleWaveletParams &p, const float *taper_dev);
 
-// CommonData::FillInData (commondata.cpp:6451-6775), the SEG-Y DATA branch: traces tapered in
-// the guard zone and zero-padded (k_pack_traces), their length-nt spectra (the x pass), spread
-// into length-mt spectra (k_spread_spectra), fine traces (the x pass inverse, scale 1 / nt), the
-// grid's columns interpolated from them (k_fill_columns).
+// CommonData::FillInData (commondata.cpp:6451-6775) from SEG-Y traces: traces tapered in the
+// guard zone (DATA grids, k_pack_traces) or trimmed of zeros and detrended (other grids,
+// k_pack_detrend) and zero-padded, their length-nt spectra (the x pass), spread into length-mt
+// spectra (k_spread_spectra), fine traces (the x pass inverse, scale 1 / nt), the grid's columns
+// interpolated from them, the trend added back for other grids (k_fill_columns).
+struct TraceInfo {  // per trace of a detrended (non-DATA) fill
+  float z0;         // time of the first kept sample
+  float first;      // trend at the first kept sample
+  float inc2;       // trend increment per fine sample
+  int n_trend;      // fine trend samples, 10 (n - 1) + 1; 0: no nonzero sample (a dead trace)
+};
 struct TraceFillParams {
   const float *samples;    // [ntraces][stride]
   const int *length;       // [ntraces]
@@ -51,8 +58,11 @@ struct TraceFillParams {
   float dz_fine;
   float *grid;             // real grid, in-place layout
   int nz, nyp, nzp, rnxp;
+  int detrend;             // non-DATA grid: k_pack_detrend's trimming and trend, RMISSING when dead
+  TraceInfo *info;         // [ntraces] when detrend
 };
 int pack_traces(cg_ctx *ctx, const TraceFillParams &p);
+int pack_detrend(cg_ctx *ctx, const TraceFillParams &p);  // stride <= 1024
 int spread_spectra(cg_ctx *ctx, const TraceFillParams &p);
 int fill_columns(cg_ctx *ctx, const TraceFillParams &p);
 

@@ -258,6 +258,68 @@ __global__ void __launch_bounds__(cg::kThreads) k_pack_traces(cg::TraceFillParam
   }
 }
 
+// FillInData's preparation of a trace for a non-DATA grid (commondata.cpp:6612-6644), one
+// workgroup per trace (at most 1024 samples, four per thread): where the first sample is zero,
+// every zero sample is dropped (kept samples compacted by a workgroup scan) and z0 moves down one
+// dz_data per leading zero (added one at a time, as the reference does); the line through the
+// first and last kept samples is subtracted, in float: s[m] - (first + m inc),
+// inc = (last - first) / (n - 1); zero padding to rnt.  The trend's fine samples
+// are first + l inc2, inc2 = (last - first) / (10 (n - 1)), l < 10 (n - 1) + 1 (:6684-6692).
+constexpr int kDetrendThreads = 256;
+__global__ void __launch_bounds__(kDetrendThreads) k_pack_detrend(cg::TraceFillParams p)
+{
+#pragma clang fp contract(off)
+  __shared__ float kept[1024];
+  __shared__ int scan[kDetrendThreads];
+  __shared__ int lead;
+  const int t = blockIdx.x, tid = threadIdx.x;
+  const int len = p.length[t];
+  const float *src = p.samples + (long)t * p.stride;
+  const bool trim = src[0] == 0.0f;
+  float v[4];
+  int c = 0;
+  if (tid == 0) lead = len;
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int n = 4 * tid + q;
+    v[q] = n < len ? src[n] : 0.0f;
+    if (n < len && v[q] != 0.0f) atomicMin(&lead, n);
+    c += (n < len && (!trim || v[q] != 0.0f)) ? 1 : 0;
+  }
+  scan[tid] = c;
+  __syncthreads();
+  for (int d = 1; d < kDetrendThreads; d <<= 1) {  // inclusive scan of the kept counts
+    const int add = tid >= d ? scan[tid - d] : 0;
+    __syncthreads();
+    scan[tid] += add;
+    __syncthreads();
+  }
+  int pos = scan[tid] - c;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int n = 4 * tid + q;
+    if (n < len && (!trim || v[q] != 0.0f)) kept[pos++] = v[q];
+  }
+  __syncthreads();
+  const int nk = scan[kDetrendThreads - 1];
+  const float first = nk > 0 ? kept[0] : 0.0f, last = nk > 0 ? kept[nk - 1] : 0.0f;
+  const float inc = (last - first) / (float)(nk - 1);
+  float *dst = p.a + (long)t * p.rnt;
+  for (int m = tid; m < p.rnt; m += kDetrendThreads) dst[m] = m < nk ? kept[m] - (first + (float)m * inc) : 0.0f;
+  if (tid == 0) {
+    float z0 = p.z0_data[t];
+    if (trim)
+      for (int i = 0; i < lead; ++i) z0 = z0 + p.dz_data;
+    cg::TraceInfo ti;
+    ti.z0 = z0;
+    ti.first = first;
+    ti.inc2 = (last - first) / (10.0f * (float)(nk - 1));
+    ti.n_trend = nk > 0 ? (int)(10.0f * (float)(nk - 1) + 1.0f) : 0;
+    p.info[t] = ti;
+  }
+}
+
 // The cnt = rnt / 2 bins of each trace's spectrum as the lowest bins of a zero spectrum of
 // cmt = rmt / 2 bins (ResampleTrace, commondata.h:166-176).
 __global__ void __launch_bounds__(cg::kThreads) k_spread_spectra(cg::TraceFillParams p)
@@ -269,9 +331,13 @@ __global__ void __launch_bounds__(cg::kThreads) k_spread_spectra(cg::TraceFillPa
   }
 }
 
+constexpr float kMissing = -99999.0f;  // RMISSING
+
 // CommonData::InterpolateGridValues (commondata.cpp:6947-6995) with GetZSimboxIndex (:7052-7062)
 // and SetTrace (:7084-7092): real cell (i, j, k) of the padded grid from the fine trace of its
-// column's trace (zero for columns without one), in the reference's float arithmetic.
+// column's trace (zero for columns without one), in the reference's float arithmetic.  Non-DATA
+// grids add the trend interpolated by InterpolateAndShiftTrend (:6997-7050: its end values
+// outside it) and take RMISSING in dead columns (:6731-6742).
 __global__ void __launch_bounds__(cg::kThreads) k_fill_columns(cg::TraceFillParams p)
 {
 #pragma clang fp contract(off)
@@ -283,8 +349,12 @@ __global__ void __launch_bounds__(cg::kThreads) k_fill_columns(cg::TraceFillPara
     const long col = idx - (long)k * plane;
     const int t = p.column_trace[col];
     float v = 0.f;
-    if (t >= 0) {
-      const float z0_shift = p.z0_grid[col] - p.z0_data[t];
+    cg::TraceInfo ti{0.0f, 0.0f, 0.0f, 1};
+    if (p.detrend && t >= 0) ti = p.info[t];
+    if (p.detrend && (t == -2 || (t >= 0 && ti.n_trend == 0))) {
+      v = kMissing;
+    } else if (t >= 0) {
+      const float z0_shift = p.z0_grid[col] - (p.detrend ? ti.z0 : p.z0_data[t]);
       const int refk = k < (p.nz + p.nzp) / 2 ? k : k - p.nzp;
       const float dl = (z0_shift + (float)refk * p.dz_grid[col]) * inv;
       const float fl = floorf(dl), cl = ceilf(dl);
@@ -300,6 +370,17 @@ __global__ void __launch_bounds__(cg::kThreads) k_fill_columns(cg::TraceFillPara
       } else {
         const float w1 = cl - dl, w2 = dl - fl;
         v = w1 * fine[l1] + w2 * fine[l2];
+      }
+      if (p.detrend) {
+        const int nt = ti.n_trend;
+        const auto trend = [&](int l) { return ti.first + (float)l * ti.inc2; };
+        float tr;
+        if (l2 < 0) tr = trend(0);
+        else if (l1 > nt - 1) tr = trend(nt - 1);
+        else if (l1 < 0) tr = trend(l2);
+        else if (l2 > nt - 1 || l1 == l2) tr = trend(l1);
+        else tr = (cl - dl) * trend(l1) + (dl - fl) * trend(l2);
+        v = v + tr;
       }
     }
     p.grid[idx] = v;
@@ -323,6 +404,15 @@ int pack_traces(cg_ctx *ctx, const TraceFillParams &p)
   cgtrace::k_pack_traces<<<grid_stride_blocks((long)p.ntraces * p.rnt), kThreads, 0, ctx->stream>>>(p);
   launch_end(ctx, CG_K_ELEMENTWISE, ev);
   return hip_fail(ctx, hipGetLastError(), "k_pack_traces");
+}
+
+int pack_detrend(cg_ctx *ctx, const TraceFillParams &p)
+{
+  hipEvent_t ev;
+  launch_begin(ctx, CG_K_ELEMENTWISE, &ev);
+  cgtrace::k_pack_detrend<<<(unsigned)p.ntraces, cgtrace::kDetrendThreads, 0, ctx->stream>>>(p);
+  launch_end(ctx, CG_K_ELEMENTWISE, ev);
+  return hip_fail(ctx, hipGetLastError(), "k_pack_detrend");
 }
 
 int spread_spectra(cg_ctx *ctx, const TraceFillParams &p)

@@ -1,12 +1,17 @@
-"""Seismic input from SEG-Y files onto the padded grid: NRLib's SegY reading (cg_segy_read) and
-CommonData::FillInData's column loop (commondata.cpp:6451-6775) for the seismic (DATA) branch,
-whose per-trace work runs on the device (FFTGrid.fillInData, cg_grid_fill_from_traces).
+"""Seismic and parameter input from SEG-Y files onto the grid: NRLib's SegY reading
+(cg_segy_read), its cut of the traces to the inversion volume (window_traces) and
+CommonData::FillInData's column loop (commondata.cpp:6451-6775) for the seismic (DATA) branch and
+the branch for other grids (background and parameter cubes, PARAMETER), whose per-trace work runs
+on the device (FFTGrid.fillInData, cg_grid_fill_from_traces).
 
 The column loop follows the reference: column (i, j) of the padded grid, i < rnxp, takes the box
 cell refi = GetFillNumber(i, nx, nxp), refj = GetFillNumber(j, ny, nyp) (no cell for i >= nxp),
 its centre from Simbox::getCoord(refi, refj, 0) (simbox.cpp:814-842), the SEG-Y trace nearest to
 that centre (SegyGeometry::FindContIndex / FindIndex and SegY::GetNearestTrace, segy.cpp:1031-1050,
-segygeometry.cpp:955-1035) and z0 + dz / 2 and dz of the cell column (commondata.cpp:6555-6640).
+segygeometry.cpp:955-1035) and z0 + dz / 2 and dz of the cell column (commondata.cpp:6555-6640;
+z0 itself for the other grids, whose FillInData call leaves is_seismic false).  The trace's first
+sample is at z0_ + (j0 + 0.5) dz, less dz / 2 for seismic data (commondata.cpp:6582-6585), z0_ the
+file's start time (the first trace's, SegY::ReadTrace) and j0 the first sample the volume cut kept.
 The lattice of the SEG-Y volume is the one its inline and crossline numbers span, with the
 first inline and crossline the smallest ones and inlines along the first axis (NRLib's
 SegyGeometry when IL0 == minIL, XL0 == minXL and firstAxisIL); the continuous inline / crossline
@@ -16,7 +21,7 @@ import math
 
 import numpy as np
 
-from . import FFTGrid, CravaError, lib  # noqa: F401
+from . import DATA, PARAMETER, FFTGrid, CravaError, lib  # noqa: F401
 
 RMISSING = -99999.0
 
@@ -60,6 +65,37 @@ def read_segy(path, header_positions=None):
     return out
 
 
+def window_traces(traces, z_top, z_bot, z_pad, inside=None, offset=None):
+    """SegY::ReadAllTraces / ReadTrace with a volume (segy.cpp:619-890): each trace cut to the
+    samples j0..j1 around the volume's top and base at the trace, j0 = max(0, int((zTop - pad -
+    z0_) / dz)), j1 = min(ns - 1, (zBot + pad - z0_) / dz) (truncated), pad = z_pad / 2, in float as
+    the reader computes them; z0_ the offset if given, else the first trace's start time.  A trace
+    outside the volume (inside[t] false: only_volume), with a missing surface value (RMISSING in
+    z_top or z_bot) or with the volume reaching past its data (zTop < z0_ or zBot > z0_ + ns dz)
+    is not kept.  Returns the traces as read_segy does, the samples as a list of the kept traces
+    (None where not kept), with start (j0 per trace) and z0 (z0_)."""
+    f32 = np.float32
+    ns = traces["samples"].shape[1]
+    dz = f32(traces["dz"])
+    z0 = f32(offset) if offset is not None else f32(traces["start_time"][0])
+    pad = f32(0.5 * z_pad)
+    s_bot = f32(z0 + f32(f32(ns) * dz))
+    kept, start = [], np.zeros(len(traces["il"]), dtype=np.int64)
+    for t in range(len(traces["il"])):
+        zt, zb = f32(z_top[t]), f32(z_bot[t])
+        if (inside is not None and not inside[t]) or zt == f32(RMISSING) or zb == f32(RMISSING) or zt < z0 or \
+                zb > s_bot:
+            kept.append(None)
+            continue
+        j0 = max(0, int(f32(f32(f32(zt - pad) - z0) / dz)))
+        j1 = min(ns - 1, int(f32(f32(f32(zb + pad) - z0) / dz)))
+        if j0 > j1:
+            raise CravaError("window_traces: lower horizon above the SEG-Y data or upper horizon below it")
+        kept.append(np.asarray(traces["samples"][t, j0:j1 + 1], dtype=np.float32))
+        start[t] = j0
+    return dict(traces, samples=kept, start=start, z0=float(z0))
+
+
 def fill_number(i, n, np_):
     """CommonData::GetFillNumber (commondata.cpp:6777-6814): the box index whose trace fills
     padded index i, None at or past np_."""
@@ -91,13 +127,17 @@ def segy_lattice(traces):
     return lat
 
 
-def resolve_columns(traces, lattice, transform, simbox, nx, ny, nz, nxp, nyp):
-    """FillInData's column loop for the seismic branch: (column_trace, z0_grid, dz_grid), each
-    (nyp, rnxp), and the traces' z0_data (ms).  transform: in_line0, cross_line0, x0, y0,
-    il_step_x, il_step_y, xl_step_x, xl_step_y (SegyGeometry::FindContILXL); simbox: x0, y0, dx,
-    dy, angle and top, bot (ny, nx) of the box cells (RMISSING where missing).  Vectorised over the
-    columns, each in the reference's arithmetic (the cell centre in double, rounded to float as the
-    reference passes it on; the lattice index truncated)."""
+def resolve_columns(traces, lattice, transform, simbox, nx, ny, nz, nxp, nyp, grid_type=DATA):
+    """FillInData's column loop: (column_trace, z0_grid, dz_grid), each (nyp, rnxp), and the
+    traces' z0_data (ms).  column_trace: the trace of the column, -1 outside the data, -2 a dead
+    column (no trace kept at its lattice node, or a missing cell).  transform: in_line0,
+    cross_line0, x0, y0, il_step_x, il_step_y, xl_step_x, xl_step_y (SegyGeometry::FindContILXL);
+    simbox: x0, y0, dx, dy, angle and top, bot (ny, nx) of the box cells (RMISSING where missing).
+    traces: read_segy's, or window_traces' (a trace not kept has no samples).  grid_type DATA: the
+    seismic call (is_seismic true); PARAMETER: the other grids'.  Vectorised over the columns, each
+    in the reference's arithmetic (the cell centre in double, rounded to float as the reference
+    passes it on; the lattice index truncated)."""
+    seismic = grid_type == DATA
     rnxp = 2 * (nxp // 2 + 1)
     ri = np.array([-1 if fill_number(i, nx, nxp) is None else fill_number(i, nx, nxp) for i in range(rnxp)])
     rj = np.array([fill_number(j, ny, nyp) for j in range(nyp)])
@@ -117,22 +157,55 @@ def resolve_columns(traces, lattice, transform, simbox, nx, ny, nz, nxp, nyp):
     xi = np.where(inside, xind, 0.0).astype(np.int64)
     yi = np.where(inside, yind, 0.0).astype(np.int64)
     t = np.where(inside, lattice["node"][xi, yi], -1)
+    samples = traces["samples"]
+    kept = np.array([samples[q] is not None for q in range(len(samples))]) if isinstance(samples, list) else \
+        np.ones(len(traces["il"]), dtype=bool)
     top = np.asarray(simbox["top"], dtype=np.float64)[RJ, RIc]
     bot = np.asarray(simbox["bot"], dtype=np.float64)[RJ, RIc]
-    live = (t >= 0) & (top != RMISSING) & (bot != RMISSING)  # else outside or dead: zeros
+    live = (t >= 0) & (top != RMISSING) & (bot != RMISSING)
+    live &= kept[np.maximum(t, 0)]
     dz = np.where(live, (bot - top) / nz, 1.0)
     z0 = top + 0.5 * dz  # Simbox::getCoord(refi, refj, 0)
-    ct = np.where(live, t, -1).astype(np.int32)
-    z0g = np.where(live, z0.astype(np.float32) + np.float32(0.5) * dz.astype(np.float32), 0.0).astype(np.float32)
+    ct = np.where(live, t, np.where(inside, -2, -1)).astype(np.int32)
+    z0g = z0.astype(np.float32)
+    if seismic:
+        z0g = z0g + np.float32(0.5) * dz.astype(np.float32)
+    z0g = np.where(live, z0g, 0.0).astype(np.float32)
     dzg = dz.astype(np.float32)
-    return ct, z0g, dzg, np.asarray(traces["start_time"], np.float32)
+    # SegY::GetNearestTrace's start time (segy.cpp:1045), less half a sample for seismic data
+    f32 = np.float32
+    dzd = f32(traces["dz"])
+    z0f = f32(traces.get("z0", traces["start_time"][0]))
+    start = np.asarray(traces.get("start", np.zeros(len(traces["il"]))), dtype=np.float32)
+    z0d = (z0f + ((start + f32(0.5)).astype(f32) * dzd).astype(f32)).astype(f32)
+    if seismic:
+        z0d = (z0d - f32(f32(0.5) * dzd)).astype(f32)
+    return ct, z0g, dzg, z0d
 
 
-def fill_in_data_from_segy(grid, path, transform, simbox, smooth_length, header_positions=None):
-    """CommonData::FillInData for a seismic FFTGrid from a SEG-Y file: the traces read, the
-    columns resolved on the host, the per-trace work on the device."""
+def kept_traces(traces, column_trace, z0_data):
+    """The traces the columns use, as fillInData takes them: (list of samples, their z0_data, the
+    columns renumbered); traces window_traces did not keep are left out."""
+    s = traces["samples"]
+    keep = [q for q in range(len(s)) if s[q] is not None]
+    remap = np.full(len(s) + 1, -1, dtype=np.int32)
+    remap[keep] = np.arange(len(keep), dtype=np.int32)
+    ct = np.where(column_trace >= 0, remap[np.maximum(column_trace, 0)], column_trace).astype(np.int32)
+    return [np.asarray(s[q], dtype=np.float32) for q in keep], np.asarray(z0_data, np.float32)[keep], ct
+
+
+def fill_in_data_from_segy(grid, path, transform, simbox, smooth_length, header_positions=None, grid_type=DATA,
+                           window=None):
+    """CommonData::FillInData for an FFTGrid from a SEG-Y file: the traces read (and cut to the
+    volume when window = dict(z_top, z_bot, z_pad[, inside, offset]) gives it), the columns
+    resolved on the host, the per-trace work on the device.  grid_type DATA: seismic data;
+    PARAMETER: a background or parameter cube (CravaError if the data do not cover the box)."""
     tr = read_segy(path, header_positions)
+    if window is not None:
+        tr = window_traces(tr, **window)
     lat = segy_lattice(tr)
-    ct, z0g, dzg, z0d = resolve_columns(tr, lat, transform, simbox, grid.nx, grid.ny, grid.nz, grid.nxp, grid.nyp)
-    grid.fillInData(list(tr["samples"]), z0d, tr["dz"], ct, z0g, dzg, smooth_length)
+    ct, z0g, dzg, z0d = resolve_columns(tr, lat, transform, simbox, grid.nx, grid.ny, grid.nz, grid.nxp, grid.nyp,
+                                        grid_type)
+    samples, z0k, ctk = kept_traces(tr, ct, z0d)
+    grid.fillInData(samples, z0k, tr["dz"], ctk, z0g, dzg, smooth_length, grid_type)
     return tr, ct

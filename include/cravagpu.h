@@ -279,13 +279,23 @@ int cg_grid_read_storm(cg_grid *grid, const char *path, cg_storm_geometry *geome
 /* CommonData::FillInData (commondata.cpp:6451-6775), the SEG-Y seismic (DATA) branch, with the
  * columns resolved by the caller as FillInData resolves them (GetFillNumber, Simbox::getCoord /
  * getdz, SegyGeometry::IsInside, SegY::GetNearestTrace): column (i, j) of the padded grid,
- * i < rnxp, takes trace column_trace[j][i], or zeros where that is -1 (outside the data, or a
- * dead trace).  Each trace is tapered over floor(smooth_length / dz_data) samples at each end
+ * i < rnxp, takes trace column_trace[j][i], or zeros where that is -1 (outside the data) or -2
+ * (a dead trace).  Each trace is tapered over floor(smooth_length / dz_data) samples at each end
  * (SmoothTraceInGuardZone, :6847-6885), resampled ten times finer through its spectrum of length
  * nt = the closest 2^a 3^b 5^c 7^d >= the longest trace (ResampleTrace, commondata.h:136-190),
  * and interpolated at the cell centres z0_grid + refk dz_grid, refk = GetZSimboxIndex(k)
  * (InterpolateGridValues, :6947-6995).  The grid becomes a real-domain DATA grid.  Traces at most
- * 1024 samples, none shorter than its taper; host arrays, copied before the call returns. */
+ * 1024 samples, none shorter than its taper; host arrays, copied before the call returns.
+ * grid_type CG_CT_PARAMETER: FillInData's branch for other grids (background and parameter cubes
+ * from SEG-Y, :6613-6644 and :6684-6721): no taper; where a trace's first sample is zero, its zero
+ * samples are dropped and z0_data moves down one dz_data per leading zero; the line through the
+ * first and last samples is subtracted before the resampling and, resampled linearly ten times
+ * finer, interpolated at the cells (InterpolateAndShiftTrend, :6997-7050, its end values outside)
+ * and added back.  column_trace -2 marks a dead trace (or a missing cell column): RMISSING there
+ * and for traces with no nonzero sample (zeros in the DATA branch).  Every box column (i < nx,
+ * j < ny) must have a trace or be dead (the reference stops and reports the grid as not covering
+ * the area): CG_EINVAL otherwise.  The grid becomes a real-domain PARAMETER grid; z0_data is then
+ * GetNearestTrace's z0 itself, z0_grid the cell centre z0 (is_seismic false). */
 typedef struct {
   const float *samples;      /* [ntraces][stride] */
   const int *length;         /* [ntraces] samples of each trace, 1..stride */
@@ -297,6 +307,7 @@ typedef struct {
   const int *column_trace;   /* [nyp][rnxp] */
   const float *z0_grid;      /* [nyp][rnxp] centre of the column's first cell, ms (z0 + dz / 2) */
   const float *dz_grid;      /* [nyp][rnxp] */
+  int grid_type;             /* CG_CT_DATA (or 0) / CG_CT_PARAMETER */
 } cg_trace_fill_args;
 int cg_grid_fill_from_traces(cg_grid *grid, const cg_trace_fill_args *args);
 

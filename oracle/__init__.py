@@ -1052,25 +1052,103 @@ def interpolate_grid_values(fine, z0_grid, dz_grid, z0_data, dz_fine, nz, nzp):
     return out
 
 
-def fill_from_traces(traces, z0_data, dz_data, column_trace, z0_grid, dz_grid, nz, nzp, smooth_length):
-    """The columns of FillInData's DATA branch: column (j, i) of the padded real grid
-    (nzp, nyp, rnxp) takes trace column_trace[j, i] (zeros where negative) with its first sample
-    at z0_data[t] (ms), tapered (smooth_trace_in_guard_zone), resampled (resample_trace with nt
-    the closest factorable length of the longest trace, rounded to float as the reference's
-    buffer is) and interpolated at z0_grid[j, i] + refk dz_grid[j, i] (interpolate_grid_values,
-    dz_fine = dz_data / 10).  Returns (grid, nt)."""
+def interpolate_and_shift_trend(trend_long, z0_grid, dz_grid, z0_data, dz_fine, nz, nzp):
+    """CommonData::InterpolateAndShiftTrend (commondata.cpp:6997-7050): as
+    interpolate_grid_values on the fine trend, but the trend's first or last value outside it."""
+    n_fine = len(trend_long)
+    f32 = np.float32
+    z0_shift = f32(f32(z0_grid) - f32(z0_data))
+    inv = f32(f32(1.0) / f32(dz_fine))
+    out = np.zeros(nzp, dtype=np.float32)
+    for k in range(nzp):
+        refk = z_simbox_index(k, nz, nzp)
+        dl = f32(f32(z0_shift + f32(f32(refk) * f32(dz_grid))) * inv)
+        l1, l2 = int(math.floor(dl)), int(math.ceil(dl))
+        if l2 < 0 or l1 > n_fine - 1:
+            v = trend_long[0] if l2 < 0 else trend_long[n_fine - 1]
+        elif l1 < 0:
+            v = trend_long[l2]
+        elif l2 > n_fine - 1 or l1 == l2:
+            v = trend_long[l1]
+        else:
+            w1, w2 = f32(math.ceil(dl) - dl), f32(dl - math.floor(dl))
+            v = f32(w1 * f32(trend_long[l1])) + f32(w2 * f32(trend_long[l2]))
+        out[k] = f32(v)
+    return out
+
+
+def trim_and_detrend(trace, z0_data, dz_data):
+    """FillInData's preparation of a trace for a parameter (not DATA) grid
+    (commondata.cpp:6612-6644): where the first sample is zero, every zero sample is dropped and z0
+    moves down one sample per leading zero; then the line through the first and last samples is
+    subtracted (float).  Returns (trace, z0, first, last) or None for an empty trace."""
+    f32 = np.float32
+    t = np.asarray(trace, dtype=np.float32)
+    z0 = f32(z0_data)
+    if len(t) and t[0] == 0:
+        lead = 0
+        while lead < len(t) and t[lead] == 0:
+            lead += 1
+        for _ in range(lead):
+            z0 = f32(z0 + f32(dz_data))
+        t = t[t != 0]
+    n = len(t)
+    if n == 0:
+        return None
+    first, last = f32(t[0]), f32(t[n - 1])
+    inc = f32(f32(last - first) / f32(n - 1))
+    k = np.arange(n).astype(np.float32)
+    t = (t - (first + (k * inc).astype(np.float32)).astype(np.float32)).astype(np.float32)
+    return t, z0, first, last
+
+
+def fill_from_traces(traces, z0_data, dz_data, column_trace, z0_grid, dz_grid, nz, nzp, smooth_length,
+                     parameter=False):
+    """The columns of FillInData (commondata.cpp:6451-6775) from SEG-Y traces: column (j, i) of the
+    padded real grid (nzp, nyp, rnxp) takes trace column_trace[j, i] with its first sample at
+    z0_data[t] (ms); -1: outside the data (zeros), -2: a dead trace (zeros, RMISSING for a
+    parameter grid).  Seismic (DATA) grids: tapered (smooth_trace_in_guard_zone), resampled
+    (resample_trace with nt the closest factorable length of the longest trace, rounded to float
+    as the reference's buffer is) and interpolated at z0_grid[j, i] + refk dz_grid[j, i]
+    (interpolate_grid_values, dz_fine = dz_data / 10).  Parameter grids (parameter=True): trimmed
+    and detrended (trim_and_detrend) instead of tapered, the trend resampled linearly ten times
+    finer (res_fac (n - 1) + 1 samples) and added back (interpolate_and_shift_trend); an empty
+    trace is dead.  Returns (grid, nt)."""
+    f32 = np.float32
     nyp, rnxp = column_trace.shape
     nt = lib().orc_factorable(max(len(t) for t in traces))
-    dz_fine = np.float32(np.float32(dz_data) / np.float32(10.0))
+    dz_fine = f32(f32(dz_data) / f32(10.0))
     fine = {}
     out = np.zeros((nzp, nyp, rnxp), dtype=np.float32)
     for j in range(nyp):
         for i in range(rnxp):
             t = int(column_trace[j, i])
+            if t == -2 and parameter:
+                out[:, j, i] = f32(RMISSING)
             if t < 0:
                 continue
             if t not in fine:
-                sm = smooth_trace_in_guard_zone(traces[t], dz_data, smooth_length)
-                fine[t] = resample_trace(sm, nt).astype(np.float32)
-            out[:, j, i] = interpolate_grid_values(fine[t], z0_grid[j, i], dz_grid[j, i], z0_data[t], dz_fine, nz, nzp)
+                if parameter:
+                    r = trim_and_detrend(traces[t], z0_data[t], dz_data)
+                    if r is None:
+                        fine[t] = None
+                    else:
+                        tr, z0, first, last = r
+                        n = len(tr)
+                        nl = int(f32(f32(10.0) * f32(n - 1)) + f32(1))
+                        inc2 = f32(f32(last - first) / f32(f32(10.0) * f32(n - 1)))
+                        tl = (first + (np.arange(nl).astype(np.float32) * inc2).astype(np.float32)).astype(np.float32)
+                        fine[t] = (resample_trace(tr, nt).astype(np.float32), z0, tl)
+                else:
+                    sm = smooth_trace_in_guard_zone(traces[t], dz_data, smooth_length)
+                    fine[t] = (resample_trace(sm, nt).astype(np.float32), f32(z0_data[t]), None)
+            if fine[t] is None:
+                out[:, j, i] = f32(RMISSING)
+                continue
+            fn, z0, tl = fine[t]
+            col = interpolate_grid_values(fn, z0_grid[j, i], dz_grid[j, i], z0, dz_fine, nz, nzp)
+            if tl is not None:
+                col = (col + interpolate_and_shift_trend(tl, z0_grid[j, i], dz_grid[j, i], z0, dz_fine, nz,
+                                                         nzp)).astype(np.float32)
+            out[:, j, i] = col
     return out, nt

@@ -86,3 +86,66 @@ def test_fill_from_traces_columns():
     assert nt == 40 and g.shape == (16, 2, 4)
     assert np.all(g[:, 0, 2] == 0) and np.all(g[:, 1, 3] == 0)  # columns without a trace
     assert np.array_equal(g[:, 0, 0], g[:, 0, 3]) and np.array_equal(g[:, 0, 1], g[:, 1, 0])
+
+
+# ---------------------------------------------------------------- the branch for other grids
+def _param_fill(traces, z0d, dz, cols, z0g, dzg, nz, nzp):
+    ct = np.asarray(cols, dtype=np.int32)[None, :]
+    z0g = np.asarray(z0g, dtype=np.float32)[None, :]
+    dzg = np.asarray(dzg, dtype=np.float32)[None, :]
+    return O.fill_from_traces(traces, np.asarray(z0d, np.float32), dz, ct, z0g, dzg, nz, nzp, 0.0, parameter=True)
+
+
+def test_parameter_fill_of_a_straight_line_is_the_line():
+    # a trace on a line leaves nothing after the detrending: the column is the line at the cell
+    # times, its first value above the trace and its last below it
+    a, b, n, dz = 2500.0, 1.25, 60, 4.0
+    tr = (a + b * np.arange(n)).astype(np.float32)
+    nz = nzp = 100
+    z0d, z0g, dzg = 1000.0, 1002.0 - 40.0, 2.9
+    g, _ = _param_fill([tr], [z0d], dz, [0], [z0g], [dzg], nz, nzp)
+    col = g[:, 0, 0].astype(np.float64)
+    t = z0g + np.arange(nzp) * dzg
+    inside = (t >= z0d) & (t <= z0d + (n - 1) * dz)
+    want = a + b * (t - z0d) / dz
+    assert np.max(np.abs(col[inside] - want[inside])) < 2e-3  # float trend, values ~2500
+    assert np.all(col[t < z0d - dz] == np.float32(a)) and np.all(col[t > z0d + (n - 1) * dz + dz] == tr[-1])
+
+
+def test_parameter_fill_reproduces_the_samples_at_their_times():
+    # odd nt (45): the ten-fold resampling of the detrended trace is exact at the coarse samples,
+    # so cells at the sample times return the trace; dz 5 ms makes the fine step 0.5 ms exact
+    rng = np.random.default_rng(3)
+    n, dz = 45, 5.0
+    tr = (1800.0 + 3.0 * np.arange(n) + 40.0 * rng.standard_normal(n)).astype(np.float32)
+    nz, nzp = 40, 64
+    g, nt = _param_fill([tr], [700.0], dz, [0], [700.0], [dz], nz, nzp)
+    assert nt == 45
+    col = g[:, 0, 0]
+    ks = np.array([k for k in range(nzp) if 0 <= O.z_simbox_index(k, nz, nzp) < n])
+    assert np.max(np.abs(col[ks] - tr[[O.z_simbox_index(k, nz, nzp) for k in ks]])) < 2e-6 * 2000
+    # the padding's negative cell indices lie above the trace: the first sample exactly
+    neg = [k for k in range(nzp) if O.z_simbox_index(k, nz, nzp) < -1]
+    assert len(neg) > 0 and np.all(col[neg] == tr[0])
+
+
+def test_parameter_fill_trims_zeros_when_the_first_sample_is_zero():
+    rng = np.random.default_rng(4)
+    x = (1000.0 + 50.0 * rng.standard_normal(30)).astype(np.float32)
+    dz, z0 = 4.0, np.float32(812.0)
+    lead = np.concatenate([np.zeros(3, np.float32), x[:10], np.zeros(2, np.float32), x[10:]])
+    z0s = z0
+    for _ in range(3):  # the reference adds dz once per leading zero
+        z0s = np.float32(z0s + np.float32(dz))
+    inner = np.concatenate([x[:10], np.zeros(2, np.float32), x[10:]])  # first sample nonzero: zeros kept
+    traces = [lead, x, inner, np.zeros(12, np.float32)]
+    nz = nzp = 48
+    cols = [0, 1, 2, 3, -2, -1]
+    z0g = [820.0, 820.0, 820.0, 820.0, 820.0, 820.0]
+    dzg = [3.3] * 6
+    g, _ = _param_fill(traces, [z0, z0s, z0, z0], dz, cols, z0g, dzg, nz, nzp)
+    c = g[:, 0, :]
+    assert np.array_equal(c[:, 0], c[:, 1])          # all zeros dropped, z0 moved three samples down
+    assert not np.array_equal(c[:, 2], c[:, 1])      # interior zeros stay when the first sample is not zero
+    assert np.all(c[:, 3] == np.float32(O.RMISSING))  # no nonzero sample: dead
+    assert np.all(c[:, 4] == np.float32(O.RMISSING)) and np.all(c[:, 5] == 0.0)

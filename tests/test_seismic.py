@@ -87,9 +87,34 @@ def test_columns_of_a_simbox_over_the_written_traces(tmp_path):
     for j in range(nyp):
         for i in range(2 * (nxp // 2 + 1)):
             ri, rj = SE.fill_number(i, nx, nxp), SE.fill_number(j, ny, nyp)
-            if ri is None or (ri, rj) == (2, 1):
+            if ri is None:
                 assert ct[j, i] == -1
+            elif (ri, rj) == (2, 1):
+                assert ct[j, i] == -2  # inside the data, a missing cell: dead
             else:
                 assert ct[j, i] == order[(ri, rj)], (i, j)
                 assert abs(dzg[j, i] - 4.0) < 1e-6 and abs(z0g[j, i] - (top[rj, ri] + 4.0)) < 1e-3
     assert np.all(z0d == 900.0)
+
+
+def test_window_traces_cuts_to_the_volume():
+    # SegY::ReadTrace with a volume: samples j0..j1 around the surfaces plus half the padding,
+    # traces outside the volume, with a missing surface or beyond the data not kept
+    rng = np.random.default_rng(5)
+    n, ns = 7, 100
+    tr = dict(samples=rng.standard_normal((n, ns)).astype(np.float32), il=np.arange(n), xl=np.zeros(n, np.int32),
+              x=np.zeros(n), y=np.zeros(n), start_time=np.full(n, 900.0, np.float32), dz=4.0)
+    z_top = np.array([1000.0, 950.5, SE.RMISSING, 880.0, 1000.0, 1000.0, 925.0])
+    z_bot = np.array([1100.0, 1200.0, 1100.0, 1000.0, 1350.0, 1100.0, 1290.0])
+    inside = np.array([True, True, True, True, True, False, True])
+    w = SE.window_traces(tr, z_top, z_bot, 40.0, inside=inside)
+    kept = [s is not None for s in w["samples"]]
+    assert kept == [True, True, False, False, False, False, True]
+    # (zTop - 20 - 900) / 4 and (zBot + 20 - 900) / 4, truncated; the second clipped at ns - 1
+    for t, (j0, j1) in {0: (20, 55), 1: (7, 80), 6: (1, 99)}.items():
+        assert w["start"][t] == j0 and np.array_equal(w["samples"][t], tr["samples"][t, j0:j1 + 1])
+    assert w["z0"] == 900.0
+    # the trace's first sample: z0_ + (j0 + 0.5) dz, less dz / 2 for seismic data
+    samples, z0k, ct = SE.kept_traces(w, np.array([[0, 1, 6, -2, -1]]), np.array([1.0, 2.0, 0, 0, 0, 0, 3.0]))
+    assert [len(s) for s in samples] == [36, 74, 99] and list(z0k) == [1.0, 2.0, 3.0]
+    assert ct.tolist() == [[0, 1, 2, -2, -1]]
