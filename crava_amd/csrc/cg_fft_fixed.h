@@ -1239,9 +1239,41 @@ struct XZArgs {
   float scale;
 };
 
+// The x transform of an xz tile: the length's plan, or for the split geometries' lengths a plan
+// with more threads per row (288: 24 instead of 12; 576: 32 instead of 24), so that the z stage's
+// N + 1 bins take one round over the workgroup's threads and more loads are in flight
+// (k_fftxz_fwd<288, 20>: 214 -> 199 us per launch at 576 x 576 x 320).
+template <int N>
+struct XZPlan {
+  static constexpr int TPS = Plan<N>::TPS;
+  template <int DIR, int ES>
+  static __device__ __forceinline__ void run(float2 *b, int t, const float2 *w)
+  {
+    Plan<N>::template run<DIR, ES>(b, t, w);
+  }
+};
+template <>
+struct XZPlan<288> {
+  static constexpr int TPS = 24;
+  template <int DIR, int ES>
+  static __device__ __forceinline__ void run(float2 *b, int t, const float2 *w)
+  {
+    Stages<288, DIR, 24, ES, 1, 12, 24>::run(b, t, w);
+  }
+};
+template <>
+struct XZPlan<576> {
+  static constexpr int TPS = 32;
+  template <int DIR, int ES>
+  static __device__ __forceinline__ void run(float2 *b, int t, const float2 *w)
+  {
+    Stages<576, DIR, 32, ES, 1, 24, 24>::run(b, t, w);
+  }
+};
+
 template <int N, int N2>
 struct XZ {
-  static constexpr int TPS = Plan<N>::TPS;
+  static constexpr int TPS = XZPlan<N>::TPS;
   static constexpr int NT = N2 * TPS;
   static constexpr int ES = N2 + 1;  // odd: slot m of row r at m ES + r
   static constexpr int CN = N + 1;
@@ -1249,7 +1281,7 @@ struct XZ {
 };
 
 template <int N, int N2>
-__global__ void __launch_bounds__(N2 * Plan<N>::TPS)
+__global__ void __launch_bounds__(N2 * XZPlan<N>::TPS)
 k_fftxz_fwd(XZArgs a, const float2 *__restrict__ twg, const float2 *__restrict__ tw_row,
             const float2 *__restrict__ twz)
 {
@@ -1285,7 +1317,7 @@ k_fftxz_fwd(XZArgs a, const float2 *__restrict__ twg, const float2 *__restrict__
     }
   }
   lds_sync();
-  Plan<N>::template run<-1, ES>(buf + tid % N2, tid / N2, s_tw);
+  XZPlan<N>::template run<-1, ES>(buf + tid % N2, tid / N2, s_tw);
   for (int idx = tid; idx < N2 * (N / 2 + 1); idx += NT) {  // R2C post-pass, as k_fftx_r2c
     const int r = idx % N2, k = idx / N2;
     const int kn = k == 0 ? 0 : N - k;
@@ -1312,7 +1344,7 @@ k_fftxz_fwd(XZArgs a, const float2 *__restrict__ twg, const float2 *__restrict__
 }
 
 template <int N, int N2, bool ADD>
-__global__ void __launch_bounds__(N2 * Plan<N>::TPS)
+__global__ void __launch_bounds__(N2 * XZPlan<N>::TPS)
 k_fftxz_inv(XZArgs a, const float2 *__restrict__ twg, const float2 *__restrict__ tw_row,
             const float2 *__restrict__ twz)
 {
@@ -1383,7 +1415,7 @@ k_fftxz_inv(XZArgs a, const float2 *__restrict__ twg, const float2 *__restrict__
     }
   }
   lds_sync();
-  Plan<N>::template run<+1, ES>(buf + tid % N2, tid / N2, s_tw);
+  XZPlan<N>::template run<+1, ES>(buf + tid % N2, tid / N2, s_tw);
 #pragma unroll
   for (int i = 0; i < PER; ++i) {
     const int f = tid + i * NT;
