@@ -168,7 +168,8 @@ class TraceFillArgs(ctypes.Structure):
                 ("z0_data", ctypes.POINTER(ctypes.c_float)), ("ntraces", ctypes.c_int), ("stride", ctypes.c_int),
                 ("dz_data", ctypes.c_float), ("smooth_length", ctypes.c_float),
                 ("column_trace", ctypes.POINTER(ctypes.c_int)), ("z0_grid", ctypes.POINTER(ctypes.c_float)),
-                ("dz_grid", ctypes.POINTER(ctypes.c_float)), ("grid_type", ctypes.c_int)]
+                ("dz_grid", ctypes.POINTER(ctypes.c_float)), ("grid_type", ctypes.c_int),
+                ("trim_keeps_z0", ctypes.c_int)]
 
 
 class SegyGeometry(ctypes.Structure):
@@ -496,7 +497,7 @@ class FFTGrid:
         self.ctx.check(lib().cg_grid_write_segy(self._h, file_name.encode(), ctypes.byref(geo)), "writeSegyFile")
 
     def fillInData(self, traces, z0_data, dz_data, column_trace, z0_grid, dz_grid, smooth_length,
-                   grid_type=DATA):
+                   grid_type=DATA, trim_keeps_z0=False):
         """CommonData::FillInData (commondata.cpp:6451-6775) from SEG-Y traces, with the columns
         resolved by the caller: traces (a list of 1-D arrays, at most 1024 samples), z0_data (time
         of each trace's first sample, ms), dz_data (ms), column_trace (nyp, rnxp) trace index per
@@ -504,7 +505,8 @@ class FFTGrid:
         each column's first cell and its thickness (ms), smooth_length the guard-zone taper (ms,
         DATA grids).  grid_type DATA: the seismic branch (tapered traces, zeros where no trace);
         PARAMETER: the branch for other grids (zero samples trimmed, the linear trend removed and
-        added back, RMISSING in dead columns; every box column must have a trace or be dead)."""
+        added back, RMISSING in dead columns; every box column must have a trace or be dead);
+        trim_keeps_z0: the trimmed zeros do not move z0_data (traces from a STORM grid)."""
         n = len(traces)
         stride = max(len(t) for t in traces)
         samples = np.zeros((n, stride), dtype=np.float32)
@@ -520,7 +522,8 @@ class FFTGrid:
             raise ValueError("fillInData: columns must be (nyp, rnxp) = %s, z0_data (%d,)" % (shape, n))
         a = TraceFillArgs(_ptr(samples, ctypes.c_float), _ptr(length, ctypes.c_int), _ptr(z0d, ctypes.c_float), n,
                           stride, float(dz_data), float(smooth_length), _ptr(ct, ctypes.c_int),
-                          _ptr(z0g, ctypes.c_float), _ptr(dzg, ctypes.c_float), int(grid_type))
+                          _ptr(z0g, ctypes.c_float), _ptr(dzg, ctypes.c_float), int(grid_type),
+                          1 if trim_keeps_z0 else 0)
         self.ctx.check(lib().cg_grid_fill_from_traces(self._h, ctypes.byref(a)), "fillInData")
 
     def readStormFile(self, file_name):

@@ -1943,6 +1943,7 @@ int cg_grid_fill_from_traces(cg_grid *g, const cg_trace_fill_args *a)
   p.rnxp = g->rnxp;
   p.grid = g->d;
   p.detrend = detrend ? 1 : 0;
+  p.trim_shift = a->trim_keeps_z0 ? 0 : 1;
   // one device block: samples, lengths, z0_data, columns (trace, z0, dz), the two trace buffers,
   // the detrended traces' TraceInfo
   Layout L;

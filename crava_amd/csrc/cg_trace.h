@@ -59,6 +59,7 @@ struct TraceFillParams {
   float *grid;             // real grid, in-place layout
   int nz, nyp, nzp, rnxp;
   int detrend;             // non-DATA grid: k_pack_detrend's trimming and trend, RMISSING when dead
+  int trim_shift;          // leading zeros trimmed move z0 (SEG-Y traces; not STORM columns)
   TraceInfo *info;         // [ntraces] when detrend
 };
 int pack_traces(cg_ctx *ctx, const TraceFillParams &p);

@@ -260,8 +260,8 @@ __global__ void __launch_bounds__(cg::kThreads) k_pack_traces(cg::TraceFillParam
 
 // FillInData's preparation of a trace for a non-DATA grid (commondata.cpp:6612-6644), one
 // workgroup per trace (at most 1024 samples, four per thread): where the first sample is zero,
-// every zero sample is dropped (kept samples compacted by a workgroup scan) and z0 moves down one
-// dz_data per leading zero (added one at a time, as the reference does); the line through the
+// every zero sample is dropped (kept samples compacted by a workgroup scan) and, for SEG-Y traces,
+// z0 moves down one dz_data per leading zero (added one at a time, as the reference does); the line through the
 // first and last kept samples is subtracted, in float: s[m] - (first + m inc),
 // inc = (last - first) / (n - 1); zero padding to rnt.  The trend's fine samples
 // are first + l inc2, inc2 = (last - first) / (10 (n - 1)), l < 10 (n - 1) + 1 (:6684-6692).
@@ -309,7 +309,7 @@ __global__ void __launch_bounds__(kDetrendThreads) k_pack_detrend(cg::TraceFillP
   for (int m = tid; m < p.rnt; m += kDetrendThreads) dst[m] = m < nk ? kept[m] - (first + (float)m * inc) : 0.0f;
   if (tid == 0) {
     float z0 = p.z0_data[t];
-    if (trim)
+    if (trim && p.trim_shift)
       for (int i = 0; i < lead; ++i) z0 = z0 + p.dz_data;
     cg::TraceInfo ti;
     ti.z0 = z0;

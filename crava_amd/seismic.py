@@ -209,3 +209,115 @@ def fill_in_data_from_segy(grid, path, transform, simbox, smooth_length, header_
     samples, z0k, ctk = kept_traces(tr, ct, z0d)
     grid.fillInData(samples, z0k, tr["dz"], ctk, z0g, dzg, smooth_length, grid_type)
     return tr, ct
+
+
+# ---------------------------------------------------------------- STORM grids as input
+def read_storm(path):
+    """StormContGrid::ReadFromFile (stormcontgrid.cpp:82-170) of a binary STORM grid whose top and
+    base surfaces are constants (a surface given by file name raises CravaError): dict of values
+    (nk, nj, ni) float32 (i fastest in the file, big-endian), x_min, lx, y_min, ly, top, bot, lz,
+    angle (radians; the header holds degrees), ni, nj, nk."""
+    with open(path, "rb") as f:
+        data = f.read()
+    pos = 0
+
+    def token():
+        nonlocal pos
+        while pos < len(data) and data[pos:pos + 1].isspace():
+            pos += 1
+        st = pos
+        while pos < len(data) and not data[pos:pos + 1].isspace():
+            pos += 1
+        return data[st:pos].decode("latin-1")
+    if token() != "storm_petro_binary":
+        raise CravaError("read_storm: %s is not a binary STORM grid" % path)
+    tok = [token() for _ in range(17)]  # 4 header tokens, the volume (8), lz, angle, ni nj nk
+    nl = data.find(b"\n", pos)  # the rest of the size line
+    if not tok[-1] or nl < 0:
+        raise CravaError("read_storm: %s ends inside the header" % path)
+    pos = nl + 1
+    try:
+        top, bot = float(tok[8]), float(tok[9])
+    except ValueError:
+        raise CravaError("read_storm: surfaces given by file name (%s, %s) are not read" % (tok[8], tok[9]))
+    ni, nj, nk = int(tok[14]), int(tok[15]), int(tok[16])
+    if len(data) - pos < 4 * ni * nj * nk:
+        raise CravaError("read_storm: %s ends inside the grid" % path)
+    values = np.frombuffer(data, dtype=">f4", count=ni * nj * nk, offset=pos).astype(np.float32)
+    return dict(values=values.reshape(nk, nj, ni), x_min=float(tok[4]), lx=float(tok[5]), y_min=float(tok[6]),
+                ly=float(tok[7]), top=top, bot=bot, lz=float(tok[12]), angle=math.radians(float(tok[13])),
+                ni=ni, nj=nj, nk=nk)
+
+
+def storm_columns(storm, simbox, nx, ny, nz, nxp, nyp, grid_type=DATA):
+    """FillInData's column loop with a STORM grid as the source (is_storm, commondata.cpp:6560-6611):
+    column (i, j) of the padded grid at the centre of box cell GetFillNumber(i), GetFillNumber(j)
+    takes the STORM column StormContGrid::FindXYIndex gives for that point when
+    StormContGrid::IsInside holds (the point as float, stormcontgrid.cpp:291-322, volume.cpp:458-575),
+    -1 outside, -2 where the box cell's top or base is missing.  The trace's first sample is at the
+    centre of the STORM column's first cell and its sample interval the distance between the
+    centres of its first and last cells over nk - 1 (FindCenterOfCell, :494-503; in float).
+    Returns (traces, z0_data, dz_data, column_trace, z0_grid, dz_grid): the STORM columns used,
+    in order of first use."""
+    seismic = grid_type == DATA
+    f32 = np.float32
+    rnxp = 2 * (nxp // 2 + 1)
+    ca, sa = math.cos(simbox["angle"]), math.sin(simbox["angle"])
+    cs, ss = math.cos(storm["angle"]), math.sin(storm["angle"])
+    dxs, dys = storm["lx"] / storm["ni"], storm["ly"] / storm["nj"]
+    nk = storm["nk"]
+    dzs = (storm["bot"] - storm["top"]) / nk
+    z_min = f32(storm["top"] + 0.5 * dzs)
+    z_max = f32(storm["top"] + (nk - 1 + 0.5) * dzs)
+    dz_data = f32(f32(z_max - z_min) / f32(nk - 1))
+    ct = np.full((nyp, rnxp), -1, dtype=np.int32)
+    z0g = np.zeros((nyp, rnxp), dtype=np.float32)
+    dzg = np.ones((nyp, rnxp), dtype=np.float32)
+    index, traces = {}, []
+    top, bot = np.asarray(simbox["top"], np.float64), np.asarray(simbox["bot"], np.float64)
+    for j in range(nyp):
+        rj = fill_number(j, ny, nyp)
+        for i in range(rnxp):
+            ri = fill_number(i, nx, nxp)
+            if ri is None:
+                continue
+            rx, ry = (ri + 0.5) * simbox["dx"], (rj + 0.5) * simbox["dy"]
+            x = rx * ca - ry * sa + simbox["x0"]
+            y = rx * sa + ry * ca + simbox["y0"]
+            xf, yf = float(f32(x)), float(f32(y))
+            lxf = (xf - storm["x_min"]) * cs + (yf - storm["y_min"]) * ss
+            lyf = -(xf - storm["x_min"]) * ss + (yf - storm["y_min"]) * cs
+            if lxf < 0.0 or lxf > storm["lx"] or lyf < 0.0 or lyf > storm["ly"]:
+                continue  # outside the STORM volume: zeros
+            t_, b_ = top[rj, ri], bot[rj, ri]
+            if t_ == RMISSING or b_ == RMISSING:
+                ct[j, i] = -2
+                continue
+            lx_ = cs * (x - storm["x_min"]) + ss * (y - storm["y_min"])
+            ly_ = -ss * (x - storm["x_min"]) + cs * (y - storm["y_min"])
+            ii, jj = int(lx_ / dxs), int(ly_ / dys)
+            if lx_ < 0.0 or ly_ < 0.0 or ii >= storm["ni"] or jj >= storm["nj"]:
+                raise CravaError("storm_columns: an xy-position outside the STORM grid (FindXYIndex)")
+            if (ii, jj) not in index:
+                index[(ii, jj)] = len(traces)
+                traces.append(np.asarray(storm["values"][:, jj, ii], dtype=np.float32))
+            dz = (b_ - t_) / nz
+            z0 = t_ + 0.5 * dz  # Simbox::getCoord(refi, refj, 0)
+            ct[j, i] = index[(ii, jj)]
+            z0g[j, i] = f32(z0) + f32(0.5) * f32(dz) if seismic else f32(z0)
+            dzg[j, i] = f32(dz)
+    z0_data = np.full(len(traces), z_min, dtype=np.float32)
+    return traces, z0_data, float(dz_data), ct, z0g, dzg
+
+
+def fill_in_data_from_storm(grid, path, simbox, smooth_length, grid_type=DATA):
+    """CommonData::FillInData for an FFTGrid from a binary STORM grid of any lateral and vertical
+    geometry (constant surfaces): the columns resolved on the host, the per-trace work on the device
+    (a trace's trimmed zeros leave its start where it is).  Returns (storm, column_trace)."""
+    storm = read_storm(path)
+    traces, z0d, dz_data, ct, z0g, dzg = storm_columns(storm, simbox, grid.nx, grid.ny, grid.nz, grid.nxp, grid.nyp,
+                                                       grid_type)
+    if not traces:
+        raise CravaError("fill_in_data_from_storm: the STORM grid does not cover the box")
+    grid.fillInData(traces, z0d, dz_data, ct, z0g, dzg, smooth_length, grid_type, trim_keeps_z0=True)
+    return storm, ct

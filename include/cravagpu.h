@@ -308,6 +308,8 @@ typedef struct {
   const float *z0_grid;      /* [nyp][rnxp] centre of the column's first cell, ms (z0 + dz / 2) */
   const float *dz_grid;      /* [nyp][rnxp] */
   int grid_type;             /* CG_CT_DATA (or 0) / CG_CT_PARAMETER */
+  int trim_keeps_z0;         /* 1: trimmed zeros leave z0_data where it is, as FillInData does for
+                              * traces from a STORM grid (the shift is SEG-Y only, :6622-6628) */
 } cg_trace_fill_args;
 int cg_grid_fill_from_traces(cg_grid *grid, const cg_trace_fill_args *args);
 

@@ -1077,11 +1077,12 @@ def interpolate_and_shift_trend(trend_long, z0_grid, dz_grid, z0_data, dz_fine, 
     return out
 
 
-def trim_and_detrend(trace, z0_data, dz_data):
+def trim_and_detrend(trace, z0_data, dz_data, shift=True):
     """FillInData's preparation of a trace for a parameter (not DATA) grid
     (commondata.cpp:6612-6644): where the first sample is zero, every zero sample is dropped and z0
-    moves down one sample per leading zero; then the line through the first and last samples is
-    subtracted (float).  Returns (trace, z0, first, last) or None for an empty trace."""
+    moves down one sample per leading zero (SEG-Y traces; shift=False: a STORM column's z0 stays,
+    :6622-6628); then the line through the first and last samples is subtracted (float).  Returns
+    (trace, z0, first, last) or None for an empty trace."""
     f32 = np.float32
     t = np.asarray(trace, dtype=np.float32)
     z0 = f32(z0_data)
@@ -1089,7 +1090,7 @@ def trim_and_detrend(trace, z0_data, dz_data):
         lead = 0
         while lead < len(t) and t[lead] == 0:
             lead += 1
-        for _ in range(lead):
+        for _ in range(lead if shift else 0):
             z0 = f32(z0 + f32(dz_data))
         t = t[t != 0]
     n = len(t)
@@ -1103,7 +1104,7 @@ def trim_and_detrend(trace, z0_data, dz_data):
 
 
 def fill_from_traces(traces, z0_data, dz_data, column_trace, z0_grid, dz_grid, nz, nzp, smooth_length,
-                     parameter=False):
+                     parameter=False, trim_shift=True):
     """The columns of FillInData (commondata.cpp:6451-6775) from SEG-Y traces: column (j, i) of the
     padded real grid (nzp, nyp, rnxp) takes trace column_trace[j, i] with its first sample at
     z0_data[t] (ms); -1: outside the data (zeros), -2: a dead trace (zeros, RMISSING for a
@@ -1113,7 +1114,8 @@ def fill_from_traces(traces, z0_data, dz_data, column_trace, z0_grid, dz_grid, n
     (interpolate_grid_values, dz_fine = dz_data / 10).  Parameter grids (parameter=True): trimmed
     and detrended (trim_and_detrend) instead of tapered, the trend resampled linearly ten times
     finer (res_fac (n - 1) + 1 samples) and added back (interpolate_and_shift_trend); an empty
-    trace is dead.  Returns (grid, nt)."""
+    trace is dead; trim_shift=False: the trimmed zeros leave z0 (a STORM grid's columns).
+    Returns (grid, nt)."""
     f32 = np.float32
     nyp, rnxp = column_trace.shape
     nt = lib().orc_factorable(max(len(t) for t in traces))
@@ -1129,7 +1131,7 @@ def fill_from_traces(traces, z0_data, dz_data, column_trace, z0_grid, dz_grid, n
                 continue
             if t not in fine:
                 if parameter:
-                    r = trim_and_detrend(traces[t], z0_data[t], dz_data)
+                    r = trim_and_detrend(traces[t], z0_data[t], dz_data, trim_shift)
                     if r is None:
                         fine[t] = None
                     else:

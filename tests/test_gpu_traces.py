@@ -232,3 +232,52 @@ def test_background_segy_file_onto_the_box(ctx, tmp_path):
     assert err < 2e-6 and int((ct >= 0).sum()) == ny * nx and dev < 0.02
     src.close()
     dst.close()
+
+
+@pytest.mark.parametrize("grid_type", [C.DATA, C.PARAMETER])
+def test_storm_grid_onto_the_box(ctx, tmp_path, grid_type):
+    """FillInData with a STORM grid as the source (is_storm): a STORM cube on its own rotated,
+    coarser lattice and its own time span resampled onto a simbox inside it -- seismic (tapered)
+    and parameter grids (zeros trimmed at the top of some columns without moving their start,
+    the trend, RMISSING for a box cell with a missing top); the device grid equals the oracle's
+    columns from the same STORM columns."""
+    import os
+    from crava_amd import seismic as SE
+    rng = np.random.default_rng(13 + grid_type)
+    nx, ny, nz, nxp, nyp, nzp = 12, 10, 40, 16, 12, 48
+    ni, nj, nk = 9, 8, 70
+    kk = np.arange(nk, dtype=np.float32)[:, None, None]
+    if grid_type == C.DATA:
+        vals = np.sin(0.3 * kk + rng.uniform(0, 3, (1, nj, ni))) + 0.2 * rng.standard_normal((nk, nj, ni))
+    else:
+        vals = 2400.0 + 5.0 * kk + 30.0 * rng.standard_normal((nk, nj, ni))
+        vals[:rng.integers(1, 6), 2, 3] = 0.0
+        vals[:4, 5, 1] = 0.0
+    vals = vals.astype(np.float32)
+    path = os.path.join(str(tmp_path), "in.storm")
+    ang = 25.0
+    head = ("storm_petro_binary\n0 2 -999.000000\nFFTGrid\n%f %f %f %f %s %s 0.0 0.0\n%f %f\n\n%d %d %d\n"
+            % (3900.0, 420.0, 6950.0, 380.0, 950, 1300, 350.0, ang, ni, nj, nk))
+    with open(path, "wb") as f:
+        f.write(head.encode())
+        f.write(np.ascontiguousarray(vals, dtype=">f4").tobytes())
+        f.write(b"0\n")
+    top = 1000.0 + rng.uniform(0.0, 30.0, (ny, nx))
+    top[4, 5] = O.RMISSING
+    simbox = dict(x0=4000.0, y0=7000.0, dx=20.0, dy=20.0, angle=np.radians(28.0), top=top, bot=top + nz * 4.0)
+    g = C.FFTGrid(ctx, nx, ny, nz, nxp, nyp, nzp, grid_type)
+    st, ct = SE.fill_in_data_from_storm(g, path, simbox, 20.0, grid_type)
+    got = g.download_real(padded=True)
+    traces, z0d, dzd, ct2, z0g, dzg = SE.storm_columns(st, simbox, nx, ny, nz, nxp, nyp, grid_type)
+    want, nt = O.fill_from_traces(traces, z0d, dzd, ct2, z0g, dzg, nz, nzp, 20.0,
+                                  parameter=grid_type == C.PARAMETER, trim_shift=False)
+    miss = want == np.float32(O.RMISSING)
+    assert np.array_equal(got == np.float32(O.RMISSING), miss)
+    err = float(np.max(np.abs(got[~miss] - want[~miss]))) / float(np.max(np.abs(want[~miss])))
+    print("STORM %dx%dx%d onto %dx%dx%d (%s): %d STORM columns, %d outside, %d dead; max |device - oracle| / max %.2e"
+          % (ni, nj, nk, nxp, nyp, nzp, "DATA" if grid_type == C.DATA else "PARAMETER", len(traces),
+             int((ct == -1).sum()), int((ct == -2).sum()), err), flush=True)
+    assert err < 2e-6 and len(traces) > 20 and int((ct == -2).sum()) > 0
+    if grid_type == C.PARAMETER:
+        assert not np.any(ct[:ny, :nx] == -1)  # the STORM volume covers the box
+    g.close()
