@@ -527,16 +527,39 @@ int cg_grid_write_segy(cg_grid *g, const char *path, const cg_segy_geometry *geo
   File file;
   file.f = fopen(path, "wb");
   if (!file.f) return cg::fail(ctx, CG_EIO, std::string("write_segy: cannot open '") + path + "'");
-  // textual header: 40 lines of 80 EBCDIC characters, "C" + line number, the caller's lines from
-  // the second line on, 75 characters each (TextualHeader::SetLine(i + 1, text), as
-  // writeSegyFile places them)
+  // textual header: 40 lines of 80 EBCDIC characters, "C" + line number, text from column 5.
+  // With the caller's lines (from the second line on, 75 characters each:
+  // TextualHeader::SetLine(i + 1, text), as writeSegyFile places them) on NRLib's empty header,
+  // whose last line "C40 END EBCDIC" is 79 characters and whose 3200th byte is the string's
+  // terminating zero (commonheaders.cpp:60-104, 152-213); with none, TextualHeader::standardHeader
+  // (:106-163): the writing program's credits on lines 12-28 and a full last line.
+  static const struct {
+    int line;
+    const char *text;
+  } kStandard[] = {{12, "       THIS SEGY FILE WAS WRITTEN BY"},
+                   {14, "       THE CRAVA PROGRAM"},
+                   {16, "       DEVELOPED BY"},
+                   {18, "       RAGNAR HAUGE"},
+                   {20, "       ANNE RANDI SYVERSVEEN           AND"},
+                   {22, "       ODD KOLBJORNSEN"},
+                   {24, "       AT"},
+                   {26, "       THE NORWEGIAN COMPUTING CENTER"},
+                   {28, "       2003"}};
   std::vector<unsigned char> head(3600, 0);
   for (int l = 0; l < 40; ++l) {
     char line[96];
-    const char *text = l >= 1 && l - 1 < geo->ntext && geo->text[l - 1] ? geo->text[l - 1] : (l == 39 ? "END EBCDIC" : "");
-    snprintf(line, sizeof(line), "C%2d  %-75.75s", l + 1, text);
+    const char *text = "";
+    if (geo->ntext > 0) {
+      if (l >= 1 && l - 1 < geo->ntext && geo->text[l - 1]) text = geo->text[l - 1];
+    } else {
+      for (const auto &st : kStandard)
+        if (st.line == l + 1) text = st.text;
+    }
+    if (l == 39) snprintf(line, sizeof(line), "C40 END EBCDIC%66s", "");
+    else snprintf(line, sizeof(line), "C%2d  %-75.75s", l + 1, text);
     for (int c = 0; c < 80; ++c) head[80 * l + c] = ebcdic(line[c]);
   }
+  if (geo->ntext > 0) head[3199] = 0;  // the empty header's string ends one character short
   // binary header (BinaryHeader::Write, commonheaders.cpp:234-270)
   unsigned char *bh = head.data() + 3200;
   put_be32(bh + 0, 9999);

@@ -708,11 +708,24 @@ def segy_write(real, nx, ny, nz, geo):
     z0 = f32(geo["z0"])
     ns = int(math.ceil((geo["zmax"] - float(z0)) / float(dzf)))
     text = list(geo.get("text", []))
+    # NRLib's TextualHeader (commonheaders.cpp:60-213): the caller's lines on the empty header
+    # (last line 79 characters, then the string's terminating zero), or with none the standard
+    # header with the writing program's credits on lines 12-28
+    standard = {12: "THIS SEGY FILE WAS WRITTEN BY", 14: "THE CRAVA PROGRAM", 16: "DEVELOPED BY",
+                18: "RAGNAR HAUGE", 20: "ANNE RANDI SYVERSVEEN           AND", 22: "ODD KOLBJORNSEN", 24: "AT",
+                26: "THE NORWEGIAN COMPUTING CENTER", 28: "2003"}
     lines = []
-    for l in range(40):
-        t = text[l - 1] if 1 <= l <= len(text) else ("END EBCDIC" if l == 39 else "")
+    for l in range(39):
+        if text:
+            t = text[l - 1] if 1 <= l <= len(text) else ""
+        else:
+            t = " " * 7 + standard[l + 1] if l + 1 in standard else ""
         lines.append(("C%2d  %-75.75s" % (l + 1, t))[:80])
-    head = "".join(lines).encode("cp037")
+    lines.append("%-80s" % "C40 END EBCDIC")
+    head = bytearray("".join(lines).encode("cp037"))
+    if text:
+        head[3199] = 0
+    head = bytes(head)
     bh = bytearray(400)
     struct.pack_into(">iii", bh, 0, 9999, 1, 1)
     struct.pack_into(">h", bh, 12, nx)

@@ -126,3 +126,24 @@ def test_segy_trace_beyond_the_grid_is_missing():
     _, traces = _parse(data, 40)
     v = traces[0]["v"]
     assert np.all(v[:10] == 1.0) and np.all(v[10:] == 0.0)  # end = nz samples: the rest written as 0
+
+
+def test_textual_header_is_nrlibs():
+    """NRLib's TextualHeader (commonheaders.cpp:60-213): the caller's lines go on the empty header,
+    whose last line "C40 END EBCDIC" (one space) is 79 characters followed by the string's
+    terminating zero; with no lines the standard header carries the writing program's credits
+    (lines 12-28) and a full last line."""
+    rng = np.random.default_rng(7)
+    real = np.zeros((12, 4, 4), np.float32)
+    geo = _geo(2, 2, rng, angle=0.0)
+    with_text = O.segy_write(real, 2, 2, 10, geo)[:3200]
+    lines = [with_text[80 * l:80 * l + 80] for l in range(40)]
+    assert lines[0].decode("cp037") == "C 1" + " " * 77
+    assert lines[1].decode("cp037").startswith("C 2  cravagpu test")
+    assert lines[39][:79].decode("cp037") == "C40 END EBCDIC" + " " * 65 and lines[39][79] == 0
+    std = O.segy_write(real, 2, 2, 10, dict(geo, text=[]))[:3200].decode("cp037")
+    sl = [std[80 * l:80 * l + 80] for l in range(40)]
+    assert sl[13] == "C14" + " " * 9 + "THE CRAVA PROGRAM" + " " * 51
+    assert sl[11].rstrip() == "C12         THIS SEGY FILE WAS WRITTEN BY" and sl[27].rstrip() == "C28         2003"
+    assert sl[39] == "C40 END EBCDIC" + " " * 66 and sl[10] == "C11" + " " * 77
+    assert sum(1 for s in sl if s[3:].strip()) == 10
