@@ -321,3 +321,23 @@ def fill_in_data_from_storm(grid, path, simbox, smooth_length, grid_type=DATA):
         raise CravaError("fill_in_data_from_storm: the STORM grid does not cover the box")
     grid.fillInData(traces, z0d, dz_data, ct, z0g, dzg, smooth_length, grid_type, trim_keeps_z0=True)
     return storm, ct
+
+
+def trace_counts(column_trace, nx, ny, traces=None, grid_type=DATA):
+    """FillInData's counters (commondata.cpp:6456-6460, 6731-6770): missing_traces_simbox (box
+    columns outside the data), missing_traces_padding (padding columns outside it),
+    dead_traces_simbox (columns inside the data without a trace -- for a non-DATA grid also those
+    whose trace has no nonzero sample) and dead_traces_map ((nyp, rnxp) bool).  The reference
+    stops a non-DATA grid at its first box column outside the data; the counters here cover every
+    column (cg_grid_fill_from_traces refuses such a grid)."""
+    ct = np.asarray(column_trace)
+    nyp, rnxp = ct.shape
+    box = np.zeros_like(ct, dtype=bool)
+    box[:ny, :nx] = True
+    dead = ct == -2
+    if grid_type != DATA and traces is not None:
+        empty = np.array([not np.any(np.asarray(t) != 0) for t in traces] + [False])
+        dead |= (ct >= 0) & empty[np.where(ct >= 0, ct, len(traces))]
+    out = ct == -1
+    return dict(missing_traces_simbox=int((out & box).sum()), missing_traces_padding=int((out & ~box).sum()),
+                dead_traces_simbox=int(dead.sum()), dead_traces_map=dead)

@@ -169,3 +169,15 @@ def test_read_storm_and_its_columns(tmp_path):
     assert east[2, 0] == -2  # the missing top
     east[2, 0] = 0
     assert np.all(ct2[:ny, :2] == -1) and np.all(east >= 0)
+
+
+def test_trace_counts():
+    ct = np.array([[0, 1, -1, -2, -1, -1], [2, -2, 1, 0, 2, -1], [-1, 0, 0, 1, 2, -1]], dtype=np.int32)
+    c = SE.trace_counts(ct, 3, 2)
+    # box (2 x 3): one outside, two dead; padding: rows 2 and columns 3..5
+    assert c["missing_traces_simbox"] == 1 and c["dead_traces_simbox"] == 2
+    assert c["missing_traces_padding"] == int((ct == -1).sum()) - 1
+    traces = [np.ones(4), np.zeros(4), np.array([0.0, 0.0, 2.0, 0.0])]
+    cp = SE.trace_counts(ct, 3, 2, traces, C.PARAMETER)
+    assert cp["dead_traces_simbox"] == 2 + int((ct == 1).sum())  # trace 1 has no nonzero sample
+    assert np.array_equal(cp["dead_traces_map"], (ct == -2) | (ct == 1))
